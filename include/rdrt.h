@@ -1,0 +1,210 @@
+/*
+ * rdrt.h -- C-ABI of the MI355X ray-query hot path (librdrt.so).
+ *
+ * The entry points replace the reference's acceleration-structure provider
+ * and ray-tracing encoder for the naive SAH-BVH backend:
+ *
+ *   shader/ray-tracing/src/api/backend.rs:120-151
+ *     GPUAccelerationStructureSystemProvider
+ *       create_bottom_level_acceleration_structure -> rdrt_blas_begin /
+ *                                rdrt_blas_add_triangles / rdrt_blas_end
+ *       delete_bottom_level_acceleration_structure -> rdrt_blas_destroy
+ *       create_top_level_acceleration_structure    -> rdrt_tlas_create
+ *       delete_top_level_acceleration_structure    -> rdrt_tlas_destroy
+ *       bind_tlas                                  -> rdrt_bind_tlas
+ *       bind_tlas_max_len                          -> rdrt_bind_tlas_max_len
+ *       create_comp_instance (lazy build)          -> rdrt_commit
+ *   shader/ray-tracing/src/backend/wavefront_compute/geometry/naive/mod.rs:322-494
+ *     NaiveSahBvhSource::build                     -> rdrt_build / rdrt_commit
+ *   shader/ray-tracing/src/backend/wavefront_compute/geometry/naive/traverse_cpu.rs:52-245
+ *   .../geometry/naive/traverse_gpu.rs:182-282
+ *     traverse (closest hit)                       -> rdrt_trace, rdrt_trace_closest
+ *
+ * Conventions (mirroring the reference's provider contract, naive/mod.rs:98-119,497-609):
+ *   - Every function returns an int32 status: RDRT_OK (0) or a negative
+ *     RDRT_ERR_* code; rdrt_last_error() gives a thread-local message.
+ *   - Host arrays passed in are copied; device pointers (documented as such)
+ *     are borrowed for the duration of the call.
+ *   - Handles are u32 indices that are never reused.
+ *   - A context is not thread-safe; all device work is enqueued on the
+ *     context's stream and is asynchronous (rdrt_sync waits).
+ *   - Matrices are column-major Mat4 (math/algebra/src/mat/mat4.rs:9-14).
+ */
+#ifndef RDRT_H
+#define RDRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RDRT_ABI_VERSION 1
+
+/* status codes */
+#define RDRT_OK 0
+#define RDRT_ERR_INVALID_ARG (-1)
+#define RDRT_ERR_HIP (-2)
+#define RDRT_ERR_NOT_BUILT (-3)
+#define RDRT_ERR_OUT_OF_MEMORY (-4)
+#define RDRT_ERR_STATE (-5)
+#define RDRT_ERR_NO_DEVICE (-6)
+#define RDRT_ERR_CAPACITY (-7)
+#define RDRT_ERR_UNSUPPORTED (-8)
+
+/* ray flags (shader/ray-tracing/src/api/ty.rs:102-114) */
+#define RDRT_RAY_FLAG_NONE 0x00u
+#define RDRT_RAY_FLAG_FORCE_OPAQUE 0x01u
+#define RDRT_RAY_FLAG_FORCE_NON_OPAQUE 0x02u
+#define RDRT_RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH 0x04u
+#define RDRT_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER 0x08u
+#define RDRT_RAY_FLAG_CULL_BACK_FACING_TRIANGLES 0x10u
+#define RDRT_RAY_FLAG_CULL_FRONT_FACING_TRIANGLES 0x20u
+#define RDRT_RAY_FLAG_CULL_OPAQUE 0x40u
+#define RDRT_RAY_FLAG_CULL_NON_OPAQUE 0x80u
+#define RDRT_RAY_FLAG_SKIP_TRIANGLES 0x100u
+
+/* hit kinds (ty.rs:139-140) */
+#define RDRT_HIT_KIND_FRONT_FACING_TRIANGLE 0xFEu
+#define RDRT_HIT_KIND_BACK_FACING_TRIANGLE 0xFFu
+
+/* geometry-instance flags (ty.rs:142-152) and geometry flags (ty.rs:154-159) */
+#define RDRT_INSTANCE_TRIANGLE_FACING_CULL_DISABLE 0x1u
+#define RDRT_INSTANCE_TRIANGLE_FLIP_FACING 0x2u
+#define RDRT_INSTANCE_FORCE_OPAQUE 0x4u
+#define RDRT_INSTANCE_FORCE_NO_OPAQUE 0x8u
+#define RDRT_GEOMETRY_FLAG_OPAQUE 0x1u
+#define RDRT_GEOMETRY_FLAG_NO_DUPLICATE_ANYHIT_INVOCATION 0x2u
+
+#define RDRT_MISS 0xffffffffu
+
+/* Traversal kernels. */
+#define RDRT_TRAVERSAL_FAST 0u      /* ordered short-stack traversal (default) */
+#define RDRT_TRAVERSAL_REFERENCE 1u /* the reference's stackless hit/miss-link order */
+
+/* Ray record (32 B): origin, t_min, direction, t_max.  The reference's
+ * ShaderRayTraceCall ray + range (ty.rs:35-99). */
+typedef struct rdrt_ray {
+  float ox, oy, oz, tmin;
+  float dx, dy, dz, tmax;
+} rdrt_ray;
+
+/* Optional per-ray trace parameters (16 B): ray flags, cull mask and the
+ * index into the bound TLAS list (traverse_cpu.rs:60-63). */
+typedef struct rdrt_ray_meta {
+  uint32_t flags, cull_mask, tlas_idx, pad;
+} rdrt_ray_meta;
+
+/* Closest-hit record (16 B).  prim_id = RDRT_MISS on a miss; t is the world
+ * distance; (u, v) are the barycentrics weighting v1 and v2
+ * (wavefront_compute/geometry/mod.rs:130-154). */
+typedef struct rdrt_hit {
+  uint32_t prim_id;
+  float t, u, v;
+} rdrt_hit;
+
+/* Optional hit context (16 B): geometry index within the BLAS, instance
+ * index within its TLAS source array, hit kind, instance custom index
+ * (shader/ray-tracing/src/api/ctx.rs:24-86). */
+typedef struct rdrt_hit_meta {
+  uint32_t geometry_id, instance_id, hit_kind, instance_custom_index;
+} rdrt_hit_meta;
+
+/* TLAS instance (96 B): TopLevelAccelerationStructureSourceInstance
+ * (shader/ray-tracing/src/api/backend.rs:160-177). */
+typedef struct rdrt_instance {
+  float transform[16];
+  uint32_t instance_custom_index, mask, sbt_offset, flags, blas;
+  uint32_t pad[3];
+} rdrt_instance;
+
+/* BVH build options (content/space/src/utils.rs:192-210 TreeBuildOption,
+ * SAH::new(bins)).  The reference uses BLAS (bin 2, depth 50), TLAS (bin 10,
+ * depth 50), 4 SAH bins (naive/mod.rs:173-177,280-284). */
+typedef struct rdrt_build_opts {
+  uint32_t blas_bin_size, blas_max_depth;
+  uint32_t tlas_bin_size, tlas_max_depth;
+  uint32_t sah_bins;
+} rdrt_build_opts;
+
+typedef struct rdrt_config {
+  int32_t device;        /* HIP device ordinal; -1 = host-only context (build, no trace) */
+  void *stream;          /* hipStream_t to enqueue on; NULL = the null stream */
+  rdrt_build_opts build; /* zero fields take the reference defaults */
+  uint32_t traversal;    /* RDRT_TRAVERSAL_* default for rdrt_trace_closest */
+  uint32_t block_size;   /* 0 = tuned default */
+} rdrt_config;
+
+typedef struct rdrt_ctx rdrt_ctx;
+
+/* One closest-hit launch.  All pointers are device pointers. */
+typedef struct rdrt_trace_args {
+  const rdrt_ray *rays;          /* n rays */
+  const rdrt_ray_meta *ray_meta; /* n records or NULL (then flags/cull_mask/tlas_idx below) */
+  rdrt_hit *hits;                /* n hits */
+  rdrt_hit_meta *hit_meta;       /* n records or NULL */
+  uint32_t *counters;            /* 3 u32 per ray or NULL: node visits, triangle tests, instance tests */
+  uint32_t n;
+  uint32_t flags, cull_mask, tlas_idx;
+  uint32_t traversal; /* RDRT_TRAVERSAL_* */
+  uint32_t pad;
+} rdrt_trace_args;
+
+/* Built-scene statistics. */
+typedef struct rdrt_scene_info {
+  uint32_t tri_nodes, tlas_nodes, tris, verts, instances, geometries, blas, tlas;
+  uint32_t max_blas_depth, max_tlas_depth, stack_entries, pad;
+  uint64_t device_bytes;
+  double build_ms;
+} rdrt_scene_info;
+
+/* ---- errors / version ---- */
+const char *rdrt_last_error(void);
+int32_t rdrt_abi_version(void);
+
+/* ---- context ---- */
+int32_t rdrt_config_default(rdrt_config *out);
+int32_t rdrt_ctx_create(const rdrt_config *cfg, rdrt_ctx **out);
+int32_t rdrt_ctx_destroy(rdrt_ctx *ctx);
+int32_t rdrt_sync(rdrt_ctx *ctx);
+
+/* ---- acceleration structures (backend.rs:120-151) ---- */
+int32_t rdrt_blas_begin(rdrt_ctx *ctx);
+/* One geometry of the pending BLAS: positions are xyz f32 triples (copied);
+ * indices_or_null = NULL means a non-indexed triangle list.  A rejected
+ * geometry aborts the pending BLAS (rdrt_blas_end then fails). */
+int32_t rdrt_blas_add_triangles(rdrt_ctx *ctx, const float *pos_xyz, uint32_t nverts,
+                                const uint32_t *indices_or_null, uint32_t ntris,
+                                uint32_t geometry_flags);
+int32_t rdrt_blas_end(rdrt_ctx *ctx, uint32_t *out_handle);
+int32_t rdrt_blas_destroy(rdrt_ctx *ctx, uint32_t handle);
+int32_t rdrt_tlas_create(rdrt_ctx *ctx, const rdrt_instance *inst, uint32_t n, uint32_t *out_handle);
+int32_t rdrt_tlas_destroy(rdrt_ctx *ctx, uint32_t handle);
+int32_t rdrt_bind_tlas(rdrt_ctx *ctx, const uint32_t *handles, uint32_t n);
+uint32_t rdrt_bind_tlas_max_len(const rdrt_ctx *ctx);
+/* Host-side build of every BVH (no device work). */
+int32_t rdrt_build(rdrt_ctx *ctx);
+/* Build if needed and upload to the device. */
+int32_t rdrt_commit(rdrt_ctx *ctx);
+int32_t rdrt_get_scene_info(const rdrt_ctx *ctx, rdrt_scene_info *out);
+/* Copies of the built reference-layout arrays (host memory, sized from
+ * rdrt_scene_info), for inspection and parity checks.  which:
+ * 0 = BLAS forest (48 B nodes), 1 = TLAS forest (48 B nodes),
+ * 2 = triangle redirect (u32), 3 = instance source order (u32). */
+int32_t rdrt_copy_built(const rdrt_ctx *ctx, uint32_t which, void *host_out, size_t bytes);
+
+/* ---- tracing ---- */
+int32_t rdrt_trace(rdrt_ctx *ctx, const rdrt_trace_args *args);
+/* Closest hit with flags 0, cull mask ~0, TLAS binding 0, the context's
+ * default traversal. */
+int32_t rdrt_trace_closest(rdrt_ctx *ctx, const rdrt_ray *rays_dev, uint32_t n, rdrt_hit *hits_dev);
+/* Average duration (ms) of the traversal kernel launches of the last
+ * rdrt_trace call, measured with HIP events on the context stream; requires
+ * rdrt_sync first. */
+int32_t rdrt_last_trace_ms(rdrt_ctx *ctx, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1032 @@
+// oracle.cpp -- CPU oracle (TEST INFRASTRUCTURE ONLY, never linked by the
+// product).  A plain restatement of the reference CPU ray-query path, with the
+// reference's FP32 operation order.  Build with -ffp-contract=off and no
+// fast-math so every float operation is a single IEEE-754 rounding, as in the
+// reference's Rust code.
+//
+// Parity pins for each part live in tests/ (brute force, closed forms,
+// invariants, golden vectors); see the file header of oracle.h for the list
+// of reference files restated here.
+
+#include "oracle.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr uint32_t INVALID_NEXT = 0xffffffffu;  // naive/mod.rs:611
+
+// ---------------------------------------------------------------- algebra --
+// math/algebra/src/vec/vec3.rs, vec/dimension.rs, vec/mod.rs
+struct V3 {
+  float x, y, z;
+};
+inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+inline V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline V3 operator*(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+inline V3 operator/(V3 a, V3 b) { return v3(a.x / b.x, a.y / b.y, a.z / b.z); }
+inline V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+inline V3 operator*(float s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+inline V3 operator/(V3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
+// vec3.rs:26-28
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// vec3.rs:116-123
+inline V3 cross(V3 a, V3 b) {
+  return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+// Rust f32::min / f32::max ignore a NaN operand, like fminf / fmaxf.
+inline float rmin(float a, float b) { return std::fmin(a, b); }
+inline float rmax(float a, float b) { return std::fmax(a, b); }
+// dimension.rs:53-60
+inline V3 vmin(V3 a, V3 b) { return v3(rmin(a.x, b.x), rmin(a.y, b.y), rmin(a.z, b.z)); }
+inline V3 vmax(V3 a, V3 b) { return v3(rmax(a.x, b.x), rmax(a.y, b.y), rmax(a.z, b.z)); }
+// vec3.rs max_channel / min_channel
+inline float max_channel(V3 a) { return rmax(rmax(a.x, a.y), a.z); }
+inline float min_channel(V3 a) { return rmin(rmin(a.x, a.y), a.z); }
+// dimension.rs:98-105 -- multiply by the reciprocal of the length
+inline V3 normalize(V3 a) {
+  float mag_sq = dot(a, a);
+  if (mag_sq > 0.0f) {
+    float inv_sqrt = 1.0f / std::sqrt(mag_sq);
+    return a * inv_sqrt;
+  }
+  return a;
+}
+inline float length(V3 a) { return std::sqrt(dot(a, a)); }
+
+// Column-major Mat4, fields named as math/algebra/src/mat/mat4.rs:9-14.
+struct M4 {
+  float a1, a2, a3, a4, b1, b2, b3, b4, c1, c2, c3, c4, d1, d2, d3, d4;
+};
+inline M4 m4_from(const float *m) {
+  M4 r;
+  std::memcpy(&r, m, sizeof(M4));
+  return r;
+}
+inline M4 m4_identity() {
+  M4 r{};
+  r.a1 = r.b2 = r.c3 = r.d4 = 1.0f;
+  return r;
+}
+// mat4.rs det
+float m4_det(const M4 &m) {
+  return m.a1 * m.b2 * m.c3 * m.d4 - m.a1 * m.b2 * m.c4 * m.d3 + m.a1 * m.b3 * m.c4 * m.d2 -
+         m.a1 * m.b3 * m.c2 * m.d4 + m.a1 * m.b4 * m.c2 * m.d3 - m.a1 * m.b4 * m.c3 * m.d2
+
+         - m.a2 * m.b3 * m.c4 * m.d1 + m.a2 * m.b3 * m.c1 * m.d4 - m.a2 * m.b4 * m.c1 * m.d3 +
+         m.a2 * m.b4 * m.c3 * m.d1 - m.a2 * m.b1 * m.c3 * m.d4 + m.a2 * m.b1 * m.c4 * m.d3
+
+         + m.a3 * m.b4 * m.c1 * m.d2 - m.a3 * m.b4 * m.c2 * m.d1 + m.a3 * m.b1 * m.c2 * m.d4 -
+         m.a3 * m.b1 * m.c4 * m.d2 + m.a3 * m.b2 * m.c4 * m.d1 - m.a3 * m.b2 * m.c1 * m.d4
+
+         - m.a4 * m.b1 * m.c2 * m.d3 + m.a4 * m.b1 * m.c3 * m.d2 - m.a4 * m.b2 * m.c3 * m.d1 +
+         m.a4 * m.b2 * m.c1 * m.d3 - m.a4 * m.b3 * m.c1 * m.d2 + m.a4 * m.b3 * m.c2 * m.d1;
+}
+// mat4.rs inverse + mat/dimension.rs:18-20 inverse_or_identity
+M4 m4_inverse_or_identity(const M4 &m) {
+  float det = m4_det(m);
+  if (det == 0.0f) return m4_identity();
+  float inv_det = 1.0f / det;
+  float ninv = -inv_det;
+  M4 r;
+  r.a1 = inv_det * (m.b2 * (m.c3 * m.d4 - m.c4 * m.d3) + m.b3 * (m.c4 * m.d2 - m.c2 * m.d4) + m.b4 * (m.c2 * m.d3 - m.c3 * m.d2));
+  r.a2 = ninv * (m.a2 * (m.c3 * m.d4 - m.c4 * m.d3) + m.a3 * (m.c4 * m.d2 - m.c2 * m.d4) + m.a4 * (m.c2 * m.d3 - m.c3 * m.d2));
+  r.a3 = inv_det * (m.a2 * (m.b3 * m.d4 - m.b4 * m.d3) + m.a3 * (m.b4 * m.d2 - m.b2 * m.d4) + m.a4 * (m.b2 * m.d3 - m.b3 * m.d2));
+  r.a4 = ninv * (m.a2 * (m.b3 * m.c4 - m.b4 * m.c3) + m.a3 * (m.b4 * m.c2 - m.b2 * m.c4) + m.a4 * (m.b2 * m.c3 - m.b3 * m.c2));
+
+  r.b1 = ninv * (m.b1 * (m.c3 * m.d4 - m.c4 * m.d3) + m.b3 * (m.c4 * m.d1 - m.c1 * m.d4) + m.b4 * (m.c1 * m.d3 - m.c3 * m.d1));
+  r.b2 = inv_det * (m.a1 * (m.c3 * m.d4 - m.c4 * m.d3) + m.a3 * (m.c4 * m.d1 - m.c1 * m.d4) + m.a4 * (m.c1 * m.d3 - m.c3 * m.d1));
+  r.b3 = ninv * (m.a1 * (m.b3 * m.d4 - m.b4 * m.d3) + m.a3 * (m.b4 * m.d1 - m.b1 * m.d4) + m.a4 * (m.b1 * m.d3 - m.b3 * m.d1));
+  r.b4 = inv_det * (m.a1 * (m.b3 * m.c4 - m.b4 * m.c3) + m.a3 * (m.b4 * m.c1 - m.b1 * m.c4) + m.a4 * (m.b1 * m.c3 - m.b3 * m.c1));
+
+  r.c1 = inv_det * (m.b1 * (m.c2 * m.d4 - m.c4 * m.d2) + m.b2 * (m.c4 * m.d1 - m.c1 * m.d4) + m.b4 * (m.c1 * m.d2 - m.c2 * m.d1));
+  r.c2 = ninv * (m.a1 * (m.c2 * m.d4 - m.c4 * m.d2) + m.a2 * (m.c4 * m.d1 - m.c1 * m.d4) + m.a4 * (m.c1 * m.d2 - m.c2 * m.d1));
+  r.c3 = inv_det * (m.a1 * (m.b2 * m.d4 - m.b4 * m.d2) + m.a2 * (m.b4 * m.d1 - m.b1 * m.d4) + m.a4 * (m.b1 * m.d2 - m.b2 * m.d1));
+  r.c4 = ninv * (m.a1 * (m.b2 * m.c4 - m.b4 * m.c2) + m.a2 * (m.b4 * m.c1 - m.b1 * m.c4) + m.a4 * (m.b1 * m.c2 - m.b2 * m.c1));
+
+  r.d1 = ninv * (m.b1 * (m.c2 * m.d3 - m.c3 * m.d2) + m.b2 * (m.c3 * m.d1 - m.c1 * m.d3) + m.b3 * (m.c1 * m.d2 - m.c2 * m.d1));
+  r.d2 = inv_det * (m.a1 * (m.c2 * m.d3 - m.c3 * m.d2) + m.a2 * (m.c3 * m.d1 - m.c1 * m.d3) + m.a3 * (m.c1 * m.d2 - m.c2 * m.d1));
+  r.d3 = ninv * (m.a1 * (m.b2 * m.d3 - m.b3 * m.d2) + m.a2 * (m.b3 * m.d1 - m.b1 * m.d3) + m.a3 * (m.b1 * m.d2 - m.b2 * m.d1));
+  r.d4 = inv_det * (m.a1 * (m.b2 * m.c3 - m.b3 * m.c2) + m.a2 * (m.b3 * m.c1 - m.b1 * m.c3) + m.a3 * (m.b1 * m.c2 - m.b2 * m.c1));
+  return r;
+}
+// mat3.rs:37-42 det of to_mat3()
+float m3_det(const M4 &m) {
+  float t11 = m.c3 * m.b2 - m.b3 * m.c2;
+  float t12 = m.b3 * m.c1 - m.c3 * m.b1;
+  float t13 = m.c2 * m.b1 - m.b2 * m.c1;
+  return m.a1 * t11 + m.a2 * t12 + m.a3 * t13;
+}
+// mat4.rs:155-169: (M * (v,1)).xyz / w
+V3 m4_mul_point(const M4 &m, V3 v) {
+  float w1 = 1.0f;
+  float x = v.x * m.a1 + v.y * m.b1 + v.z * m.c1 + w1 * m.d1;
+  float y = v.x * m.a2 + v.y * m.b2 + v.z * m.c2 + w1 * m.d2;
+  float z = v.x * m.a3 + v.y * m.b3 + v.z * m.c3 + w1 * m.d3;
+  float w = v.x * m.a4 + v.y * m.b4 + v.z * m.c4 + w1 * m.d4;
+  return v3(x, y, z) / w;
+}
+// mat3.rs:103-109 (to_mat3() * v)
+V3 m3_mul(const M4 &m, V3 v) {
+  return v3(v.x * m.a1 + v.y * m.b1 + v.z * m.c1, v.x * m.a2 + v.y * m.b2 + v.z * m.c2,
+            v.x * m.a3 + v.y * m.b3 + v.z * m.c3);
+}
+
+// ----------------------------------------------------------------- Box3 ----
+// math/geometry/src/hyper_aabb.rs:20-58, dimension3/box3.rs
+struct Box3 {
+  V3 min, max;
+};
+inline Box3 box_empty() {
+  const float inf = std::numeric_limits<float>::infinity();
+  return Box3{v3(inf, inf, inf), v3(-inf, -inf, -inf)};
+}
+inline void expand_by_point(Box3 &b, V3 p) {
+  b.min = vmin(b.min, p);
+  b.max = vmax(b.max, p);
+}
+inline void expand_by_other(Box3 &b, const Box3 &o) {
+  b.min = vmin(b.min, o.min);
+  b.max = vmax(b.max, o.max);
+}
+inline bool box_is_empty(const Box3 &b) {
+  return (b.max.x < b.min.x) || (b.max.y < b.min.y) || (b.max.z < b.min.z);
+}
+// box3.rs:90-93
+inline V3 box_center(const Box3 &b) { return (b.min + b.max) * 0.5f; }
+// box3.rs:5-11 (LebesgueMeasurable<T, 2>)
+inline float box_surface_area(const Box3 &b) {
+  float w = b.max.x - b.min.x, h = b.max.y - b.min.y, d = b.max.z - b.min.z;
+  return 2.0f * (w * h + w * d + h * d);
+}
+// box3.rs:117-133 -- ties go to Z, then Y
+inline int longest_axis(const Box3 &b) {
+  float xl = b.max.x - b.min.x, yl = b.max.y - b.min.y, zl = b.max.z - b.min.z;
+  if (xl > yl) return xl > zl ? 0 : 2;
+  return yl > zl ? 1 : 2;
+}
+inline float axis_of(V3 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+// box3.rs:22-38 apply_matrix: 8 corners in 000..111 order, then collect
+Box3 box_apply_matrix(const Box3 &b, const M4 &m) {
+  if (box_is_empty(b)) return b;
+  V3 pts[8] = {
+      m4_mul_point(m, v3(b.min.x, b.min.y, b.min.z)), m4_mul_point(m, v3(b.min.x, b.min.y, b.max.z)),
+      m4_mul_point(m, v3(b.min.x, b.max.y, b.min.z)), m4_mul_point(m, v3(b.min.x, b.max.y, b.max.z)),
+      m4_mul_point(m, v3(b.max.x, b.min.y, b.min.z)), m4_mul_point(m, v3(b.max.x, b.min.y, b.max.z)),
+      m4_mul_point(m, v3(b.max.x, b.max.y, b.min.z)), m4_mul_point(m, v3(b.max.x, b.max.y, b.max.z)),
+  };
+  Box3 r = box_empty();
+  for (auto &p : pts) expand_by_point(r, p);
+  return r;
+}
+
+// -------------------------------------------------------- BVH build --------
+// content/space/src/bvh/node.rs:124-171
+struct FNode {
+  Box3 bounding;
+  size_t start, end;  // primitive_range
+  size_t self_index;
+  bool has_child;
+  size_t left_count;
+};
+
+struct BuildPrim {
+  Box3 bounding;
+  V3 center;
+};
+
+struct TreeBuildOption {  // utils.rs:192-210
+  size_t max_tree_depth, bin_size;
+  bool should_continue(size_t count, size_t depth) const {
+    return depth < max_tree_depth && count > bin_size;
+  }
+};
+
+struct FlattenBVH {
+  std::vector<FNode> nodes;
+  std::vector<size_t> sorted;
+  size_t depth = 0;  // deepest node depth reached (root = 0)
+};
+
+struct Split {
+  Box3 lb;
+  size_t ls, le;
+  Box3 rb;
+  size_t rs, re;
+};
+
+// utils.rs:224-238 bounding_from_build_source
+Box3 bounding_of(const std::vector<size_t> &idx, const std::vector<BuildPrim> &prims, size_t s, size_t e) {
+  Box3 b = box_empty();
+  for (size_t i = s; i < e; i++) expand_by_other(b, prims[idx[i]].bounding);
+  return b;
+}
+
+// strategy.rs:66-87 BalanceTree::split + apply.rs:304-333 median_partition_at_axis
+Split balance_split(const FNode &parent, const std::vector<BuildPrim> &prims, std::vector<size_t> &idx) {
+  int axis = longest_axis(parent.bounding);
+  size_t s = parent.start, e = parent.end;
+  size_t middle = (e + s) / 2;
+  size_t range_middle = (e - s) / 2;
+  if (range_middle != 0) {
+    // select_nth_unstable_by with partial_cmp().unwrap_or(Less): a NaN compares Less.
+    auto less = [&](size_t a, size_t b) {
+      float ca = axis_of(prims[a].center, axis), cb = axis_of(prims[b].center, axis);
+      if (std::isnan(ca) || std::isnan(cb)) return true;
+      return ca < cb;
+    };
+    std::nth_element(idx.begin() + s, idx.begin() + s + range_middle, idx.begin() + e, less);
+  }
+  Split r;
+  r.ls = s;
+  r.le = middle;
+  r.rs = middle;
+  r.re = e;
+  r.lb = bounding_of(idx, prims, s, middle);
+  r.rb = bounding_of(idx, prims, middle, e);
+  return r;
+}
+
+// strategy.rs:201-284 SAH::split
+Split sah_split(const FNode &parent, const std::vector<BuildPrim> &prims, std::vector<size_t> &idx,
+                size_t nbins) {
+  std::vector<Box3> bucket_box(nbins, box_empty());
+  std::vector<std::vector<size_t>> bucket(nbins);
+  int axis = longest_axis(parent.bounding);
+  float a0 = axis_of(parent.bounding.min, axis);
+  float a1 = axis_of(parent.bounding.max, axis);
+  float step = (a1 - a0) / (float)nbins;
+  for (size_t i = parent.start; i < parent.end; i++) {
+    size_t index = idx[i];
+    float value = axis_of(prims[index].center, axis);
+    float f = std::floor((value - a0) / step);
+    // Rust `as usize` saturates: NaN and negatives map to 0.
+    size_t which;
+    if (!(f > 0.0f)) which = 0;
+    else if (f >= 18446744073709551615.0f) which = SIZE_MAX;
+    else which = (size_t)f;
+    if (which == nbins) which -= 1;  // edge case, strategy.rs:223-226
+    if (which > nbins) which = nbins - 1;  // unreachable for finite bounds (the reference panics)
+    expand_by_other(bucket_box[which], prims[index].bounding);
+    bucket[which].push_back(index);
+  }
+  size_t empty = 0;
+  for (auto &b : bucket) empty += b.empty() ? 1 : 0;
+  if (empty == nbins - 1) return balance_split(parent, prims, idx);  // strategy.rs:121-128,230-233
+
+  // step 2: partition decisions (strategy.rs:236-259)
+  size_t best = 0;
+  float best_cost = std::numeric_limits<float>::infinity();
+  std::vector<Box3> lbox(nbins - 1), rbox(nbins - 1);
+  std::vector<size_t> lcnt(nbins - 1), rcnt(nbins - 1);
+  for (size_t i = 0; i + 1 < nbins; i++) {
+    Box3 l = box_empty(), r = box_empty();
+    size_t lc = 0, rc = 0;
+    for (size_t k = 0; k <= i; k++) {
+      lc += bucket[k].size();
+      expand_by_other(l, bucket_box[k]);
+    }
+    for (size_t k = i + 1; k < nbins; k++) {
+      rc += bucket[k].size();
+      expand_by_other(r, bucket_box[k]);
+    }
+    float cost = box_surface_area(l) * (float)lc + box_surface_area(r) * (float)rc;
+    lbox[i] = l;
+    rbox[i] = r;
+    lcnt[i] = lc;
+    rcnt[i] = rc;
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = i;
+    }
+  }
+  // step 3: reorder by bucket, stable within a bucket (strategy.rs:264-271)
+  size_t ptr = parent.start;
+  for (size_t b = 0; b < nbins; b++)
+    for (size_t i : bucket[b]) idx[ptr++] = i;
+  Split r;
+  r.lb = lbox[best];
+  r.ls = parent.start;
+  r.le = parent.start + lcnt[best];
+  r.rb = rbox[best];
+  r.rs = parent.start + lcnt[best];
+  r.re = parent.end;
+  return r;
+}
+
+// strategy.rs:11-43 BVHBuildStrategy::build (recursive, DFS preorder)
+size_t build_rec(const TreeBuildOption &opt, const std::vector<BuildPrim> &prims, std::vector<size_t> &idx,
+                 std::vector<FNode> &nodes, size_t depth, size_t nbins, size_t &max_depth) {
+  if (depth > max_depth) max_depth = depth;
+  FNode node = nodes.back();
+  if (!opt.should_continue(node.end - node.start, depth)) return 1;
+  Split sp = sah_split(node, prims, idx, nbins);
+  size_t node_index = nodes.size() - 1;
+  nodes.push_back(FNode{sp.lb, sp.ls, sp.le, nodes.size(), false, 0});
+  size_t left_count = build_rec(opt, prims, idx, nodes, depth + 1, nbins, max_depth);
+  nodes.push_back(FNode{sp.rb, sp.rs, sp.re, nodes.size(), false, 0});
+  size_t right_count = build_rec(opt, prims, idx, nodes, depth + 1, nbins, max_depth);
+  nodes[node_index].has_child = true;
+  nodes[node_index].left_count = left_count;
+  return left_count + right_count + 1;
+}
+
+// mod.rs:55-79 FlattenBVH::new
+FlattenBVH flatten_bvh_new(const std::vector<Box3> &boxes, const TreeBuildOption &opt, size_t nbins) {
+  std::vector<BuildPrim> prims(boxes.size());
+  std::vector<size_t> idx(boxes.size());
+  for (size_t i = 0; i < boxes.size(); i++) {
+    idx[i] = i;
+    prims[i].bounding = boxes[i];
+    prims[i].center = box_center(boxes[i]);
+  }
+  Box3 root = bounding_of(idx, prims, 0, idx.size());
+  FlattenBVH bvh;
+  bvh.nodes.push_back(FNode{root, 0, idx.size(), 0, false, 0});
+  size_t maxd = 0;
+  build_rec(opt, prims, idx, bvh.nodes, 0, nbins, maxd);
+  bvh.sorted = std::move(idx);
+  bvh.depth = maxd;
+  return bvh;
+}
+
+// naive/mod.rs:611-632 compute_bvh_next
+std::vector<std::pair<uint32_t, uint32_t>> compute_bvh_next(const std::vector<FNode> &nodes) {
+  std::vector<std::pair<uint32_t, uint32_t>> result;
+  std::vector<uint32_t> next_stack;
+  for (const FNode &node : nodes) {
+    if (!next_stack.empty() && next_stack.back() == (uint32_t)node.self_index) next_stack.pop_back();
+    uint32_t miss = next_stack.empty() ? INVALID_NEXT : next_stack.back();
+    if (node.has_child) {
+      uint32_t hit = (uint32_t)(node.self_index + 1);
+      next_stack.push_back((uint32_t)(node.self_index + node.left_count + 1));
+      result.push_back({hit, miss});
+    } else {
+      result.push_back({miss, miss});
+    }
+  }
+  return result;
+}
+
+// naive/mod.rs:338-367 flatten_bvh_to_gpu_node
+orc_node to_gpu_node(const FNode &n, uint32_t hit, uint32_t miss, uint32_t next_offset, uint32_t prim_offset) {
+  orc_node g;
+  std::memset(&g, 0, sizeof(g));
+  g.aabb_min[0] = n.bounding.min.x;
+  g.aabb_min[1] = n.bounding.min.y;
+  g.aabb_min[2] = n.bounding.min.z;
+  g.aabb_max[0] = n.bounding.max.x;
+  g.aabb_max[1] = n.bounding.max.y;
+  g.aabb_max[2] = n.bounding.max.z;
+  g.hit_next = hit != INVALID_NEXT ? hit + next_offset : INVALID_NEXT;
+  g.miss_next = miss != INVALID_NEXT ? miss + next_offset : INVALID_NEXT;
+  g.content_range[0] = (uint32_t)n.start + prim_offset;
+  g.content_range[1] = (uint32_t)n.end + prim_offset;
+  return g;
+}
+
+// ------------------------------------------------------------ flags --------
+// shader/ray-tracing/src/api/ty.rs:102-159, naive/flag.rs
+enum : uint32_t {
+  F_FORCE_OPAQUE = 0x01,
+  F_FORCE_NON_OPAQUE = 0x02,
+  F_ACCEPT_FIRST_HIT_AND_END_SEARCH = 0x04,
+  F_CULL_BACK = 0x10,
+  F_CULL_FRONT = 0x20,
+  F_CULL_OPAQUE = 0x40,
+  F_CULL_NON_OPAQUE = 0x80,
+  F_SKIP_TRIANGLES = 0x100,
+  F_TRIANGLE_FLIP_FACING = 0x400,
+  GI_CULL_DISABLE = 0x1,
+  GI_FLIP_FACING = 0x2,
+  GI_FORCE_OPAQUE = 0x4,
+  GI_FORCE_NO_OPAQUE = 0x8,
+  GEOMETRY_FLAG_OPAQUE = 0x1,
+};
+// flag.rs:30-75
+uint32_t merge_instance_flags(uint32_t f, uint32_t gi) {
+  if (gi & GI_CULL_DISABLE) f &= ~(F_CULL_BACK | F_CULL_FRONT);
+  if (gi & GI_FLIP_FACING) f ^= F_TRIANGLE_FLIP_FACING;
+  if (gi & GI_FORCE_OPAQUE) f |= F_FORCE_OPAQUE;
+  if (gi & GI_FORCE_NO_OPAQUE) f |= F_FORCE_NON_OPAQUE;
+  return f;
+}
+// flag.rs:78-91
+void cull_geometry(uint32_t f, uint32_t gflags, bool &pass, bool &is_opaque) {
+  bool geometry_opaque = (gflags & GEOMETRY_FLAG_OPAQUE) != 0;
+  bool force_opaque = (f & F_FORCE_OPAQUE) != 0, force_non_opaque = (f & F_FORCE_NON_OPAQUE) != 0;
+  bool cull_opaque = (f & F_CULL_OPAQUE) != 0, cull_non_opaque = (f & F_CULL_NON_OPAQUE) != 0;
+  is_opaque = (geometry_opaque || force_opaque) && !force_non_opaque;
+  pass = (is_opaque && !cull_opaque) || (!is_opaque && !cull_non_opaque);
+}
+// flag.rs:94-105
+void cull_triangle(uint32_t f, bool &cull_enable, bool &cull_back) {
+  bool flip = (f & F_TRIANGLE_FLIP_FACING) != 0;
+  bool cf = (f & F_CULL_FRONT) != 0, cb = (f & F_CULL_BACK) != 0;
+  cull_enable = cf || cb;
+  cull_back = (flip && cb) || (!flip && cf);
+}
+
+// ------------------------------------------------------ intersections ------
+// geometry/mod.rs:44-64
+bool intersect_aabb(V3 o, V3 d, float tmin, float tmax, V3 bmin, V3 bmax) {
+  V3 inv_d = v3(1.0f, 1.0f, 1.0f) / d;
+  V3 t0 = (bmin - o) * inv_d;
+  V3 t1 = (bmax - o) * inv_d;
+  V3 tn = vmin(t0, t1);
+  V3 tf = vmax(t0, t1);
+  float tnm = max_channel(tn);
+  float tfm = min_channel(tf);
+  return tnm <= tfm && tmin < tfm && tnm < tmax;
+}
+
+// geometry/mod.rs:104-155.  Returns (sign, t, u, v); miss = zeros.
+// b == 0 is a miss (the reference GPU path's sign(0) = 0); the reference CPU
+// path would go on with an infinite or NaN t there.
+void intersect_triangle(V3 o, V3 d, float rnear, float rfar, V3 v0, V3 v1, V3 v2, bool cull_enable,
+                        bool cull_back, float out[4]) {
+  out[0] = out[1] = out[2] = out[3] = 0.0f;
+  V3 e1 = v1 - v0;
+  V3 e2 = v2 - v0;
+  V3 normal = normalize(cross(e1, e2));
+  float b = dot(normal, d);
+  if (b == 0.0f) return;
+  float sign = b > 0.0f ? 1.0f : -1.0f;  // f32::signum for b != 0
+  if (cull_enable) {
+    bool pass = cull_back != (b < 0.0f);
+    if (!pass) return;
+  }
+  V3 w0 = o - v0;
+  float a = -dot(normal, w0);
+  float t = a / b;
+  if (t < rnear || t > rfar) return;
+  V3 p = o + t * d;
+  float uu = dot(e1, e1);
+  float uv = dot(e1, e2);
+  float vv = dot(e2, e2);
+  V3 w = p - v0;
+  float wu = dot(w, e1);
+  float wv = dot(w, e2);
+  float inverse_d = 1.0f / (uv * uv - uu * vv);
+  float u = (uv * wv - vv * wu) * inverse_d;
+  if (u < 0.0f || u > 1.0f) return;
+  float v = (uv * wu - uu * wv) * inverse_d;
+  if (v < 0.0f || (u + v) > 1.0f) return;
+  out[0] = sign;
+  out[1] = t;
+  out[2] = u;
+  out[3] = v;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- scene -------
+struct Geometry {
+  std::vector<float> pos;
+  std::vector<uint32_t> idx;
+  bool indexed;
+  uint32_t ntris, flags;
+};
+
+struct orc_scene {
+  orc_build_opts opts;
+  std::vector<bool> blas_live;
+  std::vector<std::vector<Geometry>> blas_data;
+  std::vector<Geometry> pending;
+  std::vector<bool> tlas_live;
+  std::vector<std::vector<orc_instance>> tlas_data_src;
+  std::vector<uint32_t> binding;
+
+  // built (naive/traverse_cpu.rs:9-35)
+  std::vector<uint32_t> tlas_bvh_root;
+  std::vector<orc_node> tlas_forest;
+  std::vector<M4> inst_transform_inv;
+  std::vector<uint32_t> inst_custom, inst_sbt, inst_flags, inst_blas, inst_order;
+  std::vector<Box3> tlas_bounding_box;
+  std::vector<uint32_t> tlas_bounding_mask, tlas_bounding_flags;
+  std::vector<std::pair<uint32_t, uint32_t>> blas_meta;
+  struct GeoMeta {
+    uint32_t root, geometry_idx, primitive_start, flags;
+  };
+  std::vector<GeoMeta> geo_meta;
+  std::vector<orc_node> tri_forest;
+  std::vector<uint32_t> redirect, indices;
+  std::vector<V3> vertices;
+  uint32_t max_blas_depth = 0, max_tlas_depth = 0;
+  bool built = false;
+};
+
+extern "C" {
+
+orc_scene *orc_scene_new(const orc_build_opts *opts) {
+  orc_scene *s = new orc_scene();
+  if (opts) s->opts = *opts;
+  else s->opts = orc_build_opts{2, 50, 10, 50, 4};  // naive/mod.rs:173-177,280-284
+  if (s->opts.sah_bins < 2) s->opts.sah_bins = 4;
+  return s;
+}
+void orc_scene_free(orc_scene *s) { delete s; }
+
+void orc_blas_begin(orc_scene *s) { s->pending.clear(); }
+void orc_blas_add_triangles(orc_scene *s, const float *pos, uint32_t nverts, const uint32_t *indices,
+                            uint32_t ntris, uint32_t gflags) {
+  Geometry g;
+  g.pos.assign(pos, pos + 3 * (size_t)nverts);
+  g.indexed = indices != nullptr;
+  if (indices) g.idx.assign(indices, indices + 3 * (size_t)ntris);
+  else ntris = nverts / 3;
+  g.ntris = ntris;
+  g.flags = gflags;
+  s->pending.push_back(std::move(g));
+}
+uint32_t orc_blas_end(orc_scene *s) {
+  s->blas_data.push_back(std::move(s->pending));
+  s->blas_live.push_back(true);
+  s->pending.clear();
+  s->built = false;
+  return (uint32_t)(s->blas_data.size() - 1);
+}
+void orc_blas_delete(orc_scene *s, uint32_t h) {
+  if (h < s->blas_live.size()) s->blas_live[h] = false;
+  s->built = false;
+}
+uint32_t orc_tlas_create(orc_scene *s, const orc_instance *inst, uint32_t n) {
+  s->tlas_data_src.emplace_back(inst, inst + n);
+  s->tlas_live.push_back(true);
+  s->built = false;
+  return (uint32_t)(s->tlas_data_src.size() - 1);
+}
+void orc_tlas_delete(orc_scene *s, uint32_t h) {
+  if (h < s->tlas_live.size()) s->tlas_live[h] = false;
+  s->built = false;
+}
+void orc_bind_tlas(orc_scene *s, const uint32_t *h, uint32_t n) {
+  s->binding.assign(h, h + n);
+  s->built = false;
+}
+
+// naive/mod.rs:122-494
+int orc_build(orc_scene *s) {
+  s->tri_forest.clear();
+  s->redirect.clear();
+  s->indices.clear();
+  s->vertices.clear();
+  s->blas_meta.clear();
+  s->geo_meta.clear();
+  s->tlas_bvh_root.clear();
+  s->tlas_forest.clear();
+  s->inst_transform_inv.clear();
+  s->inst_custom.clear();
+  s->inst_sbt.clear();
+  s->inst_flags.clear();
+  s->inst_blas.clear();
+  s->inst_order.clear();
+  s->tlas_bounding_box.clear();
+  s->tlas_bounding_mask.clear();
+  s->tlas_bounding_flags.clear();
+  s->max_blas_depth = s->max_tlas_depth = 0;
+
+  TreeBuildOption blas_opt{s->opts.blas_max_depth, s->opts.blas_bin_size};
+  TreeBuildOption tlas_opt{s->opts.tlas_max_depth, s->opts.tlas_bin_size};
+  size_t nbins = s->opts.sah_bins;
+
+  // build_blas (mod.rs:122-260).  The per-BLAS box is the union of its
+  // geometries' root boxes; an empty or deleted BLAS has no box.
+  std::vector<bool> blas_has_box;
+  std::vector<Box3> blas_box;
+  struct TriBvh {
+    std::vector<orc_node> nodes;
+    uint32_t primitive_start, geometry_idx, flags;
+  };
+  std::vector<TriBvh> tri_bvh;
+  for (size_t bi = 0; bi < s->blas_data.size(); bi++) {
+    if (!s->blas_live[bi]) {
+      s->blas_meta.push_back({0, 0});
+      blas_has_box.push_back(false);
+      blas_box.push_back(box_empty());
+      continue;
+    }
+    uint32_t tri_start = (uint32_t)tri_bvh.size();
+    Box3 blas_root_box = box_empty();
+    for (size_t gi = 0; gi < s->blas_data[bi].size(); gi++) {
+      const Geometry &g = s->blas_data[bi][gi];
+      uint32_t primitive_start = (uint32_t)(s->indices.size() / 3);
+      uint32_t vertex_start = (uint32_t)s->vertices.size();
+      size_t nv = g.pos.size() / 3;
+      for (size_t v = 0; v < nv; v++) s->vertices.push_back(v3(g.pos[3 * v], g.pos[3 * v + 1], g.pos[3 * v + 2]));
+      std::vector<uint32_t> tri_idx;
+      if (g.indexed) tri_idx = g.idx;
+      else {
+        tri_idx.resize(nv);
+        for (size_t v = 0; v < nv; v++) tri_idx[v] = (uint32_t)v;
+      }
+      size_t ntri = tri_idx.size() / 3;
+      std::vector<Box3> boxes(ntri);
+      for (size_t t = 0; t < ntri; t++) {
+        Box3 b = box_empty();
+        for (int k = 0; k < 3; k++) {
+          const float *p = &g.pos[3 * (size_t)tri_idx[3 * t + k]];
+          expand_by_point(b, v3(p[0], p[1], p[2]));
+        }
+        boxes[t] = b;
+      }
+      FlattenBVH bvh = flatten_bvh_new(boxes, blas_opt, nbins);
+      if (bvh.depth > s->max_blas_depth) s->max_blas_depth = (uint32_t)bvh.depth;
+      Box3 root_box = box_empty();
+      expand_by_other(root_box, bvh.nodes[0].bounding);
+      expand_by_other(blas_root_box, root_box);
+      auto links = compute_bvh_next(bvh.nodes);
+      for (size_t k : bvh.sorted) s->redirect.push_back(primitive_start + (uint32_t)k);
+      for (size_t t = 0; t < ntri; t++)
+        for (int k = 0; k < 3; k++) s->indices.push_back(vertex_start + tri_idx[3 * t + k]);
+      TriBvh tb;
+      tb.primitive_start = primitive_start;
+      tb.geometry_idx = (uint32_t)gi;
+      tb.flags = g.flags;
+      // keep node-local links; rebased below when placed in the forest
+      for (size_t k = 0; k < bvh.nodes.size(); k++) tb.nodes.push_back(to_gpu_node(bvh.nodes[k], links[k].first, links[k].second, 0, 0));
+      tri_bvh.push_back(std::move(tb));
+    }
+    s->blas_meta.push_back({tri_start, (uint32_t)tri_bvh.size()});
+    blas_has_box.push_back(!s->blas_data[bi].empty());
+    blas_box.push_back(blas_root_box);
+  }
+  // tri forest (mod.rs:368-385)
+  for (const TriBvh &tb : tri_bvh) {
+    uint32_t bvh_start = (uint32_t)s->tri_forest.size();
+    s->geo_meta.push_back({bvh_start, tb.geometry_idx, tb.primitive_start, tb.flags});
+    for (orc_node n : tb.nodes) {
+      if (n.hit_next != INVALID_NEXT) n.hit_next += bvh_start;
+      if (n.miss_next != INVALID_NEXT) n.miss_next += bvh_start;
+      n.content_range[0] += tb.primitive_start;
+      n.content_range[1] += tb.primitive_start;
+      s->tri_forest.push_back(n);
+    }
+  }
+
+  // build_tlas (mod.rs:262-320) + forest (mod.rs:435-454)
+  for (size_t ti = 0; ti < s->tlas_data_src.size(); ti++) {
+    if (!s->tlas_live[ti]) {
+      s->tlas_bvh_root.push_back(INVALID_NEXT);
+      continue;
+    }
+    const auto &src = s->tlas_data_src[ti];
+    uint32_t bvh_start = (uint32_t)s->tlas_forest.size();
+    uint32_t primitive_start = (uint32_t)s->inst_blas.size();
+    std::vector<Box3> aabbs;
+    for (const orc_instance &in : src) {
+      if (in.blas >= blas_box.size() || !blas_has_box[in.blas]) return -1;  // reference: unwrap() panics
+      aabbs.push_back(box_apply_matrix(blas_box[in.blas], m4_from(in.transform)));
+    }
+    FlattenBVH bvh = flatten_bvh_new(aabbs, tlas_opt, nbins);
+    if (bvh.depth > s->max_tlas_depth) s->max_tlas_depth = (uint32_t)bvh.depth;
+    auto links = compute_bvh_next(bvh.nodes);
+    for (size_t k : bvh.sorted) {
+      const orc_instance &in = src[k];
+      M4 m = m4_from(in.transform);
+      uint32_t flags = in.flags;
+      if (m3_det(m) < 0.0f) flags ^= GI_FLIP_FACING;
+      s->inst_transform_inv.push_back(m4_inverse_or_identity(m));
+      s->inst_custom.push_back(in.instance_custom_index);
+      s->inst_sbt.push_back(in.sbt_offset);
+      s->inst_flags.push_back(flags);
+      s->inst_blas.push_back(in.blas);
+      s->inst_order.push_back((uint32_t)k);
+      s->tlas_bounding_box.push_back(aabbs[k]);
+      s->tlas_bounding_mask.push_back(in.mask);
+      s->tlas_bounding_flags.push_back(flags);
+    }
+    for (size_t k = 0; k < bvh.nodes.size(); k++)
+      s->tlas_forest.push_back(to_gpu_node(bvh.nodes[k], links[k].first, links[k].second, bvh_start, primitive_start));
+    s->tlas_bvh_root.push_back(bvh_start);
+  }
+  s->built = true;
+  return 0;
+}
+
+void orc_get_sizes(const orc_scene *s, orc_sizes *o) {
+  o->tri_nodes = (uint32_t)s->tri_forest.size();
+  o->tlas_nodes = (uint32_t)s->tlas_forest.size();
+  o->tris = (uint32_t)s->redirect.size();
+  o->verts = (uint32_t)s->vertices.size();
+  o->instances = (uint32_t)s->inst_blas.size();
+  o->geometries = (uint32_t)s->geo_meta.size();
+  o->blas = (uint32_t)s->blas_meta.size();
+  o->tlas = (uint32_t)s->tlas_bvh_root.size();
+  o->max_blas_depth = s->max_blas_depth;
+  o->max_tlas_depth = s->max_tlas_depth;
+}
+void orc_copy_tri_forest(const orc_scene *s, orc_node *out) {
+  if (!s->tri_forest.empty()) std::memcpy(out, s->tri_forest.data(), s->tri_forest.size() * sizeof(orc_node));
+}
+void orc_copy_tlas_forest(const orc_scene *s, orc_node *out) {
+  if (!s->tlas_forest.empty()) std::memcpy(out, s->tlas_forest.data(), s->tlas_forest.size() * sizeof(orc_node));
+}
+void orc_copy_indices_redirect(const orc_scene *s, uint32_t *out) {
+  if (!s->redirect.empty()) std::memcpy(out, s->redirect.data(), s->redirect.size() * 4);
+}
+void orc_copy_instance_order(const orc_scene *s, uint32_t *out) {
+  if (!s->inst_order.empty()) std::memcpy(out, s->inst_order.data(), s->inst_order.size() * 4);
+}
+void orc_copy_geometry_meta(const orc_scene *s, uint32_t *out) {
+  for (size_t i = 0; i < s->geo_meta.size(); i++) {
+    out[4 * i + 0] = s->geo_meta[i].root;
+    out[4 * i + 1] = s->geo_meta[i].geometry_idx;
+    out[4 * i + 2] = s->geo_meta[i].primitive_start;
+    out[4 * i + 3] = s->geo_meta[i].flags;
+  }
+}
+void orc_copy_tlas_bounding(const orc_scene *s, float *out) {
+  for (size_t i = 0; i < s->tlas_bounding_box.size(); i++) {
+    const Box3 &b = s->tlas_bounding_box[i];
+    out[8 * i + 0] = b.min.x;
+    out[8 * i + 1] = b.min.y;
+    out[8 * i + 2] = b.min.z;
+    std::memcpy(&out[8 * i + 3], &s->tlas_bounding_mask[i], 4);
+    out[8 * i + 4] = b.max.x;
+    out[8 * i + 5] = b.max.y;
+    out[8 * i + 6] = b.max.z;
+    std::memcpy(&out[8 * i + 7], &s->tlas_bounding_flags[i], 4);
+  }
+}
+void orc_copy_transform_inv(const orc_scene *s, float *out) {
+  for (size_t i = 0; i < s->inst_transform_inv.size(); i++)
+    std::memcpy(out + 16 * i, &s->inst_transform_inv[i], 64);
+}
+
+}  // extern "C"
+
+namespace {
+
+struct Counters {
+  uint32_t nodes = 0, tris = 0, inst = 0;
+};
+
+// traverse_cpu.rs:283-319 -- stackless iterator: yields leaf node indices.
+struct BvhIter {
+  const std::vector<orc_node> *bvh;
+  V3 o, d;
+  float near_, scaling;
+  const float *far_;  // shared far (RayRange.far is an Rc<Cell<f32>>)
+  uint32_t curr;
+  Counters *cnt;
+  bool next(uint32_t &out) {
+    while (curr != INVALID_NEXT) {
+      cnt->nodes++;
+      const orc_node &n = (*bvh)[curr];
+      // RayRange::get (traverse_cpu.rs:277-280)
+      float rn = near_ * scaling, rf = *far_ * scaling;
+      if (intersect_aabb(o, d, rn, rf, v3(n.aabb_min[0], n.aabb_min[1], n.aabb_min[2]),
+                         v3(n.aabb_max[0], n.aabb_max[1], n.aabb_max[2]))) {
+        uint32_t c = curr;
+        curr = n.hit_next;
+        if (n.hit_next == n.miss_next) {
+          out = c;
+          return true;
+        }
+      } else {
+        curr = n.miss_next;
+      }
+    }
+    return false;
+  }
+};
+
+struct RayIn {
+  V3 o, d;
+  float tmin, tmax;
+  uint32_t flags, mask, tlas_idx;
+};
+
+// traverse_cpu.rs:52-245
+bool traverse_one(const orc_scene *s, const RayIn &ray, orc_hit &hit, orc_hit_meta &hm, Counters &cnt) {
+  bool found = false;
+  uint32_t flags0 = ray.flags;
+  float far_ = ray.tmax;  // RayRange::new(range.x, range.y, 1.)
+  hit.prim_id = 0xffffffffu;
+  hit.t = hit.u = hit.v = 0.0f;
+  hm.geometry_id = hm.instance_id = hm.instance_custom_index = 0xffffffffu;
+  hm.hit_kind = 0;
+  if (ray.tlas_idx >= s->binding.size()) return false;
+  uint32_t th = s->binding[ray.tlas_idx];
+  if (th >= s->tlas_bvh_root.size()) return false;
+  uint32_t root = s->tlas_bvh_root[th];
+  BvhIter tlas_iter{&s->tlas_forest, ray.o, ray.d, ray.tmin, 1.0f, &far_, root, &cnt};
+  uint32_t leaf;
+  while (tlas_iter.next(leaf)) {
+    const orc_node &node = s->tlas_forest[leaf];
+    for (uint32_t ti = node.content_range[0]; ti < node.content_range[1]; ti++) {
+      cnt.inst++;
+      // the instance bounding uses the original ray range (traverse_cpu.rs:80-86)
+      const Box3 &tb = s->tlas_bounding_box[ti];
+      if (!intersect_aabb(ray.o, ray.d, ray.tmin, ray.tmax, tb.min, tb.max)) continue;
+      if ((ray.mask & s->tlas_bounding_mask[ti]) == 0) continue;
+      uint32_t blas_idx = s->inst_blas[ti];
+      uint32_t flags = merge_instance_flags(flags0, s->inst_flags[ti]);
+      const M4 &inv = s->inst_transform_inv[ti];
+      V3 bo = m4_mul_point(inv, ray.o);
+      V3 bd = m3_mul(inv, ray.d);
+      float scaling = length(bd);
+      bd = normalize(bd);
+      auto bm = s->blas_meta[blas_idx];
+      if ((flags & F_SKIP_TRIANGLES) != 0) continue;
+      for (uint32_t gr = bm.first; gr < bm.second; gr++) {
+        const auto &geo = s->geo_meta[gr];
+        bool pass, is_opaque;
+        cull_geometry(flags, geo.flags, pass, is_opaque);
+        (void)is_opaque;  // non-opaque geometry: the default any-hit accepts
+        if (!pass) continue;
+        bool cull_enable, cull_back;
+        cull_triangle(flags, cull_enable, cull_back);
+        BvhIter it{&s->tri_forest, bo, bd, ray.tmin, scaling, &far_, geo.root, &cnt};
+        uint32_t hl;
+        while (it.next(hl)) {
+          const orc_node &ln = s->tri_forest[hl];
+          for (uint32_t k = ln.content_range[0]; k < ln.content_range[1]; k++) {
+            uint32_t tri = s->redirect[k];
+            V3 v0 = s->vertices[s->indices[3 * (size_t)tri]];
+            V3 v1 = s->vertices[s->indices[3 * (size_t)tri + 1]];
+            V3 v2 = s->vertices[s->indices[3 * (size_t)tri + 2]];
+            cnt.tris++;
+            float r[4];
+            intersect_triangle(bo, bd, ray.tmin * scaling, far_ * scaling, v0, v1, v2, cull_enable, cull_back, r);
+            if (r[0] != 0.0f) {
+              float distance = r[1] / scaling;
+              far_ = distance;  // update_far
+              found = true;
+              hit.prim_id = tri - geo.primitive_start;
+              hit.t = distance;
+              hit.u = r[2];
+              hit.v = r[3];
+              hm.geometry_id = geo.geometry_idx;
+              hm.instance_id = s->inst_order[ti];
+              hm.instance_custom_index = s->inst_custom[ti];
+              hm.hit_kind = r[0] < 0.0f ? 0xFFu : 0xFEu;  // ty.rs:139-140
+              if (flags & F_ACCEPT_FIRST_HIT_AND_END_SEARCH) return true;
+            }
+          }
+        }
+      }
+    }
+  }
+  return found;
+}
+
+RayIn load_ray(const orc_ray &r, const orc_ray_meta *m, uint32_t flags, uint32_t mask, uint32_t tlas) {
+  RayIn ri;
+  ri.o = v3(r.ox, r.oy, r.oz);
+  ri.d = v3(r.dx, r.dy, r.dz);
+  ri.tmin = r.tmin;
+  ri.tmax = r.tmax;
+  ri.flags = m ? m->flags : flags;
+  ri.mask = m ? m->cull_mask : mask;
+  ri.tlas_idx = m ? m->tlas_idx : tlas;
+  return ri;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_trace(const orc_scene *s, const orc_ray *rays, const orc_ray_meta *meta, uint32_t n, uint32_t flags,
+              uint32_t cull_mask, uint32_t tlas_idx, orc_hit *hits, orc_hit_meta *hit_meta, uint32_t *counters,
+              int nthreads) {
+  if (!s->built) return -1;
+#ifdef _OPENMP
+  int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(nt)
+#endif
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    RayIn ri = load_ray(rays[i], meta ? &meta[i] : nullptr, flags, cull_mask, tlas_idx);
+    orc_hit h;
+    orc_hit_meta hm;
+    Counters c;
+    traverse_one(s, ri, h, hm, c);
+    hits[i] = h;
+    if (hit_meta) hit_meta[i] = hm;
+    if (counters) {
+      counters[3 * i + 0] = c.nodes;
+      counters[3 * i + 1] = c.tris;
+      counters[3 * i + 2] = c.inst;
+    }
+  }
+  return 0;
+}
+
+int orc_trace_brute(const orc_scene *s, const orc_ray *rays, uint32_t n, uint32_t flags, uint32_t cull_mask,
+                    uint32_t tlas_idx, orc_hit *hits, int nthreads) {
+  if (!s->built) return -1;
+#ifdef _OPENMP
+  int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
+#endif
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    RayIn ray = load_ray(rays[i], nullptr, flags, cull_mask, tlas_idx);
+    orc_hit h;
+    h.prim_id = 0xffffffffu;
+    h.t = h.u = h.v = 0.0f;
+    float far_ = ray.tmax;
+    uint32_t th = ray.tlas_idx < s->binding.size() ? s->binding[ray.tlas_idx] : 0xffffffffu;
+    if (th < s->tlas_bvh_root.size() && s->tlas_bvh_root[th] != INVALID_NEXT) {
+      const orc_node &root = s->tlas_forest[s->tlas_bvh_root[th]];
+      // instances of this TLAS occupy the root's content range
+      for (uint32_t ti = root.content_range[0]; ti < root.content_range[1]; ti++) {
+        if ((ray.mask & s->tlas_bounding_mask[ti]) == 0) continue;
+        uint32_t fl = merge_instance_flags(ray.flags, s->inst_flags[ti]);
+        if (fl & F_SKIP_TRIANGLES) continue;
+        const M4 &inv = s->inst_transform_inv[ti];
+        V3 bo = m4_mul_point(inv, ray.o);
+        V3 bd = m3_mul(inv, ray.d);
+        float scaling = length(bd);
+        bd = normalize(bd);
+        auto bm = s->blas_meta[s->inst_blas[ti]];
+        for (uint32_t gr = bm.first; gr < bm.second; gr++) {
+          const auto &geo = s->geo_meta[gr];
+          bool pass, opq, ce, cb;
+          cull_geometry(fl, geo.flags, pass, opq);
+          if (!pass) continue;
+          cull_triangle(fl, ce, cb);
+          const orc_node &gn = s->tri_forest[geo.root];
+          for (uint32_t k = gn.content_range[0]; k < gn.content_range[1]; k++) {
+            uint32_t tri = geo.primitive_start + (k - gn.content_range[0]);  // source order
+            V3 v0 = s->vertices[s->indices[3 * (size_t)tri]];
+            V3 v1 = s->vertices[s->indices[3 * (size_t)tri + 1]];
+            V3 v2 = s->vertices[s->indices[3 * (size_t)tri + 2]];
+            float r[4];
+            intersect_triangle(bo, bd, ray.tmin * scaling, far_ * scaling, v0, v1, v2, ce, cb, r);
+            if (r[0] != 0.0f) {
+              far_ = r[1] / scaling;
+              h.prim_id = tri - geo.primitive_start;
+              h.t = far_;
+              h.u = r[2];
+              h.v = r[3];
+            }
+          }
+        }
+      }
+    }
+    hits[i] = h;
+  }
+  return 0;
+}
+
+void orc_intersect_triangle(const float o[3], const float d[3], float n, float f, const float a[3], const float b[3],
+                            const float c[3], int ce, int cb, float out[4]) {
+  intersect_triangle(v3(o[0], o[1], o[2]), v3(d[0], d[1], d[2]), n, f, v3(a[0], a[1], a[2]), v3(b[0], b[1], b[2]),
+                     v3(c[0], c[1], c[2]), ce != 0, cb != 0, out);
+}
+int orc_intersect_aabb(const float o[3], const float d[3], float n, float f, const float bmin[3], const float bmax[3]) {
+  return intersect_aabb(v3(o[0], o[1], o[2]), v3(d[0], d[1], d[2]), n, f, v3(bmin[0], bmin[1], bmin[2]),
+                        v3(bmax[0], bmax[1], bmax[2]))
+             ? 1
+             : 0;
+}
+
+// sobol.rs:40-51
+float orc_van_der_corput(uint32_t n, uint32_t scramble) {
+  n = (n >> 16) | (n << 16);  // rotate_right(16)
+  n = ((n & 0x00ff00ffu) << 8) | ((n & 0xff00ff00u) >> 8);
+  n = ((n & 0x0f0f0f0fu) << 4) | ((n & 0xf0f0f0f0u) >> 4);
+  n = ((n & 0x33333333u) << 2) | ((n & 0xccccccccu) >> 2);
+  n = ((n & 0x55555555u) << 1) | ((n & 0xaaaaaaaau) >> 1);
+  n ^= scramble;
+  return rmin((float)((n >> 8) & 0xffffffu) / 16777216.0f, 1.0f - FLT_EPSILON);
+}
+// sobol.rs:55-68
+float orc_sobol(uint32_t n, uint32_t scramble) {
+  uint32_t i = 1u << 31;
+  while (n != 0) {
+    if (n & 1u) scramble ^= i;
+    n >>= 1;
+    i ^= i >> 1;
+  }
+  return rmin((float)((scramble >> 8) & 0xffffffu) / 16777216.0f, 1.0f - FLT_EPSILON);
+}
+
+// stream_compaction.rs:3-45: inclusive scan of the mask; element i is kept
+// where the scan steps, and lands at inclusive[i] - 1.
+uint32_t orc_compact(const uint32_t *values, const uint32_t *mask, uint32_t n, uint32_t *out) {
+  std::vector<uint32_t> inclusive(n);
+  uint32_t acc = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    acc += mask[i] ? 1u : 0u;
+    inclusive[i] = acc;
+  }
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t prev = i == 0 ? 0u : inclusive[i - 1];
+    if (inclusive[i] != prev) out[prev] = values ? values[i] : i;
+  }
+  return acc;
+}
+
+}  // extern "C"

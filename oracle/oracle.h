@@ -1,0 +1,154 @@
+/*
+ * oracle.h -- CPU oracle for the rendiation ray-query hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (rendiation_amd/, librdrt)
+ * links, loads or calls this library.  Only tests/, the smoke check and the
+ * cpu_baseline leg of bench.py use it, and only as the checker / CPU baseline.
+ *
+ * It restates, in plain C++ with the reference's FP32 operation order, the
+ * reference CPU path that the GPU kernels are graded against:
+ *   - BVH build: content/space/src/bvh/{mod.rs:55-79, node.rs:36-53,
+ *     strategy.rs:8-285, apply.rs:9-78}, content/space/src/utils.rs:20-66
+ *   - forest layout and stackless links:
+ *     shader/ray-tracing/src/backend/wavefront_compute/geometry/naive/mod.rs:122-494,611-632
+ *   - traversal: .../geometry/naive/traverse_cpu.rs:52-319
+ *   - ray/AABB and ray/triangle tests: .../wavefront_compute/geometry/mod.rs:44-155
+ *   - traversal flags: .../geometry/naive/flag.rs:6-118
+ *   - Sobol / van der Corput: math/statistics/src/sampling/sobol.rs:40-68
+ *   - stream compaction: shader/parallel-compute/src/stream_compaction.rs:3-45
+ *
+ * Records shared with the product ABI have the same byte layout as the
+ * product's rdrt_ray / rdrt_hit (checked by the tests), but are declared
+ * here independently so the oracle stands alone.
+ */
+#ifndef RDRT_ORACLE_H
+#define RDRT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_ray {
+  float ox, oy, oz, tmin;
+  float dx, dy, dz, tmax;
+} orc_ray; /* 32 B */
+
+typedef struct orc_ray_meta {
+  uint32_t flags, cull_mask, tlas_idx, pad;
+} orc_ray_meta; /* 16 B */
+
+typedef struct orc_hit {
+  uint32_t prim_id; /* ~0u on miss */
+  float t, u, v;
+} orc_hit; /* 16 B */
+
+typedef struct orc_hit_meta {
+  uint32_t geometry_id, instance_id, hit_kind, instance_custom_index;
+} orc_hit_meta; /* 16 B */
+
+typedef struct orc_instance {
+  float transform[16]; /* column-major Mat4 (a1..a4 = first column) */
+  uint32_t instance_custom_index, mask, sbt_offset, flags, blas;
+  uint32_t pad[3];
+} orc_instance; /* 96 B */
+
+/* 48 B node of the reference forest (naive/mod.rs:46-53) */
+typedef struct orc_node {
+  float aabb_min[3];
+  uint32_t hit_next;
+  float aabb_max[3];
+  uint32_t miss_next;
+  uint32_t content_range[2];
+  uint32_t tail[2];
+} orc_node;
+
+typedef struct orc_build_opts {
+  uint32_t blas_bin_size, blas_max_depth;
+  uint32_t tlas_bin_size, tlas_max_depth;
+  uint32_t sah_bins;
+} orc_build_opts;
+
+typedef struct orc_scene orc_scene;
+
+orc_scene *orc_scene_new(const orc_build_opts *opts);
+void orc_scene_free(orc_scene *s);
+
+/* Pending-BLAS protocol: begin, add one geometry per call, end -> handle. */
+void orc_blas_begin(orc_scene *s);
+void orc_blas_add_triangles(orc_scene *s, const float *pos_xyz, uint32_t nverts,
+                            const uint32_t *indices_or_null, uint32_t ntris,
+                            uint32_t geometry_flags);
+uint32_t orc_blas_end(orc_scene *s);
+void orc_blas_delete(orc_scene *s, uint32_t handle);
+
+uint32_t orc_tlas_create(orc_scene *s, const orc_instance *inst, uint32_t n);
+void orc_tlas_delete(orc_scene *s, uint32_t handle);
+void orc_bind_tlas(orc_scene *s, const uint32_t *handles, uint32_t n);
+
+/* Build everything (reference NaiveSahBvhSource::build). Returns 0 on success. */
+int orc_build(orc_scene *s);
+
+/* Sizes of the built arrays. */
+typedef struct orc_sizes {
+  uint32_t tri_nodes, tlas_nodes, tris, verts, instances, geometries, blas, tlas;
+  uint32_t max_blas_depth, max_tlas_depth;
+} orc_sizes;
+void orc_get_sizes(const orc_scene *s, orc_sizes *out);
+
+/* Copies of the built arrays (caller allocates with the sizes above). */
+void orc_copy_tri_forest(const orc_scene *s, orc_node *out);
+void orc_copy_tlas_forest(const orc_scene *s, orc_node *out);
+void orc_copy_indices_redirect(const orc_scene *s, uint32_t *out);
+/* per instance (forest order): source position of the instance in its TLAS */
+void orc_copy_instance_order(const orc_scene *s, uint32_t *out);
+/* geometry meta: 4 u32 per geometry: root, geometry_idx, primitive_start, flags */
+void orc_copy_geometry_meta(const orc_scene *s, uint32_t *out);
+/* tlas bounding: 8 floats/u32 per instance (world_min, mask, world_max, flags) */
+void orc_copy_tlas_bounding(const orc_scene *s, float *out);
+/* transform_inv per instance, 16 floats, forest order */
+void orc_copy_transform_inv(const orc_scene *s, float *out);
+
+/*
+ * Reference CPU traversal (traverse_cpu.rs:52-245) for n rays.
+ * meta may be NULL: then every ray uses (flags, cull_mask, tlas_idx).
+ * hit_meta and counters may be NULL.  counters: 3 u32 per ray =
+ * (BVH node visits, triangle tests, instance tests).
+ * nthreads <= 0 uses all OpenMP threads.  Returns 0 on success.
+ */
+int orc_trace(const orc_scene *s, const orc_ray *rays, const orc_ray_meta *meta,
+              uint32_t n, uint32_t flags, uint32_t cull_mask, uint32_t tlas_idx,
+              orc_hit *hits, orc_hit_meta *hit_meta, uint32_t *counters,
+              int nthreads);
+
+/* Same result definition, but every triangle of every instance is tested in
+ * source order (no BVH).  For small inputs only. */
+int orc_trace_brute(const orc_scene *s, const orc_ray *rays, uint32_t n,
+                    uint32_t flags, uint32_t cull_mask, uint32_t tlas_idx,
+                    orc_hit *hits, int nthreads);
+
+/* The FP32 ray/triangle test itself (geometry/mod.rs:104-155) on one input:
+ * out = (sign, t, u, v); a miss is all zeros. */
+void orc_intersect_triangle(const float o[3], const float d[3], float near_,
+                            float far_, const float v0[3], const float v1[3],
+                            const float v2[3], int cull_enable, int cull_back,
+                            float out[4]);
+/* The ray/AABB slab test (geometry/mod.rs:44-64). */
+int orc_intersect_aabb(const float o[3], const float d[3], float near_,
+                       float far_, const float bmin[3], const float bmax[3]);
+
+/* Sobol dimension 0 / 1 (sobol.rs:40-68). */
+float orc_van_der_corput(uint32_t n, uint32_t scramble);
+float orc_sobol(uint32_t n, uint32_t scramble);
+
+/* Stable stream compaction by mask (stream_compaction.rs:3-45): element i is
+ * written to out[exclusive_scan(mask)[i]] when mask[i] != 0; returns the
+ * kept count.  values == NULL compacts the indices themselves. */
+uint32_t orc_compact(const uint32_t *values, const uint32_t *mask, uint32_t n,
+                     uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,240 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+u32 = C.c_uint32
+f32 = C.c_float
+P = C.c_void_p
+
+
+class OrcBuildOpts(C.Structure):
+    _fields_ = [("blas_bin_size", u32), ("blas_max_depth", u32), ("tlas_bin_size", u32),
+                ("tlas_max_depth", u32), ("sah_bins", u32)]
+
+
+class OrcSizes(C.Structure):
+    _fields_ = [(n, u32) for n in ("tri_nodes", "tlas_nodes", "tris", "verts", "instances", "geometries", "blas",
+                                   "tlas", "max_blas_depth", "max_tlas_depth")]
+
+
+class OrcInstance(C.Structure):
+    _fields_ = [("transform", f32 * 16), ("instance_custom_index", u32), ("mask", u32), ("sbt_offset", u32),
+                ("flags", u32), ("blas", u32), ("pad", u32 * 3)]
+
+
+_SIG = {
+    "orc_scene_new": (P, [C.POINTER(OrcBuildOpts)]),
+    "orc_scene_free": (None, [P]),
+    "orc_blas_begin": (None, [P]),
+    "orc_blas_add_triangles": (None, [P, P, u32, P, u32, u32]),
+    "orc_blas_end": (u32, [P]),
+    "orc_blas_delete": (None, [P, u32]),
+    "orc_tlas_create": (u32, [P, C.POINTER(OrcInstance), u32]),
+    "orc_tlas_delete": (None, [P, u32]),
+    "orc_bind_tlas": (None, [P, P, u32]),
+    "orc_build": (C.c_int, [P]),
+    "orc_get_sizes": (None, [P, C.POINTER(OrcSizes)]),
+    "orc_copy_tri_forest": (None, [P, P]),
+    "orc_copy_tlas_forest": (None, [P, P]),
+    "orc_copy_indices_redirect": (None, [P, P]),
+    "orc_copy_instance_order": (None, [P, P]),
+    "orc_copy_geometry_meta": (None, [P, P]),
+    "orc_copy_tlas_bounding": (None, [P, P]),
+    "orc_copy_transform_inv": (None, [P, P]),
+    "orc_trace": (C.c_int, [P, P, P, u32, u32, u32, u32, P, P, P, C.c_int]),
+    "orc_trace_brute": (C.c_int, [P, P, u32, u32, u32, u32, P, C.c_int]),
+    "orc_intersect_triangle": (None, [P, P, f32, f32, P, P, P, C.c_int, C.c_int, P]),
+    "orc_intersect_aabb": (C.c_int, [P, P, f32, f32, P, P]),
+    "orc_van_der_corput": (f32, [u32, u32]),
+    "orc_sobol": (f32, [u32, u32]),
+    "orc_compact": (u32, [P, P, u32, P]),
+}
+
+_lib = None
+
+
+def build_oracle() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build_oracle()
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIG.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+class OracleScene:
+    """The reference NaiveSahBvhSource + NaiveSahBvhCpu, restated in C++."""
+
+    def __init__(self, blas_bin_size=2, blas_max_depth=50, tlas_bin_size=10, tlas_max_depth=50, sah_bins=4):
+        self.L = lib()
+        opts = OrcBuildOpts(blas_bin_size, blas_max_depth, tlas_bin_size, tlas_max_depth, sah_bins)
+        self.h = self.L.orc_scene_new(C.byref(opts))
+        self._keep = []
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.orc_scene_free(self.h)
+        except Exception:
+            pass
+
+    def add_blas(self, geometries) -> int:
+        """geometries: list of (positions (n,3) f32, indices (m,3) u32 or None, flags)."""
+        self.L.orc_blas_begin(self.h)
+        for pos, idx, flags in geometries:
+            pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+            if idx is None:
+                self.L.orc_blas_add_triangles(self.h, pos.ctypes.data, pos.shape[0], None, pos.shape[0] // 3, flags)
+            else:
+                idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+                self.L.orc_blas_add_triangles(self.h, pos.ctypes.data, pos.shape[0], idx.ctypes.data, idx.size // 3,
+                                              flags)
+        return int(self.L.orc_blas_end(self.h))
+
+    def delete_blas(self, h: int) -> None:
+        self.L.orc_blas_delete(self.h, h)
+
+    def add_tlas(self, instances) -> int:
+        """instances: list of dicts with transform (4x4 math order), blas, and
+        optional custom, mask, sbt, flags."""
+        arr = (OrcInstance * max(1, len(instances)))()
+        for i, d in enumerate(instances):
+            m = np.asarray(d["transform"], dtype=np.float32).reshape(4, 4).T.reshape(-1)
+            arr[i].transform[:] = [float(x) for x in m]
+            arr[i].instance_custom_index = d.get("custom", 0)
+            arr[i].mask = d.get("mask", 0xFFFFFFFF)
+            arr[i].sbt_offset = d.get("sbt", 0)
+            arr[i].flags = d.get("flags", 0)
+            arr[i].blas = d["blas"]
+        return int(self.L.orc_tlas_create(self.h, arr, len(instances)))
+
+    def delete_tlas(self, h: int) -> None:
+        self.L.orc_tlas_delete(self.h, h)
+
+    def bind(self, handles) -> None:
+        a = np.asarray(handles, dtype=np.uint32)
+        self.L.orc_bind_tlas(self.h, a.ctypes.data, a.size)
+
+    def build(self) -> None:
+        rc = self.L.orc_build(self.h)
+        if rc != 0:
+            raise RuntimeError(f"oracle build failed ({rc})")
+
+    def sizes(self) -> OrcSizes:
+        s = OrcSizes()
+        self.L.orc_get_sizes(self.h, C.byref(s))
+        return s
+
+    def tri_forest(self) -> np.ndarray:
+        n = self.sizes().tri_nodes
+        out = np.zeros((max(1, n), 12), dtype=np.uint32)
+        self.L.orc_copy_tri_forest(self.h, out.ctypes.data)
+        return out[:n]
+
+    def tlas_forest(self) -> np.ndarray:
+        n = self.sizes().tlas_nodes
+        out = np.zeros((max(1, n), 12), dtype=np.uint32)
+        self.L.orc_copy_tlas_forest(self.h, out.ctypes.data)
+        return out[:n]
+
+    def redirect(self) -> np.ndarray:
+        n = self.sizes().tris
+        out = np.zeros(max(1, n), dtype=np.uint32)
+        self.L.orc_copy_indices_redirect(self.h, out.ctypes.data)
+        return out[:n]
+
+    def instance_order(self) -> np.ndarray:
+        n = self.sizes().instances
+        out = np.zeros(max(1, n), dtype=np.uint32)
+        self.L.orc_copy_instance_order(self.h, out.ctypes.data)
+        return out[:n]
+
+    def geometry_meta(self) -> np.ndarray:
+        n = self.sizes().geometries
+        out = np.zeros((max(1, n), 4), dtype=np.uint32)
+        self.L.orc_copy_geometry_meta(self.h, out.ctypes.data)
+        return out[:n]
+
+    def trace(self, rays: np.ndarray, meta: Optional[np.ndarray] = None, flags=0, cull_mask=0xFFFFFFFF, tlas_idx=0,
+              threads: int = 0, with_meta=False, with_counters=False):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        hits = np.zeros((n, 4), dtype=np.uint32)
+        hm = np.zeros((n, 4), dtype=np.uint32) if with_meta else None
+        cnt = np.zeros((n, 3), dtype=np.uint32) if with_counters else None
+        if meta is not None:
+            meta = np.ascontiguousarray(meta, dtype=np.uint32).reshape(-1, 4)
+        rc = self.L.orc_trace(self.h, rays.ctypes.data, _p(meta), n, flags, cull_mask & 0xFFFFFFFF, tlas_idx,
+                              hits.ctypes.data, _p(hm), _p(cnt), threads)
+        if rc != 0:
+            raise RuntimeError("oracle trace failed (scene not built)")
+        return hits, hm, cnt
+
+    def trace_brute(self, rays: np.ndarray, flags=0, cull_mask=0xFFFFFFFF, tlas_idx=0, threads: int = 0):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        hits = np.zeros((rays.shape[0], 4), dtype=np.uint32)
+        rc = self.L.orc_trace_brute(self.h, rays.ctypes.data, rays.shape[0], flags, cull_mask & 0xFFFFFFFF,
+                                    tlas_idx, hits.ctypes.data, threads)
+        if rc != 0:
+            raise RuntimeError("oracle trace failed (scene not built)")
+        return hits
+
+
+def intersect_triangle(o, d, near, far, v0, v1, v2, cull_enable=False, cull_back=False) -> np.ndarray:
+    L = lib()
+    a = [np.ascontiguousarray(x, dtype=np.float32) for x in (o, d, v0, v1, v2)]
+    out = np.zeros(4, dtype=np.float32)
+    L.orc_intersect_triangle(a[0].ctypes.data, a[1].ctypes.data, near, far, a[2].ctypes.data, a[3].ctypes.data,
+                             a[4].ctypes.data, int(cull_enable), int(cull_back), out.ctypes.data)
+    return out
+
+
+def intersect_aabb(o, d, near, far, bmin, bmax) -> bool:
+    L = lib()
+    a = [np.ascontiguousarray(x, dtype=np.float32) for x in (o, d, bmin, bmax)]
+    return bool(L.orc_intersect_aabb(a[0].ctypes.data, a[1].ctypes.data, near, far, a[2].ctypes.data,
+                                     a[3].ctypes.data))
+
+
+def van_der_corput(n: int, scramble: int) -> float:
+    return float(lib().orc_van_der_corput(n, scramble))
+
+
+def sobol(n: int, scramble: int) -> float:
+    return float(lib().orc_sobol(n, scramble))
+
+
+def compact(mask, values=None):
+    mask = np.ascontiguousarray(mask, dtype=np.uint32)
+    vals = None if values is None else np.ascontiguousarray(values, dtype=np.uint32)
+    out = np.zeros(max(1, mask.size), dtype=np.uint32)
+    k = lib().orc_compact(_p(vals), mask.ctypes.data, mask.size, out.ctypes.data)
+    return out[:mask.size], int(k)

@@ -1,0 +1,128 @@
+"""ctypes binding of librdrt.so (include/rdrt.h).
+
+The library is loaded from this package directory only.  There is no
+fallback: if it is missing, importing the bindings raises, telling the user
+to build it.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "librdrt.so"
+
+u32 = C.c_uint32
+i32 = C.c_int32
+f32 = C.c_float
+
+
+class RdrtRay(C.Structure):
+    _fields_ = [("ox", f32), ("oy", f32), ("oz", f32), ("tmin", f32),
+                ("dx", f32), ("dy", f32), ("dz", f32), ("tmax", f32)]
+
+
+class RdrtRayMeta(C.Structure):
+    _fields_ = [("flags", u32), ("cull_mask", u32), ("tlas_idx", u32), ("pad", u32)]
+
+
+class RdrtHit(C.Structure):
+    _fields_ = [("prim_id", u32), ("t", f32), ("u", f32), ("v", f32)]
+
+
+class RdrtHitMeta(C.Structure):
+    _fields_ = [("geometry_id", u32), ("instance_id", u32), ("hit_kind", u32), ("instance_custom_index", u32)]
+
+
+class RdrtInstance(C.Structure):
+    _fields_ = [("transform", f32 * 16), ("instance_custom_index", u32), ("mask", u32),
+                ("sbt_offset", u32), ("flags", u32), ("blas", u32), ("pad", u32 * 3)]
+
+
+class RdrtBuildOpts(C.Structure):
+    _fields_ = [("blas_bin_size", u32), ("blas_max_depth", u32), ("tlas_bin_size", u32),
+                ("tlas_max_depth", u32), ("sah_bins", u32)]
+
+
+class RdrtConfig(C.Structure):
+    _fields_ = [("device", i32), ("stream", C.c_void_p), ("build", RdrtBuildOpts),
+                ("traversal", u32), ("block_size", u32)]
+
+
+class RdrtTraceArgs(C.Structure):
+    _fields_ = [("rays", C.c_void_p), ("ray_meta", C.c_void_p), ("hits", C.c_void_p),
+                ("hit_meta", C.c_void_p), ("counters", C.c_void_p), ("n", u32), ("flags", u32),
+                ("cull_mask", u32), ("tlas_idx", u32), ("traversal", u32), ("pad", u32)]
+
+
+class RdrtSceneInfo(C.Structure):
+    _fields_ = [("tri_nodes", u32), ("tlas_nodes", u32), ("tris", u32), ("verts", u32),
+                ("instances", u32), ("geometries", u32), ("blas", u32), ("tlas", u32),
+                ("max_blas_depth", u32), ("max_tlas_depth", u32), ("stack_entries", u32), ("pad", u32),
+                ("device_bytes", C.c_uint64), ("build_ms", C.c_double)]
+
+
+# name -> (restype, argtypes); the full exported surface of rdrt.h
+SIGNATURES = {
+    "rdrt_last_error": (C.c_char_p, []),
+    "rdrt_abi_version": (i32, []),
+    "rdrt_config_default": (i32, [C.POINTER(RdrtConfig)]),
+    "rdrt_ctx_create": (i32, [C.POINTER(RdrtConfig), C.POINTER(C.c_void_p)]),
+    "rdrt_ctx_destroy": (i32, [C.c_void_p]),
+    "rdrt_sync": (i32, [C.c_void_p]),
+    "rdrt_blas_begin": (i32, [C.c_void_p]),
+    "rdrt_blas_add_triangles": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p, u32, u32]),
+    "rdrt_blas_end": (i32, [C.c_void_p, C.POINTER(u32)]),
+    "rdrt_blas_destroy": (i32, [C.c_void_p, u32]),
+    "rdrt_tlas_create": (i32, [C.c_void_p, C.POINTER(RdrtInstance), u32, C.POINTER(u32)]),
+    "rdrt_tlas_destroy": (i32, [C.c_void_p, u32]),
+    "rdrt_bind_tlas": (i32, [C.c_void_p, C.POINTER(u32), u32]),
+    "rdrt_bind_tlas_max_len": (u32, [C.c_void_p]),
+    "rdrt_build": (i32, [C.c_void_p]),
+    "rdrt_commit": (i32, [C.c_void_p]),
+    "rdrt_get_scene_info": (i32, [C.c_void_p, C.POINTER(RdrtSceneInfo)]),
+    "rdrt_copy_built": (i32, [C.c_void_p, u32, C.c_void_p, C.c_size_t]),
+    "rdrt_trace": (i32, [C.c_void_p, C.POINTER(RdrtTraceArgs)]),
+    "rdrt_trace_closest": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p]),
+    "rdrt_last_trace_ms": (i32, [C.c_void_p, C.POINTER(f32)]),
+}
+
+RDRT_OK = 0
+ERROR_NAMES = {
+    -1: "RDRT_ERR_INVALID_ARG", -2: "RDRT_ERR_HIP", -3: "RDRT_ERR_NOT_BUILT", -4: "RDRT_ERR_OUT_OF_MEMORY",
+    -5: "RDRT_ERR_STATE", -6: "RDRT_ERR_NO_DEVICE", -7: "RDRT_ERR_CAPACITY", -8: "RDRT_ERR_UNSUPPORTED",
+}
+
+
+class RdrtError(RuntimeError):
+    """A librdrt call returned a non-zero status."""
+
+    def __init__(self, code: int, message: str):
+        self.code = code
+        super().__init__(f"{ERROR_NAMES.get(code, code)}: {message}")
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded librdrt.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m rendiation_amd.build` "
+                "(or __graft_entry__.build()); there is no non-native fallback")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != RDRT_OK:
+        msg = lib().rdrt_last_error()
+        raise RdrtError(rc, msg.decode() if msg else "")

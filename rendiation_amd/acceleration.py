@@ -1,0 +1,286 @@
+"""Host mirror of the reference's acceleration-structure provider.
+
+Names, argument meaning and error behaviour follow
+``GPUAccelerationStructureSystemProvider`` and the naive SAH-BVH system
+(shader/ray-tracing/src/api/backend.rs:104-177,
+shader/ray-tracing/src/backend/wavefront_compute/geometry/naive/mod.rs:496-609):
+
+* ``create_bottom_level_acceleration_structure(sources) -> BlasHandle``
+* ``create_top_level_acceleration_structure(instances) -> TlasHandle``
+* ``delete_bottom_level_acceleration_structure`` / ``delete_top_level_acceleration_structure``
+* ``bind_tlas(handles)``, ``bind_tlas_max_len()``
+* ``create_comp_instance()`` builds and uploads (the reference builds lazily
+  there, naive/mod.rs:521-567).
+
+Tracing takes device tensors (torch is only the allocator here) and calls
+the HIP kernels through the C-ABI; nothing runs on the CPU.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+# ray flags (shader/ray-tracing/src/api/ty.rs:102-114)
+RAY_FLAG_NONE = 0x00
+RAY_FLAG_FORCE_OPAQUE = 0x01
+RAY_FLAG_FORCE_NON_OPAQUE = 0x02
+RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH = 0x04
+RAY_FLAG_SKIP_CLOSEST_HIT_SHADER = 0x08
+RAY_FLAG_CULL_BACK_FACING_TRIANGLES = 0x10
+RAY_FLAG_CULL_FRONT_FACING_TRIANGLES = 0x20
+RAY_FLAG_CULL_OPAQUE = 0x40
+RAY_FLAG_CULL_NON_OPAQUE = 0x80
+RAY_FLAG_SKIP_TRIANGLES = 0x100
+# ty.rs:139-159
+HIT_KIND_FRONT_FACING_TRIANGLE = 0xFE
+HIT_KIND_BACK_FACING_TRIANGLE = 0xFF
+GEOMETRY_INSTANCE_TRIANGLE_FACING_CULL_DISABLE = 0x1
+GEOMETRY_INSTANCE_TRIANGLE_FLIP_FACING = 0x2
+GEOMETRY_INSTANCE_FORCE_OPAQUE = 0x4
+GEOMETRY_INSTANCE_FORCE_NO_OPAQUE = 0x8
+GEOMETRY_FLAG_OPAQUE = 0x1
+GEOMETRY_FLAG_NO_DUPLICATE_ANYHIT_INVOCATION = 0x2
+
+TRAVERSAL_FAST = 0
+TRAVERSAL_REFERENCE = 1
+MISS = 0xFFFFFFFF
+
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("prim_id", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+
+
+@dataclass(frozen=True)
+class BlasHandle:
+    index: int
+
+
+@dataclass(frozen=True)
+class TlasHandle:
+    index: int
+
+
+@dataclass
+class TriangleGeometry:
+    """``BottomLevelAccelerationStructureBuildBuffer::Triangles`` plus its flags."""
+
+    positions: np.ndarray  # (n, 3) float32
+    indices: Optional[np.ndarray] = None  # (m, 3) or (3m,) uint32; None = non-indexed
+    flags: int = GEOMETRY_FLAG_OPAQUE
+
+
+# The reference name of one BLAS source entry.
+BottomLevelAccelerationStructureBuildSource = TriangleGeometry
+
+
+@dataclass
+class TopLevelAccelerationStructureSourceInstance:
+    """One TLAS instance; ``transform`` is a 4x4 matrix in math order
+    (``transform[row, col]``), stored column-major as the reference's Mat4."""
+
+    transform: np.ndarray
+    acceleration_structure_handle: BlasHandle
+    instance_custom_index: int = 0
+    mask: int = 0xFFFFFFFF
+    instance_shader_binding_table_record_offset: int = 0
+    flags: int = 0
+
+
+@dataclass
+class BuildOptions:
+    """TreeBuildOption for BLAS / TLAS and the SAH bin count
+    (content/space/src/utils.rs:192-210); defaults are the reference's."""
+
+    blas_bin_size: int = 2
+    blas_max_depth: int = 50
+    tlas_bin_size: int = 10
+    tlas_max_depth: int = 50
+    sah_bins: int = 4
+
+
+@dataclass
+class SceneInfo:
+    tri_nodes: int
+    tlas_nodes: int
+    tris: int
+    verts: int
+    instances: int
+    geometries: int
+    blas: int
+    tlas: int
+    max_blas_depth: int
+    max_tlas_depth: int
+    stack_entries: int
+    device_bytes: int
+    build_ms: float
+
+
+def _ptr(t) -> Optional[int]:
+    if t is None:
+        return None
+    return int(t.data_ptr())
+
+
+class AccelerationStructureSystem:
+    """MI355X implementation of the naive SAH-BVH acceleration-structure
+    provider.  ``device=None`` makes a host-only system (BVH build and
+    inspection, no tracing)."""
+
+    def __init__(self, device: Optional[int] = 0, options: BuildOptions = BuildOptions(),
+                 traversal: int = TRAVERSAL_FAST, stream: Optional[int] = None, block_size: int = 0):
+        self._L = L.lib()
+        cfg = L.RdrtConfig()
+        L.check(self._L.rdrt_config_default(C.byref(cfg)))
+        cfg.device = -1 if device is None else int(device)
+        cfg.stream = stream
+        cfg.build = L.RdrtBuildOpts(options.blas_bin_size, options.blas_max_depth, options.tlas_bin_size,
+                                    options.tlas_max_depth, options.sah_bins)
+        cfg.traversal = traversal
+        cfg.block_size = block_size
+        h = C.c_void_p()
+        L.check(self._L.rdrt_ctx_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.device = device
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.rdrt_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- GPUAccelerationStructureSystemProvider ----
+    def create_bottom_level_acceleration_structure(self, sources: Sequence[TriangleGeometry]) -> BlasHandle:
+        L.check(self._L.rdrt_blas_begin(self._h))
+        keep = []
+        for s in sources:
+            pos = np.ascontiguousarray(s.positions, dtype=np.float32).reshape(-1, 3)
+            keep.append(pos)
+            if s.indices is None:
+                L.check(self._L.rdrt_blas_add_triangles(self._h, pos.ctypes.data, pos.shape[0], None,
+                                                         pos.shape[0] // 3, s.flags))
+            else:
+                idx = np.ascontiguousarray(s.indices, dtype=np.uint32).reshape(-1)
+                keep.append(idx)
+                L.check(self._L.rdrt_blas_add_triangles(self._h, pos.ctypes.data, pos.shape[0], idx.ctypes.data,
+                                                         idx.size // 3, s.flags))
+        out = L.u32()
+        L.check(self._L.rdrt_blas_end(self._h, C.byref(out)))
+        return BlasHandle(out.value)
+
+    def delete_bottom_level_acceleration_structure(self, handle: BlasHandle) -> None:
+        L.check(self._L.rdrt_blas_destroy(self._h, handle.index))
+
+    def create_top_level_acceleration_structure(
+            self, instances: Sequence[TopLevelAccelerationStructureSourceInstance]) -> TlasHandle:
+        arr = (L.RdrtInstance * max(1, len(instances)))()
+        for i, s in enumerate(instances):
+            m = np.asarray(s.transform, dtype=np.float32).reshape(4, 4)
+            col_major = m.T.reshape(-1)  # column-major storage
+            arr[i].transform[:] = [float(x) for x in col_major]
+            arr[i].instance_custom_index = s.instance_custom_index
+            arr[i].mask = s.mask & 0xFFFFFFFF
+            arr[i].sbt_offset = s.instance_shader_binding_table_record_offset
+            arr[i].flags = s.flags
+            arr[i].blas = s.acceleration_structure_handle.index
+        out = L.u32()
+        L.check(self._L.rdrt_tlas_create(self._h, arr, len(instances), C.byref(out)))
+        return TlasHandle(out.value)
+
+    def delete_top_level_acceleration_structure(self, handle: TlasHandle) -> None:
+        L.check(self._L.rdrt_tlas_destroy(self._h, handle.index))
+
+    def bind_tlas(self, tlas: Sequence[TlasHandle]) -> None:
+        arr = (L.u32 * max(1, len(tlas)))(*[t.index for t in tlas])
+        L.check(self._L.rdrt_bind_tlas(self._h, arr, len(tlas)))
+
+    def bind_tlas_max_len(self) -> int:
+        return int(self._L.rdrt_bind_tlas_max_len(self._h))
+
+    def build(self) -> None:
+        """Host build of every BVH (no device work)."""
+        L.check(self._L.rdrt_build(self._h))
+
+    def create_comp_instance(self) -> None:
+        """Build if needed and upload to the device."""
+        L.check(self._L.rdrt_commit(self._h))
+
+    commit = create_comp_instance
+
+    def scene_info(self) -> SceneInfo:
+        info = L.RdrtSceneInfo()
+        L.check(self._L.rdrt_get_scene_info(self._h, C.byref(info)))
+        return SceneInfo(info.tri_nodes, info.tlas_nodes, info.tris, info.verts, info.instances, info.geometries,
+                         info.blas, info.tlas, info.max_blas_depth, info.max_tlas_depth, info.stack_entries,
+                         info.device_bytes, info.build_ms)
+
+    def copy_built(self, which: str) -> np.ndarray:
+        """Built reference-layout arrays: 'blas_forest' / 'tlas_forest' (48 B
+        nodes as (n, 12) uint32), 'redirect', 'instance_order' (uint32)."""
+        info = self.scene_info()
+        table = {"blas_forest": (0, info.tri_nodes * 12), "tlas_forest": (1, info.tlas_nodes * 12),
+                 "redirect": (2, info.tris), "instance_order": (3, info.instances)}
+        code, n = table[which]
+        out = np.zeros(max(1, n), dtype=np.uint32)
+        L.check(self._L.rdrt_copy_built(self._h, code, out.ctypes.data, out.nbytes))
+        out = out[:n]
+        return out.reshape(-1, 12) if which.endswith("forest") else out
+
+    # ---- tracing ----
+    def trace(self, rays, hits, *, ray_meta=None, hit_meta=None, counters=None, flags: int = 0,
+              cull_mask: int = 0xFFFFFFFF, tlas_idx: int = 0, traversal: int = TRAVERSAL_FAST) -> None:
+        """Enqueue one closest-hit launch.  ``rays`` is a contiguous device
+        tensor of n*8 float32 (rdrt_ray records), ``hits`` n*4 32-bit words;
+        optional ``ray_meta`` / ``hit_meta`` n*4 words, ``counters`` n*3."""
+        n = rays.numel() // 8
+        _check_dev(rays, n * 8, "rays")
+        _check_dev(hits, n * 4, "hits")
+        if ray_meta is not None:
+            _check_dev(ray_meta, n * 4, "ray_meta")
+        if hit_meta is not None:
+            _check_dev(hit_meta, n * 4, "hit_meta")
+        if counters is not None:
+            _check_dev(counters, n * 3, "counters")
+        a = L.RdrtTraceArgs()
+        a.rays, a.ray_meta, a.hits = _ptr(rays), _ptr(ray_meta), _ptr(hits)
+        a.hit_meta, a.counters = _ptr(hit_meta), _ptr(counters)
+        a.n, a.flags, a.cull_mask, a.tlas_idx, a.traversal = n, flags, cull_mask & 0xFFFFFFFF, tlas_idx, traversal
+        L.check(self._L.rdrt_trace(self._h, C.byref(a)))
+
+    def sync(self) -> None:
+        L.check(self._L.rdrt_sync(self._h))
+
+    def last_trace_ms(self) -> float:
+        ms = L.f32()
+        L.check(self._L.rdrt_last_trace_ms(self._h, C.byref(ms)))
+        return float(ms.value)
+
+
+def _check_dev(t, numel: int, name: str) -> None:
+    if not getattr(t, "is_cuda", False):
+        raise ValueError(f"{name} must be a device tensor")
+    if not t.is_contiguous() or t.numel() != numel or t.element_size() != 4:
+        raise ValueError(f"{name} must be contiguous with {numel} 32-bit elements")
+
+
+def trace_closest(system: AccelerationStructureSystem, rays, *, with_meta: bool = False,
+                  with_counters: bool = False, **kw):
+    """Allocate outputs (torch, same device as ``rays``), trace, and return
+    ``(hits[n,4] int32 view, hit_meta or None, counters or None)``."""
+    import torch
+
+    n = rays.numel() // 8
+    hits = torch.empty((n, 4), dtype=torch.int32, device=rays.device)
+    hm = torch.empty((n, 4), dtype=torch.int32, device=rays.device) if with_meta else None
+    cnt = torch.empty((n, 3), dtype=torch.int32, device=rays.device) if with_counters else None
+    system.trace(rays, hits, hit_meta=hm, counters=cnt, **kw)
+    return hits, hm, cnt

@@ -1,0 +1,441 @@
+// api.hip -- the C-ABI of librdrt (include/rdrt.h).
+//
+// Mirrors the reference provider (shader/ray-tracing/src/backend/
+// wavefront_compute/geometry/naive/mod.rs:497-609): sources are copied at
+// creation, handles are never reused, any change invalidates the built
+// scene, and the build happens on rdrt_build / rdrt_commit instead of lazily
+// on the next create_comp_instance.
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rdrt.h"
+#include "scene.h"
+#include "trace_kernels.h"
+
+using namespace rdrt;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int32_t fail(int32_t code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define RDRT_HIP(expr)                                                                                  \
+  do {                                                                                                  \
+    hipError_t e_ = (expr);                                                                             \
+    if (e_ != hipSuccess) return fail(RDRT_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+struct DevArray {
+  T *ptr = nullptr;
+  size_t count = 0;
+};
+
+struct DeviceScene {
+  DevArray<PairNode> tri_pairs, tlas_pairs;
+  DevArray<TriPre> tris;
+  DevArray<uint32_t> redirect, indices, binding;
+  DevArray<float> vertices;
+  DevArray<GeoRec> geos;
+  DevArray<InstRec> insts;
+  DevArray<TlasRoot> tlas_roots;
+  DevArray<RefNode> tri_forest, tlas_forest;
+  uint64_t bytes = 0;
+};
+
+}  // namespace
+
+struct rdrt_ctx {
+  rdrt_config cfg{};
+  bool has_device = false;
+  hipStream_t stream = nullptr;
+  SceneSrc src;
+  std::vector<GeometrySrc> pending;
+  bool in_blas = false;
+  BuiltScene built;
+  bool built_valid = false;
+  bool uploaded = false;
+  DeviceScene dev;
+  uint32_t *status = nullptr;  // device status word of the trace kernels
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+};
+
+namespace {
+
+void invalidate(rdrt_ctx *c) {
+  c->built_valid = false;
+  c->uploaded = false;
+}
+
+template <class T>
+hipError_t upload(DevArray<T> &a, const std::vector<T> &h, uint64_t &bytes) {
+  a.count = h.size();
+  size_t n = h.empty() ? 1 : h.size();  // keep a valid pointer for empty arrays
+  hipError_t e = hipMalloc(&a.ptr, n * sizeof(T));
+  if (e != hipSuccess) return e;
+  bytes += n * sizeof(T);
+  if (!h.empty()) e = hipMemcpy(a.ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+  return e;
+}
+
+template <class T>
+void release(DevArray<T> &a) {
+  if (a.ptr) (void)hipFree(a.ptr);
+  a.ptr = nullptr;
+  a.count = 0;
+}
+
+void free_device_scene(DeviceScene &d) {
+  release(d.tri_pairs);
+  release(d.tlas_pairs);
+  release(d.tris);
+  release(d.redirect);
+  release(d.indices);
+  release(d.binding);
+  release(d.vertices);
+  release(d.geos);
+  release(d.insts);
+  release(d.tlas_roots);
+  release(d.tri_forest);
+  release(d.tlas_forest);
+  d.bytes = 0;
+}
+
+TreeOptions tree_opt(uint32_t bin, uint32_t depth, uint32_t bins, uint32_t def_bin) {
+  return TreeOptions{bin ? bin : def_bin, depth ? depth : 50u, bins ? bins : 4u};
+}
+
+int32_t need_device(rdrt_ctx *c) {
+  if (!c->has_device) return fail(RDRT_ERR_NO_DEVICE, "context was created without a device (device = -1)");
+  RDRT_HIP(hipSetDevice(c->cfg.device));
+  return RDRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *rdrt_last_error(void) { return g_last_error.c_str(); }
+int32_t rdrt_abi_version(void) { return RDRT_ABI_VERSION; }
+
+int32_t rdrt_config_default(rdrt_config *out) {
+  if (!out) return fail(RDRT_ERR_INVALID_ARG, "null config");
+  std::memset(out, 0, sizeof(*out));
+  out->device = 0;
+  out->build = rdrt_build_opts{2, 50, 10, 50, 4};
+  out->traversal = RDRT_TRAVERSAL_FAST;
+  out->block_size = 0;
+  return RDRT_OK;
+}
+
+int32_t rdrt_ctx_create(const rdrt_config *cfg, rdrt_ctx **out) {
+  if (!cfg || !out) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  if (cfg->traversal > RDRT_TRAVERSAL_REFERENCE) return fail(RDRT_ERR_INVALID_ARG, "unknown traversal");
+  rdrt_ctx *c = new rdrt_ctx();
+  c->cfg = *cfg;
+  const rdrt_build_opts &b = cfg->build;
+  c->src.blas_opt = tree_opt(b.blas_bin_size, b.blas_max_depth, b.sah_bins, 2);
+  c->src.tlas_opt = tree_opt(b.tlas_bin_size, b.tlas_max_depth, b.sah_bins, 10);
+  if (c->src.blas_opt.max_depth > 64 || c->src.tlas_opt.max_depth > 64) {
+    delete c;
+    return fail(RDRT_ERR_INVALID_ARG, "max_depth above 64 is not supported");
+  }
+  if (cfg->device >= 0) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || cfg->device >= n) {
+      delete c;
+      return fail(RDRT_ERR_NO_DEVICE, std::string("no HIP device ") + std::to_string(cfg->device) +
+                                          (e != hipSuccess ? std::string(": ") + hipGetErrorString(e) : ""));
+    }
+    c->has_device = true;
+    c->stream = (hipStream_t)cfg->stream;
+    if (hipSetDevice(cfg->device) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
+        hipEventCreate(&c->ev1) != hipSuccess || hipMalloc(&c->status, sizeof(uint32_t)) != hipSuccess ||
+        hipMemset(c->status, 0, sizeof(uint32_t)) != hipSuccess) {
+      delete c;
+      return fail(RDRT_ERR_HIP, "device set-up failed");
+    }
+  }
+  *out = c;
+  return RDRT_OK;
+}
+
+int32_t rdrt_ctx_destroy(rdrt_ctx *c) {
+  if (!c) return RDRT_OK;
+  if (c->has_device) {
+    (void)hipSetDevice(c->cfg.device);
+    (void)hipStreamSynchronize(c->stream);
+    free_device_scene(c->dev);
+    if (c->status) (void)hipFree(c->status);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+  }
+  delete c;
+  return RDRT_OK;
+}
+
+int32_t rdrt_sync(rdrt_ctx *c) {
+  if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  RDRT_HIP(hipStreamSynchronize(c->stream));
+  uint32_t st = 0;
+  RDRT_HIP(hipMemcpy(&st, c->status, sizeof(st), hipMemcpyDeviceToHost));
+  if (st) {
+    RDRT_HIP(hipMemset(c->status, 0, sizeof(uint32_t)));
+    return fail(RDRT_ERR_CAPACITY, "a traversal stack overflowed; results of the last launch are incomplete");
+  }
+  return RDRT_OK;
+}
+
+int32_t rdrt_blas_begin(rdrt_ctx *c) {
+  if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
+  c->pending.clear();
+  c->in_blas = true;
+  return RDRT_OK;
+}
+
+int32_t rdrt_blas_add_triangles(rdrt_ctx *c, const float *pos, uint32_t nverts, const uint32_t *idx, uint32_t ntris,
+                                uint32_t flags) {
+  if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
+  if (!c->in_blas) return fail(RDRT_ERR_STATE, "rdrt_blas_add_triangles outside rdrt_blas_begin/end");
+  // A rejected geometry aborts the whole pending BLAS.
+  auto abort_blas = [c](const char *msg) {
+    c->pending.clear();
+    c->in_blas = false;
+    return fail(RDRT_ERR_INVALID_ARG, msg);
+  };
+  if (nverts && !pos) return abort_blas("null positions");
+  if (idx == nullptr && ntris != 0 && ntris != nverts / 3)
+    return abort_blas("non-indexed geometry: ntris must be 0 or nverts/3");
+  if (idx)
+    for (size_t i = 0; i < 3 * (size_t)ntris; i++)
+      if (idx[i] >= nverts) return abort_blas("triangle index out of range");
+  GeometrySrc g;
+  g.pos.assign(pos, pos + 3 * (size_t)nverts);
+  g.indexed = idx != nullptr;
+  if (idx) g.idx.assign(idx, idx + 3 * (size_t)ntris);
+  g.ntris = idx ? ntris : nverts / 3;
+  g.flags = flags;
+  c->pending.push_back(std::move(g));
+  return RDRT_OK;
+}
+
+int32_t rdrt_blas_end(rdrt_ctx *c, uint32_t *out) {
+  if (!c || !out) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  if (!c->in_blas) return fail(RDRT_ERR_STATE, "rdrt_blas_end without rdrt_blas_begin");
+  c->src.blas.push_back(std::move(c->pending));
+  c->src.blas_live.push_back(1);
+  c->pending.clear();
+  c->in_blas = false;
+  invalidate(c);
+  *out = (uint32_t)(c->src.blas.size() - 1);
+  return RDRT_OK;
+}
+
+int32_t rdrt_blas_destroy(rdrt_ctx *c, uint32_t h) {
+  if (!c || h >= c->src.blas.size()) return fail(RDRT_ERR_INVALID_ARG, "unknown BLAS handle");
+  c->src.blas_live[h] = 0;
+  c->src.blas[h].clear();
+  invalidate(c);
+  return RDRT_OK;
+}
+
+int32_t rdrt_tlas_create(rdrt_ctx *c, const rdrt_instance *inst, uint32_t n, uint32_t *out) {
+  if (!c || !out || (n && !inst)) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  c->src.tlas.emplace_back(inst, inst + n);
+  c->src.tlas_live.push_back(1);
+  invalidate(c);
+  *out = (uint32_t)(c->src.tlas.size() - 1);
+  return RDRT_OK;
+}
+
+int32_t rdrt_tlas_destroy(rdrt_ctx *c, uint32_t h) {
+  if (!c || h >= c->src.tlas.size()) return fail(RDRT_ERR_INVALID_ARG, "unknown TLAS handle");
+  c->src.tlas_live[h] = 0;
+  c->src.tlas[h].clear();
+  invalidate(c);
+  return RDRT_OK;
+}
+
+int32_t rdrt_bind_tlas(rdrt_ctx *c, const uint32_t *h, uint32_t n) {
+  if (!c || (n && !h)) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  std::vector<uint32_t> b(h, h + n);
+  if (b != c->src.binding) {  // naive/mod.rs:539-545
+    c->src.binding = b;
+    invalidate(c);
+  }
+  return RDRT_OK;
+}
+
+uint32_t rdrt_bind_tlas_max_len(const rdrt_ctx *) { return 0xffffffffu; }
+
+int32_t rdrt_build(rdrt_ctx *c) {
+  if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
+  if (c->built_valid) return RDRT_OK;
+  std::string err;
+  int32_t rc = build_scene(c->src, c->built, err);
+  if (rc) return fail(rc, err);
+  c->built_valid = true;
+  c->uploaded = false;
+  return RDRT_OK;
+}
+
+int32_t rdrt_commit(rdrt_ctx *c) {
+  if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  rc = rdrt_build(c);
+  if (rc) return rc;
+  if (c->uploaded) return RDRT_OK;
+  RDRT_HIP(hipStreamSynchronize(c->stream));
+  free_device_scene(c->dev);
+  const BuiltScene &b = c->built;
+  DeviceScene &d = c->dev;
+  RDRT_HIP(upload(d.tri_pairs, b.tri_pairs, d.bytes));
+  RDRT_HIP(upload(d.tlas_pairs, b.tlas_pairs, d.bytes));
+  RDRT_HIP(upload(d.tris, b.tris, d.bytes));
+  RDRT_HIP(upload(d.redirect, b.redirect, d.bytes));
+  RDRT_HIP(upload(d.indices, b.indices, d.bytes));
+  RDRT_HIP(upload(d.binding, b.binding, d.bytes));
+  RDRT_HIP(upload(d.vertices, b.vertices, d.bytes));
+  RDRT_HIP(upload(d.geos, b.geos, d.bytes));
+  RDRT_HIP(upload(d.insts, b.insts, d.bytes));
+  RDRT_HIP(upload(d.tlas_roots, b.tlas_roots, d.bytes));
+  RDRT_HIP(upload(d.tri_forest, b.tri_forest, d.bytes));
+  RDRT_HIP(upload(d.tlas_forest, b.tlas_forest, d.bytes));
+  c->uploaded = true;
+  return RDRT_OK;
+}
+
+int32_t rdrt_get_scene_info(const rdrt_ctx *c, rdrt_scene_info *o) {
+  if (!c || !o) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  if (!c->built_valid) return fail(RDRT_ERR_NOT_BUILT, "scene is not built");
+  const BuiltScene &b = c->built;
+  std::memset(o, 0, sizeof(*o));
+  o->tri_nodes = (uint32_t)b.tri_forest.size();
+  o->tlas_nodes = (uint32_t)b.tlas_forest.size();
+  o->tris = (uint32_t)b.redirect.size();
+  o->verts = (uint32_t)(b.vertices.size() / 3);
+  o->instances = (uint32_t)b.insts.size();
+  o->geometries = (uint32_t)b.geos.size();
+  o->blas = (uint32_t)c->src.blas.size();
+  o->tlas = (uint32_t)b.tlas_roots.size();
+  o->max_blas_depth = b.max_blas_depth;
+  o->max_tlas_depth = b.max_tlas_depth;
+  o->stack_entries = stack_entries_for(b.blas_stack + b.tlas_stack);
+  o->device_bytes = c->dev.bytes;
+  o->build_ms = b.build_ms;
+  return RDRT_OK;
+}
+
+int32_t rdrt_copy_built(const rdrt_ctx *c, uint32_t which, void *out, size_t bytes) {
+  if (!c || !out) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  if (!c->built_valid) return fail(RDRT_ERR_NOT_BUILT, "scene is not built");
+  const BuiltScene &b = c->built;
+  const void *src = nullptr;
+  size_t need = 0;
+  std::vector<uint32_t> order;
+  switch (which) {
+    case 0:
+      src = b.tri_forest.data();
+      need = b.tri_forest.size() * sizeof(RefNode);
+      break;
+    case 1:
+      src = b.tlas_forest.data();
+      need = b.tlas_forest.size() * sizeof(RefNode);
+      break;
+    case 2:
+      src = b.redirect.data();
+      need = b.redirect.size() * 4;
+      break;
+    case 3:
+      for (const InstRec &r : b.insts) order.push_back(r.order);
+      src = order.data();
+      need = order.size() * 4;
+      break;
+    default:
+      return fail(RDRT_ERR_INVALID_ARG, "unknown array");
+  }
+  if (bytes < need) return fail(RDRT_ERR_INVALID_ARG, "output buffer too small");
+  if (need) std::memcpy(out, src, need);
+  return RDRT_OK;
+}
+
+int32_t rdrt_trace(rdrt_ctx *c, const rdrt_trace_args *a) {
+  if (!c || !a) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  if (!c->uploaded) return fail(RDRT_ERR_NOT_BUILT, "scene is not committed (call rdrt_commit)");
+  if (a->n && (!a->rays || !a->hits)) return fail(RDRT_ERR_INVALID_ARG, "null rays or hits");
+  if (a->traversal > RDRT_TRAVERSAL_REFERENCE) return fail(RDRT_ERR_INVALID_ARG, "unknown traversal");
+  const DeviceScene &d = c->dev;
+  TraceParams p;
+  p.rays = a->rays;
+  p.meta = a->ray_meta;
+  p.hits = a->hits;
+  p.hit_meta = a->hit_meta;
+  p.counters = a->counters;
+  p.status = c->status;
+  p.n = a->n;
+  p.flags = a->flags;
+  p.cull_mask = a->cull_mask;
+  p.tlas_idx = a->tlas_idx;
+  p.tri_pairs = d.tri_pairs.ptr;
+  p.tris = d.tris.ptr;
+  p.redirect = d.redirect.ptr;
+  p.geos = d.geos.ptr;
+  p.tlas_pairs = d.tlas_pairs.ptr;
+  p.insts = d.insts.ptr;
+  p.tlas_roots = d.tlas_roots.ptr;
+  p.binding = d.binding.ptr;
+  p.nbinding = (uint32_t)d.binding.count;
+  p.ntlas = (uint32_t)d.tlas_roots.count;
+  p.tri_forest = d.tri_forest.ptr;
+  p.tlas_forest = d.tlas_forest.ptr;
+  p.indices = d.indices.ptr;
+  p.vertices = d.vertices.ptr;
+  uint32_t push = c->built.blas_stack + c->built.tlas_stack;
+  if (a->traversal == RDRT_TRAVERSAL_FAST && stack_entries_for(push) == 0)
+    return fail(RDRT_ERR_CAPACITY, "BVH too deep for the fast traversal stack (use RDRT_TRAVERSAL_REFERENCE)");
+  uint32_t block = c->cfg.block_size ? c->cfg.block_size : 256;
+  RDRT_HIP(hipEventRecord(c->ev0, c->stream));
+  RDRT_HIP(launch_trace(p, a->traversal, push, block, c->stream));
+  RDRT_HIP(hipEventRecord(c->ev1, c->stream));
+  c->timed = true;
+  return RDRT_OK;
+}
+
+int32_t rdrt_trace_closest(rdrt_ctx *c, const rdrt_ray *rays, uint32_t n, rdrt_hit *hits) {
+  if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
+  rdrt_trace_args a;
+  std::memset(&a, 0, sizeof(a));
+  a.rays = rays;
+  a.hits = hits;
+  a.n = n;
+  a.cull_mask = 0xffffffffu;
+  a.traversal = c->cfg.traversal;
+  return rdrt_trace(c, &a);
+}
+
+int32_t rdrt_last_trace_ms(rdrt_ctx *c, float *ms) {
+  if (!c || !ms) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  if (!c->timed) return fail(RDRT_ERR_STATE, "no trace recorded");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  RDRT_HIP(hipEventElapsedTime(ms, c->ev0, c->ev1));
+  return RDRT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,214 @@
+// bvh_build.cpp -- host SAH BVH builder (see bvh_build.h).
+//
+// Compiled with -ffp-contract=off: every bound, centre, bin index and SAH cost
+// is one IEEE FP32 rounding per operation, in the operation order of
+// content/space/src/bvh/strategy.rs and math/geometry/src/dimension3/box3.rs,
+// so the tree equals the reference's node for node.
+//
+// Design: centres are kept structure-of-arrays, a node's bin ids are computed
+// in one pass, the stable per-bin reorder is a counting scatter through a
+// scratch buffer, and subtrees of large nodes are built on separate threads
+// into their own preorder vectors that are spliced afterwards (nodes store
+// subtree sizes, not absolute indices, so splicing needs no fix-up).
+
+#include "bvh_build.h"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <thread>
+
+#include "rdrt_layout.h"
+
+namespace rdrt {
+
+Box box_empty() {
+  const float inf = std::numeric_limits<float>::infinity();
+  return Box{{inf, inf, inf}, {-inf, -inf, -inf}};
+}
+
+uint32_t Tree::miss(uint32_t i) const {
+  uint32_t next = i + nodes[i].size;
+  return next < nodes.size() ? next : kInvalid;
+}
+uint32_t Tree::hit(uint32_t i) const { return nodes[i].inner ? i + 1 : miss(i); }
+
+namespace {
+
+// Rust f32::min/max drop a NaN operand, like fminf/fmaxf.
+inline void grow(Box &b, const Box &o) {
+  for (int k = 0; k < 3; k++) {
+    b.mn[k] = std::fmin(b.mn[k], o.mn[k]);
+    b.mx[k] = std::fmax(b.mx[k], o.mx[k]);
+  }
+}
+
+// box3.rs:117-133: ties resolve to Z, then Y.
+inline int longest_axis(const Box &b) {
+  float x = b.mx[0] - b.mn[0], y = b.mx[1] - b.mn[1], z = b.mx[2] - b.mn[2];
+  if (x > y) return x > z ? 0 : 2;
+  return y > z ? 1 : 2;
+}
+
+// box3.rs:5-11
+inline float surface_area(const Box &b) {
+  float w = b.mx[0] - b.mn[0], h = b.mx[1] - b.mn[1], d = b.mx[2] - b.mn[2];
+  return 2.0f * (w * h + w * d + h * d);
+}
+
+// Rust's saturating float -> usize cast (NaN and negatives -> 0).
+inline size_t saturating_index(float f) {
+  if (!(f > 0.0f)) return 0;
+  if (f >= 1.8446744e19f) return std::numeric_limits<size_t>::max();
+  return (size_t)f;
+}
+
+struct Builder {
+  const std::vector<Box> &boxes;
+  TreeOptions opt;
+  std::vector<float> centre[3];
+  std::vector<uint32_t> order;
+  std::vector<uint32_t> scratch;  // disjoint per node range, so safe across threads
+
+  Builder(const std::vector<Box> &b, const TreeOptions &o) : boxes(b), opt(o) {
+    size_t n = b.size();
+    for (int k = 0; k < 3; k++) centre[k].resize(n);
+    order.resize(n);
+    scratch.resize(n);
+    for (size_t i = 0; i < n; i++) {
+      order[i] = (uint32_t)i;
+      for (int k = 0; k < 3; k++) centre[k][i] = (b[i].mn[k] + b[i].mx[k]) * 0.5f;  // box3.rs:90-93
+    }
+  }
+
+  Box range_box(uint32_t b, uint32_t e) const {
+    Box r = box_empty();
+    for (uint32_t i = b; i < e; i++) grow(r, boxes[order[i]]);
+    return r;
+  }
+
+  struct Split {
+    Box lb, rb;
+    uint32_t mid;
+  };
+
+  // strategy.rs:66-87 + apply.rs:304-333.  Only the order inside each half
+  // follows std::nth_element rather than Rust's select_nth_unstable_by.
+  Split median_split(const Box &pb, uint32_t b, uint32_t e) {
+    int axis = longest_axis(pb);
+    uint32_t mid = (e + b) / 2;
+    uint32_t half = (e - b) / 2;
+    if (half != 0) {
+      const float *c = centre[axis].data();
+      std::nth_element(order.begin() + b, order.begin() + b + half, order.begin() + e,
+                       [c](uint32_t x, uint32_t y) {
+                         float cx = c[x], cy = c[y];
+                         if (std::isnan(cx) || std::isnan(cy)) return true;  // unwrap_or(Less)
+                         return cx < cy;
+                       });
+    }
+    return Split{range_box(b, mid), range_box(mid, e), mid};
+  }
+
+  // strategy.rs:201-284
+  Split sah_split(const Box &pb, uint32_t b, uint32_t e) {
+    const uint32_t nb = opt.bins;
+    int axis = longest_axis(pb);
+    float a0 = pb.mn[axis], a1 = pb.mx[axis];
+    float step = (a1 - a0) / (float)nb;
+    const float *c = centre[axis].data();
+
+    std::vector<uint32_t> bin(e - b);
+    std::vector<uint32_t> count(nb, 0);
+    std::vector<Box> bbox(nb, box_empty());
+    for (uint32_t i = b; i < e; i++) {
+      uint32_t p = order[i];
+      size_t w = saturating_index(std::floor((c[p] - a0) / step));
+      if (w == nb) w -= 1;
+      if (w > nb) w = nb - 1;  // unreachable with finite bounds
+      bin[i - b] = (uint32_t)w;
+      count[w]++;
+      grow(bbox[w], boxes[p]);
+    }
+    uint32_t empty = 0;
+    for (uint32_t k = 0; k < nb; k++) empty += count[k] == 0 ? 1u : 0u;
+    if (empty == nb - 1) return median_split(pb, b, e);
+
+    // first strict minimum of area(L)*|L| + area(R)*|R|
+    uint32_t best = 0;
+    float best_cost = std::numeric_limits<float>::infinity();
+    Box best_l = box_empty(), best_r = box_empty();
+    uint32_t best_lc = 0;
+    for (uint32_t s = 0; s + 1 < nb; s++) {
+      Box l = box_empty(), r = box_empty();
+      uint32_t lc = 0, rc = 0;
+      for (uint32_t k = 0; k <= s; k++) {
+        lc += count[k];
+        grow(l, bbox[k]);
+      }
+      for (uint32_t k = s + 1; k < nb; k++) {
+        rc += count[k];
+        grow(r, bbox[k]);
+      }
+      float cost = surface_area(l) * (float)lc + surface_area(r) * (float)rc;
+      if (cost < best_cost) {
+        best_cost = cost;
+        best = s;
+        best_l = l;
+        best_r = r;
+        best_lc = lc;
+      }
+    }
+    (void)best;
+    // stable reorder by bin
+    std::vector<uint32_t> off(nb, 0);
+    for (uint32_t k = 1; k < nb; k++) off[k] = off[k - 1] + count[k - 1];
+    for (uint32_t i = b; i < e; i++) scratch[b + off[bin[i - b]]++] = order[i];
+    std::copy(scratch.begin() + b, scratch.begin() + e, order.begin() + b);
+    return Split{best_l, best_r, b + best_lc};
+  }
+
+  // strategy.rs:11-43: preorder, left subtree before right.
+  void build(const Box &box, uint32_t b, uint32_t e, uint32_t depth, std::vector<TreeNode> &out,
+             uint32_t &maxd, int spawn_levels) {
+    if (depth > maxd) maxd = depth;
+    size_t self = out.size();
+    out.push_back(TreeNode{box, b, e, 1, false});
+    if (!(depth < opt.max_depth && e - b > opt.bin_size)) return;
+    Split sp = sah_split(box, b, e);
+    out[self].inner = true;
+    if (spawn_levels > 0 && e - b >= 16384) {
+      std::vector<TreeNode> left;
+      uint32_t lmax = 0;
+      std::thread worker([&] { build(sp.lb, b, sp.mid, depth + 1, left, lmax, spawn_levels - 1); });
+      std::vector<TreeNode> right;
+      uint32_t rmax = 0;
+      build(sp.rb, sp.mid, e, depth + 1, right, rmax, spawn_levels - 1);
+      worker.join();
+      out.insert(out.end(), left.begin(), left.end());
+      out.insert(out.end(), right.begin(), right.end());
+      maxd = std::max(maxd, std::max(lmax, rmax));
+    } else {
+      build(sp.lb, b, sp.mid, depth + 1, out, maxd, 0);
+      build(sp.rb, sp.mid, e, depth + 1, out, maxd, 0);
+    }
+    out[self].size = (uint32_t)(out.size() - self);
+  }
+};
+
+}  // namespace
+
+Tree build_tree(const std::vector<Box> &boxes, const TreeOptions &opt_in, unsigned threads) {
+  TreeOptions opt = opt_in;
+  if (opt.bins < 2) opt.bins = 4;
+  Builder bld(boxes, opt);
+  Tree t;
+  Box root = bld.range_box(0, (uint32_t)boxes.size());  // utils.rs:224-238
+  int levels = 0;
+  while ((1u << levels) < threads && levels < 6) levels++;
+  bld.build(root, 0, (uint32_t)boxes.size(), 0, t.nodes, t.depth, levels);
+  t.order = std::move(bld.order);
+  return t;
+}
+
+}  // namespace rdrt

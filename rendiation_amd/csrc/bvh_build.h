@@ -1,0 +1,52 @@
+// bvh_build.h -- host SAH BVH builder of the ray-query path.
+//
+// Builds the same tree as the reference's FlattenBVH::new with SAH(bins) and
+// TreeBuildOption (content/space/src/bvh/mod.rs:55-79, strategy.rs:8-285,
+// apply.rs:9-78, utils.rs:192-238): binned SAH on the longest axis of the
+// node bounds, first strict cost minimum, stable bucket order, median split
+// when every centre lands in one bin.  Nodes come out in DFS preorder, as in
+// the reference, with the stackless hit/miss links of naive/mod.rs:611-632.
+#ifndef RDRT_BVH_BUILD_H
+#define RDRT_BVH_BUILD_H
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace rdrt {
+
+struct Box {
+  float mn[3], mx[3];
+};
+Box box_empty();
+
+struct TreeOptions {
+  uint32_t bin_size;   // leaf while count <= bin_size
+  uint32_t max_depth;  // leaf at depth >= max_depth
+  uint32_t bins;       // SAH bins
+};
+
+struct TreeNode {
+  Box box;
+  uint32_t begin, end;  // range into `order`
+  uint32_t size;        // nodes in this subtree (1 for a leaf)
+  bool inner;
+};
+
+struct Tree {
+  std::vector<TreeNode> nodes;  // DFS preorder; left child = i + 1
+  std::vector<uint32_t> order;  // sorted primitive index
+  uint32_t depth = 0;           // deepest node (root = 0)
+  // stackless links (naive/mod.rs:611-632): hit = first child or the miss
+  // link of a leaf; miss = next node after the subtree, or kInvalid.
+  uint32_t hit(uint32_t i) const;
+  uint32_t miss(uint32_t i) const;
+};
+
+// Builds the tree over `boxes`.  `threads` > 1 builds large subtrees
+// concurrently; the result does not depend on it.
+Tree build_tree(const std::vector<Box> &boxes, const TreeOptions &opt, unsigned threads);
+
+}  // namespace rdrt
+
+#endif

@@ -1,0 +1,468 @@
+// scene.cpp -- assembles the BLAS/TLAS forests and the child-pair layout.
+//
+// Reference: NaiveSahBvhSource::build_blas / build_tlas / build
+// (shader/ray-tracing/src/backend/wavefront_compute/geometry/naive/mod.rs:122-494).
+// Compiled with -ffp-contract=off: matrix inverses, instance bounds and the
+// triangle pre-pass are single-rounding FP32 in the reference operation order.
+
+#include "scene.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+namespace rdrt {
+namespace {
+
+// ---- Mat4 helpers, column-major fields a1..d4 (math/algebra/src/mat/mat4.rs) ----
+struct M4 {
+  float a1, a2, a3, a4, b1, b2, b3, b4, c1, c2, c3, c4, d1, d2, d3, d4;
+};
+static_assert(sizeof(M4) == 64, "M4 layout");
+
+M4 load_m4(const float *m) {
+  M4 r;
+  std::memcpy(&r, m, 64);
+  return r;
+}
+
+// mat4.rs det: the 24-term expansion, summed left to right.
+float det4(const M4 &m) {
+  return m.a1 * m.b2 * m.c3 * m.d4 - m.a1 * m.b2 * m.c4 * m.d3 + m.a1 * m.b3 * m.c4 * m.d2 -
+         m.a1 * m.b3 * m.c2 * m.d4 + m.a1 * m.b4 * m.c2 * m.d3 - m.a1 * m.b4 * m.c3 * m.d2 -
+         m.a2 * m.b3 * m.c4 * m.d1 + m.a2 * m.b3 * m.c1 * m.d4 - m.a2 * m.b4 * m.c1 * m.d3 +
+         m.a2 * m.b4 * m.c3 * m.d1 - m.a2 * m.b1 * m.c3 * m.d4 + m.a2 * m.b1 * m.c4 * m.d3 +
+         m.a3 * m.b4 * m.c1 * m.d2 - m.a3 * m.b4 * m.c2 * m.d1 + m.a3 * m.b1 * m.c2 * m.d4 -
+         m.a3 * m.b1 * m.c4 * m.d2 + m.a3 * m.b2 * m.c4 * m.d1 - m.a3 * m.b2 * m.c1 * m.d4 -
+         m.a4 * m.b1 * m.c2 * m.d3 + m.a4 * m.b1 * m.c3 * m.d2 - m.a4 * m.b2 * m.c3 * m.d1 +
+         m.a4 * m.b2 * m.c1 * m.d3 - m.a4 * m.b3 * m.c1 * m.d2 + m.a4 * m.b3 * m.c2 * m.d1;
+}
+
+// 3x3 minor sum used by every cofactor of mat4.rs inverse:
+// p*(q1*r2 - q2*r1) + s*(...) + w*(...), evaluated in that order.
+inline float cof(float p, float q1, float r2, float q2, float r1, float s, float s1, float t2, float s2, float t1,
+                 float w, float w1, float u2, float w2, float u1) {
+  return p * (q1 * r2 - q2 * r1) + s * (s1 * t2 - s2 * t1) + w * (w1 * u2 - w2 * u1);
+}
+
+// mat4.rs inverse, falling back to identity (mat/dimension.rs:18-20).
+M4 inverse_or_identity(const M4 &m) {
+  float det = det4(m);
+  M4 r{};
+  if (det == 0.0f) {
+    r.a1 = r.b2 = r.c3 = r.d4 = 1.0f;
+    return r;
+  }
+  float pi = 1.0f / det, ni = -pi;
+  r.a1 = pi * cof(m.b2, m.c3, m.d4, m.c4, m.d3, m.b3, m.c4, m.d2, m.c2, m.d4, m.b4, m.c2, m.d3, m.c3, m.d2);
+  r.a2 = ni * cof(m.a2, m.c3, m.d4, m.c4, m.d3, m.a3, m.c4, m.d2, m.c2, m.d4, m.a4, m.c2, m.d3, m.c3, m.d2);
+  r.a3 = pi * cof(m.a2, m.b3, m.d4, m.b4, m.d3, m.a3, m.b4, m.d2, m.b2, m.d4, m.a4, m.b2, m.d3, m.b3, m.d2);
+  r.a4 = ni * cof(m.a2, m.b3, m.c4, m.b4, m.c3, m.a3, m.b4, m.c2, m.b2, m.c4, m.a4, m.b2, m.c3, m.b3, m.c2);
+  r.b1 = ni * cof(m.b1, m.c3, m.d4, m.c4, m.d3, m.b3, m.c4, m.d1, m.c1, m.d4, m.b4, m.c1, m.d3, m.c3, m.d1);
+  r.b2 = pi * cof(m.a1, m.c3, m.d4, m.c4, m.d3, m.a3, m.c4, m.d1, m.c1, m.d4, m.a4, m.c1, m.d3, m.c3, m.d1);
+  r.b3 = ni * cof(m.a1, m.b3, m.d4, m.b4, m.d3, m.a3, m.b4, m.d1, m.b1, m.d4, m.a4, m.b1, m.d3, m.b3, m.d1);
+  r.b4 = pi * cof(m.a1, m.b3, m.c4, m.b4, m.c3, m.a3, m.b4, m.c1, m.b1, m.c4, m.a4, m.b1, m.c3, m.b3, m.c1);
+  r.c1 = pi * cof(m.b1, m.c2, m.d4, m.c4, m.d2, m.b2, m.c4, m.d1, m.c1, m.d4, m.b4, m.c1, m.d2, m.c2, m.d1);
+  r.c2 = ni * cof(m.a1, m.c2, m.d4, m.c4, m.d2, m.a2, m.c4, m.d1, m.c1, m.d4, m.a4, m.c1, m.d2, m.c2, m.d1);
+  r.c3 = pi * cof(m.a1, m.b2, m.d4, m.b4, m.d2, m.a2, m.b4, m.d1, m.b1, m.d4, m.a4, m.b1, m.d2, m.b2, m.d1);
+  r.c4 = ni * cof(m.a1, m.b2, m.c4, m.b4, m.c2, m.a2, m.b4, m.c1, m.b1, m.c4, m.a4, m.b1, m.c2, m.b2, m.c1);
+  r.d1 = ni * cof(m.b1, m.c2, m.d3, m.c3, m.d2, m.b2, m.c3, m.d1, m.c1, m.d3, m.b3, m.c1, m.d2, m.c2, m.d1);
+  r.d2 = pi * cof(m.a1, m.c2, m.d3, m.c3, m.d2, m.a2, m.c3, m.d1, m.c1, m.d3, m.a3, m.c1, m.d2, m.c2, m.d1);
+  r.d3 = ni * cof(m.a1, m.b2, m.d3, m.b3, m.d2, m.a2, m.b3, m.d1, m.b1, m.d3, m.a3, m.b1, m.d2, m.b2, m.d1);
+  r.d4 = pi * cof(m.a1, m.b2, m.c3, m.b3, m.c2, m.a2, m.b3, m.c1, m.b1, m.c3, m.a3, m.b1, m.c2, m.b2, m.c1);
+  return r;
+}
+
+// mat3.rs:37-42 on the upper-left 3x3
+float det3(const M4 &m) {
+  float t11 = m.c3 * m.b2 - m.b3 * m.c2;
+  float t12 = m.b3 * m.c1 - m.c3 * m.b1;
+  float t13 = m.c2 * m.b1 - m.b2 * m.c1;
+  return m.a1 * t11 + m.a2 * t12 + m.a3 * t13;
+}
+
+// mat4.rs:155-169: (M * (p, 1)).xyz / w
+void transform_point(const M4 &m, const float p[3], float out[3]) {
+  float x = p[0] * m.a1 + p[1] * m.b1 + p[2] * m.c1 + 1.0f * m.d1;
+  float y = p[0] * m.a2 + p[1] * m.b2 + p[2] * m.c2 + 1.0f * m.d2;
+  float z = p[0] * m.a3 + p[1] * m.b3 + p[2] * m.c3 + 1.0f * m.d3;
+  float w = p[0] * m.a4 + p[1] * m.b4 + p[2] * m.c4 + 1.0f * m.d4;
+  out[0] = x / w;
+  out[1] = y / w;
+  out[2] = z / w;
+}
+
+inline void grow_point(Box &b, const float p[3]) {
+  for (int k = 0; k < 3; k++) {
+    b.mn[k] = std::fmin(b.mn[k], p[k]);
+    b.mx[k] = std::fmax(b.mx[k], p[k]);
+  }
+}
+inline void grow_box(Box &b, const Box &o) {
+  for (int k = 0; k < 3; k++) {
+    b.mn[k] = std::fmin(b.mn[k], o.mn[k]);
+    b.mx[k] = std::fmax(b.mx[k], o.mx[k]);
+  }
+}
+inline bool is_empty(const Box &b) { return b.mx[0] < b.mn[0] || b.mx[1] < b.mn[1] || b.mx[2] < b.mn[2]; }
+
+// box3.rs:22-38: the eight corners in 000..111 order.
+Box transform_box(const Box &b, const M4 &m) {
+  if (is_empty(b)) return b;
+  Box r = box_empty();
+  for (int c = 0; c < 8; c++) {
+    float p[3] = {(c & 4) ? b.mx[0] : b.mn[0], (c & 2) ? b.mx[1] : b.mn[1], (c & 1) ? b.mx[2] : b.mn[2]};
+    float q[3];
+    transform_point(m, p, q);
+    grow_point(r, q);
+  }
+  return r;
+}
+
+// The vertex-only part of the reference ray/triangle test
+// (wavefront_compute/geometry/mod.rs:115-117,138-140,144).
+TriPre pre_triangle(const float *v0, const float *v1, const float *v2) {
+  TriPre t;
+  float e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+  float e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+  float c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+  float mag_sq = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+  if (mag_sq > 0.0f) {  // vec/dimension.rs:98-105
+    float inv = 1.0f / std::sqrt(mag_sq);
+    c[0] = c[0] * inv;
+    c[1] = c[1] * inv;
+    c[2] = c[2] * inv;
+  }
+  for (int k = 0; k < 3; k++) {
+    t.v0[k] = v0[k];
+    t.e1[k] = e1[k];
+    t.e2[k] = e2[k];
+    t.n[k] = c[k];
+  }
+  t.uu = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+  t.uv = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
+  t.vv = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
+  t.inverse_d = 1.0f / (t.uv * t.uv - t.uu * t.vv);
+  return t;
+}
+
+void put_box(float *mn, float *mx, const Box &b) {
+  for (int k = 0; k < 3; k++) {
+    mn[k] = b.mn[k];
+    mx[k] = b.mx[k];
+  }
+}
+
+// Child-pair conversion of one tree.  `prim_base` offsets leaf ranges into the
+// global primitive array; pair nodes are appended to `pairs`.  Returns the
+// root ref; `stack_need` receives the deepest push count on any path.
+struct PairConv {
+  const Tree &t;
+  uint32_t prim_base;
+  std::vector<PairNode> &pairs;
+  std::string &err;
+  bool ok = true;
+
+  uint32_t leaf(uint32_t begin, uint32_t count, const Box &b, uint32_t &depth) {
+    uint32_t start = prim_base + begin;
+    if (start + count > kLeafMaxStart + 1) {
+      ok = false;
+      err = "scene too large for the child-pair layout (primitive index >= 2^26)";
+      return 0;
+    }
+    depth = 0;
+    if (count <= kLeafMaxCount) return leaf_ref(start, count);
+    // An oversized leaf (depth-capped build) becomes a chain of pair nodes
+    // with the leaf's box on both sides: same triangles, same tests.
+    uint32_t idx = (uint32_t)pairs.size();
+    pairs.emplace_back();
+    uint32_t sub_depth = 0;
+    uint32_t rest = leaf(begin + kLeafMaxCount, count - kLeafMaxCount, b, sub_depth);
+    PairNode &p = pairs[idx];
+    std::memset(&p, 0, sizeof(p));
+    put_box(p.lmn, p.lmx, b);
+    put_box(p.rmn, p.rmx, b);
+    p.lref = leaf_ref(start, kLeafMaxCount);
+    p.rref = rest;
+    depth = 1 + sub_depth;
+    return idx;
+  }
+
+  uint32_t ref_of(uint32_t i, uint32_t &depth) {
+    const TreeNode &n = t.nodes[i];
+    if (!n.inner) return leaf(n.begin, n.end - n.begin, n.box, depth);
+    uint32_t idx = (uint32_t)pairs.size();
+    pairs.emplace_back();
+    uint32_t l = i + 1, r = i + 1 + t.nodes[i + 1].size;
+    uint32_t dl = 0, dr = 0;
+    uint32_t lref = ref_of(l, dl);
+    uint32_t rref = ref_of(r, dr);
+    PairNode &p = pairs[idx];
+    std::memset(&p, 0, sizeof(p));
+    put_box(p.lmn, p.lmx, t.nodes[l].box);
+    put_box(p.rmn, p.rmx, t.nodes[r].box);
+    p.lref = lref;
+    p.rref = rref;
+    depth = 1 + std::max(dl, dr);
+    return idx;
+  }
+};
+
+// Reference forest nodes of one tree (naive/mod.rs:338-367).
+void append_forest(const Tree &t, uint32_t prim_offset, std::vector<RefNode> &out) {
+  uint32_t base = (uint32_t)out.size();
+  for (uint32_t i = 0; i < t.nodes.size(); i++) {
+    const TreeNode &n = t.nodes[i];
+    RefNode g;
+    std::memset(&g, 0, sizeof(g));
+    put_box(g.mn, g.mx, n.box);
+    uint32_t h = t.hit(i), m = t.miss(i);
+    g.hit_next = h != kInvalid ? h + base : kInvalid;
+    g.miss_next = m != kInvalid ? m + base : kInvalid;
+    g.begin = n.begin + prim_offset;
+    g.end = n.end + prim_offset;
+    out.push_back(g);
+  }
+}
+
+template <class F>
+void parallel_for(size_t n, unsigned threads, F f) {
+  if (threads <= 1 || n <= 1) {
+    for (size_t i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::vector<std::thread> pool;
+  std::atomic<size_t> next{0};
+  for (unsigned w = 0; w < threads; w++)
+    pool.emplace_back([&] {
+      for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+    });
+  for (auto &th : pool) th.join();
+}
+
+// Every ref the kernels follow must stay inside the arrays they index: inner
+// refs inside the pair array, leaf ranges inside the triangle / instance
+// array, forest links and ranges inside theirs.  Checked once per build.
+bool ref_ok(uint32_t ref, size_t npairs, size_t nprims) {
+  if (ref & kLeafBit) {
+    uint32_t start = (ref & ~kLeafBit) >> kLeafCountBits, count = ref & kLeafMaxCount;
+    return (size_t)start + count <= nprims;
+  }
+  return ref < npairs;
+}
+
+bool validate_refs(const BuiltScene &s, std::string &err) {
+  for (const PairNode &p : s.tri_pairs)
+    if (!ref_ok(p.lref, s.tri_pairs.size(), s.tris.size()) || !ref_ok(p.rref, s.tri_pairs.size(), s.tris.size())) {
+      err = "internal: BLAS child-pair ref out of range";
+      return false;
+    }
+  for (const PairNode &p : s.tlas_pairs)
+    if (!ref_ok(p.lref, s.tlas_pairs.size(), s.insts.size()) ||
+        !ref_ok(p.rref, s.tlas_pairs.size(), s.insts.size())) {
+      err = "internal: TLAS child-pair ref out of range";
+      return false;
+    }
+  for (const GeoRec &g : s.geos)
+    if (!ref_ok(g.root_ref, s.tri_pairs.size(), s.tris.size()) || g.ref_root >= s.tri_forest.size()) {
+      err = "internal: geometry root out of range";
+      return false;
+    }
+  for (const TlasRoot &r : s.tlas_roots)
+    if (r.ref_root != kInvalid &&
+        (!ref_ok(r.root_ref, s.tlas_pairs.size(), s.insts.size()) || r.ref_root >= s.tlas_forest.size())) {
+      err = "internal: TLAS root out of range";
+      return false;
+    }
+  for (const InstRec &in : s.insts)
+    if (in.geo_begin > in.geo_end || in.geo_end > s.geos.size()) {
+      err = "internal: instance geometry range out of range";
+      return false;
+    }
+  auto forest_ok = [](const std::vector<RefNode> &f, size_t nprims) {
+    for (const RefNode &n : f) {
+      if (n.hit_next != kInvalid && n.hit_next >= f.size()) return false;
+      if (n.miss_next != kInvalid && n.miss_next >= f.size()) return false;
+      if (n.begin > n.end || n.end > nprims) return false;
+    }
+    return true;
+  };
+  if (!forest_ok(s.tri_forest, s.redirect.size()) || !forest_ok(s.tlas_forest, s.insts.size())) {
+    err = "internal: forest link or range out of range";
+    return false;
+  }
+  for (uint32_t r : s.redirect)
+    if ((size_t)3 * r + 2 >= s.indices.size()) {
+      err = "internal: redirect out of range";
+      return false;
+    }
+  for (uint32_t v : s.indices)
+    if ((size_t)3 * v + 2 >= s.vertices.size()) {
+      err = "internal: vertex index out of range";
+      return false;
+    }
+  return true;
+}
+
+}  // namespace
+
+int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
+  auto t0 = std::chrono::steady_clock::now();
+  out = BuiltScene();
+  unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+
+  // ---- BLAS: one SAH tree per geometry, built concurrently ----
+  struct Job {
+    uint32_t blas, geo;
+    std::vector<uint32_t> tri_idx;  // 3 per triangle, local vertex ids
+    Tree tree;
+  };
+  std::vector<Job> jobs;
+  for (uint32_t b = 0; b < src.blas.size(); b++) {
+    if (!src.blas_live[b]) continue;
+    for (uint32_t g = 0; g < src.blas[b].size(); g++) jobs.push_back(Job{b, g, {}, {}});
+  }
+  unsigned per_job = jobs.size() >= hw ? 1u : std::max(1u, hw / (unsigned)std::max<size_t>(1, jobs.size()));
+  parallel_for(jobs.size(), std::min<unsigned>(hw, (unsigned)jobs.size()), [&](size_t j) {
+    Job &job = jobs[j];
+    const GeometrySrc &g = src.blas[job.blas][job.geo];
+    size_t nv = g.pos.size() / 3;
+    if (g.indexed) job.tri_idx = g.idx;
+    else {
+      job.tri_idx.resize(nv - nv % 3);
+      for (size_t v = 0; v < job.tri_idx.size(); v++) job.tri_idx[v] = (uint32_t)v;
+    }
+    size_t nt = job.tri_idx.size() / 3;
+    std::vector<Box> boxes(nt);
+    for (size_t t = 0; t < nt; t++) {
+      Box bx = box_empty();
+      for (int k = 0; k < 3; k++) grow_point(bx, &g.pos[3 * (size_t)job.tri_idx[3 * t + k]]);
+      boxes[t] = bx;
+    }
+    job.tree = build_tree(boxes, src.blas_opt, per_job);
+  });
+  for (const Job &job : jobs) {
+    const GeometrySrc &g = src.blas[job.blas][job.geo];
+    for (uint32_t v : job.tri_idx)
+      if (v >= g.pos.size() / 3) {
+        err = "triangle index out of range of the geometry's vertices";
+        return RDRT_ERR_INVALID_ARG;
+      }
+  }
+
+  // ---- assemble BLAS arrays in reference order ----
+  std::vector<Box> blas_box(src.blas.size(), box_empty());
+  std::vector<char> blas_has_box(src.blas.size(), 0);
+  std::vector<uint32_t> blas_geo_begin(src.blas.size(), 0), blas_geo_end(src.blas.size(), 0);
+  {
+    size_t ji = 0;
+    for (uint32_t b = 0; b < src.blas.size(); b++) {
+      blas_geo_begin[b] = (uint32_t)out.geos.size();
+      if (!src.blas_live[b]) {
+        blas_geo_end[b] = blas_geo_begin[b];
+        continue;
+      }
+      for (uint32_t gi = 0; gi < src.blas[b].size(); gi++, ji++) {
+        const Job &job = jobs[ji];
+        const GeometrySrc &g = src.blas[b][gi];
+        uint32_t primitive_start = (uint32_t)(out.indices.size() / 3);
+        uint32_t vertex_start = (uint32_t)(out.vertices.size() / 3);
+        out.vertices.insert(out.vertices.end(), g.pos.begin(), g.pos.end() - (g.pos.size() % 3));
+        for (uint32_t k : job.tree.order) out.redirect.push_back(primitive_start + k);
+        for (uint32_t v : job.tri_idx) out.indices.push_back(vertex_start + v);
+        GeoRec gr;
+        std::memset(&gr, 0, sizeof(gr));
+        put_box(gr.mn, gr.mx, job.tree.nodes[0].box);
+        gr.ref_root = (uint32_t)out.tri_forest.size();
+        gr.primitive_start = primitive_start;
+        gr.geometry_idx = gi;
+        gr.flags = g.flags;
+        append_forest(job.tree, primitive_start, out.tri_forest);
+        PairConv pc{job.tree, primitive_start, out.tri_pairs, err};
+        uint32_t d = 0;
+        gr.root_ref = pc.ref_of(0, d);
+        if (!pc.ok) return RDRT_ERR_CAPACITY;
+        out.blas_stack = std::max(out.blas_stack, d);
+        out.max_blas_depth = std::max(out.max_blas_depth, job.tree.depth);
+        out.geos.push_back(gr);
+        grow_box(blas_box[b], job.tree.nodes[0].box);
+        blas_has_box[b] = 1;
+      }
+      blas_geo_end[b] = (uint32_t)out.geos.size();
+    }
+  }
+  // triangle pre-pass, in redirect (leaf) order
+  out.tris.resize(out.redirect.size());
+  parallel_for((out.tris.size() + 65535) / 65536, hw, [&](size_t c) {
+    size_t e = std::min(out.tris.size(), (c + 1) * 65536);
+    for (size_t k = c * 65536; k < e; k++) {
+      size_t t = out.redirect[k];
+      const float *v0 = &out.vertices[3 * (size_t)out.indices[3 * t]];
+      const float *v1 = &out.vertices[3 * (size_t)out.indices[3 * t + 1]];
+      const float *v2 = &out.vertices[3 * (size_t)out.indices[3 * t + 2]];
+      out.tris[k] = pre_triangle(v0, v1, v2);
+    }
+  });
+
+  // ---- TLAS (naive/mod.rs:262-320, 435-454) ----
+  for (uint32_t ti = 0; ti < src.tlas.size(); ti++) {
+    TlasRoot root;
+    std::memset(&root, 0, sizeof(root));
+    if (!src.tlas_live[ti]) {
+      root.ref_root = kInvalid;
+      root.root_ref = kInvalid;
+      out.tlas_roots.push_back(root);
+      continue;
+    }
+    const auto &inst = src.tlas[ti];
+    std::vector<Box> boxes;
+    for (const rdrt_instance &in : inst) {
+      if (in.blas >= src.blas.size() || !blas_has_box[in.blas]) {
+        err = "instance references a deleted, empty or unknown BLAS";
+        return RDRT_ERR_INVALID_ARG;
+      }
+      boxes.push_back(transform_box(blas_box[in.blas], load_m4(in.transform)));
+    }
+    Tree tree = build_tree(boxes, src.tlas_opt, 1);
+    out.max_tlas_depth = std::max(out.max_tlas_depth, tree.depth);
+    uint32_t inst_base = (uint32_t)out.insts.size();
+    for (uint32_t k : tree.order) {
+      const rdrt_instance &in = inst[k];
+      M4 m = load_m4(in.transform);
+      uint32_t flags = in.flags;
+      if (det3(m) < 0.0f) flags ^= RDRT_INSTANCE_TRIANGLE_FLIP_FACING;
+      InstRec r;
+      std::memset(&r, 0, sizeof(r));
+      M4 inv = inverse_or_identity(m);
+      std::memcpy(r.inv, &inv, 64);
+      put_box(r.mn, r.mx, boxes[k]);
+      r.mask = in.mask;
+      r.flags = flags;
+      r.geo_begin = blas_geo_begin[in.blas];
+      r.geo_end = blas_geo_end[in.blas];
+      r.custom_index = in.instance_custom_index;
+      r.order = k;
+      r.sbt_offset = in.sbt_offset;
+      r.blas = in.blas;
+      out.insts.push_back(r);
+    }
+    root.ref_root = (uint32_t)out.tlas_forest.size();
+    append_forest(tree, inst_base, out.tlas_forest);
+    put_box(root.mn, root.mx, tree.nodes[0].box);
+    PairConv pc{tree, inst_base, out.tlas_pairs, err};
+    uint32_t d = 0;
+    root.root_ref = pc.ref_of(0, d);
+    if (!pc.ok) return RDRT_ERR_CAPACITY;
+    out.tlas_stack = std::max(out.tlas_stack, d);
+    out.tlas_roots.push_back(root);
+  }
+  out.binding = src.binding;
+  if (!validate_refs(out, err)) return RDRT_ERR_STATE;
+  out.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return RDRT_OK;
+}
+
+}  // namespace rdrt

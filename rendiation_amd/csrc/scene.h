@@ -1,0 +1,59 @@
+// scene.h -- host scene sources and the built device-ready arrays.
+#ifndef RDRT_SCENE_H
+#define RDRT_SCENE_H
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rdrt.h"
+#include "bvh_build.h"
+#include "rdrt_layout.h"
+
+namespace rdrt {
+
+// One triangle geometry of a BLAS (BottomLevelAccelerationStructureBuildSource,
+// shader/ray-tracing/src/api/backend.rs:104-128), copied at creation.
+struct GeometrySrc {
+  std::vector<float> pos;
+  std::vector<uint32_t> idx;  // empty = non-indexed
+  bool indexed = false;
+  uint32_t ntris = 0;
+  uint32_t flags = 0;
+};
+
+struct SceneSrc {
+  std::vector<char> blas_live;
+  std::vector<std::vector<GeometrySrc>> blas;
+  std::vector<char> tlas_live;
+  std::vector<std::vector<rdrt_instance>> tlas;
+  std::vector<uint32_t> binding;
+  TreeOptions blas_opt{2, 50, 4};
+  TreeOptions tlas_opt{10, 50, 4};
+};
+
+struct BuiltScene {
+  // reference layout (naive/mod.rs:322-494)
+  std::vector<RefNode> tri_forest, tlas_forest;
+  std::vector<uint32_t> redirect, indices;
+  std::vector<float> vertices;  // xyz
+  // shared records
+  std::vector<GeoRec> geos;
+  std::vector<InstRec> insts;
+  std::vector<TlasRoot> tlas_roots;
+  std::vector<uint32_t> binding;
+  // child-pair layout
+  std::vector<PairNode> tri_pairs, tlas_pairs;
+  std::vector<TriPre> tris;
+  uint32_t max_blas_depth = 0, max_tlas_depth = 0;
+  uint32_t blas_stack = 0, tlas_stack = 0;  // deepest push count of the child-pair trees
+  double build_ms = 0.0;
+};
+
+// Builds every BVH and both layouts.  Returns RDRT_OK or an RDRT_ERR_* code
+// with a message in `err`.
+int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err);
+
+}  // namespace rdrt
+
+#endif

@@ -1,0 +1,48 @@
+// trace_kernels.h -- launch interface of the closest-hit kernels.
+#ifndef RDRT_TRACE_KERNELS_H
+#define RDRT_TRACE_KERNELS_H
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/rdrt.h"
+#include "rdrt_layout.h"
+
+namespace rdrt {
+
+// Device view of a committed scene plus one launch's I/O.
+struct TraceParams {
+  const rdrt_ray *rays;
+  const rdrt_ray_meta *meta;
+  rdrt_hit *hits;
+  rdrt_hit_meta *hit_meta;
+  uint32_t *counters;
+  uint32_t *status;  // device word; bit 0 = a lane's traversal stack would have overflowed
+  uint32_t n, flags, cull_mask, tlas_idx;
+
+  const PairNode *tri_pairs;
+  const TriPre *tris;
+  const uint32_t *redirect;
+  const GeoRec *geos;
+  const PairNode *tlas_pairs;
+  const InstRec *insts;
+  const TlasRoot *tlas_roots;
+  const uint32_t *binding;
+  uint32_t nbinding, ntlas;
+
+  const RefNode *tri_forest;
+  const RefNode *tlas_forest;
+  const uint32_t *indices;
+  const float *vertices;
+};
+
+// Smallest stack (entries per lane) that a fast-traversal kernel is built
+// with for the given push depth; 0 if none is large enough.
+uint32_t stack_entries_for(uint32_t push_depth);
+
+// Enqueues one traversal launch on `stream`.  mode = RDRT_TRAVERSAL_*.
+hipError_t launch_trace(const TraceParams &p, uint32_t mode, uint32_t push_depth, uint32_t block,
+                        hipStream_t stream);
+
+}  // namespace rdrt
+
+#endif

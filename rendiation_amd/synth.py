@@ -1,0 +1,144 @@
+"""Seeded synthetic inputs for the ray-query workloads.
+
+Every random number comes from a counter-based SplitMix64 stream:
+``x_i = mix64(key(seed, stream) + (i + 1) * 0x9E3779B97F4A7C15)`` with the
+standard SplitMix64 finaliser, so any slice of any stream is reproducible
+without generating the prefix.  A float in [0, 1) is ``(x >> 40) * 2^-24``
+(exact in FP32).  The same arrays feed the GPU path and the CPU oracle.
+
+Streams used by the closest-hit workload (100k triangles, 1M rays):
+  stream 1: triangle centroids, 3 per triangle, U[0,1)
+  stream 2: vertex offsets, 9 per triangle, U[-0.01, 0.01]
+  stream 3: ray origins, 3 per ray, U[-0.5, 1.5)
+  stream 4: ray directions, 2 per ray, uniform on the unit sphere through
+            uniform_sample_sphere_dir (math/statistics/src/distribution_map.rs:61-68)
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+_STREAM_KEY = np.uint64(0xD1B54A32D192ED03)
+
+
+def _mix(z: np.ndarray) -> np.ndarray:
+    z = (z ^ (z >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+def splitmix64(seed: int, stream: int, n: int, offset: int = 0) -> np.ndarray:
+    """n uint64 values of stream ``stream`` starting at index ``offset``."""
+    with np.errstate(over="ignore"):
+        key = _mix(np.array([seed], dtype=np.uint64) * _GAMMA + np.uint64(stream) * _STREAM_KEY)[0]
+        i = np.arange(offset + 1, offset + n + 1, dtype=np.uint64)
+        return _mix(key + i * _GAMMA)
+
+
+def uniform_f32(seed: int, stream: int, n: int, offset: int = 0) -> np.ndarray:
+    """n float32 in [0, 1), 24-bit resolution."""
+    x = splitmix64(seed, stream, n, offset) >> np.uint64(40)
+    return x.astype(np.float32) * np.float32(2.0 ** -24)
+
+
+def sphere_dirs(u: np.ndarray, v: np.ndarray) -> np.ndarray:
+    """uniform_sample_sphere_dir (distribution_map.rs:61-68) in FP32; cos/sin
+    are evaluated in FP64 and rounded, so the result is platform-stable."""
+    u = u.astype(np.float32)
+    v = v.astype(np.float32)
+    z = np.float32(1.0) - np.float32(2.0) * u
+    r = np.sqrt(np.maximum(np.float32(0.0), np.float32(1.0) - z * z)).astype(np.float32)
+    phi = (np.float32(np.pi) * np.float32(2.0)) * v
+    c = np.cos(phi.astype(np.float64)).astype(np.float32)
+    s = np.sin(phi.astype(np.float64)).astype(np.float32)
+    return np.stack([c * r, s * r, z], axis=-1).astype(np.float32)
+
+
+def random_triangles(seed: int, ntris: int, spread: float = 0.01) -> np.ndarray:
+    """(3*ntris, 3) float32 non-indexed triangle soup: centroid c ~ U[0,1)^3,
+    each vertex c + U[-spread, spread]^3."""
+    c = uniform_f32(seed, 1, 3 * ntris).reshape(ntris, 1, 3)
+    off = (np.float32(2.0) * uniform_f32(seed, 2, 9 * ntris) - np.float32(1.0)) * np.float32(spread)
+    v = (c + off.reshape(ntris, 3, 3)).astype(np.float32)
+    return v.reshape(-1, 3)
+
+
+def random_rays(seed: int, nrays: int, tmin: float = 1e-4, tmax: float = np.inf, lo: float = -0.5,
+                hi: float = 1.5, offset: int = 0) -> np.ndarray:
+    """(nrays, 8) float32 rdrt_ray records: origin ~ U[lo, hi)^3, direction
+    uniform on the sphere, range [tmin, tmax].  ``offset`` selects rays
+    offset..offset+nrays of the same streams."""
+    o = np.float32(lo) + np.float32(hi - lo) * uniform_f32(seed, 3, 3 * nrays, 3 * offset).reshape(nrays, 3)
+    uv = uniform_f32(seed, 4, 2 * nrays, 2 * offset).reshape(nrays, 2)
+    d = sphere_dirs(uv[:, 0], uv[:, 1])
+    out = np.empty((nrays, 8), dtype=np.float32)
+    out[:, 0:3] = o
+    out[:, 3] = np.float32(tmin)
+    out[:, 4:7] = d
+    out[:, 7] = np.float32(tmax)
+    return out
+
+
+def parametric_grid_indices(nu: int, nv: int) -> np.ndarray:
+    """Triangle indices of triangulate_parametric
+    (content/mesh/generator/src/builder/mod.rs:73-125): vertex (u, v) has
+    index v + (nv + 1) * u, each cell (a b / c d) gives (a, c, b), (b, c, d)."""
+    u = np.arange(nu, dtype=np.uint32)[:, None]
+    v = np.arange(nv, dtype=np.uint32)[None, :]
+    a = v + (nv + 1) * u
+    b = a + 1
+    c = a + (nv + 1)
+    d = c + 1
+    tri = np.stack([np.stack([a, c, b], -1), np.stack([b, c, d], -1)], axis=2)
+    return tri.reshape(-1, 3).astype(np.uint32)
+
+
+def _grid_uv(nu: int, nv: int):
+    uu = (np.arange(nu + 1, dtype=np.float32) * np.float32(1.0 / nu))[:, None]
+    vv = (np.arange(nv + 1, dtype=np.float32) * np.float32(1.0 / nv))[None, :]
+    return np.broadcast_to(uu, (nu + 1, nv + 1)), np.broadcast_to(vv, (nu + 1, nv + 1))
+
+
+def uv_sphere(nu: int, nv: int, radius: float = 1.0):
+    """UV sphere (content/mesh/generator/src/primitive.rs UVSphere):
+    (cos 2pi u sin pi v, cos pi v, sin 2pi u sin pi v) * radius; returns
+    (positions (nverts, 3) f32, indices (ntris, 3) u32).  The pole rows give
+    zero-area triangles, as in the reference generator."""
+    u, v = _grid_uv(nu, nv)
+    a = 2.0 * np.pi * u.astype(np.float64)
+    b = np.pi * v.astype(np.float64)
+    p = np.stack([np.cos(a) * np.sin(b), np.cos(b), np.sin(a) * np.sin(b)], -1) * radius
+    return p.reshape(-1, 3).astype(np.float32), parametric_grid_indices(nu, nv)
+
+
+def torus(nu: int, nv: int, radius: float = 1.0, tube_radius: float = 0.2):
+    """Torus around the z axis (TorusMeshParameter, builtin.rs:4-21): ring
+    radius ``radius`` in the xy plane, tube radius ``tube_radius``."""
+    u, v = _grid_uv(nu, nv)
+    a = 2.0 * np.pi * u.astype(np.float64)
+    b = 2.0 * np.pi * v.astype(np.float64)
+    ring = radius + tube_radius * np.cos(b)
+    p = np.stack([ring * np.cos(a), ring * np.sin(a), tube_radius * np.sin(b)], -1)
+    return p.reshape(-1, 3).astype(np.float32), parametric_grid_indices(nu, nv)
+
+
+def random_rotation(seed: int, stream: int, index: int) -> np.ndarray:
+    """Uniform random 3x3 rotation (quaternion from 4 normals via Box-Muller)."""
+    x = uniform_f32(seed, stream, 8, 8 * index).astype(np.float64)
+    x = np.clip(x, 1e-12, 1.0)
+    r = np.sqrt(-2.0 * np.log(x[0::2]))
+    g = r * np.cos(2.0 * np.pi * x[1::2])
+    q = g / np.linalg.norm(g)
+    w, i, j, k = q
+    return np.array([[1 - 2 * (j * j + k * k), 2 * (i * j - k * w), 2 * (i * k + j * w)],
+                     [2 * (i * j + k * w), 1 - 2 * (i * i + k * k), 2 * (j * k - i * w)],
+                     [2 * (i * k - j * w), 2 * (j * k + i * w), 1 - 2 * (i * i + j * j)]])
+
+
+# Closest-hit workload sizes (100k triangles, 1M rays, seed 0).
+CLOSEST_HIT_TRIS = 100_000
+CLOSEST_HIT_RAYS = 1_000_000
+CLOSEST_HIT_LEAF = 4
