@@ -204,6 +204,70 @@ int32_t rdrt_trace_closest(rdrt_ctx *ctx, const rdrt_ray *rays_dev, uint32_t n, 
  * rdrt_sync first. */
 int32_t rdrt_last_trace_ms(rdrt_ctx *ctx, float *ms);
 
+/* ---- shading attributes (bindless mesh data of
+ *      scene/rendering/gpu-ray-tracing/src/bindless_mesh_bridge.rs:11-135) ---- */
+/* Lambertian material (content/lighting/transport/src/surface/microfacet/
+ * device/diffuse.rs): albedo and emitted radiance. */
+typedef struct rdrt_material {
+  float albedo[3];
+  float pad0;
+  float emission[3];
+  float pad1;
+} rdrt_material;
+
+/* Per-vertex normals of one geometry of a BLAS (xyz per vertex, copied);
+ * interpolated with the hit barycentrics for shading.  Without them the
+ * geometric normal is used. */
+int32_t rdrt_blas_set_vertex_normals(rdrt_ctx *ctx, uint32_t blas, uint32_t geometry, const float *normals_xyz,
+                                     uint32_t nverts);
+/* Per-triangle material offsets of one geometry (copied).  A hit's material
+ * is materials[instance_custom_index + offset], as the reference looks the
+ * surface up by the instance's custom id (feature/path_tracing/ray_hit.rs:65-75). */
+int32_t rdrt_blas_set_triangle_materials(rdrt_ctx *ctx, uint32_t blas, uint32_t geometry,
+                                         const uint32_t *offsets, uint32_t ntris);
+int32_t rdrt_set_materials(rdrt_ctx *ctx, const rdrt_material *materials, uint32_t n);
+
+/* ---- wavefront path tracing (scene/rendering/gpu-ray-tracing/src/feature/
+ *      path_tracing/mod.rs:127-235, ray_gen.rs, ray_hit.rs, ray_miss.rs) ---- */
+typedef struct rdrt_pt_params {
+  float view_projection_inv[16]; /* column-major NDC -> world */
+  float camera_position[3];
+  uint32_t width, height, spp;
+  uint32_t max_bounces; /* scattering events per path (<= 16) */
+  uint32_t rr_start;    /* first bounce with Russian roulette */
+  uint32_t seed;        /* per-pixel sample scrambles: xxhash32(px, py, seed * 2^16 + dimension pair) */
+  uint32_t sample_index_base;
+  float sky_up[3], sky_down[3];            /* miss radiance for direction.y > 0 / <= 0 */
+  float sun_direction[3], sun_radiance[3]; /* directional light (direction of travel) */
+  uint32_t use_graph;                      /* 1 = replay a captured hipGraph of the frame */
+  uint32_t pad[3];
+} rdrt_pt_params;
+
+typedef struct rdrt_pt_stats {
+  uint32_t bounces;             /* entries below that are valid: max_bounces + 1 */
+  uint32_t extension_rays[17];  /* closest-hit rays traced at each bounce */
+  uint32_t shadow_rays[17];     /* shadow rays traced after each bounce */
+  float frame_ms;               /* HIP-event time of the last frame */
+} rdrt_pt_stats;
+
+/* Renders one frame: radiance_dev receives width*height RGB f32 (the mean
+ * of spp paths per pixel).  Scene committed and materials set first. */
+int32_t rdrt_pt_render(rdrt_ctx *ctx, const rdrt_pt_params *params, float *radiance_dev);
+/* Ray counts of the last frame (synchronizes). */
+int32_t rdrt_pt_get_stats(rdrt_ctx *ctx, rdrt_pt_stats *out);
+
+/* ---- device primitives of the path (parity helpers) ---- */
+/* Stable stream compaction (shader/parallel-compute/src/stream_compaction.rs:
+ * 3-45): out_idx_dev receives the indices i with mask_dev[i] != 0 in
+ * increasing order, count_dev (device u32) their number. */
+int32_t rdrt_compact(rdrt_ctx *ctx, const uint32_t *mask_dev, uint32_t n, uint32_t *out_idx_dev,
+                     uint32_t *count_dev);
+/* Scrambled Sobol samples (math/statistics/src/sampling/sobol.rs:40-68):
+ * out_dev[2i] = van_der_corput(index[i], scramble[2i]),
+ * out_dev[2i+1] = sobol(index[i], scramble[2i+1]).  Device pointers. */
+int32_t rdrt_sample_sobol(rdrt_ctx *ctx, const uint32_t *index_dev, const uint32_t *scramble_dev, uint32_t n,
+                          float *out_dev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -496,6 +496,8 @@ struct Geometry {
   std::vector<uint32_t> idx;
   bool indexed;
   uint32_t ntris, flags;
+  std::vector<float> normals;         // per vertex (path tracer), optional
+  std::vector<uint32_t> tri_material;  // per triangle (path tracer), optional
 };
 
 struct orc_scene {
@@ -524,6 +526,10 @@ struct orc_scene {
   std::vector<V3> vertices;
   uint32_t max_blas_depth = 0, max_tlas_depth = 0;
   bool built = false;
+  // path-tracer shading data
+  std::vector<V3> normals;            // parallel to vertices; zero = none
+  std::vector<uint32_t> tri_material;  // per global triangle
+  std::vector<float> materials;        // 8 floats each: albedo xyz, pad, emission xyz, pad
 };
 
 extern "C" {
@@ -581,6 +587,8 @@ int orc_build(orc_scene *s) {
   s->redirect.clear();
   s->indices.clear();
   s->vertices.clear();
+  s->normals.clear();
+  s->tri_material.clear();
   s->blas_meta.clear();
   s->geo_meta.clear();
   s->tlas_bvh_root.clear();
@@ -649,6 +657,10 @@ int orc_build(orc_scene *s) {
       for (size_t k : bvh.sorted) s->redirect.push_back(primitive_start + (uint32_t)k);
       for (size_t t = 0; t < ntri; t++)
         for (int k = 0; k < 3; k++) s->indices.push_back(vertex_start + tri_idx[3 * t + k]);
+      for (size_t v = 0; v < nv; v++)
+        s->normals.push_back(g.normals.size() == 3 * nv ? v3(g.normals[3 * v], g.normals[3 * v + 1], g.normals[3 * v + 2])
+                                                        : v3(0.0f, 0.0f, 0.0f));
+      for (size_t t = 0; t < ntri; t++) s->tri_material.push_back(t < g.tri_material.size() ? g.tri_material[t] : 0u);
       TriBvh tb;
       tb.primitive_start = primitive_start;
       tb.geometry_idx = (uint32_t)gi;
@@ -809,7 +821,8 @@ struct RayIn {
 };
 
 // traverse_cpu.rs:52-245
-bool traverse_one(const orc_scene *s, const RayIn &ray, orc_hit &hit, orc_hit_meta &hm, Counters &cnt) {
+bool traverse_one(const orc_scene *s, const RayIn &ray, orc_hit &hit, orc_hit_meta &hm, Counters &cnt,
+                  uint32_t *ext = nullptr) {
   bool found = false;
   uint32_t flags0 = ray.flags;
   float far_ = ray.tmax;  // RayRange::new(range.x, range.y, 1.)
@@ -872,6 +885,10 @@ bool traverse_one(const orc_scene *s, const RayIn &ray, orc_hit &hit, orc_hit_me
               hm.instance_id = s->inst_order[ti];
               hm.instance_custom_index = s->inst_custom[ti];
               hm.hit_kind = r[0] < 0.0f ? 0xFFu : 0xFEu;  // ty.rs:139-140
+              if (ext) {
+                ext[0] = gr;  // global geometry index
+                ext[1] = ti;  // instance index in forest order
+              }
               if (flags & F_ACCEPT_FIRST_HIT_AND_END_SEARCH) return true;
             }
           }
@@ -1027,6 +1044,257 @@ uint32_t orc_compact(const uint32_t *values, const uint32_t *mask, uint32_t n, u
     if (inclusive[i] != prev) out[prev] = values ? values[i] : i;
   }
   return acc;
+}
+
+}  // extern "C"
+
+// ===================================================================
+// Path tracer oracle: the estimator of the wavefront path tracer in FP64.
+// Visibility (closest hit and shadow rays) uses the reference FP32
+// traversal above on FP32 rays; sampling integers are bit-identical to the
+// device; all shading arithmetic is double precision.
+// Stage structure: scene/rendering/gpu-ray-tracing/src/feature/path_tracing/
+// ray_gen.rs:89-213 (ray generation), ray_hit.rs:6-104 (closest hit: NEE
+// shadow ray + BSDF sample), ray_miss.rs:136-158 (miss).
+// ===================================================================
+namespace {
+
+struct D3 {
+  double x, y, z;
+};
+inline D3 d3(double x, double y, double z) { return D3{x, y, z}; }
+inline D3 d3f(V3 v) { return D3{v.x, v.y, v.z}; }
+inline D3 operator+(D3 a, D3 b) { return d3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline D3 operator-(D3 a, D3 b) { return d3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline D3 operator*(D3 a, double s) { return d3(a.x * s, a.y * s, a.z * s); }
+inline D3 operator*(D3 a, D3 b) { return d3(a.x * b.x, a.y * b.y, a.z * b.z); }
+inline double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline D3 dcross(D3 a, D3 b) { return d3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+inline D3 dnormalize(D3 a) {
+  double m = ddot(a, a);
+  return m > 0.0 ? a * (1.0 / std::sqrt(m)) : a;
+}
+
+// scene/rendering/gpu-ray-tracing/src/sampler.rs:33-42
+uint32_t xxhash32(uint32_t x, uint32_t y, uint32_t z) {
+  const uint32_t p1 = 2246822519u, p2 = 3266489917u, p3 = 668265263u, p4 = 374761393u;
+  uint32_t h = z + p4 + x * p2;
+  h = p3 * ((h << 17) | (h >> 15));
+  h = h + y * p2;
+  h = p3 * ((h << 17) | (h >> 15));
+  h = p1 * (h ^ (h >> 15));
+  h = p2 * (h ^ (h >> 13));
+  return h ^ (h >> 16);
+}
+uint32_t scramble_of(uint32_t px, uint32_t py, uint32_t seed, uint32_t k) { return xxhash32(px, py, seed * 65536u + k); }
+
+// ray_util.rs:6-40, on FP32 values (it moves a float by integer ulps)
+float offset_axis(float p, float n) {
+  const float int_scale = 256.0f, float_scale = 1.0f / 65536.0f, origin = 1.0f / 32.0f;
+  int32_t of_i = (int32_t)(n * int_scale);
+  int32_t pi;
+  std::memcpy(&pi, &p, 4);
+  pi += p < 0.0f ? -of_i : of_i;
+  float pf;
+  std::memcpy(&pf, &pi, 4);
+  return std::fabs(p) < origin ? p + float_scale * n : pf;
+}
+
+// distribution_map.rs:10-28 + sampling.rs:72-86 + diffuse.rs:24-36, in FP64
+D3 sample_cosine(D3 n, double s0, double s1) {
+  const double PI = 3.14159265358979323846;
+  double ox = s0 * 2.0 - 1.0, oy = s1 * 2.0 - 1.0, x = 0.0, y = 0.0;
+  if (!(ox == 0.0 && oy == 0.0)) {
+    double r, theta;
+    if (std::fabs(ox) > std::fabs(oy)) {
+      r = ox;
+      theta = PI / 4.0 * (oy / ox);
+    } else {
+      r = oy;
+      theta = PI / 2.0 - PI / 4.0 * (ox / oy);
+    }
+    x = std::cos(theta) * r;
+    y = std::sin(theta) * r;
+  }
+  double z = std::sqrt(std::max(0.0, 1.0 - x * x - y * y));
+  double sign = n.z < 0.0 ? -1.0 : 1.0;
+  double a = -1.0 / (sign + n.z);
+  double b = n.x * n.y * a;
+  D3 t = dnormalize(d3(1.0 + sign * n.x * n.x * a, sign * b, -sign * n.x));
+  D3 bt = dnormalize(d3(b, sign + n.y * n.y * a, -n.y));
+  return t * x + bt * y + n * z;
+}
+
+// world normal = transpose(inverse 3x3) * n, column-major inverse
+D3 normal_to_world(const M4 &m, D3 n) {
+  return d3(m.a1 * n.x + m.a2 * n.y + m.a3 * n.z, m.b1 * n.x + m.b2 * n.y + m.b3 * n.z,
+            m.c1 * n.x + m.c2 * n.y + m.c3 * n.z);
+}
+
+struct PtCounts {
+  std::vector<uint64_t> ext, shadow;
+};
+
+// One path; returns its radiance.
+D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *ext_cnt, uint64_t *sh_cnt) {
+  const uint32_t pixel = path / P.spp;
+  const uint32_t n = P.sample_index_base + path % P.spp;
+  const uint32_t px = pixel % P.width, py = pixel / P.width;
+  double jx = orc_van_der_corput(n, scramble_of(px, py, P.seed, 0));
+  double jy = orc_sobol(n, scramble_of(px, py, P.seed, 1));
+  double u = (double)px / P.width + jx / P.width;
+  double v = (double)py / P.height + jy / P.height;
+  const float *m = P.view_projection_inv;
+  double nx = u * 2.0 - 1.0, ny = (v * 2.0 - 1.0) * -1.0;
+  double rx = nx * m[0] + ny * m[4] + m[8] + m[12];
+  double ry = nx * m[1] + ny * m[5] + m[9] + m[13];
+  double rz = nx * m[2] + ny * m[6] + m[10] + m[14];
+  double rw = nx * m[3] + ny * m[7] + m[11] + m[15];
+  D3 cam = d3(P.camera_position[0], P.camera_position[1], P.camera_position[2]);
+  D3 dir = dnormalize(d3(rx / rw, ry / rw, rz / rw) - cam);
+  V3 o32 = v3(P.camera_position[0], P.camera_position[1], P.camera_position[2]);
+  D3 beta = d3(1, 1, 1), L = d3(0, 0, 0);
+  const D3 sky_up = d3(P.sky_up[0], P.sky_up[1], P.sky_up[2]);
+  const D3 sky_down = d3(P.sky_down[0], P.sky_down[1], P.sky_down[2]);
+  const D3 wi = d3(-P.sun_direction[0], -P.sun_direction[1], -P.sun_direction[2]);
+  const D3 sun = d3(P.sun_radiance[0], P.sun_radiance[1], P.sun_radiance[2]);
+  const bool has_sun = P.sun_radiance[0] > 0.0f || P.sun_radiance[1] > 0.0f || P.sun_radiance[2] > 0.0f;
+  for (uint32_t b = 0; b <= P.max_bounces; b++) {
+    ext_cnt[b]++;
+    RayIn ray;
+    ray.o = o32;
+    ray.d = v3((float)dir.x, (float)dir.y, (float)dir.z);
+    ray.tmin = FLT_EPSILON;
+    ray.tmax = FLT_MAX;
+    ray.flags = F_CULL_BACK;  // ray_gen.rs:162-166
+    ray.mask = 0xffffffffu;
+    ray.tlas_idx = 0;
+    orc_hit h;
+    orc_hit_meta hm;
+    Counters cnt;
+    uint32_t ext[2] = {0, 0};
+    if (!traverse_one(s, ray, h, hm, cnt, ext)) {
+      L = L + beta * (ray.d.y > 0.0f ? sky_up : sky_down);
+      break;
+    }
+    const auto &geo = s->geo_meta[ext[0]];
+    const uint32_t ti = ext[1];
+    const uint32_t tri = geo.primitive_start + h.prim_id;
+    const uint32_t i0 = s->indices[3 * (size_t)tri], i1 = s->indices[3 * (size_t)tri + 1],
+                   i2 = s->indices[3 * (size_t)tri + 2];
+    const D3 p0 = d3f(s->vertices[i0]), p1 = d3f(s->vertices[i1]), p2 = d3f(s->vertices[i2]);
+    const M4 &inv = s->inst_transform_inv[ti];
+    D3 ng = dnormalize(normal_to_world(inv, dcross(p1 - p0, p2 - p0)));
+    D3 ns = ng;
+    const double bu = h.u, bv = h.v, bw = 1.0 - bu - bv;
+    D3 nobj = d3f(s->normals[i0]) * bw + d3f(s->normals[i1]) * bu + d3f(s->normals[i2]) * bv;
+    if (ddot(nobj, nobj) > 0.0) ns = dnormalize(normal_to_world(inv, nobj));
+    const D3 dd = d3f(ray.d);
+    if (ddot(ng, dd) > 0.0) ng = ng * -1.0;
+    if (ddot(ns, ng) < 0.0) ns = ns * -1.0;
+    const D3 pos = d3f(ray.o) + dd * (double)h.t;
+    uint32_t mi = s->inst_custom[ti] + s->tri_material[tri];
+    const size_t nm = s->materials.size() / 8;
+    if (mi >= nm) mi = (uint32_t)nm - 1;
+    const float *mat = &s->materials[8 * (size_t)mi];
+    const D3 albedo = d3(mat[0], mat[1], mat[2]);
+    L = L + beta * d3(mat[4], mat[5], mat[6]);
+    if (b == P.max_bounces) break;
+    // offset origin along the geometric normal, on FP32 values
+    const V3 ngf = v3((float)ng.x, (float)ng.y, (float)ng.z);
+    const V3 posf = v3((float)pos.x, (float)pos.y, (float)pos.z);
+    const V3 origin = v3(offset_axis(posf.x, ngf.x), offset_axis(posf.y, ngf.y), offset_axis(posf.z, ngf.z));
+    const double cos_l = ddot(ns, wi);
+    if (cos_l > 0.0 && ddot(ng, wi) > 0.0 && has_sun) {
+      sh_cnt[b]++;
+      RayIn sr;
+      sr.o = origin;
+      sr.d = v3(-P.sun_direction[0], -P.sun_direction[1], -P.sun_direction[2]);
+      sr.tmin = FLT_EPSILON;
+      sr.tmax = FLT_MAX;
+      sr.flags = F_ACCEPT_FIRST_HIT_AND_END_SEARCH;  // ray_hit.rs:33-45
+      sr.mask = 0xffffffffu;
+      sr.tlas_idx = 0;
+      orc_hit sh;
+      orc_hit_meta shm;
+      Counters sc;
+      if (!traverse_one(s, sr, sh, shm, sc)) L = L + beta * albedo * (cos_l / 3.14159265358979323846) * sun;
+    }
+    const double s0 = orc_van_der_corput(n, scramble_of(px, py, P.seed, 2 + 2 * b));
+    const double s1 = orc_sobol(n, scramble_of(px, py, P.seed, 3 + 2 * b));
+    const D3 wo = sample_cosine(ns, s0, s1);
+    if (!(ddot(wo, ng) > 0.0)) break;
+    beta = beta * albedo;
+    if (b >= P.rr_start) {
+      const double q = std::min(std::max(std::max(beta.x, beta.y), beta.z), 0.95);
+      const double ur = orc_van_der_corput(n, scramble_of(px, py, P.seed, 64 + b));
+      if (ur >= q) break;
+      beta = beta * (1.0 / q);
+    }
+    o32 = origin;
+    dir = wo;
+  }
+  return L;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_set_vertex_normals(orc_scene *s, uint32_t blas, uint32_t geo, const float *nrm, uint32_t nverts) {
+  if (blas >= s->blas_data.size() || geo >= s->blas_data[blas].size()) return -1;
+  Geometry &g = s->blas_data[blas][geo];
+  if (nverts != g.pos.size() / 3) return -1;
+  g.normals.assign(nrm, nrm + 3 * (size_t)nverts);
+  s->built = false;
+  return 0;
+}
+
+int orc_set_triangle_materials(orc_scene *s, uint32_t blas, uint32_t geo, const uint32_t *off, uint32_t ntris) {
+  if (blas >= s->blas_data.size() || geo >= s->blas_data[blas].size()) return -1;
+  Geometry &g = s->blas_data[blas][geo];
+  g.tri_material.assign(off, off + ntris);
+  s->built = false;
+  return 0;
+}
+
+void orc_set_materials(orc_scene *s, const float *mats, uint32_t n) { s->materials.assign(mats, mats + 8 * (size_t)n); }
+
+int orc_pt_render(const orc_scene *s, const orc_pt_params *P, double *out, uint64_t *ray_counts, int nthreads) {
+  if (!s->built || s->materials.empty() || P->spp == 0 || P->max_bounces > 16) return -1;
+  const uint32_t npix = P->width * P->height;
+  const uint32_t nb = P->max_bounces + 1;
+  std::vector<uint64_t> ext_all(nb, 0), sh_all(nb, 0);
+#ifdef _OPENMP
+  int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+#endif
+  {
+    std::vector<uint64_t> ext(nb, 0), sh(nb, 0);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+    for (int64_t pix = 0; pix < (int64_t)npix; pix++) {
+      D3 acc = d3(0, 0, 0);
+      for (uint32_t k = 0; k < P->spp; k++) acc = acc + pt_path(s, *P, (uint32_t)pix * P->spp + k, ext.data(), sh.data());
+      out[3 * pix + 0] = acc.x / P->spp;
+      out[3 * pix + 1] = acc.y / P->spp;
+      out[3 * pix + 2] = acc.z / P->spp;
+    }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+    for (uint32_t b = 0; b < nb; b++) {
+      ext_all[b] += ext[b];
+      sh_all[b] += sh[b];
+    }
+  }
+  if (ray_counts)
+    for (uint32_t b = 0; b < nb; b++) {
+      ray_counts[b] = ext_all[b];
+      ray_counts[nb + b] = sh_all[b];
+    }
+  return 0;
 }
 
 }  // extern "C"

@@ -148,6 +148,31 @@ float orc_sobol(uint32_t n, uint32_t scramble);
 uint32_t orc_compact(const uint32_t *values, const uint32_t *mask, uint32_t n,
                      uint32_t *out);
 
+/* ---- path tracer (FP64 shading, FP32 reference visibility) ---- */
+/* Same layout as the product's rdrt_pt_params. */
+typedef struct orc_pt_params {
+  float view_projection_inv[16];
+  float camera_position[3];
+  uint32_t width, height, spp;
+  uint32_t max_bounces, rr_start, seed, sample_index_base;
+  float sky_up[3], sky_down[3];
+  float sun_direction[3], sun_radiance[3];
+  uint32_t use_graph;
+  uint32_t pad[3];
+} orc_pt_params;
+
+int orc_set_vertex_normals(orc_scene *s, uint32_t blas, uint32_t geometry, const float *normals_xyz,
+                           uint32_t nverts);
+int orc_set_triangle_materials(orc_scene *s, uint32_t blas, uint32_t geometry, const uint32_t *offsets,
+                               uint32_t ntris);
+/* 8 floats per material: albedo xyz, pad, emission xyz, pad */
+void orc_set_materials(orc_scene *s, const float *materials, uint32_t n);
+/* Renders width*height RGB doubles (mean over spp).  ray_counts, if not
+ * NULL, receives 2*(max_bounces+1) u64: extension rays per bounce, then
+ * shadow rays per bounce. */
+int orc_pt_render(const orc_scene *s, const orc_pt_params *params, double *out_rgb, uint64_t *ray_counts,
+                  int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,6 +36,13 @@ class OrcInstance(C.Structure):
                 ("flags", u32), ("blas", u32), ("pad", u32 * 3)]
 
 
+class OrcPtParams(C.Structure):
+    _fields_ = [("view_projection_inv", f32 * 16), ("camera_position", f32 * 3), ("width", u32), ("height", u32),
+                ("spp", u32), ("max_bounces", u32), ("rr_start", u32), ("seed", u32), ("sample_index_base", u32),
+                ("sky_up", f32 * 3), ("sky_down", f32 * 3), ("sun_direction", f32 * 3), ("sun_radiance", f32 * 3),
+                ("use_graph", u32), ("pad", u32 * 3)]
+
+
 _SIG = {
     "orc_scene_new": (P, [C.POINTER(OrcBuildOpts)]),
     "orc_scene_free": (None, [P]),
@@ -62,6 +69,10 @@ _SIG = {
     "orc_van_der_corput": (f32, [u32, u32]),
     "orc_sobol": (f32, [u32, u32]),
     "orc_compact": (u32, [P, P, u32, P]),
+    "orc_set_vertex_normals": (C.c_int, [P, u32, u32, P, u32]),
+    "orc_set_triangle_materials": (C.c_int, [P, u32, u32, P, u32]),
+    "orc_set_materials": (None, [P, P, u32]),
+    "orc_pt_render": (C.c_int, [P, C.POINTER(OrcPtParams), P, P, C.c_int]),
 }
 
 _lib = None
@@ -197,6 +208,37 @@ class OracleScene:
         if rc != 0:
             raise RuntimeError("oracle trace failed (scene not built)")
         return hits, hm, cnt
+
+    # ---- path tracer ----
+    def set_vertex_normals(self, blas: int, geometry: int, normals) -> None:
+        n = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+        if self.L.orc_set_vertex_normals(self.h, blas, geometry, n.ctypes.data, n.shape[0]) != 0:
+            raise ValueError("bad normals")
+
+    def set_triangle_materials(self, blas: int, geometry: int, offsets) -> None:
+        o = np.ascontiguousarray(offsets, dtype=np.uint32).reshape(-1)
+        if self.L.orc_set_triangle_materials(self.h, blas, geometry, o.ctypes.data, o.size) != 0:
+            raise ValueError("bad material offsets")
+
+    def set_materials(self, albedo, emission) -> None:
+        a = np.asarray(albedo, dtype=np.float32).reshape(-1, 3)
+        e = np.asarray(emission, dtype=np.float32).reshape(-1, 3)
+        m = np.zeros((a.shape[0], 8), dtype=np.float32)
+        m[:, 0:3] = a
+        m[:, 4:7] = e
+        self._mats = m
+        self.L.orc_set_materials(self.h, m.ctypes.data, m.shape[0])
+
+    def pt_render(self, params: "OrcPtParams", threads: int = 0):
+        """Returns (height, width, 3) float64 radiance and the per-bounce
+        (extension, shadow) ray counts."""
+        out = np.zeros((params.height, params.width, 3), dtype=np.float64)
+        nb = params.max_bounces + 1
+        counts = np.zeros(2 * nb, dtype=np.uint64)
+        rc = self.L.orc_pt_render(self.h, C.byref(params), out.ctypes.data, counts.ctypes.data, threads)
+        if rc != 0:
+            raise RuntimeError("oracle pt_render failed")
+        return out, counts[:nb], counts[nb:]
 
     def trace_brute(self, rays: np.ndarray, flags=0, cull_mask=0xFFFFFFFF, tlas_idx=0, threads: int = 0):
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)

@@ -62,6 +62,21 @@ class RdrtSceneInfo(C.Structure):
                 ("device_bytes", C.c_uint64), ("build_ms", C.c_double)]
 
 
+class RdrtMaterial(C.Structure):
+    _fields_ = [("albedo", f32 * 3), ("pad0", f32), ("emission", f32 * 3), ("pad1", f32)]
+
+
+class RdrtPtParams(C.Structure):
+    _fields_ = [("view_projection_inv", f32 * 16), ("camera_position", f32 * 3), ("width", u32), ("height", u32),
+                ("spp", u32), ("max_bounces", u32), ("rr_start", u32), ("seed", u32), ("sample_index_base", u32),
+                ("sky_up", f32 * 3), ("sky_down", f32 * 3), ("sun_direction", f32 * 3), ("sun_radiance", f32 * 3),
+                ("use_graph", u32), ("pad", u32 * 3)]
+
+
+class RdrtPtStats(C.Structure):
+    _fields_ = [("bounces", u32), ("extension_rays", u32 * 17), ("shadow_rays", u32 * 17), ("frame_ms", f32)]
+
+
 # name -> (restype, argtypes); the full exported surface of rdrt.h
 SIGNATURES = {
     "rdrt_last_error": (C.c_char_p, []),
@@ -85,6 +100,13 @@ SIGNATURES = {
     "rdrt_trace": (i32, [C.c_void_p, C.POINTER(RdrtTraceArgs)]),
     "rdrt_trace_closest": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p]),
     "rdrt_last_trace_ms": (i32, [C.c_void_p, C.POINTER(f32)]),
+    "rdrt_blas_set_vertex_normals": (i32, [C.c_void_p, u32, u32, C.c_void_p, u32]),
+    "rdrt_blas_set_triangle_materials": (i32, [C.c_void_p, u32, u32, C.c_void_p, u32]),
+    "rdrt_set_materials": (i32, [C.c_void_p, C.POINTER(RdrtMaterial), u32]),
+    "rdrt_pt_render": (i32, [C.c_void_p, C.POINTER(RdrtPtParams), C.c_void_p]),
+    "rdrt_pt_get_stats": (i32, [C.c_void_p, C.POINTER(RdrtPtStats)]),
+    "rdrt_compact": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p]),
+    "rdrt_sample_sobol": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, u32, C.c_void_p]),
 }
 
 RDRT_OK = 0

@@ -235,6 +235,36 @@ class AccelerationStructureSystem:
         out = out[:n]
         return out.reshape(-1, 12) if which.endswith("forest") else out
 
+    # ---- shading attributes ----
+    def set_vertex_normals(self, blas: BlasHandle, geometry: int, normals: np.ndarray) -> None:
+        n = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+        L.check(self._L.rdrt_blas_set_vertex_normals(self._h, blas.index, geometry, n.ctypes.data, n.shape[0]))
+
+    def set_triangle_materials(self, blas: BlasHandle, geometry: int, offsets: np.ndarray) -> None:
+        o = np.ascontiguousarray(offsets, dtype=np.uint32).reshape(-1)
+        L.check(self._L.rdrt_blas_set_triangle_materials(self._h, blas.index, geometry, o.ctypes.data, o.size))
+
+    def set_materials(self, albedo: np.ndarray, emission: np.ndarray) -> None:
+        """Lambertian materials: albedo (n, 3), emission (n, 3)."""
+        a = np.asarray(albedo, dtype=np.float32).reshape(-1, 3)
+        e = np.asarray(emission, dtype=np.float32).reshape(-1, 3)
+        arr = (L.RdrtMaterial * max(1, a.shape[0]))()
+        for i in range(a.shape[0]):
+            arr[i].albedo[:] = [float(x) for x in a[i]]
+            arr[i].emission[:] = [float(x) for x in e[i]]
+        L.check(self._L.rdrt_set_materials(self._h, arr, a.shape[0]))
+
+    # ---- device primitives ----
+    def compact(self, mask, out_idx, count) -> None:
+        """Stable compaction of the indices where ``mask`` (device u32) is
+        non-zero into ``out_idx``; ``count`` (device, 1 word) gets their number."""
+        n = mask.numel()
+        L.check(self._L.rdrt_compact(self._h, _ptr(mask), n, _ptr(out_idx), _ptr(count)))
+
+    def sample_sobol(self, index, scramble, out) -> None:
+        """out[2i] = van_der_corput(index[i], scramble[2i]); out[2i+1] = sobol(index[i], scramble[2i+1])."""
+        L.check(self._L.rdrt_sample_sobol(self._h, _ptr(index), _ptr(scramble), index.numel(), _ptr(out)))
+
     # ---- tracing ----
     def trace(self, rays, hits, *, ray_meta=None, hit_meta=None, counters=None, flags: int = 0,
               cull_mask: int = 0xFFFFFFFF, tlas_idx: int = 0, traversal: int = TRAVERSAL_FAST) -> None:

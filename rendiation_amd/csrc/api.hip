@@ -13,86 +13,28 @@
 #include <string>
 #include <vector>
 
-#include "../../include/rdrt.h"
-#include "scene.h"
-#include "trace_kernels.h"
+#include "context.h"
 
 using namespace rdrt;
 
 namespace {
-
 thread_local std::string g_last_error;
+
+TreeOptions tree_opt(uint32_t bin, uint32_t depth, uint32_t bins, uint32_t def_bin) {
+  return TreeOptions{bin ? bin : def_bin, depth ? depth : 50u, bins ? bins : 4u};
+}
+}  // namespace
+
+namespace rdrt {
 
 int32_t fail(int32_t code, const std::string &msg) {
   g_last_error = msg;
   return code;
 }
 
-#define RDRT_HIP(expr)                                                                                  \
-  do {                                                                                                  \
-    hipError_t e_ = (expr);                                                                             \
-    if (e_ != hipSuccess) return fail(RDRT_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
-  } while (0)
-
-template <class T>
-struct DevArray {
-  T *ptr = nullptr;
-  size_t count = 0;
-};
-
-struct DeviceScene {
-  DevArray<PairNode> tri_pairs, tlas_pairs;
-  DevArray<TriPre> tris;
-  DevArray<uint32_t> redirect, indices, binding;
-  DevArray<float> vertices;
-  DevArray<GeoRec> geos;
-  DevArray<InstRec> insts;
-  DevArray<TlasRoot> tlas_roots;
-  DevArray<RefNode> tri_forest, tlas_forest;
-  uint64_t bytes = 0;
-};
-
-}  // namespace
-
-struct rdrt_ctx {
-  rdrt_config cfg{};
-  bool has_device = false;
-  hipStream_t stream = nullptr;
-  SceneSrc src;
-  std::vector<GeometrySrc> pending;
-  bool in_blas = false;
-  BuiltScene built;
-  bool built_valid = false;
-  bool uploaded = false;
-  DeviceScene dev;
-  uint32_t *status = nullptr;  // device status word of the trace kernels
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  bool timed = false;
-};
-
-namespace {
-
 void invalidate(rdrt_ctx *c) {
   c->built_valid = false;
   c->uploaded = false;
-}
-
-template <class T>
-hipError_t upload(DevArray<T> &a, const std::vector<T> &h, uint64_t &bytes) {
-  a.count = h.size();
-  size_t n = h.empty() ? 1 : h.size();  // keep a valid pointer for empty arrays
-  hipError_t e = hipMalloc(&a.ptr, n * sizeof(T));
-  if (e != hipSuccess) return e;
-  bytes += n * sizeof(T);
-  if (!h.empty()) e = hipMemcpy(a.ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
-  return e;
-}
-
-template <class T>
-void release(DevArray<T> &a) {
-  if (a.ptr) (void)hipFree(a.ptr);
-  a.ptr = nullptr;
-  a.count = 0;
 }
 
 void free_device_scene(DeviceScene &d) {
@@ -102,7 +44,9 @@ void free_device_scene(DeviceScene &d) {
   release(d.redirect);
   release(d.indices);
   release(d.binding);
+  release(d.tri_material);
   release(d.vertices);
+  release(d.normals);
   release(d.geos);
   release(d.insts);
   release(d.tlas_roots);
@@ -111,17 +55,36 @@ void free_device_scene(DeviceScene &d) {
   d.bytes = 0;
 }
 
-TreeOptions tree_opt(uint32_t bin, uint32_t depth, uint32_t bins, uint32_t def_bin) {
-  return TreeOptions{bin ? bin : def_bin, depth ? depth : 50u, bins ? bins : 4u};
-}
-
 int32_t need_device(rdrt_ctx *c) {
   if (!c->has_device) return fail(RDRT_ERR_NO_DEVICE, "context was created without a device (device = -1)");
   RDRT_HIP(hipSetDevice(c->cfg.device));
   return RDRT_OK;
 }
 
-}  // namespace
+TraceParams scene_trace_params(const rdrt_ctx *c) {
+  const DeviceScene &d = c->dev;
+  TraceParams p{};
+  p.status = c->status;
+  p.tri_pairs = d.tri_pairs.ptr;
+  p.tris = d.tris.ptr;
+  p.redirect = d.redirect.ptr;
+  p.geos = d.geos.ptr;
+  p.tlas_pairs = d.tlas_pairs.ptr;
+  p.insts = d.insts.ptr;
+  p.tlas_roots = d.tlas_roots.ptr;
+  p.binding = d.binding.ptr;
+  p.nbinding = (uint32_t)d.binding.count;
+  p.ntlas = (uint32_t)d.tlas_roots.count;
+  p.tri_forest = d.tri_forest.ptr;
+  p.tlas_forest = d.tlas_forest.ptr;
+  p.indices = d.indices.ptr;
+  p.vertices = d.vertices.ptr;
+  return p;
+}
+
+uint32_t push_depth(const rdrt_ctx *c) { return c->built.blas_stack + c->built.tlas_stack; }
+
+}  // namespace rdrt
 
 extern "C" {
 
@@ -166,6 +129,16 @@ int32_t rdrt_ctx_create(const rdrt_config *cfg, rdrt_ctx **out) {
       delete c;
       return fail(RDRT_ERR_HIP, "device set-up failed");
     }
+    // Graph capture needs a real stream: the user's, or a blocking stream of
+    // our own (ordered with the null stream) at the default priority.
+    c->work_stream = c->stream;
+    if (!c->work_stream) {
+      if (hipStreamCreate(&c->work_stream) != hipSuccess) {
+        delete c;
+        return fail(RDRT_ERR_HIP, "stream creation failed");
+      }
+      c->owns_work_stream = true;
+    }
   }
   *out = c;
   return RDRT_OK;
@@ -176,7 +149,11 @@ int32_t rdrt_ctx_destroy(rdrt_ctx *c) {
   if (c->has_device) {
     (void)hipSetDevice(c->cfg.device);
     (void)hipStreamSynchronize(c->stream);
+    if (c->work_stream) (void)hipStreamSynchronize(c->work_stream);
     free_device_scene(c->dev);
+    c->pt.free_all();
+    c->compact_scratch.free();
+    if (c->owns_work_stream) (void)hipStreamDestroy(c->work_stream);
     if (c->status) (void)hipFree(c->status);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -190,11 +167,13 @@ int32_t rdrt_sync(rdrt_ctx *c) {
   int32_t rc = need_device(c);
   if (rc) return rc;
   RDRT_HIP(hipStreamSynchronize(c->stream));
+  if (c->work_stream != c->stream) RDRT_HIP(hipStreamSynchronize(c->work_stream));
   uint32_t st = 0;
   RDRT_HIP(hipMemcpy(&st, c->status, sizeof(st), hipMemcpyDeviceToHost));
   if (st) {
     RDRT_HIP(hipMemset(c->status, 0, sizeof(uint32_t)));
-    return fail(RDRT_ERR_CAPACITY, "a traversal stack overflowed; results of the last launch are incomplete");
+    return fail(RDRT_ERR_CAPACITY, (st & 1u) ? "a traversal stack overflowed; results of the last launch are incomplete"
+                                             : "a queue compaction did not complete; results of the last launch are invalid");
   }
   return RDRT_OK;
 }
@@ -315,6 +294,8 @@ int32_t rdrt_commit(rdrt_ctx *c) {
   RDRT_HIP(upload(d.tlas_roots, b.tlas_roots, d.bytes));
   RDRT_HIP(upload(d.tri_forest, b.tri_forest, d.bytes));
   RDRT_HIP(upload(d.tlas_forest, b.tlas_forest, d.bytes));
+  RDRT_HIP(upload(d.normals, b.normals, d.bytes));
+  RDRT_HIP(upload(d.tri_material, b.tri_material, d.bytes));
   c->uploaded = true;
   return RDRT_OK;
 }
@@ -380,33 +361,17 @@ int32_t rdrt_trace(rdrt_ctx *c, const rdrt_trace_args *a) {
   if (!c->uploaded) return fail(RDRT_ERR_NOT_BUILT, "scene is not committed (call rdrt_commit)");
   if (a->n && (!a->rays || !a->hits)) return fail(RDRT_ERR_INVALID_ARG, "null rays or hits");
   if (a->traversal > RDRT_TRAVERSAL_REFERENCE) return fail(RDRT_ERR_INVALID_ARG, "unknown traversal");
-  const DeviceScene &d = c->dev;
-  TraceParams p;
+  TraceParams p = scene_trace_params(c);
   p.rays = a->rays;
   p.meta = a->ray_meta;
   p.hits = a->hits;
   p.hit_meta = a->hit_meta;
   p.counters = a->counters;
-  p.status = c->status;
   p.n = a->n;
   p.flags = a->flags;
   p.cull_mask = a->cull_mask;
   p.tlas_idx = a->tlas_idx;
-  p.tri_pairs = d.tri_pairs.ptr;
-  p.tris = d.tris.ptr;
-  p.redirect = d.redirect.ptr;
-  p.geos = d.geos.ptr;
-  p.tlas_pairs = d.tlas_pairs.ptr;
-  p.insts = d.insts.ptr;
-  p.tlas_roots = d.tlas_roots.ptr;
-  p.binding = d.binding.ptr;
-  p.nbinding = (uint32_t)d.binding.count;
-  p.ntlas = (uint32_t)d.tlas_roots.count;
-  p.tri_forest = d.tri_forest.ptr;
-  p.tlas_forest = d.tlas_forest.ptr;
-  p.indices = d.indices.ptr;
-  p.vertices = d.vertices.ptr;
-  uint32_t push = c->built.blas_stack + c->built.tlas_stack;
+  uint32_t push = push_depth(c);
   if (a->traversal == RDRT_TRAVERSAL_FAST && stack_entries_for(push) == 0)
     return fail(RDRT_ERR_CAPACITY, "BVH too deep for the fast traversal stack (use RDRT_TRAVERSAL_REFERENCE)");
   uint32_t block = c->cfg.block_size ? c->cfg.block_size : 256;

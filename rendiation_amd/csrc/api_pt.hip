@@ -1,0 +1,247 @@
+// api_pt.hip -- C-ABI of the path tracer, shading attributes and the device
+// primitives (compaction, Sobol) of include/rdrt.h.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "context.h"
+
+using namespace rdrt;
+
+namespace rdrt {
+
+void PtState::free_all() {
+  for (Scratch *s : {&rays, &shadow_rays, &hits, &shadow_hits, &hitx, &beta, &L, &nee, &qa, &qb, &qs, &counts,
+                     &lookback, &tickets, &materials})
+    s->free();
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+  have_cached = false;
+  capacity = 0;
+}
+
+}  // namespace rdrt
+
+namespace {
+
+int32_t reserve_pt(rdrt_ctx *c, uint32_t npaths) {
+  PtState &s = c->pt;
+  if (npaths <= s.capacity) return RDRT_OK;
+  size_t n = npaths;
+  RDRT_HIP(s.rays.reserve(n * sizeof(rdrt_ray)));
+  RDRT_HIP(s.shadow_rays.reserve(n * sizeof(rdrt_ray)));
+  RDRT_HIP(s.hits.reserve(n * sizeof(rdrt_hit)));
+  RDRT_HIP(s.shadow_hits.reserve(n * sizeof(rdrt_hit)));
+  RDRT_HIP(s.hitx.reserve(n * sizeof(uint2)));
+  RDRT_HIP(s.beta.reserve(n * 12));
+  RDRT_HIP(s.L.reserve(n * 12));
+  RDRT_HIP(s.nee.reserve(n * 12));
+  RDRT_HIP(s.qa.reserve(n * 4));
+  RDRT_HIP(s.qb.reserve(n * 4));
+  RDRT_HIP(s.qs.reserve(n * 4));
+  size_t stride = (n + kPtBlock - 1) / kPtBlock;
+  RDRT_HIP(s.counts.reserve(2 * (kPtMaxBounces + 1) * 4));
+  RDRT_HIP(s.lookback.reserve((kPtMaxBounces + 1) * stride * 8));
+  RDRT_HIP(s.tickets.reserve((kPtMaxBounces + 1) * 4));
+  s.capacity = npaths;
+  s.have_cached = false;
+  return RDRT_OK;
+}
+
+GeometrySrc *geometry_of(rdrt_ctx *c, uint32_t blas, uint32_t geometry) {
+  if (blas >= c->src.blas.size() || !c->src.blas_live[blas] || geometry >= c->src.blas[blas].size()) return nullptr;
+  return &c->src.blas[blas][geometry];
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t rdrt_blas_set_vertex_normals(rdrt_ctx *c, uint32_t blas, uint32_t geometry, const float *normals,
+                                     uint32_t nverts) {
+  if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
+  GeometrySrc *g = geometry_of(c, blas, geometry);
+  if (!g) return fail(RDRT_ERR_INVALID_ARG, "unknown BLAS or geometry");
+  if (nverts != g->pos.size() / 3 || (nverts && !normals))
+    return fail(RDRT_ERR_INVALID_ARG, "normals must give one xyz triple per vertex");
+  g->normals.assign(normals, normals + 3 * (size_t)nverts);
+  invalidate(c);
+  return RDRT_OK;
+}
+
+int32_t rdrt_blas_set_triangle_materials(rdrt_ctx *c, uint32_t blas, uint32_t geometry, const uint32_t *offsets,
+                                         uint32_t ntris) {
+  if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
+  GeometrySrc *g = geometry_of(c, blas, geometry);
+  if (!g) return fail(RDRT_ERR_INVALID_ARG, "unknown BLAS or geometry");
+  uint32_t nt = g->indexed ? (uint32_t)(g->idx.size() / 3) : (uint32_t)(g->pos.size() / 9);
+  if (ntris != nt || (ntris && !offsets)) return fail(RDRT_ERR_INVALID_ARG, "one material offset per triangle");
+  g->tri_material.assign(offsets, offsets + ntris);
+  invalidate(c);
+  return RDRT_OK;
+}
+
+int32_t rdrt_set_materials(rdrt_ctx *c, const rdrt_material *m, uint32_t n) {
+  if (!c || (n && !m)) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  c->materials.resize(n);
+  for (uint32_t i = 0; i < n; i++) {
+    Material &d = c->materials[i];
+    std::memset(&d, 0, sizeof(d));
+    for (int k = 0; k < 3; k++) {
+      d.albedo[k] = m[i].albedo[k];
+      d.emission[k] = m[i].emission[k];
+    }
+  }
+  c->pt.materials_dirty = true;
+  return RDRT_OK;
+}
+
+int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) {
+  if (!c || !prm || !radiance) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  if (!c->uploaded) return fail(RDRT_ERR_NOT_BUILT, "scene is not committed (call rdrt_commit)");
+  if (prm->width == 0 || prm->height == 0 || prm->spp == 0) return fail(RDRT_ERR_INVALID_ARG, "empty frame");
+  if (prm->max_bounces > kPtMaxBounces) return fail(RDRT_ERR_INVALID_ARG, "max_bounces above 16");
+  uint64_t npaths64 = (uint64_t)prm->width * prm->height * prm->spp;
+  if (npaths64 > (1ull << 30)) return fail(RDRT_ERR_CAPACITY, "more than 2^30 paths per frame");
+  if (c->materials.empty()) return fail(RDRT_ERR_STATE, "no materials set (rdrt_set_materials)");
+  // every instance must address a material: custom index + largest triangle offset
+  const BuiltScene &b = c->built;
+  for (const InstRec &in : b.insts) {
+    uint32_t mx = 0;
+    for (uint32_t g = in.geo_begin; g < in.geo_end; g++) mx = std::max(mx, b.geo_max_material[g]);
+    if ((uint64_t)in.custom_index + mx >= c->materials.size())
+      return fail(RDRT_ERR_INVALID_ARG, "an instance addresses a material beyond rdrt_set_materials");
+  }
+  if (stack_entries_for(push_depth(c)) == 0) return fail(RDRT_ERR_CAPACITY, "BVH too deep for the fast traversal");
+  const uint32_t npaths = (uint32_t)npaths64;
+  rc = reserve_pt(c, npaths);
+  if (rc) return rc;
+  PtState &s = c->pt;
+  if (s.materials_dirty) {
+    RDRT_HIP(s.materials.reserve(c->materials.size() * sizeof(Material)));
+    RDRT_HIP(hipMemcpy(s.materials.ptr, c->materials.data(), c->materials.size() * sizeof(Material),
+                       hipMemcpyHostToDevice));
+    s.nmaterials_uploaded = (uint32_t)c->materials.size();
+    s.materials_dirty = false;
+  }
+
+  PtLaunch L;
+  std::memset(&L, 0, sizeof(L));
+  std::memcpy(L.vp_inv, prm->view_projection_inv, sizeof(L.vp_inv));
+  std::memcpy(L.cam, prm->camera_position, sizeof(L.cam));
+  L.width = prm->width;
+  L.height = prm->height;
+  L.spp = prm->spp;
+  L.npaths = npaths;
+  L.max_bounces = prm->max_bounces;
+  L.rr_start = prm->rr_start;
+  L.seed = prm->seed;
+  L.sample_base = prm->sample_index_base;
+  std::memcpy(L.sky_up, prm->sky_up, 12);
+  std::memcpy(L.sky_down, prm->sky_down, 12);
+  std::memcpy(L.sun_dir, prm->sun_direction, 12);
+  std::memcpy(L.sun_radiance, prm->sun_radiance, 12);
+  const DeviceScene &d = c->dev;
+  L.insts = d.insts.ptr;
+  L.geos = d.geos.ptr;
+  L.indices = d.indices.ptr;
+  L.vertices = d.vertices.ptr;
+  L.normals = d.normals.count ? d.normals.ptr : nullptr;
+  L.tri_material = d.tri_material.count ? d.tri_material.ptr : nullptr;
+  L.materials = (const Material *)s.materials.ptr;
+  L.nmaterials = s.nmaterials_uploaded;
+  L.rays = (rdrt_ray *)s.rays.ptr;
+  L.shadow_rays = (rdrt_ray *)s.shadow_rays.ptr;
+  L.hits = (rdrt_hit *)s.hits.ptr;
+  L.shadow_hits = (rdrt_hit *)s.shadow_hits.ptr;
+  L.hitx = (uint2 *)s.hitx.ptr;
+  L.beta = (float *)s.beta.ptr;
+  L.L = (float *)s.L.ptr;
+  L.nee = (float *)s.nee.ptr;
+  L.qa = (uint32_t *)s.qa.ptr;
+  L.qb = (uint32_t *)s.qb.ptr;
+  L.qs = (uint32_t *)s.qs.ptr;
+  L.counts = (uint32_t *)s.counts.ptr;
+  L.lookback = (unsigned long long *)s.lookback.ptr;
+  L.tickets = (uint32_t *)s.tickets.ptr;
+  L.lookback_stride = (npaths + kPtBlock - 1) / kPtBlock;
+  L.status = c->status;
+  L.out = radiance;
+
+  const TraceParams tp = scene_trace_params(c);
+  const uint32_t block = c->cfg.block_size ? c->cfg.block_size : 256;
+  hipStream_t st = c->work_stream;
+  RDRT_HIP(hipEventRecord(c->ev0, st));
+  if (prm->use_graph) {
+    if (!s.exec || !s.have_cached || std::memcmp(&s.cached, &L, sizeof(L)) != 0) {
+      if (s.exec) (void)hipGraphExecDestroy(s.exec);
+      if (s.graph) (void)hipGraphDestroy(s.graph);
+      s.exec = nullptr;
+      s.graph = nullptr;
+      RDRT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      hipError_t e = enqueue_pt_frame(L, tp, push_depth(c), block, st);
+      hipGraph_t g = nullptr;
+      hipError_t e2 = hipStreamEndCapture(st, &g);
+      if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(RDRT_ERR_HIP, std::string("frame capture failed: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+      }
+      s.graph = g;
+      RDRT_HIP(hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0));
+      s.cached = L;
+      s.have_cached = true;
+    }
+    RDRT_HIP(hipGraphLaunch(s.exec, st));
+  } else {
+    RDRT_HIP(enqueue_pt_frame(L, tp, push_depth(c), block, st));
+  }
+  RDRT_HIP(hipEventRecord(c->ev1, st));
+  s.last_bounces = prm->max_bounces;
+  return RDRT_OK;
+}
+
+int32_t rdrt_pt_get_stats(rdrt_ctx *c, rdrt_pt_stats *o) {
+  if (!c || !o) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  if (!c->pt.counts.ptr) return fail(RDRT_ERR_STATE, "no frame rendered");
+  RDRT_HIP(hipStreamSynchronize(c->work_stream));
+  std::memset(o, 0, sizeof(*o));
+  uint32_t mb = c->pt.last_bounces;
+  uint32_t counts[2 * (kPtMaxBounces + 1)];
+  RDRT_HIP(hipMemcpy(counts, c->pt.counts.ptr, 2 * (mb + 1) * 4, hipMemcpyDeviceToHost));
+  o->bounces = mb + 1;
+  for (uint32_t b = 0; b <= mb; b++) {
+    o->extension_rays[b] = counts[b];
+    o->shadow_rays[b] = b < mb ? counts[mb + 1 + b] : 0u;
+  }
+  RDRT_HIP(hipEventElapsedTime(&o->frame_ms, c->ev0, c->ev1));
+  return RDRT_OK;
+}
+
+int32_t rdrt_compact(rdrt_ctx *c, const uint32_t *mask, uint32_t n, uint32_t *out_idx, uint32_t *count) {
+  if (!c || !count || (n && (!mask || !out_idx))) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  if (n > (1u << 31) - 1) return fail(RDRT_ERR_CAPACITY, "compaction supports fewer than 2^31 elements");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  size_t nb = ((size_t)n + kPtBlock - 1) / kPtBlock;
+  RDRT_HIP(c->compact_scratch.reserve((nb + 2) * 8));
+  RDRT_HIP(enqueue_compact(mask, n, out_idx, count, (unsigned long long *)c->compact_scratch.ptr, c->stream));
+  return RDRT_OK;
+}
+
+int32_t rdrt_sample_sobol(rdrt_ctx *c, const uint32_t *index, const uint32_t *scr, uint32_t n, float *out) {
+  if (!c || (n && (!index || !scr || !out))) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  RDRT_HIP(enqueue_sobol(index, scr, n, out, c->stream));
+  return RDRT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,133 @@
+// context.h -- the rdrt_ctx behind the C-ABI, shared by the API sources.
+#ifndef RDRT_CONTEXT_H
+#define RDRT_CONTEXT_H
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/rdrt.h"
+#include "pt.h"
+#include "scene.h"
+#include "trace_kernels.h"
+
+namespace rdrt {
+
+int32_t fail(int32_t code, const std::string &msg);
+
+#define RDRT_HIP(expr)                                                                                        \
+  do {                                                                                                        \
+    hipError_t e_ = (expr);                                                                                   \
+    if (e_ != hipSuccess) return ::rdrt::fail(RDRT_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+struct DevArray {
+  T *ptr = nullptr;
+  size_t count = 0;
+};
+
+template <class T>
+hipError_t upload(DevArray<T> &a, const std::vector<T> &h, uint64_t &bytes) {
+  a.count = h.size();
+  size_t n = h.empty() ? 1 : h.size();  // keep a valid pointer for empty arrays
+  hipError_t e = hipMalloc(&a.ptr, n * sizeof(T));
+  if (e != hipSuccess) return e;
+  bytes += n * sizeof(T);
+  if (!h.empty()) e = hipMemcpy(a.ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+  return e;
+}
+
+template <class T>
+void release(DevArray<T> &a) {
+  if (a.ptr) (void)hipFree(a.ptr);
+  a.ptr = nullptr;
+  a.count = 0;
+}
+
+// Grow-only device buffer.
+struct Scratch {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  hipError_t reserve(size_t n) {
+    if (n <= bytes) return hipSuccess;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&ptr, n);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  void free() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+};
+
+struct DeviceScene {
+  DevArray<PairNode> tri_pairs, tlas_pairs;
+  DevArray<TriPre> tris;
+  DevArray<uint32_t> redirect, indices, binding, tri_material;
+  DevArray<float> vertices, normals;
+  DevArray<GeoRec> geos;
+  DevArray<InstRec> insts;
+  DevArray<TlasRoot> tlas_roots;
+  DevArray<RefNode> tri_forest, tlas_forest;
+  uint64_t bytes = 0;
+};
+
+// Path-tracer state of one context (grown on demand).
+struct PtState {
+  uint32_t capacity = 0;  // paths
+  uint32_t pixels = 0;
+  Scratch rays, shadow_rays, hits, shadow_hits, hitx, beta, L, nee, qa, qb, qs, counts, lookback, tickets;
+  Scratch materials;
+  uint32_t nmaterials_uploaded = 0;
+  bool materials_dirty = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  PtLaunch cached{};
+  bool have_cached = false;
+  uint32_t last_bounces = 0;
+  float last_ms = 0.0f;
+  void free_all();
+};
+
+void free_device_scene(DeviceScene &d);
+
+}  // namespace rdrt
+
+struct rdrt_ctx {
+  rdrt_config cfg{};
+  bool has_device = false;
+  hipStream_t stream = nullptr;       // user stream (may be the null stream)
+  hipStream_t work_stream = nullptr;  // capturable stream: the user stream, or one owned by the context
+  bool owns_work_stream = false;
+  rdrt::SceneSrc src;
+  std::vector<rdrt::GeometrySrc> pending;
+  bool in_blas = false;
+  rdrt::BuiltScene built;
+  bool built_valid = false;
+  bool uploaded = false;
+  rdrt::DeviceScene dev;
+  uint32_t *status = nullptr;  // device status word of the kernels
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+  std::vector<rdrt::Material> materials;
+  rdrt::PtState pt;
+  rdrt::Scratch compact_scratch;
+};
+
+namespace rdrt {
+
+int32_t need_device(rdrt_ctx *c);
+void invalidate(rdrt_ctx *c);
+// Device view of the committed scene for the trace kernels (no rays).
+TraceParams scene_trace_params(const rdrt_ctx *c);
+uint32_t push_depth(const rdrt_ctx *c);
+
+}  // namespace rdrt
+
+#endif

@@ -1,0 +1,66 @@
+// pt.h -- wavefront path tracer: launch description and entry points.
+#ifndef RDRT_PT_H
+#define RDRT_PT_H
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/rdrt.h"
+#include "rdrt_layout.h"
+#include "trace_kernels.h"
+
+namespace rdrt {
+
+constexpr uint32_t kPtBlock = 256;       // threads per block of the per-path kernels
+constexpr uint32_t kPtMaxBounces = 16;   // upper bound on max_bounces
+
+// Everything one frame's kernels read, passed by value (so a captured graph
+// holds it).  Device arrays are indexed by path id unless noted.
+struct PtLaunch {
+  float vp_inv[16];
+  float cam[3];
+  uint32_t width, height, spp, npaths;
+  uint32_t max_bounces, rr_start, seed, sample_base;
+  float sky_up[3], sky_down[3];
+  float sun_dir[3], sun_radiance[3];
+
+  // scene
+  const InstRec *insts;
+  const GeoRec *geos;
+  const uint32_t *indices;
+  const float *vertices;
+  const float *normals;          // per-vertex normals or null (geometric normal)
+  const uint32_t *tri_material;  // per-triangle material offset or null
+  const Material *materials;
+  uint32_t nmaterials;
+
+  // path state
+  rdrt_ray *rays, *shadow_rays;
+  rdrt_hit *hits, *shadow_hits;
+  uint2 *hitx;  // (global geometry, forest instance) of each hit
+  float *beta, *L, *nee;
+  uint32_t *qa, *qb, *qs;  // extension queues (ping-pong) and the shadow queue
+  uint32_t *counts;        // [0..max_bounces+1]: extension queue sizes; then shadow sizes per bounce
+  unsigned long long *lookback;  // per bounce: one status word per block
+  uint32_t *tickets;             // per bounce: block ticket counter
+  uint32_t lookback_stride;
+  uint32_t *status;
+  float *out;  // width * height * 3
+};
+
+// Enqueues one frame on `stream` (capturable: no memset nodes).
+hipError_t enqueue_pt_frame(const PtLaunch &L, const TraceParams &scene_tp, uint32_t push_depth, uint32_t block,
+                            hipStream_t stream);
+
+// Stable stream compaction of indices whose mask is non-zero (single pass,
+// decoupled look-back); out_count receives the kept count.  `scratch` holds
+// ceil(n / kPtBlock) status words + 1 ticket word, zeroed by this call.
+hipError_t enqueue_compact(const uint32_t *mask, uint32_t n, uint32_t *out_idx, uint32_t *out_count,
+                           unsigned long long *scratch, hipStream_t stream);
+
+// Sobol / van der Corput samples: out[2i] = vdc(index[i], scr[2i]),
+// out[2i+1] = sobol1(index[i], scr[2i+1]).
+hipError_t enqueue_sobol(const uint32_t *index, const uint32_t *scr, uint32_t n, float *out, hipStream_t stream);
+
+}  // namespace rdrt
+
+#endif

@@ -1,0 +1,385 @@
+// pt_kernels.hip -- wavefront path tracer for gfx950.
+//
+// Stage structure of the reference path tracer (scene/rendering/gpu-ray-
+// tracing/src/feature/path_tracing/{ray_gen.rs, ray_hit.rs, ray_miss.rs}):
+// ray generation, closest hit (emission, next-event estimation toward the
+// directional light with an any-hit shadow ray, Lambertian BSDF sample),
+// miss (sky).  Instead of the reference's task graph, each bounce is four
+// launches over device-resident queues:
+//   trace extension rays -> shade (+ stable compaction of the survivors and
+//   of the shadow rays) -> trace shadow rays -> add unoccluded light.
+// Queue sizes stay on the device, so a whole frame is one capturable
+// sequence with no host round trip.
+//
+// Compaction: one pass fused into the shade kernel.  Each wave ballots its
+// keep flags (rank = mbcnt), each block sums its waves in LDS and gets its
+// exclusive offset by decoupled look-back over per-block status words; block
+// order comes from an atomic ticket, so every predecessor a block waits on
+// is already running.  The output equals a stable copy_if
+// (shader/parallel-compute/src/stream_compaction.rs:3-45).
+
+#include "pt.h"
+#include "pt_device.h"
+
+#pragma clang fp contract(off)
+
+namespace rdrt {
+namespace {
+
+using u64 = unsigned long long;
+using pt::V3;
+using pt::v3;
+
+constexpr uint32_t kMiss = 0xffffffffu;
+constexpr float kFltEps = 1.1920928955078125e-7f;
+constexpr float kFltMax = 3.402823466e+38f;
+constexpr float kInvPi = 0.318309886183790671538f;
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+__device__ __forceinline__ uint32_t lane_rank(u64 m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+struct Pair {
+  uint32_t a, b;
+};
+
+__device__ __forceinline__ u64 pack(u64 flag, Pair v) { return (flag << 62) | ((u64)v.a << 31) | (u64)v.b; }
+
+// Decoupled look-back for block `bid` (called by one thread): publishes the
+// block's aggregate, walks back to the nearest inclusive prefix, publishes its
+// own inclusive prefix and returns the exclusive one.  Status word: bits
+// 63-62 flag (0 empty, 1 aggregate, 2 inclusive), 61-31 count a, 30-0 count b.
+__device__ Pair lookback(u64 *status, uint32_t bid, Pair total, uint32_t *err) {
+  Pair excl{0u, 0u};
+  if (bid == 0) {
+    __hip_atomic_store(status, pack(2, total), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    return excl;
+  }
+  __hip_atomic_store(status + bid, pack(1, total), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  int32_t j = (int32_t)bid - 1;
+  uint32_t spins = 0;
+  while (j >= 0) {
+    u64 s = __hip_atomic_load(status + j, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    u64 flag = s >> 62;
+    if (flag == 0) {
+      // bounded wait: every predecessor already holds its ticket and publishes
+      // its aggregate without waiting; the limit only guards against a bug
+      if (++spins > kSpinLimit) {
+        atomicOr(err, 2u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    excl.a += (uint32_t)((s >> 31) & 0x7fffffffu);
+    excl.b += (uint32_t)(s & 0x7fffffffu);
+    if (flag == 2) break;
+    j--;
+  }
+  __hip_atomic_store(status + bid, pack(2, Pair{excl.a + total.a, excl.b + total.b}), __ATOMIC_RELEASE,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  return excl;
+}
+
+// Block-wide stable compaction of two flag sets: writes `item` to
+// out_a[offset] / out_b[offset] in thread order across the whole grid.
+// Must be called by every thread of the block.  Returns true on the block
+// that holds the last item (nblocks - 1) with the grid totals.
+template <int B>
+__device__ __forceinline__ void compact2(bool ka, bool kb, uint32_t item, uint32_t *out_a, uint32_t *out_b,
+                                         u64 *status, uint32_t bid, uint32_t nblocks, uint32_t *total_a,
+                                         uint32_t *total_b, uint32_t *err) {
+  __shared__ uint32_t s_wa[B / 64], s_wb[B / 64];
+  __shared__ uint32_t s_ea, s_eb;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const u64 ma = __ballot(ka), mb = __ballot(kb);
+  const uint32_t ra = lane_rank(ma), rb = lane_rank(mb);
+  if (lane == 0) {
+    s_wa[wave] = (uint32_t)__popcll(ma);
+    s_wb[wave] = (uint32_t)__popcll(mb);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t ta = 0, tb = 0;
+    for (int w = 0; w < B / 64; w++) {
+      uint32_t x = s_wa[w], y = s_wb[w];
+      s_wa[w] = ta;
+      s_wb[w] = tb;
+      ta += x;
+      tb += y;
+    }
+    Pair e = lookback(status, bid, Pair{ta, tb}, err);
+    s_ea = e.a;
+    s_eb = e.b;
+    if (bid == nblocks - 1) {
+      if (total_a) *total_a = e.a + ta;
+      if (total_b) *total_b = e.b + tb;
+    }
+  }
+  __syncthreads();
+  if (ka) out_a[s_ea + s_wa[wave] + ra] = item;
+  if (kb) out_b[s_eb + s_wb[wave] + rb] = item;
+}
+
+__device__ __forceinline__ V3 load3(const float *p) { return v3(p[0], p[1], p[2]); }
+__device__ __forceinline__ void store3(float *p, V3 v) {
+  p[0] = v.x;
+  p[1] = v.y;
+  p[2] = v.z;
+}
+
+// world normal = transpose(inverse 3x3) * n, from the column-major inverse
+__device__ __forceinline__ V3 normal_to_world(const float *inv, V3 n) {
+  return v3(inv[0] * n.x + inv[1] * n.y + inv[2] * n.z, inv[4] * n.x + inv[5] * n.y + inv[6] * n.z,
+            inv[8] * n.x + inv[9] * n.y + inv[10] * n.z);
+}
+
+__device__ __forceinline__ void pixel_of(const PtLaunch &P, uint32_t path, uint32_t &px, uint32_t &py,
+                                         uint32_t &n) {
+  uint32_t pixel = path / P.spp;
+  n = P.sample_base + path % P.spp;
+  px = pixel % P.width;
+  py = pixel / P.width;
+}
+
+__device__ __forceinline__ void write_ray(rdrt_ray *r, V3 o, float tmin, V3 d, float tmax) {
+  float4 *q = reinterpret_cast<float4 *>(r);
+  q[0] = make_float4(o.x, o.y, o.z, tmin);
+  q[1] = make_float4(d.x, d.y, d.z, tmax);
+}
+
+__global__ void k_pt_begin(PtLaunch P) {
+  const uint32_t nc = 2 * (P.max_bounces + 1);
+  const uint32_t nl = (P.max_bounces + 1) * P.lookback_stride;
+  const uint32_t n = nl > nc ? nl : nc;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (i < nl) P.lookback[i] = 0ull;
+    if (i < nc) P.counts[i] = i == 0 ? P.npaths : 0u;
+    if (i < P.max_bounces + 1) P.tickets[i] = 0u;
+  }
+}
+
+// ray generation (ray_gen.rs:89-107 + camera.rs:66-99)
+__global__ __launch_bounds__(kPtBlock) void k_pt_camera(PtLaunch P) {
+  const uint32_t path = blockIdx.x * kPtBlock + threadIdx.x;
+  if (path >= P.npaths) return;
+  uint32_t px, py, n;
+  pixel_of(P, path, px, py, n);
+  float jx = pt::van_der_corput(n, pt::scramble(px, py, P.seed, 0));
+  float jy = pt::sobol1(n, pt::scramble(px, py, P.seed, 1));
+  float u = (float)px / (float)P.width + jx / (float)P.width;
+  float v = (float)py / (float)P.height + jy / (float)P.height;
+  V3 target = pt::unproject(P.vp_inv, u, v);
+  V3 cam = v3(P.cam[0], P.cam[1], P.cam[2]);
+  V3 d = pt::normalize3(target - cam);
+  write_ray(P.rays + path, cam, kFltEps, d, kFltMax);
+  store3(P.beta + 3 * (size_t)path, v3(1.0f, 1.0f, 1.0f));
+  store3(P.L + 3 * (size_t)path, v3(0.0f, 0.0f, 0.0f));
+  P.qa[path] = path;
+}
+
+// closest hit + miss for one path (ray_hit.rs:6-104, ray_miss.rs:136-158)
+__device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &keep, bool &shadow) {
+  const uint4 h = *reinterpret_cast<const uint4 *>(P.hits + path);
+  const float4 *rq = reinterpret_cast<const float4 *>(P.rays + path);
+  const float4 r0 = rq[0], r1 = rq[1];
+  const V3 o = v3(r0.x, r0.y, r0.z), d = v3(r1.x, r1.y, r1.z);
+  V3 beta = load3(P.beta + 3 * (size_t)path);
+  V3 L = load3(P.L + 3 * (size_t)path);
+  if (h.x == kMiss) {
+    const float *sky = d.y > 0.0f ? P.sky_up : P.sky_down;
+    L = L + beta * load3(sky);
+    store3(P.L + 3 * (size_t)path, L);
+    return;
+  }
+  const float t = __uint_as_float(h.y), bu = __uint_as_float(h.z), bv = __uint_as_float(h.w);
+  const uint2 hx = P.hitx[path];
+  const GeoRec &g = P.geos[hx.x];
+  const InstRec &in = P.insts[hx.y];
+  const uint32_t tri = g.primitive_start + h.x;
+  const uint32_t i0 = P.indices[3 * (size_t)tri], i1 = P.indices[3 * (size_t)tri + 1],
+                 i2 = P.indices[3 * (size_t)tri + 2];
+  const V3 p0 = load3(P.vertices + 3 * (size_t)i0), p1 = load3(P.vertices + 3 * (size_t)i1),
+           p2 = load3(P.vertices + 3 * (size_t)i2);
+  V3 ng = pt::normalize3(normal_to_world(in.inv, pt::cross3(p1 - p0, p2 - p0)));
+  V3 ns = ng;
+  if (P.normals) {
+    const float bw = 1.0f - bu - bv;
+    V3 n_obj = load3(P.normals + 3 * (size_t)i0) * bw + load3(P.normals + 3 * (size_t)i1) * bu +
+               load3(P.normals + 3 * (size_t)i2) * bv;
+    if (pt::dot3(n_obj, n_obj) > 0.0f) ns = pt::normalize3(normal_to_world(in.inv, n_obj));
+  }
+  // face the incoming ray (bindless_mesh_bridge.rs:86-123)
+  if (pt::dot3(ng, d) > 0.0f) ng = ng * -1.0f;
+  if (pt::dot3(ns, ng) < 0.0f) ns = ns * -1.0f;
+  const V3 pos = o + d * t;
+  uint32_t mi = in.custom_index + (P.tri_material ? P.tri_material[tri] : 0u);
+  if (mi >= P.nmaterials) mi = P.nmaterials - 1;
+  const Material &m = P.materials[mi];
+  const V3 albedo = v3(m.albedo[0], m.albedo[1], m.albedo[2]);
+  L = L + beta * v3(m.emission[0], m.emission[1], m.emission[2]);
+  if (b < P.max_bounces) {
+    uint32_t px, py, n;
+    pixel_of(P, path, px, py, n);
+    const V3 origin = pt::offset_ray_hit(pos, ng);
+    // next-event estimation toward the directional light
+    const V3 wi = v3(-P.sun_dir[0], -P.sun_dir[1], -P.sun_dir[2]);
+    const float cos_l = pt::dot3(ns, wi);
+    if (cos_l > 0.0f && pt::dot3(ng, wi) > 0.0f &&
+        (P.sun_radiance[0] > 0.0f || P.sun_radiance[1] > 0.0f || P.sun_radiance[2] > 0.0f)) {
+      const V3 c = beta * albedo * (cos_l * kInvPi) * load3(P.sun_radiance);
+      store3(P.nee + 3 * (size_t)path, c);
+      write_ray(P.shadow_rays + path, origin, kFltEps, wi, kFltMax);
+      shadow = true;
+    }
+    // Lambertian importance sample: f cos / pdf = albedo
+    const float s0 = pt::van_der_corput(n, pt::scramble(px, py, P.seed, 2 + 2 * b));
+    const float s1 = pt::sobol1(n, pt::scramble(px, py, P.seed, 3 + 2 * b));
+    const V3 wo = pt::sample_cosine(ns, s0, s1);
+    bool alive = pt::dot3(wo, ng) > 0.0f;
+    V3 nb = beta * albedo;
+    if (alive && b >= P.rr_start) {
+      const float q = fminf(fmaxf(fmaxf(nb.x, nb.y), nb.z), 0.95f);
+      const float u = pt::van_der_corput(n, pt::scramble(px, py, P.seed, 64 + b));
+      if (u >= q) alive = false;
+      else nb = v3(nb.x / q, nb.y / q, nb.z / q);
+    }
+    if (alive) {
+      write_ray(P.rays + path, origin, kFltEps, wo, kFltMax);
+      store3(P.beta + 3 * (size_t)path, nb);
+      keep = true;
+    }
+  }
+  store3(P.L + 3 * (size_t)path, L);
+}
+
+template <int B>
+__global__ __launch_bounds__(B) void k_pt_shade(PtLaunch P, uint32_t b) {
+  __shared__ uint32_t s_bid;
+  if (threadIdx.x == 0) s_bid = atomicAdd(P.tickets + b, 1u);
+  __syncthreads();
+  const uint32_t bid = s_bid;
+  const uint32_t count_in = P.counts[b];
+  const uint32_t nblocks = (count_in + B - 1) / B;
+  if (bid >= nblocks) return;  // whole block
+  const uint32_t *qin = (b & 1) ? P.qb : P.qa;
+  uint32_t *qout = (b & 1) ? P.qa : P.qb;
+  const uint32_t i = bid * B + threadIdx.x;
+  bool keep = false, shadow = false;
+  uint32_t path = 0;
+  if (i < count_in) {
+    path = qin[i];
+    shade_one(P, b, path, keep, shadow);
+  }
+  compact2<B>(keep, shadow, path, qout, P.qs, P.lookback + (size_t)b * P.lookback_stride, bid, nblocks,
+              P.counts + b + 1, P.counts + P.max_bounces + 1 + b, P.status);
+}
+
+// add the light of unoccluded shadow rays
+__global__ __launch_bounds__(kPtBlock) void k_pt_nee(PtLaunch P, uint32_t b) {
+  const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+  if (i >= P.counts[P.max_bounces + 1 + b]) return;
+  const uint32_t path = P.qs[i];
+  if (P.shadow_hits[path].prim_id != kMiss) return;
+  V3 L = load3(P.L + 3 * (size_t)path) + load3(P.nee + 3 * (size_t)path);
+  store3(P.L + 3 * (size_t)path, L);
+}
+
+__global__ __launch_bounds__(kPtBlock) void k_pt_finalize(PtLaunch P) {
+  const uint32_t pixel = blockIdx.x * kPtBlock + threadIdx.x;
+  if (pixel >= P.width * P.height) return;
+  V3 s = v3(0.0f, 0.0f, 0.0f);
+  for (uint32_t k = 0; k < P.spp; k++) s = s + load3(P.L + 3 * ((size_t)pixel * P.spp + k));
+  const float inv = 1.0f / (float)P.spp;
+  store3(P.out + 3 * (size_t)pixel, s * inv);
+}
+
+// standalone stable compaction (same machinery as the shade kernel)
+__global__ void k_fill_u64(u64 *p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
+__global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
+
+__global__ __launch_bounds__(kPtBlock) void k_compact(const uint32_t *mask, uint32_t n, uint32_t *out,
+                                                      uint32_t *out_count, u64 *status, uint32_t *ticket,
+                                                      uint32_t *err) {
+  __shared__ uint32_t s_bid;
+  if (threadIdx.x == 0) s_bid = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint32_t bid = s_bid;
+  const uint32_t nblocks = (n + kPtBlock - 1) / kPtBlock;
+  const uint32_t i = bid * kPtBlock + threadIdx.x;
+  const bool keep = i < n && mask[i] != 0u;
+  compact2<kPtBlock>(keep, false, i, out, out, status, bid, nblocks, out_count, nullptr, err);
+}
+
+__global__ void k_sobol(const uint32_t *index, const uint32_t *scr, uint32_t n, float *out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[2 * i] = pt::van_der_corput(index[i], scr[2 * i]);
+  out[2 * i + 1] = pt::sobol1(index[i], scr[2 * i + 1]);
+}
+
+}  // namespace
+
+hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_t push_depth, uint32_t block,
+                            hipStream_t st) {
+  const uint32_t grid = (P.npaths + kPtBlock - 1) / kPtBlock;
+  hipLaunchKernelGGL(k_pt_begin, dim3(256), dim3(256), 0, st, P);
+  hipLaunchKernelGGL(k_pt_camera, dim3(grid), dim3(kPtBlock), 0, st, P);
+  for (uint32_t b = 0; b <= P.max_bounces; b++) {
+    TraceParams tp = scene;
+    tp.rays = P.rays;
+    tp.hits = P.hits;
+    tp.hitx = P.hitx;
+    tp.meta = nullptr;
+    tp.hit_meta = nullptr;
+    tp.counters = nullptr;
+    tp.queue = (b & 1) ? P.qb : P.qa;
+    tp.count = P.counts + b;
+    tp.n = P.npaths;
+    tp.flags = RDRT_RAY_FLAG_CULL_BACK_FACING_TRIANGLES;  // ray_gen.rs:162-166
+    tp.cull_mask = 0xffffffffu;
+    tp.tlas_idx = 0;
+    hipError_t e = launch_trace(tp, RDRT_TRAVERSAL_FAST, push_depth, block, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_pt_shade<kPtBlock>), dim3(grid), dim3(kPtBlock), 0, st, P, b);
+    if (b == P.max_bounces) break;
+    TraceParams sp = tp;
+    sp.rays = P.shadow_rays;
+    sp.hits = P.shadow_hits;
+    sp.hitx = nullptr;
+    sp.queue = P.qs;
+    sp.count = P.counts + P.max_bounces + 1 + b;
+    sp.flags = RDRT_RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH;  // ray_hit.rs:33-45
+    e = launch_trace(sp, RDRT_TRAVERSAL_FAST, push_depth, block, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_pt_nee, dim3(grid), dim3(kPtBlock), 0, st, P, b);
+  }
+  hipLaunchKernelGGL(k_pt_finalize, dim3((P.width * P.height + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, st, P);
+  return hipGetLastError();
+}
+
+hipError_t enqueue_compact(const uint32_t *mask, uint32_t n, uint32_t *out_idx, uint32_t *out_count,
+                           unsigned long long *scratch, hipStream_t st) {
+  const uint32_t nb = (n + kPtBlock - 1) / kPtBlock;
+  // status words, then the ticket (as a u64 slot), then an error word
+  hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, st, scratch, nb + 2);
+  uint32_t *ticket = reinterpret_cast<uint32_t *>(scratch + nb);
+  uint32_t *err = reinterpret_cast<uint32_t *>(scratch + nb + 1);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, out_count);
+  if (nb > 0)
+    hipLaunchKernelGGL(k_compact, dim3(nb), dim3(kPtBlock), 0, st, mask, n, out_idx, out_count, scratch, ticket,
+                       err);
+  return hipGetLastError();
+}
+
+hipError_t enqueue_sobol(const uint32_t *index, const uint32_t *scr, uint32_t n, float *out, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sobol, dim3((n + 255) / 256), dim3(256), 0, st, index, scr, n, out);
+  return hipGetLastError();
+}
+
+}  // namespace rdrt

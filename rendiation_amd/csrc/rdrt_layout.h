@@ -100,6 +100,16 @@ struct alignas(16) TlasRoot {
 };
 static_assert(sizeof(TlasRoot) == 32, "TlasRoot must be 32 B");
 
+// Lambertian material (content/lighting/transport/src/surface/microfacet/
+// device/diffuse.rs): albedo and emitted radiance, 32 B.
+struct alignas(16) Material {
+  float albedo[3];
+  float pad0;
+  float emission[3];
+  float pad1;
+};
+static_assert(sizeof(Material) == 32, "Material must be 32 B");
+
 }  // namespace rdrt
 
 #endif

@@ -357,6 +357,13 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
   std::vector<Box> blas_box(src.blas.size(), box_empty());
   std::vector<char> blas_has_box(src.blas.size(), 0);
   std::vector<uint32_t> blas_geo_begin(src.blas.size(), 0), blas_geo_end(src.blas.size(), 0);
+  bool any_normals = false, any_material = false;
+  for (uint32_t b = 0; b < src.blas.size(); b++)
+    if (src.blas_live[b])
+      for (const GeometrySrc &g : src.blas[b]) {
+        any_normals |= !g.normals.empty();
+        any_material |= !g.tri_material.empty();
+      }
   {
     size_t ji = 0;
     for (uint32_t b = 0; b < src.blas.size(); b++) {
@@ -373,6 +380,21 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
         out.vertices.insert(out.vertices.end(), g.pos.begin(), g.pos.end() - (g.pos.size() % 3));
         for (uint32_t k : job.tree.order) out.redirect.push_back(primitive_start + k);
         for (uint32_t v : job.tri_idx) out.indices.push_back(vertex_start + v);
+        if (any_normals) {
+          size_t nv = g.pos.size() / 3;
+          if (g.normals.size() == 3 * nv) out.normals.insert(out.normals.end(), g.normals.begin(), g.normals.end());
+          else out.normals.resize(out.normals.size() + 3 * nv, 0.0f);  // zero = use the geometric normal
+        }
+        uint32_t max_mat = 0;
+        if (any_material) {
+          size_t nt = job.tri_idx.size() / 3;
+          for (size_t t = 0; t < nt; t++) {
+            uint32_t m = t < g.tri_material.size() ? g.tri_material[t] : 0u;
+            out.tri_material.push_back(m);
+            max_mat = std::max(max_mat, m);
+          }
+        }
+        out.geo_max_material.push_back(max_mat);
         GeoRec gr;
         std::memset(&gr, 0, sizeof(gr));
         put_box(gr.mn, gr.mx, job.tree.nodes[0].box);

@@ -20,6 +20,10 @@ struct GeometrySrc {
   bool indexed = false;
   uint32_t ntris = 0;
   uint32_t flags = 0;
+  // shading attributes (path tracer): per-vertex normals (xyz, empty = none)
+  // and per-triangle material offsets (empty = 0)
+  std::vector<float> normals;
+  std::vector<uint32_t> tri_material;
 };
 
 struct SceneSrc {
@@ -37,6 +41,12 @@ struct BuiltScene {
   std::vector<RefNode> tri_forest, tlas_forest;
   std::vector<uint32_t> redirect, indices;
   std::vector<float> vertices;  // xyz
+  // shading attributes: normals parallel to vertices (empty when no geometry
+  // has any), material offset per triangle (empty when all are 0), largest
+  // material offset per geometry
+  std::vector<float> normals;
+  std::vector<uint32_t> tri_material;
+  std::vector<uint32_t> geo_max_material;
   // shared records
   std::vector<GeoRec> geos;
   std::vector<InstRec> insts;

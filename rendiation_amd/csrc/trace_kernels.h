@@ -18,6 +18,12 @@ struct TraceParams {
   uint32_t *counters;
   uint32_t *status;  // device word; bit 0 = a lane's traversal stack would have overflowed
   uint32_t n, flags, cull_mask, tlas_idx;
+  // Optional work queue: thread i traces ray queue[i] for i < *count (both
+  // device pointers; count null = n).  Rays and outputs are indexed by the
+  // queue entry.
+  const uint32_t *queue;
+  const uint32_t *count;
+  uint2 *hitx;  // optional: (global geometry index, forest instance index) per hit
 
   const PairNode *tri_pairs;
   const TriPre *tris;

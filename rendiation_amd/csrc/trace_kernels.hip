@@ -189,6 +189,7 @@ __device__ __forceinline__ void write_result(const TraceParams &p, uint32_t rid,
   }
   *reinterpret_cast<uint4 *>(p.hits + rid) = h;
   if (p.hit_meta) *reinterpret_cast<uint4 *>(p.hit_meta + rid) = hm;
+  if (p.hitx) p.hitx[rid] = make_uint2(best.geo, best.inst);
   if (p.counters) {
     p.counters[3 * rid + 0] = cn;
     p.counters[3 * rid + 1] = ct;
@@ -202,8 +203,9 @@ __device__ __forceinline__ void write_result(const TraceParams &p, uint32_t rid,
 template <int S, int B>
 __global__ __launch_bounds__(B) void k_trace_fast(TraceParams p) {
   __shared__ uint32_t lds[S * B];
-  const uint32_t rid = blockIdx.x * B + threadIdx.x;
-  if (rid >= p.n) return;
+  const uint32_t slot = blockIdx.x * B + threadIdx.x;
+  if (slot >= (p.count ? *p.count : p.n)) return;
+  const uint32_t rid = p.queue ? p.queue[slot] : slot;
   uint32_t *stk = lds + threadIdx.x;  // entry e at stk[e * B]
 
   RayIn r = load_ray(p, rid);
@@ -352,8 +354,9 @@ __device__ __forceinline__ F3 vertex(const TraceParams &p, uint32_t i) {
 
 template <int B>
 __global__ __launch_bounds__(B) void k_trace_ref(TraceParams p) {
-  const uint32_t rid = blockIdx.x * B + threadIdx.x;
-  if (rid >= p.n) return;
+  const uint32_t slot = blockIdx.x * B + threadIdx.x;
+  if (slot >= (p.count ? *p.count : p.n)) return;
+  const uint32_t rid = p.queue ? p.queue[slot] : slot;
   RayIn r = load_ray(p, rid);
   float far_ = r.tmax;
   Best best;

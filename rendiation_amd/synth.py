@@ -138,6 +138,56 @@ def random_rotation(seed: int, stream: int, index: int) -> np.ndarray:
                      [2 * (i * k - j * w), 2 * (j * k + i * w), 1 - 2 * (i * i + j * j)]])
 
 
+def sphere_normals(nu: int, nv: int) -> np.ndarray:
+    p, _ = uv_sphere(nu, nv)
+    n = p.astype(np.float64)
+    n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+    return n.astype(np.float32)
+
+
+def torus_normals(nu: int, nv: int, radius: float = 1.0, tube_radius: float = 0.2) -> np.ndarray:
+    u, v = _grid_uv(nu, nv)
+    a = 2.0 * np.pi * u.astype(np.float64)
+    b = 2.0 * np.pi * v.astype(np.float64)
+    n = np.stack([np.cos(b) * np.cos(a), np.cos(b) * np.sin(a), np.sin(b)], -1)
+    return n.reshape(-1, 3).astype(np.float32)
+
+
+def pt_scene(seed: int = 0, n_meshes: int = 64, tess: int = 256, n_emissive: int = 4, emission: float = 5.0,
+             materials_per_mesh: int = 1, extent: float = 10.0) -> dict:
+    """Parametric path-tracing scene: ``n_meshes`` tessellated UV spheres
+    (even i) and tori (odd i, R = 1, r = 0.2) at tess x tess, each its own
+    BLAS, placed by a uniform random rotation, uniform scale U[0.5, 2] and a
+    translation U[-extent, extent]^3.  Materials: albedo U[0.1, 0.9]^3,
+    meshes 0..n_emissive-1 emit ``emission``.  With materials_per_mesh = m,
+    triangles of a mesh get m bands of material offsets along u.
+      streams 20 rotation, 21 scale, 22 translation, 23 albedo"""
+    sph_p, sph_i = uv_sphere(tess, tess)
+    tor_p, tor_i = torus(tess, tess)
+    sph_n = sphere_normals(tess, tess)
+    tor_n = torus_normals(tess, tess)
+    ntri = sph_i.shape[0]
+    band = (np.arange(ntri, dtype=np.int64) * materials_per_mesh // ntri).astype(np.uint32)
+    meshes, instances = [], []
+    nmat = n_meshes * materials_per_mesh
+    albedo = (np.float32(0.1) + np.float32(0.8) * uniform_f32(seed, 23, 3 * nmat)).reshape(nmat, 3)
+    emit = np.zeros((nmat, 3), dtype=np.float32)
+    scale = np.float32(0.5) + np.float32(1.5) * uniform_f32(seed, 21, n_meshes)
+    trans = (np.float32(2.0 * extent) * uniform_f32(seed, 22, 3 * n_meshes) - np.float32(extent)).reshape(-1, 3)
+    for i in range(n_meshes):
+        kind = i % 2
+        meshes.append({"positions": sph_p if kind == 0 else tor_p, "indices": sph_i if kind == 0 else tor_i,
+                       "normals": sph_n if kind == 0 else tor_n,
+                       "tri_material": band if materials_per_mesh > 1 else None})
+        m = np.eye(4)
+        m[:3, :3] = random_rotation(seed, 20, i) * float(scale[i])
+        m[:3, 3] = trans[i]
+        instances.append({"transform": m, "mesh": i, "material": i * materials_per_mesh})
+        if i < n_emissive:
+            emit[i * materials_per_mesh:(i + 1) * materials_per_mesh] = emission
+    return {"meshes": meshes, "instances": instances, "albedo": albedo, "emission": emit}
+
+
 # Closest-hit workload sizes (100k triangles, 1M rays, seed 0).
 CLOSEST_HIT_TRIS = 100_000
 CLOSEST_HIT_RAYS = 1_000_000

@@ -1,0 +1,61 @@
+"""Shared set-up of path-tracing scenes in the product and in the oracle."""
+
+from __future__ import annotations
+
+import numpy as np
+
+from oracle.oracle import OracleScene, OrcPtParams
+from rendiation_amd.pathtracer import Camera, PathTracingConfig
+
+
+def oracle_scene_from(scene: dict, blas_bin_size: int = 2) -> OracleScene:
+    o = OracleScene(blas_bin_size=blas_bin_size)
+    for i, mesh in enumerate(scene["meshes"]):
+        o.add_blas([(mesh["positions"], mesh["indices"], 1)])
+        if mesh.get("normals") is not None:
+            o.set_vertex_normals(i, 0, mesh["normals"])
+        if mesh.get("tri_material") is not None:
+            o.set_triangle_materials(i, 0, mesh["tri_material"])
+    t = o.add_tlas([{"transform": d["transform"], "blas": d["mesh"], "custom": d["material"]}
+                    for d in scene["instances"]])
+    o.bind([t])
+    o.set_materials(scene["albedo"], scene["emission"])
+    o.build()
+    return o
+
+
+def oracle_params(camera: Camera, cfg: PathTracingConfig) -> OrcPtParams:
+    c = cfg.to_c(camera)
+    p = OrcPtParams()
+    for name, _ in OrcPtParams._fields_:
+        v = getattr(c, name)
+        if hasattr(v, "__len__"):
+            getattr(p, name)[:] = list(v)
+        else:
+            setattr(p, name, v)
+    return p
+
+
+def inward_sphere(tess: int = 32, radius: float = 1.0):
+    """UV sphere with the triangle winding reversed (normals point in)."""
+    from rendiation_amd import synth
+
+    p, idx = synth.uv_sphere(tess, tess, radius)
+    return p, idx[:, [0, 2, 1]].copy()
+
+
+def furnace_scene(albedo: float, emission: float, tess: int = 32) -> dict:
+    p, idx = inward_sphere(tess, 1.0)
+    return {"meshes": [{"positions": p, "indices": idx, "normals": None, "tri_material": None}],
+            "instances": [{"transform": np.eye(4), "mesh": 0, "material": 0}],
+            "albedo": np.full((1, 3), albedo, np.float32), "emission": np.full((1, 3), emission, np.float32)}
+
+
+def floor_scene(albedo: float, emission: float, size: float = 1000.0) -> dict:
+    """One large upward-facing quad at y = 0 (two triangles, +y normal)."""
+    s = size
+    p = np.array([[-s, 0, -s], [s, 0, -s], [s, 0, s], [-s, 0, s]], np.float32)
+    idx = np.array([[0, 2, 1], [0, 3, 2]], np.uint32)  # cross(e1, e2) points +y
+    return {"meshes": [{"positions": p, "indices": idx, "normals": None, "tri_material": None}],
+            "instances": [{"transform": np.eye(4), "mesh": 0, "material": 0}],
+            "albedo": np.full((1, 3), albedo, np.float32), "emission": np.full((1, 3), emission, np.float32)}
