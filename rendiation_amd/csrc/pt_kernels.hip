@@ -89,7 +89,7 @@ __device__ Pair lookback(u64 *status, uint32_t bid, Pair total, uint32_t *err) {
 template <int B>
 __device__ __forceinline__ void compact2(bool ka, bool kb, uint32_t item, uint32_t *out_a, uint32_t *out_b,
                                          u64 *status, uint32_t bid, uint32_t nblocks, uint32_t *total_a,
-                                         uint32_t *total_b, uint32_t *err) {
+                                         uint32_t *total_b, uint32_t *err, uint32_t capacity) {
   __shared__ uint32_t s_wa[B / 64], s_wb[B / 64];
   __shared__ uint32_t s_ea, s_eb;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
@@ -118,8 +118,10 @@ __device__ __forceinline__ void compact2(bool ka, bool kb, uint32_t item, uint32
     }
   }
   __syncthreads();
-  if (ka) out_a[s_ea + s_wa[wave] + ra] = item;
-  if (kb) out_b[s_eb + s_wb[wave] + rb] = item;
+  // the bound only matters if a look-back gave up (error flag set)
+  const uint32_t ia = s_ea + s_wa[wave] + ra, ib = s_eb + s_wb[wave] + rb;
+  if (ka && ia < capacity) out_a[ia] = item;
+  if (kb && ib < capacity) out_b[ib] = item;
 }
 
 __device__ __forceinline__ V3 load3(const float *p) { return v3(p[0], p[1], p[2]); }
@@ -268,18 +270,18 @@ __global__ __launch_bounds__(B) void k_pt_shade(PtLaunch P, uint32_t b) {
   const uint32_t i = bid * B + threadIdx.x;
   bool keep = false, shadow = false;
   uint32_t path = 0;
-  if (i < count_in) {
+  if (i < count_in && i < P.npaths) {
     path = qin[i];
-    shade_one(P, b, path, keep, shadow);
+    if (path < P.npaths) shade_one(P, b, path, keep, shadow);
   }
   compact2<B>(keep, shadow, path, qout, P.qs, P.lookback + (size_t)b * P.lookback_stride, bid, nblocks,
-              P.counts + b + 1, P.counts + P.max_bounces + 1 + b, P.status);
+              P.counts + b + 1, P.counts + P.max_bounces + 1 + b, P.status, P.npaths);
 }
 
 // add the light of unoccluded shadow rays
 __global__ __launch_bounds__(kPtBlock) void k_pt_nee(PtLaunch P, uint32_t b) {
   const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
-  if (i >= P.counts[P.max_bounces + 1 + b]) return;
+  if (i >= P.counts[P.max_bounces + 1 + b] || i >= P.npaths) return;
   const uint32_t path = P.qs[i];
   if (P.shadow_hits[path].prim_id != kMiss) return;
   V3 L = load3(P.L + 3 * (size_t)path) + load3(P.nee + 3 * (size_t)path);
@@ -312,7 +314,7 @@ __global__ __launch_bounds__(kPtBlock) void k_compact(const uint32_t *mask, uint
   const uint32_t nblocks = (n + kPtBlock - 1) / kPtBlock;
   const uint32_t i = bid * kPtBlock + threadIdx.x;
   const bool keep = i < n && mask[i] != 0u;
-  compact2<kPtBlock>(keep, false, i, out, out, status, bid, nblocks, out_count, nullptr, err);
+  compact2<kPtBlock>(keep, false, i, out, out, status, bid, nblocks, out_count, nullptr, err, n);
 }
 
 __global__ void k_sobol(const uint32_t *index, const uint32_t *scr, uint32_t n, float *out) {

@@ -204,8 +204,9 @@ template <int S, int B>
 __global__ __launch_bounds__(B) void k_trace_fast(TraceParams p) {
   __shared__ uint32_t lds[S * B];
   const uint32_t slot = blockIdx.x * B + threadIdx.x;
-  if (slot >= (p.count ? *p.count : p.n)) return;
+  if (slot >= p.n || (p.count && slot >= *p.count)) return;
   const uint32_t rid = p.queue ? p.queue[slot] : slot;
+  if (rid >= p.n) return;  // a queue entry always names one of the n rays
   uint32_t *stk = lds + threadIdx.x;  // entry e at stk[e * B]
 
   RayIn r = load_ray(p, rid);
@@ -355,8 +356,9 @@ __device__ __forceinline__ F3 vertex(const TraceParams &p, uint32_t i) {
 template <int B>
 __global__ __launch_bounds__(B) void k_trace_ref(TraceParams p) {
   const uint32_t slot = blockIdx.x * B + threadIdx.x;
-  if (slot >= (p.count ? *p.count : p.n)) return;
+  if (slot >= p.n || (p.count && slot >= *p.count)) return;
   const uint32_t rid = p.queue ? p.queue[slot] : slot;
+  if (rid >= p.n) return;  // a queue entry always names one of the n rays
   RayIn r = load_ray(p, rid);
   float far_ = r.tmax;
   Best best;
