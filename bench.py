@@ -1,17 +1,22 @@
 """Benchmark of the ray-query hot path on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload closest_hit]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pt|closest_hit]
 
-closest_hit: 100k random triangles (binned SAH, 4 bins, leaf <= 4), 1M rays
-(origins U[-0.5,1.5]^3, directions uniform on the sphere, t in [1e-4, inf)),
-seed 0, FP32.  One step = one closest-hit launch over all rays, inputs
-resident in HBM.  Prints one JSON line.
+pt (default): wavefront path tracer on the parametric scene -- 64 UV spheres
+  and tori tessellated at 256x256 (8,388,608 triangles, one BLAS each, 64
+  random affine instances in [-10, 10]^3), pinhole camera at 1920x1080, 1 spp,
+  Lambertian materials (albedo U[0.1, 0.9], 4 emissive meshes), up to 4
+  bounces, Russian roulette from bounce 2, Sobol per-pixel scrambles, FP32
+  radiance.  One step = one frame (a replayed hipGraph).  value = all rays
+  traced in a frame (closest-hit + shadow) / frame time.
+closest_hit: 100k random triangles (SAH 4 bins, leaf <= 4), 1M rays, seed 0;
+  one step = one closest-hit launch.
 
-roofline: algorithmic bytes per ray = 32 (ray) + 16 (hit) + 48 * node visits
-+ 52 * triangle tests + 176 * instance tests, with the visit counts of the
-reference traversal order (counted by the reference-order kernel on this
-input), divided by the average duration of the timed traversal launches.
-cpu_baseline: the CPU oracle's reference traversal on this host's cores.
+roofline: HBM-bound.  Algorithmic bytes per traced ray = 32 (ray) + 16 (hit)
++ 48 x node visits + 52 x triangle tests + 176 x instance tests, with visit
+counts of the reference traversal order on the same rays, divided by the
+time of the traversal launches (HIP events on their stream).
+cpu_baseline: the CPU oracle (OpenMP over rays / pixels on this host).
 """
 
 from __future__ import annotations
@@ -49,6 +54,141 @@ def _host_threads() -> int:
         return os.cpu_count() or 1
 
 
+def _bytes(n_rays: int, visits) -> int:
+    return 48 * n_rays + 48 * int(visits[0]) + 52 * int(visits[1]) + 176 * int(visits[2])
+
+
+def _record(value, unit, args, ms_per_step, config, roofline, extra=None):
+    rec = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": unit,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": config,
+        "roofline": roofline,
+    }
+    if extra:
+        rec.update(extra)
+    return rec
+
+
+# --------------------------------------------------------------- path tracer
+def path_tracing(args):
+    import numpy as np
+    import torch
+
+    from rendiation_amd import _lib, synth
+    from rendiation_amd.acceleration import AccelerationStructureSystem
+    from rendiation_amd.pathtracer import Camera, PathTracer, PathTracingConfig, load_scene
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    t0 = time.perf_counter()
+    scene = synth.pt_scene(seed=0, n_meshes=64, tess=256)
+    gen_ms = (time.perf_counter() - t0) * 1e3
+    system = AccelerationStructureSystem(device=0, stream=stream.cuda_stream)
+    t0 = time.perf_counter()
+    load_scene(system, scene)
+    build_upload_ms = (time.perf_counter() - t0) * 1e3
+    info = system.scene_info()
+    cam = Camera()
+    cfg = PathTracingConfig(width=1920, height=1080, spp=1, max_bounces=4, rr_start=2, seed=0, use_graph=True)
+    pt = PathTracer(system)
+    with torch.cuda.stream(stream):
+        out = torch.empty((cfg.height, cfg.width, 3), dtype=torch.float32, device=dev)
+    stream.synchronize()
+
+    for _ in range(args.warmup):
+        pt.render(cam, cfg, out=out)
+    system.sync()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        pt.render(cam, cfg, out=out)
+    e1.record(stream)
+    system.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    frame_ms_ev = e0.elapsed_time(e1) / args.steps
+    ms_per_step = wall * 1e3 / args.steps
+    st = pt.stats()
+    rays = int(sum(st["extension_rays"]) + sum(st["shadow_rays"]))
+    value = rays / (ms_per_step * 1e-3) / 1e6
+    image = out.cpu().numpy().astype(np.float64)
+
+    # diagnostics frames: traversal launch times, then reference-order visits
+    import dataclasses
+
+    pt.render(cam, dataclasses.replace(cfg, diagnostics=_lib.RDRT_PT_DIAG_TIME_TRACES))
+    d_time = pt.stats()
+    pt.render(cam, dataclasses.replace(cfg, diagnostics=_lib.RDRT_PT_DIAG_COUNT_VISITS))
+    d_vis = pt.stats()
+    system.sync()
+    bytes_frame = _bytes(rays, d_vis["visits"])
+    achieved = bytes_frame / (d_time["trace_ms"] * 1e-3) / 1e9
+
+    config = {
+        "workload": "pt: 64 spheres/tori at 256x256 (8.39M tris), 1920x1080, 1 spp, <=4 bounces, RR from 2, seed 0",
+        "tris": info.tris, "instances": info.instances, "width": cfg.width, "height": cfg.height, "spp": cfg.spp,
+        "max_bounces": cfg.max_bounces, "rays_per_frame": rays, "extension_rays": st["extension_rays"],
+        "shadow_rays": st["shadow_rays"][:-1], "frame_ms_events": round(frame_ms_ev, 4),
+        "scene_gen_ms": round(gen_ms, 1), "build_upload_ms": round(build_upload_ms, 1),
+        "device_scene_mb": round(info.device_bytes / 2 ** 20, 1), "graph_replay": True,
+    }
+    roofline = {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+        "kernel": "k_trace_fast (all traversal launches of one frame)",
+        "trace_ms_per_frame": round(d_time["trace_ms"], 4), "trace_launches": d_time["trace_launches"],
+        "bytes_per_frame": bytes_frame,
+        "visits_per_ray": [round(v / rays, 3) for v in d_vis["visits"]],
+    }
+    extra = {"ms_per_frame": round(ms_per_step, 4)}
+    if not args.no_cpu_baseline:
+        extra["cpu_baseline"], err = cpu_baseline_pt(scene, cam, cfg, image)
+        extra["image_error"] = err
+    system.close()
+    return _record(value, "Mrays/s", args, ms_per_step, config, roofline, extra)
+
+
+def cpu_baseline_pt(scene, cam, cfg, image):
+    """FP64 oracle path tracer on a band of 24 rows (46,080 paths) of the same
+    frame: its rays/s on this host, and the GPU image error on those pixels."""
+    import numpy as np
+
+    from tests.pt_util import oracle_params, oracle_scene_from
+
+    o = oracle_scene_from(scene)
+    prm = oracle_params(cam, cfg)
+    threads = _host_threads()
+    r0, r1 = cfg.height // 2 - 12, cfg.height // 2 + 12
+    t0 = time.perf_counter()
+    ref, ext, sh = o.pt_render_rows(prm, r0, r1, threads=threads)
+    dt = time.perf_counter() - t0
+    rays = int(ext.sum() + sh.sum())
+    gpu = image[r0:r1]
+    rmse = float(np.sqrt(np.mean((gpu - ref) ** 2)))
+    pix = np.abs(gpu - ref).max(axis=2) / np.maximum(ref.max(axis=2), 1e-6)
+    base = {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"rows {r0}-{r1 - 1} of the frame ({(r1 - r0) * cfg.width} paths, {rays} rays), FP64 shading",
+            "cpu_model": _cpu_model()}
+    err = {"vs": "FP64 CPU oracle on the same rows", "rmse": rmse, "rel_rmse": rmse / max(float(ref.mean()), 1e-12),
+           "pixels_within_1e-3": float((pix <= 1e-3).mean()), "bound_rel_rmse": 1e-2}
+    return base, err
+
+
+# --------------------------------------------------------------- closest hit
 def closest_hit(args):
     import numpy as np
     import torch
@@ -83,11 +223,10 @@ def closest_hit(args):
         hits = torch.empty((n, 4), dtype=torch.int32, device=dev)
         cnt = torch.empty((n, 3), dtype=torch.int32, device=dev)
     stream.synchronize()
-    # reference-order visit counts of this input (algorithmic bytes)
     system.trace(rays, hits, counters=cnt, traversal=TRAVERSAL_REFERENCE)
     system.sync()
     c = cnt.cpu().numpy().astype(np.int64).sum(axis=0)
-    bytes_per_launch = 48 * n + 48 * int(c[0]) + 52 * int(c[1]) + 176 * int(c[2])
+    bytes_per_launch = _bytes(n, c)
 
     for _ in range(args.warmup):
         system.trace(rays, hits, traversal=TRAVERSAL_FAST)
@@ -108,46 +247,26 @@ def closest_hit(args):
     value = n / (ms_per_step * 1e-3) / 1e6
     achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
     hit_rate = float((hits.cpu().numpy()[:, 0] != -1).mean())
-
-    rec = {
-        "metric": METRIC,
-        "value": round(value, 3),
-        "unit": "Mrays/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": "closest_hit: 100k random triangles (SAH 4 bins, leaf<=4), 1M random rays, seed 0",
-            "tris": synth.CLOSEST_HIT_TRIS, "rays": n, "leaf": synth.CLOSEST_HIT_LEAF,
-            "bvh_nodes": info.tri_nodes, "bvh_depth": info.max_blas_depth, "traversal": "fast",
-            "hit_fraction": round(hit_rate, 4), "build_upload_ms": round(build_upload_ms, 2),
-        },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
-            "kernel": "k_trace_fast",
-            "kernel_ms": round(kernel_ms, 4),
-            "bytes_per_launch": bytes_per_launch,
-            "visits_per_ray": [round(float(x) / n, 3) for x in c],
-        },
+    config = {
+        "workload": "closest_hit: 100k random triangles (SAH 4 bins, leaf<=4), 1M random rays, seed 0",
+        "tris": synth.CLOSEST_HIT_TRIS, "rays": n, "leaf": synth.CLOSEST_HIT_LEAF,
+        "bvh_nodes": info.tri_nodes, "bvh_depth": info.max_blas_depth, "traversal": "fast",
+        "hit_fraction": round(hit_rate, 4), "build_upload_ms": round(build_upload_ms, 2),
     }
+    roofline = {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": "k_trace_fast",
+        "kernel_ms": round(kernel_ms, 4), "bytes_per_launch": bytes_per_launch,
+        "visits_per_ray": [round(float(x) / n, 3) for x in c],
+    }
+    extra = {}
     if not args.no_cpu_baseline:
-        rec["cpu_baseline"] = cpu_baseline_closest_hit(pos, rays_np, args)
+        extra["cpu_baseline"] = cpu_baseline_closest_hit(pos, rays_np)
     system.close()
-    return rec
+    return _record(value, "Mrays/s", args, ms_per_step, config, roofline, extra)
 
 
-def cpu_baseline_closest_hit(pos, rays_np, args):
+def cpu_baseline_closest_hit(pos, rays_np):
     """The oracle's reference traversal (OpenMP over rays, all host threads)
     on the same input; repeated 3 times (about 10 CPU-seconds)."""
     import numpy as np
@@ -170,13 +289,9 @@ def cpu_baseline_closest_hit(pos, rays_np, args):
     orc.trace(one, threads=1)
     dt1 = time.perf_counter() - t1
     return {
-        "value": round(reps * rays_np.shape[0] / dt / 1e6, 3),
-        "unit": "Mrays/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"full workload ({rays_np.shape[0]} rays) x{reps}, OpenMP dynamic,1024 over rays",
-        "single_thread_mrays": round(one.shape[0] / dt1 / 1e6, 4),
-        "cpu_model": _cpu_model(),
+        "value": round(reps * rays_np.shape[0] / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+        "kind": "port", "sample": f"full workload ({rays_np.shape[0]} rays) x{reps}, OpenMP dynamic,1024 over rays",
+        "single_thread_mrays": round(one.shape[0] / dt1 / 1e6, 4), "cpu_model": _cpu_model(),
     }
 
 
@@ -185,14 +300,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="closest_hit", choices=["closest_hit"])
+    ap.add_argument("--workload", default="pt", choices=["pt", "closest_hit"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see the sharding design)",
               file=sys.stderr)
         sys.exit(2)
-    rec = closest_hit(args)
+    rec = path_tracing(args) if args.workload == "pt" else closest_hit(args)
     print(json.dumps(rec))
 
 

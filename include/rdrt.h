@@ -240,14 +240,23 @@ typedef struct rdrt_pt_params {
   float sky_up[3], sky_down[3];            /* miss radiance for direction.y > 0 / <= 0 */
   float sun_direction[3], sun_radiance[3]; /* directional light (direction of travel) */
   uint32_t use_graph;                      /* 1 = replay a captured hipGraph of the frame */
-  uint32_t pad[3];
+  uint32_t diagnostics;                    /* RDRT_PT_DIAG_* bits; any bit forces an eager frame */
+  uint32_t pad[2];
 } rdrt_pt_params;
+
+/* time every traversal launch of the frame with HIP events */
+#define RDRT_PT_DIAG_TIME_TRACES 0x1u
+/* trace in the reference order and count node / triangle / instance visits */
+#define RDRT_PT_DIAG_COUNT_VISITS 0x2u
 
 typedef struct rdrt_pt_stats {
   uint32_t bounces;             /* entries below that are valid: max_bounces + 1 */
   uint32_t extension_rays[17];  /* closest-hit rays traced at each bounce */
   uint32_t shadow_rays[17];     /* shadow rays traced after each bounce */
   float frame_ms;               /* HIP-event time of the last frame */
+  float trace_ms;               /* sum over its traversal launches (RDRT_PT_DIAG_TIME_TRACES) */
+  uint32_t trace_launches;
+  uint64_t visits[3];           /* node visits, triangle tests, instance tests (RDRT_PT_DIAG_COUNT_VISITS) */
 } rdrt_pt_stats;
 
 /* Renders one frame: radiance_dev receives width*height RGB f32 (the mean

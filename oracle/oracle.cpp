@@ -617,6 +617,33 @@ int orc_build(orc_scene *s) {
     uint32_t primitive_start, geometry_idx, flags;
   };
   std::vector<TriBvh> tri_bvh;
+  // Each geometry's tree depends on that geometry alone: build the trees
+  // concurrently, then assemble them in the reference order below.
+  std::vector<std::pair<size_t, size_t>> jobs;
+  for (size_t bi = 0; bi < s->blas_data.size(); bi++)
+    if (s->blas_live[bi])
+      for (size_t gi = 0; gi < s->blas_data[bi].size(); gi++) jobs.push_back({bi, gi});
+  std::vector<FlattenBVH> trees(jobs.size());
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int64_t j = 0; j < (int64_t)jobs.size(); j++) {
+    const Geometry &g = s->blas_data[jobs[j].first][jobs[j].second];
+    size_t nv = g.pos.size() / 3;
+    size_t ntri = g.indexed ? g.idx.size() / 3 : nv / 3;
+    std::vector<Box3> boxes(ntri);
+    for (size_t t = 0; t < ntri; t++) {
+      Box3 b = box_empty();
+      for (int k = 0; k < 3; k++) {
+        size_t vi = g.indexed ? g.idx[3 * t + k] : 3 * t + k;
+        const float *p = &g.pos[3 * vi];
+        expand_by_point(b, v3(p[0], p[1], p[2]));
+      }
+      boxes[t] = b;
+    }
+    trees[j] = flatten_bvh_new(boxes, blas_opt, nbins);
+  }
+  size_t next_tree = 0;
   for (size_t bi = 0; bi < s->blas_data.size(); bi++) {
     if (!s->blas_live[bi]) {
       s->blas_meta.push_back({0, 0});
@@ -639,16 +666,7 @@ int orc_build(orc_scene *s) {
         for (size_t v = 0; v < nv; v++) tri_idx[v] = (uint32_t)v;
       }
       size_t ntri = tri_idx.size() / 3;
-      std::vector<Box3> boxes(ntri);
-      for (size_t t = 0; t < ntri; t++) {
-        Box3 b = box_empty();
-        for (int k = 0; k < 3; k++) {
-          const float *p = &g.pos[3 * (size_t)tri_idx[3 * t + k]];
-          expand_by_point(b, v3(p[0], p[1], p[2]));
-        }
-        boxes[t] = b;
-      }
-      FlattenBVH bvh = flatten_bvh_new(boxes, blas_opt, nbins);
+      FlattenBVH bvh = std::move(trees[next_tree++]);
       if (bvh.depth > s->max_blas_depth) s->max_blas_depth = (uint32_t)bvh.depth;
       Box3 root_box = box_empty();
       expand_by_other(root_box, bvh.nodes[0].bounding);
@@ -1261,8 +1279,15 @@ int orc_set_triangle_materials(orc_scene *s, uint32_t blas, uint32_t geo, const 
 void orc_set_materials(orc_scene *s, const float *mats, uint32_t n) { s->materials.assign(mats, mats + 8 * (size_t)n); }
 
 int orc_pt_render(const orc_scene *s, const orc_pt_params *P, double *out, uint64_t *ray_counts, int nthreads) {
+  return orc_pt_render_rows(s, P, 0, P->height, out, ray_counts, nthreads);
+}
+
+int orc_pt_render_rows(const orc_scene *s, const orc_pt_params *P, uint32_t row_begin, uint32_t row_end, double *out,
+                       uint64_t *ray_counts, int nthreads) {
   if (!s->built || s->materials.empty() || P->spp == 0 || P->max_bounces > 16) return -1;
-  const uint32_t npix = P->width * P->height;
+  if (row_begin > row_end || row_end > P->height) return -1;
+  const uint32_t pix0 = row_begin * P->width;
+  const uint32_t npix = (row_end - row_begin) * P->width;
   const uint32_t nb = P->max_bounces + 1;
   std::vector<uint64_t> ext_all(nb, 0), sh_all(nb, 0);
 #ifdef _OPENMP
@@ -1276,7 +1301,8 @@ int orc_pt_render(const orc_scene *s, const orc_pt_params *P, double *out, uint6
 #endif
     for (int64_t pix = 0; pix < (int64_t)npix; pix++) {
       D3 acc = d3(0, 0, 0);
-      for (uint32_t k = 0; k < P->spp; k++) acc = acc + pt_path(s, *P, (uint32_t)pix * P->spp + k, ext.data(), sh.data());
+      const uint32_t gpix = pix0 + (uint32_t)pix;
+      for (uint32_t k = 0; k < P->spp; k++) acc = acc + pt_path(s, *P, gpix * P->spp + k, ext.data(), sh.data());
       out[3 * pix + 0] = acc.x / P->spp;
       out[3 * pix + 1] = acc.y / P->spp;
       out[3 * pix + 2] = acc.z / P->spp;

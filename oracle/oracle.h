@@ -157,8 +157,9 @@ typedef struct orc_pt_params {
   uint32_t max_bounces, rr_start, seed, sample_index_base;
   float sky_up[3], sky_down[3];
   float sun_direction[3], sun_radiance[3];
-  uint32_t use_graph;
-  uint32_t pad[3];
+  uint32_t use_graph;   /* ignored by the oracle */
+  uint32_t diagnostics; /* ignored by the oracle */
+  uint32_t pad[2];
 } orc_pt_params;
 
 int orc_set_vertex_normals(orc_scene *s, uint32_t blas, uint32_t geometry, const float *normals_xyz,
@@ -172,6 +173,9 @@ void orc_set_materials(orc_scene *s, const float *materials, uint32_t n);
  * shadow rays per bounce. */
 int orc_pt_render(const orc_scene *s, const orc_pt_params *params, double *out_rgb, uint64_t *ray_counts,
                   int nthreads);
+/* Rows [row_begin, row_end) of the same frame; out_rgb holds those rows. */
+int orc_pt_render_rows(const orc_scene *s, const orc_pt_params *params, uint32_t row_begin, uint32_t row_end,
+                       double *out_rgb, uint64_t *ray_counts, int nthreads);
 
 #ifdef __cplusplus
 }

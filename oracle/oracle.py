@@ -40,7 +40,7 @@ class OrcPtParams(C.Structure):
     _fields_ = [("view_projection_inv", f32 * 16), ("camera_position", f32 * 3), ("width", u32), ("height", u32),
                 ("spp", u32), ("max_bounces", u32), ("rr_start", u32), ("seed", u32), ("sample_index_base", u32),
                 ("sky_up", f32 * 3), ("sky_down", f32 * 3), ("sun_direction", f32 * 3), ("sun_radiance", f32 * 3),
-                ("use_graph", u32), ("pad", u32 * 3)]
+                ("use_graph", u32), ("diagnostics", u32), ("pad", u32 * 2)]
 
 
 _SIG = {
@@ -73,6 +73,7 @@ _SIG = {
     "orc_set_triangle_materials": (C.c_int, [P, u32, u32, P, u32]),
     "orc_set_materials": (None, [P, P, u32]),
     "orc_pt_render": (C.c_int, [P, C.POINTER(OrcPtParams), P, P, C.c_int]),
+    "orc_pt_render_rows": (C.c_int, [P, C.POINTER(OrcPtParams), u32, u32, P, P, C.c_int]),
 }
 
 _lib = None
@@ -238,6 +239,17 @@ class OracleScene:
         rc = self.L.orc_pt_render(self.h, C.byref(params), out.ctypes.data, counts.ctypes.data, threads)
         if rc != 0:
             raise RuntimeError("oracle pt_render failed")
+        return out, counts[:nb], counts[nb:]
+
+    def pt_render_rows(self, params: "OrcPtParams", row_begin: int, row_end: int, threads: int = 0):
+        """Rows [row_begin, row_end) of the frame: ((rows, width, 3) float64, ext counts, shadow counts)."""
+        out = np.zeros((row_end - row_begin, params.width, 3), dtype=np.float64)
+        nb = params.max_bounces + 1
+        counts = np.zeros(2 * nb, dtype=np.uint64)
+        rc = self.L.orc_pt_render_rows(self.h, C.byref(params), row_begin, row_end, out.ctypes.data,
+                                       counts.ctypes.data, threads)
+        if rc != 0:
+            raise RuntimeError("oracle pt_render_rows failed")
         return out, counts[:nb], counts[nb:]
 
     def trace_brute(self, rays: np.ndarray, flags=0, cull_mask=0xFFFFFFFF, tlas_idx=0, threads: int = 0):

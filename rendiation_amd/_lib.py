@@ -70,11 +70,16 @@ class RdrtPtParams(C.Structure):
     _fields_ = [("view_projection_inv", f32 * 16), ("camera_position", f32 * 3), ("width", u32), ("height", u32),
                 ("spp", u32), ("max_bounces", u32), ("rr_start", u32), ("seed", u32), ("sample_index_base", u32),
                 ("sky_up", f32 * 3), ("sky_down", f32 * 3), ("sun_direction", f32 * 3), ("sun_radiance", f32 * 3),
-                ("use_graph", u32), ("pad", u32 * 3)]
+                ("use_graph", u32), ("diagnostics", u32), ("pad", u32 * 2)]
 
 
 class RdrtPtStats(C.Structure):
-    _fields_ = [("bounces", u32), ("extension_rays", u32 * 17), ("shadow_rays", u32 * 17), ("frame_ms", f32)]
+    _fields_ = [("bounces", u32), ("extension_rays", u32 * 17), ("shadow_rays", u32 * 17), ("frame_ms", f32),
+                ("trace_ms", f32), ("trace_launches", u32), ("visits", C.c_uint64 * 3)]
+
+
+RDRT_PT_DIAG_TIME_TRACES = 0x1
+RDRT_PT_DIAG_COUNT_VISITS = 0x2
 
 
 # name -> (restype, argtypes); the full exported surface of rdrt.h

@@ -14,8 +14,12 @@ namespace rdrt {
 
 void PtState::free_all() {
   for (Scratch *s : {&rays, &shadow_rays, &hits, &shadow_hits, &hitx, &beta, &L, &nee, &qa, &qb, &qs, &counts,
-                     &lookback, &tickets, &materials})
+                     &lookback, &tickets, &materials, &diag_counters, &diag_totals})
     s->free();
+  for (hipEvent_t e : ev_begin) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ev_end) (void)hipEventDestroy(e);
+  ev_begin.clear();
+  ev_end.clear();
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   exec = nullptr;
@@ -44,7 +48,7 @@ int32_t reserve_pt(rdrt_ctx *c, uint32_t npaths) {
   RDRT_HIP(s.qb.reserve(n * 4));
   RDRT_HIP(s.qs.reserve(n * 4));
   size_t stride = (n + kPtBlock - 1) / kPtBlock;
-  RDRT_HIP(s.counts.reserve(2 * (kPtMaxBounces + 1) * 4));
+  RDRT_HIP(s.counts.reserve((2 * kPtMaxBounces + 3) * 4));
   RDRT_HIP(s.lookback.reserve((kPtMaxBounces + 1) * stride * 8));
   RDRT_HIP(s.tickets.reserve((kPtMaxBounces + 1) * 4));
   s.capacity = npaths;
@@ -177,8 +181,35 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
   const TraceParams tp = scene_trace_params(c);
   const uint32_t block = c->cfg.block_size ? c->cfg.block_size : 256;
   hipStream_t st = c->work_stream;
+  if (prm->diagnostics & ~(RDRT_PT_DIAG_TIME_TRACES | RDRT_PT_DIAG_COUNT_VISITS))
+    return fail(RDRT_ERR_INVALID_ARG, "unknown diagnostics bits");
+  PtDiag diag;
+  if (prm->diagnostics) {
+    const size_t launches = 2 * (kPtMaxBounces + 1);
+    if (prm->diagnostics & RDRT_PT_DIAG_TIME_TRACES) {
+      while (s.ev_begin.size() < launches) {
+        hipEvent_t a, b;
+        RDRT_HIP(hipEventCreate(&a));
+        RDRT_HIP(hipEventCreate(&b));
+        s.ev_begin.push_back(a);
+        s.ev_end.push_back(b);
+      }
+      diag.ev_begin = s.ev_begin.data();
+      diag.ev_end = s.ev_end.data();
+    }
+    if (prm->diagnostics & RDRT_PT_DIAG_COUNT_VISITS) {
+      RDRT_HIP(s.diag_counters.reserve((size_t)npaths * 12));
+      RDRT_HIP(s.diag_totals.reserve(3 * 8));
+      diag.counters = (uint32_t *)s.diag_counters.ptr;
+      diag.totals = (unsigned long long *)s.diag_totals.ptr;
+    }
+  }
+  s.last_diag = prm->diagnostics;
+  s.last_launches = 2 * prm->max_bounces + 1;
   RDRT_HIP(hipEventRecord(c->ev0, st));
-  if (prm->use_graph) {
+  if (prm->diagnostics) {
+    RDRT_HIP(enqueue_pt_frame(L, tp, push_depth(c), block, st, &diag));
+  } else if (prm->use_graph) {
     if (!s.exec || !s.have_cached || std::memcmp(&s.cached, &L, sizeof(L)) != 0) {
       if (s.exec) (void)hipGraphExecDestroy(s.exec);
       if (s.graph) (void)hipGraphDestroy(s.graph);
@@ -214,14 +245,24 @@ int32_t rdrt_pt_get_stats(rdrt_ctx *c, rdrt_pt_stats *o) {
   RDRT_HIP(hipStreamSynchronize(c->work_stream));
   std::memset(o, 0, sizeof(*o));
   uint32_t mb = c->pt.last_bounces;
-  uint32_t counts[2 * (kPtMaxBounces + 1)];
-  RDRT_HIP(hipMemcpy(counts, c->pt.counts.ptr, 2 * (mb + 1) * 4, hipMemcpyDeviceToHost));
+  uint32_t counts[2 * kPtMaxBounces + 3];
+  RDRT_HIP(hipMemcpy(counts, c->pt.counts.ptr, (2 * mb + 3) * 4, hipMemcpyDeviceToHost));
   o->bounces = mb + 1;
   for (uint32_t b = 0; b <= mb; b++) {
     o->extension_rays[b] = counts[b];
-    o->shadow_rays[b] = b < mb ? counts[mb + 1 + b] : 0u;
+    o->shadow_rays[b] = b < mb ? counts[mb + 2 + b] : 0u;
   }
   RDRT_HIP(hipEventElapsedTime(&o->frame_ms, c->ev0, c->ev1));
+  if (c->pt.last_diag & RDRT_PT_DIAG_TIME_TRACES) {
+    o->trace_launches = c->pt.last_launches;
+    for (uint32_t i = 0; i < c->pt.last_launches; i++) {
+      float ms = 0.0f;
+      RDRT_HIP(hipEventElapsedTime(&ms, c->pt.ev_begin[i], c->pt.ev_end[i]));
+      o->trace_ms += ms;
+    }
+  }
+  if (c->pt.last_diag & RDRT_PT_DIAG_COUNT_VISITS)
+    RDRT_HIP(hipMemcpy(o->visits, c->pt.diag_totals.ptr, 24, hipMemcpyDeviceToHost));
   return RDRT_OK;
 }
 

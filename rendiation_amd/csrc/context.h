@@ -92,6 +92,10 @@ struct PtState {
   bool have_cached = false;
   uint32_t last_bounces = 0;
   float last_ms = 0.0f;
+  // diagnostics (eager frames)
+  std::vector<hipEvent_t> ev_begin, ev_end;
+  Scratch diag_counters, diag_totals;
+  uint32_t last_diag = 0, last_launches = 0;
   void free_all();
 };
 

@@ -39,7 +39,7 @@ struct PtLaunch {
   uint2 *hitx;  // (global geometry, forest instance) of each hit
   float *beta, *L, *nee;
   uint32_t *qa, *qb, *qs;  // extension queues (ping-pong) and the shadow queue
-  uint32_t *counts;        // [0..max_bounces+1]: extension queue sizes; then shadow sizes per bounce
+  uint32_t *counts;        // [0..max_bounces+1]: extension queue sizes; [max_bounces+2+b]: shadow queue sizes
   unsigned long long *lookback;  // per bounce: one status word per block
   uint32_t *tickets;             // per bounce: block ticket counter
   uint32_t lookback_stride;
@@ -47,9 +47,17 @@ struct PtLaunch {
   float *out;  // width * height * 3
 };
 
-// Enqueues one frame on `stream` (capturable: no memset nodes).
+// Optional per-frame diagnostics (eager frames only).
+struct PtDiag {
+  hipEvent_t *ev_begin = nullptr, *ev_end = nullptr;  // one pair per traversal launch, or null
+  uint32_t *counters = nullptr;                       // 3 u32 per path: reference-order visits, or null
+  unsigned long long *totals = nullptr;               // 3 u64 sums of the counters
+};
+
+// Enqueues one frame on `stream` (capturable when diag is null: no memset
+// nodes, no host round trips).
 hipError_t enqueue_pt_frame(const PtLaunch &L, const TraceParams &scene_tp, uint32_t push_depth, uint32_t block,
-                            hipStream_t stream);
+                            hipStream_t stream, const PtDiag *diag = nullptr);
 
 // Stable stream compaction of indices whose mask is non-zero (single pass,
 // decoupled look-back); out_count receives the kept count.  `scratch` holds

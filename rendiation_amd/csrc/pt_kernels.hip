@@ -152,7 +152,7 @@ __device__ __forceinline__ void write_ray(rdrt_ray *r, V3 o, float tmin, V3 d, f
 }
 
 __global__ void k_pt_begin(PtLaunch P) {
-  const uint32_t nc = 2 * (P.max_bounces + 1);
+  const uint32_t nc = 2 * P.max_bounces + 3;
   const uint32_t nl = (P.max_bounces + 1) * P.lookback_stride;
   const uint32_t n = nl > nc ? nl : nc;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -275,13 +275,13 @@ __global__ __launch_bounds__(B) void k_pt_shade(PtLaunch P, uint32_t b) {
     if (path < P.npaths) shade_one(P, b, path, keep, shadow);
   }
   compact2<B>(keep, shadow, path, qout, P.qs, P.lookback + (size_t)b * P.lookback_stride, bid, nblocks,
-              P.counts + b + 1, P.counts + P.max_bounces + 1 + b, P.status, P.npaths);
+              P.counts + b + 1, P.counts + P.max_bounces + 2 + b, P.status, P.npaths);
 }
 
 // add the light of unoccluded shadow rays
 __global__ __launch_bounds__(kPtBlock) void k_pt_nee(PtLaunch P, uint32_t b) {
   const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
-  if (i >= P.counts[P.max_bounces + 1 + b] || i >= P.npaths) return;
+  if (i >= P.counts[P.max_bounces + 2 + b] || i >= P.npaths) return;
   const uint32_t path = P.qs[i];
   if (P.shadow_hits[path].prim_id != kMiss) return;
   V3 L = load3(P.L + 3 * (size_t)path) + load3(P.nee + 3 * (size_t)path);
@@ -303,6 +303,22 @@ __global__ void k_fill_u64(u64 *p, uint32_t n) {
 }
 
 __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
+
+// Sums the per-ray visit counters of the rays of one queued launch.
+__global__ __launch_bounds__(kPtBlock) void k_sum_visits(const uint32_t *queue, const uint32_t *count, uint32_t n,
+                                                         const uint32_t *counters, u64 *totals) {
+  __shared__ u64 s[3];
+  if (threadIdx.x < 3) s[threadIdx.x] = 0ull;
+  __syncthreads();
+  const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+  if (i < n && i < *count) {
+    const uint32_t r = queue[i];
+    if (r < n)
+      for (int k = 0; k < 3; k++) atomicAdd(&s[k], (u64)counters[3 * (size_t)r + k]);
+  }
+  __syncthreads();
+  if (threadIdx.x < 3 && s[threadIdx.x]) atomicAdd(&totals[threadIdx.x], s[threadIdx.x]);
+}
 
 __global__ __launch_bounds__(kPtBlock) void k_compact(const uint32_t *mask, uint32_t n, uint32_t *out,
                                                       uint32_t *out_count, u64 *status, uint32_t *ticket,
@@ -327,8 +343,24 @@ __global__ void k_sobol(const uint32_t *index, const uint32_t *scr, uint32_t n, 
 }  // namespace
 
 hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_t push_depth, uint32_t block,
-                            hipStream_t st) {
+                            hipStream_t st, const PtDiag *diag) {
   const uint32_t grid = (P.npaths + kPtBlock - 1) / kPtBlock;
+  const bool count = diag && diag->counters;
+  const uint32_t mode = count ? RDRT_TRAVERSAL_REFERENCE : RDRT_TRAVERSAL_FAST;
+  uint32_t launch = 0;
+  // one traversal launch, optionally timed and visit-counted
+  auto trace = [&](TraceParams t) -> hipError_t {
+    if (count) t.counters = diag->counters;
+    if (diag && diag->ev_begin) (void)hipEventRecord(diag->ev_begin[launch], st);
+    hipError_t e = launch_trace(t, mode, push_depth, block, st);
+    if (diag && diag->ev_end) (void)hipEventRecord(diag->ev_end[launch], st);
+    launch++;
+    if (e == hipSuccess && count)
+      hipLaunchKernelGGL(k_sum_visits, dim3((t.n + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, st, t.queue, t.count,
+                         t.n, diag->counters, diag->totals);
+    return e;
+  };
+  if (count) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, diag->totals, 3u);
   hipLaunchKernelGGL(k_pt_begin, dim3(256), dim3(256), 0, st, P);
   hipLaunchKernelGGL(k_pt_camera, dim3(grid), dim3(kPtBlock), 0, st, P);
   for (uint32_t b = 0; b <= P.max_bounces; b++) {
@@ -345,7 +377,7 @@ hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_
     tp.flags = RDRT_RAY_FLAG_CULL_BACK_FACING_TRIANGLES;  // ray_gen.rs:162-166
     tp.cull_mask = 0xffffffffu;
     tp.tlas_idx = 0;
-    hipError_t e = launch_trace(tp, RDRT_TRAVERSAL_FAST, push_depth, block, st);
+    hipError_t e = trace(tp);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_pt_shade<kPtBlock>), dim3(grid), dim3(kPtBlock), 0, st, P, b);
     if (b == P.max_bounces) break;
@@ -354,9 +386,9 @@ hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_
     sp.hits = P.shadow_hits;
     sp.hitx = nullptr;
     sp.queue = P.qs;
-    sp.count = P.counts + P.max_bounces + 1 + b;
+    sp.count = P.counts + P.max_bounces + 2 + b;
     sp.flags = RDRT_RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH;  // ray_hit.rs:33-45
-    e = launch_trace(sp, RDRT_TRAVERSAL_FAST, push_depth, block, st);
+    e = trace(sp);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_pt_nee, dim3(grid), dim3(kPtBlock), 0, st, P, b);
   }

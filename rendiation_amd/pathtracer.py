@@ -88,6 +88,7 @@ class PathTracingConfig:
     sun_direction: Sequence[float] = (-0.27, -0.92, -0.28)
     sun_radiance: Sequence[float] = (3.0, 3.0, 3.0)
     use_graph: bool = True
+    diagnostics: int = 0  # L.RDRT_PT_DIAG_* bits (eager frame)
 
     def to_c(self, camera: Camera) -> L.RdrtPtParams:
         p = L.RdrtPtParams()
@@ -103,6 +104,7 @@ class PathTracingConfig:
         p.sun_direction[:] = [float(x) for x in sd]
         p.sun_radiance[:] = [float(x) for x in self.sun_radiance]
         p.use_graph = 1 if self.use_graph else 0
+        p.diagnostics = self.diagnostics
         return p
 
 
@@ -130,7 +132,8 @@ class PathTracer:
         L.check(self.system._L.rdrt_pt_get_stats(self.system._h, C.byref(s)))
         nb = s.bounces
         return {"extension_rays": list(s.extension_rays[:nb]), "shadow_rays": list(s.shadow_rays[:nb]),
-                "frame_ms": float(s.frame_ms)}
+                "frame_ms": float(s.frame_ms), "trace_ms": float(s.trace_ms), "trace_launches": int(s.trace_launches),
+                "visits": [int(x) for x in s.visits]}
 
 
 def load_scene(system: AccelerationStructureSystem, scene: dict) -> None:
