@@ -46,39 +46,55 @@ struct Pair {
 
 __device__ __forceinline__ u64 pack(u64 flag, Pair v) { return (flag << 62) | ((u64)v.a << 31) | (u64)v.b; }
 
-// Decoupled look-back for block `bid` (called by one thread): publishes the
-// block's aggregate, walks back to the nearest inclusive prefix, publishes its
-// own inclusive prefix and returns the exclusive one.  Status word: bits
-// 63-62 flag (0 empty, 1 aggregate, 2 inclusive), 61-31 count a, 30-0 count b.
-__device__ Pair lookback(u64 *status, uint32_t bid, Pair total, uint32_t *err) {
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Decoupled look-back for block `bid`, run by the 64 lanes of one wave:
+// publishes the block's aggregate, then inspects 64 predecessors per step
+// (lane l reads block base - l), sums the aggregates up to the nearest
+// inclusive prefix, publishes its own inclusive prefix and returns the
+// exclusive one to every lane.  Status word: bits 63-62 flag (0 empty,
+// 1 aggregate, 2 inclusive), 61-31 count a, 30-0 count b.
+__device__ Pair lookback_wave(u64 *status, uint32_t bid, Pair total, uint32_t *err) {
+  const uint32_t lane = threadIdx.x & 63u;
   Pair excl{0u, 0u};
   if (bid == 0) {
-    __hip_atomic_store(status, pack(2, total), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane == 0) __hip_atomic_store(status, pack(2, total), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     return excl;
   }
-  __hip_atomic_store(status + bid, pack(1, total), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  int32_t j = (int32_t)bid - 1;
+  if (lane == 0) __hip_atomic_store(status + bid, pack(1, total), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  int32_t base = (int32_t)bid - 1;
   uint32_t spins = 0;
-  while (j >= 0) {
-    u64 s = __hip_atomic_load(status + j, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    u64 flag = s >> 62;
-    if (flag == 0) {
+  for (;;) {
+    const int32_t j = base - (int32_t)lane;
+    // before block 0 reads as an inclusive zero
+    const u64 s = j >= 0 ? __hip_atomic_load(status + j, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : pack(2, Pair{0, 0});
+    const uint32_t flag = (uint32_t)(s >> 62);
+    const u64 incl = __ballot(flag == 2);
+    const u64 empty = __ballot(flag == 0);
+    const u64 low = incl & (~incl + 1ull);  // nearest inclusive predecessor
+    const u64 upto = incl ? (low | (low - 1ull)) : ~0ull;
+    if (empty & upto) {
       // bounded wait: every predecessor already holds its ticket and publishes
       // its aggregate without waiting; the limit only guards against a bug
       if (++spins > kSpinLimit) {
-        atomicOr(err, 2u);
+        if (lane == 0) atomicOr(err, 2u);
         break;
       }
       __builtin_amdgcn_s_sleep(1);
       continue;
     }
-    excl.a += (uint32_t)((s >> 31) & 0x7fffffffu);
-    excl.b += (uint32_t)(s & 0x7fffffffu);
-    if (flag == 2) break;
-    j--;
+    const bool mine = (upto >> lane) & 1ull;
+    excl.a += wave_sum(mine ? (uint32_t)((s >> 31) & 0x7fffffffu) : 0u);
+    excl.b += wave_sum(mine ? (uint32_t)(s & 0x7fffffffu) : 0u);
+    if (incl) break;
+    base -= 64;
   }
-  __hip_atomic_store(status + bid, pack(2, Pair{excl.a + total.a, excl.b + total.b}), __ATOMIC_RELEASE,
-                     __HIP_MEMORY_SCOPE_AGENT);
+  if (lane == 0)
+    __hip_atomic_store(status + bid, pack(2, Pair{excl.a + total.a, excl.b + total.b}), __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_AGENT);
   return excl;
 }
 
@@ -100,26 +116,30 @@ __device__ __forceinline__ void compact2(bool ka, bool kb, uint32_t item, uint32
     s_wb[wave] = (uint32_t)__popcll(mb);
   }
   __syncthreads();
-  if (tid == 0) {
-    uint32_t ta = 0, tb = 0;
-    for (int w = 0; w < B / 64; w++) {
-      uint32_t x = s_wa[w], y = s_wb[w];
-      s_wa[w] = ta;
-      s_wb[w] = tb;
-      ta += x;
-      tb += y;
+  // every thread derives its wave's offset and the block totals
+  uint32_t pa = 0, pb = 0, ta = 0, tb = 0;
+  for (uint32_t w = 0; w < B / 64; w++) {
+    if (w == wave) {
+      pa = ta;
+      pb = tb;
     }
-    Pair e = lookback(status, bid, Pair{ta, tb}, err);
-    s_ea = e.a;
-    s_eb = e.b;
-    if (bid == nblocks - 1) {
-      if (total_a) *total_a = e.a + ta;
-      if (total_b) *total_b = e.b + tb;
+    ta += s_wa[w];
+    tb += s_wb[w];
+  }
+  if (wave == 0) {
+    Pair e = lookback_wave(status, bid, Pair{ta, tb}, err);
+    if (lane == 0) {
+      s_ea = e.a;
+      s_eb = e.b;
+      if (bid == nblocks - 1) {
+        if (total_a) *total_a = e.a + ta;
+        if (total_b) *total_b = e.b + tb;
+      }
     }
   }
   __syncthreads();
   // the bound only matters if a look-back gave up (error flag set)
-  const uint32_t ia = s_ea + s_wa[wave] + ra, ib = s_eb + s_wb[wave] + rb;
+  const uint32_t ia = s_ea + pa + ra, ib = s_eb + pb + rb;
   if (ka && ia < capacity) out_a[ia] = item;
   if (kb && ib < capacity) out_b[ib] = item;
 }

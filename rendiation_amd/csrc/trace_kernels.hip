@@ -197,17 +197,44 @@ __device__ __forceinline__ void write_result(const TraceParams &p, uint32_t rid,
   }
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2).
+// Remap the first `used` physical blocks so each XCD works on one contiguous
+// range of rays: neighbouring (coherent) rays then share an L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t i, uint32_t used) {
+  const uint32_t xcd = i % 8u, local = i / 8u, per = used / 8u, rem = used % 8u;
+  return xcd < rem ? xcd * (per + 1u) + local : rem * (per + 1u) + (xcd - rem) * per + local;
+}
+
 // ------------------------------------------------------------------------
-// Fast kernel: front-to-back child-pair traversal, LDS stack of S entries.
+// Fast kernel: front-to-back child-pair traversal.  Stack: the first K
+// entries of each lane live in LDS (column layout), deeper ones (rare) in
+// the lane's private memory; S bounds the total.
 // ------------------------------------------------------------------------
-template <int S, int B>
+template <int K, int S, int B, bool CNT>
 __global__ __launch_bounds__(B) void k_trace_fast(TraceParams p) {
-  __shared__ uint32_t lds[S * B];
-  const uint32_t slot = blockIdx.x * B + threadIdx.x;
-  if (slot >= p.n || (p.count && slot >= *p.count)) return;
+  __shared__ uint32_t lds[K * B];
+  uint32_t spill[S > K ? S - K : 1];
+  const uint32_t n = p.count ? min(p.n, *p.count) : p.n;
+  const uint32_t used = (n + B - 1) / B;
+  if (blockIdx.x >= used) return;
+  const uint32_t slot = xcd_block(blockIdx.x, used) * B + threadIdx.x;
+  if (slot >= n) return;
   const uint32_t rid = p.queue ? p.queue[slot] : slot;
   if (rid >= p.n) return;  // a queue entry always names one of the n rays
-  uint32_t *stk = lds + threadIdx.x;  // entry e at stk[e * B]
+  uint32_t *stk = lds + threadIdx.x;  // entry e < K at stk[e * B]
+  auto push = [&](uint32_t &sp, uint32_t v) {
+    if (sp < (uint32_t)K) stk[sp * B] = v;
+    else if (sp < (uint32_t)S) spill[sp - K] = v;
+    else {
+      atomicOr(p.status, 1u);  // unreachable when S covers the built depth
+      return;
+    }
+    sp++;
+  };
+  auto pop = [&](uint32_t &sp) -> uint32_t {
+    sp--;
+    return sp < (uint32_t)K ? stk[sp * B] : spill[sp - K];
+  };
 
   RayIn r = load_ray(p, rid);
   float far_ = r.tmax;
@@ -232,12 +259,7 @@ __global__ __launch_bounds__(B) void k_trace_fast(TraceParams p) {
         uint32_t lref = __float_as_uint(a.w), rref = __float_as_uint(c.w);
         if (hl && hr) {
           bool rfirst = tr < tl;
-          if (sp < S) {
-            stk[sp * B] = rfirst ? lref : rref;
-            sp++;
-          } else {
-            atomicOr(p.status, 1u);  // unreachable when S covers the built depth
-          }
+          push(sp, rfirst ? lref : rref);
           ref = rfirst ? rref : lref;
           continue;
         }
@@ -248,7 +270,7 @@ __global__ __launch_bounds__(B) void k_trace_fast(TraceParams p) {
       } else {
         const uint32_t i0 = (ref & ~kLeafBit) >> kLeafCountBits, i1 = i0 + (ref & kLeafMaxCount);
         for (uint32_t i = i0; i < i1 && !done; i++) {
-          ci++;
+          if constexpr (CNT) ci++;
           const InstRec &in = p.insts[i];
           float ti;
           if (!slab(r.o, winv, r.tmin, far_, in.mn[0], in.mn[1], in.mn[2], in.mx[0], in.mx[1], in.mx[2], ti))
@@ -275,7 +297,7 @@ __global__ __launch_bounds__(B) void k_trace_fast(TraceParams p) {
             uint32_t bref = geo.root_ref;
             for (;;) {
               if (!(bref & kLeafBit)) {
-                cn++;
+                if constexpr (CNT) cn++;
                 const float4 *q = reinterpret_cast<const float4 *>(p.tri_pairs + bref);
                 float4 a = q[0], b = q[1], c = q[2], d = q[3];
                 const float fs = far_ * s;
@@ -285,12 +307,7 @@ __global__ __launch_bounds__(B) void k_trace_fast(TraceParams p) {
                 uint32_t lref = __float_as_uint(a.w), rref = __float_as_uint(c.w);
                 if (hl && hr) {
                   bool rfirst = tr < tl;
-                  if (bsp < S) {
-                    stk[bsp * B] = rfirst ? lref : rref;
-                    bsp++;
-                  } else {
-                    atomicOr(p.status, 1u);  // unreachable when S covers the built depth
-                  }
+                  push(bsp, rfirst ? lref : rref);
                   bref = rfirst ? rref : lref;
                   continue;
                 }
@@ -301,7 +318,7 @@ __global__ __launch_bounds__(B) void k_trace_fast(TraceParams p) {
               } else {
                 const uint32_t k0 = (bref & ~kLeafBit) >> kLeafCountBits, k1 = k0 + (bref & kLeafMaxCount);
                 for (uint32_t k = k0; k < k1; k++) {
-                  ct++;
+                  if constexpr (CNT) ct++;
                   const float4 *q = reinterpret_cast<const float4 *>(p.tris + k);
                   float4 a = q[0], b = q[1], c = q[2], d = q[3];
                   float t, u, v;
@@ -325,16 +342,14 @@ __global__ __launch_bounds__(B) void k_trace_fast(TraceParams p) {
                 if (done) break;
               }
               if (bsp == sp) break;
-              bsp--;
-              bref = stk[bsp * B];
+              bref = pop(bsp);
             }
           }
         }
         if (done) break;
       }
       if (sp == 0) break;
-      sp--;
-      ref = stk[sp * B];
+      ref = pop(sp);
     }
   }
   write_result(p, rid, best, cn, ct, ci);
@@ -440,10 +455,14 @@ __global__ __launch_bounds__(B) void k_trace_ref(TraceParams p) {
   write_result(p, rid, best, cn, ct, ci);
 }
 
+constexpr int kLdsEntries = 16;  // stack entries per lane kept in LDS
+
 template <int S, int B>
 hipError_t launch_fast_b(const TraceParams &p, hipStream_t st) {
+  constexpr int K = S < kLdsEntries ? S : kLdsEntries;
   dim3 grid((p.n + B - 1) / B);
-  hipLaunchKernelGGL((k_trace_fast<S, B>), grid, dim3(B), 0, st, p);
+  if (p.counters) hipLaunchKernelGGL((k_trace_fast<K, S, B, true>), grid, dim3(B), 0, st, p);
+  else hipLaunchKernelGGL((k_trace_fast<K, S, B, false>), grid, dim3(B), 0, st, p);
   return hipGetLastError();
 }
 
@@ -475,7 +494,6 @@ hipError_t launch_trace(const TraceParams &p, uint32_t mode, uint32_t push_depth
     return hipGetLastError();
   }
   uint32_t s = stack_entries_for(push_depth);
-  if (block > 128 && s == 64) block = 128;  // keep LDS per block <= 32 KiB
   switch (s) {
     case 8: return launch_fast_s<8>(p, block, stream);
     case 16: return launch_fast_s<16>(p, block, stream);
