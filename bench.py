@@ -1,6 +1,6 @@
 """Benchmark of the ray-query hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pt|closest_hit]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pt|closest_hit|batched]
 
 pt (default): wavefront path tracer on the parametric scene -- 64 UV spheres
   and tori tessellated at 256x256 (8,388,608 triangles, one BLAS each, 64
@@ -11,6 +11,10 @@ pt (default): wavefront path tracer on the parametric scene -- 64 UV spheres
   traced in a frame (closest-hit + shadow) / frame time.
 closest_hit: 100k random triangles (SAH 4 bins, leaf <= 4), 1M rays, seed 0;
   one step = one closest-hit launch.
+batched: 64 random-triangle scenes (triangles log-uniform in [10k, 2M), rays
+  log-uniform in [16k, 4M)), 3 bounces of cosine-weighted continuation rays
+  with per-bounce compaction; one step = one graph-replayed persistent call
+  over all scenes, compared with the per-scene sequential launches.
 
 roofline: HBM-bound.  Algorithmic bytes per traced ray = 32 (ray) + 16 (hit)
 + 48 x node visits + 52 x triangle tests + 176 x instance tests, with visit
@@ -295,19 +299,168 @@ def cpu_baseline_closest_hit(pos, rays_np):
     }
 
 
+# ------------------------------------------------------------------ batched
+def batched(args):
+    import numpy as np
+    import torch
+
+    from rendiation_amd import synth
+    from rendiation_amd.acceleration import (
+        BATCH_PERSISTENT,
+        BATCH_SEQUENTIAL,
+        TRAVERSAL_REFERENCE,
+        AccelerationStructureSystem,
+        BuildOptions,
+        TopLevelAccelerationStructureSourceInstance,
+        TriangleGeometry,
+    )
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    t0 = time.perf_counter()
+    scenes, rays_np, meta_np, offs = synth.batched_scenes(0, synth.BATCHED_SCENES, synth.BATCHED_TRIS,
+                                                          synth.BATCHED_RAYS)
+    gen_ms = (time.perf_counter() - t0) * 1e3
+    n = rays_np.shape[0]
+    bounces = synth.BATCHED_BOUNCES
+    system = AccelerationStructureSystem(device=0, options=BuildOptions(blas_bin_size=synth.CLOSEST_HIT_LEAF),
+                                         stream=stream.cuda_stream)
+    tl = []
+    for v in scenes:
+        b = system.create_bottom_level_acceleration_structure([TriangleGeometry(v)])
+        tl.append(system.create_top_level_acceleration_structure(
+            [TopLevelAccelerationStructureSourceInstance(np.eye(4), b)]))
+    system.bind_tlas(tl)
+    t0 = time.perf_counter()
+    system.create_comp_instance()
+    build_upload_ms = (time.perf_counter() - t0) * 1e3
+    info = system.scene_info()
+    with torch.cuda.stream(stream):
+        rays = torch.from_numpy(rays_np).to(dev)
+        meta = torch.from_numpy(meta_np.view(np.int32)).to(dev)
+        hits = torch.empty((bounces + 1) * n * 4, dtype=torch.int32, device=dev)
+        spawned = torch.empty(bounces * n * 8, dtype=torch.float32, device=dev)
+    stream.synchronize()
+
+    def run(mode, steps, record=False):
+        for _ in range(steps):
+            system.trace_batched(rays, meta, hits, bounces=bounces, seed=0, spawned=spawned if record else None,
+                                 scene_offsets=offs if mode == BATCH_SEQUENTIAL else None, mode=mode)
+
+    def timed(mode):
+        run(mode, args.warmup)
+        system.sync()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        w0 = time.perf_counter()
+        e0.record(stream)
+        run(mode, args.steps)
+        e1.record(stream)
+        system.sync()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - w0) * 1e3 / args.steps, e0.elapsed_time(e1) / args.steps, system.batch_stats()
+
+    ms_p, ev_p, st_p = timed(BATCH_PERSISTENT)
+    hits_p = hits.view(torch.int64).sum().item()
+    ms_s, ev_s, st_s = timed(BATCH_SEQUENTIAL)
+    same = hits.view(torch.int64).sum().item() == hits_p
+    total = int(sum(st_p.rays))
+    value = total / (ms_p * 1e-3) / 1e6
+
+    # visit counts of the reference order on every round's rays
+    run(BATCH_PERSISTENT, 1, record=True)
+    system.sync()
+    cnt = torch.empty((n, 3), dtype=torch.int32, device=dev)
+    hh = torch.empty((n, 4), dtype=torch.int32, device=dev)
+    visits = np.zeros(3, dtype=np.int64)
+    hv = hits.view(bounces + 1, n, 4)
+    sv = spawned.view(bounces, n, 8)
+    for r in range(bounces + 1):
+        live = torch.ones(n, dtype=torch.bool, device=dev) if r == 0 else (hv[r - 1, :, 0] != -1)
+        idx = torch.nonzero(live).flatten()
+        if idx.numel() == 0:
+            break
+        src = rays if r == 0 else sv[r - 1]
+        rr = src[idx].contiguous()
+        mm = meta[idx].contiguous()
+        system.trace(rr.view(-1), hh[:idx.numel()].view(-1), ray_meta=mm.view(-1),
+                     counters=cnt[:idx.numel()].view(-1), traversal=TRAVERSAL_REFERENCE)
+        system.sync()
+        visits += cnt[:idx.numel()].to(torch.int64).sum(dim=0).cpu().numpy()
+    bytes_call = _bytes(total, visits)
+    achieved = bytes_call / (ev_p * 1e-3) / 1e9
+    config = {
+        "workload": "batched: 64 random-triangle scenes (tris log-U[10k,2M), rays log-U[16k,4M)), 3 bounces, seed 0",
+        "scenes": len(scenes), "tris": info.tris, "primary_rays": n, "rays_per_round": st_p.rays,
+        "rays_per_call": total, "launches_persistent": st_p.launches, "launches_sequential": st_s.launches,
+        "ms_persistent_events": round(ev_p, 4), "ms_sequential": round(ms_s, 4),
+        "ms_sequential_events": round(ev_s, 4), "sequential_mrays": round(total / (ms_s * 1e-3) / 1e6, 3),
+        "speedup_vs_sequential": round(ms_s / ms_p, 4), "sequential_equal": bool(same),
+        "scene_gen_ms": round(gen_ms, 1), "build_upload_ms": round(build_upload_ms, 1),
+        "device_scene_mb": round(info.device_bytes / 2 ** 20, 1),
+    }
+    roofline = {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+        "kernel": "k_trace_persist + k_spawn (whole batched call, graph replay)",
+        "bytes_per_call": bytes_call, "visits_per_ray": [round(float(v) / total, 3) for v in visits],
+    }
+    extra = {}
+    if not args.no_cpu_baseline:
+        extra["cpu_baseline"] = cpu_baseline_batched(scenes, rays_np, meta_np, offs, bounces)
+    system.close()
+    return _record(value, "Mrays/s", args, ms_p, config, roofline, extra)
+
+
+def cpu_baseline_batched(scenes, rays_np, meta_np, offs, bounces):
+    """The oracle's batch rounds (trace + FP64 spawn, OpenMP over rays) on the
+    first 4 scenes with at most 262,144 rays each."""
+    import numpy as np
+
+    from oracle.oracle import OracleScene
+    from rendiation_amd import synth
+
+    k = 4
+    orc = OracleScene(blas_bin_size=synth.CLOSEST_HIT_LEAF)
+    tl = []
+    for v in scenes[:k]:
+        b = orc.add_blas([(v, None, 1)])
+        tl.append(orc.add_tlas([{"transform": np.eye(4), "blas": b}]))
+    orc.bind(tl)
+    orc.build()
+    sel = np.concatenate([np.arange(offs[i], min(offs[i + 1], offs[i] + 262_144)) for i in range(k)])
+    rays, meta = rays_np[sel].copy(), meta_np[sel].copy()
+    threads = _host_threads()
+    traced = 0
+    t0 = time.perf_counter()
+    live = np.ones(rays.shape[0], dtype=bool)
+    for r in range(bounces + 1):
+        traced += int(live.sum())
+        hits, sp, alive = orc.batch_round(rays, meta, r + 1, 0, threads=threads)
+        live = live & (alive != 0)
+        rays = sp
+        rays[~live] = np.array([0, 0, 0, 1, 0, 0, 1, 0], dtype=np.float32)
+    dt = time.perf_counter() - t0
+    return {"value": round(traced / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"scenes 0-3, first <=262144 rays each ({sel.size} paths, {traced} rays over {bounces + 1} "
+                      "rounds); ended paths still cost one empty-range query per round",
+            "cpu_model": _cpu_model()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt", choices=["pt", "closest_hit"])
+    ap.add_argument("--workload", default="pt", choices=["pt", "closest_hit", "batched"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see the sharding design)",
               file=sys.stderr)
         sys.exit(2)
-    rec = path_tracing(args) if args.workload == "pt" else closest_hit(args)
+    rec = {"pt": path_tracing, "closest_hit": closest_hit, "batched": batched}[args.workload](args)
     print(json.dumps(rec))
 
 

@@ -265,6 +265,37 @@ int32_t rdrt_pt_render(rdrt_ctx *ctx, const rdrt_pt_params *params, float *radia
 /* Ray counts of the last frame (synchronizes). */
 int32_t rdrt_pt_get_stats(rdrt_ctx *ctx, rdrt_pt_stats *out);
 
+/* ---- batched tracing: many scenes and uneven ray sets in one queue ---- */
+/* Each ray's rdrt_ray_meta.tlas_idx selects its scene among the bound TLAS
+ * (the reference's per-ray TLAS selection, traverse_cpu.rs:63).  Round 0
+ * traces the given rays; each later round traces, for every ray that hit, a
+ * new ray from the offset hit point (ray_util.rs:6-40) in a cosine-weighted
+ * direction around the geometric normal facing the ray, sampled from the
+ * ray's Sobol (0,2)-sequence at the round index, XOR-scrambled by
+ * xxhash32(ray index, seed, 2 / 3).  Survivors are compacted (stable).
+ * A ray that misses in round r gets miss records in every later round. */
+#define RDRT_BATCH_PERSISTENT 0u /* one persistent launch per round over all scenes, graph-captured */
+#define RDRT_BATCH_SEQUENTIAL 1u /* per scene, per round: plain launches (baseline) */
+
+typedef struct rdrt_batch_params {
+  const rdrt_ray *rays;          /* n primary rays (device), grouped by scene */
+  const rdrt_ray_meta *ray_meta; /* n records (device): flags, cull mask, tlas_idx = scene */
+  rdrt_hit *hits;                /* (bounces + 1) * n (device): round r, ray i at r * n + i */
+  rdrt_ray *spawned;             /* bounces * n (device) or NULL: rays of round r >= 1 at (r - 1) * n + i */
+  const uint32_t *scene_offsets; /* host, nscenes + 1 (sequential mode): scene k = rays [off[k], off[k+1]) */
+  uint32_t n, nscenes, bounces, seed, mode, pad;
+} rdrt_batch_params;
+
+typedef struct rdrt_batch_stats {
+  uint32_t rounds;
+  uint32_t rays[17]; /* rays traced in each round */
+  uint32_t launches; /* kernel launches of the last call */
+  float ms;          /* HIP-event time of the last call */
+} rdrt_batch_stats;
+
+int32_t rdrt_trace_batched(rdrt_ctx *ctx, const rdrt_batch_params *params);
+int32_t rdrt_batch_get_stats(rdrt_ctx *ctx, rdrt_batch_stats *out);
+
 /* ---- device primitives of the path (parity helpers) ---- */
 /* Stable stream compaction (shader/parallel-compute/src/stream_compaction.rs:
  * 3-45): out_idx_dev receives the indices i with mask_dev[i] != 0 in

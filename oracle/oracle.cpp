@@ -1323,4 +1323,57 @@ int orc_pt_render_rows(const orc_scene *s, const orc_pt_params *P, uint32_t row_
   return 0;
 }
 
+// One round of batched tracing: closest hit of every ray (traverse_cpu.rs:
+// 52-245 with per-ray TLAS selection, traverse_cpu.rs:63), then for each hit
+// a continuation ray from the offset hit point (ray_util.rs:6-40) in a
+// cosine-weighted direction (sampling.rs:72-86, diffuse.rs:24-36) around the
+// geometric normal facing the ray, sample (vdc, sobol)(round) of the ray's
+// sequence scrambled by xxhash32(ray, seed, 2 / 3) (sampler.rs:33-42,
+// sobol.rs:40-68).  Shading math in FP64, the offset on FP32 values.
+int orc_batch_round(const orc_scene *s, const orc_ray *rays, const orc_ray_meta *meta, uint32_t n, uint32_t round,
+                    uint32_t seed, orc_hit *hits, orc_ray *spawned, uint32_t *alive, int nthreads) {
+  if (!s->built || !meta) return -1;
+#ifdef _OPENMP
+  int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nt)
+#endif
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    const RayIn ri = load_ray(rays[i], &meta[i], 0, 0, 0);
+    orc_hit h;
+    orc_hit_meta hm;
+    Counters c;
+    uint32_t ext[2] = {0, 0};
+    const bool hit = traverse_one(s, ri, h, hm, c, ext);
+    hits[i] = h;
+    alive[i] = hit ? 1u : 0u;
+    orc_ray out{};
+    if (hit) {
+      const auto &geo = s->geo_meta[ext[0]];
+      const uint32_t tri = geo.primitive_start + h.prim_id;
+      const D3 p0 = d3f(s->vertices[s->indices[3 * (size_t)tri]]);
+      const D3 p1 = d3f(s->vertices[s->indices[3 * (size_t)tri + 1]]);
+      const D3 p2 = d3f(s->vertices[s->indices[3 * (size_t)tri + 2]]);
+      D3 ng = dnormalize(normal_to_world(s->inst_transform_inv[ext[1]], dcross(p1 - p0, p2 - p0)));
+      const D3 dd = d3f(ri.d);
+      if (ddot(ng, dd) > 0.0) ng = ng * -1.0;
+      const D3 pos = d3f(ri.o) + dd * (double)h.t;
+      const V3 ngf = v3((float)ng.x, (float)ng.y, (float)ng.z);
+      const V3 posf = v3((float)pos.x, (float)pos.y, (float)pos.z);
+      const double s0 = orc_van_der_corput(round, xxhash32((uint32_t)i, seed, 2u));
+      const double s1 = orc_sobol(round, xxhash32((uint32_t)i, seed, 3u));
+      const D3 wo = sample_cosine(ng, s0, s1);
+      out.ox = offset_axis(posf.x, ngf.x);
+      out.oy = offset_axis(posf.y, ngf.y);
+      out.oz = offset_axis(posf.z, ngf.z);
+      out.tmin = FLT_EPSILON;
+      out.dx = (float)wo.x;
+      out.dy = (float)wo.y;
+      out.dz = (float)wo.z;
+      out.tmax = FLT_MAX;
+    }
+    spawned[i] = out;
+  }
+  return 0;
+}
+
 }  // extern "C"

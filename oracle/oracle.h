@@ -177,6 +177,12 @@ int orc_pt_render(const orc_scene *s, const orc_pt_params *params, double *out_r
 int orc_pt_render_rows(const orc_scene *s, const orc_pt_params *params, uint32_t row_begin, uint32_t row_end,
                        double *out_rgb, uint64_t *ray_counts, int nthreads);
 
+/* One round of batched tracing: hits[i] = closest hit of ray i (per-ray
+ * TLAS from meta), alive[i] = 1 if it hit, and then spawned[i] = the
+ * continuation ray of path i at bounce `round`. */
+int orc_batch_round(const orc_scene *s, const orc_ray *rays, const orc_ray_meta *meta, uint32_t n, uint32_t round,
+                    uint32_t seed, orc_hit *hits, orc_ray *spawned, uint32_t *alive, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,7 @@ _SIG = {
     "orc_set_materials": (None, [P, P, u32]),
     "orc_pt_render": (C.c_int, [P, C.POINTER(OrcPtParams), P, P, C.c_int]),
     "orc_pt_render_rows": (C.c_int, [P, C.POINTER(OrcPtParams), u32, u32, P, P, C.c_int]),
+    "orc_batch_round": (C.c_int, [P, P, P, u32, u32, u32, P, P, P, C.c_int]),
 }
 
 _lib = None
@@ -209,6 +210,23 @@ class OracleScene:
         if rc != 0:
             raise RuntimeError("oracle trace failed (scene not built)")
         return hits, hm, cnt
+
+    def batch_round(self, rays: np.ndarray, meta: np.ndarray, round_: int, seed: int, threads: int = 0):
+        """One round of batched tracing: (hits (n,4) u32, spawned (n,8) f32,
+        alive (n,) u32) -- closest hits, then continuation rays of the hits."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        meta = np.ascontiguousarray(meta, dtype=np.uint32).reshape(-1, 4)
+        n = rays.shape[0]
+        if meta.shape[0] != n:
+            raise ValueError("one metadata record per ray")
+        hits = np.zeros((n, 4), dtype=np.uint32)
+        spawned = np.zeros((n, 8), dtype=np.float32)
+        alive = np.zeros(n, dtype=np.uint32)
+        rc = self.L.orc_batch_round(self.h, rays.ctypes.data, meta.ctypes.data, n, round_, seed & 0xFFFFFFFF,
+                                    hits.ctypes.data, spawned.ctypes.data, alive.ctypes.data, threads)
+        if rc != 0:
+            raise RuntimeError("oracle batch round failed (scene not built)")
+        return hits, spawned, alive
 
     # ---- path tracer ----
     def set_vertex_normals(self, blas: int, geometry: int, normals) -> None:

@@ -82,6 +82,20 @@ RDRT_PT_DIAG_TIME_TRACES = 0x1
 RDRT_PT_DIAG_COUNT_VISITS = 0x2
 
 
+class RdrtBatchParams(C.Structure):
+    _fields_ = [("rays", C.c_void_p), ("ray_meta", C.c_void_p), ("hits", C.c_void_p), ("spawned", C.c_void_p),
+                ("scene_offsets", C.POINTER(u32)), ("n", u32), ("nscenes", u32), ("bounces", u32), ("seed", u32),
+                ("mode", u32), ("pad", u32)]
+
+
+class RdrtBatchStats(C.Structure):
+    _fields_ = [("rounds", u32), ("rays", u32 * 17), ("launches", u32), ("ms", f32)]
+
+
+RDRT_BATCH_PERSISTENT = 0
+RDRT_BATCH_SEQUENTIAL = 1
+
+
 # name -> (restype, argtypes); the full exported surface of rdrt.h
 SIGNATURES = {
     "rdrt_last_error": (C.c_char_p, []),
@@ -110,6 +124,8 @@ SIGNATURES = {
     "rdrt_set_materials": (i32, [C.c_void_p, C.POINTER(RdrtMaterial), u32]),
     "rdrt_pt_render": (i32, [C.c_void_p, C.POINTER(RdrtPtParams), C.c_void_p]),
     "rdrt_pt_get_stats": (i32, [C.c_void_p, C.POINTER(RdrtPtStats)]),
+    "rdrt_trace_batched": (i32, [C.c_void_p, C.POINTER(RdrtBatchParams)]),
+    "rdrt_batch_get_stats": (i32, [C.c_void_p, C.POINTER(RdrtBatchStats)]),
     "rdrt_compact": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p]),
     "rdrt_sample_sobol": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, u32, C.c_void_p]),
 }

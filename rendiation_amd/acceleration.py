@@ -49,6 +49,8 @@ GEOMETRY_FLAG_NO_DUPLICATE_ANYHIT_INVOCATION = 0x2
 
 TRAVERSAL_FAST = 0
 TRAVERSAL_REFERENCE = 1
+BATCH_PERSISTENT = 0
+BATCH_SEQUENTIAL = 1
 MISS = 0xFFFFFFFF
 
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
@@ -118,6 +120,14 @@ class SceneInfo:
     stack_entries: int
     device_bytes: int
     build_ms: float
+
+
+@dataclass
+class BatchStats:
+    rounds: int
+    rays: list  # rays traced per round
+    launches: int
+    ms: float
 
 
 def _ptr(t) -> Optional[int]:
@@ -285,6 +295,37 @@ class AccelerationStructureSystem:
         a.hit_meta, a.counters = _ptr(hit_meta), _ptr(counters)
         a.n, a.flags, a.cull_mask, a.tlas_idx, a.traversal = n, flags, cull_mask & 0xFFFFFFFF, tlas_idx, traversal
         L.check(self._L.rdrt_trace(self._h, C.byref(a)))
+
+    def trace_batched(self, rays, ray_meta, hits, *, bounces: int, seed: int = 0, spawned=None,
+                      scene_offsets=None, mode: int = BATCH_PERSISTENT) -> None:
+        """Enqueue a batched multi-scene trace of ``bounces + 1`` rounds.
+
+        ``rays`` (n*8 f32) and ``ray_meta`` (n*4 words, tlas_idx = scene)
+        hold the primary rays of all scenes, grouped by scene; ``hits`` gets
+        (bounces+1)*n records, round r at [r*n, (r+1)*n); ``spawned``
+        (optional, bounces*n*8 f32) the rays of rounds 1.. .  Sequential mode
+        needs ``scene_offsets`` (nscenes + 1 host ints)."""
+        n = rays.numel() // 8
+        _check_dev(rays, n * 8, "rays")
+        _check_dev(ray_meta, n * 4, "ray_meta")
+        _check_dev(hits, (bounces + 1) * n * 4, "hits")
+        if spawned is not None:
+            _check_dev(spawned, bounces * n * 8, "spawned")
+        a = L.RdrtBatchParams()
+        a.rays, a.ray_meta, a.hits, a.spawned = _ptr(rays), _ptr(ray_meta), _ptr(hits), _ptr(spawned)
+        offs = None
+        if scene_offsets is not None:
+            offs = (L.u32 * len(scene_offsets))(*[int(x) for x in scene_offsets])
+            a.scene_offsets = C.cast(offs, C.POINTER(L.u32))
+            a.nscenes = len(scene_offsets) - 1
+        a.n, a.bounces, a.seed, a.mode = n, bounces, seed & 0xFFFFFFFF, mode
+        L.check(self._L.rdrt_trace_batched(self._h, C.byref(a)))
+
+    def batch_stats(self) -> BatchStats:
+        """Counts and timing of the last batched trace (synchronizes)."""
+        o = L.RdrtBatchStats()
+        L.check(self._L.rdrt_batch_get_stats(self._h, C.byref(o)))
+        return BatchStats(rounds=o.rounds, rays=list(o.rays)[:o.rounds], launches=o.launches, ms=float(o.ms))
 
     def sync(self) -> None:
         L.check(self._L.rdrt_sync(self._h))

@@ -152,6 +152,7 @@ int32_t rdrt_ctx_destroy(rdrt_ctx *c) {
     if (c->work_stream) (void)hipStreamSynchronize(c->work_stream);
     free_device_scene(c->dev);
     c->pt.free_all();
+    c->batch.free_all();
     c->compact_scratch.free();
     if (c->owns_work_stream) (void)hipStreamDestroy(c->work_stream);
     if (c->status) (void)hipFree(c->status);

@@ -1,0 +1,230 @@
+// api_batch.hip -- C-ABI of batched multi-scene tracing (include/rdrt.h).
+//
+// Persistent mode: per round, one persistent traversal launch over the
+// queue of all scenes, then one spawn + compaction launch; the whole call is
+// a captured hipGraph replayed while its parameters are unchanged.
+// Sequential mode (the baseline): the same kernels scene by scene and round
+// by round, with plain (one-ray-per-thread) traversal launches.
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "batch.h"
+#include "context.h"
+
+using namespace rdrt;
+
+namespace rdrt {
+
+void BatchState::free_all() {
+  for (Scratch *s : {&wrays, &hitx, &qa, &qb, &counts, &lookback, &tickets, &work}) s->free();
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+  cached.clear();
+}
+
+}  // namespace rdrt
+
+namespace {
+
+constexpr uint32_t kBlock = 256;
+
+struct Plan {
+  uint32_t rounds, nscenes;
+  std::vector<uint32_t> off;        // scene ray offsets
+  std::vector<size_t> lb_off;       // look-back word offset of (scene, round)
+  size_t lb_words = 0;
+};
+
+Plan make_plan(const rdrt_batch_params *a) {
+  Plan p;
+  p.rounds = a->bounces + 1;
+  if (a->mode == RDRT_BATCH_SEQUENTIAL) {
+    p.nscenes = a->nscenes;
+    p.off.assign(a->scene_offsets, a->scene_offsets + a->nscenes + 1);
+  } else {
+    p.nscenes = 1;
+    p.off = {0u, a->n};
+  }
+  for (uint32_t k = 0; k < p.nscenes; k++) {
+    size_t blocks = (p.off[k + 1] - p.off[k] + kBlock - 1) / kBlock;
+    for (uint32_t r = 0; r < p.rounds; r++) {
+      p.lb_off.push_back(p.lb_words);
+      p.lb_words += blocks;
+    }
+  }
+  return p;
+}
+
+// Enqueue all rounds of scene range k (the whole batch in persistent mode).
+hipError_t enqueue_scene(rdrt_ctx *c, const rdrt_batch_params *a, const Plan &pl, uint32_t k, bool persist,
+                         uint32_t &launches) {
+  BatchState &s = c->batch;
+  const uint32_t first = pl.off[k], nk = pl.off[k + 1] - first;
+  if (nk == 0) return hipSuccess;
+  const uint32_t R = pl.rounds;
+  uint32_t *counts = (uint32_t *)s.counts.ptr + (size_t)k * (R + 1);
+  uint32_t *queues[2] = {(uint32_t *)s.qa.ptr + first, (uint32_t *)s.qb.ptr + first};
+  TraceParams base = scene_trace_params(c);
+  base.meta = a->ray_meta;
+  base.cull_mask = 0xffffffffu;
+  base.n = a->n;
+  base.slots = nk;
+  base.hitx = (uint2 *)s.hitx.ptr;
+  base.work = (uint32_t *)s.work.ptr;
+  const uint32_t block = c->cfg.block_size ? c->cfg.block_size : 256;
+  for (uint32_t r = 0; r < R; r++) {
+    TraceParams tp = base;
+    tp.rays = r == 0 ? a->rays : (const rdrt_ray *)s.wrays.ptr;
+    tp.hits = a->hits + (size_t)r * a->n;
+    tp.queue = queues[r & 1];
+    tp.count = counts + r;
+    hipError_t e = launch_trace(tp, RDRT_TRAVERSAL_FAST, push_depth(c), block, c->work_stream, persist);
+    if (e != hipSuccess) return e;
+    launches += persist ? 2 : 1;
+    if (r + 1 == R) break;
+    SpawnArgs sa;
+    std::memset(&sa, 0, sizeof(sa));
+    sa.qin = queues[r & 1];
+    sa.count_in = counts + r;
+    sa.slots = nk;
+    sa.qout = queues[(r + 1) & 1];
+    sa.count_out = counts + r + 1;
+    sa.lookback = (unsigned long long *)s.lookback.ptr + pl.lb_off[(size_t)k * R + r];
+    sa.ticket = (uint32_t *)s.tickets.ptr + (size_t)k * R + r;
+    sa.rays = tp.rays;
+    sa.out_rays = (rdrt_ray *)s.wrays.ptr;
+    sa.record = a->spawned ? a->spawned + (size_t)r * a->n : nullptr;
+    sa.hits = tp.hits;
+    sa.hits_later = tp.hits + a->n;
+    sa.later_rounds = R - 1 - r;
+    sa.hitx = base.hitx;
+    sa.n = a->n;
+    sa.round = r + 1;
+    sa.seed = a->seed;
+    sa.insts = c->dev.insts.ptr;
+    sa.geos = c->dev.geos.ptr;
+    sa.indices = c->dev.indices.ptr;
+    sa.vertices = c->dev.vertices.ptr;
+    sa.status = c->status;
+    e = enqueue_spawn(sa, c->work_stream);
+    if (e != hipSuccess) return e;
+    launches++;
+  }
+  return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t rdrt_trace_batched(rdrt_ctx *c, const rdrt_batch_params *a) {
+  if (!c || !a) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  if (!c->uploaded) return fail(RDRT_ERR_NOT_BUILT, "scene is not committed (call rdrt_commit)");
+  if (a->n == 0) return RDRT_OK;
+  if (!a->rays || !a->ray_meta || !a->hits) return fail(RDRT_ERR_INVALID_ARG, "null rays, metadata or hits");
+  if (a->bounces > 16) return fail(RDRT_ERR_INVALID_ARG, "bounces above 16");
+  if (a->mode > RDRT_BATCH_SEQUENTIAL) return fail(RDRT_ERR_INVALID_ARG, "unknown batch mode");
+  if (a->n > (1u << 30)) return fail(RDRT_ERR_CAPACITY, "more than 2^30 rays per batch");
+  if (a->mode == RDRT_BATCH_SEQUENTIAL) {
+    if (!a->scene_offsets || a->nscenes == 0) return fail(RDRT_ERR_INVALID_ARG, "sequential mode needs scene offsets");
+    if (a->scene_offsets[0] != 0 || a->scene_offsets[a->nscenes] != a->n)
+      return fail(RDRT_ERR_INVALID_ARG, "scene offsets must start at 0 and end at n");
+    for (uint32_t k = 0; k < a->nscenes; k++)
+      if (a->scene_offsets[k] > a->scene_offsets[k + 1]) return fail(RDRT_ERR_INVALID_ARG, "offsets must increase");
+  }
+  if (stack_entries_for(push_depth(c)) == 0) return fail(RDRT_ERR_CAPACITY, "BVH too deep for the fast traversal");
+  const Plan pl = make_plan(a);
+  BatchState &s = c->batch;
+  const size_t n = a->n;
+  RDRT_HIP(s.wrays.reserve(n * sizeof(rdrt_ray)));
+  RDRT_HIP(s.hitx.reserve(n * 8));
+  RDRT_HIP(s.qa.reserve(n * 4));
+  RDRT_HIP(s.qb.reserve(n * 4));
+  RDRT_HIP(s.counts.reserve((size_t)pl.nscenes * (pl.rounds + 1) * 4));
+  RDRT_HIP(s.lookback.reserve(std::max<size_t>(pl.lb_words, 1) * 8));
+  RDRT_HIP(s.tickets.reserve((size_t)pl.nscenes * pl.rounds * 4));
+  RDRT_HIP(s.work.reserve(4));
+  hipStream_t st = c->work_stream;
+  s.rounds = pl.rounds;
+  s.nscenes = pl.nscenes;
+  s.mode = a->mode;
+  if (a->mode == RDRT_BATCH_SEQUENTIAL) {
+    // per-scene queue sizes start at the scene's ray count; uploaded (and
+    // waited for) ahead of the timed region so the host array may be reused
+    s.host_counts.assign((size_t)pl.nscenes * (pl.rounds + 1), 0u);
+    for (uint32_t k = 0; k < pl.nscenes; k++) s.host_counts[(size_t)k * (pl.rounds + 1)] = pl.off[k + 1] - pl.off[k];
+    RDRT_HIP(hipMemcpyAsync(s.counts.ptr, s.host_counts.data(), s.host_counts.size() * 4, hipMemcpyHostToDevice, st));
+    RDRT_HIP(hipStreamSynchronize(st));
+  }
+  RDRT_HIP(hipEventRecord(c->ev0, st));
+  if (a->mode == RDRT_BATCH_PERSISTENT) {
+    // graph key: every parameter and buffer the captured launches use
+    std::vector<uint8_t> key(sizeof(*a) + 8 * sizeof(void *));
+    std::memcpy(key.data(), a, sizeof(*a));
+    const void *ptrs[8] = {s.wrays.ptr, s.hitx.ptr, s.qa.ptr, s.qb.ptr, s.counts.ptr, s.lookback.ptr, s.tickets.ptr,
+                           s.work.ptr};
+    std::memcpy(key.data() + sizeof(*a), ptrs, sizeof(ptrs));
+    if (!s.exec || key != s.cached) {
+      if (s.exec) (void)hipGraphExecDestroy(s.exec);
+      if (s.graph) (void)hipGraphDestroy(s.graph);
+      s.exec = nullptr;
+      s.graph = nullptr;
+      uint32_t launches = 1;
+      RDRT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      hipError_t e = enqueue_batch_init((uint32_t *)s.qa.ptr, 0u, a->n, (uint32_t *)s.counts.ptr,
+                                        (unsigned long long *)s.lookback.ptr, (uint32_t)pl.lb_words,
+                                        (uint32_t *)s.tickets.ptr, pl.rounds, (uint32_t *)s.counts.ptr + 1,
+                                        pl.rounds, st);
+      if (e == hipSuccess) e = enqueue_scene(c, a, pl, 0, true, launches);
+      hipGraph_t g = nullptr;
+      hipError_t e2 = hipStreamEndCapture(st, &g);
+      if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(RDRT_ERR_HIP, std::string("batch capture failed: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+      }
+      s.graph = g;
+      RDRT_HIP(hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0));
+      s.cached = key;
+      s.launches = launches;
+    }
+    RDRT_HIP(hipGraphLaunch(s.exec, st));
+  } else {
+    uint32_t launches = 1;
+    // identity queue over all rays (scene k owns [off[k], off[k+1])), zeroed
+    // look-back and tickets; the count slot written is the unused work word
+    RDRT_HIP(enqueue_batch_init((uint32_t *)s.qa.ptr, 0u, a->n, (uint32_t *)s.work.ptr,
+                                (unsigned long long *)s.lookback.ptr, (uint32_t)pl.lb_words, (uint32_t *)s.tickets.ptr,
+                                pl.nscenes * pl.rounds, nullptr, 0u, st));
+    for (uint32_t k = 0; k < pl.nscenes; k++) RDRT_HIP(enqueue_scene(c, a, pl, k, false, launches));
+    s.launches = launches;
+  }
+  RDRT_HIP(hipEventRecord(c->ev1, st));
+  return RDRT_OK;
+}
+
+int32_t rdrt_batch_get_stats(rdrt_ctx *c, rdrt_batch_stats *o) {
+  if (!c || !o) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  BatchState &s = c->batch;
+  if (!s.counts.ptr || s.rounds == 0) return fail(RDRT_ERR_STATE, "no batch traced");
+  RDRT_HIP(hipStreamSynchronize(c->work_stream));
+  std::memset(o, 0, sizeof(*o));
+  std::vector<uint32_t> counts((size_t)s.nscenes * (s.rounds + 1));
+  RDRT_HIP(hipMemcpy(counts.data(), s.counts.ptr, counts.size() * 4, hipMemcpyDeviceToHost));
+  o->rounds = s.rounds;
+  for (uint32_t k = 0; k < s.nscenes; k++)
+    for (uint32_t r = 0; r < s.rounds; r++) o->rays[r] += counts[(size_t)k * (s.rounds + 1) + r];
+  o->launches = s.launches;
+  RDRT_HIP(hipEventElapsedTime(&o->ms, c->ev0, c->ev1));
+  return RDRT_OK;
+}
+
+}  // extern "C"

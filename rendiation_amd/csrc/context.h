@@ -99,6 +99,17 @@ struct PtState {
   void free_all();
 };
 
+// Batched-tracing state of one context (grown on demand).
+struct BatchState {
+  Scratch wrays, hitx, qa, qb, counts, lookback, tickets, work;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint8_t> cached;  // launch key of the captured graph
+  std::vector<uint32_t> host_counts;
+  uint32_t rounds = 0, nscenes = 1, mode = 0, launches = 0;
+  void free_all();
+};
+
 void free_device_scene(DeviceScene &d);
 
 }  // namespace rdrt
@@ -121,6 +132,7 @@ struct rdrt_ctx {
   bool timed = false;
   std::vector<rdrt::Material> materials;
   rdrt::PtState pt;
+  rdrt::BatchState batch;
   rdrt::Scratch compact_scratch;
 };
 

@@ -23,7 +23,9 @@ struct TraceParams {
   // queue entry.
   const uint32_t *queue;
   const uint32_t *count;
-  uint2 *hitx;  // optional: (global geometry index, forest instance index) per hit
+  uint32_t slots;  // queue slots the grid covers (0 = n); ray ids stay < n
+  uint2 *hitx;     // optional: (global geometry index, forest instance index) per hit
+  uint32_t *work;  // device work counter of the persistent kernel
 
   const PairNode *tri_pairs;
   const TriPre *tris;
@@ -41,13 +43,17 @@ struct TraceParams {
   const float *vertices;
 };
 
+__host__ __device__ inline uint32_t slots_of(const TraceParams &p) { return p.slots ? p.slots : p.n; }
+
 // Smallest stack (entries per lane) that a fast-traversal kernel is built
 // with for the given push depth; 0 if none is large enough.
 uint32_t stack_entries_for(uint32_t push_depth);
 
 // Enqueues one traversal launch on `stream`.  mode = RDRT_TRAVERSAL_*.
+// persist = resident-size grid whose waves pull 64-ray chunks from p.work
+// (zeroed by a kernel launched first, so the pair is capturable).
 hipError_t launch_trace(const TraceParams &p, uint32_t mode, uint32_t push_depth, uint32_t block,
-                        hipStream_t stream);
+                        hipStream_t stream, bool persist = false);
 
 }  // namespace rdrt
 

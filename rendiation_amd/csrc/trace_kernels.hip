@@ -23,6 +23,10 @@
 
 #include "trace_kernels.h"
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+
 #pragma clang fp contract(off)
 
 namespace rdrt {
@@ -211,17 +215,8 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t i, uint32_t used) {
 // the lane's private memory; S bounds the total.
 // ------------------------------------------------------------------------
 template <int K, int S, int B, bool CNT>
-__global__ __launch_bounds__(B) void k_trace_fast(TraceParams p) {
-  __shared__ uint32_t lds[K * B];
+__device__ __forceinline__ void trace_fast_one(const TraceParams &p, uint32_t rid, uint32_t *stk) {
   uint32_t spill[S > K ? S - K : 1];
-  const uint32_t n = p.count ? min(p.n, *p.count) : p.n;
-  const uint32_t used = (n + B - 1) / B;
-  if (blockIdx.x >= used) return;
-  const uint32_t slot = xcd_block(blockIdx.x, used) * B + threadIdx.x;
-  if (slot >= n) return;
-  const uint32_t rid = p.queue ? p.queue[slot] : slot;
-  if (rid >= p.n) return;  // a queue entry always names one of the n rays
-  uint32_t *stk = lds + threadIdx.x;  // entry e < K at stk[e * B]
   auto push = [&](uint32_t &sp, uint32_t v) {
     if (sp < (uint32_t)K) stk[sp * B] = v;
     else if (sp < (uint32_t)S) spill[sp - K] = v;
@@ -355,6 +350,43 @@ __global__ __launch_bounds__(B) void k_trace_fast(TraceParams p) {
   write_result(p, rid, best, cn, ct, ci);
 }
 
+// One ray per thread over a grid sized for p.n; the live count may be
+// smaller (device queue size).
+template <int K, int S, int B, bool CNT>
+__global__ __launch_bounds__(B) void k_trace_fast(TraceParams p) {
+  __shared__ uint32_t lds[K * B];
+  const uint32_t n = p.count ? min(slots_of(p), *p.count) : slots_of(p);
+  const uint32_t used = (n + B - 1) / B;
+  if (blockIdx.x >= used) return;
+  const uint32_t slot = xcd_block(blockIdx.x, used) * B + threadIdx.x;
+  if (slot >= n) return;
+  const uint32_t rid = p.queue ? p.queue[slot] : slot;
+  if (rid >= p.n) return;  // a queue entry always names one of the n rays
+  trace_fast_one<K, S, B, CNT>(p, rid, lds + threadIdx.x);
+}
+
+// Persistent variant: a grid sized to the resident capacity; each wave
+// takes chunks of 64 consecutive queue slots from the device counter
+// p.work until the queue is exhausted, so uneven rays and scenes balance
+// across the chip.  Every wave exits once a fetch lands past the count.
+template <int K, int S, int B, bool CNT>
+__global__ __launch_bounds__(B) void k_trace_persist(TraceParams p) {
+  __shared__ uint32_t lds[K * B];
+  const uint32_t n = p.count ? min(slots_of(p), *p.count) : slots_of(p);
+  const uint32_t lane = threadIdx.x & 63u;
+  for (;;) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(p.work, 64u);
+    base = __shfl(base, 0, 64);
+    if (base >= n) break;
+    const uint32_t slot = base + lane;
+    if (slot < n) {
+      const uint32_t rid = p.queue ? p.queue[slot] : slot;
+      if (rid < p.n) trace_fast_one<K, S, B, CNT>(p, rid, lds + threadIdx.x);
+    }
+  }
+}
+
 // ------------------------------------------------------------------------
 // Reference-order kernel: traverse_cpu.rs:52-245 on the 48-byte forest.
 // ------------------------------------------------------------------------
@@ -371,7 +403,7 @@ __device__ __forceinline__ F3 vertex(const TraceParams &p, uint32_t i) {
 template <int B>
 __global__ __launch_bounds__(B) void k_trace_ref(TraceParams p) {
   const uint32_t slot = blockIdx.x * B + threadIdx.x;
-  if (slot >= p.n || (p.count && slot >= *p.count)) return;
+  if (slot >= slots_of(p) || (p.count && slot >= *p.count)) return;
   const uint32_t rid = p.queue ? p.queue[slot] : slot;
   if (rid >= p.n) return;  // a queue entry always names one of the n rays
   RayIn r = load_ray(p, rid);
@@ -457,21 +489,55 @@ __global__ __launch_bounds__(B) void k_trace_ref(TraceParams p) {
 
 constexpr int kLdsEntries = 16;  // stack entries per lane kept in LDS
 
+__global__ void k_zero_work(uint32_t *w) { *w = 0u; }
+
+// Resident blocks of a kernel on the current device (cached per device and kernel).
+uint32_t resident_grid(const void *kernel, int block) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void *>, uint32_t> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1024;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find({dev, kernel});
+  if (it != cache.end()) return it->second;
+  int cus = 0, per = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, 0) != hipSuccess || per < 1 || cus < 1)
+    return 1024;
+  uint32_t g = (uint32_t)(cus * per);
+  cache[{dev, kernel}] = g;
+  return g;
+}
+
 template <int S, int B>
-hipError_t launch_fast_b(const TraceParams &p, hipStream_t st) {
+hipError_t launch_fast_b(const TraceParams &p, bool persist, hipStream_t st) {
   constexpr int K = S < kLdsEntries ? S : kLdsEntries;
-  dim3 grid((p.n + B - 1) / B);
+  if (persist) {
+    if (!p.work) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(1), 0, st, p.work);
+    if (p.counters) {
+      auto k = k_trace_persist<K, S, B, true>;
+      uint32_t g = std::min(resident_grid((const void *)k, B), (slots_of(p) + B - 1) / B);
+      hipLaunchKernelGGL(k, dim3(g), dim3(B), 0, st, p);
+    } else {
+      auto k = k_trace_persist<K, S, B, false>;
+      uint32_t g = std::min(resident_grid((const void *)k, B), (slots_of(p) + B - 1) / B);
+      hipLaunchKernelGGL(k, dim3(g), dim3(B), 0, st, p);
+    }
+    return hipGetLastError();
+  }
+  dim3 grid((slots_of(p) + B - 1) / B);
   if (p.counters) hipLaunchKernelGGL((k_trace_fast<K, S, B, true>), grid, dim3(B), 0, st, p);
   else hipLaunchKernelGGL((k_trace_fast<K, S, B, false>), grid, dim3(B), 0, st, p);
   return hipGetLastError();
 }
 
 template <int S>
-hipError_t launch_fast_s(const TraceParams &p, uint32_t block, hipStream_t st) {
+hipError_t launch_fast_s(const TraceParams &p, uint32_t block, bool persist, hipStream_t st) {
   switch (block) {
-    case 64: return launch_fast_b<S, 64>(p, st);
-    case 128: return launch_fast_b<S, 128>(p, st);
-    default: return launch_fast_b<S, 256>(p, st);
+    case 64: return launch_fast_b<S, 64>(p, persist, st);
+    case 128: return launch_fast_b<S, 128>(p, persist, st);
+    default: return launch_fast_b<S, 256>(p, persist, st);
   }
 }
 
@@ -486,21 +552,21 @@ uint32_t stack_entries_for(uint32_t push_depth) {
 }
 
 hipError_t launch_trace(const TraceParams &p, uint32_t mode, uint32_t push_depth, uint32_t block,
-                        hipStream_t stream) {
-  if (p.n == 0) return hipSuccess;
+                        hipStream_t stream, bool persist) {
+  if (p.n == 0 || slots_of(p) == 0) return hipSuccess;
   if (mode == RDRT_TRAVERSAL_REFERENCE) {
-    dim3 grid((p.n + 255) / 256);
+    dim3 grid((slots_of(p) + 255) / 256);
     hipLaunchKernelGGL((k_trace_ref<256>), grid, dim3(256), 0, stream, p);
     return hipGetLastError();
   }
   uint32_t s = stack_entries_for(push_depth);
   switch (s) {
-    case 8: return launch_fast_s<8>(p, block, stream);
-    case 16: return launch_fast_s<16>(p, block, stream);
-    case 24: return launch_fast_s<24>(p, block, stream);
-    case 32: return launch_fast_s<32>(p, block, stream);
-    case 48: return launch_fast_s<48>(p, block, stream);
-    case 64: return launch_fast_s<64>(p, block, stream);
+    case 8: return launch_fast_s<8>(p, block, persist, stream);
+    case 16: return launch_fast_s<16>(p, block, persist, stream);
+    case 24: return launch_fast_s<24>(p, block, persist, stream);
+    case 32: return launch_fast_s<32>(p, block, persist, stream);
+    case 48: return launch_fast_s<48>(p, block, persist, stream);
+    case 64: return launch_fast_s<64>(p, block, persist, stream);
     default: return hipErrorInvalidValue;
   }
 }

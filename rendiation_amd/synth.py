@@ -192,3 +192,40 @@ def pt_scene(seed: int = 0, n_meshes: int = 64, tess: int = 256, n_emissive: int
 CLOSEST_HIT_TRIS = 100_000
 CLOSEST_HIT_RAYS = 1_000_000
 CLOSEST_HIT_LEAF = 4
+
+
+def log_uniform_counts(seed: int, stream: int, n: int, lo: int, hi: int) -> np.ndarray:
+    """n integer counts floor(exp(U[ln lo, ln hi))) (log-uniform in [lo, hi))."""
+    u = uniform_f32(seed, stream, n).astype(np.float64)
+    c = np.floor(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(np.int64)
+    return np.clip(c, lo, hi - 1)
+
+
+def batched_scenes(seed: int = 0, nscenes: int = 64, tris=(10_000, 2_000_000), rays=(16_384, 4_194_304),
+                   spread: float = 0.01):
+    """Independent random-triangle scenes with uneven ray sets: scene k has a
+    log-uniform triangle count (stream 30) and ray count (stream 31); its
+    triangles and rays come from the closest-hit distributions under seed
+    ``seed * 1009 + k + 1``.  Returns (scenes, rays (N, 8) f32, meta (N, 4)
+    u32 with tlas_idx = k, offsets (nscenes + 1))."""
+    ntri = log_uniform_counts(seed, 30, nscenes, *tris)
+    nray = log_uniform_counts(seed, 31, nscenes, *rays)
+    offsets = np.zeros(nscenes + 1, dtype=np.int64)
+    offsets[1:] = np.cumsum(nray)
+    all_rays = np.empty((int(offsets[-1]), 8), dtype=np.float32)
+    meta = np.zeros((int(offsets[-1]), 4), dtype=np.uint32)
+    meta[:, 1] = 0xFFFFFFFF
+    scenes = []
+    for k in range(nscenes):
+        sk = seed * 1009 + k + 1
+        scenes.append(random_triangles(sk, int(ntri[k]), spread))
+        all_rays[offsets[k]:offsets[k + 1]] = random_rays(sk, int(nray[k]))
+        meta[offsets[k]:offsets[k + 1], 2] = k
+    return scenes, all_rays, meta, offsets
+
+
+# Batched workload: 64 scenes, triangles log-U[10k, 2M), rays log-U[16k, 4M), 3 bounces.
+BATCHED_SCENES = 64
+BATCHED_TRIS = (10_000, 2_000_000)
+BATCHED_RAYS = (16_384, 4_194_304)
+BATCHED_BOUNCES = 3

@@ -4,6 +4,7 @@ from __future__ import annotations
 
 import numpy as np
 
+from oracle.oracle import OracleScene
 from rendiation_amd import synth
 from rendiation_amd.acceleration import (
     GEOMETRY_INSTANCE_FORCE_NO_OPAQUE,
@@ -17,6 +18,9 @@ from rendiation_amd.acceleration import (
     RAY_FLAG_CULL_OPAQUE,
     RAY_FLAG_FORCE_NON_OPAQUE,
     RAY_FLAG_FORCE_OPAQUE,
+    AccelerationStructureSystem,
+    TopLevelAccelerationStructureSourceInstance,
+    TriangleGeometry,
 )
 
 
@@ -75,3 +79,37 @@ def instanced_scene(seed: int = 3, nrays: int = 50_000):
     meta[:, 1] = mask_set[((sel >> np.uint64(8)) % np.uint64(len(mask_set))).astype(np.int64)]
     meta[:, 2] = ((sel >> np.uint64(16)) % np.uint64(7)).astype(np.uint32) % 3  # 0,1 bound; 2 unbound
     return blas_list, tlas_list, binding, rays, meta
+
+
+def build_pair(blas_list, tlas_list, binding, opts):
+    """Same scene in the product system and in the oracle.
+    blas_list: list of list of (pos, idx, flags); tlas_list: list of list of
+    instance dicts (transform, blas, custom, mask, flags)."""
+    sys = AccelerationStructureSystem(device=0, options=opts)
+    orc = OracleScene(opts.blas_bin_size, opts.blas_max_depth, opts.tlas_bin_size, opts.tlas_max_depth, opts.sah_bins)
+    bh = []
+    for geos in blas_list:
+        bh.append(sys.create_bottom_level_acceleration_structure(
+            [TriangleGeometry(p, i, f) for (p, i, f) in geos]))
+        orc.add_blas(geos)
+    th = []
+    for insts in tlas_list:
+        th.append(sys.create_top_level_acceleration_structure([
+            TopLevelAccelerationStructureSourceInstance(
+                d["transform"], bh[d["blas"]], instance_custom_index=d.get("custom", 0), mask=d.get("mask", 0xFFFFFFFF),
+                flags=d.get("flags", 0)) for d in insts]))
+        orc.add_tlas(insts)
+    sys.bind_tlas([th[i] for i in binding])
+    orc.bind(binding)
+    sys.create_comp_instance()
+    orc.build()
+    return sys, orc
+
+
+def batched_pair(scenes, opts):
+    """One BLAS (a triangle soup) and one identity-instance TLAS per scene,
+    TLAS k bound at index k, in the product system and in the oracle."""
+    eye = np.eye(4)
+    blas_list = [[(v, None, 1)] for v in scenes]
+    tlas_list = [[{"transform": eye, "blas": k}] for k in range(len(scenes))]
+    return build_pair(blas_list, tlas_list, list(range(len(scenes))), opts)

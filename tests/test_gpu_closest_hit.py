@@ -26,7 +26,7 @@ from rendiation_amd.acceleration import (
     TriangleGeometry,
 )
 from tests.parity import compare
-from tests.scenes import instanced_scene
+from tests.scenes import build_pair as _build_pair, instanced_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -35,29 +35,6 @@ def _say(msg):
     print(f"[gpu-case] {msg}", flush=True)
 
 
-def _build_pair(blas_list, tlas_list, binding, opts):
-    """Same scene in the product system and in the oracle.
-    blas_list: list of list of (pos, idx, flags); tlas_list: list of list of
-    instance dicts (transform, blas, custom, mask, flags)."""
-    sys = AccelerationStructureSystem(device=0, options=opts)
-    orc = OracleScene(opts.blas_bin_size, opts.blas_max_depth, opts.tlas_bin_size, opts.tlas_max_depth, opts.sah_bins)
-    bh = []
-    for geos in blas_list:
-        bh.append(sys.create_bottom_level_acceleration_structure(
-            [TriangleGeometry(p, i, f) for (p, i, f) in geos]))
-        orc.add_blas(geos)
-    th = []
-    for insts in tlas_list:
-        th.append(sys.create_top_level_acceleration_structure([
-            TopLevelAccelerationStructureSourceInstance(
-                d["transform"], bh[d["blas"]], instance_custom_index=d.get("custom", 0), mask=d.get("mask", 0xFFFFFFFF),
-                flags=d.get("flags", 0)) for d in insts]))
-        orc.add_tlas(insts)
-    sys.bind_tlas([th[i] for i in binding])
-    orc.bind(binding)
-    sys.create_comp_instance()
-    orc.build()
-    return sys, orc
 
 
 def _gpu_trace(sys, rays_np, device, meta_np=None, traversal=TRAVERSAL_FAST, **kw):

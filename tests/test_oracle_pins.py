@@ -242,3 +242,79 @@ def test_compaction_is_stable_copy_if():
         out, k = O.compact(mask)
         assert k == int(mask.sum())
         assert np.array_equal(out[:k], np.nonzero(mask)[0])
+
+
+# ------------------------------------------------------------ batch spawn --
+def test_batch_round_spawn_closed_forms():
+    """Continuation rays of the oracle's batch round, on a plane x + y = 0
+    instanced with scale (2, 1, 1) (world plane x/2 + y = 0, normal along
+    (1/2, 1, 0) by the inverse transpose): hit point on the plane, origin
+    offset to the side the ray came from, unit directions in that
+    hemisphere, cos ~ the cosine-weighted law (E cos = 2/3, cos^2 ~ U[0,1])."""
+    from tests.scenes import affine
+
+    big = np.array([[-50, 50, -50], [50, -50, -50], [0, 0, 80]], dtype=np.float32)  # in the plane x + y = 0
+    s = O.OracleScene()
+    s.add_blas([(big, None, 1)])
+    s.add_tlas([{"transform": affine(None, (2.0, 1.0, 1.0)), "blas": 0}])
+    s.bind([0])
+    s.build()
+    n = 40_000
+    nw = np.array([0.5, 1.0, 0.0]) / np.linalg.norm([0.5, 1.0, 0.0])
+    side = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+    tang = np.array([2.0, -1.0, 0.0]) / np.sqrt(5.0)
+    u = synth.uniform_f32(3, 40, 2 * n).reshape(n, 2).astype(np.float64)
+    o = side[:, None] * 3.0 * nw + (u[:, :1] - 0.5) * 4.0 * tang + (u[:, 1:] - 0.5) * 4.0 * np.array([0, 0, 1.0])
+    rays = np.zeros((n, 8), dtype=np.float32)
+    rays[:, 0:3] = o
+    rays[:, 3] = 1e-4
+    rays[:, 4:7] = -side[:, None] * nw
+    rays[:, 7] = np.inf
+    meta = np.zeros((n, 4), dtype=np.uint32)
+    meta[:, 1] = 0xFFFFFFFF
+    hits, sp, alive = s.batch_round(rays, meta, 1, 7)
+    assert alive.all() and np.all(hits[:, 0] != MISS)
+    t = hits[:, 1].view(np.float32).astype(np.float64)
+    assert np.allclose(t, 3.0, rtol=1e-5)
+    facing = side[:, None] * nw  # normal on the side of the ray origin
+    pos = rays[:, 0:3].astype(np.float64) + rays[:, 4:7].astype(np.float64) * t[:, None]
+    off = sp[:, 0:3].astype(np.float64) - pos
+    assert np.all(np.sum(off * facing, axis=1) > 0) and np.all(np.abs(off) < 1e-3)
+    d = sp[:, 4:7].astype(np.float64)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-6)
+    c = np.sum(d * facing, axis=1)
+    assert np.all(c >= 0)
+    assert abs(c.mean() - 2.0 / 3.0) < 5 * np.sqrt(1.0 / 18.0 / n)
+    # cos^2 = 1 - r^2 with r^2 uniform on [0, 1]: Kolmogorov-Smirnov distance
+    c2 = np.sort(c * c)
+    ks = np.max(np.abs(c2 - (np.arange(1, n + 1) - 0.5) / n))
+    assert ks < 1.63 / np.sqrt(n)
+    assert np.all(sp[:, 3] == np.float32(np.finfo(np.float32).eps)) and np.all(sp[:, 7] == np.finfo(np.float32).max)
+    # the sample depends on (ray, round, seed) only
+    _, sp2, _ = s.batch_round(rays, meta, 1, 7)
+    _, sp3, _ = s.batch_round(rays, meta, 2, 7)
+    assert np.array_equal(sp, sp2) and not np.array_equal(sp[:, 4:7], sp3[:, 4:7])
+
+
+def test_batch_round_per_ray_scene_selection():
+    """tlas_idx picks the scene per ray: the same rays against two bound
+    scenes equal plain traces of each scene; misses spawn nothing."""
+    a = synth.random_triangles(21, 3000, spread=0.05)
+    b = synth.random_triangles(22, 1500, spread=0.05)
+    s = O.OracleScene(4)
+    s.add_blas([(a, None, 1)])
+    s.add_blas([(b, None, 1)])
+    s.add_tlas([{"transform": np.eye(4), "blas": 0}])
+    s.add_tlas([{"transform": np.eye(4), "blas": 1}])
+    s.bind([0, 1])
+    s.build()
+    rays = synth.random_rays(23, 10_000)
+    meta = np.zeros((10_000, 4), dtype=np.uint32)
+    meta[:, 1] = 0xFFFFFFFF
+    meta[5000:, 2] = 1
+    hits, _, alive = s.batch_round(rays, meta, 1, 0)
+    h0, _, _ = s.trace(rays[:5000], tlas_idx=0)
+    h1, _, _ = s.trace(rays[5000:], tlas_idx=1)
+    assert np.array_equal(hits, np.concatenate([h0, h1]))
+    assert np.array_equal(alive, (hits[:, 0] != MISS).astype(np.uint32))
+    assert 0.05 < alive.mean() < 0.95
