@@ -368,26 +368,27 @@ def batched(args):
     total = int(sum(st_p.rays))
     value = total / (ms_p * 1e-3) / 1e6
 
-    # visit counts of the reference order on every round's rays
+    # visit counts of the reference order on every round's rays (all work on the system's stream)
     run(BATCH_PERSISTENT, 1, record=True)
     system.sync()
-    cnt = torch.empty((n, 3), dtype=torch.int32, device=dev)
-    hh = torch.empty((n, 4), dtype=torch.int32, device=dev)
     visits = np.zeros(3, dtype=np.int64)
-    hv = hits.view(bounces + 1, n, 4)
-    sv = spawned.view(bounces, n, 8)
-    for r in range(bounces + 1):
-        live = torch.ones(n, dtype=torch.bool, device=dev) if r == 0 else (hv[r - 1, :, 0] != -1)
-        idx = torch.nonzero(live).flatten()
-        if idx.numel() == 0:
-            break
-        src = rays if r == 0 else sv[r - 1]
-        rr = src[idx].contiguous()
-        mm = meta[idx].contiguous()
-        system.trace(rr.view(-1), hh[:idx.numel()].view(-1), ray_meta=mm.view(-1),
-                     counters=cnt[:idx.numel()].view(-1), traversal=TRAVERSAL_REFERENCE)
-        system.sync()
-        visits += cnt[:idx.numel()].to(torch.int64).sum(dim=0).cpu().numpy()
+    with torch.cuda.stream(stream):
+        cnt = torch.empty((n, 3), dtype=torch.int32, device=dev)
+        hh = torch.empty((n, 4), dtype=torch.int32, device=dev)
+        hv = hits.view(bounces + 1, n, 4)
+        sv = spawned.view(bounces, n, 8)
+        for r in range(bounces + 1):
+            live = torch.ones(n, dtype=torch.bool, device=dev) if r == 0 else (hv[r - 1, :, 0] != -1)
+            idx = torch.nonzero(live).flatten()
+            if idx.numel() == 0:
+                break
+            src = rays if r == 0 else sv[r - 1]
+            rr = src[idx].contiguous()
+            mm = meta[idx].contiguous()
+            system.trace(rr.view(-1), hh[:idx.numel()].view(-1), ray_meta=mm.view(-1),
+                         counters=cnt[:idx.numel()].view(-1), traversal=TRAVERSAL_REFERENCE)
+            visits += cnt[:idx.numel()].to(torch.int64).sum(dim=0).cpu().numpy()
+    system.sync()
     bytes_call = _bytes(total, visits)
     achieved = bytes_call / (ev_p * 1e-3) / 1e9
     config = {

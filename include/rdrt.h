@@ -241,7 +241,8 @@ typedef struct rdrt_pt_params {
   float sun_direction[3], sun_radiance[3]; /* directional light (direction of travel) */
   uint32_t use_graph;                      /* 1 = replay a captured hipGraph of the frame */
   uint32_t diagnostics;                    /* RDRT_PT_DIAG_* bits; any bit forces an eager frame */
-  uint32_t pad[2];
+  uint32_t record;                         /* 1 = keep the path vertices for rdrt_pt_backward */
+  uint32_t pad;
 } rdrt_pt_params;
 
 /* time every traversal launch of the frame with HIP events */
@@ -264,6 +265,24 @@ typedef struct rdrt_pt_stats {
 int32_t rdrt_pt_render(rdrt_ctx *ctx, const rdrt_pt_params *params, float *radiance_dev);
 /* Ray counts of the last frame (synchronizes). */
 int32_t rdrt_pt_get_stats(rdrt_ctx *ctx, rdrt_pt_stats *out);
+
+/* ---- differentiable path tracing (no counterpart in the reference: the
+ *      adjoint of the path tracer's frame radiance) ---- */
+/* Replaces the materials with device arrays (albedo and emission, n * 3 f32
+ * each), packed on the context's stream: no host round trip, so a caller
+ * can update them from an optimizer each step. */
+int32_t rdrt_pt_set_material_params(rdrt_ctx *ctx, const float *albedo_dev, const float *emission_dev, uint32_t n);
+/* Gradient of sum(adj * frame) over the last frame rendered with record = 1,
+ * w.r.t. the material albedo and emission (n * 3 f32 each, overwritten),
+ * with visibility and sampled directions held fixed.  adj_dev holds
+ * width * height RGB f32 (dLoss/dframe).  The frame must not use Russian
+ * roulette (rr_start >= max_bounces): its termination probability depends on
+ * the albedo. */
+int32_t rdrt_pt_backward(rdrt_ctx *ctx, const float *adj_dev, float *grad_albedo_dev, float *grad_emission_dev);
+/* L2 loss helper: adj_dev[i] = 2 (a[i] - b[i]); *loss_dev (one f64) =
+ * sum (a[i] - b[i])^2.  Device pointers; enqueued on the context's stream. */
+int32_t rdrt_l2_adjoint(rdrt_ctx *ctx, const float *a_dev, const float *b_dev, uint32_t n, float *adj_dev,
+                        double *loss_dev);
 
 /* ---- batched tracing: many scenes and uneven ray sets in one queue ---- */
 /* Each ray's rdrt_ray_meta.tlas_idx selects its scene among the bound TLAS

@@ -1153,8 +1153,17 @@ struct PtCounts {
   std::vector<uint64_t> ext, shadow;
 };
 
-// One path; returns its radiance.
-D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *ext_cnt, uint64_t *sh_cnt) {
+// One path vertex as the adjoint needs it: the material hit (or the sky the
+// ray escaped to) and the unoccluded light weight cos_l / pi (0 if none).
+constexpr uint32_t kVertSkyUp = 0xFFFFFFF0u, kVertSkyDown = 0xFFFFFFF1u;
+struct PtVert {
+  uint32_t code;
+  double w;
+};
+
+// One path; returns its radiance.  `rec`, if given, receives its vertices.
+D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *ext_cnt, uint64_t *sh_cnt,
+           std::vector<PtVert> *rec = nullptr) {
   const uint32_t pixel = path / P.spp;
   const uint32_t n = P.sample_index_base + path % P.spp;
   const uint32_t px = pixel % P.width, py = pixel / P.width;
@@ -1193,6 +1202,7 @@ D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *
     uint32_t ext[2] = {0, 0};
     if (!traverse_one(s, ray, h, hm, cnt, ext)) {
       L = L + beta * (ray.d.y > 0.0f ? sky_up : sky_down);
+      if (rec) rec->push_back({ray.d.y > 0.0f ? kVertSkyUp : kVertSkyDown, 0.0});
       break;
     }
     const auto &geo = s->geo_meta[ext[0]];
@@ -1217,6 +1227,7 @@ D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *
     const float *mat = &s->materials[8 * (size_t)mi];
     const D3 albedo = d3(mat[0], mat[1], mat[2]);
     L = L + beta * d3(mat[4], mat[5], mat[6]);
+    if (rec) rec->push_back({mi, 0.0});
     if (b == P.max_bounces) break;
     // offset origin along the geometric normal, on FP32 values
     const V3 ngf = v3((float)ng.x, (float)ng.y, (float)ng.z);
@@ -1236,7 +1247,10 @@ D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *
       orc_hit sh;
       orc_hit_meta shm;
       Counters sc;
-      if (!traverse_one(s, sr, sh, shm, sc)) L = L + beta * albedo * (cos_l / 3.14159265358979323846) * sun;
+      if (!traverse_one(s, sr, sh, shm, sc)) {
+        L = L + beta * albedo * (cos_l / 3.14159265358979323846) * sun;
+        if (rec) rec->back().w = cos_l / 3.14159265358979323846;
+      }
     }
     const double s0 = orc_van_der_corput(n, scramble_of(px, py, P.seed, 2 + 2 * b));
     const double s1 = orc_sobol(n, scramble_of(px, py, P.seed, 3 + 2 * b));
@@ -1320,6 +1334,78 @@ int orc_pt_render_rows(const orc_scene *s, const orc_pt_params *P, uint32_t row_
       ray_counts[b] = ext_all[b];
       ray_counts[nb + b] = sh_all[b];
     }
+  return 0;
+}
+
+// Adjoint of the frame radiance w.r.t. the material albedo and emission, in
+// FP64, by reverse accumulation over each path's recorded vertices (visibility
+// and sampled directions fixed).  With T_k the radiance from vertex k on
+// divided by the throughput beta_k:
+//   T_k = e_k + a_k * (w_k sun + T_{k+1}),  T_{m+1} = sky or 0,
+//   dC/de_k = beta_k,  dC/da_k = beta_k * (w_k sun + T_{k+1}).
+// Paths that Russian roulette could end are refused (rr_start < max_bounces):
+// their termination probability depends on the albedo.
+int orc_pt_grad(const orc_scene *s, const orc_pt_params *P, const double *adj, double *g_albedo, double *g_emission,
+                int nthreads) {
+  if (!s->built || s->materials.empty() || P->spp == 0 || P->max_bounces > 16) return -1;
+  if (P->rr_start < P->max_bounces) return -2;
+  const size_t nm = s->materials.size() / 8;
+  const uint32_t npix = P->width * P->height;
+  std::fill(g_albedo, g_albedo + 3 * nm, 0.0);
+  std::fill(g_emission, g_emission + 3 * nm, 0.0);
+  const D3 sky_up = d3(P->sky_up[0], P->sky_up[1], P->sky_up[2]);
+  const D3 sky_down = d3(P->sky_down[0], P->sky_down[1], P->sky_down[2]);
+  const D3 sun = d3(P->sun_radiance[0], P->sun_radiance[1], P->sun_radiance[2]);
+#ifdef _OPENMP
+  int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+#endif
+  {
+    std::vector<double> ga(3 * nm, 0.0), ge(3 * nm, 0.0);
+    std::vector<uint64_t> ext(P->max_bounces + 1, 0), sh(P->max_bounces + 1, 0);
+    std::vector<PtVert> rec;
+    std::vector<D3> beta;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+    for (int64_t pix = 0; pix < (int64_t)npix; pix++) {
+      const D3 a_pix = d3(adj[3 * pix], adj[3 * pix + 1], adj[3 * pix + 2]) * (1.0 / P->spp);
+      for (uint32_t k = 0; k < P->spp; k++) {
+        rec.clear();
+        pt_path(s, *P, (uint32_t)pix * P->spp + k, ext.data(), sh.data(), &rec);
+        beta.assign(1, d3(1, 1, 1));
+        size_t m = 0;  // vertices with a material
+        while (m < rec.size() && rec[m].code < kVertSkyUp) {
+          const float *mat = &s->materials[8 * (size_t)rec[m].code];
+          beta.push_back(beta.back() * d3(mat[0], mat[1], mat[2]));
+          m++;
+        }
+        D3 T = d3(0, 0, 0);
+        if (m < rec.size()) T = rec[m].code == kVertSkyUp ? sky_up : sky_down;
+        for (size_t j = m; j-- > 0;) {
+          const uint32_t mi = rec[j].code;
+          const float *mat = &s->materials[8 * (size_t)mi];
+          const D3 a = d3(mat[0], mat[1], mat[2]), e = d3(mat[4], mat[5], mat[6]);
+          const D3 lit = sun * rec[j].w;
+          const D3 da = a_pix * beta[j] * (lit + T), de = a_pix * beta[j];
+          ga[3 * mi] += da.x;
+          ga[3 * mi + 1] += da.y;
+          ga[3 * mi + 2] += da.z;
+          ge[3 * mi] += de.x;
+          ge[3 * mi + 1] += de.y;
+          ge[3 * mi + 2] += de.z;
+          T = e + a * (lit + T);
+        }
+      }
+    }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+    for (size_t i = 0; i < 3 * nm; i++) {
+      g_albedo[i] += ga[i];
+      g_emission[i] += ge[i];
+    }
+  }
   return 0;
 }
 

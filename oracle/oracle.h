@@ -159,7 +159,8 @@ typedef struct orc_pt_params {
   float sun_direction[3], sun_radiance[3];
   uint32_t use_graph;   /* ignored by the oracle */
   uint32_t diagnostics; /* ignored by the oracle */
-  uint32_t pad[2];
+  uint32_t record;      /* ignored by the oracle */
+  uint32_t pad;
 } orc_pt_params;
 
 int orc_set_vertex_normals(orc_scene *s, uint32_t blas, uint32_t geometry, const float *normals_xyz,
@@ -176,6 +177,13 @@ int orc_pt_render(const orc_scene *s, const orc_pt_params *params, double *out_r
 /* Rows [row_begin, row_end) of the same frame; out_rgb holds those rows. */
 int orc_pt_render_rows(const orc_scene *s, const orc_pt_params *params, uint32_t row_begin, uint32_t row_end,
                        double *out_rgb, uint64_t *ray_counts, int nthreads);
+
+/* Gradient of sum(adj * frame) w.r.t. material albedo and emission (nmat*3
+ * doubles each), FP64, visibility and sampled directions fixed.  adj holds
+ * width*height RGB doubles.  Returns -2 when Russian roulette is active
+ * (rr_start < max_bounces). */
+int orc_pt_grad(const orc_scene *s, const orc_pt_params *params, const double *adj, double *grad_albedo,
+                double *grad_emission, int nthreads);
 
 /* One round of batched tracing: hits[i] = closest hit of ray i (per-ray
  * TLAS from meta), alive[i] = 1 if it hit, and then spawned[i] = the

@@ -40,7 +40,7 @@ class OrcPtParams(C.Structure):
     _fields_ = [("view_projection_inv", f32 * 16), ("camera_position", f32 * 3), ("width", u32), ("height", u32),
                 ("spp", u32), ("max_bounces", u32), ("rr_start", u32), ("seed", u32), ("sample_index_base", u32),
                 ("sky_up", f32 * 3), ("sky_down", f32 * 3), ("sun_direction", f32 * 3), ("sun_radiance", f32 * 3),
-                ("use_graph", u32), ("diagnostics", u32), ("pad", u32 * 2)]
+                ("use_graph", u32), ("diagnostics", u32), ("record", u32), ("pad", u32)]
 
 
 _SIG = {
@@ -75,6 +75,7 @@ _SIG = {
     "orc_pt_render": (C.c_int, [P, C.POINTER(OrcPtParams), P, P, C.c_int]),
     "orc_pt_render_rows": (C.c_int, [P, C.POINTER(OrcPtParams), u32, u32, P, P, C.c_int]),
     "orc_batch_round": (C.c_int, [P, P, P, u32, u32, u32, P, P, P, C.c_int]),
+    "orc_pt_grad": (C.c_int, [P, C.POINTER(OrcPtParams), P, P, P, C.c_int]),
 }
 
 _lib = None
@@ -269,6 +270,19 @@ class OracleScene:
         if rc != 0:
             raise RuntimeError("oracle pt_render_rows failed")
         return out, counts[:nb], counts[nb:]
+
+    def pt_grad(self, params: "OrcPtParams", adj: np.ndarray, nmaterials: int, threads: int = 0):
+        """Gradient of sum(adj * frame) w.r.t. (albedo, emission), each
+        (nmaterials, 3) float64, in FP64 with visibility fixed."""
+        adj = np.ascontiguousarray(adj, dtype=np.float64).reshape(params.height, params.width, 3)
+        ga = np.zeros((nmaterials, 3), dtype=np.float64)
+        ge = np.zeros((nmaterials, 3), dtype=np.float64)
+        rc = self.L.orc_pt_grad(self.h, C.byref(params), adj.ctypes.data, ga.ctypes.data, ge.ctypes.data, threads)
+        if rc == -2:
+            raise ValueError("the gradient needs rr_start >= max_bounces (no Russian roulette)")
+        if rc != 0:
+            raise RuntimeError("oracle pt_grad failed")
+        return ga, ge
 
     def trace_brute(self, rays: np.ndarray, flags=0, cull_mask=0xFFFFFFFF, tlas_idx=0, threads: int = 0):
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)

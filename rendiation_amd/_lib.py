@@ -70,7 +70,7 @@ class RdrtPtParams(C.Structure):
     _fields_ = [("view_projection_inv", f32 * 16), ("camera_position", f32 * 3), ("width", u32), ("height", u32),
                 ("spp", u32), ("max_bounces", u32), ("rr_start", u32), ("seed", u32), ("sample_index_base", u32),
                 ("sky_up", f32 * 3), ("sky_down", f32 * 3), ("sun_direction", f32 * 3), ("sun_radiance", f32 * 3),
-                ("use_graph", u32), ("diagnostics", u32), ("pad", u32 * 2)]
+                ("use_graph", u32), ("diagnostics", u32), ("record", u32), ("pad", u32)]
 
 
 class RdrtPtStats(C.Structure):
@@ -124,6 +124,9 @@ SIGNATURES = {
     "rdrt_set_materials": (i32, [C.c_void_p, C.POINTER(RdrtMaterial), u32]),
     "rdrt_pt_render": (i32, [C.c_void_p, C.POINTER(RdrtPtParams), C.c_void_p]),
     "rdrt_pt_get_stats": (i32, [C.c_void_p, C.POINTER(RdrtPtStats)]),
+    "rdrt_pt_set_material_params": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, u32]),
+    "rdrt_pt_backward": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "rdrt_l2_adjoint": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p]),
     "rdrt_trace_batched": (i32, [C.c_void_p, C.POINTER(RdrtBatchParams)]),
     "rdrt_batch_get_stats": (i32, [C.c_void_p, C.POINTER(RdrtBatchStats)]),
     "rdrt_compact": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p]),

@@ -156,6 +156,7 @@ class AccelerationStructureSystem:
         L.check(self._L.rdrt_ctx_create(C.byref(cfg), C.byref(h)))
         self._h = h
         self.device = device
+        self.stream = stream  # HIP stream handle the work is enqueued on (None: a stream of the context)
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -14,7 +14,7 @@ namespace rdrt {
 
 void PtState::free_all() {
   for (Scratch *s : {&rays, &shadow_rays, &hits, &shadow_hits, &hitx, &beta, &L, &nee, &qa, &qb, &qs, &counts,
-                     &lookback, &tickets, &materials, &diag_counters, &diag_totals})
+                     &lookback, &tickets, &materials, &diag_counters, &diag_totals, &record})
     s->free();
   for (hipEvent_t e : ev_begin) (void)hipEventDestroy(e);
   for (hipEvent_t e : ev_end) (void)hipEventDestroy(e);
@@ -177,6 +177,10 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
   L.lookback_stride = (npaths + kPtBlock - 1) / kPtBlock;
   L.status = c->status;
   L.out = radiance;
+  if (prm->record) {
+    RDRT_HIP(s.record.reserve((size_t)(prm->max_bounces + 2) * npaths * sizeof(uint2)));
+    L.record = (uint2 *)s.record.ptr;
+  }
 
   const TraceParams tp = scene_trace_params(c);
   const uint32_t block = c->cfg.block_size ? c->cfg.block_size : 256;
@@ -234,6 +238,49 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
   }
   RDRT_HIP(hipEventRecord(c->ev1, st));
   s.last_bounces = prm->max_bounces;
+  s.last_frame = L;
+  s.last_recorded = prm->record != 0;
+  return RDRT_OK;
+}
+
+int32_t rdrt_pt_set_material_params(rdrt_ctx *c, const float *albedo, const float *emission, uint32_t n) {
+  if (!c || !albedo || !emission) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  if (n == 0 || n >= kVertSkyUp) return fail(RDRT_ERR_INVALID_ARG, "material count out of range");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  PtState &s = c->pt;
+  RDRT_HIP(s.materials.reserve((size_t)n * sizeof(Material)));
+  RDRT_HIP(enqueue_pack_materials(albedo, emission, n, (Material *)s.materials.ptr, c->work_stream));
+  // the device copy is now the authoritative one; the host list keeps the count
+  c->materials.assign(n, Material{});
+  s.nmaterials_uploaded = n;
+  s.materials_dirty = false;
+  return RDRT_OK;
+}
+
+int32_t rdrt_pt_backward(rdrt_ctx *c, const float *adj, float *grad_albedo, float *grad_emission) {
+  if (!c || !adj || !grad_albedo || !grad_emission) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  PtState &s = c->pt;
+  if (!s.last_recorded || !s.last_frame.record)
+    return fail(RDRT_ERR_STATE, "no recorded frame (render with record = 1 first)");
+  if (s.last_frame.rr_start < s.last_frame.max_bounces)
+    return fail(RDRT_ERR_UNSUPPORTED, "the adjoint needs a frame without Russian roulette (rr_start >= max_bounces)");
+  PtLaunch L = s.last_frame;
+  // materials as they are now (set_material_params may have moved them)
+  L.materials = (const Material *)s.materials.ptr;
+  if (s.nmaterials_uploaded != L.nmaterials)
+    return fail(RDRT_ERR_STATE, "the material count changed since the recorded frame");
+  RDRT_HIP(enqueue_pt_backward(L, adj, grad_albedo, grad_emission, c->work_stream));
+  return RDRT_OK;
+}
+
+int32_t rdrt_l2_adjoint(rdrt_ctx *c, const float *a, const float *b, uint32_t n, float *adj, double *loss) {
+  if (!c || !loss || (n && (!a || !b || !adj))) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  RDRT_HIP(enqueue_l2_adjoint(a, b, n, adj, loss, c->work_stream));
   return RDRT_OK;
 }
 

@@ -83,7 +83,7 @@ struct PtState {
   uint32_t capacity = 0;  // paths
   uint32_t pixels = 0;
   Scratch rays, shadow_rays, hits, shadow_hits, hitx, beta, L, nee, qa, qb, qs, counts, lookback, tickets;
-  Scratch materials;
+  Scratch materials, record;
   uint32_t nmaterials_uploaded = 0;
   bool materials_dirty = true;
   hipGraph_t graph = nullptr;
@@ -96,6 +96,9 @@ struct PtState {
   std::vector<hipEvent_t> ev_begin, ev_end;
   Scratch diag_counters, diag_totals;
   uint32_t last_diag = 0, last_launches = 0;
+  // last frame (for the adjoint)
+  PtLaunch last_frame{};
+  bool last_recorded = false;
   void free_all();
 };
 

@@ -45,7 +45,14 @@ struct PtLaunch {
   uint32_t lookback_stride;
   uint32_t *status;
   float *out;  // width * height * 3
+
+  // path vertices for the adjoint, or null: slot k of path p at
+  // record[k * npaths + p], max_bounces + 2 slots; (material, cos_l / pi of
+  // unoccluded light or 0), or (kVertSkyUp / kVertSkyDown / kVertEnd, 0)
+  uint2 *record;
 };
+
+constexpr uint32_t kVertSkyUp = 0xFFFFFFF0u, kVertSkyDown = 0xFFFFFFF1u, kVertEnd = 0xFFFFFFF2u;
 
 // Optional per-frame diagnostics (eager frames only).
 struct PtDiag {
@@ -58,6 +65,17 @@ struct PtDiag {
 // nodes, no host round trips).
 hipError_t enqueue_pt_frame(const PtLaunch &L, const TraceParams &scene_tp, uint32_t push_depth, uint32_t block,
                             hipStream_t stream, const PtDiag *diag = nullptr);
+
+// Gradient of sum(adj * frame) w.r.t. material albedo and emission from the
+// recorded vertices of frame L (grads zeroed here, then accumulated).
+hipError_t enqueue_pt_backward(const PtLaunch &L, const float *adj, float *grad_albedo, float *grad_emission,
+                               hipStream_t stream);
+// Packs device albedo / emission arrays (n * 3 each) into Material records.
+hipError_t enqueue_pack_materials(const float *albedo, const float *emission, uint32_t n, Material *out,
+                                  hipStream_t stream);
+// adj[i] = 2 (a[i] - b[i]); *loss = sum (a[i] - b[i])^2.
+hipError_t enqueue_l2_adjoint(const float *a, const float *b, uint32_t n, float *adj, double *loss,
+                              hipStream_t stream);
 
 // Stable stream compaction of indices whose mask is non-zero (single pass,
 // decoupled look-back); out_count receives the kept count.  `scratch` holds

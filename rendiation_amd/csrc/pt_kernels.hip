@@ -18,6 +18,8 @@
 // is already running.  The output equals a stable copy_if
 // (shader/parallel-compute/src/stream_compaction.rs:3-45).
 
+#include <algorithm>
+
 #include "compact_device.h"
 #include "pt.h"
 #include "pt_device.h"
@@ -102,10 +104,12 @@ __device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &ke
   const V3 o = v3(r0.x, r0.y, r0.z), d = v3(r1.x, r1.y, r1.z);
   V3 beta = load3(P.beta + 3 * (size_t)path);
   V3 L = load3(P.L + 3 * (size_t)path);
+  uint2 *rec = P.record ? P.record + (size_t)b * P.npaths + path : nullptr;
   if (h.x == kMiss) {
     const float *sky = d.y > 0.0f ? P.sky_up : P.sky_down;
     L = L + beta * load3(sky);
     store3(P.L + 3 * (size_t)path, L);
+    if (rec) *rec = make_uint2(d.y > 0.0f ? kVertSkyUp : kVertSkyDown, 0u);
     return;
   }
   const float t = __uint_as_float(h.y), bu = __uint_as_float(h.z), bv = __uint_as_float(h.w);
@@ -134,6 +138,7 @@ __device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &ke
   const Material &m = P.materials[mi];
   const V3 albedo = v3(m.albedo[0], m.albedo[1], m.albedo[2]);
   L = L + beta * v3(m.emission[0], m.emission[1], m.emission[2]);
+  float w_light = 0.0f;
   if (b < P.max_bounces) {
     uint32_t px, py, n;
     pixel_of(P, path, px, py, n);
@@ -147,6 +152,7 @@ __device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &ke
       store3(P.nee + 3 * (size_t)path, c);
       write_ray(P.shadow_rays + path, origin, kFltEps, wi, kFltMax);
       shadow = true;
+      w_light = cos_l * kInvPi;
     }
     // Lambertian importance sample: f cos / pdf = albedo
     const float s0 = pt::van_der_corput(n, pt::scramble(px, py, P.seed, 2 + 2 * b));
@@ -167,6 +173,11 @@ __device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &ke
     }
   }
   store3(P.L + 3 * (size_t)path, L);
+  if (rec) {
+    // the light weight stands until the shadow ray finds an occluder
+    *rec = make_uint2(mi, __float_as_uint(w_light));
+    if (!keep) rec[P.npaths] = make_uint2(kVertEnd, 0u);
+  }
 }
 
 template <int B>
@@ -196,7 +207,10 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_nee(PtLaunch P, uint32_t b) {
   const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
   if (i >= P.counts[P.max_bounces + 2 + b] || i >= P.npaths) return;
   const uint32_t path = P.qs[i];
-  if (P.shadow_hits[path].prim_id != kMiss) return;
+  if (P.shadow_hits[path].prim_id != kMiss) {
+    if (P.record) P.record[(size_t)b * P.npaths + path].y = 0u;
+    return;
+  }
   V3 L = load3(P.L + 3 * (size_t)path) + load3(P.nee + 3 * (size_t)path);
   store3(P.L + 3 * (size_t)path, L);
 }
@@ -245,6 +259,135 @@ __global__ __launch_bounds__(kPtBlock) void k_compact(const uint32_t *mask, uint
   const bool keep = i < n && mask[i] != 0u;
   compact2<kPtBlock>(keep, false, i, out, out, status, bid, nblocks, out_count, nullptr, err, n);
 }
+
+// ---- adjoint ----
+// Per path, from its recorded vertices k = 0..m-1 (materials a_k, e_k, light
+// weights w_k) and the radiance T_m beyond the last (sky or 0), with
+// beta_0 = 1, beta_{k+1} = beta_k a_k and T_k = e_k + a_k (w_k sun + T_{k+1}):
+//   dC/de_k = beta_k,   dC/da_k = beta_k (w_k sun + T_{k+1}),
+// scaled by the pixel's adjoint / spp.  Fully unrolled over S slots so the
+// per-vertex arrays stay in registers.  Gradients gather in LDS (when
+// `lds`) and are flushed with one global atomic add per touched entry.
+template <int S>
+__global__ __launch_bounds__(kPtBlock) void k_pt_backward(PtLaunch P, const float *adj, float *ga, float *ge,
+                                                          bool lds) {
+  extern __shared__ float g_lds[];
+  const uint32_t nm = P.nmaterials, slots = P.max_bounces + 2;
+  if (lds) {
+    for (uint32_t i = threadIdx.x; i < 6 * nm; i += kPtBlock) g_lds[i] = 0.0f;
+    __syncthreads();
+  }
+  float *g_a = lds ? g_lds : ga;
+  float *g_e = lds ? g_lds + 3 * nm : ge;
+  const V3 sun = load3(P.sun_radiance);
+  const float inv_spp = 1.0f / (float)P.spp;
+  for (uint32_t path = blockIdx.x * kPtBlock + threadIdx.x; path < P.npaths; path += gridDim.x * kPtBlock) {
+    uint32_t code[S];
+    float w[S];
+    uint32_t m = 0, tail = kVertEnd;
+    bool open = true;
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+      code[k] = 0u;
+      w[k] = 0.0f;
+      if (open && (uint32_t)k < slots) {
+        const uint2 r = P.record[(size_t)k * P.npaths + path];
+        if (r.x >= kVertSkyUp) {
+          tail = r.x;
+          open = false;
+        } else {
+          code[k] = r.x < nm ? r.x : nm - 1;
+          w[k] = __uint_as_float(r.y);
+          m = k + 1;
+        }
+      }
+    }
+    if (m == 0) continue;
+    const uint32_t pixel = path / P.spp;
+    const V3 a_pix = load3(adj + 3 * (size_t)pixel) * inv_spp;
+    V3 beta[S];
+    beta[0] = v3(1.0f, 1.0f, 1.0f);
+#pragma unroll
+    for (int k = 0; k + 1 < S; k++)
+      if ((uint32_t)k < m) {
+        const Material &mt = P.materials[code[k]];
+        beta[k + 1] = beta[k] * v3(mt.albedo[0], mt.albedo[1], mt.albedo[2]);
+      }
+    V3 T = tail == kVertSkyUp ? load3(P.sky_up) : tail == kVertSkyDown ? load3(P.sky_down) : v3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int k = S - 1; k >= 0; k--)
+      if ((uint32_t)k < m) {
+        const uint32_t mi = code[k];
+        const Material &mt = P.materials[mi];
+        const V3 a = v3(mt.albedo[0], mt.albedo[1], mt.albedo[2]);
+        const V3 e = v3(mt.emission[0], mt.emission[1], mt.emission[2]);
+        const V3 lit = sun * w[k];
+        const V3 de = a_pix * beta[k];
+        const V3 da = de * (lit + T);
+        if (lds) {
+          atomicAdd(g_a + 3 * mi, da.x);
+          atomicAdd(g_a + 3 * mi + 1, da.y);
+          atomicAdd(g_a + 3 * mi + 2, da.z);
+          atomicAdd(g_e + 3 * mi, de.x);
+          atomicAdd(g_e + 3 * mi + 1, de.y);
+          atomicAdd(g_e + 3 * mi + 2, de.z);
+        } else {
+          unsafeAtomicAdd(g_a + 3 * mi, da.x);
+          unsafeAtomicAdd(g_a + 3 * mi + 1, da.y);
+          unsafeAtomicAdd(g_a + 3 * mi + 2, da.z);
+          unsafeAtomicAdd(g_e + 3 * mi, de.x);
+          unsafeAtomicAdd(g_e + 3 * mi + 1, de.y);
+          unsafeAtomicAdd(g_e + 3 * mi + 2, de.z);
+        }
+        T = e + a * (lit + T);
+      }
+  }
+  if (lds) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 6 * nm; i += kPtBlock) {
+      const float v = g_lds[i];
+      if (v != 0.0f) unsafeAtomicAdd(i < 3 * nm ? ga + i : ge + (i - 3 * nm), v);
+    }
+  }
+}
+
+__global__ void k_fill_f32(float *p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.0f;
+}
+
+__global__ void k_pack_materials(const float *albedo, const float *emission, uint32_t n, Material *out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Material m;
+  for (int c = 0; c < 3; c++) {
+    m.albedo[c] = albedo[3 * (size_t)i + c];
+    m.emission[c] = emission[3 * (size_t)i + c];
+  }
+  m.pad0 = 0.0f;
+  m.pad1 = 0.0f;
+  out[i] = m;
+}
+
+__global__ __launch_bounds__(kPtBlock) void k_l2_adjoint(const float *a, const float *b, uint32_t n, float *adj,
+                                                         double *loss) {
+  __shared__ double s[kPtBlock / 64];
+  double acc = 0.0;
+  for (uint32_t i = blockIdx.x * kPtBlock + threadIdx.x; i < n; i += gridDim.x * kPtBlock) {
+    const float d = a[i] - b[i];
+    adj[i] = 2.0f * d;
+    acc += (double)d * (double)d;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63u) == 0) s[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < kPtBlock / 64; k++) t += s[k];
+    unsafeAtomicAdd(loss, t);
+  }
+}
+
+__global__ void k_zero_f64(double *p) { *p = 0.0; }
 
 __global__ void k_sobol(const uint32_t *index, const uint32_t *scr, uint32_t n, float *out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -306,6 +449,40 @@ hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_
     hipLaunchKernelGGL(k_pt_nee, dim3(grid), dim3(kPtBlock), 0, st, P, b);
   }
   hipLaunchKernelGGL(k_pt_finalize, dim3((P.width * P.height + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, st, P);
+  return hipGetLastError();
+}
+
+hipError_t enqueue_pt_backward(const PtLaunch &P, const float *adj, float *ga, float *ge, hipStream_t st) {
+  const uint32_t nm = P.nmaterials;
+  hipLaunchKernelGGL(k_fill_f32, dim3(64), dim3(256), 0, st, ga, 3 * nm);
+  hipLaunchKernelGGL(k_fill_f32, dim3(64), dim3(256), 0, st, ge, 3 * nm);
+  const size_t lds_bytes = (size_t)6 * nm * sizeof(float);
+  const bool lds = lds_bytes <= 48 * 1024;
+  const uint32_t grid = std::min<uint32_t>((P.npaths + kPtBlock - 1) / kPtBlock, 2048u);
+  const uint32_t slots = P.max_bounces + 2;
+  const size_t sh = lds ? lds_bytes : 0;
+  if (slots <= 4)
+    hipLaunchKernelGGL(k_pt_backward<4>, dim3(grid), dim3(kPtBlock), sh, st, P, adj, ga, ge, lds);
+  else if (slots <= 8)
+    hipLaunchKernelGGL(k_pt_backward<8>, dim3(grid), dim3(kPtBlock), sh, st, P, adj, ga, ge, lds);
+  else
+    hipLaunchKernelGGL(k_pt_backward<kPtMaxBounces + 2>, dim3(grid), dim3(kPtBlock), sh, st, P, adj, ga, ge, lds);
+  return hipGetLastError();
+}
+
+hipError_t enqueue_pack_materials(const float *albedo, const float *emission, uint32_t n, Material *out,
+                                  hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pack_materials, dim3((n + 255) / 256), dim3(256), 0, st, albedo, emission, n, out);
+  return hipGetLastError();
+}
+
+hipError_t enqueue_l2_adjoint(const float *a, const float *b, uint32_t n, float *adj, double *loss,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, st, loss);
+  if (n == 0) return hipGetLastError();
+  const uint32_t grid = std::min<uint32_t>((n + kPtBlock - 1) / kPtBlock, 1024u);
+  hipLaunchKernelGGL(k_l2_adjoint, dim3(grid), dim3(kPtBlock), 0, st, a, b, n, adj, loss);
   return hipGetLastError();
 }
 

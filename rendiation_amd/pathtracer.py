@@ -89,6 +89,7 @@ class PathTracingConfig:
     sun_radiance: Sequence[float] = (3.0, 3.0, 3.0)
     use_graph: bool = True
     diagnostics: int = 0  # L.RDRT_PT_DIAG_* bits (eager frame)
+    record: bool = False  # keep the path vertices for PathTracer.backward
 
     def to_c(self, camera: Camera) -> L.RdrtPtParams:
         p = L.RdrtPtParams()
@@ -105,6 +106,7 @@ class PathTracingConfig:
         p.sun_radiance[:] = [float(x) for x in self.sun_radiance]
         p.use_graph = 1 if self.use_graph else 0
         p.diagnostics = self.diagnostics
+        p.record = 1 if self.record else 0
         return p
 
 
@@ -113,6 +115,7 @@ class PathTracer:
 
     def __init__(self, system: AccelerationStructureSystem):
         self.system = system
+        self.recorded_frames = 0  # frames rendered with record = True
 
     def render(self, camera: Camera, cfg: PathTracingConfig, out=None):
         """Render one frame; returns the (height, width, 3) float32 device
@@ -125,7 +128,39 @@ class PathTracer:
             raise ValueError("out must be a contiguous height*width*3 float32 tensor")
         prm = cfg.to_c(camera)
         L.check(self.system._L.rdrt_pt_render(self.system._h, C.byref(prm), int(out.data_ptr())))
+        if cfg.record:
+            self.recorded_frames += 1
         return out
+
+    def set_material_params(self, albedo, emission) -> None:
+        """Replace the materials with device tensors (n, 3) float32 each,
+        copied on the system's stream (no host round trip)."""
+        for t, name in ((albedo, "albedo"), (emission, "emission")):
+            if not getattr(t, "is_cuda", False) or not t.is_contiguous() or t.dtype != _f32():
+                raise ValueError(f"{name} must be a contiguous float32 device tensor")
+        n = albedo.numel() // 3
+        if albedo.numel() != 3 * n or emission.numel() != 3 * n:
+            raise ValueError("albedo and emission must both be (n, 3)")
+        L.check(self.system._L.rdrt_pt_set_material_params(self.system._h, int(albedo.data_ptr()),
+                                                            int(emission.data_ptr()), n))
+
+    def backward(self, adj, grad_albedo, grad_emission) -> None:
+        """Gradient of sum(adj * frame) of the last recorded frame w.r.t. the
+        material albedo and emission (written to the (n, 3) float32 tensors)."""
+        for t in (adj, grad_albedo, grad_emission):
+            if not getattr(t, "is_cuda", False) or not t.is_contiguous() or t.dtype != _f32():
+                raise ValueError("backward takes contiguous float32 device tensors")
+        L.check(self.system._L.rdrt_pt_backward(self.system._h, int(adj.data_ptr()), int(grad_albedo.data_ptr()),
+                                                int(grad_emission.data_ptr())))
+
+    def l2_adjoint(self, image, target, adj, loss) -> None:
+        """adj = 2 (image - target); loss (one float64 device element) =
+        sum (image - target)^2, enqueued on the system's stream."""
+        n = image.numel()
+        if target.numel() != n or adj.numel() != n or loss.numel() != 1:
+            raise ValueError("shape mismatch")
+        L.check(self.system._L.rdrt_l2_adjoint(self.system._h, int(image.data_ptr()), int(target.data_ptr()), n,
+                                               int(adj.data_ptr()), int(loss.data_ptr())))
 
     def stats(self) -> dict:
         s = L.RdrtPtStats()
@@ -134,6 +169,61 @@ class PathTracer:
         return {"extension_rays": list(s.extension_rays[:nb]), "shadow_rays": list(s.shadow_rays[:nb]),
                 "frame_ms": float(s.frame_ms), "trace_ms": float(s.trace_ms), "trace_launches": int(s.trace_launches),
                 "visits": [int(x) for x in s.visits]}
+
+
+def _f32():
+    import torch
+
+    return torch.float32
+
+
+def render_materials(tracer: PathTracer, camera: Camera, cfg: PathTracingConfig, albedo, emission):
+    """Differentiable frame: an autograd op of (albedo, emission) -> frame
+    (height, width, 3), whose backward runs the adjoint kernel on the
+    recorded path vertices (visibility and sampled directions fixed)."""
+    import dataclasses
+
+    import torch
+
+    def ordered(fn):
+        # torch's stream and the system's stream: wait on both sides unless they are one stream
+        cur = torch.cuda.current_stream()
+        same = tracer.system.stream == cur.cuda_stream
+        if not same:
+            cur.synchronize()
+        out = fn()
+        if not same:
+            tracer.system.sync()
+        return out
+
+    class _Render(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, a, e):
+            a = a.detach().contiguous()
+            e = e.detach().contiguous()
+            img = torch.empty((cfg.height, cfg.width, 3), dtype=torch.float32, device=a.device)
+
+            def run():
+                tracer.set_material_params(a, e)
+                return tracer.render(camera, dataclasses.replace(cfg, record=True), out=img)
+
+            ctx.shapes = (a.shape, e.shape)
+            ctx.device = a.device
+            out = ordered(run)
+            ctx.frame = tracer.recorded_frames
+            return out
+
+        @staticmethod
+        def backward(ctx, grad):
+            if ctx.frame != tracer.recorded_frames:
+                raise RuntimeError("the path vertices of this frame were replaced by a later recorded frame")
+            ga = torch.empty(ctx.shapes[0], dtype=torch.float32, device=ctx.device)
+            ge = torch.empty(ctx.shapes[1], dtype=torch.float32, device=ctx.device)
+            g = grad.contiguous()
+            ordered(lambda: tracer.backward(g, ga, ge))
+            return ga, ge
+
+    return _Render.apply(albedo, emission)
 
 
 def load_scene(system: AccelerationStructureSystem, scene: dict) -> None:
