@@ -1,6 +1,6 @@
 """Benchmark of the ray-query hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pt|closest_hit|batched]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pt|closest_hit|batched|diff]
 
 pt (default): wavefront path tracer on the parametric scene -- 64 UV spheres
   and tori tessellated at 256x256 (8,388,608 triangles, one BLAS each, 64
@@ -15,6 +15,11 @@ batched: 64 random-triangle scenes (triangles log-uniform in [10k, 2M), rays
   log-uniform in [16k, 4M)), 3 bounces of cosine-weighted continuation rays
   with per-bounce compaction; one step = one graph-replayed persistent call
   over all scenes, compared with the per-scene sequential launches.
+diff: differentiable path tracing -- 64 spheres/tori at 90x90 (1.04M
+  triangles), 256 materials, 512x512 at 4 spp, 3 bounces; L2 loss against a
+  target rendered from parameters perturbed by N(0, 0.05); value = ms of
+  forward + adjoint per frame (lower is better), with the forward alone and
+  the gradient error against the FP64 oracle beside it.
 
 roofline: HBM-bound.  Algorithmic bytes per traced ray = 32 (ray) + 16 (hit)
 + 48 x node visits + 52 x triangle tests + 176 x instance tests, with visit
@@ -299,6 +304,142 @@ def cpu_baseline_closest_hit(pos, rays_np):
     }
 
 
+# ------------------------------------------------------- differentiable PT
+def differentiable(args):
+    import dataclasses
+
+    import numpy as np
+    import torch
+
+    from rendiation_amd import synth
+    from rendiation_amd.acceleration import AccelerationStructureSystem
+    from rendiation_amd.pathtracer import Camera, PathTracer, PathTracingConfig, load_scene
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    scene = synth.pt_scene(seed=0, n_meshes=64, tess=90, materials_per_mesh=4)
+    system = AccelerationStructureSystem(device=0, stream=stream.cuda_stream)
+    t0 = time.perf_counter()
+    load_scene(system, scene)
+    build_upload_ms = (time.perf_counter() - t0) * 1e3
+    info = system.scene_info()
+    cam = Camera(aspect=1.0)
+    cfg = PathTracingConfig(width=512, height=512, spp=4, max_bounces=3, rr_start=3, seed=0, use_graph=True)
+    tracer = PathTracer(system)
+    nmat = scene["albedo"].shape[0]
+    rng = np.random.default_rng(0)
+    a_t = np.clip(scene["albedo"] + rng.normal(0.0, 0.05, scene["albedo"].shape), 0.01, 0.99).astype(np.float32)
+    e_t = np.maximum(scene["emission"] + rng.normal(0.0, 0.05, scene["emission"].shape), 0.0).astype(np.float32)
+    with torch.cuda.stream(stream):
+        albedo = torch.from_numpy(np.asarray(scene["albedo"], np.float32)).to(dev)
+        emission = torch.from_numpy(np.asarray(scene["emission"], np.float32)).to(dev)
+        target = torch.empty((cfg.height, cfg.width, 3), dtype=torch.float32, device=dev)
+        img = torch.empty_like(target)
+        adj = torch.empty_like(target)
+        loss = torch.zeros(1, dtype=torch.float64, device=dev)
+        ga = torch.empty_like(albedo)
+        ge = torch.empty_like(emission)
+        tracer.set_material_params(torch.from_numpy(a_t).to(dev), torch.from_numpy(e_t).to(dev))
+        tracer.render(cam, cfg, out=target)
+    system.sync()
+    fwd_cfg = cfg
+    rec_cfg = dataclasses.replace(cfg, record=True)
+
+    def forward():
+        tracer.set_material_params(albedo, emission)
+        tracer.render(cam, fwd_cfg, out=img)
+
+    def forward_backward():
+        tracer.set_material_params(albedo, emission)
+        tracer.render(cam, rec_cfg, out=img)
+        tracer.l2_adjoint(img, target, adj, loss)
+        tracer.backward(adj, ga, ge)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        system.sync()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        w0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            fn()
+        e1.record(stream)
+        system.sync()
+        return (time.perf_counter() - w0) * 1e3 / args.steps, e0.elapsed_time(e1) / args.steps
+
+    ms_f, ev_f = timed(forward)
+    ms_fb, ev_fb = timed(forward_backward)
+    # backward kernel alone, HIP events on the system's stream
+    forward_backward()
+    system.sync()
+    b0 = torch.cuda.Event(enable_timing=True)
+    b1 = torch.cuda.Event(enable_timing=True)
+    b0.record(stream)
+    for _ in range(args.steps):
+        tracer.backward(adj, ga, ge)
+    b1.record(stream)
+    system.sync()
+    bwd_ms = b0.elapsed_time(b1) / args.steps
+    st = tracer.stats()
+    npaths = cfg.width * cfg.height * cfg.spp
+    # backward bytes: vertex records (8 B x slots), pixel adjoint (12 B / path), gradient flush (24 B / material / block)
+    slots = cfg.max_bounces + 2
+    blocks = min((npaths + 255) // 256, 2048)
+    bwd_bytes = npaths * (8 * slots + 12) + 24 * nmat * blocks
+    achieved = bwd_bytes / (bwd_ms * 1e-3) / 1e9
+    config = {
+        "workload": "diff: 64 spheres/tori at 90x90 (1.04M tris), 256 materials, 512x512 x 4 spp, 3 bounces, "
+                    "target from params + N(0, 0.05), L2 loss, grad w.r.t. 256x6 params",
+        "tris": info.tris, "materials": nmat, "paths": npaths, "ms_forward": round(ms_f, 4),
+        "ms_forward_events": round(ev_f, 4), "ms_forward_backward": round(ms_fb, 4),
+        "ms_forward_backward_events": round(ev_fb, 4), "fwd_bwd_over_fwd": round(ms_fb / ms_f, 4),
+        "ms_backward_kernel": round(bwd_ms, 4), "loss": float(loss.item()),
+        "extension_rays": st["extension_rays"], "build_upload_ms": round(build_upload_ms, 1),
+    }
+    roofline = {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": "k_pt_backward",
+        "bytes_per_launch": bwd_bytes,
+    }
+    extra = {}
+    if not args.no_cpu_baseline:
+        extra["cpu_baseline"], extra["gradient_error"] = cpu_baseline_diff(scene, cam, cfg, adj, ga, ge)
+    system.close()
+    rec = _record(ms_fb, "ms", args, ms_fb, config, roofline, extra)
+    rec["higher_is_better"] = False
+    return rec
+
+
+def cpu_baseline_diff(scene, cam, cfg, adj, ga, ge):
+    """FP64 oracle gradient of the same frame (full workload, all host
+    threads): its time, and the relative error of the GPU gradient."""
+    import numpy as np
+
+    from tests.pt_util import oracle_params, oracle_scene_from
+
+    o = oracle_scene_from(scene)
+    prm = oracle_params(cam, cfg)
+    threads = _host_threads()
+    adj_np = adj.cpu().numpy().astype(np.float64)
+    t0 = time.perf_counter()
+    oa, oe = o.pt_grad(prm, adj_np, scene["albedo"].shape[0], threads=threads)
+    dt = time.perf_counter() - t0
+    g_a = ga.cpu().numpy().astype(np.float64)
+    g_e = ge.cpu().numpy().astype(np.float64)
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+    npaths = cfg.width * cfg.height * cfg.spp
+    base = {"value": round(dt * 1e3, 1), "unit": "ms (FP64 forward replay + adjoint)", "cores": threads,
+            "kind": "port", "sample": f"full workload ({npaths} paths)", "cpu_model": _cpu_model()}
+    err = {"vs": "FP64 CPU oracle gradient, same adjoint image", "rel_err_albedo": rel(g_a, oa),
+           "rel_err_emission": rel(g_e, oe), "bound": 1e-3}
+    return base, err
+
+
 # ------------------------------------------------------------------ batched
 def batched(args):
     import numpy as np
@@ -454,14 +595,26 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt", choices=["pt", "closest_hit", "batched"])
+    ap.add_argument("--workload", default="pt", choices=["pt", "closest_hit", "batched", "diff"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one 256 MiB device-to-device copy (a reference dispatch for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see the sharding design)",
               file=sys.stderr)
         sys.exit(2)
-    rec = {"pt": path_tracing, "closest_hit": closest_hit, "batched": batched}[args.workload](args)
+    if args.copy_calibration:
+        import torch
+
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    rec = {"pt": path_tracing, "closest_hit": closest_hit, "batched": batched,
+           "diff": differentiable}[args.workload](args)
     print(json.dumps(rec))
 
 

@@ -243,6 +243,13 @@ typedef struct rdrt_pt_params {
   uint32_t diagnostics;                    /* RDRT_PT_DIAG_* bits; any bit forces an eager frame */
   uint32_t record;                         /* 1 = keep the path vertices for rdrt_pt_backward */
   uint32_t pad;
+  /* Several cameras in one frame: views > 1 renders views * height * width
+   * pixels (view-major); view_params (host) holds 20 f32 per view:
+   * view_projection_inv[16], camera_position[3], pad.  View v scrambles its
+   * samples with seed + v.  views <= 1 with view_params NULL: the camera above. */
+  uint32_t views;
+  uint32_t pad2;
+  const float *view_params;
 } rdrt_pt_params;
 
 /* time every traversal launch of the frame with HIP events */
@@ -260,8 +267,8 @@ typedef struct rdrt_pt_stats {
   uint64_t visits[3];           /* node visits, triangle tests, instance tests (RDRT_PT_DIAG_COUNT_VISITS) */
 } rdrt_pt_stats;
 
-/* Renders one frame: radiance_dev receives width*height RGB f32 (the mean
- * of spp paths per pixel).  Scene committed and materials set first. */
+/* Renders one frame: radiance_dev receives views*width*height RGB f32 (the
+ * mean of spp paths per pixel).  Scene committed and materials set first. */
 int32_t rdrt_pt_render(rdrt_ctx *ctx, const rdrt_pt_params *params, float *radiance_dev);
 /* Ray counts of the last frame (synchronizes). */
 int32_t rdrt_pt_get_stats(rdrt_ctx *ctx, rdrt_pt_stats *out);

@@ -1153,6 +1153,31 @@ struct PtCounts {
   std::vector<uint64_t> ext, shadow;
 };
 
+// The visibility side of a hit, in FP32 with the device's operation order
+// (the reference computes it in f32 too): the geometric normal facing the
+// ray and the continuation origin offset off the surface (ray_util.rs:6-40).
+// Shading quantities stay in FP64.
+struct VisFrame {
+  V3 ng, origin;
+  bool flipped;
+};
+VisFrame vis_frame(const orc_scene *s, uint32_t tri, uint32_t ti, V3 o, V3 d, float t) {
+  const V3 p0 = s->vertices[s->indices[3 * (size_t)tri]], p1 = s->vertices[s->indices[3 * (size_t)tri + 1]],
+           p2 = s->vertices[s->indices[3 * (size_t)tri + 2]];
+  const V3 c = cross(p1 - p0, p2 - p0);
+  const M4 &m = s->inst_transform_inv[ti];
+  VisFrame f;
+  f.ng = normalize(v3(m.a1 * c.x + m.a2 * c.y + m.a3 * c.z, m.b1 * c.x + m.b2 * c.y + m.b3 * c.z,
+                      m.c1 * c.x + m.c2 * c.y + m.c3 * c.z));
+  f.flipped = dot(f.ng, d) > 0.0f;
+  if (f.flipped) f.ng = f.ng * -1.0f;
+  const V3 pos = o + d * t;
+  f.origin = v3(offset_axis(pos.x, f.ng.x), offset_axis(pos.y, f.ng.y), offset_axis(pos.z, f.ng.z));
+  return f;
+}
+
+uint32_t orc_views(const orc_pt_params *P) { return P->views > 1 ? P->views : 1u; }
+
 // One path vertex as the adjoint needs it: the material hit (or the sky the
 // ray escaped to) and the unoccluded light weight cos_l / pi (0 if none).
 constexpr uint32_t kVertSkyUp = 0xFFFFFFF0u, kVertSkyDown = 0xFFFFFFF1u;
@@ -1164,22 +1189,27 @@ struct PtVert {
 // One path; returns its radiance.  `rec`, if given, receives its vertices.
 D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *ext_cnt, uint64_t *sh_cnt,
            std::vector<PtVert> *rec = nullptr) {
-  const uint32_t pixel = path / P.spp;
+  const uint32_t gpix = path / P.spp;
+  const uint32_t npix = P.width * P.height;
+  const uint32_t view = gpix / npix, pixel = gpix % npix;
   const uint32_t n = P.sample_index_base + path % P.spp;
   const uint32_t px = pixel % P.width, py = pixel / P.width;
-  double jx = orc_van_der_corput(n, scramble_of(px, py, P.seed, 0));
-  double jy = orc_sobol(n, scramble_of(px, py, P.seed, 1));
+  const uint32_t seed = P.seed + view;
+  const float *vp = P.view_params ? P.view_params + 20 * (size_t)view : P.view_projection_inv;
+  const float *campos = P.view_params ? P.view_params + 20 * (size_t)view + 16 : P.camera_position;
+  double jx = orc_van_der_corput(n, scramble_of(px, py, seed, 0));
+  double jy = orc_sobol(n, scramble_of(px, py, seed, 1));
   double u = (double)px / P.width + jx / P.width;
   double v = (double)py / P.height + jy / P.height;
-  const float *m = P.view_projection_inv;
+  const float *m = vp;
   double nx = u * 2.0 - 1.0, ny = (v * 2.0 - 1.0) * -1.0;
   double rx = nx * m[0] + ny * m[4] + m[8] + m[12];
   double ry = nx * m[1] + ny * m[5] + m[9] + m[13];
   double rz = nx * m[2] + ny * m[6] + m[10] + m[14];
   double rw = nx * m[3] + ny * m[7] + m[11] + m[15];
-  D3 cam = d3(P.camera_position[0], P.camera_position[1], P.camera_position[2]);
+  D3 cam = d3(campos[0], campos[1], campos[2]);
   D3 dir = dnormalize(d3(rx / rw, ry / rw, rz / rw) - cam);
-  V3 o32 = v3(P.camera_position[0], P.camera_position[1], P.camera_position[2]);
+  V3 o32 = v3(campos[0], campos[1], campos[2]);
   D3 beta = d3(1, 1, 1), L = d3(0, 0, 0);
   const D3 sky_up = d3(P.sky_up[0], P.sky_up[1], P.sky_up[2]);
   const D3 sky_down = d3(P.sky_down[0], P.sky_down[1], P.sky_down[2]);
@@ -1217,10 +1247,9 @@ D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *
     const double bu = h.u, bv = h.v, bw = 1.0 - bu - bv;
     D3 nobj = d3f(s->normals[i0]) * bw + d3f(s->normals[i1]) * bu + d3f(s->normals[i2]) * bv;
     if (ddot(nobj, nobj) > 0.0) ns = dnormalize(normal_to_world(inv, nobj));
-    const D3 dd = d3f(ray.d);
-    if (ddot(ng, dd) > 0.0) ng = ng * -1.0;
+    const VisFrame vf = vis_frame(s, tri, ti, ray.o, ray.d, h.t);
+    if (vf.flipped) ng = ng * -1.0;
     if (ddot(ns, ng) < 0.0) ns = ns * -1.0;
-    const D3 pos = d3f(ray.o) + dd * (double)h.t;
     uint32_t mi = s->inst_custom[ti] + s->tri_material[tri];
     const size_t nm = s->materials.size() / 8;
     if (mi >= nm) mi = (uint32_t)nm - 1;
@@ -1229,12 +1258,10 @@ D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *
     L = L + beta * d3(mat[4], mat[5], mat[6]);
     if (rec) rec->push_back({mi, 0.0});
     if (b == P.max_bounces) break;
-    // offset origin along the geometric normal, on FP32 values
-    const V3 ngf = v3((float)ng.x, (float)ng.y, (float)ng.z);
-    const V3 posf = v3((float)pos.x, (float)pos.y, (float)pos.z);
-    const V3 origin = v3(offset_axis(posf.x, ngf.x), offset_axis(posf.y, ngf.y), offset_axis(posf.z, ngf.z));
+    const V3 origin = vf.origin;
     const double cos_l = ddot(ns, wi);
-    if (cos_l > 0.0 && ddot(ng, wi) > 0.0 && has_sun) {
+    const V3 wif = v3(-P.sun_direction[0], -P.sun_direction[1], -P.sun_direction[2]);
+    if (cos_l > 0.0 && dot(vf.ng, wif) > 0.0f && has_sun) {
       sh_cnt[b]++;
       RayIn sr;
       sr.o = origin;
@@ -1252,14 +1279,14 @@ D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *
         if (rec) rec->back().w = cos_l / 3.14159265358979323846;
       }
     }
-    const double s0 = orc_van_der_corput(n, scramble_of(px, py, P.seed, 2 + 2 * b));
-    const double s1 = orc_sobol(n, scramble_of(px, py, P.seed, 3 + 2 * b));
+    const double s0 = orc_van_der_corput(n, scramble_of(px, py, seed, 2 + 2 * b));
+    const double s1 = orc_sobol(n, scramble_of(px, py, seed, 3 + 2 * b));
     const D3 wo = sample_cosine(ns, s0, s1);
     if (!(ddot(wo, ng) > 0.0)) break;
     beta = beta * albedo;
     if (b >= P.rr_start) {
       const double q = std::min(std::max(std::max(beta.x, beta.y), beta.z), 0.95);
-      const double ur = orc_van_der_corput(n, scramble_of(px, py, P.seed, 64 + b));
+      const double ur = orc_van_der_corput(n, scramble_of(px, py, seed, 64 + b));
       if (ur >= q) break;
       beta = beta * (1.0 / q);
     }
@@ -1293,13 +1320,14 @@ int orc_set_triangle_materials(orc_scene *s, uint32_t blas, uint32_t geo, const 
 void orc_set_materials(orc_scene *s, const float *mats, uint32_t n) { s->materials.assign(mats, mats + 8 * (size_t)n); }
 
 int orc_pt_render(const orc_scene *s, const orc_pt_params *P, double *out, uint64_t *ray_counts, int nthreads) {
-  return orc_pt_render_rows(s, P, 0, P->height, out, ray_counts, nthreads);
+  return orc_pt_render_rows(s, P, 0, orc_views(P) * P->height, out, ray_counts, nthreads);
 }
 
 int orc_pt_render_rows(const orc_scene *s, const orc_pt_params *P, uint32_t row_begin, uint32_t row_end, double *out,
                        uint64_t *ray_counts, int nthreads) {
   if (!s->built || s->materials.empty() || P->spp == 0 || P->max_bounces > 16) return -1;
-  if (row_begin > row_end || row_end > P->height) return -1;
+  if (row_begin > row_end || row_end > orc_views(P) * P->height) return -1;
+  if (P->views > 1 && !P->view_params) return -1;
   const uint32_t pix0 = row_begin * P->width;
   const uint32_t npix = (row_end - row_begin) * P->width;
   const uint32_t nb = P->max_bounces + 1;
@@ -1349,8 +1377,9 @@ int orc_pt_grad(const orc_scene *s, const orc_pt_params *P, const double *adj, d
                 int nthreads) {
   if (!s->built || s->materials.empty() || P->spp == 0 || P->max_bounces > 16) return -1;
   if (P->rr_start < P->max_bounces) return -2;
+  if (P->views > 1 && !P->view_params) return -1;
   const size_t nm = s->materials.size() / 8;
-  const uint32_t npix = P->width * P->height;
+  const uint32_t npix = orc_views(P) * P->width * P->height;
   std::fill(g_albedo, g_albedo + 3 * nm, 0.0);
   std::fill(g_emission, g_emission + 3 * nm, 0.0);
   const D3 sky_up = d3(P->sky_up[0], P->sky_up[1], P->sky_up[2]);
@@ -1440,17 +1469,14 @@ int orc_batch_round(const orc_scene *s, const orc_ray *rays, const orc_ray_meta 
       const D3 p1 = d3f(s->vertices[s->indices[3 * (size_t)tri + 1]]);
       const D3 p2 = d3f(s->vertices[s->indices[3 * (size_t)tri + 2]]);
       D3 ng = dnormalize(normal_to_world(s->inst_transform_inv[ext[1]], dcross(p1 - p0, p2 - p0)));
-      const D3 dd = d3f(ri.d);
-      if (ddot(ng, dd) > 0.0) ng = ng * -1.0;
-      const D3 pos = d3f(ri.o) + dd * (double)h.t;
-      const V3 ngf = v3((float)ng.x, (float)ng.y, (float)ng.z);
-      const V3 posf = v3((float)pos.x, (float)pos.y, (float)pos.z);
+      const VisFrame vf = vis_frame(s, tri, ext[1], ri.o, ri.d, h.t);
+      if (vf.flipped) ng = ng * -1.0;
       const double s0 = orc_van_der_corput(round, xxhash32((uint32_t)i, seed, 2u));
       const double s1 = orc_sobol(round, xxhash32((uint32_t)i, seed, 3u));
       const D3 wo = sample_cosine(ng, s0, s1);
-      out.ox = offset_axis(posf.x, ngf.x);
-      out.oy = offset_axis(posf.y, ngf.y);
-      out.oz = offset_axis(posf.z, ngf.z);
+      out.ox = vf.origin.x;
+      out.oy = vf.origin.y;
+      out.oz = vf.origin.z;
       out.tmin = FLT_EPSILON;
       out.dx = (float)wo.x;
       out.dy = (float)wo.y;

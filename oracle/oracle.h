@@ -161,6 +161,9 @@ typedef struct orc_pt_params {
   uint32_t diagnostics; /* ignored by the oracle */
   uint32_t record;      /* ignored by the oracle */
   uint32_t pad;
+  uint32_t views;       /* as the product: views * height rows, 20 f32 per view */
+  uint32_t pad2;
+  const float *view_params;
 } orc_pt_params;
 
 int orc_set_vertex_normals(orc_scene *s, uint32_t blas, uint32_t geometry, const float *normals_xyz,

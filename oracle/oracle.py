@@ -40,7 +40,8 @@ class OrcPtParams(C.Structure):
     _fields_ = [("view_projection_inv", f32 * 16), ("camera_position", f32 * 3), ("width", u32), ("height", u32),
                 ("spp", u32), ("max_bounces", u32), ("rr_start", u32), ("seed", u32), ("sample_index_base", u32),
                 ("sky_up", f32 * 3), ("sky_down", f32 * 3), ("sun_direction", f32 * 3), ("sun_radiance", f32 * 3),
-                ("use_graph", u32), ("diagnostics", u32), ("record", u32), ("pad", u32)]
+                ("use_graph", u32), ("diagnostics", u32), ("record", u32), ("pad", u32),
+                ("views", u32), ("pad2", u32), ("view_params", C.c_void_p)]
 
 
 _SIG = {
@@ -252,7 +253,7 @@ class OracleScene:
     def pt_render(self, params: "OrcPtParams", threads: int = 0):
         """Returns (height, width, 3) float64 radiance and the per-bounce
         (extension, shadow) ray counts."""
-        out = np.zeros((params.height, params.width, 3), dtype=np.float64)
+        out = np.zeros((max(params.views, 1) * params.height, params.width, 3), dtype=np.float64)
         nb = params.max_bounces + 1
         counts = np.zeros(2 * nb, dtype=np.uint64)
         rc = self.L.orc_pt_render(self.h, C.byref(params), out.ctypes.data, counts.ctypes.data, threads)
@@ -274,7 +275,8 @@ class OracleScene:
     def pt_grad(self, params: "OrcPtParams", adj: np.ndarray, nmaterials: int, threads: int = 0):
         """Gradient of sum(adj * frame) w.r.t. (albedo, emission), each
         (nmaterials, 3) float64, in FP64 with visibility fixed."""
-        adj = np.ascontiguousarray(adj, dtype=np.float64).reshape(params.height, params.width, 3)
+        adj = np.ascontiguousarray(adj, dtype=np.float64).reshape(max(params.views, 1) * params.height,
+                                                                   params.width, 3)
         ga = np.zeros((nmaterials, 3), dtype=np.float64)
         ge = np.zeros((nmaterials, 3), dtype=np.float64)
         rc = self.L.orc_pt_grad(self.h, C.byref(params), adj.ctypes.data, ga.ctypes.data, ge.ctypes.data, threads)

@@ -70,7 +70,8 @@ class RdrtPtParams(C.Structure):
     _fields_ = [("view_projection_inv", f32 * 16), ("camera_position", f32 * 3), ("width", u32), ("height", u32),
                 ("spp", u32), ("max_bounces", u32), ("rr_start", u32), ("seed", u32), ("sample_index_base", u32),
                 ("sky_up", f32 * 3), ("sky_down", f32 * 3), ("sun_direction", f32 * 3), ("sun_radiance", f32 * 3),
-                ("use_graph", u32), ("diagnostics", u32), ("record", u32), ("pad", u32)]
+                ("use_graph", u32), ("diagnostics", u32), ("record", u32), ("pad", u32),
+                ("views", u32), ("pad2", u32), ("view_params", C.c_void_p)]
 
 
 class RdrtPtStats(C.Structure):

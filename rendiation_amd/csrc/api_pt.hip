@@ -14,8 +14,11 @@ namespace rdrt {
 
 void PtState::free_all() {
   for (Scratch *s : {&rays, &shadow_rays, &hits, &shadow_hits, &hitx, &beta, &L, &nee, &qa, &qb, &qs, &counts,
-                     &lookback, &tickets, &materials, &diag_counters, &diag_totals, &record})
+                     &lookback, &tickets, &materials, &diag_counters, &diag_totals, &record, &views})
     s->free();
+  host_views.clear();
+  materials_dirty = true;
+  last_recorded = false;
   for (hipEvent_t e : ev_begin) (void)hipEventDestroy(e);
   for (hipEvent_t e : ev_end) (void)hipEventDestroy(e);
   ev_begin.clear();
@@ -111,7 +114,9 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
   if (!c->uploaded) return fail(RDRT_ERR_NOT_BUILT, "scene is not committed (call rdrt_commit)");
   if (prm->width == 0 || prm->height == 0 || prm->spp == 0) return fail(RDRT_ERR_INVALID_ARG, "empty frame");
   if (prm->max_bounces > kPtMaxBounces) return fail(RDRT_ERR_INVALID_ARG, "max_bounces above 16");
-  uint64_t npaths64 = (uint64_t)prm->width * prm->height * prm->spp;
+  const uint32_t nviews = prm->views > 1 ? prm->views : 1u;
+  if (prm->views > 1 && !prm->view_params) return fail(RDRT_ERR_INVALID_ARG, "views > 1 needs view_params");
+  uint64_t npaths64 = (uint64_t)nviews * prm->width * prm->height * prm->spp;
   if (npaths64 > (1ull << 30)) return fail(RDRT_ERR_CAPACITY, "more than 2^30 paths per frame");
   if (c->materials.empty()) return fail(RDRT_ERR_STATE, "no materials set (rdrt_set_materials)");
   // every instance must address a material: custom index + largest triangle offset
@@ -127,7 +132,25 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
   rc = reserve_pt(c, npaths);
   if (rc) return rc;
   PtState &s = c->pt;
+  // camera list: view_params, or the single camera of the parameters
+  std::vector<float> views(20 * (size_t)nviews, 0.0f);
+  for (uint32_t v = 0; v < nviews; v++) {
+    if (prm->view_params) {
+      std::memcpy(&views[20 * (size_t)v], prm->view_params + 20 * (size_t)v, 19 * sizeof(float));
+    } else {
+      std::memcpy(&views[0], prm->view_projection_inv, 16 * sizeof(float));
+      std::memcpy(&views[16], prm->camera_position, 3 * sizeof(float));
+    }
+  }
+  // synchronous uploads wait for the frames still reading the old contents
+  if (views != s.host_views) {
+    RDRT_HIP(hipStreamSynchronize(c->work_stream));
+    RDRT_HIP(s.views.reserve(views.size() * sizeof(float)));
+    RDRT_HIP(hipMemcpy(s.views.ptr, views.data(), views.size() * sizeof(float), hipMemcpyHostToDevice));
+    s.host_views = views;
+  }
   if (s.materials_dirty) {
+    RDRT_HIP(hipStreamSynchronize(c->work_stream));
     RDRT_HIP(s.materials.reserve(c->materials.size() * sizeof(Material)));
     RDRT_HIP(hipMemcpy(s.materials.ptr, c->materials.data(), c->materials.size() * sizeof(Material),
                        hipMemcpyHostToDevice));
@@ -137,8 +160,8 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
 
   PtLaunch L;
   std::memset(&L, 0, sizeof(L));
-  std::memcpy(L.vp_inv, prm->view_projection_inv, sizeof(L.vp_inv));
-  std::memcpy(L.cam, prm->camera_position, sizeof(L.cam));
+  L.views = (const float *)s.views.ptr;
+  L.nviews = nviews;
   L.width = prm->width;
   L.height = prm->height;
   L.spp = prm->spp;

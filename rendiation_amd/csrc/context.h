@@ -83,7 +83,8 @@ struct PtState {
   uint32_t capacity = 0;  // paths
   uint32_t pixels = 0;
   Scratch rays, shadow_rays, hits, shadow_hits, hitx, beta, L, nee, qa, qb, qs, counts, lookback, tickets;
-  Scratch materials, record;
+  Scratch materials, record, views;
+  std::vector<float> host_views;  // contents of `views`
   uint32_t nmaterials_uploaded = 0;
   bool materials_dirty = true;
   hipGraph_t graph = nullptr;

@@ -16,8 +16,8 @@ constexpr uint32_t kPtMaxBounces = 16;   // upper bound on max_bounces
 // Everything one frame's kernels read, passed by value (so a captured graph
 // holds it).  Device arrays are indexed by path id unless noted.
 struct PtLaunch {
-  float vp_inv[16];
-  float cam[3];
+  const float *views;  // nviews * 20: view_projection_inv[16], camera_position[3], pad (device)
+  uint32_t nviews;
   uint32_t width, height, spp, npaths;
   uint32_t max_bounces, rr_start, seed, sample_base;
   float sky_up[3], sky_down[3];

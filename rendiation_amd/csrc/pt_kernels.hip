@@ -52,12 +52,17 @@ __device__ __forceinline__ V3 normal_to_world(const float *inv, V3 n) {
             inv[8] * n.x + inv[9] * n.y + inv[10] * n.z);
 }
 
-__device__ __forceinline__ void pixel_of(const PtLaunch &P, uint32_t path, uint32_t &px, uint32_t &py,
-                                         uint32_t &n) {
-  uint32_t pixel = path / P.spp;
+// path -> view, pixel (px, py) within the view, sample index n, and the
+// view's sample seed (seed + view)
+__device__ __forceinline__ void pixel_of(const PtLaunch &P, uint32_t path, uint32_t &px, uint32_t &py, uint32_t &n,
+                                         uint32_t &view, uint32_t &seed) {
+  const uint32_t gpix = path / P.spp, npix = P.width * P.height;
+  view = gpix / npix;
+  const uint32_t pixel = gpix - view * npix;
   n = P.sample_base + path % P.spp;
   px = pixel % P.width;
   py = pixel / P.width;
+  seed = P.seed + view;
 }
 
 __device__ __forceinline__ void write_ray(rdrt_ray *r, V3 o, float tmin, V3 d, float tmax) {
@@ -81,14 +86,15 @@ __global__ void k_pt_begin(PtLaunch P) {
 __global__ __launch_bounds__(kPtBlock) void k_pt_camera(PtLaunch P) {
   const uint32_t path = blockIdx.x * kPtBlock + threadIdx.x;
   if (path >= P.npaths) return;
-  uint32_t px, py, n;
-  pixel_of(P, path, px, py, n);
-  float jx = pt::van_der_corput(n, pt::scramble(px, py, P.seed, 0));
-  float jy = pt::sobol1(n, pt::scramble(px, py, P.seed, 1));
+  uint32_t px, py, n, view, seed;
+  pixel_of(P, path, px, py, n, view, seed);
+  const float *vw = P.views + 20 * (size_t)view;
+  float jx = pt::van_der_corput(n, pt::scramble(px, py, seed, 0));
+  float jy = pt::sobol1(n, pt::scramble(px, py, seed, 1));
   float u = (float)px / (float)P.width + jx / (float)P.width;
   float v = (float)py / (float)P.height + jy / (float)P.height;
-  V3 target = pt::unproject(P.vp_inv, u, v);
-  V3 cam = v3(P.cam[0], P.cam[1], P.cam[2]);
+  V3 target = pt::unproject(vw, u, v);
+  V3 cam = v3(vw[16], vw[17], vw[18]);
   V3 d = pt::normalize3(target - cam);
   write_ray(P.rays + path, cam, kFltEps, d, kFltMax);
   store3(P.beta + 3 * (size_t)path, v3(1.0f, 1.0f, 1.0f));
@@ -140,8 +146,8 @@ __device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &ke
   L = L + beta * v3(m.emission[0], m.emission[1], m.emission[2]);
   float w_light = 0.0f;
   if (b < P.max_bounces) {
-    uint32_t px, py, n;
-    pixel_of(P, path, px, py, n);
+    uint32_t px, py, n, view, seed;
+    pixel_of(P, path, px, py, n, view, seed);
     const V3 origin = pt::offset_ray_hit(pos, ng);
     // next-event estimation toward the directional light
     const V3 wi = v3(-P.sun_dir[0], -P.sun_dir[1], -P.sun_dir[2]);
@@ -155,14 +161,14 @@ __device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &ke
       w_light = cos_l * kInvPi;
     }
     // Lambertian importance sample: f cos / pdf = albedo
-    const float s0 = pt::van_der_corput(n, pt::scramble(px, py, P.seed, 2 + 2 * b));
-    const float s1 = pt::sobol1(n, pt::scramble(px, py, P.seed, 3 + 2 * b));
+    const float s0 = pt::van_der_corput(n, pt::scramble(px, py, seed, 2 + 2 * b));
+    const float s1 = pt::sobol1(n, pt::scramble(px, py, seed, 3 + 2 * b));
     const V3 wo = pt::sample_cosine(ns, s0, s1);
     bool alive = pt::dot3(wo, ng) > 0.0f;
     V3 nb = beta * albedo;
     if (alive && b >= P.rr_start) {
       const float q = fminf(fmaxf(fmaxf(nb.x, nb.y), nb.z), 0.95f);
-      const float u = pt::van_der_corput(n, pt::scramble(px, py, P.seed, 64 + b));
+      const float u = pt::van_der_corput(n, pt::scramble(px, py, seed, 64 + b));
       if (u >= q) alive = false;
       else nb = v3(nb.x / q, nb.y / q, nb.z / q);
     }
@@ -217,7 +223,7 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_nee(PtLaunch P, uint32_t b) {
 
 __global__ __launch_bounds__(kPtBlock) void k_pt_finalize(PtLaunch P) {
   const uint32_t pixel = blockIdx.x * kPtBlock + threadIdx.x;
-  if (pixel >= P.width * P.height) return;
+  if (pixel >= P.nviews * P.width * P.height) return;
   V3 s = v3(0.0f, 0.0f, 0.0f);
   for (uint32_t k = 0; k < P.spp; k++) s = s + load3(P.L + 3 * ((size_t)pixel * P.spp + k));
   const float inv = 1.0f / (float)P.spp;
@@ -448,7 +454,8 @@ hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_pt_nee, dim3(grid), dim3(kPtBlock), 0, st, P, b);
   }
-  hipLaunchKernelGGL(k_pt_finalize, dim3((P.width * P.height + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, st, P);
+  hipLaunchKernelGGL(k_pt_finalize, dim3((P.nviews * P.width * P.height + kPtBlock - 1) / kPtBlock), dim3(kPtBlock),
+                     0, st, P);
   return hipGetLastError();
 }
 

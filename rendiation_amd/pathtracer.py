@@ -117,16 +117,29 @@ class PathTracer:
         self.system = system
         self.recorded_frames = 0  # frames rendered with record = True
 
-    def render(self, camera: Camera, cfg: PathTracingConfig, out=None):
+    def render(self, camera, cfg: PathTracingConfig, out=None):
         """Render one frame; returns the (height, width, 3) float32 device
-        radiance tensor (``out`` if given)."""
+        radiance tensor (``out`` if given).  ``camera`` may be a list of
+        cameras: all views in one frame, (views, height, width, 3)."""
         import torch
 
+        cams = list(camera) if isinstance(camera, (list, tuple)) else [camera]
+        nv = len(cams)
+        shape = (cfg.height, cfg.width, 3) if nv == 1 and not isinstance(camera, (list, tuple)) else (
+            nv, cfg.height, cfg.width, 3)
         if out is None:
-            out = torch.empty((cfg.height, cfg.width, 3), dtype=torch.float32, device=f"cuda:{self.system.device}")
-        if not out.is_contiguous() or out.numel() != cfg.width * cfg.height * 3:
-            raise ValueError("out must be a contiguous height*width*3 float32 tensor")
-        prm = cfg.to_c(camera)
+            out = torch.empty(shape, dtype=torch.float32, device=f"cuda:{self.system.device}")
+        if not out.is_contiguous() or out.numel() != nv * cfg.width * cfg.height * 3:
+            raise ValueError("out must be a contiguous views*height*width*3 float32 tensor")
+        prm = cfg.to_c(cams[0])
+        views = None
+        if nv > 1:
+            views = (L.f32 * (20 * nv))()
+            for v, cam in enumerate(cams):
+                views[20 * v:20 * v + 16] = [float(x) for x in cam.view_projection_inv().T.reshape(-1)]
+                views[20 * v + 16:20 * v + 19] = [float(np.float32(x)) for x in cam.position]
+            prm.views = nv
+            prm.view_params = C.cast(views, C.c_void_p)
         L.check(self.system._L.rdrt_pt_render(self.system._h, C.byref(prm), int(out.data_ptr())))
         if cfg.record:
             self.recorded_frames += 1

@@ -24,8 +24,12 @@ def oracle_scene_from(scene: dict, blas_bin_size: int = 2) -> OracleScene:
     return o
 
 
-def oracle_params(camera: Camera, cfg: PathTracingConfig) -> OrcPtParams:
-    c = cfg.to_c(camera)
+def oracle_params(camera, cfg: PathTracingConfig) -> OrcPtParams:
+    """Oracle parameters of a frame; ``camera`` may be a list of views."""
+    import ctypes as C
+
+    cams = list(camera) if isinstance(camera, (list, tuple)) else [camera]
+    c = cfg.to_c(cams[0])
     p = OrcPtParams()
     for name, _ in OrcPtParams._fields_:
         v = getattr(c, name)
@@ -33,6 +37,14 @@ def oracle_params(camera: Camera, cfg: PathTracingConfig) -> OrcPtParams:
             getattr(p, name)[:] = list(v)
         else:
             setattr(p, name, v)
+    if len(cams) > 1:
+        views = (C.c_float * (20 * len(cams)))()
+        for k, cam in enumerate(cams):
+            views[20 * k:20 * k + 16] = [float(x) for x in cam.view_projection_inv().T.reshape(-1)]
+            views[20 * k + 16:20 * k + 19] = [float(np.float32(x)) for x in cam.position]
+        p.views = len(cams)
+        p.view_params = C.cast(views, C.c_void_p)
+        p._views_keepalive = views
     return p
 
 

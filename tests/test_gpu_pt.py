@@ -166,3 +166,27 @@ def test_graph_replay_equals_eager(device):
         out = pt.render(cam, cfg, out=out)
         s.sync()
         assert np.array_equal(out.cpu().numpy(), eager.cpu().numpy())
+
+
+def test_views_in_one_frame(device):
+    """Three cameras in one frame equal three single-camera frames with seeds
+    seed + v (bit for bit), and the FP64 oracle's multi-view frame."""
+    import dataclasses
+
+    scene = synth.pt_scene(seed=3, n_meshes=16, tess=24, extent=5.0)
+    cams = [Camera(position=(0.0, 0.0, 18.0)), Camera(position=(14.0, 6.0, 9.0)),
+            Camera(position=(-10.0, -12.0, 8.0), up=(0.0, 0.0, 1.0))]
+    _say("three views in one frame")
+    s = _system(scene)
+    pt = PathTracer(s)
+    cfg = PathTracingConfig(width=64, height=40, spp=2, max_bounces=3, seed=9)
+    multi = pt.render(cams, cfg).cpu().numpy()
+    assert multi.shape == (3, 40, 64, 3)
+    for v, cam in enumerate(cams):
+        single = pt.render(cam, dataclasses.replace(cfg, seed=cfg.seed + v)).cpu().numpy()
+        assert np.array_equal(multi[v], single), v
+    o = oracle_scene_from(scene)
+    ref, _, _ = o.pt_render(oracle_params(cams, cfg))
+    ref = ref.reshape(3, 40, 64, 3)
+    rel = np.sqrt(np.mean((multi - ref) ** 2)) / ref.mean()
+    assert rel <= IMAGE_RMSE_REL, rel
