@@ -1184,6 +1184,16 @@ constexpr uint32_t kVertSkyUp = 0xFFFFFFF0u, kVertSkyDown = 0xFFFFFFF1u;
 struct PtVert {
   uint32_t code;
   double w;
+  // geometry adjoint: the hit, the traced world ray, the sign in
+  // ns = sigma * normalize(A n) (A = normal_to_world, n = the interpolated
+  // object-space normal, or e1 x e2 when `interp` is false), and the
+  // continuation direction with its cosine to ns
+  uint32_t tri = 0, ti = 0;
+  double sigma = 1.0;
+  bool interp = false, cont = false;
+  V3 o{}, d{};
+  D3 wo{};
+  double cos_wo = 0.0;
 };
 
 // One path; returns its radiance.  `rec`, if given, receives its vertices.
@@ -1232,7 +1242,7 @@ D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *
     uint32_t ext[2] = {0, 0};
     if (!traverse_one(s, ray, h, hm, cnt, ext)) {
       L = L + beta * (ray.d.y > 0.0f ? sky_up : sky_down);
-      if (rec) rec->push_back({ray.d.y > 0.0f ? kVertSkyUp : kVertSkyDown, 0.0});
+      if (rec) rec->push_back(PtVert{ray.d.y > 0.0f ? kVertSkyUp : kVertSkyDown, 0.0});
       break;
     }
     const auto &geo = s->geo_meta[ext[0]];
@@ -1246,17 +1256,32 @@ D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *
     D3 ns = ng;
     const double bu = h.u, bv = h.v, bw = 1.0 - bu - bv;
     D3 nobj = d3f(s->normals[i0]) * bw + d3f(s->normals[i1]) * bu + d3f(s->normals[i2]) * bv;
-    if (ddot(nobj, nobj) > 0.0) ns = dnormalize(normal_to_world(inv, nobj));
+    const bool interp = ddot(nobj, nobj) > 0.0;
+    if (interp) ns = dnormalize(normal_to_world(inv, nobj));
     const VisFrame vf = vis_frame(s, tri, ti, ray.o, ray.d, h.t);
+    double sigma = 1.0;
     if (vf.flipped) ng = ng * -1.0;
-    if (ddot(ns, ng) < 0.0) ns = ns * -1.0;
+    if (!interp && vf.flipped) sigma = -1.0;  // ns is ng before the flip
+    if (ddot(ns, ng) < 0.0) {
+      ns = ns * -1.0;
+      if (interp) sigma = -1.0;
+    }
     uint32_t mi = s->inst_custom[ti] + s->tri_material[tri];
     const size_t nm = s->materials.size() / 8;
     if (mi >= nm) mi = (uint32_t)nm - 1;
     const float *mat = &s->materials[8 * (size_t)mi];
     const D3 albedo = d3(mat[0], mat[1], mat[2]);
     L = L + beta * d3(mat[4], mat[5], mat[6]);
-    if (rec) rec->push_back({mi, 0.0});
+    if (rec) {
+      PtVert pv{mi, 0.0};
+      pv.tri = tri;
+      pv.ti = ti;
+      pv.sigma = sigma;
+      pv.interp = interp;
+      pv.o = ray.o;
+      pv.d = ray.d;
+      rec->push_back(pv);
+    }
     if (b == P.max_bounces) break;
     const V3 origin = vf.origin;
     const double cos_l = ddot(ns, wi);
@@ -1283,6 +1308,11 @@ D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *
     const double s1 = orc_sobol(n, scramble_of(px, py, seed, 3 + 2 * b));
     const D3 wo = sample_cosine(ns, s0, s1);
     if (!(ddot(wo, ng) > 0.0)) break;
+    if (rec) {
+      rec->back().cont = true;
+      rec->back().wo = wo;
+      rec->back().cos_wo = ddot(ns, wo);
+    }
     beta = beta * albedo;
     if (b >= P.rr_start) {
       const double q = std::min(std::max(std::max(beta.x, beta.y), beta.z), 0.95);
@@ -1438,6 +1468,184 @@ int orc_pt_grad(const orc_scene *s, const orc_pt_params *P, const double *adj, d
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// ---- geometry adjoint (FP64) ----
+// The function differentiated: the frame radiance with every path's traced
+// rays, hit triangles, light visibility, facing decisions and sampled
+// directions held fixed, as a function of the vertex positions P and
+// normals N.  At each hit the shading normal is
+//   ns = sigma * normalize(A n),  n = w0 N0 + u N1 + v N2
+// (or n = e1 x e2 without vertex normals), with (u, v) where the fixed
+// object-space ray meets the triangle of P; the light term is
+// beta a sun max(ns . wi, 0) / pi and the throughput factor of the
+// continuation is a (ns . wo) / cos_wo (cos_wo: its value in the frame).
+struct GeoEval {
+  D3 ns, m;  // shading normal and the unnormalized A n
+  double mlen, u, v;
+  D3 e1, e2, c0;  // edges and -bd (object space)
+  uint32_t i0, i1, i2;
+};
+
+GeoEval geo_eval(const orc_scene *s, const PtVert &pv, const double *pos, const double *nrm) {
+  GeoEval g;
+  g.i0 = s->indices[3 * (size_t)pv.tri];
+  g.i1 = s->indices[3 * (size_t)pv.tri + 1];
+  g.i2 = s->indices[3 * (size_t)pv.tri + 2];
+  auto P3 = [&](uint32_t i) {
+    return pos ? d3(pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2]) : d3f(s->vertices[i]);
+  };
+  auto N3 = [&](uint32_t i) {
+    return nrm ? d3(nrm[3 * (size_t)i], nrm[3 * (size_t)i + 1], nrm[3 * (size_t)i + 2]) : d3f(s->normals[i]);
+  };
+  const M4 &m = s->inst_transform_inv[pv.ti];
+  const D3 o = d3f(pv.o), d = d3f(pv.d);
+  const D3 bo = d3(m.a1 * o.x + m.b1 * o.y + m.c1 * o.z + m.d1, m.a2 * o.x + m.b2 * o.y + m.c2 * o.z + m.d2,
+                   m.a3 * o.x + m.b3 * o.y + m.c3 * o.z + m.d3);
+  const D3 bd = d3(m.a1 * d.x + m.b1 * d.y + m.c1 * d.z, m.a2 * d.x + m.b2 * d.y + m.c2 * d.z,
+                   m.a3 * d.x + m.b3 * d.y + m.c3 * d.z);
+  const D3 p0 = P3(g.i0), p1 = P3(g.i1), p2 = P3(g.i2);
+  g.e1 = p1 - p0;
+  g.e2 = p2 - p0;
+  g.c0 = bd * -1.0;
+  const D3 sv = bo - p0;
+  const double det = ddot(g.c0, dcross(g.e1, g.e2));
+  g.u = ddot(g.c0, dcross(sv, g.e2)) / det;
+  g.v = ddot(g.c0, dcross(g.e1, sv)) / det;
+  const D3 n = pv.interp ? N3(g.i0) * (1.0 - g.u - g.v) + N3(g.i1) * g.u + N3(g.i2) * g.v : dcross(g.e1, g.e2);
+  g.m = normal_to_world(m, n);
+  g.mlen = std::sqrt(ddot(g.m, g.m));
+  g.ns = g.m * (pv.sigma / g.mlen);
+  return g;
+}
+
+// Replays one path; returns sum_c adj_c C_c and, if g_pos / g_nrm are given,
+// adds the gradient of that value.
+double replay_path(const orc_scene *s, const orc_pt_params &P, const std::vector<PtVert> &rec, D3 adj,
+                   const double *pos, const double *nrm, double *g_pos, double *g_nrm) {
+  const double inv_pi = 1.0 / 3.14159265358979323846;
+  const D3 sun = d3(P.sun_radiance[0], P.sun_radiance[1], P.sun_radiance[2]);
+  const D3 wi = d3(-P.sun_direction[0], -P.sun_direction[1], -P.sun_direction[2]);
+  const size_t nv = rec.size();
+  std::vector<GeoEval> ge(nv);
+  std::vector<D3> beta(nv + 1), direct(nv);
+  std::vector<double> ratio(nv, 1.0), cosl(nv, 0.0);
+  beta[0] = d3(1, 1, 1);
+  D3 tail = d3(0, 0, 0);
+  size_t m = 0;
+  for (; m < nv; m++) {
+    const PtVert &pv = rec[m];
+    if (pv.code >= kVertSkyUp) {
+      tail = beta[m] * (pv.code == kVertSkyUp ? d3(P.sky_up[0], P.sky_up[1], P.sky_up[2])
+                                               : d3(P.sky_down[0], P.sky_down[1], P.sky_down[2]));
+      break;
+    }
+    const float *mat = &s->materials[8 * (size_t)pv.code];
+    const D3 a = d3(mat[0], mat[1], mat[2]), e = d3(mat[4], mat[5], mat[6]);
+    ge[m] = geo_eval(s, pv, pos, nrm);
+    direct[m] = beta[m] * e;
+    if (pv.w > 0.0) {
+      cosl[m] = ddot(ge[m].ns, wi);
+      if (cosl[m] > 0.0) direct[m] = direct[m] + beta[m] * a * sun * (cosl[m] * inv_pi);
+    }
+    if (!pv.cont) {
+      m++;
+      break;
+    }
+    ratio[m] = ddot(ge[m].ns, pv.wo) / pv.cos_wo;
+    beta[m + 1] = beta[m] * a * ratio[m];
+  }
+  double total = ddot(adj, tail);
+  for (size_t k = 0; k < m; k++) total += ddot(adj, direct[k]);
+  if (!g_pos) return total;
+  D3 rem = tail;  // radiance after vertex k
+  for (size_t k = m; k-- > 0;) {
+    const PtVert &pv = rec[k];
+    const float *mat = &s->materials[8 * (size_t)pv.code];
+    const D3 a = d3(mat[0], mat[1], mat[2]);
+    const GeoEval &g = ge[k];
+    D3 g_ns = d3(0, 0, 0);
+    if (pv.w > 0.0 && cosl[k] > 0.0) g_ns = g_ns + wi * (ddot(adj, beta[k] * a * sun) * inv_pi);
+    if (pv.cont && ratio[k] != 0.0) g_ns = g_ns + pv.wo * (ddot(adj, rem) / ratio[k] / pv.cos_wo);
+    rem = rem + direct[k];
+    // ns = sigma m / |m|
+    const D3 g_m = (g_ns - g.ns * ddot(g.ns, g_ns)) * (pv.sigma / g.mlen);
+    const M4 &mm = s->inst_transform_inv[pv.ti];
+    const D3 g_n = d3(mm.a1 * g_m.x + mm.b1 * g_m.y + mm.c1 * g_m.z, mm.a2 * g_m.x + mm.b2 * g_m.y + mm.c2 * g_m.z,
+                      mm.a3 * g_m.x + mm.b3 * g_m.y + mm.c3 * g_m.z);
+    auto add = [](double *arr, uint32_t i, D3 v) {
+      arr[3 * (size_t)i] += v.x;
+      arr[3 * (size_t)i + 1] += v.y;
+      arr[3 * (size_t)i + 2] += v.z;
+    };
+    if (pv.interp) {
+      const double w0 = 1.0 - g.u - g.v;
+      add(g_nrm, g.i0, g_n * w0);
+      add(g_nrm, g.i1, g_n * g.u);
+      add(g_nrm, g.i2, g_n * g.v);
+      auto N3 = [&](uint32_t i) {
+        return nrm ? d3(nrm[3 * (size_t)i], nrm[3 * (size_t)i + 1], nrm[3 * (size_t)i + 2]) : d3f(s->normals[i]);
+      };
+      const double g_u = ddot(g_n, N3(g.i1) - N3(g.i0)), g_v = ddot(g_n, N3(g.i2) - N3(g.i0));
+      // (t, u, v) solves [c0 e1 e2] x = bo - p0
+      const double det = ddot(g.c0, dcross(g.e1, g.e2));
+      const D3 G = (dcross(g.e2, g.c0) * g_u + dcross(g.c0, g.e1) * g_v) * (1.0 / det);
+      add(g_pos, g.i0, G * -w0);
+      add(g_pos, g.i1, G * -g.u);
+      add(g_pos, g.i2, G * -g.v);
+    } else {
+      // n = e1 x e2
+      const D3 g_e1 = dcross(g.e2, g_n), g_e2 = dcross(g_n, g.e1);
+      add(g_pos, g.i1, g_e1);
+      add(g_pos, g.i2, g_e2);
+      add(g_pos, g.i0, (g_e1 + g_e2) * -1.0);
+    }
+  }
+  return total;
+}
+
+template <class F>
+int for_each_path(const orc_scene *s, const orc_pt_params *P, int nthreads, F &&fn) {
+  const uint32_t npix = orc_views(P) * P->width * P->height;
+#ifdef _OPENMP
+  int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+#endif
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    std::vector<uint64_t> ext(P->max_bounces + 1, 0), sh(P->max_bounces + 1, 0);
+    std::vector<PtVert> rec;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+    for (int64_t pix = 0; pix < (int64_t)npix; pix++)
+      for (uint32_t k = 0; k < P->spp; k++) {
+        rec.clear();
+        pt_path(s, *P, (uint32_t)pix * P->spp + k, ext.data(), sh.data(), &rec);
+        fn(tid, (uint32_t)pix, rec);
+      }
+  }
+  return 0;
+}
+
+int nthreads_of(int nthreads) {
+#ifdef _OPENMP
+  return nthreads > 0 ? nthreads : omp_get_max_threads();
+#else
+  (void)nthreads;
+  return 1;
+#endif
+}
+
+}  // namespace
+
+extern "C" {
+
 // One round of batched tracing: closest hit of every ray (traverse_cpu.rs:
 // 52-245 with per-ray TLAS selection, traverse_cpu.rs:63), then for each hit
 // a continuation ray from the offset hit point (ray_util.rs:6-40) in a
@@ -1485,6 +1693,55 @@ int orc_batch_round(const orc_scene *s, const orc_ray *rays, const orc_ray_meta 
     }
     spawned[i] = out;
   }
+  return 0;
+}
+
+uint32_t orc_vertex_count(const orc_scene *s) { return (uint32_t)s->vertices.size(); }
+
+// Gradient of sum(adj * frame) w.r.t. vertex positions and normals (global
+// vertex order, 3 doubles each), FP64, paths held fixed as above.
+int orc_pt_grad_geometry(const orc_scene *s, const orc_pt_params *P, const double *adj, double *g_pos,
+                         double *g_nrm, int nthreads) {
+  if (!s->built || s->materials.empty() || P->spp == 0 || P->max_bounces > 16) return -1;
+  if (P->rr_start < P->max_bounces) return -2;
+  if (P->views > 1 && !P->view_params) return -1;
+  const size_t n3 = 3 * s->vertices.size();
+  const int nt = nthreads_of(nthreads);
+  std::vector<std::vector<double>> gp(nt, std::vector<double>(n3, 0.0)), gn(nt, std::vector<double>(n3, 0.0));
+  const double inv_spp = 1.0 / P->spp;
+  for_each_path(s, P, nt, [&](int tid, uint32_t pix, const std::vector<PtVert> &rec) {
+    const D3 a = d3(adj[3 * (size_t)pix], adj[3 * (size_t)pix + 1], adj[3 * (size_t)pix + 2]) * inv_spp;
+    replay_path(s, *P, rec, a, nullptr, nullptr, gp[tid].data(), gn[tid].data());
+  });
+  for (size_t i = 0; i < n3; i++) {
+    double a = 0.0, b = 0.0;
+    for (int t = 0; t < nt; t++) {
+      a += gp[t][i];
+      b += gn[t][i];
+    }
+    g_pos[i] = a;
+    g_nrm[i] = b;
+  }
+  return 0;
+}
+
+// sum(adj * frame) of the fixed paths re-shaded with positions `pos` and
+// normals `nrm` (3 doubles per vertex each, NULL = the scene's own).
+int orc_pt_replay_geometry(const orc_scene *s, const orc_pt_params *P, const double *adj, const double *pos,
+                           const double *nrm, double *out, int nthreads) {
+  if (!s->built || s->materials.empty() || P->spp == 0 || P->max_bounces > 16) return -1;
+  if (P->rr_start < P->max_bounces) return -2;
+  if (P->views > 1 && !P->view_params) return -1;
+  const int nt = nthreads_of(nthreads);
+  std::vector<double> part(nt, 0.0);
+  const double inv_spp = 1.0 / P->spp;
+  for_each_path(s, P, nt, [&](int tid, uint32_t pix, const std::vector<PtVert> &rec) {
+    const D3 a = d3(adj[3 * (size_t)pix], adj[3 * (size_t)pix + 1], adj[3 * (size_t)pix + 2]) * inv_spp;
+    part[tid] += replay_path(s, *P, rec, a, pos, nrm, nullptr, nullptr);
+  });
+  double t = 0.0;
+  for (double v : part) t += v;
+  *out = t;
   return 0;
 }
 

@@ -188,6 +188,18 @@ int orc_pt_render_rows(const orc_scene *s, const orc_pt_params *params, uint32_t
 int orc_pt_grad(const orc_scene *s, const orc_pt_params *params, const double *adj, double *grad_albedo,
                 double *grad_emission, int nthreads);
 
+/* ---- geometry adjoint (paths fixed: traced rays, hit triangles, light
+ * visibility, facing and sampled directions) ---- */
+uint32_t orc_vertex_count(const orc_scene *s);
+/* Gradient of sum(adj * frame) w.r.t. vertex positions and normals (global
+ * vertex order, 3 doubles each), FP64.  Returns -2 with Russian roulette. */
+int orc_pt_grad_geometry(const orc_scene *s, const orc_pt_params *params, const double *adj, double *grad_pos,
+                         double *grad_nrm, int nthreads);
+/* sum(adj * frame) of the fixed paths re-shaded with positions / normals
+ * (3 doubles per vertex, NULL = the scene's own). */
+int orc_pt_replay_geometry(const orc_scene *s, const orc_pt_params *params, const double *adj, const double *pos,
+                           const double *nrm, double *out, int nthreads);
+
 /* One round of batched tracing: hits[i] = closest hit of ray i (per-ray
  * TLAS from meta), alive[i] = 1 if it hit, and then spawned[i] = the
  * continuation ray of path i at bounce `round`. */

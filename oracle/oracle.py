@@ -77,6 +77,9 @@ _SIG = {
     "orc_pt_render_rows": (C.c_int, [P, C.POINTER(OrcPtParams), u32, u32, P, P, C.c_int]),
     "orc_batch_round": (C.c_int, [P, P, P, u32, u32, u32, P, P, P, C.c_int]),
     "orc_pt_grad": (C.c_int, [P, C.POINTER(OrcPtParams), P, P, P, C.c_int]),
+    "orc_vertex_count": (u32, [P]),
+    "orc_pt_grad_geometry": (C.c_int, [P, C.POINTER(OrcPtParams), P, P, P, C.c_int]),
+    "orc_pt_replay_geometry": (C.c_int, [P, C.POINTER(OrcPtParams), P, P, P, P, C.c_int]),
 }
 
 _lib = None
@@ -285,6 +288,41 @@ class OracleScene:
         if rc != 0:
             raise RuntimeError("oracle pt_grad failed")
         return ga, ge
+
+    def vertex_count(self) -> int:
+        return int(self.L.orc_vertex_count(self.h))
+
+    def _adj(self, params, adj):
+        return np.ascontiguousarray(adj, dtype=np.float64).reshape(max(params.views, 1) * params.height,
+                                                                   params.width, 3)
+
+    def pt_grad_geometry(self, params: "OrcPtParams", adj: np.ndarray, threads: int = 0):
+        """Gradient of sum(adj * frame) w.r.t. (positions, normals), each
+        (vertex_count, 3) float64, paths held fixed."""
+        adj = self._adj(params, adj)
+        nv = self.vertex_count()
+        gp = np.zeros((nv, 3), dtype=np.float64)
+        gn = np.zeros((nv, 3), dtype=np.float64)
+        rc = self.L.orc_pt_grad_geometry(self.h, C.byref(params), adj.ctypes.data, gp.ctypes.data, gn.ctypes.data,
+                                         threads)
+        if rc == -2:
+            raise ValueError("the gradient needs rr_start >= max_bounces (no Russian roulette)")
+        if rc != 0:
+            raise RuntimeError("oracle pt_grad_geometry failed")
+        return gp, gn
+
+    def pt_replay_geometry(self, params: "OrcPtParams", adj: np.ndarray, pos=None, nrm=None, threads: int = 0):
+        """sum(adj * frame) of the fixed paths re-shaded with vertex positions
+        / normals ((vertex_count, 3) float64, None = the scene's own)."""
+        adj = self._adj(params, adj)
+        pos = None if pos is None else np.ascontiguousarray(pos, dtype=np.float64)
+        nrm = None if nrm is None else np.ascontiguousarray(nrm, dtype=np.float64)
+        out = C.c_double()
+        rc = self.L.orc_pt_replay_geometry(self.h, C.byref(params), adj.ctypes.data, _p(pos), _p(nrm),
+                                           C.byref(out), threads)
+        if rc != 0:
+            raise RuntimeError("oracle pt_replay_geometry failed")
+        return float(out.value)
 
     def trace_brute(self, rays: np.ndarray, flags=0, cull_mask=0xFFFFFFFF, tlas_idx=0, threads: int = 0):
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)

@@ -229,3 +229,43 @@ BATCHED_SCENES = 64
 BATCHED_TRIS = (10_000, 2_000_000)
 BATCHED_RAYS = (16_384, 4_194_304)
 BATCHED_BOUNCES = 3
+
+
+def height_field(n: int = 1000, seed: int = 0, amplitude: float = 0.05, waves: int = 8, extent: float = 1.0):
+    """n x n vertex grid over [-extent, extent]^2 in the xz-plane displaced
+    along +y by sum_k amplitude * sin(w_k . (x, z) + phi_k), k < waves, with
+    w_k = 2 pi f_k (cos t_k, sin t_k), f_k ~ U[0.5, 3], t_k ~ U[0, 2 pi),
+    phi_k ~ U[0, 2 pi) (stream 50).  Returns (positions (n*n, 3) f32,
+    indices (2 (n-1)^2, 3) u32 facing +y, analytic normals (n*n, 3) f32)."""
+    u = uniform_f32(seed, 50, 3 * waves).astype(np.float64).reshape(waves, 3)
+    f = 0.5 + 2.5 * u[:, 0]
+    th = 2.0 * np.pi * u[:, 1]
+    phi = 2.0 * np.pi * u[:, 2]
+    wx, wz = 2.0 * np.pi * f * np.cos(th), 2.0 * np.pi * f * np.sin(th)
+    g = np.linspace(-extent, extent, n)
+    x, z = np.meshgrid(g, g, indexing="xy")
+    x, z = x.reshape(-1), z.reshape(-1)
+    arg = x[:, None] * wx[None, :] + z[:, None] * wz[None, :] + phi[None, :]
+    y = amplitude * np.sin(arg).sum(axis=1)
+    dy_dx = amplitude * (np.cos(arg) * wx[None, :]).sum(axis=1)
+    dy_dz = amplitude * (np.cos(arg) * wz[None, :]).sum(axis=1)
+    pos = np.stack([x, y, z], axis=1).astype(np.float32)
+    nrm = np.stack([-dy_dx, np.ones_like(x), -dy_dz], axis=1)
+    nrm = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
+    i = np.arange(n - 1, dtype=np.uint32)
+    r, c = np.meshgrid(i, i, indexing="ij")
+    v00 = (r * n + c).reshape(-1)
+    v01, v10, v11 = v00 + 1, v00 + n, v00 + n + 1
+    # (v00, v10, v01): e1 = +z, e2 = +x, e1 x e2 = +y
+    idx = np.concatenate([np.stack([v00, v10, v01], axis=1), np.stack([v01, v10, v11], axis=1)]).astype(np.uint32)
+    return pos, idx, nrm
+
+
+def sphere_views(count: int = 16, radius: float = 3.0, min_elevation_deg: float = 20.0):
+    """Camera positions on the upper part of a sphere (Fibonacci spiral over
+    elevations [min_elevation, 80] degrees), all looking at the origin."""
+    k = np.arange(count, dtype=np.float64)
+    el = np.radians(min_elevation_deg + (80.0 - min_elevation_deg) * (k + 0.5) / count)
+    az = k * np.pi * (3.0 - np.sqrt(5.0))
+    return [(float(radius * np.cos(e) * np.cos(a)), float(radius * np.sin(e)), float(radius * np.cos(e) * np.sin(a)))
+            for e, a in zip(el, az)]

@@ -83,3 +83,74 @@ def test_gradient_refuses_russian_roulette(grad_case):
             o.pt_grad(prm, adj, albedo.shape[0])
     finally:
         prm.rr_start = old
+
+
+# ----------------------------------------------------------- geometry ------
+@pytest.fixture(scope="module")
+def geo_case():
+    """A 24 x 24 height field with vertex normals under a sphere without
+    normals (geometric shading), two views, 2 spp, 3 bounces."""
+    hp, hi, hn = synth.height_field(24, seed=1)
+    sp, si = synth.uv_sphere(12, 8, 0.3)
+    sp = (sp + np.array([0.2, 0.45, -0.1], np.float32)).astype(np.float32)
+    scene = {"meshes": [{"positions": hp, "indices": hi, "normals": hn, "tri_material": None},
+                        {"positions": sp, "indices": si, "normals": None, "tri_material": None}],
+             "instances": [{"transform": np.eye(4), "mesh": 0, "material": 0},
+                           {"transform": np.eye(4), "mesh": 1, "material": 1}],
+             "albedo": np.array([[0.6, 0.5, 0.4], [0.7, 0.7, 0.7]], np.float32),
+             "emission": np.zeros((2, 3), np.float32)}
+    o = oracle_scene_from(scene)
+    cams = [Camera(position=p, aspect=1.0) for p in synth.sphere_views(2, radius=2.5)]
+    cfg = PathTracingConfig(width=32, height=32, spp=2, max_bounces=3, rr_start=3, seed=2)
+    prm = oracle_params(cams, cfg)
+    rng = np.random.default_rng(5)
+    adj = rng.uniform(-1.0, 1.0, size=(2 * cfg.height, cfg.width, 3))
+    gp, gn = o.pt_grad_geometry(prm, adj)
+    pos0 = np.concatenate([hp, sp]).astype(np.float64)
+    nrm0 = np.concatenate([hn, np.zeros_like(sp)]).astype(np.float64)
+    return o, prm, adj, gp, gn, pos0, nrm0, hp.shape[0]
+
+
+def test_replay_reproduces_the_frame(geo_case):
+    o, prm, adj, _, _, pos0, nrm0, _ = geo_case
+    img, _, _ = o.pt_render(prm)
+    f = float(np.sum(adj * img))
+    assert abs(o.pt_replay_geometry(prm, adj) - f) <= 1e-6 * np.sum(np.abs(adj * img))
+    same = o.pt_replay_geometry(prm, adj, pos0, nrm0)  # explicit arrays equal to the scene's own
+    assert abs(same - f) <= 1e-6 * np.sum(np.abs(adj * img))
+
+
+@pytest.mark.parametrize("which", ["pos", "nrm"])
+def test_geometry_gradient_against_central_differences(geo_case, which):
+    o, prm, adj, gp, gn, pos0, nrm0, nh = geo_case
+    g = gp if which == "pos" else gn
+    assert np.count_nonzero(g) > 100
+
+    def f(x):
+        return o.pt_replay_geometry(prm, adj, x if which == "pos" else pos0, x if which == "nrm" else nrm0)
+
+    base = pos0 if which == "pos" else nrm0
+    rng = np.random.default_rng(7)
+    d = rng.uniform(-1.0, 1.0, size=base.shape)
+    lin = float(np.sum(g * d))
+    # moving every vertex at once deforms triangles of edge ~0.09 by ~1%:
+    # the O(h^2) term of the central difference shows, so positions are also
+    # checked for its quadratic decay at h / 10
+    steps = [(H, 1e-5)] if which == "nrm" else [(H, 1e-2), (H / 10, 1e-4)]
+    for h, tol in steps:
+        fd = (f(base + h * d) - f(base - h * d)) / (2 * h)
+        assert abs(fd - lin) <= tol * abs(lin), (h, fd, lin)
+    scale = np.abs(g).max()
+    for idx in np.argsort(-np.abs(g).ravel())[:4]:
+        e = np.zeros(base.size)
+        e[idx] = 1.0
+        e = e.reshape(base.shape)
+        h = H if which == "nrm" else H / 10
+        fd1 = (f(base + h * e) - f(base - h * e)) / (2 * h)
+        assert abs(fd1 - g.ravel()[idx]) <= 1e-5 * scale, (idx, fd1, g.ravel()[idx])
+
+
+def test_geometric_shading_has_no_normal_gradient(geo_case):
+    _, _, _, gp, gn, _, _, nh = geo_case
+    assert np.all(gn[nh:] == 0.0)  # the sphere has no vertex normals
+    assert np.count_nonzero(gp[nh:]) > 0 and np.count_nonzero(gn[:nh]) > 0
