@@ -218,6 +218,11 @@ typedef struct rdrt_material {
 /* Per-vertex normals of one geometry of a BLAS (xyz per vertex, copied);
  * interpolated with the hit barycentrics for shading.  Without them the
  * geometric normal is used. */
+/* Where a geometry's vertices sit in the scene vertex order (all live BLAS
+ * in creation order, their geometries in order): the layout of the geometry
+ * gradients of rdrt_pt_backward_geometry. */
+int32_t rdrt_geometry_vertex_range(rdrt_ctx *ctx, uint32_t blas, uint32_t geometry, uint32_t *first,
+                                   uint32_t *count);
 int32_t rdrt_blas_set_vertex_normals(rdrt_ctx *ctx, uint32_t blas, uint32_t geometry, const float *normals_xyz,
                                      uint32_t nverts);
 /* Per-triangle material offsets of one geometry (copied).  A hit's material
@@ -286,6 +291,19 @@ int32_t rdrt_pt_set_material_params(rdrt_ctx *ctx, const float *albedo_dev, cons
  * roulette (rr_start >= max_bounces): its termination probability depends on
  * the albedo. */
 int32_t rdrt_pt_backward(rdrt_ctx *ctx, const float *adj_dev, float *grad_albedo_dev, float *grad_emission_dev);
+/* Gradient of sum(adj * frame) over the last frame rendered with record = 1
+ * w.r.t. vertex positions and normals (scene vertex order, see
+ * rdrt_geometry_vertex_range; vertex_count * 3 f32 each, overwritten), with
+ * every path held fixed: traced rays, hit triangles, light visibility, facing
+ * and sampled directions.  The shading normal -- the barycentric
+ * interpolation of the vertex normals, or the face normal without them --
+ * enters the light term and the continuation factor a (ns . wo) / cos_wo;
+ * positions act through the barycentrics of the fixed ray's hit (or the face
+ * normal).  The paths are recomputed by replaying the frame, not stored.
+ * grad_nrm_dev may be NULL when no geometry has vertex normals.  Same frame
+ * conditions as rdrt_pt_backward. */
+int32_t rdrt_pt_backward_geometry(rdrt_ctx *ctx, const float *adj_dev, float *grad_positions_dev,
+                                  float *grad_normals_dev);
 /* L2 loss helper: adj_dev[i] = 2 (a[i] - b[i]); *loss_dev (one f64) =
  * sum (a[i] - b[i])^2.  Device pointers; enqueued on the context's stream. */
 int32_t rdrt_l2_adjoint(rdrt_ctx *ctx, const float *a_dev, const float *b_dev, uint32_t n, float *adj_dev,

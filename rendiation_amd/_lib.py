@@ -128,6 +128,8 @@ SIGNATURES = {
     "rdrt_pt_set_material_params": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, u32]),
     "rdrt_pt_backward": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "rdrt_l2_adjoint": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p]),
+    "rdrt_pt_backward_geometry": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "rdrt_geometry_vertex_range": (i32, [C.c_void_p, u32, u32, C.POINTER(u32), C.POINTER(u32)]),
     "rdrt_trace_batched": (i32, [C.c_void_p, C.POINTER(RdrtBatchParams)]),
     "rdrt_batch_get_stats": (i32, [C.c_void_p, C.POINTER(RdrtBatchStats)]),
     "rdrt_compact": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p]),

@@ -328,6 +328,12 @@ class AccelerationStructureSystem:
         L.check(self._L.rdrt_batch_get_stats(self._h, C.byref(o)))
         return BatchStats(rounds=o.rounds, rays=list(o.rays)[:o.rounds], launches=o.launches, ms=float(o.ms))
 
+    def geometry_vertex_range(self, blas: BlasHandle, geometry: int = 0):
+        """(first, count) of a geometry's vertices in the scene vertex order."""
+        first, count = L.u32(), L.u32()
+        L.check(self._L.rdrt_geometry_vertex_range(self._h, blas.index, geometry, C.byref(first), C.byref(count)))
+        return int(first.value), int(count.value)
+
     def sync(self) -> None:
         L.check(self._L.rdrt_sync(self._h))
 

@@ -68,6 +68,27 @@ GeometrySrc *geometry_of(rdrt_ctx *c, uint32_t blas, uint32_t geometry) {
 
 extern "C" {
 
+int32_t rdrt_geometry_vertex_range(rdrt_ctx *c, uint32_t blas, uint32_t geometry, uint32_t *first,
+                                   uint32_t *count) {
+  if (!c || !first || !count) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  if (!geometry_of(c, blas, geometry)) return fail(RDRT_ERR_INVALID_ARG, "unknown BLAS or geometry");
+  // vertices are concatenated over live BLAS in creation order, geometries in order
+  uint64_t at = 0;
+  for (uint32_t b = 0; b < c->src.blas.size(); b++) {
+    if (!c->src.blas_live[b]) continue;
+    for (uint32_t gi = 0; gi < c->src.blas[b].size(); gi++) {
+      const uint64_t nv = c->src.blas[b][gi].pos.size() / 3;
+      if (b == blas && gi == geometry) {
+        *first = (uint32_t)at;
+        *count = (uint32_t)nv;
+        return RDRT_OK;
+      }
+      at += nv;
+    }
+  }
+  return fail(RDRT_ERR_INVALID_ARG, "unknown BLAS or geometry");
+}
+
 int32_t rdrt_blas_set_vertex_normals(rdrt_ctx *c, uint32_t blas, uint32_t geometry, const float *normals,
                                      uint32_t nverts) {
   if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
@@ -296,6 +317,30 @@ int32_t rdrt_pt_backward(rdrt_ctx *c, const float *adj, float *grad_albedo, floa
   if (s.nmaterials_uploaded != L.nmaterials)
     return fail(RDRT_ERR_STATE, "the material count changed since the recorded frame");
   RDRT_HIP(enqueue_pt_backward(L, adj, grad_albedo, grad_emission, c->work_stream));
+  return RDRT_OK;
+}
+
+int32_t rdrt_pt_backward_geometry(rdrt_ctx *c, const float *adj, float *grad_pos, float *grad_nrm) {
+  if (!c || !adj || !grad_pos) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  PtState &s = c->pt;
+  if (!s.last_recorded || !s.last_frame.record)
+    return fail(RDRT_ERR_STATE, "no recorded frame (render with record = 1 first)");
+  if (s.last_frame.rr_start < s.last_frame.max_bounces)
+    return fail(RDRT_ERR_UNSUPPORTED, "the adjoint needs a frame without Russian roulette (rr_start >= max_bounces)");
+  if (s.nmaterials_uploaded != s.last_frame.nmaterials)
+    return fail(RDRT_ERR_STATE, "the material count changed since the recorded frame");
+  if (c->dev.normals.count && !grad_nrm) return fail(RDRT_ERR_INVALID_ARG, "the scene has vertex normals: grad_nrm needed");
+  PtLaunch L = s.last_frame;
+  L.materials = (const Material *)s.materials.ptr;
+  L.adj = adj;
+  L.g_pos = grad_pos;
+  L.g_nrm = grad_nrm;  // zeroed by the replay; stays zero without vertex normals
+  L.nvertices = (uint32_t)(c->dev.vertices.count / 3);
+  const TraceParams tp = scene_trace_params(c);
+  const uint32_t block = c->cfg.block_size ? c->cfg.block_size : 256;
+  RDRT_HIP(enqueue_pt_replay(L, tp, push_depth(c), block, c->work_stream));
   return RDRT_OK;
 }
 

@@ -50,6 +50,12 @@ struct PtLaunch {
   // record[k * npaths + p], max_bounces + 2 slots; (material, cos_l / pi of
   // unoccluded light or 0), or (kVertSkyUp / kVertSkyDown / kVertEnd, 0)
   uint2 *record;
+
+  // geometry adjoint (replay frames): pixel adjoint (views * width * height
+  // * 3), gradients w.r.t. vertex positions and normals (nvertices * 3)
+  const float *adj;
+  float *g_pos, *g_nrm;
+  uint32_t nvertices;
 };
 
 constexpr uint32_t kVertSkyUp = 0xFFFFFFF0u, kVertSkyDown = 0xFFFFFFF1u, kVertEnd = 0xFFFFFFF2u;
@@ -70,6 +76,11 @@ hipError_t enqueue_pt_frame(const PtLaunch &L, const TraceParams &scene_tp, uint
 // recorded vertices of frame L (grads zeroed here, then accumulated).
 hipError_t enqueue_pt_backward(const PtLaunch &L, const float *adj, float *grad_albedo, float *grad_emission,
                                hipStream_t stream);
+// Replays the recorded frame L (same kernels, no shadow rays) and
+// accumulates the gradient w.r.t. vertex positions and normals into
+// L.g_pos / L.g_nrm (zeroed here).
+hipError_t enqueue_pt_replay(const PtLaunch &L, const TraceParams &scene_tp, uint32_t push_depth, uint32_t block,
+                             hipStream_t stream);
 // Packs device albedo / emission arrays (n * 3 each) into Material records.
 hipError_t enqueue_pack_materials(const float *albedo, const float *emission, uint32_t n, Material *out,
                                   hipStream_t stream);

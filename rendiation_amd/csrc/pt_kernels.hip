@@ -102,16 +102,52 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_camera(PtLaunch P) {
   P.qa[path] = path;
 }
 
-// closest hit + miss for one path (ray_hit.rs:6-104, ray_miss.rs:136-158)
+// Radiance beyond vertex `first` of a path, divided by the throughput at
+// that vertex, from its recorded vertices: the sky or nothing at the end,
+// e + a (w sun + rest) at each material vertex.
+__device__ V3 recorded_suffix(const PtLaunch &P, uint32_t path, uint32_t first) {
+  const V3 sun = load3(P.sun_radiance);
+  V3 T = v3(0.0f, 0.0f, 0.0f), prod = v3(1.0f, 1.0f, 1.0f);
+  for (uint32_t j = first; j < P.max_bounces + 2; j++) {
+    const uint2 r = P.record[(size_t)j * P.npaths + path];
+    if (r.x >= kVertSkyUp) {
+      if (r.x != kVertEnd) T = T + prod * load3(r.x == kVertSkyUp ? P.sky_up : P.sky_down);
+      break;
+    }
+    const Material &m = P.materials[r.x < P.nmaterials ? r.x : P.nmaterials - 1];
+    const V3 a = v3(m.albedo[0], m.albedo[1], m.albedo[2]);
+    T = T + prod * (v3(m.emission[0], m.emission[1], m.emission[2]) + a * sun * __uint_as_float(r.y));
+    prod = prod * a;
+  }
+  return T;
+}
+
+__device__ __forceinline__ void atomic_add3(float *g, uint32_t i, V3 v) {
+  unsafeAtomicAdd(g + 3 * (size_t)i, v.x);
+  unsafeAtomicAdd(g + 3 * (size_t)i + 1, v.y);
+  unsafeAtomicAdd(g + 3 * (size_t)i + 2, v.z);
+}
+
+// closest hit + miss for one path (ray_hit.rs:6-104, ray_miss.rs:136-158).
+// REPLAY: the same path again after a recorded frame (bit-identical rays,
+// hits and samples), adding the gradient of sum(adj * frame) w.r.t. vertex
+// positions and normals at this vertex instead of radiance and shadow rays.
+// With the path held fixed, the shading normal ns = sigma normalize(R n)
+// (R = normal_to_world, n the interpolated object normal or e1 x e2) enters
+// the light term beta a sun (ns . wi) / pi and the continuation factor
+// a (ns . wo) / cos_wo; (u, v) are where the fixed object ray meets the
+// triangle.
+template <bool REPLAY>
 __device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &keep, bool &shadow) {
   const uint4 h = *reinterpret_cast<const uint4 *>(P.hits + path);
   const float4 *rq = reinterpret_cast<const float4 *>(P.rays + path);
   const float4 r0 = rq[0], r1 = rq[1];
   const V3 o = v3(r0.x, r0.y, r0.z), d = v3(r1.x, r1.y, r1.z);
   V3 beta = load3(P.beta + 3 * (size_t)path);
-  V3 L = load3(P.L + 3 * (size_t)path);
-  uint2 *rec = P.record ? P.record + (size_t)b * P.npaths + path : nullptr;
+  V3 L = REPLAY ? v3(0.0f, 0.0f, 0.0f) : load3(P.L + 3 * (size_t)path);
+  uint2 *rec = (!REPLAY && P.record) ? P.record + (size_t)b * P.npaths + path : nullptr;
   if (h.x == kMiss) {
+    if (REPLAY) return;
     const float *sky = d.y > 0.0f ? P.sky_up : P.sky_down;
     L = L + beta * load3(sky);
     store3(P.L + 3 * (size_t)path, L);
@@ -127,17 +163,28 @@ __device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &ke
                  i2 = P.indices[3 * (size_t)tri + 2];
   const V3 p0 = load3(P.vertices + 3 * (size_t)i0), p1 = load3(P.vertices + 3 * (size_t)i1),
            p2 = load3(P.vertices + 3 * (size_t)i2);
-  V3 ng = pt::normalize3(normal_to_world(in.inv, pt::cross3(p1 - p0, p2 - p0)));
-  V3 ns = ng;
+  const V3 e1 = p1 - p0, e2 = p2 - p0;
+  const V3 m_geo = normal_to_world(in.inv, pt::cross3(e1, e2));
+  V3 ng = pt::normalize3(m_geo);
+  V3 ns = ng, m_ns = m_geo;
+  bool interp = false;
   if (P.normals) {
     const float bw = 1.0f - bu - bv;
     V3 n_obj = load3(P.normals + 3 * (size_t)i0) * bw + load3(P.normals + 3 * (size_t)i1) * bu +
                load3(P.normals + 3 * (size_t)i2) * bv;
-    if (pt::dot3(n_obj, n_obj) > 0.0f) ns = pt::normalize3(normal_to_world(in.inv, n_obj));
+    if (pt::dot3(n_obj, n_obj) > 0.0f) {
+      m_ns = normal_to_world(in.inv, n_obj);
+      ns = pt::normalize3(m_ns);
+      interp = true;
+    }
   }
   // face the incoming ray (bindless_mesh_bridge.rs:86-123)
   if (pt::dot3(ng, d) > 0.0f) ng = ng * -1.0f;
-  if (pt::dot3(ns, ng) < 0.0f) ns = ns * -1.0f;
+  float sigma = 1.0f;
+  if (pt::dot3(ns, ng) < 0.0f) {
+    ns = ns * -1.0f;
+    sigma = -1.0f;
+  }
   const V3 pos = o + d * t;
   uint32_t mi = in.custom_index + (P.tri_material ? P.tri_material[tri] : 0u);
   if (mi >= P.nmaterials) mi = P.nmaterials - 1;
@@ -152,7 +199,7 @@ __device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &ke
     // next-event estimation toward the directional light
     const V3 wi = v3(-P.sun_dir[0], -P.sun_dir[1], -P.sun_dir[2]);
     const float cos_l = pt::dot3(ns, wi);
-    if (cos_l > 0.0f && pt::dot3(ng, wi) > 0.0f &&
+    if (!REPLAY && cos_l > 0.0f && pt::dot3(ng, wi) > 0.0f &&
         (P.sun_radiance[0] > 0.0f || P.sun_radiance[1] > 0.0f || P.sun_radiance[2] > 0.0f)) {
       const V3 c = beta * albedo * (cos_l * kInvPi) * load3(P.sun_radiance);
       store3(P.nee + 3 * (size_t)path, c);
@@ -177,7 +224,52 @@ __device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &ke
       store3(P.beta + 3 * (size_t)path, nb);
       keep = true;
     }
+    if (REPLAY) {
+      const float w = __uint_as_float(P.record[(size_t)b * P.npaths + path].y);
+      const V3 a_pix = load3(P.adj + 3 * (size_t)(path / P.spp)) * (1.0f / (float)P.spp);
+      V3 g_ns = v3(0.0f, 0.0f, 0.0f);
+      if (w != 0.0f) g_ns = wi * (pt::dot3(a_pix, beta * albedo * load3(P.sun_radiance)) * kInvPi);
+      const float cos_wo = pt::dot3(ns, wo);
+      if (alive && cos_wo > 1e-6f) {
+        const V3 rem = nb * recorded_suffix(P, path, b + 1);
+        g_ns = g_ns + wo * (pt::dot3(a_pix, rem) / cos_wo);
+      }
+      if (g_ns.x != 0.0f || g_ns.y != 0.0f || g_ns.z != 0.0f) {
+        const V3 mv = interp ? m_ns : m_geo;
+        const float inv_len = 1.0f / sqrtf(pt::dot3(mv, mv));
+        const V3 g_m = (g_ns - ns * pt::dot3(ns, g_ns)) * (sigma * inv_len);
+        const float *R = in.inv;
+        const V3 g_n = v3(R[0] * g_m.x + R[4] * g_m.y + R[8] * g_m.z, R[1] * g_m.x + R[5] * g_m.y + R[9] * g_m.z,
+                          R[2] * g_m.x + R[6] * g_m.y + R[10] * g_m.z);
+        if (interp) {
+          const float bw = 1.0f - bu - bv;
+          atomic_add3(P.g_nrm, i0, g_n * bw);
+          atomic_add3(P.g_nrm, i1, g_n * bu);
+          atomic_add3(P.g_nrm, i2, g_n * bv);
+          const V3 n0 = load3(P.normals + 3 * (size_t)i0);
+          const float g_u = pt::dot3(g_n, load3(P.normals + 3 * (size_t)i1) - n0);
+          const float g_v = pt::dot3(g_n, load3(P.normals + 3 * (size_t)i2) - n0);
+          // (t, u, v) solves [-bd e1 e2] x = bo - p0 for the object-space ray
+          const V3 c0 = v3(-(R[0] * d.x + R[4] * d.y + R[8] * d.z), -(R[1] * d.x + R[5] * d.y + R[9] * d.z),
+                           -(R[2] * d.x + R[6] * d.y + R[10] * d.z));
+          const float det = pt::dot3(c0, pt::cross3(e1, e2));
+          if (det != 0.0f) {
+            const V3 G = (pt::cross3(e2, c0) * g_u + pt::cross3(c0, e1) * g_v) * (1.0f / det);
+            atomic_add3(P.g_pos, i0, G * -bw);
+            atomic_add3(P.g_pos, i1, G * -bu);
+            atomic_add3(P.g_pos, i2, G * -bv);
+          }
+        } else {
+          // n = e1 x e2
+          const V3 g_e1 = pt::cross3(e2, g_n), g_e2 = pt::cross3(g_n, e1);
+          atomic_add3(P.g_pos, i1, g_e1);
+          atomic_add3(P.g_pos, i2, g_e2);
+          atomic_add3(P.g_pos, i0, (g_e1 + g_e2) * -1.0f);
+        }
+      }
+    }
   }
+  if (REPLAY) return;
   store3(P.L + 3 * (size_t)path, L);
   if (rec) {
     // the light weight stands until the shadow ray finds an occluder
@@ -186,7 +278,7 @@ __device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &ke
   }
 }
 
-template <int B>
+template <int B, bool REPLAY>
 __global__ __launch_bounds__(B) void k_pt_shade(PtLaunch P, uint32_t b) {
   __shared__ uint32_t s_bid;
   if (threadIdx.x == 0) s_bid = atomicAdd(P.tickets + b, 1u);
@@ -202,7 +294,7 @@ __global__ __launch_bounds__(B) void k_pt_shade(PtLaunch P, uint32_t b) {
   uint32_t path = 0;
   if (i < count_in && i < P.npaths) {
     path = qin[i];
-    if (path < P.npaths) shade_one(P, b, path, keep, shadow);
+    if (path < P.npaths) shade_one<REPLAY>(P, b, path, keep, shadow);
   }
   compact2<B>(keep, shadow, path, qout, P.qs, P.lookback + (size_t)b * P.lookback_stride, bid, nblocks,
               P.counts + b + 1, P.counts + P.max_bounces + 2 + b, P.status, P.npaths);
@@ -441,7 +533,7 @@ hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_
     tp.tlas_idx = 0;
     hipError_t e = trace(tp);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_pt_shade<kPtBlock>), dim3(grid), dim3(kPtBlock), 0, st, P, b);
+    hipLaunchKernelGGL((k_pt_shade<kPtBlock, false>), dim3(grid), dim3(kPtBlock), 0, st, P, b);
     if (b == P.max_bounces) break;
     TraceParams sp = tp;
     sp.rays = P.shadow_rays;
@@ -456,6 +548,36 @@ hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_
   }
   hipLaunchKernelGGL(k_pt_finalize, dim3((P.nviews * P.width * P.height + kPtBlock - 1) / kPtBlock), dim3(kPtBlock),
                      0, st, P);
+  return hipGetLastError();
+}
+
+hipError_t enqueue_pt_replay(const PtLaunch &P, const TraceParams &scene, uint32_t push_depth, uint32_t block,
+                             hipStream_t st) {
+  const uint32_t grid = (P.npaths + kPtBlock - 1) / kPtBlock;
+  const uint32_t nv = P.nvertices;
+  hipLaunchKernelGGL(k_fill_f32, dim3(256), dim3(256), 0, st, P.g_pos, 3 * nv);
+  if (P.g_nrm) hipLaunchKernelGGL(k_fill_f32, dim3(256), dim3(256), 0, st, P.g_nrm, 3 * nv);
+  hipLaunchKernelGGL(k_pt_begin, dim3(256), dim3(256), 0, st, P);
+  hipLaunchKernelGGL(k_pt_camera, dim3(grid), dim3(kPtBlock), 0, st, P);
+  for (uint32_t b = 0; b <= P.max_bounces; b++) {
+    TraceParams tp = scene;
+    tp.rays = P.rays;
+    tp.hits = P.hits;
+    tp.hitx = P.hitx;
+    tp.meta = nullptr;
+    tp.hit_meta = nullptr;
+    tp.counters = nullptr;
+    tp.queue = (b & 1) ? P.qb : P.qa;
+    tp.count = P.counts + b;
+    tp.n = P.npaths;
+    tp.flags = RDRT_RAY_FLAG_CULL_BACK_FACING_TRIANGLES;
+    tp.cull_mask = 0xffffffffu;
+    tp.tlas_idx = 0;
+    hipError_t e = launch_trace(tp, RDRT_TRAVERSAL_FAST, push_depth, block, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_pt_shade<kPtBlock, true>), dim3(grid), dim3(kPtBlock), 0, st, P, b);
+    if (b == P.max_bounces) break;
+  }
   return hipGetLastError();
 }
 

@@ -166,6 +166,17 @@ class PathTracer:
         L.check(self.system._L.rdrt_pt_backward(self.system._h, int(adj.data_ptr()), int(grad_albedo.data_ptr()),
                                                 int(grad_emission.data_ptr())))
 
+    def backward_geometry(self, adj, grad_positions, grad_normals=None) -> None:
+        """Gradient of sum(adj * frame) of the last recorded frame w.r.t.
+        vertex positions and normals ((vertex_count, 3) float32 tensors in
+        scene vertex order), replaying the frame with its paths held fixed."""
+        for t in (adj, grad_positions) + (() if grad_normals is None else (grad_normals,)):
+            if not getattr(t, "is_cuda", False) or not t.is_contiguous() or t.dtype != _f32():
+                raise ValueError("backward_geometry takes contiguous float32 device tensors")
+        L.check(self.system._L.rdrt_pt_backward_geometry(
+            self.system._h, int(adj.data_ptr()), int(grad_positions.data_ptr()),
+            None if grad_normals is None else int(grad_normals.data_ptr())))
+
     def l2_adjoint(self, image, target, adj, loss) -> None:
         """adj = 2 (image - target); loss (one float64 device element) =
         sum (image - target)^2, enqueued on the system's stream."""
