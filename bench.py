@@ -1,6 +1,6 @@
 """Benchmark of the ray-query hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pt|closest_hit|batched|diff]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pt|closest_hit|batched|diff|diffgeo]
 
 pt (default): wavefront path tracer on the parametric scene -- 64 UV spheres
   and tori tessellated at 256x256 (8,388,608 triangles, one BLAS each, 64
@@ -20,6 +20,9 @@ diff: differentiable path tracing -- 64 spheres/tori at 90x90 (1.04M
   target rendered from parameters perturbed by N(0, 0.05); value = ms of
   forward + adjoint per frame (lower is better), with the forward alone and
   the gradient error against the FP64 oracle beside it.
+diffgeo: gradient w.r.t. 1M vertex positions and normals of a 1000x1000
+  height field, 16 views at 512x512 in one frame, 2 spp, 3 bounces, paths
+  recomputed by a replay of the frame; value = ms of forward + adjoint.
 
 roofline: HBM-bound.  Algorithmic bytes per traced ray = 32 (ray) + 16 (hit)
 + 48 x node visits + 52 x triangle tests + 176 x instance tests, with visit
@@ -115,6 +118,12 @@ def path_tracing(args):
         out = torch.empty((cfg.height, cfg.width, 3), dtype=torch.float32, device=dev)
     stream.synchronize()
 
+    if args.profile_only:  # the frames alone, for hardware counters
+        for _ in range(args.steps):
+            pt.render(cam, cfg, out=out)
+        system.sync()
+        system.close()
+        return None
     for _ in range(args.warmup):
         pt.render(cam, cfg, out=out)
     system.sync()
@@ -232,6 +241,12 @@ def closest_hit(args):
         hits = torch.empty((n, 4), dtype=torch.int32, device=dev)
         cnt = torch.empty((n, 3), dtype=torch.int32, device=dev)
     stream.synchronize()
+    if args.profile_only:  # the launches alone, for hardware counters
+        for _ in range(args.steps):
+            system.trace(rays, hits, traversal=TRAVERSAL_FAST)
+        system.sync()
+        system.close()
+        return None
     system.trace(rays, hits, counters=cnt, traversal=TRAVERSAL_REFERENCE)
     system.sync()
     c = cnt.cpu().numpy().astype(np.int64).sum(axis=0)
@@ -440,6 +455,134 @@ def cpu_baseline_diff(scene, cam, cfg, adj, ga, ge):
     return base, err
 
 
+# ------------------------------------------------- differentiable geometry
+def differentiable_geometry(args):
+    import numpy as np
+    import torch
+
+    from rendiation_amd import synth
+    from rendiation_amd.acceleration import AccelerationStructureSystem
+    from rendiation_amd.pathtracer import Camera, PathTracer, PathTracingConfig, load_scene
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    n = 1000
+    t0 = time.perf_counter()
+    hp, hi, hn = synth.height_field(n, seed=0)
+    gen_ms = (time.perf_counter() - t0) * 1e3
+    scene = {"meshes": [{"positions": hp, "indices": hi, "normals": hn, "tri_material": None}],
+             "instances": [{"transform": np.eye(4), "mesh": 0, "material": 0}],
+             "albedo": np.array([[0.6, 0.6, 0.6]], np.float32), "emission": np.zeros((1, 3), np.float32)}
+    system = AccelerationStructureSystem(device=0, stream=stream.cuda_stream)
+    t0 = time.perf_counter()
+    load_scene(system, scene)
+    build_upload_ms = (time.perf_counter() - t0) * 1e3
+    info = system.scene_info()
+    cams = [Camera(position=p, aspect=1.0, fov_y_deg=40.0) for p in synth.sphere_views(16, radius=3.0)]
+    cfg = PathTracingConfig(width=512, height=512, spp=2, max_bounces=3, rr_start=3, seed=0, use_graph=True)
+    tracer = PathTracer(system)
+    nv = info.verts
+    # target: the same views of a height field with other sinusoids
+    tp, _, tn = synth.height_field(n, seed=1)
+    tsys = AccelerationStructureSystem(device=0, stream=stream.cuda_stream)
+    load_scene(tsys, {**scene, "meshes": [{"positions": tp, "indices": hi, "normals": tn, "tri_material": None}]})
+    with torch.cuda.stream(stream):
+        target = PathTracer(tsys).render(cams, cfg).clone()
+        img = torch.empty_like(target)
+        adj = torch.empty_like(target)
+        loss = torch.zeros(1, dtype=torch.float64, device=dev)
+        g_pos = torch.empty((nv, 3), dtype=torch.float32, device=dev)
+        g_nrm = torch.empty((nv, 3), dtype=torch.float32, device=dev)
+    system.sync()
+    tsys.close()
+    import dataclasses
+
+    rec_cfg = dataclasses.replace(cfg, record=True)
+
+    def forward():
+        tracer.render(cams, cfg, out=img)
+
+    def forward_backward():
+        tracer.render(cams, rec_cfg, out=img)
+        tracer.l2_adjoint(img, target, adj, loss)
+        tracer.backward_geometry(adj, g_pos, g_nrm)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        system.sync()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        w0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            fn()
+        e1.record(stream)
+        system.sync()
+        return (time.perf_counter() - w0) * 1e3 / args.steps, e0.elapsed_time(e1) / args.steps
+
+    ms_f, ev_f = timed(forward)
+    ms_fb, ev_fb = timed(forward_backward)
+    st = tracer.stats()
+    npaths = len(cams) * cfg.width * cfg.height * cfg.spp
+    config = {
+        "workload": "diffgeo: 1000x1000-vertex height field (8 sinusoids, amplitude 0.05), 16 views at 512x512 in "
+                    "one frame, 2 spp, 3 bounces, L2 loss, grad w.r.t. 1M vertex positions + normals",
+        "tris": info.tris, "vertices": nv, "views": len(cams), "paths": npaths,
+        "ms_forward": round(ms_f, 4), "ms_forward_events": round(ev_f, 4),
+        "ms_forward_backward": round(ms_fb, 4), "ms_forward_backward_events": round(ev_fb, 4),
+        "fwd_bwd_over_fwd": round(ms_fb / ms_f, 4), "loss": float(loss.item()),
+        "extension_rays": st["extension_rays"], "scene_gen_ms": round(gen_ms, 1),
+        "build_upload_ms": round(build_upload_ms, 1),
+    }
+    extra = {}
+    if not args.no_cpu_baseline:
+        extra["cpu_baseline"], extra["gradient_error"] = cpu_baseline_diffgeo(scene, cams[:2], cfg, target, system,
+                                                                             tracer)
+    system.close()
+    rec = _record(ms_fb, "ms", args, ms_fb, config, None, extra)
+    rec["higher_is_better"] = False
+    return rec
+
+
+def cpu_baseline_diffgeo(scene, cams, cfg, target, system, tracer):
+    """The FP64 oracle geometry gradient of the first two views (all host
+    threads), and the relative error of the GPU gradient of the same frame."""
+    import dataclasses
+
+    import numpy as np
+    import torch
+
+    from tests.pt_util import oracle_params, oracle_scene_from
+
+    rec_cfg = dataclasses.replace(cfg, record=True)
+    nv = system.scene_info().verts
+    img = tracer.render(cams, rec_cfg)
+    adj = (2.0 * (img - target[:len(cams)])).contiguous()
+    g_pos = torch.empty((nv, 3), dtype=torch.float32, device=img.device)
+    g_nrm = torch.empty((nv, 3), dtype=torch.float32, device=img.device)
+    tracer.backward_geometry(adj, g_pos, g_nrm)
+    system.sync()
+    o = oracle_scene_from(scene)
+    threads = _host_threads()
+    t0 = time.perf_counter()
+    op, on = o.pt_grad_geometry(oracle_params(cams, cfg), adj.cpu().numpy().astype(np.float64).reshape(
+        -1, cfg.width, 3), threads=threads)
+    dt = time.perf_counter() - t0
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+    gp = g_pos.cpu().numpy().astype(np.float64)
+    gn = g_nrm.cpu().numpy().astype(np.float64)
+    npaths = len(cams) * cfg.width * cfg.height * cfg.spp
+    base = {"value": round(dt * 1e3, 1), "unit": "ms (FP64 forward + geometry adjoint)", "cores": threads,
+            "kind": "port", "sample": f"views 0-1 of the frame ({npaths} paths)", "cpu_model": _cpu_model()}
+    err = {"vs": "FP64 CPU oracle gradient, same adjoint image, views 0-1", "rel_err_positions": rel(gp, op),
+           "rel_err_normals": rel(gn, on), "bound": 1e-3}
+    return base, err
+
+
 # ------------------------------------------------------------------ batched
 def batched(args):
     import numpy as np
@@ -590,15 +733,73 @@ def cpu_baseline_batched(scenes, rays_np, meta_np, offs, bounces):
             "cpu_model": _cpu_model()}
 
 
+# ------------------------------------------------------ HBM traffic (PMC)
+PMC_STEPS = 3
+
+
+def pmc_traffic(workload: str, kernel: str):
+    """Physical DRAM-side bytes of `kernel` per step: two rocprofv3 --pmc
+    child runs (FETCH_SIZE, then WRITE_SIZE) of this benchmark in
+    --profile-only mode, each counter scaled by what it reads for a 256 MiB
+    device copy in the same run (--copy-calibration).  Run before this
+    process touches the GPU.  Returns a dict, or None when rocprofv3 is
+    unavailable or a pass fails."""
+    import csv
+    import shutil
+    import signal
+    import tempfile
+
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(exe):
+        return None
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="rdrt_pmc_", dir="/tmp")
+        cmd = [exe, "--pmc", counter, "-f", "csv", "-d", d, "-o", "p", "--", sys.executable, str(ROOT / "bench.py"),
+               "--workload", workload, "--steps", str(PMC_STEPS), "--warmup", "0", "--no-cpu-baseline", "--no-traffic",
+               "--copy-calibration", "--profile-only"]
+        env = dict(os.environ, TMPDIR="/tmp")
+        try:
+            proc = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                                    start_new_session=True)
+            try:
+                proc.wait(timeout=150)
+            except subprocess.TimeoutExpired:
+                os.killpg(proc.pid, signal.SIGKILL)
+                proc.wait()
+                return None
+            rows = []
+            for root, _, files in os.walk(d):
+                for f in files:
+                    if f.endswith("counter_collection.csv"):
+                        rows += list(csv.DictReader(open(os.path.join(root, f))))
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        copy = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]
+                and float(r["Counter_Value"]) > 2 ** 16]
+        vals = [float(r["Counter_Value"]) for r in rows if kernel in r["Kernel_Name"]]
+        if not copy or not vals:
+            return None
+        factor = 262144.0 / copy[0]  # KiB of the 256 MiB copy / what the counter read
+        out[counter] = {"kib_per_step": sum(vals) / PMC_STEPS, "copy_factor": factor, "dispatches": len(vals)}
+    total = sum(v["kib_per_step"] * v["copy_factor"] * 1024.0 for v in out.values())
+    return {"bytes_per_step": total, "fetch_kib": out["FETCH_SIZE"]["kib_per_step"],
+            "fetch_copy_factor": out["FETCH_SIZE"]["copy_factor"], "write_kib": out["WRITE_SIZE"]["kib_per_step"],
+            "write_copy_factor": out["WRITE_SIZE"]["copy_factor"], "dispatches_per_step":
+                out["FETCH_SIZE"]["dispatches"] / PMC_STEPS}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt", choices=["pt", "closest_hit", "batched", "diff"])
+    ap.add_argument("--workload", default="pt", choices=["pt", "closest_hit", "batched", "diff", "diffgeo"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 256 MiB device-to-device copy (a reference dispatch for HBM counters)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic passes")
+    ap.add_argument("--profile-only", action="store_true", help="run the timed launches only (counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see the sharding design)",
@@ -613,8 +814,16 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    traffic = None
+    if not args.no_traffic and not args.profile_only and args.workload in ("pt", "closest_hit"):
+        traffic = pmc_traffic(args.workload, "k_trace_fast")
     rec = {"pt": path_tracing, "closest_hit": closest_hit, "batched": batched,
-           "diff": differentiable}[args.workload](args)
+           "diff": differentiable, "diffgeo": differentiable_geometry}[args.workload](args)
+    if rec is None:
+        return
+    if traffic is not None and rec.get("roofline"):
+        rec["roofline"]["traffic"] = round(traffic["bytes_per_step"])
+        rec["roofline"]["traffic_detail"] = traffic
     print(json.dumps(rec))
 
 

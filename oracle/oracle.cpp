@@ -1118,29 +1118,65 @@ float offset_axis(float p, float n) {
   return std::fabs(p) < origin ? p + float_scale * n : pf;
 }
 
-// distribution_map.rs:10-28 + sampling.rs:72-86 + diffuse.rs:24-36, in FP64
-D3 sample_cosine(D3 n, double s0, double s1) {
-  const double PI = 3.14159265358979323846;
-  double ox = s0 * 2.0 - 1.0, oy = s1 * 2.0 - 1.0, x = 0.0, y = 0.0;
-  if (!(ox == 0.0 && oy == 0.0)) {
-    double r, theta;
-    if (std::fabs(ox) > std::fabs(oy)) {
-      r = ox;
-      theta = PI / 4.0 * (oy / ox);
-    } else {
-      r = oy;
-      theta = PI / 2.0 - PI / 4.0 * (ox / oy);
-    }
-    x = std::cos(theta) * r;
-    y = std::sin(theta) * r;
+
+// ---- the path's geometry in FP32 ----
+// Rays, sampled directions and every decision of a path (facing, light
+// gating, continuation, roulette) are computed in FP32 with the device's
+// operation order, as the reference computes them in f32; only radiance,
+// throughput and gradients are carried in FP64.  The device and the oracle
+// then trace the same paths, and FP32-vs-FP64 differences show only in the
+// shading arithmetic.
+
+// sin / cos on [-pi/4, pi/4]: Taylor series through x^9 / x^10, Horner in FP32
+void f_sincos_quarter(float x, float &s, float &c) {
+  const float x2 = x * x;
+  s = x * (1.0f + x2 * (-1.0f / 6.0f + x2 * (1.0f / 120.0f + x2 * (-1.0f / 5040.0f + x2 * (1.0f / 362880.0f)))));
+  c = 1.0f + x2 * (-0.5f + x2 * (1.0f / 24.0f + x2 * (-1.0f / 720.0f + x2 * (1.0f / 40320.0f +
+                                                                                 x2 * (-1.0f / 3628800.0f)))));
+}
+// distribution_map.rs:10-28 (second wedge: angle pi/2 - phi, cos / sin swap)
+void f_concentric_disk(float s0, float s1, float &x, float &y) {
+  const float PI_OVER_4 = 0.785398163397448309616f;
+  float ox = s0 * 2.0f - 1.0f, oy = s1 * 2.0f - 1.0f;
+  if (ox == 0.0f && oy == 0.0f) {
+    x = y = 0.0f;
+    return;
   }
-  double z = std::sqrt(std::max(0.0, 1.0 - x * x - y * y));
-  double sign = n.z < 0.0 ? -1.0 : 1.0;
-  double a = -1.0 / (sign + n.z);
-  double b = n.x * n.y * a;
-  D3 t = dnormalize(d3(1.0 + sign * n.x * n.x * a, sign * b, -sign * n.x));
-  D3 bt = dnormalize(d3(b, sign + n.y * n.y * a, -n.y));
+  float sn, cs;
+  if (std::fabs(ox) > std::fabs(oy)) {
+    f_sincos_quarter(PI_OVER_4 * (oy / ox), sn, cs);
+    x = cs * ox;
+    y = sn * ox;
+  } else {
+    f_sincos_quarter(PI_OVER_4 * (ox / oy), sn, cs);
+    x = sn * oy;
+    y = cs * oy;
+  }
+}
+// sampling.rs:72-86 + diffuse.rs:24-36
+V3 f_sample_cosine(V3 n, float s0, float s1) {
+  float x, y;
+  f_concentric_disk(s0, s1, x, y);
+  const float z = std::sqrt(std::fmax(0.0f, 1.0f - x * x - y * y));
+  const float sign = n.z < 0.0f ? -1.0f : 1.0f;
+  const float a = -1.0f / (sign + n.z);
+  const float b = n.x * n.y * a;
+  const V3 t = normalize(v3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x));
+  const V3 bt = normalize(v3(b, sign + n.y * n.y * a, -n.y));
   return t * x + bt * y + n * z;
+}
+// camera.rs:66-99: uv -> NDC at depth 1 through the inverse view-projection
+V3 f_unproject(const float *m, float u, float v) {
+  float x = u * 2.0f - 1.0f, y = (v * 2.0f - 1.0f) * -1.0f, z = 1.0f, w = 1.0f;
+  float rx = x * m[0] + y * m[4] + z * m[8] + w * m[12];
+  float ry = x * m[1] + y * m[5] + z * m[9] + w * m[13];
+  float rz = x * m[2] + y * m[6] + z * m[10] + w * m[14];
+  float rw = x * m[3] + y * m[7] + z * m[11] + w * m[15];
+  return v3(rx / rw, ry / rw, rz / rw);
+}
+V3 f_normal_to_world(const M4 &m, V3 n) {
+  return v3(m.a1 * n.x + m.a2 * n.y + m.a3 * n.z, m.b1 * n.x + m.b2 * n.y + m.b3 * n.z,
+            m.c1 * n.x + m.c2 * n.y + m.c3 * n.z);
 }
 
 // world normal = transpose(inverse 3x3) * n, column-major inverse
@@ -1207,30 +1243,26 @@ D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *
   const uint32_t seed = P.seed + view;
   const float *vp = P.view_params ? P.view_params + 20 * (size_t)view : P.view_projection_inv;
   const float *campos = P.view_params ? P.view_params + 20 * (size_t)view + 16 : P.camera_position;
-  double jx = orc_van_der_corput(n, scramble_of(px, py, seed, 0));
-  double jy = orc_sobol(n, scramble_of(px, py, seed, 1));
-  double u = (double)px / P.width + jx / P.width;
-  double v = (double)py / P.height + jy / P.height;
-  const float *m = vp;
-  double nx = u * 2.0 - 1.0, ny = (v * 2.0 - 1.0) * -1.0;
-  double rx = nx * m[0] + ny * m[4] + m[8] + m[12];
-  double ry = nx * m[1] + ny * m[5] + m[9] + m[13];
-  double rz = nx * m[2] + ny * m[6] + m[10] + m[14];
-  double rw = nx * m[3] + ny * m[7] + m[11] + m[15];
-  D3 cam = d3(campos[0], campos[1], campos[2]);
-  D3 dir = dnormalize(d3(rx / rw, ry / rw, rz / rw) - cam);
+  // ray generation (ray_gen.rs:89-107, camera.rs:66-99), FP32
+  const float jx = orc_van_der_corput(n, scramble_of(px, py, seed, 0));
+  const float jy = orc_sobol(n, scramble_of(px, py, seed, 1));
+  const float u = (float)px / (float)P.width + jx / (float)P.width;
+  const float v = (float)py / (float)P.height + jy / (float)P.height;
   V3 o32 = v3(campos[0], campos[1], campos[2]);
+  V3 dir = normalize(f_unproject(vp, u, v) - o32);
   D3 beta = d3(1, 1, 1), L = d3(0, 0, 0);
+  V3 beta_f = v3(1.0f, 1.0f, 1.0f);  // the FP32 throughput the roulette decides on
   const D3 sky_up = d3(P.sky_up[0], P.sky_up[1], P.sky_up[2]);
   const D3 sky_down = d3(P.sky_down[0], P.sky_down[1], P.sky_down[2]);
   const D3 wi = d3(-P.sun_direction[0], -P.sun_direction[1], -P.sun_direction[2]);
+  const V3 wif = v3(-P.sun_direction[0], -P.sun_direction[1], -P.sun_direction[2]);
   const D3 sun = d3(P.sun_radiance[0], P.sun_radiance[1], P.sun_radiance[2]);
   const bool has_sun = P.sun_radiance[0] > 0.0f || P.sun_radiance[1] > 0.0f || P.sun_radiance[2] > 0.0f;
   for (uint32_t b = 0; b <= P.max_bounces; b++) {
     ext_cnt[b]++;
     RayIn ray;
     ray.o = o32;
-    ray.d = v3((float)dir.x, (float)dir.y, (float)dir.z);
+    ray.d = dir;
     ray.tmin = FLT_EPSILON;
     ray.tmax = FLT_MAX;
     ray.flags = F_CULL_BACK;  // ray_gen.rs:162-166
@@ -1250,22 +1282,24 @@ D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *
     const uint32_t tri = geo.primitive_start + h.prim_id;
     const uint32_t i0 = s->indices[3 * (size_t)tri], i1 = s->indices[3 * (size_t)tri + 1],
                    i2 = s->indices[3 * (size_t)tri + 2];
-    const D3 p0 = d3f(s->vertices[i0]), p1 = d3f(s->vertices[i1]), p2 = d3f(s->vertices[i2]);
     const M4 &inv = s->inst_transform_inv[ti];
-    D3 ng = dnormalize(normal_to_world(inv, dcross(p1 - p0, p2 - p0)));
-    D3 ns = ng;
-    const double bu = h.u, bv = h.v, bw = 1.0 - bu - bv;
-    D3 nobj = d3f(s->normals[i0]) * bw + d3f(s->normals[i1]) * bu + d3f(s->normals[i2]) * bv;
-    const bool interp = ddot(nobj, nobj) > 0.0;
-    if (interp) ns = dnormalize(normal_to_world(inv, nobj));
+    // FP32 frame: geometric normal facing the ray, shading normal, offset origin
     const VisFrame vf = vis_frame(s, tri, ti, ray.o, ray.d, h.t);
+    const float bwf = 1.0f - h.u - h.v;
+    const V3 nobj_f = s->normals[i0] * bwf + s->normals[i1] * h.u + s->normals[i2] * h.v;
+    const bool interp = dot(nobj_f, nobj_f) > 0.0f;
+    V3 ns_f = interp ? normalize(f_normal_to_world(inv, nobj_f)) : (vf.flipped ? vf.ng * -1.0f : vf.ng);
     double sigma = 1.0;
-    if (vf.flipped) ng = ng * -1.0;
-    if (!interp && vf.flipped) sigma = -1.0;  // ns is ng before the flip
-    if (ddot(ns, ng) < 0.0) {
-      ns = ns * -1.0;
-      if (interp) sigma = -1.0;
+    if (dot(ns_f, vf.ng) < 0.0f) {
+      ns_f = ns_f * -1.0f;
+      sigma = -1.0;
     }
+    // FP64 shading normal with the same orientation
+    const D3 p0 = d3f(s->vertices[i0]), p1 = d3f(s->vertices[i1]), p2 = d3f(s->vertices[i2]);
+    const double bu = h.u, bv = h.v, bw = 1.0 - bu - bv;
+    const D3 nobj = interp ? d3f(s->normals[i0]) * bw + d3f(s->normals[i1]) * bu + d3f(s->normals[i2]) * bv
+                           : dcross(p1 - p0, p2 - p0);
+    const D3 ns = dnormalize(normal_to_world(inv, nobj)) * sigma;
     uint32_t mi = s->inst_custom[ti] + s->tri_material[tri];
     const size_t nm = s->materials.size() / 8;
     if (mi >= nm) mi = (uint32_t)nm - 1;
@@ -1284,13 +1318,12 @@ D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *
     }
     if (b == P.max_bounces) break;
     const V3 origin = vf.origin;
-    const double cos_l = ddot(ns, wi);
-    const V3 wif = v3(-P.sun_direction[0], -P.sun_direction[1], -P.sun_direction[2]);
-    if (cos_l > 0.0 && dot(vf.ng, wif) > 0.0f && has_sun) {
+    const float cos_lf = dot(ns_f, wif);
+    if (cos_lf > 0.0f && dot(vf.ng, wif) > 0.0f && has_sun) {
       sh_cnt[b]++;
       RayIn sr;
       sr.o = origin;
-      sr.d = v3(-P.sun_direction[0], -P.sun_direction[1], -P.sun_direction[2]);
+      sr.d = wif;
       sr.tmin = FLT_EPSILON;
       sr.tmax = FLT_MAX;
       sr.flags = F_ACCEPT_FIRST_HIT_AND_END_SEARCH;  // ray_hit.rs:33-45
@@ -1300,25 +1333,28 @@ D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *
       orc_hit_meta shm;
       Counters sc;
       if (!traverse_one(s, sr, sh, shm, sc)) {
+        const double cos_l = std::max(ddot(ns, wi), 0.0);
         L = L + beta * albedo * (cos_l / 3.14159265358979323846) * sun;
         if (rec) rec->back().w = cos_l / 3.14159265358979323846;
       }
     }
-    const double s0 = orc_van_der_corput(n, scramble_of(px, py, seed, 2 + 2 * b));
-    const double s1 = orc_sobol(n, scramble_of(px, py, seed, 3 + 2 * b));
-    const D3 wo = sample_cosine(ns, s0, s1);
-    if (!(ddot(wo, ng) > 0.0)) break;
+    const float s0 = orc_van_der_corput(n, scramble_of(px, py, seed, 2 + 2 * b));
+    const float s1 = orc_sobol(n, scramble_of(px, py, seed, 3 + 2 * b));
+    const V3 wo = f_sample_cosine(ns_f, s0, s1);
+    if (!(dot(wo, vf.ng) > 0.0f)) break;
     if (rec) {
       rec->back().cont = true;
-      rec->back().wo = wo;
-      rec->back().cos_wo = ddot(ns, wo);
+      rec->back().wo = d3f(wo);
+      rec->back().cos_wo = ddot(ns, d3f(wo));
     }
     beta = beta * albedo;
+    beta_f = beta_f * v3(mat[0], mat[1], mat[2]);
     if (b >= P.rr_start) {
-      const double q = std::min(std::max(std::max(beta.x, beta.y), beta.z), 0.95);
-      const double ur = orc_van_der_corput(n, scramble_of(px, py, seed, 64 + b));
+      const float q = std::fmin(std::fmax(std::fmax(beta_f.x, beta_f.y), beta_f.z), 0.95f);
+      const float ur = orc_van_der_corput(n, scramble_of(px, py, seed, 64 + b));
       if (ur >= q) break;
-      beta = beta * (1.0 / q);
+      beta = beta * (1.0 / (double)q);
+      beta_f = v3(beta_f.x / q, beta_f.y / q, beta_f.z / q);
     }
     o32 = origin;
     dir = wo;
@@ -1673,15 +1709,10 @@ int orc_batch_round(const orc_scene *s, const orc_ray *rays, const orc_ray_meta 
     if (hit) {
       const auto &geo = s->geo_meta[ext[0]];
       const uint32_t tri = geo.primitive_start + h.prim_id;
-      const D3 p0 = d3f(s->vertices[s->indices[3 * (size_t)tri]]);
-      const D3 p1 = d3f(s->vertices[s->indices[3 * (size_t)tri + 1]]);
-      const D3 p2 = d3f(s->vertices[s->indices[3 * (size_t)tri + 2]]);
-      D3 ng = dnormalize(normal_to_world(s->inst_transform_inv[ext[1]], dcross(p1 - p0, p2 - p0)));
       const VisFrame vf = vis_frame(s, tri, ext[1], ri.o, ri.d, h.t);
-      if (vf.flipped) ng = ng * -1.0;
-      const double s0 = orc_van_der_corput(round, xxhash32((uint32_t)i, seed, 2u));
-      const double s1 = orc_sobol(round, xxhash32((uint32_t)i, seed, 3u));
-      const D3 wo = sample_cosine(ng, s0, s1);
+      const float s0 = orc_van_der_corput(round, xxhash32((uint32_t)i, seed, 2u));
+      const float s1 = orc_sobol(round, xxhash32((uint32_t)i, seed, 3u));
+      const V3 wo = f_sample_cosine(vf.ng, s0, s1);
       out.ox = vf.origin.x;
       out.oy = vf.origin.y;
       out.oz = vf.origin.z;

@@ -5,7 +5,8 @@ live rays, spawns a continuation ray at every hit and compacts the survivors.
 
 * Every round's hits equal the oracle's on the same input rays (round 0: the
   primary rays; round r: the rays the GPU spawned), except near-ties.
-* Spawned rays agree with the oracle's FP64 spawn within 1e-5.
+* Spawned rays equal the oracle's bit for bit (both compute the spawn in
+  FP32 with the same operations).
 * Rays that ended carry miss records in every later round, and the per-round
   queue sizes equal the number of hits of the previous round.
 * The persistent graph-captured mode and the per-scene sequential mode give
@@ -25,7 +26,6 @@ pytestmark = pytest.mark.gpu
 
 BOUNCES = 3
 SEED = 11
-SPAWN_TOL = 1e-5
 
 
 def _say(msg):
@@ -83,10 +83,8 @@ def test_rounds_match_oracle(batch_case, device):
             break
         nxt = live & (h[r][:, 0] != MISS)
         same = nxt & (h[r][:, 0] == oh[:, 0])
-        g, o = sp[r][same].astype(np.float64), osp[same].astype(np.float64)
-        assert np.all(np.abs(g[:, 0:3] - o[:, 0:3]) <= SPAWN_TOL * (1.0 + np.abs(o[:, 0:3]))), r
-        assert np.all(np.abs(g[:, 4:7] - o[:, 4:7]) <= SPAWN_TOL), r
-        assert np.array_equal(sp[r][same][:, [3, 7]], osp[same][:, [3, 7]]), r
+        # the oracle computes the spawn in FP32 with the device's operations
+        assert np.array_equal(sp[r][same], osp[same]), r
         live = nxt
     assert st.rays[1] > 0 and st.rays[2] > 0, st.rays
 
