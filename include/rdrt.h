@@ -204,6 +204,24 @@ int32_t rdrt_trace_closest(rdrt_ctx *ctx, const rdrt_ray *rays_dev, uint32_t n, 
  * rdrt_sync first. */
 int32_t rdrt_last_trace_ms(rdrt_ctx *ctx, float *ms);
 
+/* ---- launch tuning of the fast traversal (no counterpart in the reference:
+ *      its workgroup size is fixed at 128, traverse_gpu.rs) ---- */
+typedef struct rdrt_tune {
+  uint32_t block_size;  /* threads per block: 64, 128, 256 */
+  uint32_t lds_entries; /* traversal stack entries per lane in LDS (8, 16); deeper ones in private memory */
+  uint32_t waves;       /* 8 = compile for 8 waves per SIMD (at most 64 VGPRs); 0 = the compiler's choice */
+  uint32_t persistent;  /* 1 = resident-size grid whose waves pull 64-ray chunks from a counter */
+  float ms;             /* rdrt_autotune: per-launch time of the chosen shape on the sample */
+} rdrt_tune;
+/* Times every shape of the search space (3 block sizes x 2 LDS stack depths
+ * x 2 occupancy targets x grid / persistent) on a sample launch (the args of
+ * rdrt_trace; its hits are overwritten), `reps` timed launches each, and
+ * keeps the fastest for every later traversal of this context (rdrt_trace,
+ * the path tracer, batched tracing).  Synchronizes.  best may be NULL. */
+int32_t rdrt_autotune(rdrt_ctx *ctx, const rdrt_trace_args *sample, uint32_t reps, rdrt_tune *best);
+int32_t rdrt_set_tune(rdrt_ctx *ctx, const rdrt_tune *tune);
+int32_t rdrt_get_tune(rdrt_ctx *ctx, rdrt_tune *tune);
+
 /* ---- shading attributes (bindless mesh data of
  *      scene/rendering/gpu-ray-tracing/src/bindless_mesh_bridge.rs:11-135) ---- */
 /* Lambertian material (content/lighting/transport/src/surface/microfacet/

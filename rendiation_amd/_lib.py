@@ -93,6 +93,10 @@ class RdrtBatchStats(C.Structure):
     _fields_ = [("rounds", u32), ("rays", u32 * 17), ("launches", u32), ("ms", f32)]
 
 
+class RdrtTune(C.Structure):
+    _fields_ = [("block_size", u32), ("lds_entries", u32), ("waves", u32), ("persistent", u32), ("ms", f32)]
+
+
 RDRT_BATCH_PERSISTENT = 0
 RDRT_BATCH_SEQUENTIAL = 1
 
@@ -131,6 +135,9 @@ SIGNATURES = {
     "rdrt_pt_backward_geometry": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "rdrt_geometry_vertex_range": (i32, [C.c_void_p, u32, u32, C.POINTER(u32), C.POINTER(u32)]),
     "rdrt_trace_batched": (i32, [C.c_void_p, C.POINTER(RdrtBatchParams)]),
+    "rdrt_autotune": (i32, [C.c_void_p, C.POINTER(RdrtTraceArgs), u32, C.POINTER(RdrtTune)]),
+    "rdrt_set_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),
+    "rdrt_get_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),
     "rdrt_batch_get_stats": (i32, [C.c_void_p, C.POINTER(RdrtBatchStats)]),
     "rdrt_compact": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p]),
     "rdrt_sample_sobol": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, u32, C.c_void_p]),

@@ -334,6 +334,30 @@ class AccelerationStructureSystem:
         L.check(self._L.rdrt_geometry_vertex_range(self._h, blas.index, geometry, C.byref(first), C.byref(count)))
         return int(first.value), int(count.value)
 
+    def autotune(self, rays, hits, *, ray_meta=None, flags: int = 0, cull_mask: int = 0xFFFFFFFF, tlas_idx: int = 0,
+                 reps: int = 3) -> dict:
+        """Time every traversal launch shape on a sample launch and keep the
+        fastest for this system; returns the choice."""
+        n = rays.numel() // 8
+        _check_dev(rays, n * 8, "rays")
+        _check_dev(hits, n * 4, "hits")
+        a = L.RdrtTraceArgs()
+        a.rays, a.ray_meta, a.hits = _ptr(rays), _ptr(ray_meta), _ptr(hits)
+        a.n, a.flags, a.cull_mask, a.tlas_idx, a.traversal = n, flags, cull_mask & 0xFFFFFFFF, tlas_idx, TRAVERSAL_FAST
+        t = L.RdrtTune()
+        L.check(self._L.rdrt_autotune(self._h, C.byref(a), reps, C.byref(t)))
+        return self.tune()
+
+    def tune(self) -> dict:
+        t = L.RdrtTune()
+        L.check(self._L.rdrt_get_tune(self._h, C.byref(t)))
+        return {"block_size": t.block_size, "lds_entries": t.lds_entries, "waves": t.waves,
+                "persistent": t.persistent, "ms": float(t.ms)}
+
+    def set_tune(self, block_size: int = 256, lds_entries: int = 16, waves: int = 0, persistent: int = 0) -> None:
+        t = L.RdrtTune(block_size, lds_entries, waves, persistent, 0.0)
+        L.check(self._L.rdrt_set_tune(self._h, C.byref(t)))
+
     def sync(self) -> None:
         L.check(self._L.rdrt_sync(self._h))
 

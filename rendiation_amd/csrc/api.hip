@@ -104,8 +104,11 @@ int32_t rdrt_config_default(rdrt_config *out) {
 int32_t rdrt_ctx_create(const rdrt_config *cfg, rdrt_ctx **out) {
   if (!cfg || !out) return fail(RDRT_ERR_INVALID_ARG, "null argument");
   if (cfg->traversal > RDRT_TRAVERSAL_REFERENCE) return fail(RDRT_ERR_INVALID_ARG, "unknown traversal");
+  if (cfg->block_size != 0 && cfg->block_size != 64 && cfg->block_size != 128 && cfg->block_size != 256)
+    return fail(RDRT_ERR_INVALID_ARG, "block_size must be 0, 64, 128 or 256");
   rdrt_ctx *c = new rdrt_ctx();
   c->cfg = *cfg;
+  c->tune.block = cfg->block_size ? cfg->block_size : 256;
   const rdrt_build_opts &b = cfg->build;
   c->src.blas_opt = tree_opt(b.blas_bin_size, b.blas_max_depth, b.sah_bins, 2);
   c->src.tlas_opt = tree_opt(b.tlas_bin_size, b.tlas_max_depth, b.sah_bins, 10);
@@ -154,6 +157,7 @@ int32_t rdrt_ctx_destroy(rdrt_ctx *c) {
     c->pt.free_all();
     c->batch.free_all();
     c->compact_scratch.free();
+    c->trace_work.free();
     if (c->owns_work_stream) (void)hipStreamDestroy(c->work_stream);
     if (c->status) (void)hipFree(c->status);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -375,11 +379,88 @@ int32_t rdrt_trace(rdrt_ctx *c, const rdrt_trace_args *a) {
   uint32_t push = push_depth(c);
   if (a->traversal == RDRT_TRAVERSAL_FAST && stack_entries_for(push) == 0)
     return fail(RDRT_ERR_CAPACITY, "BVH too deep for the fast traversal stack (use RDRT_TRAVERSAL_REFERENCE)");
-  uint32_t block = c->cfg.block_size ? c->cfg.block_size : 256;
+  if (c->tune.persist) {
+    RDRT_HIP(c->trace_work.reserve(4));
+    p.work = (uint32_t *)c->trace_work.ptr;
+  }
   RDRT_HIP(hipEventRecord(c->ev0, c->stream));
-  RDRT_HIP(launch_trace(p, a->traversal, push, block, c->stream));
+  RDRT_HIP(launch_trace(p, a->traversal, push, c->tune, c->stream));
   RDRT_HIP(hipEventRecord(c->ev1, c->stream));
   c->timed = true;
+  return RDRT_OK;
+}
+
+int32_t rdrt_set_tune(rdrt_ctx *c, const rdrt_tune *t) {
+  if (!c || !t) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  if ((t->block_size != 64 && t->block_size != 128 && t->block_size != 256) ||
+      (t->lds_entries != 8 && t->lds_entries != 16) || (t->waves != 0 && t->waves != 8) || t->persistent > 1)
+    return fail(RDRT_ERR_INVALID_ARG, "block_size 64/128/256, lds_entries 8/16, waves 0/8, persistent 0/1");
+  c->tune.block = t->block_size;
+  c->tune.lds = t->lds_entries;
+  c->tune.waves = t->waves;
+  c->tune.persist = t->persistent;
+  return RDRT_OK;
+}
+
+int32_t rdrt_get_tune(rdrt_ctx *c, rdrt_tune *t) {
+  if (!c || !t) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  std::memset(t, 0, sizeof(*t));
+  t->block_size = c->tune.block;
+  t->lds_entries = c->tune.lds;
+  t->waves = c->tune.waves;
+  t->persistent = c->tune.persist;
+  t->ms = c->tune_ms;
+  return RDRT_OK;
+}
+
+int32_t rdrt_autotune(rdrt_ctx *c, const rdrt_trace_args *a, uint32_t reps, rdrt_tune *best) {
+  if (!c || !a) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  if (!c->uploaded) return fail(RDRT_ERR_NOT_BUILT, "scene is not committed (call rdrt_commit)");
+  if (a->n == 0 || !a->rays || !a->hits) return fail(RDRT_ERR_INVALID_ARG, "the sample needs rays and hits");
+  const uint32_t push = push_depth(c);
+  if (stack_entries_for(push) == 0) return fail(RDRT_ERR_CAPACITY, "BVH too deep for the fast traversal stack");
+  if (reps == 0) reps = 3;
+  TraceParams p = scene_trace_params(c);
+  p.rays = a->rays;
+  p.meta = a->ray_meta;
+  p.hits = a->hits;
+  p.n = a->n;
+  p.flags = a->flags;
+  p.cull_mask = a->cull_mask;
+  p.tlas_idx = a->tlas_idx;
+  RDRT_HIP(c->trace_work.reserve(4));
+  p.work = (uint32_t *)c->trace_work.ptr;
+  hipStream_t st = c->stream;
+  TraceTune chosen = c->tune;
+  float chosen_ms = -1.0f;
+  for (uint32_t block : {64u, 128u, 256u})
+    for (uint32_t lds : {8u, 16u})
+      for (uint32_t waves : {0u, 8u})
+        for (uint32_t persist : {0u, 1u}) {
+          TraceTune t;
+          t.block = block;
+          t.lds = lds;
+          t.waves = waves;
+          t.persist = persist;
+          RDRT_HIP(launch_trace(p, RDRT_TRAVERSAL_FAST, push, t, st));  // warm-up
+          RDRT_HIP(hipEventRecord(c->ev0, st));
+          for (uint32_t r = 0; r < reps; r++) RDRT_HIP(launch_trace(p, RDRT_TRAVERSAL_FAST, push, t, st));
+          RDRT_HIP(hipEventRecord(c->ev1, st));
+          RDRT_HIP(hipEventSynchronize(c->ev1));
+          float ms = 0.0f;
+          RDRT_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+          ms /= (float)reps;
+          if (chosen_ms < 0.0f || ms < chosen_ms) {
+            chosen_ms = ms;
+            chosen = t;
+          }
+        }
+  c->tune = chosen;
+  c->tune_ms = chosen_ms;
+  c->timed = false;
+  if (best) return rdrt_get_tune(c, best);
   return RDRT_OK;
 }
 

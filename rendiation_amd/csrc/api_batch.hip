@@ -76,14 +76,15 @@ hipError_t enqueue_scene(rdrt_ctx *c, const rdrt_batch_params *a, const Plan &pl
   base.slots = nk;
   base.hitx = (uint2 *)s.hitx.ptr;
   base.work = (uint32_t *)s.work.ptr;
-  const uint32_t block = c->cfg.block_size ? c->cfg.block_size : 256;
+  TraceTune tune = c->tune;
+  tune.persist = persist ? 1u : 0u;
   for (uint32_t r = 0; r < R; r++) {
     TraceParams tp = base;
     tp.rays = r == 0 ? a->rays : (const rdrt_ray *)s.wrays.ptr;
     tp.hits = a->hits + (size_t)r * a->n;
     tp.queue = queues[r & 1];
     tp.count = counts + r;
-    hipError_t e = launch_trace(tp, RDRT_TRAVERSAL_FAST, push_depth(c), block, c->work_stream, persist);
+    hipError_t e = launch_trace(tp, RDRT_TRAVERSAL_FAST, push_depth(c), tune, c->work_stream);
     if (e != hipSuccess) return e;
     launches += persist ? 2 : 1;
     if (r + 1 == R) break;

@@ -14,7 +14,7 @@ namespace rdrt {
 
 void PtState::free_all() {
   for (Scratch *s : {&rays, &shadow_rays, &hits, &shadow_hits, &hitx, &beta, &L, &nee, &qa, &qb, &qs, &counts,
-                     &lookback, &tickets, &materials, &diag_counters, &diag_totals, &record, &views})
+                     &lookback, &tickets, &materials, &diag_counters, &diag_totals, &record, &views, &work})
     s->free();
   host_views.clear();
   materials_dirty = true;
@@ -220,6 +220,8 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
   L.tickets = (uint32_t *)s.tickets.ptr;
   L.lookback_stride = (npaths + kPtBlock - 1) / kPtBlock;
   L.status = c->status;
+  RDRT_HIP(s.work.reserve(4));
+  L.work = (uint32_t *)s.work.ptr;
   L.out = radiance;
   if (prm->record) {
     RDRT_HIP(s.record.reserve((size_t)(prm->max_bounces + 2) * npaths * sizeof(uint2)));
@@ -227,7 +229,6 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
   }
 
   const TraceParams tp = scene_trace_params(c);
-  const uint32_t block = c->cfg.block_size ? c->cfg.block_size : 256;
   hipStream_t st = c->work_stream;
   if (prm->diagnostics & ~(RDRT_PT_DIAG_TIME_TRACES | RDRT_PT_DIAG_COUNT_VISITS))
     return fail(RDRT_ERR_INVALID_ARG, "unknown diagnostics bits");
@@ -256,7 +257,7 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
   s.last_launches = 2 * prm->max_bounces + 1;
   RDRT_HIP(hipEventRecord(c->ev0, st));
   if (prm->diagnostics) {
-    RDRT_HIP(enqueue_pt_frame(L, tp, push_depth(c), block, st, &diag));
+    RDRT_HIP(enqueue_pt_frame(L, tp, push_depth(c), c->tune, st, &diag));
   } else if (prm->use_graph) {
     if (!s.exec || !s.have_cached || std::memcmp(&s.cached, &L, sizeof(L)) != 0) {
       if (s.exec) (void)hipGraphExecDestroy(s.exec);
@@ -264,7 +265,7 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
       s.exec = nullptr;
       s.graph = nullptr;
       RDRT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      hipError_t e = enqueue_pt_frame(L, tp, push_depth(c), block, st);
+      hipError_t e = enqueue_pt_frame(L, tp, push_depth(c), c->tune, st);
       hipGraph_t g = nullptr;
       hipError_t e2 = hipStreamEndCapture(st, &g);
       if (e != hipSuccess || e2 != hipSuccess) {
@@ -278,7 +279,7 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
     }
     RDRT_HIP(hipGraphLaunch(s.exec, st));
   } else {
-    RDRT_HIP(enqueue_pt_frame(L, tp, push_depth(c), block, st));
+    RDRT_HIP(enqueue_pt_frame(L, tp, push_depth(c), c->tune, st));
   }
   RDRT_HIP(hipEventRecord(c->ev1, st));
   s.last_bounces = prm->max_bounces;
@@ -339,8 +340,7 @@ int32_t rdrt_pt_backward_geometry(rdrt_ctx *c, const float *adj, float *grad_pos
   L.g_nrm = grad_nrm;  // zeroed by the replay; stays zero without vertex normals
   L.nvertices = (uint32_t)(c->dev.vertices.count / 3);
   const TraceParams tp = scene_trace_params(c);
-  const uint32_t block = c->cfg.block_size ? c->cfg.block_size : 256;
-  RDRT_HIP(enqueue_pt_replay(L, tp, push_depth(c), block, c->work_stream));
+  RDRT_HIP(enqueue_pt_replay(L, tp, push_depth(c), c->tune, c->work_stream));
   return RDRT_OK;
 }
 

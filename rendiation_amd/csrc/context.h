@@ -83,7 +83,7 @@ struct PtState {
   uint32_t capacity = 0;  // paths
   uint32_t pixels = 0;
   Scratch rays, shadow_rays, hits, shadow_hits, hitx, beta, L, nee, qa, qb, qs, counts, lookback, tickets;
-  Scratch materials, record, views;
+  Scratch materials, record, views, work;
   std::vector<float> host_views;  // contents of `views`
   uint32_t nmaterials_uploaded = 0;
   bool materials_dirty = true;
@@ -138,6 +138,9 @@ struct rdrt_ctx {
   rdrt::PtState pt;
   rdrt::BatchState batch;
   rdrt::Scratch compact_scratch;
+  rdrt::TraceTune tune;  // launch shape of the fast traversal (config block size, rdrt_autotune)
+  rdrt::Scratch trace_work;  // work counter of persistent rdrt_trace launches
+  float tune_ms = 0.0f;      // per-launch time of the autotuned choice
 };
 
 namespace rdrt {

@@ -44,6 +44,7 @@ struct PtLaunch {
   uint32_t *tickets;             // per bounce: block ticket counter
   uint32_t lookback_stride;
   uint32_t *status;
+  uint32_t *work;  // work counter of persistent traversal launches
   float *out;  // width * height * 3
 
   // path vertices for the adjoint, or null: slot k of path p at
@@ -69,8 +70,8 @@ struct PtDiag {
 
 // Enqueues one frame on `stream` (capturable when diag is null: no memset
 // nodes, no host round trips).
-hipError_t enqueue_pt_frame(const PtLaunch &L, const TraceParams &scene_tp, uint32_t push_depth, uint32_t block,
-                            hipStream_t stream, const PtDiag *diag = nullptr);
+hipError_t enqueue_pt_frame(const PtLaunch &L, const TraceParams &scene_tp, uint32_t push_depth,
+                            const TraceTune &tune, hipStream_t stream, const PtDiag *diag = nullptr);
 
 // Gradient of sum(adj * frame) w.r.t. material albedo and emission from the
 // recorded vertices of frame L (grads zeroed here, then accumulated).
@@ -79,8 +80,8 @@ hipError_t enqueue_pt_backward(const PtLaunch &L, const float *adj, float *grad_
 // Replays the recorded frame L (same kernels, no shadow rays) and
 // accumulates the gradient w.r.t. vertex positions and normals into
 // L.g_pos / L.g_nrm (zeroed here).
-hipError_t enqueue_pt_replay(const PtLaunch &L, const TraceParams &scene_tp, uint32_t push_depth, uint32_t block,
-                             hipStream_t stream);
+hipError_t enqueue_pt_replay(const PtLaunch &L, const TraceParams &scene_tp, uint32_t push_depth,
+                             const TraceTune &tune, hipStream_t stream);
 // Packs device albedo / emission arrays (n * 3 each) into Material records.
 hipError_t enqueue_pack_materials(const float *albedo, const float *emission, uint32_t n, Material *out,
                                   hipStream_t stream);

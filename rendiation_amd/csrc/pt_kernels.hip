@@ -496,8 +496,8 @@ __global__ void k_sobol(const uint32_t *index, const uint32_t *scr, uint32_t n, 
 
 }  // namespace
 
-hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_t push_depth, uint32_t block,
-                            hipStream_t st, const PtDiag *diag) {
+hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_t push_depth,
+                            const TraceTune &tune, hipStream_t st, const PtDiag *diag) {
   const uint32_t grid = (P.npaths + kPtBlock - 1) / kPtBlock;
   const bool count = diag && diag->counters;
   const uint32_t mode = count ? RDRT_TRAVERSAL_REFERENCE : RDRT_TRAVERSAL_FAST;
@@ -506,7 +506,7 @@ hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_
   auto trace = [&](TraceParams t) -> hipError_t {
     if (count) t.counters = diag->counters;
     if (diag && diag->ev_begin) (void)hipEventRecord(diag->ev_begin[launch], st);
-    hipError_t e = launch_trace(t, mode, push_depth, block, st);
+    hipError_t e = launch_trace(t, mode, push_depth, tune, st);
     if (diag && diag->ev_end) (void)hipEventRecord(diag->ev_end[launch], st);
     launch++;
     if (e == hipSuccess && count)
@@ -531,6 +531,7 @@ hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_
     tp.flags = RDRT_RAY_FLAG_CULL_BACK_FACING_TRIANGLES;  // ray_gen.rs:162-166
     tp.cull_mask = 0xffffffffu;
     tp.tlas_idx = 0;
+    tp.work = P.work;
     hipError_t e = trace(tp);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_pt_shade<kPtBlock, false>), dim3(grid), dim3(kPtBlock), 0, st, P, b);
@@ -551,8 +552,8 @@ hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_
   return hipGetLastError();
 }
 
-hipError_t enqueue_pt_replay(const PtLaunch &P, const TraceParams &scene, uint32_t push_depth, uint32_t block,
-                             hipStream_t st) {
+hipError_t enqueue_pt_replay(const PtLaunch &P, const TraceParams &scene, uint32_t push_depth,
+                             const TraceTune &tune, hipStream_t st) {
   const uint32_t grid = (P.npaths + kPtBlock - 1) / kPtBlock;
   const uint32_t nv = P.nvertices;
   hipLaunchKernelGGL(k_fill_f32, dim3(256), dim3(256), 0, st, P.g_pos, 3 * nv);
@@ -573,7 +574,8 @@ hipError_t enqueue_pt_replay(const PtLaunch &P, const TraceParams &scene, uint32
     tp.flags = RDRT_RAY_FLAG_CULL_BACK_FACING_TRIANGLES;
     tp.cull_mask = 0xffffffffu;
     tp.tlas_idx = 0;
-    hipError_t e = launch_trace(tp, RDRT_TRAVERSAL_FAST, push_depth, block, st);
+    tp.work = P.work;
+    hipError_t e = launch_trace(tp, RDRT_TRAVERSAL_FAST, push_depth, tune, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_pt_shade<kPtBlock, true>), dim3(grid), dim3(kPtBlock), 0, st, P, b);
     if (b == P.max_bounces) break;

@@ -1,0 +1,447 @@
+// trace_device.h -- device code of the closest-hit traversal kernels for
+// gfx950 (wave64), shared by the translation units that instantiate them.
+//
+// Two kernels share one result definition, that of the reference CPU
+// traversal (shader/ray-tracing/src/backend/wavefront_compute/geometry/naive/
+// traverse_cpu.rs:52-245): the last accepted triangle wins, a triangle is
+// accepted when its t lies in [t_min, far] of the ray range in object space
+// (wavefront_compute/geometry/mod.rs:104-155), and far shrinks to each accepted
+// hit.
+//
+//  * k_trace_ref walks the reference's 48-byte forest by its stackless
+//    hit/miss links (traverse_cpu.rs:283-319) in the reference order, with the
+//    reference's per-test arithmetic; its hits and visit counters equal the
+//    oracle's bit for bit.
+//  * k_trace_fast walks the 64-byte child-pair layout front to back with a
+//    per-lane stack in LDS (column layout [entry][lane]: each lane owns one
+//    bank at every depth, so stack traffic is conflict-free) and tests
+//    triangles from 64-byte pre-pass records.  The box and triangle
+//    arithmetic is the reference's, so t, u, v are bit-identical; only the
+//    visit order differs, which can change the winner among hits with equal t.
+//
+// Compiled with -ffp-contract=off and IEEE division/sqrt: one rounding per
+// operation, as in the reference.
+#ifndef RDRT_TRACE_DEVICE_H
+#define RDRT_TRACE_DEVICE_H
+
+#include <algorithm>
+
+#include "trace_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace rdrt {
+namespace {
+
+constexpr uint32_t F_FORCE_OPAQUE = 0x01, F_FORCE_NON_OPAQUE = 0x02, F_END_SEARCH = 0x04, F_CULL_BACK = 0x10,
+                   F_CULL_FRONT = 0x20, F_CULL_OPAQUE = 0x40, F_CULL_NON_OPAQUE = 0x80,
+                   F_SKIP_TRIANGLES = 0x100, F_FLIP = 0x400;
+constexpr uint32_t GI_CULL_DISABLE = 0x1, GI_FLIP = 0x2, GI_FORCE_OPAQUE = 0x4, GI_FORCE_NO_OPAQUE = 0x8;
+
+struct F3 {
+  float x, y, z;
+};
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+__device__ __forceinline__ F3 sub(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ F3 cross(F3 a, F3 b) {
+  return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ F3 normalize(F3 a) {
+  float m = dot(a, a);
+  if (m > 0.0f) {
+    float inv = 1.0f / sqrtf(m);
+    return f3(a.x * inv, a.y * inv, a.z * inv);
+  }
+  return a;
+}
+__device__ __forceinline__ F3 recip(F3 a) { return f3(1.0f / a.x, 1.0f / a.y, 1.0f / a.z); }
+
+// Ray/AABB slab test (wavefront_compute/geometry/mod.rs:44-64).  inv_d is
+// 1/dir, hoisted out of the node loop; the per-node values are unchanged.
+__device__ __forceinline__ bool slab(F3 o, F3 inv_d, float tmin, float tmax, float mnx, float mny, float mnz,
+                                     float mxx, float mxy, float mxz, float &tnear) {
+  float t0x = (mnx - o.x) * inv_d.x, t0y = (mny - o.y) * inv_d.y, t0z = (mnz - o.z) * inv_d.z;
+  float t1x = (mxx - o.x) * inv_d.x, t1y = (mxy - o.y) * inv_d.y, t1z = (mxz - o.z) * inv_d.z;
+  float nx = fminf(t0x, t1x), ny = fminf(t0y, t1y), nz = fminf(t0z, t1z);
+  float fx = fmaxf(t0x, t1x), fy = fmaxf(t0y, t1y), fz = fmaxf(t0z, t1z);
+  float tn = fmaxf(fmaxf(nx, ny), nz);
+  float tf = fminf(fminf(fx, fy), fz);
+  tnear = tn;
+  return tn <= tf && tmin < tf && tn < tmax;
+}
+
+// Triangle test given the vertex-only quantities (geometry/mod.rs:118-154).
+// b == 0 is a miss.
+__device__ __forceinline__ bool tri_core(F3 o, F3 d, float rn, float rf, F3 v0, F3 e1, F3 e2, F3 n, float uu,
+                                         float uv, float vv, float inverse_d, bool cull_enable, bool cull_back,
+                                         float &t, float &u, float &v, bool &back) {
+  float b = dot(n, d);
+  if (b == 0.0f) return false;
+  if (cull_enable && cull_back == (b < 0.0f)) return false;
+  F3 w0 = sub(o, v0);
+  float a = -dot(n, w0);
+  float tt = a / b;
+  if (tt < rn || tt > rf) return false;
+  F3 p = f3(o.x + tt * d.x, o.y + tt * d.y, o.z + tt * d.z);
+  F3 w = sub(p, v0);
+  float wu = dot(w, e1);
+  float wv = dot(w, e2);
+  float uu_ = (uv * wv - vv * wu) * inverse_d;
+  if (uu_ < 0.0f || uu_ > 1.0f) return false;
+  float vv_ = (uv * wu - uu * wv) * inverse_d;
+  if (vv_ < 0.0f || (uu_ + vv_) > 1.0f) return false;
+  t = tt;
+  u = uu_;
+  v = vv_;
+  back = b < 0.0f;
+  return true;
+}
+
+// flag.rs:30-75
+__device__ __forceinline__ uint32_t merge_flags(uint32_t f, uint32_t gi) {
+  if (gi & GI_CULL_DISABLE) f &= ~(F_CULL_BACK | F_CULL_FRONT);
+  if (gi & GI_FLIP) f ^= F_FLIP;
+  if (gi & GI_FORCE_OPAQUE) f |= F_FORCE_OPAQUE;
+  if (gi & GI_FORCE_NO_OPAQUE) f |= F_FORCE_NON_OPAQUE;
+  return f;
+}
+// flag.rs:78-91 (pass only: non-opaque geometry takes the default any-hit, accept)
+__device__ __forceinline__ bool geometry_passes(uint32_t f, uint32_t gflags) {
+  bool opaque = ((gflags & RDRT_GEOMETRY_FLAG_OPAQUE) || (f & F_FORCE_OPAQUE)) && !(f & F_FORCE_NON_OPAQUE);
+  return (opaque && !(f & F_CULL_OPAQUE)) || (!opaque && !(f & F_CULL_NON_OPAQUE));
+}
+// flag.rs:94-105
+__device__ __forceinline__ void cull_triangle(uint32_t f, bool &enable, bool &back) {
+  bool flip = f & F_FLIP, cf = f & F_CULL_FRONT, cb = f & F_CULL_BACK;
+  enable = cf || cb;
+  back = (flip && cb) || (!flip && cf);
+}
+
+// Object-space ray of an instance (traverse_cpu.rs:99-104): inverse transform
+// of the origin with the w divide, 3x3 part on the direction, scaling = its
+// length, direction normalized.
+__device__ __forceinline__ void object_ray(const float *m, F3 o, F3 d, F3 &bo, F3 &bd, float &scaling) {
+  float x = o.x * m[0] + o.y * m[4] + o.z * m[8] + 1.0f * m[12];
+  float y = o.x * m[1] + o.y * m[5] + o.z * m[9] + 1.0f * m[13];
+  float z = o.x * m[2] + o.y * m[6] + o.z * m[10] + 1.0f * m[14];
+  float w = o.x * m[3] + o.y * m[7] + o.z * m[11] + 1.0f * m[15];
+  bo = f3(x / w, y / w, z / w);
+  F3 dd = f3(d.x * m[0] + d.y * m[4] + d.z * m[8], d.x * m[1] + d.y * m[5] + d.z * m[9],
+             d.x * m[2] + d.y * m[6] + d.z * m[10]);
+  scaling = sqrtf(dot(dd, dd));
+  bd = normalize(dd);
+}
+
+struct RayIn {
+  F3 o, d;
+  float tmin, tmax;
+  uint32_t flags, mask, tlas_idx;
+};
+
+__device__ __forceinline__ RayIn load_ray(const TraceParams &p, uint32_t rid) {
+  const float4 *rp = reinterpret_cast<const float4 *>(p.rays + rid);
+  float4 a = rp[0], b = rp[1];
+  RayIn r;
+  r.o = f3(a.x, a.y, a.z);
+  r.tmin = a.w;
+  r.d = f3(b.x, b.y, b.z);
+  r.tmax = b.w;
+  r.flags = p.flags;
+  r.mask = p.cull_mask;
+  r.tlas_idx = p.tlas_idx;
+  if (p.meta) {
+    uint4 m = *reinterpret_cast<const uint4 *>(p.meta + rid);
+    r.flags = m.x;
+    r.mask = m.y;
+    r.tlas_idx = m.z;
+  }
+  return r;
+}
+
+// Root of the TLAS bound at tlas_idx, or null when unbound / deleted.
+__device__ __forceinline__ const TlasRoot *bound_root(const TraceParams &p, uint32_t tlas_idx) {
+  if (tlas_idx >= p.nbinding) return nullptr;
+  uint32_t th = p.binding[tlas_idx];
+  if (th >= p.ntlas) return nullptr;
+  const TlasRoot *r = p.tlas_roots + th;
+  return r->ref_root == kInvalid ? nullptr : r;
+}
+
+struct Best {
+  uint32_t k = kInvalid;  // triangle slot (redirect index)
+  uint32_t geo = 0, inst = 0;
+  float t = 0.0f, u = 0.0f, v = 0.0f;
+  bool back = false;
+};
+
+__device__ __forceinline__ void write_result(const TraceParams &p, uint32_t rid, const Best &best, uint32_t cn,
+                                             uint32_t ct, uint32_t ci) {
+  uint4 h = make_uint4(RDRT_MISS, 0u, 0u, 0u);
+  uint4 hm = make_uint4(RDRT_MISS, RDRT_MISS, 0u, RDRT_MISS);
+  if (best.k != kInvalid) {
+    const GeoRec &g = p.geos[best.geo];
+    h.x = p.redirect[best.k] - g.primitive_start;
+    h.y = __float_as_uint(best.t);
+    h.z = __float_as_uint(best.u);
+    h.w = __float_as_uint(best.v);
+    if (p.hit_meta) {
+      const InstRec &in = p.insts[best.inst];
+      hm = make_uint4(g.geometry_idx, in.order, best.back ? RDRT_HIT_KIND_BACK_FACING_TRIANGLE
+                                                          : RDRT_HIT_KIND_FRONT_FACING_TRIANGLE,
+                      in.custom_index);
+    }
+  }
+  *reinterpret_cast<uint4 *>(p.hits + rid) = h;
+  if (p.hit_meta) *reinterpret_cast<uint4 *>(p.hit_meta + rid) = hm;
+  if (p.hitx) p.hitx[rid] = make_uint2(best.geo, best.inst);
+  if (p.counters) {
+    p.counters[3 * rid + 0] = cn;
+    p.counters[3 * rid + 1] = ct;
+    p.counters[3 * rid + 2] = ci;
+  }
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2).
+// Remap the first `used` physical blocks so each XCD works on one contiguous
+// range of rays: neighbouring (coherent) rays then share an L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t i, uint32_t used) {
+  const uint32_t xcd = i % 8u, local = i / 8u, per = used / 8u, rem = used % 8u;
+  return xcd < rem ? xcd * (per + 1u) + local : rem * (per + 1u) + (xcd - rem) * per + local;
+}
+
+// ------------------------------------------------------------------------
+// Fast kernel: front-to-back child-pair traversal.  Stack: the first K
+// entries of each lane live in LDS (column layout), deeper ones (rare) in
+// the lane's private memory; S bounds the total.
+// ------------------------------------------------------------------------
+template <int K, int S, int B, bool CNT>
+__device__ __forceinline__ void trace_fast_one(const TraceParams &p, uint32_t rid, uint32_t *stk) {
+  uint32_t spill[S > K ? S - K : 1];
+  auto push = [&](uint32_t &sp, uint32_t v) {
+    if (sp < (uint32_t)K) stk[sp * B] = v;
+    else if (sp < (uint32_t)S) spill[sp - K] = v;
+    else {
+      atomicOr(p.status, 1u);  // unreachable when S covers the built depth
+      return;
+    }
+    sp++;
+  };
+  auto pop = [&](uint32_t &sp) -> uint32_t {
+    sp--;
+    return sp < (uint32_t)K ? stk[sp * B] : spill[sp - K];
+  };
+
+  RayIn r = load_ray(p, rid);
+  float far_ = r.tmax;
+  Best best;
+  uint32_t cn = 0, ct = 0, ci = 0;
+  bool done = false;
+
+  const TlasRoot *root = bound_root(p, r.tlas_idx);
+  const F3 winv = recip(r.d);
+  float tn;
+  if (root && slab(r.o, winv, r.tmin, far_, root->mn[0], root->mn[1], root->mn[2], root->mx[0], root->mx[1],
+                   root->mx[2], tn)) {
+    uint32_t sp = 0;
+    uint32_t ref = root->root_ref;
+    for (;;) {
+      if (!(ref & kLeafBit)) {
+        const float4 *q = reinterpret_cast<const float4 *>(p.tlas_pairs + ref);
+        float4 a = q[0], b = q[1], c = q[2], d = q[3];
+        float tl, tr;
+        bool hl = slab(r.o, winv, r.tmin, far_, a.x, a.y, a.z, b.x, b.y, b.z, tl);
+        bool hr = slab(r.o, winv, r.tmin, far_, c.x, c.y, c.z, d.x, d.y, d.z, tr);
+        uint32_t lref = __float_as_uint(a.w), rref = __float_as_uint(c.w);
+        if (hl && hr) {
+          bool rfirst = tr < tl;
+          push(sp, rfirst ? lref : rref);
+          ref = rfirst ? rref : lref;
+          continue;
+        }
+        if (hl || hr) {
+          ref = hl ? lref : rref;
+          continue;
+        }
+      } else {
+        const uint32_t i0 = (ref & ~kLeafBit) >> kLeafCountBits, i1 = i0 + (ref & kLeafMaxCount);
+        for (uint32_t i = i0; i < i1 && !done; i++) {
+          if constexpr (CNT) ci++;
+          const InstRec &in = p.insts[i];
+          float ti;
+          if (!slab(r.o, winv, r.tmin, far_, in.mn[0], in.mn[1], in.mn[2], in.mx[0], in.mx[1], in.mx[2], ti))
+            continue;
+          if ((r.mask & in.mask) == 0) continue;
+          const uint32_t fl = merge_flags(r.flags, in.flags);
+          if (fl & F_SKIP_TRIANGLES) continue;
+          F3 bo, bd;
+          float s;
+          object_ray(in.inv, r.o, r.d, bo, bd, s);
+          const F3 binv = recip(bd);
+          const float near_s = r.tmin * s;
+          bool ce, cb;
+          cull_triangle(fl, ce, cb);
+          for (uint32_t g = in.geo_begin; g < in.geo_end && !done; g++) {
+            const GeoRec &geo = p.geos[g];
+            if (!geometry_passes(fl, geo.flags)) continue;
+            float tg;
+            if (!slab(bo, binv, near_s, far_ * s, geo.mn[0], geo.mn[1], geo.mn[2], geo.mx[0], geo.mx[1],
+                      geo.mx[2], tg))
+              continue;
+            // BLAS traversal; its stack entries sit above the TLAS ones.
+            uint32_t bsp = sp;
+            uint32_t bref = geo.root_ref;
+            for (;;) {
+              if (!(bref & kLeafBit)) {
+                if constexpr (CNT) cn++;
+                const float4 *q = reinterpret_cast<const float4 *>(p.tri_pairs + bref);
+                float4 a = q[0], b = q[1], c = q[2], d = q[3];
+                const float fs = far_ * s;
+                float tl, tr;
+                bool hl = slab(bo, binv, near_s, fs, a.x, a.y, a.z, b.x, b.y, b.z, tl);
+                bool hr = slab(bo, binv, near_s, fs, c.x, c.y, c.z, d.x, d.y, d.z, tr);
+                uint32_t lref = __float_as_uint(a.w), rref = __float_as_uint(c.w);
+                if (hl && hr) {
+                  bool rfirst = tr < tl;
+                  push(bsp, rfirst ? lref : rref);
+                  bref = rfirst ? rref : lref;
+                  continue;
+                }
+                if (hl || hr) {
+                  bref = hl ? lref : rref;
+                  continue;
+                }
+              } else {
+                const uint32_t k0 = (bref & ~kLeafBit) >> kLeafCountBits, k1 = k0 + (bref & kLeafMaxCount);
+                for (uint32_t k = k0; k < k1; k++) {
+                  if constexpr (CNT) ct++;
+                  const float4 *q = reinterpret_cast<const float4 *>(p.tris + k);
+                  float4 a = q[0], b = q[1], c = q[2], d = q[3];
+                  float t, u, v;
+                  bool back;
+                  if (tri_core(bo, bd, near_s, far_ * s, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(c.x, c.y, c.z),
+                               f3(d.x, d.y, d.z), a.w, b.w, c.w, d.w, ce, cb, t, u, v, back)) {
+                    far_ = t / s;
+                    best.k = k;
+                    best.geo = g;
+                    best.inst = i;
+                    best.t = far_;
+                    best.u = u;
+                    best.v = v;
+                    best.back = back;
+                    if (fl & F_END_SEARCH) {
+                      done = true;
+                      break;
+                    }
+                  }
+                }
+                if (done) break;
+              }
+              if (bsp == sp) break;
+              bref = pop(bsp);
+            }
+          }
+        }
+        if (done) break;
+      }
+      if (sp == 0) break;
+      ref = pop(sp);
+    }
+  }
+  write_result(p, rid, best, cn, ct, ci);
+}
+
+// One ray per thread over a grid sized for p.n; the live count may be
+// smaller (device queue size).
+template <int K, int S, int B, int W, bool CNT>
+__global__ __launch_bounds__(B, W) void k_trace_fast(TraceParams p) {
+  __shared__ uint32_t lds[K * B];
+  const uint32_t n = p.count ? min(slots_of(p), *p.count) : slots_of(p);
+  const uint32_t used = (n + B - 1) / B;
+  if (blockIdx.x >= used) return;
+  const uint32_t slot = xcd_block(blockIdx.x, used) * B + threadIdx.x;
+  if (slot >= n) return;
+  const uint32_t rid = p.queue ? p.queue[slot] : slot;
+  if (rid >= p.n) return;  // a queue entry always names one of the n rays
+  trace_fast_one<K, S, B, CNT>(p, rid, lds + threadIdx.x);
+}
+
+// Persistent variant: a grid sized to the resident capacity; each wave
+// takes chunks of 64 consecutive queue slots from the device counter
+// p.work until the queue is exhausted, so uneven rays and scenes balance
+// across the chip.  Every wave exits once a fetch lands past the count.
+template <int K, int S, int B, int W, bool CNT>
+__global__ __launch_bounds__(B, W) void k_trace_persist(TraceParams p) {
+  __shared__ uint32_t lds[K * B];
+  const uint32_t n = p.count ? min(slots_of(p), *p.count) : slots_of(p);
+  const uint32_t lane = threadIdx.x & 63u;
+  for (;;) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(p.work, 64u);
+    base = __shfl(base, 0, 64);
+    if (base >= n) break;
+    const uint32_t slot = base + lane;
+    if (slot < n) {
+      const uint32_t rid = p.queue ? p.queue[slot] : slot;
+      if (rid < p.n) trace_fast_one<K, S, B, CNT>(p, rid, lds + threadIdx.x);
+    }
+  }
+}
+
+__global__ void k_zero_work(uint32_t *w) { *w = 0u; }
+
+// One fast-traversal launch with block size B, total stack S and tuning t.
+// K = stack entries per lane in LDS, W = minimum waves per SIMD asked of the
+// register allocator (1 = its own choice).
+template <int K, int S, int B, int W>
+hipError_t launch_fast_kw(const TraceParams &p, bool persist, hipStream_t st) {
+  if (persist) {
+    if (!p.work) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(1), 0, st, p.work);
+    if (p.counters) {
+      auto k = k_trace_persist<K, S, B, W, true>;
+      uint32_t g = std::min(resident_grid((const void *)k, B), (slots_of(p) + B - 1) / B);
+      hipLaunchKernelGGL(k, dim3(g), dim3(B), 0, st, p);
+    } else {
+      auto k = k_trace_persist<K, S, B, W, false>;
+      uint32_t g = std::min(resident_grid((const void *)k, B), (slots_of(p) + B - 1) / B);
+      hipLaunchKernelGGL(k, dim3(g), dim3(B), 0, st, p);
+    }
+    return hipGetLastError();
+  }
+  dim3 grid((slots_of(p) + B - 1) / B);
+  if (p.counters) hipLaunchKernelGGL((k_trace_fast<K, S, B, W, true>), grid, dim3(B), 0, st, p);
+  else hipLaunchKernelGGL((k_trace_fast<K, S, B, W, false>), grid, dim3(B), 0, st, p);
+  return hipGetLastError();
+}
+
+template <int S, int B>
+hipError_t launch_fast_s(const TraceParams &p, const TraceTune &t, hipStream_t st) {
+  constexpr int K8 = S < 8 ? S : 8;
+  constexpr int K16 = S < 16 ? S : 16;
+  const bool persist = t.persist != 0;
+  if (t.waves >= 8) {
+    if (t.lds <= 8) return launch_fast_kw<K8, S, B, 8>(p, persist, st);
+    return launch_fast_kw<K16, S, B, 8>(p, persist, st);
+  }
+  if (t.lds <= 8) return launch_fast_kw<K8, S, B, 1>(p, persist, st);
+  return launch_fast_kw<K16, S, B, 1>(p, persist, st);
+}
+
+template <int B>
+hipError_t launch_fast_block(const TraceParams &p, uint32_t s, const TraceTune &t, hipStream_t st) {
+  switch (s) {
+    case 8: return launch_fast_s<8, B>(p, t, st);
+    case 16: return launch_fast_s<16, B>(p, t, st);
+    case 24: return launch_fast_s<24, B>(p, t, st);
+    case 32: return launch_fast_s<32, B>(p, t, st);
+    case 48: return launch_fast_s<48, B>(p, t, st);
+    case 64: return launch_fast_s<64, B>(p, t, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+}  // namespace rdrt
+
+#endif

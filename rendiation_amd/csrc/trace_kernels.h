@@ -49,11 +49,26 @@ __host__ __device__ inline uint32_t slots_of(const TraceParams &p) { return p.sl
 // with for the given push depth; 0 if none is large enough.
 uint32_t stack_entries_for(uint32_t push_depth);
 
-// Enqueues one traversal launch on `stream`.  mode = RDRT_TRAVERSAL_*.
-// persist = resident-size grid whose waves pull 64-ray chunks from p.work
-// (zeroed by a kernel launched first, so the pair is capturable).
-hipError_t launch_trace(const TraceParams &p, uint32_t mode, uint32_t push_depth, uint32_t block,
-                        hipStream_t stream, bool persist = false);
+// Launch shape of the fast traversal (the autotuner's search space).
+struct TraceTune {
+  uint32_t block = 256;  // threads per block: 64, 128 or 256
+  uint32_t lds = 16;     // stack entries per lane kept in LDS: 8 or 16 (deeper ones in private memory)
+  uint32_t waves = 0;    // 8 = ask the register allocator for 8 waves per SIMD; 0 = its own choice
+  uint32_t persist = 0;  // 1 = resident-size grid whose waves pull 64-ray chunks from a counter
+};
+
+// Enqueues one traversal launch on `stream`.  mode = RDRT_TRAVERSAL_*.  A
+// persistent launch needs p.work (zeroed by a kernel launched first, so the
+// pair is capturable).
+hipError_t launch_trace(const TraceParams &p, uint32_t mode, uint32_t push_depth, const TraceTune &tune,
+                        hipStream_t stream);
+
+// Resident blocks of `kernel` at `block` threads on the current device.
+uint32_t resident_grid(const void *kernel, int block);
+// Per-block-size instantiations (trace_fast_<B>.hip); `stack` from stack_entries_for.
+hipError_t launch_fast_64(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
+hipError_t launch_fast_128(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
+hipError_t launch_fast_256(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
 
 }  // namespace rdrt
 

@@ -140,3 +140,27 @@ def test_edge_cases(device):
         assert np.all(gh[:, 0] == 0xFFFFFFFF)
         rh, _, _ = orc.trace(rays, tlas_idx=5)
         assert np.array_equal(gh, rh)
+
+
+def test_launch_shapes_agree_and_autotune(closest_hit_scene, device):
+    """Every launch shape of the search space traverses in the same order, so
+    hits are bit-identical to the default shape; the autotuner keeps one."""
+    import torch
+
+    sys, rays_np, _ = closest_hit_scene
+    _say("launch shapes + autotune, 100k tris x 1M rays")
+    base, _, _ = _gpu_trace(sys, rays_np, device)
+    rays = torch.from_numpy(np.ascontiguousarray(rays_np)).to(device)
+    hits = torch.empty((rays_np.shape[0], 4), dtype=torch.int32, device=device)
+    for shape in [(64, 8, 0, 0), (128, 16, 8, 1), (256, 8, 8, 0), (256, 16, 0, 1)]:
+        sys.set_tune(*shape)
+        sys.trace(rays, hits)
+        sys.sync()
+        assert np.array_equal(hits.cpu().numpy().view(np.uint32), base), shape
+    choice = sys.autotune(rays, hits, reps=2)
+    print("autotune:", choice)
+    assert choice["block_size"] in (64, 128, 256) and choice["lds_entries"] in (8, 16) and choice["ms"] > 0
+    sys.trace(rays, hits)
+    sys.sync()
+    assert np.array_equal(hits.cpu().numpy().view(np.uint32), base)
+    sys.set_tune()

@@ -108,8 +108,14 @@ def test_furnace_on_gpu(device):
     img, st = _render(s, cam, cfg)
     a32 = float(np.float32(a))
     want = E * sum(a32 ** k for k in range(N + 1))
-    assert np.allclose(img, want, rtol=1e-5)
-    assert st["extension_rays"] == [64 * 64 * 2] * (N + 1)
+    # The reference triangle test is not watertight: a ray can slip between
+    # two triangles of the closed sphere.  The oracle traces the same paths,
+    # so it leaks at the same places; elsewhere both equal the closed form.
+    ref, ext, _ = oracle_scene_from(furnace_scene(a, E)).pt_render(oracle_params(cam, cfg))
+    assert np.allclose(img, ref, rtol=1e-5)
+    assert [int(x) for x in st["extension_rays"]] == [int(x) for x in ext]
+    closed = np.abs(img - want) <= 1e-5 * want
+    assert closed.mean() >= 0.999, closed.mean()
 
 
 def test_floor_on_gpu(device):
