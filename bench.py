@@ -241,6 +241,8 @@ def closest_hit(args):
         hits = torch.empty((n, 4), dtype=torch.int32, device=dev)
         cnt = torch.empty((n, 3), dtype=torch.int32, device=dev)
     stream.synchronize()
+    if args.tune:
+        system.set_tune(*[int(x) for x in args.tune.split(",")])
     if args.profile_only:  # the launches alone, for hardware counters
         for _ in range(args.steps):
             system.trace(rays, hits, traversal=TRAVERSAL_FAST)
@@ -250,6 +252,7 @@ def closest_hit(args):
     system.trace(rays, hits, counters=cnt, traversal=TRAVERSAL_REFERENCE)
     system.sync()
     c = cnt.cpu().numpy().astype(np.int64).sum(axis=0)
+    tune = system.autotune(rays, hits, reps=3) if not (args.no_autotune or args.tune) else system.tune()
     bytes_per_launch = _bytes(n, c)
 
     for _ in range(args.warmup):
@@ -275,7 +278,7 @@ def closest_hit(args):
         "workload": "closest_hit: 100k random triangles (SAH 4 bins, leaf<=4), 1M random rays, seed 0",
         "tris": synth.CLOSEST_HIT_TRIS, "rays": n, "leaf": synth.CLOSEST_HIT_LEAF,
         "bvh_nodes": info.tri_nodes, "bvh_depth": info.max_blas_depth, "traversal": "fast",
-        "hit_fraction": round(hit_rate, 4), "build_upload_ms": round(build_upload_ms, 2),
+        "hit_fraction": round(hit_rate, 4), "build_upload_ms": round(build_upload_ms, 2), "launch_shape": tune,
     }
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
@@ -737,13 +740,12 @@ def cpu_baseline_batched(scenes, rays_np, meta_np, offs, bounces):
 PMC_STEPS = 3
 
 
-def pmc_traffic(workload: str, kernel: str):
+def pmc_traffic(workload: str, kernel: str, extra=()):
     """Physical DRAM-side bytes of `kernel` per step: two rocprofv3 --pmc
     child runs (FETCH_SIZE, then WRITE_SIZE) of this benchmark in
     --profile-only mode, each counter scaled by what it reads for a 256 MiB
-    device copy in the same run (--copy-calibration).  Run before this
-    process touches the GPU.  Returns a dict, or None when rocprofv3 is
-    unavailable or a pass fails."""
+    device copy in the same run (--copy-calibration).  Returns a dict, or
+    None when rocprofv3 is unavailable or a pass fails."""
     import csv
     import shutil
     import signal
@@ -757,7 +759,7 @@ def pmc_traffic(workload: str, kernel: str):
         d = tempfile.mkdtemp(prefix="rdrt_pmc_", dir="/tmp")
         cmd = [exe, "--pmc", counter, "-f", "csv", "-d", d, "-o", "p", "--", sys.executable, str(ROOT / "bench.py"),
                "--workload", workload, "--steps", str(PMC_STEPS), "--warmup", "0", "--no-cpu-baseline", "--no-traffic",
-               "--copy-calibration", "--profile-only"]
+               "--copy-calibration", "--profile-only", *extra]
         env = dict(os.environ, TMPDIR="/tmp")
         try:
             proc = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
@@ -799,6 +801,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 256 MiB device-to-device copy (a reference dispatch for HBM counters)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic passes")
+    ap.add_argument("--no-autotune", action="store_true", help="closest_hit: keep the default launch shape")
+    ap.add_argument("--tune", default="", help="closest_hit: launch shape block,lds,waves,persistent (no autotune)")
     ap.add_argument("--profile-only", action="store_true", help="run the timed launches only (counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -814,13 +818,19 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    traffic = None
-    if not args.no_traffic and not args.profile_only and args.workload in ("pt", "closest_hit"):
-        traffic = pmc_traffic(args.workload, "k_trace_fast")
     rec = {"pt": path_tracing, "closest_hit": closest_hit, "batched": batched,
            "diff": differentiable, "diffgeo": differentiable_geometry}[args.workload](args)
     if rec is None:
         return
+    traffic = None
+    if not args.no_traffic and args.workload in ("pt", "closest_hit"):
+        # counter passes in child processes, with this run's launch shape
+        shape = rec["config"].get("launch_shape")
+        extra, kernel = (), "k_trace_fast"
+        if shape:
+            extra = ("--tune", f"{shape['block_size']},{shape['lds_entries']},{shape['waves']},{shape['persistent']}")
+            kernel = "k_trace_persist" if shape["persistent"] else "k_trace_fast"
+        traffic = pmc_traffic(args.workload, kernel, extra)
     if traffic is not None and rec.get("roofline"):
         rec["roofline"]["traffic"] = round(traffic["bytes_per_step"])
         rec["roofline"]["traffic_detail"] = traffic

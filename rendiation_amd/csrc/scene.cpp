@@ -211,6 +211,52 @@ struct PairConv {
   }
 };
 
+// Child-pair conversion of the fast kernel's TLAS tree, whose leaves hold
+// single instances addressed by their forest position: map[order index].  A
+// depth-capped leaf with several instances becomes a chain of pair nodes.
+struct SinglePairConv {
+  const Tree &t;
+  const std::vector<uint32_t> &map;
+  std::vector<PairNode> &pairs;
+
+  uint32_t leaf(uint32_t begin, uint32_t count, const Box &b, uint32_t &depth) {
+    depth = 0;
+    if (count == 0) return leaf_ref(0, 0);  // empty TLAS
+    if (count == 1) return leaf_ref(map[t.order[begin]], 1);
+    uint32_t idx = (uint32_t)pairs.size();
+    pairs.emplace_back();
+    uint32_t sub = 0;
+    uint32_t rest = leaf(begin + 1, count - 1, b, sub);
+    PairNode &p = pairs[idx];
+    std::memset(&p, 0, sizeof(p));
+    put_box(p.lmn, p.lmx, b);
+    put_box(p.rmn, p.rmx, b);
+    p.lref = leaf_ref(map[t.order[begin]], 1);
+    p.rref = rest;
+    depth = 1 + sub;
+    return idx;
+  }
+
+  uint32_t ref_of(uint32_t i, uint32_t &depth) {
+    const TreeNode &n = t.nodes[i];
+    if (!n.inner) return leaf(n.begin, n.end - n.begin, n.box, depth);
+    uint32_t idx = (uint32_t)pairs.size();
+    pairs.emplace_back();
+    uint32_t l = i + 1, r = i + 1 + t.nodes[i + 1].size;
+    uint32_t dl = 0, dr = 0;
+    uint32_t lref = ref_of(l, dl);
+    uint32_t rref = ref_of(r, dr);
+    PairNode &p = pairs[idx];
+    std::memset(&p, 0, sizeof(p));
+    put_box(p.lmn, p.lmx, t.nodes[l].box);
+    put_box(p.rmn, p.rmx, t.nodes[r].box);
+    p.lref = lref;
+    p.rref = rref;
+    depth = 1 + std::max(dl, dr);
+    return idx;
+  }
+};
+
 // Reference forest nodes of one tree (naive/mod.rs:338-367).
 void append_forest(const Tree &t, uint32_t prim_offset, std::vector<RefNode> &out) {
   uint32_t base = (uint32_t)out.size();
@@ -474,10 +520,26 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
     root.ref_root = (uint32_t)out.tlas_forest.size();
     append_forest(tree, inst_base, out.tlas_forest);
     put_box(root.mn, root.mx, tree.nodes[0].box);
-    PairConv pc{tree, inst_base, out.tlas_pairs, err};
+    // The fast kernel's TLAS: the same binned SAH over the same boxes but with
+    // one instance per leaf, so a ray tests only instances whose boxes it
+    // enters (the reference's 10-instance leaves are linear scans).  Any tree
+    // gives the same closest hit; only its visit order differs.
+    if ((uint64_t)inst_base + inst.size() > (uint64_t)kLeafMaxStart + 1) {
+      err = "too many instances for the child-pair layout";
+      return RDRT_ERR_CAPACITY;
+    }
+    std::vector<Box> fboxes(inst.size());
+    std::vector<uint32_t> fmap(inst.size());
+    for (uint32_t j = 0; j < tree.order.size(); j++) {
+      fboxes[j] = boxes[tree.order[j]];
+      fmap[j] = inst_base + j;
+    }
+    TreeOptions fine = src.tlas_opt;
+    fine.bin_size = 1;
+    Tree ftree = build_tree(fboxes, fine, 1);
+    SinglePairConv pc{ftree, fmap, out.tlas_pairs};
     uint32_t d = 0;
     root.root_ref = pc.ref_of(0, d);
-    if (!pc.ok) return RDRT_ERR_CAPACITY;
     out.tlas_stack = std::max(out.tlas_stack, d);
     out.tlas_roots.push_back(root);
   }

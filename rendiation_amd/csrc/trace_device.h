@@ -239,7 +239,7 @@ __device__ __forceinline__ void trace_fast_one(const TraceParams &p, uint32_t ri
   bool done = false;
 
   const TlasRoot *root = bound_root(p, r.tlas_idx);
-  const F3 winv = recip(r.d);
+  F3 winv = recip(r.d);
   float tn;
   if (root && slab(r.o, winv, r.tmin, far_, root->mn[0], root->mn[1], root->mn[2], root->mx[0], root->mx[1],
                    root->mx[2], tn)) {
@@ -325,7 +325,6 @@ __device__ __forceinline__ void trace_fast_one(const TraceParams &p, uint32_t ri
                     best.k = k;
                     best.geo = g;
                     best.inst = i;
-                    best.t = far_;
                     best.u = u;
                     best.v = v;
                     best.back = back;
@@ -341,6 +340,9 @@ __device__ __forceinline__ void trace_fast_one(const TraceParams &p, uint32_t ri
               bref = pop(bsp);
             }
           }
+          // re-read the world ray instead of keeping it live through the BLAS loop
+          r = load_ray(p, rid);
+          winv = recip(r.d);
         }
         if (done) break;
       }
@@ -348,6 +350,7 @@ __device__ __forceinline__ void trace_fast_one(const TraceParams &p, uint32_t ri
       ref = pop(sp);
     }
   }
+  best.t = far_;  // far shrinks to each accepted hit
   write_result(p, rid, best, cn, ct, ci);
 }
 
