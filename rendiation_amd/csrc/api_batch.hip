@@ -19,7 +19,7 @@ using namespace rdrt;
 namespace rdrt {
 
 void BatchState::free_all() {
-  for (Scratch *s : {&wrays, &hitx, &qa, &qb, &counts, &lookback, &tickets, &work}) s->free();
+  for (Scratch *s : {&wrays, &hitx, &qa, &qb, &counts, &tmp, &blk, &work}) s->free();
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   exec = nullptr;
@@ -31,13 +31,9 @@ void BatchState::free_all() {
 
 namespace {
 
-constexpr uint32_t kBlock = 256;
-
 struct Plan {
   uint32_t rounds, nscenes;
-  std::vector<uint32_t> off;        // scene ray offsets
-  std::vector<size_t> lb_off;       // look-back word offset of (scene, round)
-  size_t lb_words = 0;
+  std::vector<uint32_t> off;  // scene ray offsets
 };
 
 Plan make_plan(const rdrt_batch_params *a) {
@@ -49,13 +45,6 @@ Plan make_plan(const rdrt_batch_params *a) {
   } else {
     p.nscenes = 1;
     p.off = {0u, a->n};
-  }
-  for (uint32_t k = 0; k < p.nscenes; k++) {
-    size_t blocks = (p.off[k + 1] - p.off[k] + kBlock - 1) / kBlock;
-    for (uint32_t r = 0; r < p.rounds; r++) {
-      p.lb_off.push_back(p.lb_words);
-      p.lb_words += blocks;
-    }
   }
   return p;
 }
@@ -95,8 +84,9 @@ hipError_t enqueue_scene(rdrt_ctx *c, const rdrt_batch_params *a, const Plan &pl
     sa.slots = nk;
     sa.qout = queues[(r + 1) & 1];
     sa.count_out = counts + r + 1;
-    sa.lookback = (unsigned long long *)s.lookback.ptr + pl.lb_off[(size_t)k * R + r];
-    sa.ticket = (uint32_t *)s.tickets.ptr + (size_t)k * R + r;
+    // compaction scratch, shared by every spawn (they run in stream order)
+    sa.tmp = (uint32_t *)s.tmp.ptr;
+    sa.blk = (uint32_t *)s.blk.ptr;
     sa.rays = tp.rays;
     sa.out_rays = (rdrt_ray *)s.wrays.ptr;
     sa.record = a->spawned ? a->spawned + (size_t)r * a->n : nullptr;
@@ -149,8 +139,9 @@ int32_t rdrt_trace_batched(rdrt_ctx *c, const rdrt_batch_params *a) {
   RDRT_HIP(s.qa.reserve(n * 4));
   RDRT_HIP(s.qb.reserve(n * 4));
   RDRT_HIP(s.counts.reserve((size_t)pl.nscenes * (pl.rounds + 1) * 4));
-  RDRT_HIP(s.lookback.reserve(std::max<size_t>(pl.lb_words, 1) * 8));
-  RDRT_HIP(s.tickets.reserve((size_t)pl.nscenes * pl.rounds * 4));
+  const size_t nblk = (n + kSpawnBlock - 1) / kSpawnBlock;
+  RDRT_HIP(s.tmp.reserve(nblk * kSpawnBlock * 4));
+  RDRT_HIP(s.blk.reserve(2 * nblk * 4));
   RDRT_HIP(s.work.reserve(4));
   hipStream_t st = c->work_stream;
   s.rounds = pl.rounds;
@@ -169,7 +160,7 @@ int32_t rdrt_trace_batched(rdrt_ctx *c, const rdrt_batch_params *a) {
     // graph key: every parameter and buffer the captured launches use
     std::vector<uint8_t> key(sizeof(*a) + 8 * sizeof(void *));
     std::memcpy(key.data(), a, sizeof(*a));
-    const void *ptrs[8] = {s.wrays.ptr, s.hitx.ptr, s.qa.ptr, s.qb.ptr, s.counts.ptr, s.lookback.ptr, s.tickets.ptr,
+    const void *ptrs[8] = {s.wrays.ptr, s.hitx.ptr, s.qa.ptr, s.qb.ptr, s.counts.ptr, s.tmp.ptr, s.blk.ptr,
                            s.work.ptr};
     std::memcpy(key.data() + sizeof(*a), ptrs, sizeof(ptrs));
     if (!s.exec || key != s.cached) {
@@ -180,9 +171,7 @@ int32_t rdrt_trace_batched(rdrt_ctx *c, const rdrt_batch_params *a) {
       uint32_t launches = 1;
       RDRT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       hipError_t e = enqueue_batch_init((uint32_t *)s.qa.ptr, 0u, a->n, (uint32_t *)s.counts.ptr,
-                                        (unsigned long long *)s.lookback.ptr, (uint32_t)pl.lb_words,
-                                        (uint32_t *)s.tickets.ptr, pl.rounds, (uint32_t *)s.counts.ptr + 1,
-                                        pl.rounds, st);
+                                        (uint32_t *)s.counts.ptr + 1, pl.rounds, st);
       if (e == hipSuccess) e = enqueue_scene(c, a, pl, 0, true, launches);
       hipGraph_t g = nullptr;
       hipError_t e2 = hipStreamEndCapture(st, &g);
@@ -198,11 +187,9 @@ int32_t rdrt_trace_batched(rdrt_ctx *c, const rdrt_batch_params *a) {
     RDRT_HIP(hipGraphLaunch(s.exec, st));
   } else {
     uint32_t launches = 1;
-    // identity queue over all rays (scene k owns [off[k], off[k+1])), zeroed
-    // look-back and tickets; the count slot written is the unused work word
-    RDRT_HIP(enqueue_batch_init((uint32_t *)s.qa.ptr, 0u, a->n, (uint32_t *)s.work.ptr,
-                                (unsigned long long *)s.lookback.ptr, (uint32_t)pl.lb_words, (uint32_t *)s.tickets.ptr,
-                                pl.nscenes * pl.rounds, nullptr, 0u, st));
+    // identity queue over all rays (scene k owns [off[k], off[k+1])); the count
+    // slot written is the unused work word
+    RDRT_HIP(enqueue_batch_init((uint32_t *)s.qa.ptr, 0u, a->n, (uint32_t *)s.work.ptr, nullptr, 0u, st));
     for (uint32_t k = 0; k < pl.nscenes; k++) RDRT_HIP(enqueue_scene(c, a, pl, k, false, launches));
     s.launches = launches;
   }

@@ -14,7 +14,7 @@ namespace rdrt {
 
 void PtState::free_all() {
   for (Scratch *s : {&rays, &shadow_rays, &hits, &shadow_hits, &hitx, &beta, &L, &nee, &qa, &qb, &qs, &counts,
-                     &lookback, &tickets, &materials, &diag_counters, &diag_totals, &record, &views, &work})
+                     &tmp_a, &tmp_b, &blk, &materials, &diag_counters, &diag_totals, &record, &views, &work})
     s->free();
   host_views.clear();
   materials_dirty = true;
@@ -52,8 +52,9 @@ int32_t reserve_pt(rdrt_ctx *c, uint32_t npaths) {
   RDRT_HIP(s.qs.reserve(n * 4));
   size_t stride = (n + kPtBlock - 1) / kPtBlock;
   RDRT_HIP(s.counts.reserve((2 * kPtMaxBounces + 3) * 4));
-  RDRT_HIP(s.lookback.reserve((kPtMaxBounces + 1) * stride * 8));
-  RDRT_HIP(s.tickets.reserve((kPtMaxBounces + 1) * 4));
+  RDRT_HIP(s.tmp_a.reserve(stride * kPtBlock * 4));
+  RDRT_HIP(s.tmp_b.reserve(stride * kPtBlock * 4));
+  RDRT_HIP(s.blk.reserve(4 * stride * 4));
   s.capacity = npaths;
   s.have_cached = false;
   return RDRT_OK;
@@ -216,9 +217,10 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
   L.qb = (uint32_t *)s.qb.ptr;
   L.qs = (uint32_t *)s.qs.ptr;
   L.counts = (uint32_t *)s.counts.ptr;
-  L.lookback = (unsigned long long *)s.lookback.ptr;
-  L.tickets = (uint32_t *)s.tickets.ptr;
-  L.lookback_stride = (npaths + kPtBlock - 1) / kPtBlock;
+  L.tmp_a = (uint32_t *)s.tmp_a.ptr;
+  L.tmp_b = (uint32_t *)s.tmp_b.ptr;
+  L.blk = (uint32_t *)s.blk.ptr;
+  L.blk_stride = (npaths + kPtBlock - 1) / kPtBlock;
   L.status = c->status;
   RDRT_HIP(s.work.reserve(4));
   L.work = (uint32_t *)s.work.ptr;
@@ -386,9 +388,8 @@ int32_t rdrt_compact(rdrt_ctx *c, const uint32_t *mask, uint32_t n, uint32_t *ou
   if (n > (1u << 31) - 1) return fail(RDRT_ERR_CAPACITY, "compaction supports fewer than 2^31 elements");
   int32_t rc = need_device(c);
   if (rc) return rc;
-  size_t nb = ((size_t)n + kPtBlock - 1) / kPtBlock;
-  RDRT_HIP(c->compact_scratch.reserve((nb + 2) * 8));
-  RDRT_HIP(enqueue_compact(mask, n, out_idx, count, (unsigned long long *)c->compact_scratch.ptr, c->stream));
+  RDRT_HIP(c->compact_scratch.reserve(compact_scratch_words(n) * 4));
+  RDRT_HIP(enqueue_compact(mask, n, out_idx, count, (uint32_t *)c->compact_scratch.ptr, c->stream));
   return RDRT_OK;
 }
 

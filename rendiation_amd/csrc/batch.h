@@ -20,8 +20,8 @@ struct SpawnArgs {
   uint32_t slots;        // grid size bound (max queue length)
   uint32_t *qout;
   uint32_t *count_out;
-  unsigned long long *lookback;
-  uint32_t *ticket;
+  uint32_t *tmp;  // block-local compacted runs (slots words)
+  uint32_t *blk;  // per block: count, then offset (2 * ceil(slots / 256) words)
   const rdrt_ray *rays;  // rays of this round (by path id)
   rdrt_ray *out_rays;    // rays of the next round (by path id)
   rdrt_ray *record;      // optional second copy of the spawned rays (by path id)
@@ -38,11 +38,12 @@ struct SpawnArgs {
   uint32_t *status;
 };
 
+// Spawn kernel, then the scan of its block counts and the scatter into qout.
 hipError_t enqueue_spawn(const SpawnArgs &a, hipStream_t st);
-// queue[i] = first + i for i < n; *count = n; zero `words` u64 words of `zero` and `tickets` u32 words.
-hipError_t enqueue_batch_init(uint32_t *queue, uint32_t first, uint32_t n, uint32_t *count,
-                              unsigned long long *zero, uint32_t words, uint32_t *tickets, uint32_t nticket,
-                              uint32_t *zero_counts, uint32_t ncounts, hipStream_t st);
+constexpr uint32_t kSpawnBlock = 256;
+// queue[i] = first + i for i < n; *count = n; zero `ncounts` u32 words of zero_counts.
+hipError_t enqueue_batch_init(uint32_t *queue, uint32_t first, uint32_t n, uint32_t *count, uint32_t *zero_counts,
+                              uint32_t ncounts, hipStream_t st);
 
 }  // namespace rdrt
 

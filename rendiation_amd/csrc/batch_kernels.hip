@@ -15,7 +15,7 @@ namespace {
 
 using pt::V3;
 using pt::v3;
-constexpr uint32_t kBlock = 256;
+constexpr uint32_t kBlock = kSpawnBlock;
 constexpr float kFltEps = 1.1920928955078125e-7f;
 constexpr float kFltMax = 3.402823466e+38f;
 
@@ -60,10 +60,7 @@ __device__ bool spawn_one(const SpawnArgs &a, uint32_t path) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_spawn(SpawnArgs a) {
-  __shared__ uint32_t s_bid;
-  if (threadIdx.x == 0) s_bid = atomicAdd(a.ticket, 1u);
-  __syncthreads();
-  const uint32_t bid = s_bid;
+  const uint32_t bid = blockIdx.x;
   const uint32_t count = min(*a.count_in, a.slots);
   const uint32_t nblocks = (count + kBlock - 1) / kBlock;
   if (bid >= nblocks) return;  // whole block
@@ -79,17 +76,13 @@ __global__ __launch_bounds__(kBlock) void k_spawn(SpawnArgs a) {
           *reinterpret_cast<uint4 *>(a.hits_later + (size_t)r * a.n + path) = make_uint4(RDRT_MISS, 0u, 0u, 0u);
     }
   }
-  compact::compact2<kBlock>(keep, false, path, a.qout, a.qout, a.lookback, bid, nblocks, a.count_out, nullptr,
-                            a.status, a.slots);
+  compact::block_compact2<kBlock>(keep, false, path, bid, a.tmp, nullptr, a.blk, nullptr);
 }
 
-__global__ void k_batch_init(uint32_t *queue, uint32_t first, uint32_t n, uint32_t *count,
-                             unsigned long long *zero, uint32_t words, uint32_t *tickets, uint32_t nticket,
-                             uint32_t *zero_counts, uint32_t ncounts) {
+__global__ void k_batch_init(uint32_t *queue, uint32_t first, uint32_t n, uint32_t *count, uint32_t *zero_counts,
+                             uint32_t ncounts) {
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) queue[i] = first + i;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) zero[i] = 0ull;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nticket; i += stride) tickets[i] = 0u;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ncounts; i += stride) zero_counts[i] = 0u;
   if (blockIdx.x == 0 && threadIdx.x == 0) *count = n;
 }
@@ -98,15 +91,22 @@ __global__ void k_batch_init(uint32_t *queue, uint32_t first, uint32_t n, uint32
 
 hipError_t enqueue_spawn(const SpawnArgs &a, hipStream_t st) {
   if (a.slots == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_spawn, dim3((a.slots + kBlock - 1) / kBlock), dim3(kBlock), 0, st, a);
+  const uint32_t grid = (a.slots + kBlock - 1) / kBlock;
+  uint32_t *cnt = a.blk, *off = a.blk + grid;
+  hipLaunchKernelGGL(k_spawn, dim3(grid), dim3(kBlock), 0, st, a);
+  hipLaunchKernelGGL(compact::k_block_scan, dim3(1), dim3(compact::kScanThreads), 0, st, a.count_in, a.slots, kBlock,
+                     (const uint32_t *)cnt, (const uint32_t *)nullptr, off, (uint32_t *)nullptr, a.count_out,
+                     (uint32_t *)nullptr);
+  hipLaunchKernelGGL((compact::k_scatter2<kBlock>), dim3(grid), dim3(kBlock), 0, st, a.count_in, a.slots,
+                     (const uint32_t *)a.tmp, (const uint32_t *)nullptr, (const uint32_t *)cnt,
+                     (const uint32_t *)nullptr, (const uint32_t *)off, (const uint32_t *)nullptr, a.qout,
+                     (uint32_t *)nullptr, a.slots);
   return hipGetLastError();
 }
 
-hipError_t enqueue_batch_init(uint32_t *queue, uint32_t first, uint32_t n, uint32_t *count,
-                              unsigned long long *zero, uint32_t words, uint32_t *tickets, uint32_t nticket,
-                              uint32_t *zero_counts, uint32_t ncounts, hipStream_t st) {
-  hipLaunchKernelGGL(k_batch_init, dim3(1024), dim3(256), 0, st, queue, first, n, count, zero, words, tickets,
-                     nticket, zero_counts, ncounts);
+hipError_t enqueue_batch_init(uint32_t *queue, uint32_t first, uint32_t n, uint32_t *count, uint32_t *zero_counts,
+                              uint32_t ncounts, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_init, dim3(1024), dim3(256), 0, st, queue, first, n, count, zero_counts, ncounts);
   return hipGetLastError();
 }
 

@@ -1,7 +1,9 @@
-// compact_device.h -- single-pass stable stream compaction for gfx950:
-// wave ballot ranks, block sums in LDS, wave-parallel decoupled look-back.
-// The output equals a stable copy_if (shader/parallel-compute/src/
-// stream_compaction.rs:3-45).  Included by HIP sources only.
+// compact_device.h -- stable stream compaction for gfx950: wave ballot ranks
+// (mbcnt), block sums in LDS, then a scan of the block counts and a scatter.
+// No block ever waits on another (a single-pass decoupled look-back chains
+// every block to its predecessors and serialized large queues).  The output
+// equals a stable copy_if (shader/parallel-compute/src/stream_compaction.rs:
+// 3-45).  Included by HIP sources only.
 #ifndef RDRT_COMPACT_DEVICE_H
 #define RDRT_COMPACT_DEVICE_H
 
@@ -12,89 +14,33 @@ namespace rdrt {
 namespace compact {
 
 using u64 = unsigned long long;
-constexpr uint32_t kSpinLimit = 1u << 22;
 
 __device__ __forceinline__ uint32_t lane_rank(u64 m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-struct Pair {
-  uint32_t a, b;
-};
+// ---- three-pass compaction (no block waits on another) ----
+// 1. block_compact2 (inside the producing kernel): each block compacts its
+//    own items stably into tmp[bid * B + local rank] and writes its counts.
+// 2. k_block_scan: one workgroup turns the block counts into exclusive
+//    offsets and the totals.
+// 3. k_scatter2: each block copies its compacted run to out[offset + j].
+// Block i holds the input slots [i * B, (i + 1) * B), so the output is a
+// stable copy_if whatever order the blocks run in.
 
-__device__ __forceinline__ u64 pack(u64 flag, Pair v) { return (flag << 62) | ((u64)v.a << 31) | (u64)v.b; }
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// Decoupled look-back for block `bid`, run by the 64 lanes of one wave:
-// publishes the block's aggregate, then inspects 64 predecessors per step
-// (lane l reads block base - l), sums the aggregates up to the nearest
-// inclusive prefix, publishes its own inclusive prefix and returns the
-// exclusive one to every lane.  Status word: bits 63-62 flag (0 empty,
-// 1 aggregate, 2 inclusive), 61-31 count a, 30-0 count b.
-__device__ Pair lookback_wave(u64 *status, uint32_t bid, Pair total, uint32_t *err) {
-  const uint32_t lane = threadIdx.x & 63u;
-  Pair excl{0u, 0u};
-  if (bid == 0) {
-    if (lane == 0) __hip_atomic_store(status, pack(2, total), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    return excl;
-  }
-  if (lane == 0) __hip_atomic_store(status + bid, pack(1, total), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  int32_t base = (int32_t)bid - 1;
-  uint32_t spins = 0;
-  for (;;) {
-    const int32_t j = base - (int32_t)lane;
-    // before block 0 reads as an inclusive zero
-    const u64 s = j >= 0 ? __hip_atomic_load(status + j, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : pack(2, Pair{0, 0});
-    const uint32_t flag = (uint32_t)(s >> 62);
-    const u64 incl = __ballot(flag == 2);
-    const u64 empty = __ballot(flag == 0);
-    const u64 low = incl & (~incl + 1ull);  // nearest inclusive predecessor
-    const u64 upto = incl ? (low | (low - 1ull)) : ~0ull;
-    if (empty & upto) {
-      // bounded wait: every predecessor already holds its ticket and publishes
-      // its aggregate without waiting; the limit only guards against a bug
-      if (++spins > kSpinLimit) {
-        if (lane == 0) atomicOr(err, 2u);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-      continue;
-    }
-    const bool mine = (upto >> lane) & 1ull;
-    excl.a += wave_sum(mine ? (uint32_t)((s >> 31) & 0x7fffffffu) : 0u);
-    excl.b += wave_sum(mine ? (uint32_t)(s & 0x7fffffffu) : 0u);
-    if (incl) break;
-    base -= 64;
-  }
-  if (lane == 0)
-    __hip_atomic_store(status + bid, pack(2, Pair{excl.a + total.a, excl.b + total.b}), __ATOMIC_RELEASE,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  return excl;
-}
-
-// Block-wide stable compaction of two flag sets: writes `item` to
-// out_a[offset] / out_b[offset] in thread order across the whole grid.
-// Must be called by every thread of the block.  Returns true on the block
-// that holds the last item (nblocks - 1) with the grid totals.
+// Stable block-local compaction of two flag sets.  Every thread of the
+// block calls it.
 template <int B>
-__device__ __forceinline__ void compact2(bool ka, bool kb, uint32_t item, uint32_t *out_a, uint32_t *out_b,
-                                         u64 *status, uint32_t bid, uint32_t nblocks, uint32_t *total_a,
-                                         uint32_t *total_b, uint32_t *err, uint32_t capacity) {
+__device__ __forceinline__ void block_compact2(bool ka, bool kb, uint32_t item, uint32_t bid, uint32_t *tmp_a,
+                                               uint32_t *tmp_b, uint32_t *cnt_a, uint32_t *cnt_b) {
   __shared__ uint32_t s_wa[B / 64], s_wb[B / 64];
-  __shared__ uint32_t s_ea, s_eb;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const u64 ma = __ballot(ka), mb = __ballot(kb);
-  const uint32_t ra = lane_rank(ma), rb = lane_rank(mb);
   if (lane == 0) {
     s_wa[wave] = (uint32_t)__popcll(ma);
     s_wb[wave] = (uint32_t)__popcll(mb);
   }
   __syncthreads();
-  // every thread derives its wave's offset and the block totals
   uint32_t pa = 0, pb = 0, ta = 0, tb = 0;
   for (uint32_t w = 0; w < B / 64; w++) {
     if (w == wave) {
@@ -104,23 +50,91 @@ __device__ __forceinline__ void compact2(bool ka, bool kb, uint32_t item, uint32
     ta += s_wa[w];
     tb += s_wb[w];
   }
-  if (wave == 0) {
-    Pair e = lookback_wave(status, bid, Pair{ta, tb}, err);
-    if (lane == 0) {
-      s_ea = e.a;
-      s_eb = e.b;
-      if (bid == nblocks - 1) {
-        if (total_a) *total_a = e.a + ta;
-        if (total_b) *total_b = e.b + tb;
-      }
+  const size_t base = (size_t)bid * B;
+  if (ka) tmp_a[base + pa + lane_rank(ma)] = item;
+  if (kb && tmp_b) tmp_b[base + pb + lane_rank(mb)] = item;
+  if (tid == 0) {
+    cnt_a[bid] = ta;
+    if (cnt_b) cnt_b[bid] = tb;
+  }
+}
+
+constexpr uint32_t kScanThreads = 1024;
+
+namespace {
+
+// Exclusive scan of the block counts of the first ceil(min(*count_in,
+// slots) / B) blocks (one workgroup of kScanThreads): offsets and totals.
+// cnt_b / off_b / total_b may be null.
+__global__ __launch_bounds__(kScanThreads) void k_block_scan(const uint32_t *count_in, uint32_t slots, uint32_t B,
+                                                             const uint32_t *cnt_a, const uint32_t *cnt_b,
+                                                             uint32_t *off_a, uint32_t *off_b, uint32_t *total_a,
+                                                             uint32_t *total_b) {
+  __shared__ uint32_t s_a[kScanThreads / 64], s_b[kScanThreads / 64];
+  const uint32_t n = count_in ? min(*count_in, slots) : slots;
+  const uint32_t nb = (n + B - 1) / B;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t chunk = (nb + kScanThreads - 1) / kScanThreads;
+  const uint32_t lo = min(tid * chunk, nb), hi = min(lo + chunk, nb);
+  uint32_t sa = 0, sb = 0;
+  for (uint32_t i = lo; i < hi; i++) {
+    sa += cnt_a[i];
+    if (cnt_b) sb += cnt_b[i];
+  }
+  // inclusive wave scans, then wave totals through LDS
+  uint32_t ia = sa, ib = sb;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t xa = __shfl_up(ia, o, 64), xb = __shfl_up(ib, o, 64);
+    if (lane >= (uint32_t)o) {
+      ia += xa;
+      ib += xb;
     }
   }
+  if (lane == 63) {
+    s_a[wave] = ia;
+    s_b[wave] = ib;
+  }
   __syncthreads();
-  // the bound only matters if a look-back gave up (error flag set)
-  const uint32_t ia = s_ea + pa + ra, ib = s_eb + pb + rb;
-  if (ka && ia < capacity) out_a[ia] = item;
-  if (kb && ib < capacity) out_b[ib] = item;
+  uint32_t wa = 0, wb = 0, ta = 0, tb = 0;
+  for (uint32_t w = 0; w < kScanThreads / 64; w++) {
+    if (w == wave) {
+      wa = ta;
+      wb = tb;
+    }
+    ta += s_a[w];
+    tb += s_b[w];
+  }
+  uint32_t ea = wa + ia - sa, eb = wb + ib - sb;  // exclusive prefix of this thread's chunk
+  for (uint32_t i = lo; i < hi; i++) {
+    off_a[i] = ea;
+    ea += cnt_a[i];
+    if (cnt_b) {
+      off_b[i] = eb;
+      eb += cnt_b[i];
+    }
+  }
+  if (tid == 0) {
+    if (total_a) *total_a = ta;
+    if (total_b) *total_b = tb;
+  }
 }
+
+// Copies each block's compacted run to its offset (grid: one block of B
+// threads per possible input block; blocks past the live count return).
+template <int B>
+__global__ __launch_bounds__(B) void k_scatter2(const uint32_t *count_in, uint32_t slots, const uint32_t *tmp_a,
+                                                const uint32_t *tmp_b, const uint32_t *cnt_a, const uint32_t *cnt_b,
+                                                const uint32_t *off_a, const uint32_t *off_b, uint32_t *out_a,
+                                                uint32_t *out_b, uint32_t capacity) {
+  const uint32_t n = count_in ? min(*count_in, slots) : slots;
+  const uint32_t nb = (n + B - 1) / B, bid = blockIdx.x, j = threadIdx.x;
+  if (bid >= nb) return;
+  const size_t base = (size_t)bid * B;
+  if (j < cnt_a[bid] && off_a[bid] + j < capacity) out_a[off_a[bid] + j] = tmp_a[base + j];
+  if (cnt_b && j < cnt_b[bid] && off_b[bid] + j < capacity) out_b[off_b[bid] + j] = tmp_b[base + j];
+}
+
+}  // namespace
 
 }  // namespace compact
 }  // namespace rdrt

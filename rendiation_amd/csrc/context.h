@@ -82,7 +82,7 @@ struct DeviceScene {
 struct PtState {
   uint32_t capacity = 0;  // paths
   uint32_t pixels = 0;
-  Scratch rays, shadow_rays, hits, shadow_hits, hitx, beta, L, nee, qa, qb, qs, counts, lookback, tickets;
+  Scratch rays, shadow_rays, hits, shadow_hits, hitx, beta, L, nee, qa, qb, qs, counts, tmp_a, tmp_b, blk;
   Scratch materials, record, views, work;
   std::vector<float> host_views;  // contents of `views`
   uint32_t nmaterials_uploaded = 0;
@@ -105,7 +105,7 @@ struct PtState {
 
 // Batched-tracing state of one context (grown on demand).
 struct BatchState {
-  Scratch wrays, hitx, qa, qb, counts, lookback, tickets, work;
+  Scratch wrays, hitx, qa, qb, counts, tmp, blk, work;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<uint8_t> cached;  // launch key of the captured graph

@@ -40,9 +40,10 @@ struct PtLaunch {
   float *beta, *L, *nee;
   uint32_t *qa, *qb, *qs;  // extension queues (ping-pong) and the shadow queue
   uint32_t *counts;        // [0..max_bounces+1]: extension queue sizes; [max_bounces+2+b]: shadow queue sizes
-  unsigned long long *lookback;  // per bounce: one status word per block
-  uint32_t *tickets;             // per bounce: block ticket counter
-  uint32_t lookback_stride;
+  // compaction scratch: block-local runs of the two output queues, and per
+  // block [count a | count b | offset a | offset b], blk_stride words each
+  uint32_t *tmp_a, *tmp_b, *blk;
+  uint32_t blk_stride;
   uint32_t *status;
   uint32_t *work;  // work counter of persistent traversal launches
   float *out;  // width * height * 3
@@ -89,11 +90,12 @@ hipError_t enqueue_pack_materials(const float *albedo, const float *emission, ui
 hipError_t enqueue_l2_adjoint(const float *a, const float *b, uint32_t n, float *adj, double *loss,
                               hipStream_t stream);
 
-// Stable stream compaction of indices whose mask is non-zero (single pass,
-// decoupled look-back); out_count receives the kept count.  `scratch` holds
-// ceil(n / kPtBlock) status words + 1 ticket word, zeroed by this call.
+// Stable stream compaction of indices whose mask is non-zero (block-local
+// runs, a scan of the block counts, a scatter); out_count receives the kept
+// count.  `scratch` holds compact_scratch_words(n) u32.
+size_t compact_scratch_words(uint32_t n);
 hipError_t enqueue_compact(const uint32_t *mask, uint32_t n, uint32_t *out_idx, uint32_t *out_count,
-                           unsigned long long *scratch, hipStream_t stream);
+                           uint32_t *scratch, hipStream_t stream);
 
 // Sobol / van der Corput samples: out[2i] = vdc(index[i], scr[2i]),
 // out[2i+1] = sobol1(index[i], scr[2i+1]).

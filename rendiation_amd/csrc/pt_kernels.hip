@@ -32,7 +32,10 @@ namespace {
 using u64 = unsigned long long;
 using pt::V3;
 using pt::v3;
-using compact::compact2;
+using compact::block_compact2;
+using compact::k_block_scan;
+using compact::k_scatter2;
+using compact::kScanThreads;
 
 constexpr uint32_t kMiss = 0xffffffffu;
 constexpr float kFltEps = 1.1920928955078125e-7f;
@@ -73,13 +76,8 @@ __device__ __forceinline__ void write_ray(rdrt_ray *r, V3 o, float tmin, V3 d, f
 
 __global__ void k_pt_begin(PtLaunch P) {
   const uint32_t nc = 2 * P.max_bounces + 3;
-  const uint32_t nl = (P.max_bounces + 1) * P.lookback_stride;
-  const uint32_t n = nl > nc ? nl : nc;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    if (i < nl) P.lookback[i] = 0ull;
-    if (i < nc) P.counts[i] = i == 0 ? P.npaths : 0u;
-    if (i < P.max_bounces + 1) P.tickets[i] = 0u;
-  }
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x)
+    P.counts[i] = i == 0 ? P.npaths : 0u;
 }
 
 // ray generation (ray_gen.rs:89-107 + camera.rs:66-99)
@@ -278,26 +276,23 @@ __device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &ke
   }
 }
 
+// Shades queue slots [bid * B, bid * B + B) and compacts the survivors and
+// the shadow rays block-locally (the scan and scatter launches follow).
 template <int B, bool REPLAY>
 __global__ __launch_bounds__(B) void k_pt_shade(PtLaunch P, uint32_t b) {
-  __shared__ uint32_t s_bid;
-  if (threadIdx.x == 0) s_bid = atomicAdd(P.tickets + b, 1u);
-  __syncthreads();
-  const uint32_t bid = s_bid;
-  const uint32_t count_in = P.counts[b];
+  const uint32_t bid = blockIdx.x;
+  const uint32_t count_in = min(P.counts[b], P.npaths);
   const uint32_t nblocks = (count_in + B - 1) / B;
   if (bid >= nblocks) return;  // whole block
   const uint32_t *qin = (b & 1) ? P.qb : P.qa;
-  uint32_t *qout = (b & 1) ? P.qa : P.qb;
   const uint32_t i = bid * B + threadIdx.x;
   bool keep = false, shadow = false;
   uint32_t path = 0;
-  if (i < count_in && i < P.npaths) {
+  if (i < count_in) {
     path = qin[i];
     if (path < P.npaths) shade_one<REPLAY>(P, b, path, keep, shadow);
   }
-  compact2<B>(keep, shadow, path, qout, P.qs, P.lookback + (size_t)b * P.lookback_stride, bid, nblocks,
-              P.counts + b + 1, P.counts + P.max_bounces + 2 + b, P.status, P.npaths);
+  block_compact2<B>(keep, shadow, path, bid, P.tmp_a, P.tmp_b, P.blk, P.blk + P.blk_stride);
 }
 
 // add the light of unoccluded shadow rays
@@ -345,17 +340,12 @@ __global__ __launch_bounds__(kPtBlock) void k_sum_visits(const uint32_t *queue, 
   if (threadIdx.x < 3 && s[threadIdx.x]) atomicAdd(&totals[threadIdx.x], s[threadIdx.x]);
 }
 
-__global__ __launch_bounds__(kPtBlock) void k_compact(const uint32_t *mask, uint32_t n, uint32_t *out,
-                                                      uint32_t *out_count, u64 *status, uint32_t *ticket,
-                                                      uint32_t *err) {
-  __shared__ uint32_t s_bid;
-  if (threadIdx.x == 0) s_bid = atomicAdd(ticket, 1u);
-  __syncthreads();
-  const uint32_t bid = s_bid;
-  const uint32_t nblocks = (n + kPtBlock - 1) / kPtBlock;
+__global__ __launch_bounds__(kPtBlock) void k_compact(const uint32_t *mask, uint32_t n, uint32_t *tmp,
+                                                      uint32_t *cnt) {
+  const uint32_t bid = blockIdx.x;
   const uint32_t i = bid * kPtBlock + threadIdx.x;
   const bool keep = i < n && mask[i] != 0u;
-  compact2<kPtBlock>(keep, false, i, out, out, status, bid, nblocks, out_count, nullptr, err, n);
+  block_compact2<kPtBlock>(keep, false, i, bid, tmp, nullptr, cnt, nullptr);
 }
 
 // ---- adjoint ----
@@ -494,6 +484,18 @@ __global__ void k_sobol(const uint32_t *index, const uint32_t *scr, uint32_t n, 
   out[2 * i + 1] = pt::sobol1(index[i], scr[2 * i + 1]);
 }
 
+// Scan of the shade kernel's block counts and scatter into the next
+// extension queue and the shadow queue (sizes to counts[b + 1] and
+// counts[max_bounces + 2 + b]).
+void enqueue_queue_scatter(const PtLaunch &P, uint32_t b, uint32_t grid, hipStream_t st) {
+  uint32_t *qout = (b & 1) ? P.qa : P.qb;
+  const uint32_t s = P.blk_stride;
+  hipLaunchKernelGGL(k_block_scan, dim3(1), dim3(kScanThreads), 0, st, P.counts + b, P.npaths, kPtBlock, P.blk,
+                     P.blk + s, P.blk + 2 * s, P.blk + 3 * s, P.counts + b + 1, P.counts + P.max_bounces + 2 + b);
+  hipLaunchKernelGGL((k_scatter2<kPtBlock>), dim3(grid), dim3(kPtBlock), 0, st, P.counts + b, P.npaths, P.tmp_a,
+                     P.tmp_b, P.blk, P.blk + s, P.blk + 2 * s, P.blk + 3 * s, qout, P.qs, P.npaths);
+}
+
 }  // namespace
 
 hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_t push_depth,
@@ -535,6 +537,7 @@ hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_
     hipError_t e = trace(tp);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_pt_shade<kPtBlock, false>), dim3(grid), dim3(kPtBlock), 0, st, P, b);
+    enqueue_queue_scatter(P, b, grid, st);
     if (b == P.max_bounces) break;
     TraceParams sp = tp;
     sp.rays = P.shadow_rays;
@@ -578,6 +581,7 @@ hipError_t enqueue_pt_replay(const PtLaunch &P, const TraceParams &scene, uint32
     hipError_t e = launch_trace(tp, RDRT_TRAVERSAL_FAST, push_depth, tune, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_pt_shade<kPtBlock, true>), dim3(grid), dim3(kPtBlock), 0, st, P, b);
+    enqueue_queue_scatter(P, b, grid, st);
     if (b == P.max_bounces) break;
   }
   return hipGetLastError();
@@ -617,17 +621,26 @@ hipError_t enqueue_l2_adjoint(const float *a, const float *b, uint32_t n, float 
   return hipGetLastError();
 }
 
+size_t compact_scratch_words(uint32_t n) {
+  const size_t nb = ((size_t)n + kPtBlock - 1) / kPtBlock;
+  return nb * kPtBlock + 2 * nb + 1;
+}
+
 hipError_t enqueue_compact(const uint32_t *mask, uint32_t n, uint32_t *out_idx, uint32_t *out_count,
-                           unsigned long long *scratch, hipStream_t st) {
+                           uint32_t *scratch, hipStream_t st) {
   const uint32_t nb = (n + kPtBlock - 1) / kPtBlock;
-  // status words, then the ticket (as a u64 slot), then an error word
-  hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, st, scratch, nb + 2);
-  uint32_t *ticket = reinterpret_cast<uint32_t *>(scratch + nb);
-  uint32_t *err = reinterpret_cast<uint32_t *>(scratch + nb + 1);
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, out_count);
-  if (nb > 0)
-    hipLaunchKernelGGL(k_compact, dim3(nb), dim3(kPtBlock), 0, st, mask, n, out_idx, out_count, scratch, ticket,
-                       err);
+  if (nb == 0) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, out_count);
+    return hipGetLastError();
+  }
+  uint32_t *tmp = scratch, *cnt = scratch + (size_t)nb * kPtBlock, *off = cnt + nb;
+  hipLaunchKernelGGL(k_compact, dim3(nb), dim3(kPtBlock), 0, st, mask, n, tmp, cnt);
+  hipLaunchKernelGGL(k_block_scan, dim3(1), dim3(kScanThreads), 0, st, (const uint32_t *)nullptr, n, kPtBlock, cnt,
+                     (const uint32_t *)nullptr, off, (uint32_t *)nullptr, out_count, (uint32_t *)nullptr);
+  hipLaunchKernelGGL((k_scatter2<kPtBlock>), dim3(nb), dim3(kPtBlock), 0, st, (const uint32_t *)nullptr, n,
+                     (const uint32_t *)tmp, (const uint32_t *)nullptr, (const uint32_t *)cnt,
+                     (const uint32_t *)nullptr, (const uint32_t *)off, (const uint32_t *)nullptr, out_idx,
+                     (uint32_t *)nullptr, n);
   return hipGetLastError();
 }
 
