@@ -42,6 +42,7 @@ void free_device_scene(DeviceScene &d) {
   release(d.tlas_pairs);
   release(d.tris);
   release(d.redirect);
+  release(d.tri_ids);
   release(d.indices);
   release(d.binding);
   release(d.tri_material);
@@ -68,6 +69,7 @@ TraceParams scene_trace_params(const rdrt_ctx *c) {
   p.tri_pairs = d.tri_pairs.ptr;
   p.tris = d.tris.ptr;
   p.redirect = d.redirect.ptr;
+  p.tri_ids = d.tri_ids.ptr;
   p.geos = d.geos.ptr;
   p.tlas_pairs = d.tlas_pairs.ptr;
   p.insts = d.insts.ptr;
@@ -291,6 +293,7 @@ int32_t rdrt_commit(rdrt_ctx *c) {
   RDRT_HIP(upload(d.tlas_pairs, b.tlas_pairs, d.bytes));
   RDRT_HIP(upload(d.tris, b.tris, d.bytes));
   RDRT_HIP(upload(d.redirect, b.redirect, d.bytes));
+  RDRT_HIP(upload(d.tri_ids, b.tri_ids, d.bytes));
   RDRT_HIP(upload(d.indices, b.indices, d.bytes));
   RDRT_HIP(upload(d.binding, b.binding, d.bytes));
   RDRT_HIP(upload(d.vertices, b.vertices, d.bytes));

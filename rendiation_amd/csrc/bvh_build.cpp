@@ -198,6 +198,187 @@ struct Builder {
 
 }  // namespace
 
+namespace {
+
+// ---- fast-traversal tree: full binned SAH on the centre bounds ----
+constexpr uint32_t kMaxFastBins = 32;
+
+struct FastBuilder {
+  const std::vector<Box> &boxes;
+  FastTreeOptions opt;
+  std::vector<float> centre[3];
+  std::vector<uint32_t> order, scratch;
+
+  FastBuilder(const std::vector<Box> &b, const FastTreeOptions &o) : boxes(b), opt(o) {
+    size_t n = b.size();
+    for (int k = 0; k < 3; k++) centre[k].resize(n);
+    order.resize(n);
+    scratch.resize(n);
+    for (size_t i = 0; i < n; i++) {
+      order[i] = (uint32_t)i;
+      for (int k = 0; k < 3; k++) centre[k][i] = (b[i].mn[k] + b[i].mx[k]) * 0.5f;
+    }
+  }
+
+  Box range_box(uint32_t b, uint32_t e) const {
+    Box r = box_empty();
+    for (uint32_t i = b; i < e; i++) grow(r, boxes[order[i]]);
+    return r;
+  }
+
+  // Returns the split point (b < mid < e) after reordering [b, e) and the
+  // child boxes, or b when a leaf is cheaper (only for count <= max_leaf).
+  // One pass bins every primitive on all three axes.
+  uint32_t split(const Box &nb, uint32_t b, uint32_t e, Box &lb, Box &rb) {
+    const uint32_t n = e - b, K = opt.bins;
+    float cmn[3], cmx[3];
+    for (int k = 0; k < 3; k++) {
+      cmn[k] = std::numeric_limits<float>::infinity();
+      cmx[k] = -std::numeric_limits<float>::infinity();
+    }
+    for (uint32_t i = b; i < e; i++)
+      for (int k = 0; k < 3; k++) {
+        const float c = centre[k][order[i]];
+        cmn[k] = std::fmin(cmn[k], c);
+        cmx[k] = std::fmax(cmx[k], c);
+      }
+    float scale[3];
+    bool live[3];
+    for (int k = 0; k < 3; k++) {
+      const float ext = cmx[k] - cmn[k];
+      live[k] = ext > 0.0f;
+      scale[k] = live[k] ? (float)K / ext : 0.0f;
+    }
+    if (!live[0] && !live[1] && !live[2]) {
+      // every centre coincides: split the range in half (object median)
+      lb = range_box(b, b + n / 2);
+      rb = range_box(b + n / 2, e);
+      return b + n / 2;
+    }
+    uint32_t cnt[3 * kMaxFastBins];
+    Box bb[3 * kMaxFastBins];
+    for (uint32_t k = 0; k < 3 * K; k++) {
+      cnt[k] = 0u;
+      bb[k] = box_empty();
+    }
+    for (uint32_t i = b; i < e; i++) {
+      const uint32_t p = order[i];
+      const Box &pb = boxes[p];
+      for (int k = 0; k < 3; k++) {
+        if (!live[k]) continue;
+        const uint32_t w = bin_of(centre[k][p], cmn[k], scale[k], K);
+        cnt[k * K + w]++;
+        grow(bb[k * K + w], pb);
+      }
+    }
+    const float parent_area = surface_area(nb);
+    double best_cost = std::numeric_limits<double>::infinity();
+    int best_axis = -1;
+    uint32_t best_bin = 0;
+    double right_cost[kMaxFastBins];
+    for (int axis = 0; axis < 3; axis++) {
+      if (!live[axis]) continue;
+      const uint32_t *c = &cnt[axis * K];
+      const Box *bx = &bb[axis * K];
+      Box r = box_empty();
+      uint32_t rc = 0;
+      for (uint32_t k = K - 1; k >= 1; k--) {
+        grow(r, bx[k]);
+        rc += c[k];
+        right_cost[k] = (double)surface_area(r) * rc;
+      }
+      Box l = box_empty();
+      uint32_t lc = 0;
+      for (uint32_t k = 0; k + 1 < K; k++) {
+        grow(l, bx[k]);
+        lc += c[k];
+        if (lc == 0 || lc == n) continue;
+        const double cost = (double)surface_area(l) * lc + right_cost[k + 1];
+        if (cost < best_cost) {
+          best_cost = cost;
+          best_axis = axis;
+          best_bin = k;
+        }
+      }
+    }
+    const double split_cost =
+        opt.node_cost + opt.prim_cost * best_cost / std::max((double)parent_area, 1e-30);
+    if (n <= opt.max_leaf && split_cost >= opt.prim_cost * n) return b;
+    lb = box_empty();
+    rb = box_empty();
+    for (uint32_t k = 0; k < K; k++) grow(k <= best_bin ? lb : rb, bb[best_axis * K + k]);
+    // stable partition by the chosen bin boundary
+    uint32_t lo = b, hi = 0;
+    for (uint32_t i = b; i < e; i++) {
+      const uint32_t p = order[i];
+      if (bin_of(centre[best_axis][p], cmn[best_axis], scale[best_axis], K) <= best_bin) order[lo++] = p;
+      else scratch[b + hi++] = p;
+    }
+    std::copy(scratch.begin() + b, scratch.begin() + b + hi, order.begin() + lo);
+    return lo;
+  }
+
+  static uint32_t bin_of(float c, float mn, float scale, uint32_t K) {
+    return (uint32_t)std::min((float)(K - 1), std::max(0.0f, (c - mn) * scale));
+  }
+
+  void build(const Box &box, uint32_t b, uint32_t e, uint32_t depth, std::vector<TreeNode> &out, uint32_t &maxd,
+             int spawn_levels) {
+    if (depth > maxd) maxd = depth;
+    size_t self = out.size();
+    out.push_back(TreeNode{box, b, e, 1, false});
+    if (e - b <= 1 || depth >= opt.max_depth) return;
+    Box lb, rb;
+    const uint32_t mid = split(box, b, e, lb, rb);
+    if (mid == b) return;  // leaf
+    out[self].inner = true;
+    if (spawn_levels > 0 && e - b >= 16384) {
+      std::vector<TreeNode> left;
+      uint32_t lmax = 0;
+      std::thread worker([&] { build(lb, b, mid, depth + 1, left, lmax, spawn_levels - 1); });
+      std::vector<TreeNode> right;
+      uint32_t rmax = 0;
+      build(rb, mid, e, depth + 1, right, rmax, spawn_levels - 1);
+      worker.join();
+      out.insert(out.end(), left.begin(), left.end());
+      out.insert(out.end(), right.begin(), right.end());
+      maxd = std::max(maxd, std::max(lmax, rmax));
+    } else {
+      build(lb, b, mid, depth + 1, out, maxd, 0);
+      build(rb, mid, e, depth + 1, out, maxd, 0);
+    }
+    out[self].size = (uint32_t)(out.size() - self);
+  }
+};
+
+}  // namespace
+
+Tree build_fast_tree(const std::vector<Box> &boxes, const FastTreeOptions &opt_in, unsigned threads) {
+  FastTreeOptions opt = opt_in;
+  if (opt.bins < 2) opt.bins = 16;
+  opt.bins = std::min(opt.bins, kMaxFastBins);
+  opt.max_leaf = std::max(1u, std::min(opt.max_leaf, kLeafMaxCount));
+  FastBuilder bld(boxes, opt);
+  Tree t;
+  Box root = bld.range_box(0, (uint32_t)boxes.size());
+  int levels = 0;
+  while ((1u << levels) < threads && levels < 6) levels++;
+  bld.build(root, 0, (uint32_t)boxes.size(), 0, t.nodes, t.depth, levels);
+  t.order = std::move(bld.order);
+  return t;
+}
+
+double sah_cost(const Tree &t, const FastTreeOptions &opt) {
+  if (t.nodes.empty()) return 0.0;
+  const double root = std::max((double)surface_area(t.nodes[0].box), 1e-30);
+  double c = 0.0;
+  for (const TreeNode &n : t.nodes) {
+    const double a = surface_area(n.box) / root;
+    c += n.inner ? a * opt.node_cost : a * opt.prim_cost * (n.end - n.begin);
+  }
+  return c;
+}
+
 Tree build_tree(const std::vector<Box> &boxes, const TreeOptions &opt_in, unsigned threads) {
   TreeOptions opt = opt_in;
   if (opt.bins < 2) opt.bins = 4;

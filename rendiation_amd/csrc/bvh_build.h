@@ -47,6 +47,22 @@ struct Tree {
 // concurrently; the result does not depend on it.
 Tree build_tree(const std::vector<Box> &boxes, const TreeOptions &opt, unsigned threads);
 
+// The tree the fast traversal walks (any tree gives the same closest hit):
+// binned SAH over all three axes of the centre bounds, `bins` bins, leaves of
+// at most `max_leaf` primitives closed when the SAH prefers them, a median
+// split when all centres coincide.  Same Tree structure as build_tree.
+struct FastTreeOptions {
+  uint32_t bins = 16;
+  uint32_t max_leaf = 4;   // <= 31
+  uint32_t max_depth = 60;
+  float node_cost = 1.0f;  // SAH cost of visiting a child pair, per unit area ratio
+  float prim_cost = 1.0f;  // SAH cost of one primitive test
+};
+Tree build_fast_tree(const std::vector<Box> &boxes, const FastTreeOptions &opt, unsigned threads);
+// SAH cost of a tree (node_cost per inner node visit, prim_cost per
+// primitive test, weighted by area relative to the root).
+double sah_cost(const Tree &t, const FastTreeOptions &opt);
+
 }  // namespace rdrt
 
 #endif

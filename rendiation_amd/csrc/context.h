@@ -69,7 +69,7 @@ struct Scratch {
 struct DeviceScene {
   DevArray<PairNode> tri_pairs, tlas_pairs;
   DevArray<TriPre> tris;
-  DevArray<uint32_t> redirect, indices, binding, tri_material;
+  DevArray<uint32_t> redirect, tri_ids, indices, binding, tri_material;
   DevArray<float> vertices, normals;
   DevArray<GeoRec> geos;
   DevArray<InstRec> insts;

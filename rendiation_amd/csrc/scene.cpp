@@ -340,6 +340,15 @@ bool validate_refs(const BuiltScene &s, std::string &err) {
     err = "internal: forest link or range out of range";
     return false;
   }
+  if (s.tri_ids.size() != s.redirect.size() || s.tris.size() != s.tri_ids.size()) {
+    err = "internal: triangle arrays disagree";
+    return false;
+  }
+  for (uint32_t r : s.tri_ids)
+    if (r >= s.indices.size() / 3) {
+      err = "internal: triangle id out of range";
+      return false;
+    }
   for (uint32_t r : s.redirect)
     if ((size_t)3 * r + 2 >= s.indices.size()) {
       err = "internal: redirect out of range";
@@ -364,12 +373,13 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
   struct Job {
     uint32_t blas, geo;
     std::vector<uint32_t> tri_idx;  // 3 per triangle, local vertex ids
-    Tree tree;
+    Tree tree;  // the reference's tree (forest, redirect, k_trace_ref)
+    Tree fast;  // the fast traversal's tree (child pairs, TriPre order)
   };
   std::vector<Job> jobs;
   for (uint32_t b = 0; b < src.blas.size(); b++) {
     if (!src.blas_live[b]) continue;
-    for (uint32_t g = 0; g < src.blas[b].size(); g++) jobs.push_back(Job{b, g, {}, {}});
+    for (uint32_t g = 0; g < src.blas[b].size(); g++) jobs.push_back(Job{b, g, {}, {}, {}});
   }
   unsigned per_job = jobs.size() >= hw ? 1u : std::max(1u, hw / (unsigned)std::max<size_t>(1, jobs.size()));
   parallel_for(jobs.size(), std::min<unsigned>(hw, (unsigned)jobs.size()), [&](size_t j) {
@@ -389,6 +399,7 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
       boxes[t] = bx;
     }
     job.tree = build_tree(boxes, src.blas_opt, per_job);
+    job.fast = build_fast_tree(boxes, FastTreeOptions{}, per_job);
   });
   for (const Job &job : jobs) {
     const GeometrySrc &g = src.blas[job.blas][job.geo];
@@ -425,6 +436,7 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
         uint32_t vertex_start = (uint32_t)(out.vertices.size() / 3);
         out.vertices.insert(out.vertices.end(), g.pos.begin(), g.pos.end() - (g.pos.size() % 3));
         for (uint32_t k : job.tree.order) out.redirect.push_back(primitive_start + k);
+        for (uint32_t k : job.fast.order) out.tri_ids.push_back(primitive_start + k);
         for (uint32_t v : job.tri_idx) out.indices.push_back(vertex_start + v);
         if (any_normals) {
           size_t nv = g.pos.size() / 3;
@@ -449,7 +461,7 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
         gr.geometry_idx = gi;
         gr.flags = g.flags;
         append_forest(job.tree, primitive_start, out.tri_forest);
-        PairConv pc{job.tree, primitive_start, out.tri_pairs, err};
+        PairConv pc{job.fast, primitive_start, out.tri_pairs, err};
         uint32_t d = 0;
         gr.root_ref = pc.ref_of(0, d);
         if (!pc.ok) return RDRT_ERR_CAPACITY;
@@ -462,12 +474,12 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
       blas_geo_end[b] = (uint32_t)out.geos.size();
     }
   }
-  // triangle pre-pass, in redirect (leaf) order
-  out.tris.resize(out.redirect.size());
+  // triangle pre-pass, in the fast tree's leaf order
+  out.tris.resize(out.tri_ids.size());
   parallel_for((out.tris.size() + 65535) / 65536, hw, [&](size_t c) {
     size_t e = std::min(out.tris.size(), (c + 1) * 65536);
     for (size_t k = c * 65536; k < e; k++) {
-      size_t t = out.redirect[k];
+      size_t t = out.tri_ids[k];
       const float *v0 = &out.vertices[3 * (size_t)out.indices[3 * t]];
       const float *v1 = &out.vertices[3 * (size_t)out.indices[3 * t + 1]];
       const float *v2 = &out.vertices[3 * (size_t)out.indices[3 * t + 2]];

@@ -40,6 +40,7 @@ struct BuiltScene {
   // reference layout (naive/mod.rs:322-494)
   std::vector<RefNode> tri_forest, tlas_forest;
   std::vector<uint32_t> redirect, indices;
+  std::vector<uint32_t> tri_ids;  // fast-traversal triangle slot -> global triangle index
   std::vector<float> vertices;  // xyz
   // shading attributes: normals parallel to vertices (empty when no geometry
   // has any), material offset per triangle (empty when all are 0), largest

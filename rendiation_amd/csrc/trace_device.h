@@ -175,13 +175,14 @@ struct Best {
   bool back = false;
 };
 
-__device__ __forceinline__ void write_result(const TraceParams &p, uint32_t rid, const Best &best, uint32_t cn,
-                                             uint32_t ct, uint32_t ci) {
+// ids maps best.k to the global triangle: p.tri_ids (fast) or p.redirect (reference).
+__device__ __forceinline__ void write_result(const TraceParams &p, const uint32_t *ids, uint32_t rid,
+                                             const Best &best, uint32_t cn, uint32_t ct, uint32_t ci) {
   uint4 h = make_uint4(RDRT_MISS, 0u, 0u, 0u);
   uint4 hm = make_uint4(RDRT_MISS, RDRT_MISS, 0u, RDRT_MISS);
   if (best.k != kInvalid) {
     const GeoRec &g = p.geos[best.geo];
-    h.x = p.redirect[best.k] - g.primitive_start;
+    h.x = ids[best.k] - g.primitive_start;
     h.y = __float_as_uint(best.t);
     h.z = __float_as_uint(best.u);
     h.w = __float_as_uint(best.v);
@@ -351,7 +352,7 @@ __device__ __forceinline__ void trace_fast_one(const TraceParams &p, uint32_t ri
     }
   }
   best.t = far_;  // far shrinks to each accepted hit
-  write_result(p, rid, best, cn, ct, ci);
+  write_result(p, p.tri_ids, rid, best, cn, ct, ci);
 }
 
 // One ray per thread over a grid sized for p.n; the live count may be

@@ -29,7 +29,8 @@ struct TraceParams {
 
   const PairNode *tri_pairs;
   const TriPre *tris;
-  const uint32_t *redirect;
+  const uint32_t *tri_ids;   // TriPre slot -> global triangle (fast traversal)
+  const uint32_t *redirect;  // forest slot -> global triangle (reference traversal)
   const GeoRec *geos;
   const PairNode *tlas_pairs;
   const InstRec *insts;

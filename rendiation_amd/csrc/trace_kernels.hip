@@ -106,7 +106,7 @@ __global__ __launch_bounds__(B) void k_trace_ref(TraceParams p) {
       }
     }
   }
-  write_result(p, rid, best, cn, ct, ci);
+  write_result(p, p.redirect, rid, best, cn, ct, ci);
 }
 
 constexpr uint32_t kStackSizes[] = {8, 16, 24, 32, 48, 64};
