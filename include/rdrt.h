@@ -211,10 +211,12 @@ typedef struct rdrt_tune {
   uint32_t lds_entries; /* traversal stack entries per lane in LDS (8, 16); deeper ones in private memory */
   uint32_t waves;       /* 8 = compile for 8 waves per SIMD (at most 64 VGPRs); 0 = the compiler's choice */
   uint32_t persistent;  /* 1 = resident-size grid whose waves pull 64-ray chunks from a counter */
+  uint32_t cull;        /* 1 = a first pass finishes rays that miss their scene's root box and queues the rest */
   float ms;             /* rdrt_autotune: per-launch time of the chosen shape on the sample */
 } rdrt_tune;
 /* Times every shape of the search space (3 block sizes x 2 LDS stack depths
- * x 2 occupancy targets x grid / persistent) on a sample launch (the args of
+ * x 2 occupancy targets x grid / persistent x with / without the cull pass)
+ * on a sample launch (the args of
  * rdrt_trace; its hits are overwritten), `reps` timed launches each, and
  * keeps the fastest for every later traversal of this context (rdrt_trace,
  * the path tracer, batched tracing).  Synchronizes.  best may be NULL. */

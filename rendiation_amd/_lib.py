@@ -94,7 +94,8 @@ class RdrtBatchStats(C.Structure):
 
 
 class RdrtTune(C.Structure):
-    _fields_ = [("block_size", u32), ("lds_entries", u32), ("waves", u32), ("persistent", u32), ("ms", f32)]
+    _fields_ = [("block_size", u32), ("lds_entries", u32), ("waves", u32), ("persistent", u32), ("cull", u32),
+                ("ms", f32)]
 
 
 RDRT_BATCH_PERSISTENT = 0

@@ -160,6 +160,7 @@ int32_t rdrt_ctx_destroy(rdrt_ctx *c) {
     c->batch.free_all();
     c->compact_scratch.free();
     c->trace_work.free();
+    c->trace_cull.free();
     if (c->owns_work_stream) (void)hipStreamDestroy(c->work_stream);
     if (c->status) (void)hipFree(c->status);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -362,6 +363,29 @@ int32_t rdrt_copy_built(const rdrt_ctx *c, uint32_t which, void *out, size_t byt
   return RDRT_OK;
 }
 
+}  // extern "C"
+
+namespace rdrt {
+
+// Carves the cull pass's scratch out of one grow-only buffer of the context.
+int32_t cull_scratch(rdrt_ctx *c, uint32_t n, CullScratch &cs) {
+  RDRT_HIP(c->trace_cull.reserve(cull_scratch_words(n) * 4));
+  const size_t nb = ((size_t)n + kCullBlock - 1) / kCullBlock;
+  uint32_t *w = (uint32_t *)c->trace_cull.ptr;
+  cs.tmp = w;
+  cs.blk = w + nb * kCullBlock;
+  cs.queue = cs.blk + 2 * nb;
+  cs.count = cs.queue + n;
+  cs.later = nullptr;
+  cs.later_rounds = 0;
+  cs.stride = n;
+  return RDRT_OK;
+}
+
+}  // namespace rdrt
+
+extern "C" {
+
 int32_t rdrt_trace(rdrt_ctx *c, const rdrt_trace_args *a) {
   if (!c || !a) return fail(RDRT_ERR_INVALID_ARG, "null argument");
   int32_t rc = need_device(c);
@@ -386,8 +410,15 @@ int32_t rdrt_trace(rdrt_ctx *c, const rdrt_trace_args *a) {
     RDRT_HIP(c->trace_work.reserve(4));
     p.work = (uint32_t *)c->trace_work.ptr;
   }
+  const bool cull = c->tune.cull && a->traversal == RDRT_TRAVERSAL_FAST;
+  CullScratch cs{};
+  if (cull) {
+    rc = cull_scratch(c, a->n, cs);
+    if (rc) return rc;
+  }
   RDRT_HIP(hipEventRecord(c->ev0, c->stream));
-  RDRT_HIP(launch_trace(p, a->traversal, push, c->tune, c->stream));
+  if (cull) RDRT_HIP(launch_trace_culled(p, push, c->tune, cs, c->stream));
+  else RDRT_HIP(launch_trace(p, a->traversal, push, c->tune, c->stream));
   RDRT_HIP(hipEventRecord(c->ev1, c->stream));
   c->timed = true;
   return RDRT_OK;
@@ -396,12 +427,14 @@ int32_t rdrt_trace(rdrt_ctx *c, const rdrt_trace_args *a) {
 int32_t rdrt_set_tune(rdrt_ctx *c, const rdrt_tune *t) {
   if (!c || !t) return fail(RDRT_ERR_INVALID_ARG, "null argument");
   if ((t->block_size != 64 && t->block_size != 128 && t->block_size != 256) ||
-      (t->lds_entries != 8 && t->lds_entries != 16) || (t->waves != 0 && t->waves != 8) || t->persistent > 1)
-    return fail(RDRT_ERR_INVALID_ARG, "block_size 64/128/256, lds_entries 8/16, waves 0/8, persistent 0/1");
+      (t->lds_entries != 8 && t->lds_entries != 16) || (t->waves != 0 && t->waves != 8) || t->persistent > 1 ||
+      t->cull > 1)
+    return fail(RDRT_ERR_INVALID_ARG, "block_size 64/128/256, lds_entries 8/16, waves 0/8, persistent 0/1, cull 0/1");
   c->tune.block = t->block_size;
   c->tune.lds = t->lds_entries;
   c->tune.waves = t->waves;
   c->tune.persist = t->persistent;
+  c->tune.cull = t->cull;
   return RDRT_OK;
 }
 
@@ -412,6 +445,7 @@ int32_t rdrt_get_tune(rdrt_ctx *c, rdrt_tune *t) {
   t->lds_entries = c->tune.lds;
   t->waves = c->tune.waves;
   t->persistent = c->tune.persist;
+  t->cull = c->tune.cull;
   t->ms = c->tune_ms;
   return RDRT_OK;
 }
@@ -435,21 +469,28 @@ int32_t rdrt_autotune(rdrt_ctx *c, const rdrt_trace_args *a, uint32_t reps, rdrt
   p.tlas_idx = a->tlas_idx;
   RDRT_HIP(c->trace_work.reserve(4));
   p.work = (uint32_t *)c->trace_work.ptr;
+  CullScratch cs{};
+  rc = cull_scratch(c, a->n, cs);
+  if (rc) return rc;
   hipStream_t st = c->stream;
   TraceTune chosen = c->tune;
   float chosen_ms = -1.0f;
   for (uint32_t block : {64u, 128u, 256u})
     for (uint32_t lds : {8u, 16u})
       for (uint32_t waves : {0u, 8u})
-        for (uint32_t persist : {0u, 1u}) {
+        for (uint32_t shape : {0u, 1u, 2u, 3u}) {
           TraceTune t;
           t.block = block;
           t.lds = lds;
           t.waves = waves;
-          t.persist = persist;
-          RDRT_HIP(launch_trace(p, RDRT_TRAVERSAL_FAST, push, t, st));  // warm-up
+          t.persist = shape & 1u;
+          t.cull = shape >> 1;
+          auto run = [&]() -> hipError_t {
+            return t.cull ? launch_trace_culled(p, push, t, cs, st) : launch_trace(p, RDRT_TRAVERSAL_FAST, push, t, st);
+          };
+          RDRT_HIP(run());  // warm-up
           RDRT_HIP(hipEventRecord(c->ev0, st));
-          for (uint32_t r = 0; r < reps; r++) RDRT_HIP(launch_trace(p, RDRT_TRAVERSAL_FAST, push, t, st));
+          for (uint32_t r = 0; r < reps; r++) RDRT_HIP(run());
           RDRT_HIP(hipEventRecord(c->ev1, st));
           RDRT_HIP(hipEventSynchronize(c->ev1));
           float ms = 0.0f;

@@ -158,11 +158,12 @@ int32_t rdrt_trace_batched(rdrt_ctx *c, const rdrt_batch_params *a) {
   RDRT_HIP(hipEventRecord(c->ev0, st));
   if (a->mode == RDRT_BATCH_PERSISTENT) {
     // graph key: every parameter and buffer the captured launches use
-    std::vector<uint8_t> key(sizeof(*a) + 8 * sizeof(void *));
+    std::vector<uint8_t> key(sizeof(*a) + 8 * sizeof(void *) + sizeof(TraceTune));
     std::memcpy(key.data(), a, sizeof(*a));
     const void *ptrs[8] = {s.wrays.ptr, s.hitx.ptr, s.qa.ptr, s.qb.ptr, s.counts.ptr, s.tmp.ptr, s.blk.ptr,
                            s.work.ptr};
     std::memcpy(key.data() + sizeof(*a), ptrs, sizeof(ptrs));
+    std::memcpy(key.data() + sizeof(*a) + sizeof(ptrs), &c->tune, sizeof(TraceTune));  // kernels captured
     if (!s.exec || key != s.cached) {
       if (s.exec) (void)hipGraphExecDestroy(s.exec);
       if (s.graph) (void)hipGraphDestroy(s.graph);
@@ -170,8 +171,32 @@ int32_t rdrt_trace_batched(rdrt_ctx *c, const rdrt_batch_params *a) {
       s.graph = nullptr;
       uint32_t launches = 1;
       RDRT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      hipError_t e = enqueue_batch_init((uint32_t *)s.qa.ptr, 0u, a->n, (uint32_t *)s.counts.ptr,
-                                        (uint32_t *)s.counts.ptr + 1, pl.rounds, st);
+      hipError_t e;
+      if (c->tune.cull) {
+        // round 0's queue from the cull pass: rays that miss their scene's
+        // root box get miss records for every round and are not queued
+        e = enqueue_batch_init((uint32_t *)s.qa.ptr, 0u, 0u, (uint32_t *)s.work.ptr, (uint32_t *)s.counts.ptr + 1,
+                               pl.rounds, st);
+        TraceParams tp = scene_trace_params(c);
+        tp.rays = a->rays;
+        tp.meta = a->ray_meta;
+        tp.hits = a->hits;
+        tp.hitx = (uint2 *)s.hitx.ptr;
+        tp.n = a->n;
+        tp.cull_mask = 0xffffffffu;
+        CullScratch cs;
+        cs.tmp = (uint32_t *)s.tmp.ptr;
+        cs.blk = (uint32_t *)s.blk.ptr;
+        cs.queue = (uint32_t *)s.qa.ptr;
+        cs.count = (uint32_t *)s.counts.ptr;
+        cs.later = a->hits;
+        cs.later_rounds = pl.rounds - 1;
+        cs.stride = a->n;
+        if (e == hipSuccess) e = enqueue_cull(tp, cs, st);
+      } else {
+        e = enqueue_batch_init((uint32_t *)s.qa.ptr, 0u, a->n, (uint32_t *)s.counts.ptr, (uint32_t *)s.counts.ptr + 1,
+                               pl.rounds, st);
+      }
       if (e == hipSuccess) e = enqueue_scene(c, a, pl, 0, true, launches);
       hipGraph_t g = nullptr;
       hipError_t e2 = hipStreamEndCapture(st, &g);

@@ -140,6 +140,7 @@ struct rdrt_ctx {
   rdrt::Scratch compact_scratch;
   rdrt::TraceTune tune;  // launch shape of the fast traversal (config block size, rdrt_autotune)
   rdrt::Scratch trace_work;  // work counter of persistent rdrt_trace launches
+  rdrt::Scratch trace_cull;  // cull pass scratch of rdrt_trace
   float tune_ms = 0.0f;      // per-launch time of the autotuned choice
 };
 
@@ -150,6 +151,7 @@ void invalidate(rdrt_ctx *c);
 // Device view of the committed scene for the trace kernels (no rays).
 TraceParams scene_trace_params(const rdrt_ctx *c);
 uint32_t push_depth(const rdrt_ctx *c);
+int32_t cull_scratch(rdrt_ctx *c, uint32_t n, CullScratch &cs);
 
 }  // namespace rdrt
 

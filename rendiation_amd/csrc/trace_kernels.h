@@ -56,7 +56,28 @@ struct TraceTune {
   uint32_t lds = 16;     // stack entries per lane kept in LDS: 8 or 16 (deeper ones in private memory)
   uint32_t waves = 0;    // 8 = ask the register allocator for 8 waves per SIMD; 0 = its own choice
   uint32_t persist = 0;  // 1 = resident-size grid whose waves pull 64-ray chunks from a counter
+  uint32_t cull = 0;     // 1 = a first pass finishes the rays that miss their scene's root box and
+                         //     queues the others densely for the traversal
 };
+
+// Scratch of a culled launch: tmp (slots words), blk (2 * ceil(slots /
+// kCullBlock) words), queue (slots words), count (1 word).
+constexpr uint32_t kCullBlock = 256;
+struct CullScratch {
+  uint32_t *tmp, *blk, *queue, *count;
+  // rays that miss also get miss records in `later_rounds` further hit
+  // arrays spaced `stride` records apart (batched rounds); 0 = none
+  rdrt_hit *later;
+  uint32_t later_rounds, stride;
+};
+size_t cull_scratch_words(uint32_t slots);
+
+// The cull pass and the scan and scatter of its queue into cs.queue /
+// cs.count (rays 0..p.n-1; p must have no queue of its own).
+hipError_t enqueue_cull(const TraceParams &p, const CullScratch &cs, hipStream_t stream);
+// enqueue_cull, then the fast traversal over the queue.
+hipError_t launch_trace_culled(const TraceParams &p, uint32_t push_depth, const TraceTune &tune,
+                               const CullScratch &cs, hipStream_t stream);
 
 // Enqueues one traversal launch on `stream`.  mode = RDRT_TRAVERSAL_*.  A
 // persistent launch needs p.work (zeroed by a kernel launched first, so the

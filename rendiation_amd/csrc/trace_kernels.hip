@@ -4,6 +4,7 @@
 #include <map>
 #include <mutex>
 
+#include "compact_device.h"
 #include "trace_device.h"
 
 namespace rdrt {
@@ -111,7 +112,62 @@ __global__ __launch_bounds__(B) void k_trace_ref(TraceParams p) {
 
 constexpr uint32_t kStackSizes[] = {8, 16, 24, 32, 48, 64};
 
+// Cull pass: a ray whose range misses the root box of its TLAS (or whose
+// TLAS is unbound) cannot hit anything; it gets its miss record here (what
+// the traversal would write: no node visited), and the others are compacted
+// stably into a dense queue, so traversal waves carry no idle lanes from
+// the start.
+__global__ __launch_bounds__(kCullBlock) void k_trace_cull(TraceParams p, CullScratch cs) {
+  const uint32_t bid = blockIdx.x, i = bid * kCullBlock + threadIdx.x;
+  bool keep = false;
+  if (i < p.n) {
+    const RayIn r = load_ray(p, i);
+    const TlasRoot *root = bound_root(p, r.tlas_idx);
+    float tn;
+    keep = root && slab(r.o, recip(r.d), r.tmin, r.tmax, root->mn[0], root->mn[1], root->mn[2], root->mx[0],
+                        root->mx[1], root->mx[2], tn);
+    if (!keep) {
+      write_result(p, p.tri_ids, i, Best{}, 0u, 0u, 0u);
+      for (uint32_t k = 0; k < cs.later_rounds; k++)
+        *reinterpret_cast<uint4 *>(cs.later + (size_t)(k + 1) * cs.stride + i) = make_uint4(RDRT_MISS, 0u, 0u, 0u);
+    }
+  }
+  compact::block_compact2<kCullBlock>(keep, false, i, bid, cs.tmp, nullptr, cs.blk, nullptr);
+}
+
 }  // namespace
+
+size_t cull_scratch_words(uint32_t slots) {
+  const size_t nb = ((size_t)slots + kCullBlock - 1) / kCullBlock;
+  return nb * kCullBlock + 2 * nb + (size_t)slots + 1;
+}
+
+hipError_t enqueue_cull(const TraceParams &p, const CullScratch &cs, hipStream_t st) {
+  if (p.n == 0) return hipSuccess;
+  if (p.queue || p.count) return hipErrorInvalidValue;
+  const uint32_t nb = (p.n + kCullBlock - 1) / kCullBlock;
+  hipLaunchKernelGGL(k_trace_cull, dim3(nb), dim3(kCullBlock), 0, st, p, cs);
+  hipLaunchKernelGGL(compact::k_block_scan, dim3(1), dim3(compact::kScanThreads), 0, st, (const uint32_t *)nullptr,
+                     p.n, kCullBlock, (const uint32_t *)cs.blk, (const uint32_t *)nullptr, cs.blk + nb,
+                     (uint32_t *)nullptr, cs.count, (uint32_t *)nullptr);
+  hipLaunchKernelGGL((compact::k_scatter2<kCullBlock>), dim3(nb), dim3(kCullBlock), 0, st, (const uint32_t *)nullptr,
+                     p.n, (const uint32_t *)cs.tmp, (const uint32_t *)nullptr, (const uint32_t *)cs.blk,
+                     (const uint32_t *)nullptr, (const uint32_t *)(cs.blk + nb), (const uint32_t *)nullptr, cs.queue,
+                     (uint32_t *)nullptr, p.n);
+  return hipGetLastError();
+}
+
+hipError_t launch_trace_culled(const TraceParams &p, uint32_t push_depth, const TraceTune &t, const CullScratch &cs,
+                               hipStream_t st) {
+  if (p.n == 0) return hipSuccess;
+  hipError_t e = enqueue_cull(p, cs, st);
+  if (e != hipSuccess) return e;
+  TraceParams q = p;
+  q.queue = cs.queue;
+  q.count = cs.count;
+  q.slots = p.n;
+  return launch_trace(q, RDRT_TRAVERSAL_FAST, push_depth, t, st);
+}
 
 // Resident blocks of a kernel on the current device (cached per device and kernel).
 uint32_t resident_grid(const void *kernel, int block) {

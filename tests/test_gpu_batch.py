@@ -174,3 +174,23 @@ def test_invalid_batch_arguments(batch_case, device):
         _run(sys, rays, meta, device, BATCH_SEQUENTIAL, offs=bad)
     with pytest.raises(RdrtError):
         _run(sys, rays, meta, device, BATCH_PERSISTENT, bounces=17)
+
+
+def test_cull_pass_equals_plain_rounds(batch_case, device):
+    """With the cull pass (rays missing their scene's root box finish before
+    the traversal, the rest are queued densely), every round is bit-identical
+    to the plain identity queue."""
+    sys, _, _, rays, meta, _ = batch_case
+    _say("batched: cull pass vs identity queue")
+    hp, sp, stp = _run(sys, rays, meta, device, BATCH_PERSISTENT)
+    sys.set_tune(cull=1)
+    try:
+        hc, sc, stc = _run(sys, rays, meta, device, BATCH_PERSISTENT)
+    finally:
+        sys.set_tune()
+    assert np.array_equal(hp, hc)
+    for r in range(BOUNCES):
+        alive = hp[r][:, 0] != MISS
+        assert np.array_equal(sp[r][alive], sc[r][alive]), r
+    assert stc.rays[0] < stp.rays[0]  # round 0 traverses only the rays that enter a scene box
+    assert stc.rays[1:] == stp.rays[1:]

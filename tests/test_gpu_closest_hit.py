@@ -152,7 +152,7 @@ def test_launch_shapes_agree_and_autotune(closest_hit_scene, device):
     base, _, _ = _gpu_trace(sys, rays_np, device)
     rays = torch.from_numpy(np.ascontiguousarray(rays_np)).to(device)
     hits = torch.empty((rays_np.shape[0], 4), dtype=torch.int32, device=device)
-    for shape in [(64, 8, 0, 0), (128, 16, 8, 1), (256, 8, 8, 0), (256, 16, 0, 1)]:
+    for shape in [(64, 8, 0, 0, 0), (128, 16, 8, 1, 0), (256, 8, 8, 0, 1), (256, 16, 0, 1, 1), (64, 16, 8, 0, 1)]:
         sys.set_tune(*shape)
         sys.trace(rays, hits)
         sys.sync()
