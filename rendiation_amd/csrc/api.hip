@@ -180,8 +180,7 @@ int32_t rdrt_sync(rdrt_ctx *c) {
   RDRT_HIP(hipMemcpy(&st, c->status, sizeof(st), hipMemcpyDeviceToHost));
   if (st) {
     RDRT_HIP(hipMemset(c->status, 0, sizeof(uint32_t)));
-    return fail(RDRT_ERR_CAPACITY, (st & 1u) ? "a traversal stack overflowed; results of the last launch are incomplete"
-                                             : "a queue compaction did not complete; results of the last launch are invalid");
+    return fail(RDRT_ERR_CAPACITY, "a traversal stack overflowed; results of the last launch are incomplete");
   }
   return RDRT_OK;
 }
