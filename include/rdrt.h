@@ -277,10 +277,16 @@ typedef struct rdrt_pt_params {
   const float *view_params;
 } rdrt_pt_params;
 
-/* time every traversal launch of the frame with HIP events */
+/* time every traversal launch of the frame with HIP events (a frame traces
+ * the shadow rays of bounce b and the extension rays of bounce b + 1 in one
+ * launch: max_bounces + 1 launches) */
 #define RDRT_PT_DIAG_TIME_TRACES 0x1u
-/* trace in the reference order and count node / triangle / instance visits */
+/* trace in the reference order, one launch per ray set (2 * max_bounces + 1),
+ * and count node / triangle / instance visits */
 #define RDRT_PT_DIAG_COUNT_VISITS 0x2u
+/* as RDRT_PT_DIAG_COUNT_VISITS with the fast traversal: its own tree and
+ * order (node-pair visits, triangle tests, instance tests) */
+#define RDRT_PT_DIAG_COUNT_FAST 0x4u
 
 typedef struct rdrt_pt_stats {
   uint32_t bounces;             /* entries below that are valid: max_bounces + 1 */
@@ -289,7 +295,9 @@ typedef struct rdrt_pt_stats {
   float frame_ms;               /* HIP-event time of the last frame */
   float trace_ms;               /* sum over its traversal launches (RDRT_PT_DIAG_TIME_TRACES) */
   uint32_t trace_launches;
-  uint64_t visits[3];           /* node visits, triangle tests, instance tests (RDRT_PT_DIAG_COUNT_VISITS) */
+  uint64_t visits[3];           /* node visits, triangle tests, instance tests (RDRT_PT_DIAG_COUNT_*) */
+  uint32_t max_ray_visits[33];  /* per launch in frame order: the largest visit sum of one ray (RDRT_PT_DIAG_COUNT_*) */
+  uint32_t pad;
 } rdrt_pt_stats;
 
 /* Renders one frame: radiance_dev receives views*width*height RGB f32 (the

@@ -76,10 +76,12 @@ class RdrtPtParams(C.Structure):
 
 class RdrtPtStats(C.Structure):
     _fields_ = [("bounces", u32), ("extension_rays", u32 * 17), ("shadow_rays", u32 * 17), ("frame_ms", f32),
-                ("trace_ms", f32), ("trace_launches", u32), ("visits", C.c_uint64 * 3)]
+                ("trace_ms", f32), ("trace_launches", u32), ("visits", C.c_uint64 * 3),
+                ("max_ray_visits", u32 * 33), ("pad", u32)]
 
 
 RDRT_PT_DIAG_TIME_TRACES = 0x1
+RDRT_PT_DIAG_COUNT_FAST = 0x4
 RDRT_PT_DIAG_COUNT_VISITS = 0x2
 
 

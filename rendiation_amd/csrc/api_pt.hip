@@ -232,8 +232,11 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
 
   const TraceParams tp = scene_trace_params(c);
   hipStream_t st = c->work_stream;
-  if (prm->diagnostics & ~(RDRT_PT_DIAG_TIME_TRACES | RDRT_PT_DIAG_COUNT_VISITS))
+  if (prm->diagnostics & ~(RDRT_PT_DIAG_TIME_TRACES | RDRT_PT_DIAG_COUNT_VISITS | RDRT_PT_DIAG_COUNT_FAST))
     return fail(RDRT_ERR_INVALID_ARG, "unknown diagnostics bits");
+  const uint32_t counting = prm->diagnostics & (RDRT_PT_DIAG_COUNT_VISITS | RDRT_PT_DIAG_COUNT_FAST);
+  if (counting == (RDRT_PT_DIAG_COUNT_VISITS | RDRT_PT_DIAG_COUNT_FAST))
+    return fail(RDRT_ERR_INVALID_ARG, "count visits in one order per frame");
   PtDiag diag;
   if (prm->diagnostics) {
     const size_t launches = 2 * (kPtMaxBounces + 1);
@@ -248,15 +251,19 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
       diag.ev_begin = s.ev_begin.data();
       diag.ev_end = s.ev_end.data();
     }
-    if (prm->diagnostics & RDRT_PT_DIAG_COUNT_VISITS) {
+    if (counting) {
       RDRT_HIP(s.diag_counters.reserve((size_t)npaths * 12));
-      RDRT_HIP(s.diag_totals.reserve(3 * 8));
+      RDRT_HIP(s.diag_totals.reserve(3 * 8 + (2 * kPtMaxBounces + 1) * 4));
       diag.counters = (uint32_t *)s.diag_counters.ptr;
       diag.totals = (unsigned long long *)s.diag_totals.ptr;
+      diag.max_visits = (uint32_t *)((char *)s.diag_totals.ptr + 3 * 8);
+      diag.fast_order = counting == RDRT_PT_DIAG_COUNT_FAST;
     }
   }
   s.last_diag = prm->diagnostics;
-  s.last_launches = 2 * prm->max_bounces + 1;
+  // shadow rays of bounce b share a launch with extension rays of b + 1,
+  // except in visit-counting frames
+  s.last_launches = counting ? 2 * prm->max_bounces + 1 : prm->max_bounces + 1;
   RDRT_HIP(hipEventRecord(c->ev0, st));
   if (prm->diagnostics) {
     RDRT_HIP(enqueue_pt_frame(L, tp, push_depth(c), c->tune, st, &diag));
@@ -370,16 +377,19 @@ int32_t rdrt_pt_get_stats(rdrt_ctx *c, rdrt_pt_stats *o) {
     o->shadow_rays[b] = b < mb ? counts[mb + 2 + b] : 0u;
   }
   RDRT_HIP(hipEventElapsedTime(&o->frame_ms, c->ev0, c->ev1));
+  o->trace_launches = c->pt.last_launches;
   if (c->pt.last_diag & RDRT_PT_DIAG_TIME_TRACES) {
-    o->trace_launches = c->pt.last_launches;
     for (uint32_t i = 0; i < c->pt.last_launches; i++) {
       float ms = 0.0f;
       RDRT_HIP(hipEventElapsedTime(&ms, c->pt.ev_begin[i], c->pt.ev_end[i]));
       o->trace_ms += ms;
     }
   }
-  if (c->pt.last_diag & RDRT_PT_DIAG_COUNT_VISITS)
+  if (c->pt.last_diag & (RDRT_PT_DIAG_COUNT_VISITS | RDRT_PT_DIAG_COUNT_FAST)) {
     RDRT_HIP(hipMemcpy(o->visits, c->pt.diag_totals.ptr, 24, hipMemcpyDeviceToHost));
+    RDRT_HIP(hipMemcpy(o->max_ray_visits, (char *)c->pt.diag_totals.ptr + 24, c->pt.last_launches * 4,
+                       hipMemcpyDeviceToHost));
+  }
   return RDRT_OK;
 }
 

@@ -65,8 +65,10 @@ constexpr uint32_t kVertSkyUp = 0xFFFFFFF0u, kVertSkyDown = 0xFFFFFFF1u, kVertEn
 // Optional per-frame diagnostics (eager frames only).
 struct PtDiag {
   hipEvent_t *ev_begin = nullptr, *ev_end = nullptr;  // one pair per traversal launch, or null
-  uint32_t *counters = nullptr;                       // 3 u32 per path: reference-order visits, or null
+  uint32_t *counters = nullptr;                       // 3 u32 per path: visit counters, or null
   unsigned long long *totals = nullptr;               // 3 u64 sums of the counters
+  uint32_t *max_visits = nullptr;                     // per launch: largest visit sum of one ray
+  bool fast_order = false;                            // count the fast traversal's visits
 };
 
 // Enqueues one frame on `stream` (capturable when diag is null: no memset

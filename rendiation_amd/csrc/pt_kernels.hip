@@ -324,20 +324,34 @@ __global__ void k_fill_u64(u64 *p, uint32_t n) {
 
 __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
 
+__global__ void k_fill_u32(uint32_t *p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
 // Sums the per-ray visit counters of the rays of one queued launch.
 __global__ __launch_bounds__(kPtBlock) void k_sum_visits(const uint32_t *queue, const uint32_t *count, uint32_t n,
-                                                         const uint32_t *counters, u64 *totals) {
+                                                         const uint32_t *counters, u64 *totals, uint32_t *max_visits) {
   __shared__ u64 s[3];
+  __shared__ uint32_t m;
   if (threadIdx.x < 3) s[threadIdx.x] = 0ull;
+  if (threadIdx.x == 0) m = 0u;
   __syncthreads();
   const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
   if (i < n && i < *count) {
     const uint32_t r = queue[i];
-    if (r < n)
-      for (int k = 0; k < 3; k++) atomicAdd(&s[k], (u64)counters[3 * (size_t)r + k]);
+    if (r < n) {
+      uint32_t sum = 0;
+      for (int k = 0; k < 3; k++) {
+        const uint32_t c = counters[3 * (size_t)r + k];
+        atomicAdd(&s[k], (u64)c);
+        sum += c;
+      }
+      atomicMax(&m, sum);
+    }
   }
   __syncthreads();
   if (threadIdx.x < 3 && s[threadIdx.x]) atomicAdd(&totals[threadIdx.x], s[threadIdx.x]);
+  if (threadIdx.x == 0 && m) atomicMax(max_visits, m);
 }
 
 __global__ __launch_bounds__(kPtBlock) void k_compact(const uint32_t *mask, uint32_t n, uint32_t *tmp,
@@ -502,7 +516,7 @@ hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_
                             const TraceTune &tune, hipStream_t st, const PtDiag *diag) {
   const uint32_t grid = (P.npaths + kPtBlock - 1) / kPtBlock;
   const bool count = diag && diag->counters;
-  const uint32_t mode = count ? RDRT_TRAVERSAL_REFERENCE : RDRT_TRAVERSAL_FAST;
+  const uint32_t mode = count && !diag->fast_order ? RDRT_TRAVERSAL_REFERENCE : RDRT_TRAVERSAL_FAST;
   uint32_t launch = 0;
   // one traversal launch, optionally timed and visit-counted
   auto trace = [&](TraceParams t) -> hipError_t {
@@ -513,13 +527,11 @@ hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_
     launch++;
     if (e == hipSuccess && count)
       hipLaunchKernelGGL(k_sum_visits, dim3((t.n + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, st, t.queue, t.count,
-                         t.n, diag->counters, diag->totals);
+                         t.n, diag->counters, diag->totals, diag->max_visits + (launch - 1));
     return e;
   };
-  if (count) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, diag->totals, 3u);
-  hipLaunchKernelGGL(k_pt_begin, dim3(256), dim3(256), 0, st, P);
-  hipLaunchKernelGGL(k_pt_camera, dim3(grid), dim3(kPtBlock), 0, st, P);
-  for (uint32_t b = 0; b <= P.max_bounces; b++) {
+  // extension rays of bounce b and shadow rays of bounce b
+  auto extension = [&](uint32_t b) {
     TraceParams tp = scene;
     tp.rays = P.rays;
     tp.hits = P.hits;
@@ -534,19 +546,48 @@ hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_
     tp.cull_mask = 0xffffffffu;
     tp.tlas_idx = 0;
     tp.work = P.work;
-    hipError_t e = trace(tp);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_pt_shade<kPtBlock, false>), dim3(grid), dim3(kPtBlock), 0, st, P, b);
-    enqueue_queue_scatter(P, b, grid, st);
-    if (b == P.max_bounces) break;
-    TraceParams sp = tp;
+    return tp;
+  };
+  auto shadow = [&](uint32_t b) {
+    TraceParams sp = extension(b);
     sp.rays = P.shadow_rays;
     sp.hits = P.shadow_hits;
     sp.hitx = nullptr;
     sp.queue = P.qs;
     sp.count = P.counts + P.max_bounces + 2 + b;
     sp.flags = RDRT_RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH;  // ray_hit.rs:33-45
-    e = trace(sp);
+    return sp;
+  };
+  if (count) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, diag->totals, 3u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, diag->max_visits, 2u * kPtMaxBounces + 1u);
+  }
+  hipLaunchKernelGGL(k_pt_begin, dim3(256), dim3(256), 0, st, P);
+  hipLaunchKernelGGL(k_pt_camera, dim3(grid), dim3(kPtBlock), 0, st, P);
+  hipError_t e = trace(extension(0));
+  if (e != hipSuccess) return e;
+  for (uint32_t b = 0; b <= P.max_bounces; b++) {
+    hipLaunchKernelGGL((k_pt_shade<kPtBlock, false>), dim3(grid), dim3(kPtBlock), 0, st, P, b);
+    enqueue_queue_scatter(P, b, grid, st);
+    if (b == P.max_bounces) break;
+    // The shadow rays of bounce b and the extension rays of bounce b + 1
+    // both come from shade(b) and are independent: one launch traces both
+    // (visit counting keeps them apart, one launch each).
+    const TraceParams sp = shadow(b), ep = extension(b + 1);
+    if (count) {
+      e = trace(sp);
+      if (e == hipSuccess) e = trace(ep);
+    } else {
+      TraceParams two = sp;
+      two.rays2 = ep.rays;
+      two.hits2 = ep.hits;
+      two.hitx2 = ep.hitx;
+      two.queue2 = ep.queue;
+      two.count2 = ep.count;
+      two.slots2 = P.npaths;
+      two.flags2 = ep.flags;
+      e = trace(two);
+    }
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_pt_nee, dim3(grid), dim3(kPtBlock), 0, st, P, b);
   }

@@ -355,15 +355,41 @@ __device__ __forceinline__ void trace_fast_one(const TraceParams &p, uint32_t ri
   write_result(p, p.tri_ids, rid, best, cn, ct, ci);
 }
 
-// One ray per thread over a grid sized for p.n; the live count may be
-// smaller (device queue size).
+// Live slots of the first and the second segment of a launch.
+__device__ __forceinline__ uint32_t live_first(const TraceParams &p) {
+  return p.count ? min(slots_of(p), *p.count) : slots_of(p);
+}
+__device__ __forceinline__ uint32_t live_second(const TraceParams &p) {
+  return p.count2 ? min(p.slots2, *p.count2) : 0u;
+}
+// The launch parameters of the second segment.
+__device__ __forceinline__ void select_second(TraceParams &p) {
+  p.rays = p.rays2;
+  p.hits = p.hits2;
+  p.hitx = p.hitx2;
+  p.queue = p.queue2;
+  p.flags = p.flags2;
+  p.counters = nullptr;
+}
+
+// One ray per thread over a grid sized for both segments' slots; the live
+// counts may be smaller (device queue sizes).  The blocks of the second
+// segment follow those of the first, so a block's segment is uniform.
 template <int K, int S, int B, int W, bool CNT>
 __global__ __launch_bounds__(B, W) void k_trace_fast(TraceParams p) {
   __shared__ uint32_t lds[K * B];
-  const uint32_t n = p.count ? min(slots_of(p), *p.count) : slots_of(p);
-  const uint32_t used = (n + B - 1) / B;
+  const uint32_t n0 = live_first(p), n1 = live_second(p);
+  const uint32_t b0 = (n0 + B - 1) / B;
+  const uint32_t used = b0 + (n1 + B - 1) / B;
   if (blockIdx.x >= used) return;
-  const uint32_t slot = xcd_block(blockIdx.x, used) * B + threadIdx.x;
+  uint32_t blk = xcd_block(blockIdx.x, used);
+  uint32_t n = n0;
+  if (blk >= b0) {
+    blk -= b0;
+    n = n1;
+    select_second(p);
+  }
+  const uint32_t slot = blk * B + threadIdx.x;
   if (slot >= n) return;
   const uint32_t rid = p.queue ? p.queue[slot] : slot;
   if (rid >= p.n) return;  // a queue entry always names one of the n rays
@@ -374,20 +400,31 @@ __global__ __launch_bounds__(B, W) void k_trace_fast(TraceParams p) {
 // takes chunks of 64 consecutive queue slots from the device counter
 // p.work until the queue is exhausted, so uneven rays and scenes balance
 // across the chip.  Every wave exits once a fetch lands past the count.
+// The second segment's slots start at the first 64-aligned position after
+// the first segment's, so every chunk lies in one segment.
 template <int K, int S, int B, int W, bool CNT>
 __global__ __launch_bounds__(B, W) void k_trace_persist(TraceParams p) {
   __shared__ uint32_t lds[K * B];
-  const uint32_t n = p.count ? min(slots_of(p), *p.count) : slots_of(p);
+  const uint32_t n0 = live_first(p), n1 = live_second(p);
+  const uint32_t start1 = (n0 + 63u) & ~63u;
+  const uint32_t total = n1 ? start1 + n1 : n0;
   const uint32_t lane = threadIdx.x & 63u;
   for (;;) {
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(p.work, 64u);
-    base = __shfl(base, 0, 64);
-    if (base >= n) break;
+    base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));  // wave-uniform: keeps q in scalar registers
+    if (base >= total) break;
+    TraceParams q = p;
+    uint32_t n = n0;
+    if (base >= start1) {
+      base -= start1;
+      n = n1;
+      select_second(q);
+    }
     const uint32_t slot = base + lane;
     if (slot < n) {
-      const uint32_t rid = p.queue ? p.queue[slot] : slot;
-      if (rid < p.n) trace_fast_one<K, S, B, CNT>(p, rid, lds + threadIdx.x);
+      const uint32_t rid = q.queue ? q.queue[slot] : slot;
+      if (rid < q.n) trace_fast_one<K, S, B, CNT>(q, rid, lds + threadIdx.x);
     }
   }
 }
@@ -404,16 +441,16 @@ hipError_t launch_fast_kw(const TraceParams &p, bool persist, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(1), 0, st, p.work);
     if (p.counters) {
       auto k = k_trace_persist<K, S, B, W, true>;
-      uint32_t g = std::min(resident_grid((const void *)k, B), (slots_of(p) + B - 1) / B);
+      uint32_t g = std::min(resident_grid((const void *)k, B), grid_slots(p, B) / B);
       hipLaunchKernelGGL(k, dim3(g), dim3(B), 0, st, p);
     } else {
       auto k = k_trace_persist<K, S, B, W, false>;
-      uint32_t g = std::min(resident_grid((const void *)k, B), (slots_of(p) + B - 1) / B);
+      uint32_t g = std::min(resident_grid((const void *)k, B), grid_slots(p, B) / B);
       hipLaunchKernelGGL(k, dim3(g), dim3(B), 0, st, p);
     }
     return hipGetLastError();
   }
-  dim3 grid((slots_of(p) + B - 1) / B);
+  dim3 grid(grid_slots(p, B) / B);
   if (p.counters) hipLaunchKernelGGL((k_trace_fast<K, S, B, W, true>), grid, dim3(B), 0, st, p);
   else hipLaunchKernelGGL((k_trace_fast<K, S, B, W, false>), grid, dim3(B), 0, st, p);
   return hipGetLastError();

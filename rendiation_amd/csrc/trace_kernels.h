@@ -26,6 +26,18 @@ struct TraceParams {
   uint32_t slots;  // queue slots the grid covers (0 = n); ray ids stay < n
   uint2 *hitx;     // optional: (global geometry index, forest instance index) per hit
   uint32_t *work;  // device work counter of the persistent kernel
+  // Optional second segment traced by the same launch (fast traversal
+  // only): queue2[0 .. min(slots2, *count2)) with its own rays, hits, hit
+  // extras and flags; ray ids index rays2 / hits2 and stay < n.  The
+  // scene, mask, TLAS index and counters are shared.  Used to trace two
+  // independent queues (shadow rays of one bounce and extension rays of
+  // the next) in one launch instead of two.
+  const rdrt_ray *rays2;
+  rdrt_hit *hits2;
+  uint2 *hitx2;
+  const uint32_t *queue2;
+  const uint32_t *count2;  // null = no second segment
+  uint32_t slots2, flags2;
 
   const PairNode *tri_pairs;
   const TriPre *tris;
@@ -45,6 +57,11 @@ struct TraceParams {
 };
 
 __host__ __device__ inline uint32_t slots_of(const TraceParams &p) { return p.slots ? p.slots : p.n; }
+// Queue slots a launch's grid covers: both segments.
+__host__ __device__ inline uint32_t grid_slots(const TraceParams &p, uint32_t block) {
+  const uint32_t a = (slots_of(p) + block - 1) / block, b = p.count2 ? (p.slots2 + block - 1) / block : 0u;
+  return (a + b) * block;
+}
 
 // Smallest stack (entries per lane) that a fast-traversal kernel is built
 // with for the given push depth; 0 if none is large enough.

@@ -195,8 +195,9 @@ uint32_t stack_entries_for(uint32_t push_depth) {
 
 hipError_t launch_trace(const TraceParams &p, uint32_t mode, uint32_t push_depth, const TraceTune &t,
                         hipStream_t stream) {
-  if (p.n == 0 || slots_of(p) == 0) return hipSuccess;
+  if (p.n == 0 || grid_slots(p, 64) == 0) return hipSuccess;
   if (mode == RDRT_TRAVERSAL_REFERENCE) {
+    if (p.count2) return hipErrorInvalidValue;  // one segment per reference launch
     dim3 grid((slots_of(p) + 255) / 256);
     hipLaunchKernelGGL((k_trace_ref<256>), grid, dim3(256), 0, stream, p);
     return hipGetLastError();

@@ -196,3 +196,35 @@ def test_views_in_one_frame(device):
     ref = ref.reshape(3, 40, 64, 3)
     rel = np.sqrt(np.mean((multi - ref) ** 2)) / ref.mean()
     assert rel <= IMAGE_RMSE_REL, rel
+
+
+def test_shared_launches_equal_separate_launches(device):
+    """A frame traces the shadow rays of bounce b with the extension rays of
+    bounce b + 1 in one launch; a fast-order counting frame traces every ray
+    set in a launch of its own.  Both give the same image bit for bit, and
+    the launch counts are max_bounces + 1 and 2 * max_bounces + 1."""
+    import dataclasses
+
+    from rendiation_amd import _lib
+
+    scene = synth.pt_scene(seed=3, n_meshes=16, tess=32, extent=5.0)
+    cam = Camera(position=(0.0, 0.0, 18.0))
+    _say("shared shadow + extension launches vs separate launches")
+    s = _system(scene)
+    pt = PathTracer(s)
+    cfg = PathTracingConfig(width=160, height=90, spp=2, max_bounces=4, use_graph=False,
+                            diagnostics=_lib.RDRT_PT_DIAG_TIME_TRACES)
+    shared = pt.render(cam, cfg).cpu().numpy()
+    st_shared = pt.stats()
+    sep = pt.render(cam, dataclasses.replace(cfg, diagnostics=_lib.RDRT_PT_DIAG_COUNT_FAST)).cpu().numpy()
+    st_sep = pt.stats()
+    assert np.array_equal(shared, sep)
+    assert st_shared["trace_launches"] == cfg.max_bounces + 1
+    assert st_sep["trace_launches"] == 2 * cfg.max_bounces + 1
+    assert st_shared["extension_rays"] == st_sep["extension_rays"]
+    assert st_shared["shadow_rays"] == st_sep["shadow_rays"]
+    assert len(st_sep["max_ray_visits"]) == 2 * cfg.max_bounces + 1
+    assert st_sep["max_ray_visits"][0] > 0 and st_sep["visits"][0] > 0
+    with pytest.raises(Exception):
+        pt.render(cam, dataclasses.replace(cfg, diagnostics=_lib.RDRT_PT_DIAG_COUNT_FAST
+                                                 | _lib.RDRT_PT_DIAG_COUNT_VISITS))
