@@ -94,7 +94,7 @@ def _call_kernels(tune) -> str:
 
 
 def _tune_arg(tune) -> str:
-    return ",".join(str(tune[k]) for k in ("block_size", "lds_entries", "waves", "persistent", "cull"))
+    return ",".join(str(tune[k]) for k in ("block_size", "lds_entries", "waves", "persistent", "cull", "wide"))
 
 
 def _record(value, unit, args, ms_per_step, config, roofline, extra=None):
@@ -854,7 +854,7 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic passes")
     ap.add_argument("--no-autotune", action="store_true", help="closest_hit: keep the default launch shape")
     ap.add_argument("--tune", default="",
-                    help="closest_hit: launch shape block,lds,waves,persistent[,cull] (no autotune)")
+                    help="closest_hit: launch shape block,lds,waves,persistent[,cull[,wide]] (no autotune)")
     ap.add_argument("--profile-only", action="store_true", help="run the timed launches only (counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:

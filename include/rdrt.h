@@ -154,7 +154,9 @@ typedef struct rdrt_trace_args {
 /* Built-scene statistics. */
 typedef struct rdrt_scene_info {
   uint32_t tri_nodes, tlas_nodes, tris, verts, instances, geometries, blas, tlas;
-  uint32_t max_blas_depth, max_tlas_depth, stack_entries, pad;
+  uint32_t max_blas_depth, max_tlas_depth;
+  uint32_t stack_entries;       /* traversal stack per lane of the child-pair layout (0 = too deep) */
+  uint32_t quad_stack_entries;  /* the same for the child-quad layout */
   uint64_t device_bytes;
   double build_ms;
 } rdrt_scene_info;
@@ -212,11 +214,12 @@ typedef struct rdrt_tune {
   uint32_t waves;       /* 8 = compile for 8 waves per SIMD (at most 64 VGPRs); 0 = the compiler's choice */
   uint32_t persistent;  /* 1 = resident-size grid whose waves pull 64-ray chunks from a counter */
   uint32_t cull;        /* 1 = a first pass finishes rays that miss their scene's root box and queues the rest */
+  uint32_t wide;        /* 1 = walk the child-quad layout (four child boxes per 128-byte node) */
   float ms;             /* rdrt_autotune: per-launch time of the chosen shape on the sample */
 } rdrt_tune;
 /* Times every shape of the search space (3 block sizes x 2 LDS stack depths
- * x 2 occupancy targets x grid / persistent x with / without the cull pass)
- * on a sample launch (the args of
+ * x 2 occupancy targets x grid / persistent x with / without the cull pass
+ * x child-pair / child-quad layout) on a sample launch (the args of
  * rdrt_trace; its hits are overwritten), `reps` timed launches each, and
  * keeps the fastest for every later traversal of this context (rdrt_trace,
  * the path tracer, batched tracing).  Synchronizes.  best may be NULL. */

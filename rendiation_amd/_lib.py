@@ -58,7 +58,7 @@ class RdrtTraceArgs(C.Structure):
 class RdrtSceneInfo(C.Structure):
     _fields_ = [("tri_nodes", u32), ("tlas_nodes", u32), ("tris", u32), ("verts", u32),
                 ("instances", u32), ("geometries", u32), ("blas", u32), ("tlas", u32),
-                ("max_blas_depth", u32), ("max_tlas_depth", u32), ("stack_entries", u32), ("pad", u32),
+                ("max_blas_depth", u32), ("max_tlas_depth", u32), ("stack_entries", u32), ("quad_stack_entries", u32),
                 ("device_bytes", C.c_uint64), ("build_ms", C.c_double)]
 
 
@@ -97,7 +97,7 @@ class RdrtBatchStats(C.Structure):
 
 class RdrtTune(C.Structure):
     _fields_ = [("block_size", u32), ("lds_entries", u32), ("waves", u32), ("persistent", u32), ("cull", u32),
-                ("ms", f32)]
+                ("wide", u32), ("ms", f32)]
 
 
 RDRT_BATCH_PERSISTENT = 0

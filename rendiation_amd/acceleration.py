@@ -118,6 +118,7 @@ class SceneInfo:
     max_blas_depth: int
     max_tlas_depth: int
     stack_entries: int
+    quad_stack_entries: int
     device_bytes: int
     build_ms: float
 
@@ -232,7 +233,7 @@ class AccelerationStructureSystem:
         L.check(self._L.rdrt_get_scene_info(self._h, C.byref(info)))
         return SceneInfo(info.tri_nodes, info.tlas_nodes, info.tris, info.verts, info.instances, info.geometries,
                          info.blas, info.tlas, info.max_blas_depth, info.max_tlas_depth, info.stack_entries,
-                         info.device_bytes, info.build_ms)
+                         info.quad_stack_entries, info.device_bytes, info.build_ms)
 
     def copy_built(self, which: str) -> np.ndarray:
         """Built reference-layout arrays: 'blas_forest' / 'tlas_forest' (48 B
@@ -352,11 +353,11 @@ class AccelerationStructureSystem:
         t = L.RdrtTune()
         L.check(self._L.rdrt_get_tune(self._h, C.byref(t)))
         return {"block_size": t.block_size, "lds_entries": t.lds_entries, "waves": t.waves,
-                "persistent": t.persistent, "cull": t.cull, "ms": float(t.ms)}
+                "persistent": t.persistent, "cull": t.cull, "wide": t.wide, "ms": float(t.ms)}
 
     def set_tune(self, block_size: int = 256, lds_entries: int = 16, waves: int = 0, persistent: int = 0,
-                 cull: int = 0) -> None:
-        t = L.RdrtTune(block_size, lds_entries, waves, persistent, cull, 0.0)
+                 cull: int = 0, wide: int = 0) -> None:
+        t = L.RdrtTune(block_size, lds_entries, waves, persistent, cull, wide, 0.0)
         L.check(self._L.rdrt_set_tune(self._h, C.byref(t)))
 
     def sync(self) -> None:

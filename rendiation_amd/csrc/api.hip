@@ -40,6 +40,9 @@ void invalidate(rdrt_ctx *c) {
 void free_device_scene(DeviceScene &d) {
   release(d.tri_pairs);
   release(d.tlas_pairs);
+  release(d.tri_quads);
+  release(d.tlas_quads);
+  release(d.tlas_quad_roots);
   release(d.tris);
   release(d.redirect);
   release(d.tri_ids);
@@ -72,6 +75,9 @@ TraceParams scene_trace_params(const rdrt_ctx *c) {
   p.tri_ids = d.tri_ids.ptr;
   p.geos = d.geos.ptr;
   p.tlas_pairs = d.tlas_pairs.ptr;
+  p.tri_quads = d.tri_quads.ptr;
+  p.tlas_quads = d.tlas_quads.ptr;
+  p.tlas_quad_roots = d.tlas_quad_roots.ptr;
   p.insts = d.insts.ptr;
   p.tlas_roots = d.tlas_roots.ptr;
   p.binding = d.binding.ptr;
@@ -84,7 +90,9 @@ TraceParams scene_trace_params(const rdrt_ctx *c) {
   return p;
 }
 
-uint32_t push_depth(const rdrt_ctx *c) { return c->built.blas_stack + c->built.tlas_stack; }
+uint32_t push_depth(const rdrt_ctx *c, const TraceTune &t) {
+  return t.wide ? c->built.blas_stack4 + c->built.tlas_stack4 : c->built.blas_stack + c->built.tlas_stack;
+}
 
 }  // namespace rdrt
 
@@ -291,6 +299,9 @@ int32_t rdrt_commit(rdrt_ctx *c) {
   DeviceScene &d = c->dev;
   RDRT_HIP(upload(d.tri_pairs, b.tri_pairs, d.bytes));
   RDRT_HIP(upload(d.tlas_pairs, b.tlas_pairs, d.bytes));
+  RDRT_HIP(upload(d.tri_quads, b.tri_quads, d.bytes));
+  RDRT_HIP(upload(d.tlas_quads, b.tlas_quads, d.bytes));
+  RDRT_HIP(upload(d.tlas_quad_roots, b.tlas_quad_roots, d.bytes));
   RDRT_HIP(upload(d.tris, b.tris, d.bytes));
   RDRT_HIP(upload(d.redirect, b.redirect, d.bytes));
   RDRT_HIP(upload(d.tri_ids, b.tri_ids, d.bytes));
@@ -324,6 +335,7 @@ int32_t rdrt_get_scene_info(const rdrt_ctx *c, rdrt_scene_info *o) {
   o->max_blas_depth = b.max_blas_depth;
   o->max_tlas_depth = b.max_tlas_depth;
   o->stack_entries = stack_entries_for(b.blas_stack + b.tlas_stack);
+  o->quad_stack_entries = stack_entries_for(b.blas_stack4 + b.tlas_stack4);
   o->device_bytes = c->dev.bytes;
   o->build_ms = b.build_ms;
   return RDRT_OK;
@@ -402,7 +414,7 @@ int32_t rdrt_trace(rdrt_ctx *c, const rdrt_trace_args *a) {
   p.flags = a->flags;
   p.cull_mask = a->cull_mask;
   p.tlas_idx = a->tlas_idx;
-  uint32_t push = push_depth(c);
+  uint32_t push = push_depth(c, c->tune);
   if (a->traversal == RDRT_TRAVERSAL_FAST && stack_entries_for(push) == 0)
     return fail(RDRT_ERR_CAPACITY, "BVH too deep for the fast traversal stack (use RDRT_TRAVERSAL_REFERENCE)");
   if (c->tune.persist) {
@@ -427,13 +439,19 @@ int32_t rdrt_set_tune(rdrt_ctx *c, const rdrt_tune *t) {
   if (!c || !t) return fail(RDRT_ERR_INVALID_ARG, "null argument");
   if ((t->block_size != 64 && t->block_size != 128 && t->block_size != 256) ||
       (t->lds_entries != 8 && t->lds_entries != 16) || (t->waves != 0 && t->waves != 8) || t->persistent > 1 ||
-      t->cull > 1)
-    return fail(RDRT_ERR_INVALID_ARG, "block_size 64/128/256, lds_entries 8/16, waves 0/8, persistent 0/1, cull 0/1");
-  c->tune.block = t->block_size;
-  c->tune.lds = t->lds_entries;
-  c->tune.waves = t->waves;
-  c->tune.persist = t->persistent;
-  c->tune.cull = t->cull;
+      t->cull > 1 || t->wide > 1)
+    return fail(RDRT_ERR_INVALID_ARG,
+                "block_size 64/128/256, lds_entries 8/16, waves 0/8, persistent 0/1, cull 0/1, wide 0/1");
+  TraceTune n;
+  n.block = t->block_size;
+  n.lds = t->lds_entries;
+  n.waves = t->waves;
+  n.persist = t->persistent;
+  n.cull = t->cull;
+  n.wide = t->wide;
+  if (c->built_valid && stack_entries_for(push_depth(c, n)) == 0)
+    return fail(RDRT_ERR_CAPACITY, "BVH too deep for the fast traversal stack of this layout");
+  c->tune = n;
   return RDRT_OK;
 }
 
@@ -445,6 +463,7 @@ int32_t rdrt_get_tune(rdrt_ctx *c, rdrt_tune *t) {
   t->waves = c->tune.waves;
   t->persistent = c->tune.persist;
   t->cull = c->tune.cull;
+  t->wide = c->tune.wide;
   t->ms = c->tune_ms;
   return RDRT_OK;
 }
@@ -455,8 +474,8 @@ int32_t rdrt_autotune(rdrt_ctx *c, const rdrt_trace_args *a, uint32_t reps, rdrt
   if (rc) return rc;
   if (!c->uploaded) return fail(RDRT_ERR_NOT_BUILT, "scene is not committed (call rdrt_commit)");
   if (a->n == 0 || !a->rays || !a->hits) return fail(RDRT_ERR_INVALID_ARG, "the sample needs rays and hits");
-  const uint32_t push = push_depth(c);
-  if (stack_entries_for(push) == 0) return fail(RDRT_ERR_CAPACITY, "BVH too deep for the fast traversal stack");
+  if (stack_entries_for(push_depth(c, TraceTune{})) == 0)
+    return fail(RDRT_ERR_CAPACITY, "BVH too deep for the fast traversal stack");
   if (reps == 0) reps = 3;
   TraceParams p = scene_trace_params(c);
   p.rays = a->rays;
@@ -477,13 +496,16 @@ int32_t rdrt_autotune(rdrt_ctx *c, const rdrt_trace_args *a, uint32_t reps, rdrt
   for (uint32_t block : {64u, 128u, 256u})
     for (uint32_t lds : {8u, 16u})
       for (uint32_t waves : {0u, 8u})
-        for (uint32_t shape : {0u, 1u, 2u, 3u}) {
+        for (uint32_t shape = 0; shape < 8; shape++) {
           TraceTune t;
           t.block = block;
           t.lds = lds;
           t.waves = waves;
           t.persist = shape & 1u;
-          t.cull = shape >> 1;
+          t.cull = (shape >> 1) & 1u;
+          t.wide = shape >> 2;
+          const uint32_t push = push_depth(c, t);
+          if (stack_entries_for(push) == 0) continue;  // layout too deep for the stack
           auto run = [&]() -> hipError_t {
             return t.cull ? launch_trace_culled(p, push, t, cs, st) : launch_trace(p, RDRT_TRAVERSAL_FAST, push, t, st);
           };

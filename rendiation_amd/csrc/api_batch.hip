@@ -73,7 +73,7 @@ hipError_t enqueue_scene(rdrt_ctx *c, const rdrt_batch_params *a, const Plan &pl
     tp.hits = a->hits + (size_t)r * a->n;
     tp.queue = queues[r & 1];
     tp.count = counts + r;
-    hipError_t e = launch_trace(tp, RDRT_TRAVERSAL_FAST, push_depth(c), tune, c->work_stream);
+    hipError_t e = launch_trace(tp, RDRT_TRAVERSAL_FAST, push_depth(c, c->tune), tune, c->work_stream);
     if (e != hipSuccess) return e;
     launches += persist ? 2 : 1;
     if (r + 1 == R) break;
@@ -130,7 +130,7 @@ int32_t rdrt_trace_batched(rdrt_ctx *c, const rdrt_batch_params *a) {
     for (uint32_t k = 0; k < a->nscenes; k++)
       if (a->scene_offsets[k] > a->scene_offsets[k + 1]) return fail(RDRT_ERR_INVALID_ARG, "offsets must increase");
   }
-  if (stack_entries_for(push_depth(c)) == 0) return fail(RDRT_ERR_CAPACITY, "BVH too deep for the fast traversal");
+  if (stack_entries_for(push_depth(c, c->tune)) == 0) return fail(RDRT_ERR_CAPACITY, "BVH too deep for the fast traversal");
   const Plan pl = make_plan(a);
   BatchState &s = c->batch;
   const size_t n = a->n;

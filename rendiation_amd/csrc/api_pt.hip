@@ -149,7 +149,7 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
     if ((uint64_t)in.custom_index + mx >= c->materials.size())
       return fail(RDRT_ERR_INVALID_ARG, "an instance addresses a material beyond rdrt_set_materials");
   }
-  if (stack_entries_for(push_depth(c)) == 0) return fail(RDRT_ERR_CAPACITY, "BVH too deep for the fast traversal");
+  if (stack_entries_for(push_depth(c, c->tune)) == 0) return fail(RDRT_ERR_CAPACITY, "BVH too deep for the fast traversal");
   const uint32_t npaths = (uint32_t)npaths64;
   rc = reserve_pt(c, npaths);
   if (rc) return rc;
@@ -266,15 +266,16 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
   s.last_launches = counting ? 2 * prm->max_bounces + 1 : prm->max_bounces + 1;
   RDRT_HIP(hipEventRecord(c->ev0, st));
   if (prm->diagnostics) {
-    RDRT_HIP(enqueue_pt_frame(L, tp, push_depth(c), c->tune, st, &diag));
+    RDRT_HIP(enqueue_pt_frame(L, tp, push_depth(c, c->tune), c->tune, st, &diag));
   } else if (prm->use_graph) {
-    if (!s.exec || !s.have_cached || std::memcmp(&s.cached, &L, sizeof(L)) != 0) {
+    if (!s.exec || !s.have_cached || std::memcmp(&s.cached, &L, sizeof(L)) != 0 ||
+        std::memcmp(&s.cached_tune, &c->tune, sizeof(TraceTune)) != 0) {
       if (s.exec) (void)hipGraphExecDestroy(s.exec);
       if (s.graph) (void)hipGraphDestroy(s.graph);
       s.exec = nullptr;
       s.graph = nullptr;
       RDRT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      hipError_t e = enqueue_pt_frame(L, tp, push_depth(c), c->tune, st);
+      hipError_t e = enqueue_pt_frame(L, tp, push_depth(c, c->tune), c->tune, st);
       hipGraph_t g = nullptr;
       hipError_t e2 = hipStreamEndCapture(st, &g);
       if (e != hipSuccess || e2 != hipSuccess) {
@@ -284,11 +285,12 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
       s.graph = g;
       RDRT_HIP(hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0));
       s.cached = L;
+      s.cached_tune = c->tune;
       s.have_cached = true;
     }
     RDRT_HIP(hipGraphLaunch(s.exec, st));
   } else {
-    RDRT_HIP(enqueue_pt_frame(L, tp, push_depth(c), c->tune, st));
+    RDRT_HIP(enqueue_pt_frame(L, tp, push_depth(c, c->tune), c->tune, st));
   }
   RDRT_HIP(hipEventRecord(c->ev1, st));
   s.last_bounces = prm->max_bounces;
@@ -349,7 +351,7 @@ int32_t rdrt_pt_backward_geometry(rdrt_ctx *c, const float *adj, float *grad_pos
   L.g_nrm = grad_nrm;  // zeroed by the replay; stays zero without vertex normals
   L.nvertices = (uint32_t)(c->dev.vertices.count / 3);
   const TraceParams tp = scene_trace_params(c);
-  RDRT_HIP(enqueue_pt_replay(L, tp, push_depth(c), c->tune, c->work_stream));
+  RDRT_HIP(enqueue_pt_replay(L, tp, push_depth(c, c->tune), c->tune, c->work_stream));
   return RDRT_OK;
 }
 

@@ -68,8 +68,9 @@ struct Scratch {
 
 struct DeviceScene {
   DevArray<PairNode> tri_pairs, tlas_pairs;
+  DevArray<QuadNode> tri_quads, tlas_quads;
   DevArray<TriPre> tris;
-  DevArray<uint32_t> redirect, tri_ids, indices, binding, tri_material;
+  DevArray<uint32_t> redirect, tri_ids, indices, binding, tri_material, tlas_quad_roots;
   DevArray<float> vertices, normals;
   DevArray<GeoRec> geos;
   DevArray<InstRec> insts;
@@ -90,6 +91,7 @@ struct PtState {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   PtLaunch cached{};
+  TraceTune cached_tune{};  // launch shape the graph was captured with
   bool have_cached = false;
   uint32_t last_bounces = 0;
   float last_ms = 0.0f;
@@ -150,7 +152,8 @@ int32_t need_device(rdrt_ctx *c);
 void invalidate(rdrt_ctx *c);
 // Device view of the committed scene for the trace kernels (no rays).
 TraceParams scene_trace_params(const rdrt_ctx *c);
-uint32_t push_depth(const rdrt_ctx *c);
+// Deepest push count of the traversal layout that t selects.
+uint32_t push_depth(const rdrt_ctx *c, const TraceTune &t);
 int32_t cull_scratch(rdrt_ctx *c, uint32_t n, CullScratch &cs);
 
 }  // namespace rdrt

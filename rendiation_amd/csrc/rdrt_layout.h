@@ -38,6 +38,18 @@ struct alignas(64) PairNode {
 };
 static_assert(sizeof(PairNode) == 64, "PairNode must be 64 B");
 
+// Child-quad node, 128 B = one 128-byte line: the boxes of up to four
+// children in SoA order (x bounds of all four, then y, then z), so eight
+// 16-byte loads test four children and the tree is half as deep as the
+// child-pair tree it is collapsed from.  Unused slots have ref kInvalid.
+// Child refs use the encoding below, inner refs indexing QuadNodes.
+struct alignas(128) QuadNode {
+  float mnx[4], mxx[4], mny[4], mxy[4], mnz[4], mxz[4];
+  uint32_t ref[4];
+  uint32_t pad[4];
+};
+static_assert(sizeof(QuadNode) == 128, "QuadNode must be 128 B");
+
 // Child ref: bit 31 clear = index of an inner PairNode; bit 31 set = leaf
 // with `count` (bits 0..4, < 32) primitives from `start` (bits 5..30).
 constexpr uint32_t kLeafBit = 0x80000000u;
@@ -72,7 +84,8 @@ struct alignas(16) GeoRec {
   uint32_t root_ref;  // child-pair ref of the root
   float mx[3];
   uint32_t ref_root;  // root index in the reference forest
-  uint32_t primitive_start, geometry_idx, flags, pad;
+  uint32_t primitive_start, geometry_idx, flags;
+  uint32_t quad_root;  // child-quad ref of the root
 };
 static_assert(sizeof(GeoRec) == 48, "GeoRec must be 48 B");
 

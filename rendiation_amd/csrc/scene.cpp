@@ -211,6 +211,70 @@ struct PairConv {
   }
 };
 
+// Collapse of a child-pair tree into child-quad nodes.  A quad starts from
+// the two children of one pair node and opens its inner child of largest
+// surface area (replacing it by that node's two children) until it holds
+// four children or only leaves; leaves keep their refs.  `need` receives the
+// deepest push count below `ref`: a quad of n children pushes at most n - 1.
+struct QuadConv {
+  const std::vector<PairNode> &pairs;
+  std::vector<QuadNode> &quads;
+
+  struct Child {
+    float mn[3], mx[3];
+    uint32_t ref;
+  };
+  static Child left(const PairNode &p) { return Child{{p.lmn[0], p.lmn[1], p.lmn[2]}, {p.lmx[0], p.lmx[1], p.lmx[2]}, p.lref}; }
+  static Child right(const PairNode &p) {
+    return Child{{p.rmn[0], p.rmn[1], p.rmn[2]}, {p.rmx[0], p.rmx[1], p.rmx[2]}, p.rref};
+  }
+  static double area(const Child &c) {
+    const double x = (double)c.mx[0] - c.mn[0], y = (double)c.mx[1] - c.mn[1], z = (double)c.mx[2] - c.mn[2];
+    return x * y + y * z + z * x;
+  }
+
+  uint32_t ref_of(uint32_t ref, uint32_t &need) {
+    need = 0;
+    if (ref & kLeafBit) return ref;
+    Child ch[4] = {left(pairs[ref]), right(pairs[ref])};
+    int n = 2;
+    while (n < 4) {
+      int best = -1;
+      double best_area = -1.0;
+      for (int k = 0; k < n; k++)
+        if (!(ch[k].ref & kLeafBit) && area(ch[k]) > best_area) {
+          best = k;
+          best_area = area(ch[k]);
+        }
+      if (best < 0) break;
+      const PairNode &q = pairs[ch[best].ref];
+      ch[best] = left(q);
+      ch[n++] = right(q);
+    }
+    const uint32_t idx = (uint32_t)quads.size();
+    quads.emplace_back();
+    uint32_t refs[4] = {kInvalid, kInvalid, kInvalid, kInvalid}, deepest = 0;
+    for (int k = 0; k < n; k++) {
+      uint32_t d = 0;
+      refs[k] = ref_of(ch[k].ref, d);
+      deepest = std::max(deepest, d);
+    }
+    QuadNode &qn = quads[idx];
+    std::memset(&qn, 0, sizeof(qn));
+    for (int k = 0; k < n; k++) {
+      qn.mnx[k] = ch[k].mn[0];
+      qn.mny[k] = ch[k].mn[1];
+      qn.mnz[k] = ch[k].mn[2];
+      qn.mxx[k] = ch[k].mx[0];
+      qn.mxy[k] = ch[k].mx[1];
+      qn.mxz[k] = ch[k].mx[2];
+    }
+    for (int k = 0; k < 4; k++) qn.ref[k] = refs[k];
+    need = (uint32_t)(n - 1) + deepest;
+    return idx;
+  }
+};
+
 // Child-pair conversion of the fast kernel's TLAS tree, whose leaves hold
 // single instances addressed by their forest position: map[order index].  A
 // depth-capped leaf with several instances becomes a chain of pair nodes.
@@ -321,6 +385,35 @@ bool validate_refs(const BuiltScene &s, std::string &err) {
     if (r.ref_root != kInvalid &&
         (!ref_ok(r.root_ref, s.tlas_pairs.size(), s.insts.size()) || r.ref_root >= s.tlas_forest.size())) {
       err = "internal: TLAS root out of range";
+      return false;
+    }
+  auto quad_ok = [](const QuadNode &q, size_t nquads, size_t nprims) {
+    for (int k = 0; k < 4; k++)
+      if (q.ref[k] != kInvalid && !ref_ok(q.ref[k], nquads, nprims)) return false;
+    return q.ref[0] != kInvalid && q.ref[1] != kInvalid;
+  };
+  for (const QuadNode &q : s.tri_quads)
+    if (!quad_ok(q, s.tri_quads.size(), s.tris.size())) {
+      err = "internal: BLAS child-quad ref out of range";
+      return false;
+    }
+  for (const QuadNode &q : s.tlas_quads)
+    if (!quad_ok(q, s.tlas_quads.size(), s.insts.size())) {
+      err = "internal: TLAS child-quad ref out of range";
+      return false;
+    }
+  for (const GeoRec &g : s.geos)
+    if (!ref_ok(g.quad_root, s.tri_quads.size(), s.tris.size())) {
+      err = "internal: geometry quad root out of range";
+      return false;
+    }
+  if (s.tlas_quad_roots.size() != s.tlas_roots.size()) {
+    err = "internal: TLAS quad roots disagree";
+    return false;
+  }
+  for (size_t i = 0; i < s.tlas_roots.size(); i++)
+    if (s.tlas_roots[i].ref_root != kInvalid && !ref_ok(s.tlas_quad_roots[i], s.tlas_quads.size(), s.insts.size())) {
+      err = "internal: TLAS quad root out of range";
       return false;
     }
   for (const InstRec &in : s.insts)
@@ -466,6 +559,9 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
         gr.root_ref = pc.ref_of(0, d);
         if (!pc.ok) return RDRT_ERR_CAPACITY;
         out.blas_stack = std::max(out.blas_stack, d);
+        QuadConv qc{out.tri_pairs, out.tri_quads};
+        gr.quad_root = qc.ref_of(gr.root_ref, d);
+        out.blas_stack4 = std::max(out.blas_stack4, d);
         out.max_blas_depth = std::max(out.max_blas_depth, job.tree.depth);
         out.geos.push_back(gr);
         grow_box(blas_box[b], job.tree.nodes[0].box);
@@ -495,6 +591,7 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
       root.ref_root = kInvalid;
       root.root_ref = kInvalid;
       out.tlas_roots.push_back(root);
+      out.tlas_quad_roots.push_back(kInvalid);
       continue;
     }
     const auto &inst = src.tlas[ti];
@@ -554,6 +651,9 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
     root.root_ref = pc.ref_of(0, d);
     out.tlas_stack = std::max(out.tlas_stack, d);
     out.tlas_roots.push_back(root);
+    QuadConv qc{out.tlas_pairs, out.tlas_quads};
+    out.tlas_quad_roots.push_back(qc.ref_of(root.root_ref, d));
+    out.tlas_stack4 = std::max(out.tlas_stack4, d);
   }
   out.binding = src.binding;
   if (!validate_refs(out, err)) return RDRT_ERR_STATE;

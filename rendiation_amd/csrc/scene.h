@@ -58,6 +58,10 @@ struct BuiltScene {
   std::vector<TriPre> tris;
   uint32_t max_blas_depth = 0, max_tlas_depth = 0;
   uint32_t blas_stack = 0, tlas_stack = 0;  // deepest push count of the child-pair trees
+  // child-quad layout (collapsed from the child-pair trees; same leaves)
+  std::vector<QuadNode> tri_quads, tlas_quads;
+  std::vector<uint32_t> tlas_quad_roots;      // per TLAS (kInvalid when deleted)
+  uint32_t blas_stack4 = 0, tlas_stack4 = 0;  // deepest push count of the child-quad trees
   double build_ms = 0.0;
 };
 

@@ -211,12 +211,80 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t i, uint32_t used) {
   return xcd < rem ? xcd * (per + 1u) + local : rem * (per + 1u) + (xcd - rem) * per + local;
 }
 
+// Inner-node steps of the fast traversal.  Each tests the children of node
+// `ref`; when some are hit it sets ref to the nearest, pushes the others
+// (farthest first, so they pop near to far) and returns true.
+template <class Push>
+__device__ __forceinline__ bool step_pair(const PairNode *nodes, uint32_t &ref, F3 o, F3 inv_d, float tmin, float tmax,
+                                          Push &&push) {
+  const float4 *q = reinterpret_cast<const float4 *>(nodes + ref);
+  const float4 a = q[0], b = q[1], c = q[2], d = q[3];
+  float tl, tr;
+  const bool hl = slab(o, inv_d, tmin, tmax, a.x, a.y, a.z, b.x, b.y, b.z, tl);
+  const bool hr = slab(o, inv_d, tmin, tmax, c.x, c.y, c.z, d.x, d.y, d.z, tr);
+  const uint32_t lref = __float_as_uint(a.w), rref = __float_as_uint(c.w);
+  if (hl && hr) {
+    const bool rfirst = tr < tl;
+    push(rfirst ? lref : rref);
+    ref = rfirst ? rref : lref;
+    return true;
+  }
+  if (hl || hr) {
+    ref = hl ? lref : rref;
+    return true;
+  }
+  return false;
+}
+
+// Sort key of one quad child: its entry distance when hit (never NaN, always
+// below +inf: a hit's entry lies below tmax), +inf when missed or unused.
+__device__ __forceinline__ float quad_key(bool hit, float tnear) {
+  return hit ? fmaxf(tnear, -3.402823466e38f) : __builtin_inff();
+}
+__device__ __forceinline__ void order2(float &ka, uint32_t &ra, float &kb, uint32_t &rb) {
+  const bool sw = kb < ka;
+  const float k = sw ? kb : ka;
+  const uint32_t r = sw ? rb : ra;
+  kb = sw ? ka : kb;
+  rb = sw ? ra : rb;
+  ka = k;
+  ra = r;
+}
+
+template <class Push>
+__device__ __forceinline__ bool step_quad(const QuadNode *nodes, uint32_t &ref, F3 o, F3 inv_d, float tmin, float tmax,
+                                          Push &&push) {
+  const float4 *q = reinterpret_cast<const float4 *>(nodes + ref);
+  const float4 x0 = q[0], x1 = q[1], y0 = q[2], y1 = q[3], z0 = q[4], z1 = q[5];
+  const uint4 rf = *reinterpret_cast<const uint4 *>(q + 6);
+  float t0, t1, t2, t3;
+  const bool h0 = slab(o, inv_d, tmin, tmax, x0.x, y0.x, z0.x, x1.x, y1.x, z1.x, t0);
+  const bool h1 = slab(o, inv_d, tmin, tmax, x0.y, y0.y, z0.y, x1.y, y1.y, z1.y, t1);  // slots 0, 1 always used
+  const bool h2 = slab(o, inv_d, tmin, tmax, x0.z, y0.z, z0.z, x1.z, y1.z, z1.z, t2) && rf.z != kInvalid;
+  const bool h3 = slab(o, inv_d, tmin, tmax, x0.w, y0.w, z0.w, x1.w, y1.w, z1.w, t3) && rf.w != kInvalid;
+  const uint32_t n = (uint32_t)h0 + (uint32_t)h1 + (uint32_t)h2 + (uint32_t)h3;
+  if (n == 0) return false;
+  float k0 = quad_key(h0, t0), k1 = quad_key(h1, t1), k2 = quad_key(h2, t2), k3 = quad_key(h3, t3);
+  uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
+  // sorting network of four: hits ascend by entry distance, misses last
+  order2(k0, r0, k1, r1);
+  order2(k2, r2, k3, r3);
+  order2(k0, r0, k2, r2);
+  order2(k1, r1, k3, r3);
+  order2(k1, r1, k2, r2);
+  if (n > 3) push(r3);
+  if (n > 2) push(r2);
+  if (n > 1) push(r1);
+  ref = r0;
+  return true;
+}
+
 // ------------------------------------------------------------------------
 // Fast kernel: front-to-back child-pair traversal.  Stack: the first K
 // entries of each lane live in LDS (column layout), deeper ones (rare) in
 // the lane's private memory; S bounds the total.
 // ------------------------------------------------------------------------
-template <int K, int S, int B, bool CNT>
+template <int K, int S, int B, bool CNT, bool QUAD>
 __device__ __forceinline__ void trace_fast_one(const TraceParams &p, uint32_t rid, uint32_t *stk) {
   uint32_t spill[S > K ? S - K : 1];
   auto push = [&](uint32_t &sp, uint32_t v) {
@@ -245,24 +313,14 @@ __device__ __forceinline__ void trace_fast_one(const TraceParams &p, uint32_t ri
   if (root && slab(r.o, winv, r.tmin, far_, root->mn[0], root->mn[1], root->mn[2], root->mx[0], root->mx[1],
                    root->mx[2], tn)) {
     uint32_t sp = 0;
-    uint32_t ref = root->root_ref;
+    uint32_t ref = QUAD ? p.tlas_quad_roots[root - p.tlas_roots] : root->root_ref;
+    auto push_tlas = [&](uint32_t v) { push(sp, v); };
     for (;;) {
       if (!(ref & kLeafBit)) {
-        const float4 *q = reinterpret_cast<const float4 *>(p.tlas_pairs + ref);
-        float4 a = q[0], b = q[1], c = q[2], d = q[3];
-        float tl, tr;
-        bool hl = slab(r.o, winv, r.tmin, far_, a.x, a.y, a.z, b.x, b.y, b.z, tl);
-        bool hr = slab(r.o, winv, r.tmin, far_, c.x, c.y, c.z, d.x, d.y, d.z, tr);
-        uint32_t lref = __float_as_uint(a.w), rref = __float_as_uint(c.w);
-        if (hl && hr) {
-          bool rfirst = tr < tl;
-          push(sp, rfirst ? lref : rref);
-          ref = rfirst ? rref : lref;
-          continue;
-        }
-        if (hl || hr) {
-          ref = hl ? lref : rref;
-          continue;
+        if constexpr (QUAD) {
+          if (step_quad(p.tlas_quads, ref, r.o, winv, r.tmin, far_, push_tlas)) continue;
+        } else {
+          if (step_pair(p.tlas_pairs, ref, r.o, winv, r.tmin, far_, push_tlas)) continue;
         }
       } else {
         const uint32_t i0 = (ref & ~kLeafBit) >> kLeafCountBits, i1 = i0 + (ref & kLeafMaxCount);
@@ -291,26 +349,16 @@ __device__ __forceinline__ void trace_fast_one(const TraceParams &p, uint32_t ri
               continue;
             // BLAS traversal; its stack entries sit above the TLAS ones.
             uint32_t bsp = sp;
-            uint32_t bref = geo.root_ref;
+            uint32_t bref = QUAD ? geo.quad_root : geo.root_ref;
+            auto push_blas = [&](uint32_t v) { push(bsp, v); };
             for (;;) {
               if (!(bref & kLeafBit)) {
                 if constexpr (CNT) cn++;
-                const float4 *q = reinterpret_cast<const float4 *>(p.tri_pairs + bref);
-                float4 a = q[0], b = q[1], c = q[2], d = q[3];
                 const float fs = far_ * s;
-                float tl, tr;
-                bool hl = slab(bo, binv, near_s, fs, a.x, a.y, a.z, b.x, b.y, b.z, tl);
-                bool hr = slab(bo, binv, near_s, fs, c.x, c.y, c.z, d.x, d.y, d.z, tr);
-                uint32_t lref = __float_as_uint(a.w), rref = __float_as_uint(c.w);
-                if (hl && hr) {
-                  bool rfirst = tr < tl;
-                  push(bsp, rfirst ? lref : rref);
-                  bref = rfirst ? rref : lref;
-                  continue;
-                }
-                if (hl || hr) {
-                  bref = hl ? lref : rref;
-                  continue;
+                if constexpr (QUAD) {
+                  if (step_quad(p.tri_quads, bref, bo, binv, near_s, fs, push_blas)) continue;
+                } else {
+                  if (step_pair(p.tri_pairs, bref, bo, binv, near_s, fs, push_blas)) continue;
                 }
               } else {
                 const uint32_t k0 = (bref & ~kLeafBit) >> kLeafCountBits, k1 = k0 + (bref & kLeafMaxCount);
@@ -375,7 +423,7 @@ __device__ __forceinline__ void select_second(TraceParams &p) {
 // One ray per thread over a grid sized for both segments' slots; the live
 // counts may be smaller (device queue sizes).  The blocks of the second
 // segment follow those of the first, so a block's segment is uniform.
-template <int K, int S, int B, int W, bool CNT>
+template <int K, int S, int B, int W, bool CNT, bool QUAD>
 __global__ __launch_bounds__(B, W) void k_trace_fast(TraceParams p) {
   __shared__ uint32_t lds[K * B];
   const uint32_t n0 = live_first(p), n1 = live_second(p);
@@ -393,7 +441,7 @@ __global__ __launch_bounds__(B, W) void k_trace_fast(TraceParams p) {
   if (slot >= n) return;
   const uint32_t rid = p.queue ? p.queue[slot] : slot;
   if (rid >= p.n) return;  // a queue entry always names one of the n rays
-  trace_fast_one<K, S, B, CNT>(p, rid, lds + threadIdx.x);
+  trace_fast_one<K, S, B, CNT, QUAD>(p, rid, lds + threadIdx.x);
 }
 
 // Persistent variant: a grid sized to the resident capacity; each wave
@@ -402,7 +450,7 @@ __global__ __launch_bounds__(B, W) void k_trace_fast(TraceParams p) {
 // across the chip.  Every wave exits once a fetch lands past the count.
 // The second segment's slots start at the first 64-aligned position after
 // the first segment's, so every chunk lies in one segment.
-template <int K, int S, int B, int W, bool CNT>
+template <int K, int S, int B, int W, bool CNT, bool QUAD>
 __global__ __launch_bounds__(B, W) void k_trace_persist(TraceParams p) {
   __shared__ uint32_t lds[K * B];
   const uint32_t n0 = live_first(p), n1 = live_second(p);
@@ -424,7 +472,7 @@ __global__ __launch_bounds__(B, W) void k_trace_persist(TraceParams p) {
     const uint32_t slot = base + lane;
     if (slot < n) {
       const uint32_t rid = q.queue ? q.queue[slot] : slot;
-      if (rid < q.n) trace_fast_one<K, S, B, CNT>(q, rid, lds + threadIdx.x);
+      if (rid < q.n) trace_fast_one<K, S, B, CNT, QUAD>(q, rid, lds + threadIdx.x);
     }
   }
 }
@@ -433,51 +481,51 @@ __global__ void k_zero_work(uint32_t *w) { *w = 0u; }
 
 // One fast-traversal launch with block size B, total stack S and tuning t.
 // K = stack entries per lane in LDS, W = minimum waves per SIMD asked of the
-// register allocator (1 = its own choice).
-template <int K, int S, int B, int W>
+// register allocator (1 = its own choice), QUAD = child-quad layout.
+template <int K, int S, int B, int W, bool QUAD>
 hipError_t launch_fast_kw(const TraceParams &p, bool persist, hipStream_t st) {
   if (persist) {
     if (!p.work) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(1), 0, st, p.work);
     if (p.counters) {
-      auto k = k_trace_persist<K, S, B, W, true>;
+      auto k = k_trace_persist<K, S, B, W, true, QUAD>;
       uint32_t g = std::min(resident_grid((const void *)k, B), grid_slots(p, B) / B);
       hipLaunchKernelGGL(k, dim3(g), dim3(B), 0, st, p);
     } else {
-      auto k = k_trace_persist<K, S, B, W, false>;
+      auto k = k_trace_persist<K, S, B, W, false, QUAD>;
       uint32_t g = std::min(resident_grid((const void *)k, B), grid_slots(p, B) / B);
       hipLaunchKernelGGL(k, dim3(g), dim3(B), 0, st, p);
     }
     return hipGetLastError();
   }
   dim3 grid(grid_slots(p, B) / B);
-  if (p.counters) hipLaunchKernelGGL((k_trace_fast<K, S, B, W, true>), grid, dim3(B), 0, st, p);
-  else hipLaunchKernelGGL((k_trace_fast<K, S, B, W, false>), grid, dim3(B), 0, st, p);
+  if (p.counters) hipLaunchKernelGGL((k_trace_fast<K, S, B, W, true, QUAD>), grid, dim3(B), 0, st, p);
+  else hipLaunchKernelGGL((k_trace_fast<K, S, B, W, false, QUAD>), grid, dim3(B), 0, st, p);
   return hipGetLastError();
 }
 
-template <int S, int B>
+template <int S, int B, bool QUAD>
 hipError_t launch_fast_s(const TraceParams &p, const TraceTune &t, hipStream_t st) {
   constexpr int K8 = S < 8 ? S : 8;
   constexpr int K16 = S < 16 ? S : 16;
   const bool persist = t.persist != 0;
   if (t.waves >= 8) {
-    if (t.lds <= 8) return launch_fast_kw<K8, S, B, 8>(p, persist, st);
-    return launch_fast_kw<K16, S, B, 8>(p, persist, st);
+    if (t.lds <= 8) return launch_fast_kw<K8, S, B, 8, QUAD>(p, persist, st);
+    return launch_fast_kw<K16, S, B, 8, QUAD>(p, persist, st);
   }
-  if (t.lds <= 8) return launch_fast_kw<K8, S, B, 1>(p, persist, st);
-  return launch_fast_kw<K16, S, B, 1>(p, persist, st);
+  if (t.lds <= 8) return launch_fast_kw<K8, S, B, 1, QUAD>(p, persist, st);
+  return launch_fast_kw<K16, S, B, 1, QUAD>(p, persist, st);
 }
 
-template <int B>
+template <int B, bool QUAD>
 hipError_t launch_fast_block(const TraceParams &p, uint32_t s, const TraceTune &t, hipStream_t st) {
   switch (s) {
-    case 8: return launch_fast_s<8, B>(p, t, st);
-    case 16: return launch_fast_s<16, B>(p, t, st);
-    case 24: return launch_fast_s<24, B>(p, t, st);
-    case 32: return launch_fast_s<32, B>(p, t, st);
-    case 48: return launch_fast_s<48, B>(p, t, st);
-    case 64: return launch_fast_s<64, B>(p, t, st);
+    case 8: return launch_fast_s<8, B, QUAD>(p, t, st);
+    case 16: return launch_fast_s<16, B, QUAD>(p, t, st);
+    case 24: return launch_fast_s<24, B, QUAD>(p, t, st);
+    case 32: return launch_fast_s<32, B, QUAD>(p, t, st);
+    case 48: return launch_fast_s<48, B, QUAD>(p, t, st);
+    case 64: return launch_fast_s<64, B, QUAD>(p, t, st);
     default: return hipErrorInvalidValue;
   }
 }

@@ -47,6 +47,9 @@ struct TraceParams {
   const PairNode *tlas_pairs;
   const InstRec *insts;
   const TlasRoot *tlas_roots;
+  const QuadNode *tri_quads;         // child-quad layout of the same trees
+  const QuadNode *tlas_quads;
+  const uint32_t *tlas_quad_roots;   // per TLAS
   const uint32_t *binding;
   uint32_t nbinding, ntlas;
 
@@ -75,6 +78,7 @@ struct TraceTune {
   uint32_t persist = 0;  // 1 = resident-size grid whose waves pull 64-ray chunks from a counter
   uint32_t cull = 0;     // 1 = a first pass finishes the rays that miss their scene's root box and
                          //     queues the others densely for the traversal
+  uint32_t wide = 0;     // 1 = walk the child-quad layout (four boxes per node) instead of child pairs
 };
 
 // Scratch of a culled launch: tmp (slots words), blk (2 * ceil(slots /
@@ -98,7 +102,8 @@ hipError_t launch_trace_culled(const TraceParams &p, uint32_t push_depth, const 
 
 // Enqueues one traversal launch on `stream`.  mode = RDRT_TRAVERSAL_*.  A
 // persistent launch needs p.work (zeroed by a kernel launched first, so the
-// pair is capturable).
+// pair is capturable).  push_depth: the deepest push count of the layout
+// tune.wide selects.
 hipError_t launch_trace(const TraceParams &p, uint32_t mode, uint32_t push_depth, const TraceTune &tune,
                         hipStream_t stream);
 
@@ -108,6 +113,10 @@ uint32_t resident_grid(const void *kernel, int block);
 hipError_t launch_fast_64(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
 hipError_t launch_fast_128(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
 hipError_t launch_fast_256(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
+// The same over the child-quad layout (trace_quad_<B>.hip).
+hipError_t launch_quad_64(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
+hipError_t launch_quad_128(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
+hipError_t launch_quad_256(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
 
 }  // namespace rdrt
 

@@ -204,6 +204,14 @@ hipError_t launch_trace(const TraceParams &p, uint32_t mode, uint32_t push_depth
   }
   const uint32_t s = stack_entries_for(push_depth);
   if (s == 0) return hipErrorInvalidValue;
+  if (t.wide) {
+    if (!p.tri_quads || !p.tlas_quads || !p.tlas_quad_roots) return hipErrorInvalidValue;
+    switch (t.block) {
+      case 64: return launch_quad_64(p, s, t, stream);
+      case 128: return launch_quad_128(p, s, t, stream);
+      default: return launch_quad_256(p, s, t, stream);
+    }
+  }
   switch (t.block) {
     case 64: return launch_fast_64(p, s, t, stream);
     case 128: return launch_fast_128(p, s, t, stream);

@@ -63,11 +63,16 @@ def _dead_rays(rays, dead):
     return out
 
 
-def test_rounds_match_oracle(batch_case, device):
+@pytest.mark.parametrize("wide", [0, 1])
+def test_rounds_match_oracle(batch_case, wide, device):
     sys, orc, _, rays, meta, _ = batch_case
     n = rays.shape[0]
-    _say(f"batched persistent: 6 scenes, {n} rays, {BOUNCES} bounces")
-    h, sp, st = _run(sys, rays, meta, device, BATCH_PERSISTENT)
+    _say(f"batched persistent: 6 scenes, {n} rays, {BOUNCES} bounces, wide={wide}")
+    sys.set_tune(wide=wide)
+    try:
+        h, sp, st = _run(sys, rays, meta, device, BATCH_PERSISTENT)
+    finally:
+        sys.set_tune()
     assert st.rounds == BOUNCES + 1
     live = np.ones(n, dtype=bool)
     for r in range(BOUNCES + 1):

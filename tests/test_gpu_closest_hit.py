@@ -3,7 +3,8 @@
 * RDRT_TRAVERSAL_REFERENCE must equal the oracle bit for bit: hit records,
   hit context and the per-ray node / triangle / instance visit counters.
 * RDRT_TRAVERSAL_FAST must equal it except on near-ties (|dt| <= 1e-5 t),
-  with bit-identical t, u, v wherever the primitive ids agree.
+  with bit-identical t, u, v wherever the primitive ids agree, over both
+  node layouts (child pairs and child quads).
 """
 
 import numpy as np
@@ -71,10 +72,15 @@ def test_reference_order_kernel_is_bit_exact(closest_hit_scene, device):
     assert np.array_equal(gc, rc)
 
 
-def test_fast_kernel_matches_except_near_ties(closest_hit_scene, device):
+@pytest.mark.parametrize("wide", [0, 1])
+def test_fast_kernel_matches_except_near_ties(closest_hit_scene, wide, device):
     sys, rays, (rh, rm, _) = closest_hit_scene
-    _say("closest-hit 100k tris x 1M rays, fast kernel")
-    gh, gm, _ = _gpu_trace(sys, rays, device, traversal=TRAVERSAL_FAST)
+    _say(f"closest-hit 100k tris x 1M rays, fast kernel, wide={wide}")
+    sys.set_tune(wide=wide)
+    try:
+        gh, gm, _ = _gpu_trace(sys, rays, device, traversal=TRAVERSAL_FAST)
+    finally:
+        sys.set_tune()
     d = compare(gh, rh)
     print(d)
     assert d.ok(), d
@@ -83,13 +89,14 @@ def test_fast_kernel_matches_except_near_ties(closest_hit_scene, device):
     assert np.array_equal(gm[same], rm[same])
 
 
-@pytest.mark.parametrize("traversal", [TRAVERSAL_REFERENCE, TRAVERSAL_FAST])
-def test_instanced_scene_with_flags(traversal, device):
+@pytest.mark.parametrize("traversal,wide", [(TRAVERSAL_REFERENCE, 0), (TRAVERSAL_FAST, 0), (TRAVERSAL_FAST, 1)])
+def test_instanced_scene_with_flags(traversal, wide, device):
     blas_list, tlas_list, binding, rays, meta = instanced_scene(seed=3, nrays=60_000)
     opts = BuildOptions()
     sys, orc = _build_pair(blas_list, tlas_list, binding, opts)
-    _say(f"instanced scene, traversal={traversal}")
+    _say(f"instanced scene, traversal={traversal} wide={wide}")
     rh, rm, rc = orc.trace(rays, meta=meta, with_meta=True, with_counters=True)
+    sys.set_tune(wide=wide)
     gh, gm, gc = _gpu_trace(sys, rays, device, meta_np=meta, traversal=traversal)
     end_search = (meta[:, 0] & RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH) != 0
     if traversal == TRAVERSAL_REFERENCE:
@@ -115,7 +122,8 @@ def test_uniform_ray_flags(flags, device):
     sys, orc = _build_pair([[(pos, None, 1)]], [[{"transform": np.eye(4), "blas": 0}]], [0], BuildOptions())
     _say(f"uniform flags {flags:#x}")
     rh, _, rc = orc.trace(rays, flags=flags, with_counters=True)
-    for trav in (TRAVERSAL_REFERENCE, TRAVERSAL_FAST):
+    for trav, wide in ((TRAVERSAL_REFERENCE, 0), (TRAVERSAL_FAST, 0), (TRAVERSAL_FAST, 1)):
+        sys.set_tune(wide=wide)
         gh, _, gc = _gpu_trace(sys, rays, device, traversal=trav, flags=flags)
         if trav == TRAVERSAL_REFERENCE:
             assert np.array_equal(gh, rh)
@@ -149,18 +157,23 @@ def test_launch_shapes_agree_and_autotune(closest_hit_scene, device):
 
     sys, rays_np, _ = closest_hit_scene
     _say("launch shapes + autotune, 100k tris x 1M rays")
-    base, _, _ = _gpu_trace(sys, rays_np, device)
+    base = {}
+    for wide in (0, 1):  # each layout has its own visit order
+        sys.set_tune(wide=wide)
+        base[wide], _, _ = _gpu_trace(sys, rays_np, device)
     rays = torch.from_numpy(np.ascontiguousarray(rays_np)).to(device)
     hits = torch.empty((rays_np.shape[0], 4), dtype=torch.int32, device=device)
-    for shape in [(64, 8, 0, 0, 0), (128, 16, 8, 1, 0), (256, 8, 8, 0, 1), (256, 16, 0, 1, 1), (64, 16, 8, 0, 1)]:
+    for shape in [(64, 8, 0, 0, 0, 0), (128, 16, 8, 1, 0, 0), (256, 8, 8, 0, 1, 0), (256, 16, 0, 1, 1, 0),
+                  (64, 16, 8, 0, 1, 0), (64, 8, 0, 0, 0, 1), (128, 16, 8, 1, 0, 1), (256, 8, 8, 0, 1, 1),
+                  (256, 16, 0, 1, 1, 1)]:
         sys.set_tune(*shape)
         sys.trace(rays, hits)
         sys.sync()
-        assert np.array_equal(hits.cpu().numpy().view(np.uint32), base), shape
+        assert np.array_equal(hits.cpu().numpy().view(np.uint32), base[shape[5]]), shape
     choice = sys.autotune(rays, hits, reps=2)
     print("autotune:", choice)
     assert choice["block_size"] in (64, 128, 256) and choice["lds_entries"] in (8, 16) and choice["ms"] > 0
     sys.trace(rays, hits)
     sys.sync()
-    assert np.array_equal(hits.cpu().numpy().view(np.uint32), base)
+    assert np.array_equal(hits.cpu().numpy().view(np.uint32), base[choice["wide"]])
     sys.set_tune()

@@ -134,13 +134,14 @@ def test_floor_on_gpu(device):
     assert st["shadow_rays"][:1] == [32 * 32]
 
 
-@pytest.mark.parametrize("spp", [1, 4])
-def test_parametric_scene_against_fp64_oracle(spp, device):
+@pytest.mark.parametrize("spp,wide", [(1, 0), (4, 0), (1, 1)])
+def test_parametric_scene_against_fp64_oracle(spp, wide, device):
     scene = synth.pt_scene(seed=1, n_meshes=64, tess=48)
     cam = Camera()
     cfg = PathTracingConfig(width=192, height=108, spp=spp, max_bounces=4, rr_start=2, seed=5)
-    _say(f"parametric scene 64 meshes tess 48, 192x108 spp {spp}")
+    _say(f"parametric scene 64 meshes tess 48, 192x108 spp {spp}, wide={wide}")
     s = _system(scene)
+    s.set_tune(wide=wide)
     img, st = _render(s, cam, cfg)
     ref, ext, sh = oracle_scene_from(scene).pt_render(oracle_params(cam, cfg))
     rmse = float(np.sqrt(np.mean((img - ref) ** 2)))
@@ -172,6 +173,17 @@ def test_graph_replay_equals_eager(device):
         out = pt.render(cam, cfg, out=out)
         s.sync()
         assert np.array_equal(out.cpu().numpy(), eager.cpu().numpy())
+    # a new launch shape recaptures the graph
+    s.set_tune(block_size=128, wide=1)
+    cfg.use_graph = False
+    eager_wide = pt.render(cam, cfg).clone()
+    s.sync()
+    cfg.use_graph = True
+    for _ in range(2):
+        out = pt.render(cam, cfg, out=out)
+        s.sync()
+        assert np.array_equal(out.cpu().numpy(), eager_wide.cpu().numpy())
+    s.set_tune()
 
 
 def test_views_in_one_frame(device):
