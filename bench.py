@@ -145,6 +145,13 @@ def path_tracing(args):
         out = torch.empty((cfg.height, cfg.width, 3), dtype=torch.float32, device=dev)
     stream.synchronize()
 
+    if args.tune:
+        system.set_tune(*[int(x) for x in args.tune.split(",")])
+        tune = system.tune()
+    elif not args.no_autotune:
+        tune = pt.autotune(cam, cfg, out=out, reps=3)
+    else:
+        tune = system.tune()
     if args.profile_only:  # the frames alone, for hardware counters
         for _ in range(args.steps):
             pt.render(cam, cfg, out=out)
@@ -189,12 +196,12 @@ def path_tracing(args):
         "max_bounces": cfg.max_bounces, "rays_per_frame": rays, "extension_rays": st["extension_rays"],
         "shadow_rays": st["shadow_rays"][:-1], "frame_ms_events": round(frame_ms_ev, 4),
         "scene_gen_ms": round(gen_ms, 1), "build_upload_ms": round(build_upload_ms, 1),
-        "device_scene_mb": round(info.device_bytes / 2 ** 20, 1), "graph_replay": True,
+        "device_scene_mb": round(info.device_bytes / 2 ** 20, 1), "graph_replay": True, "launch_shape": tune,
     }
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-        "kernel": "k_trace_fast (all traversal launches of one frame)",
+        "kernel": f"{_call_kernels(dict(tune, cull=0))} (all traversal launches of one frame)",
         "trace_ms_per_frame": round(d_time["trace_ms"], 4), "trace_launches": d_time["trace_launches"],
         "bytes_per_frame": bytes_frame,
         "visits_per_ray": [round(v / rays, 3) for v in d_vis["visits"]],
@@ -662,12 +669,15 @@ def batched(args):
         spawned = torch.empty(bounces * n * 8, dtype=torch.float32, device=dev)
     stream.synchronize()
 
-    # launch shape: autotuned on the first million primary rays
-    m = min(n, 1 << 20)
-    with torch.cuda.stream(stream):
-        sample_hits = torch.empty((m, 4), dtype=torch.int32, device=dev)
-    tune = system.autotune(rays[:m], sample_hits, ray_meta=meta[:m], reps=2) if not args.no_autotune else system.tune()
-    del sample_hits
+    # launch shape: autotuned on whole persistent calls of this workload
+    if args.tune:
+        system.set_tune(*[int(x) for x in args.tune.split(",")])
+        tune = system.tune()
+    elif not args.no_autotune:
+        with torch.cuda.stream(stream):
+            tune = system.batch_autotune(rays, meta, hits, bounces=bounces, seed=0, reps=1)
+    else:
+        tune = system.tune()
 
     def run(mode, steps, record=False):
         for _ in range(steps):
@@ -688,6 +698,11 @@ def batched(args):
         torch.cuda.synchronize()
         return (time.perf_counter() - w0) * 1e3 / args.steps, e0.elapsed_time(e1) / args.steps, system.batch_stats()
 
+    if args.profile_only:  # the persistent calls alone, for hardware counters
+        run(BATCH_PERSISTENT, args.steps)
+        system.sync()
+        system.close()
+        return None
     ms_p, ev_p, st_p = timed(BATCH_PERSISTENT)
     hits_p = hits.view(torch.int64).sum().item()
     ms_s, ev_s, st_s = timed(BATCH_SEQUENTIAL)
@@ -852,9 +867,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 256 MiB device-to-device copy (a reference dispatch for HBM counters)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic passes")
-    ap.add_argument("--no-autotune", action="store_true", help="closest_hit: keep the default launch shape")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--tune", default="",
-                    help="closest_hit: launch shape block,lds,waves,persistent[,cull[,wide]] (no autotune)")
+                    help="launch shape block,lds,waves,persistent[,cull[,wide]] (no autotune)")
     ap.add_argument("--profile-only", action="store_true", help="run the timed launches only (counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -875,7 +890,7 @@ def main():
     if rec is None:
         return
     traffic = None
-    if not args.no_traffic and args.workload in ("pt", "closest_hit"):
+    if not args.no_traffic and args.workload in ("pt", "closest_hit", "batched"):
         # counter passes in child processes, with this run's launch shape
         shape = rec["config"].get("launch_shape")
         extra, kernels = (), ("k_trace_fast", "k_trace_persist")
@@ -883,6 +898,9 @@ def main():
             extra = ("--tune", _tune_arg(shape))
             if shape.get("cull"):
                 kernels = kernels + ("k_trace_cull", "k_block_scan", "k_scatter2")
+        if args.workload == "batched":  # the whole call
+            kernels = kernels + ("k_trace_cull", "k_block_scan", "k_scatter2", "k_spawn", "k_batch_init",
+                                 "k_zero_work")
         traffic = pmc_traffic(args.workload, kernels, extra)
     if traffic is not None and rec.get("roofline"):
         rec["roofline"]["traffic"] = round(traffic["bytes_per_step"])

@@ -308,6 +308,15 @@ typedef struct rdrt_pt_stats {
 int32_t rdrt_pt_render(rdrt_ctx *ctx, const rdrt_pt_params *params, float *radiance_dev);
 /* Ray counts of the last frame (synchronizes). */
 int32_t rdrt_pt_get_stats(rdrt_ctx *ctx, rdrt_pt_stats *out);
+/* Times every launch shape of the fast traversal (3 block sizes x 2 LDS
+ * stack depths x 2 occupancy targets x grid / persistent x child-pair /
+ * child-quad layout) on eager frames of these parameters (record and
+ * diagnostics off), `reps` timed frames each, and keeps the fastest for the
+ * context, as rdrt_autotune does for single launches; best->ms is the frame
+ * time.  radiance_dev is overwritten.  Synchronizes.  best may be NULL.
+ * (No counterpart in the reference.) */
+int32_t rdrt_pt_autotune(rdrt_ctx *ctx, const rdrt_pt_params *params, float *radiance_dev, uint32_t reps,
+                         rdrt_tune *best);
 
 /* ---- differentiable path tracing (no counterpart in the reference: the
  *      adjoint of the path tracer's frame radiance) ---- */
@@ -370,6 +379,13 @@ typedef struct rdrt_batch_stats {
 
 int32_t rdrt_trace_batched(rdrt_ctx *ctx, const rdrt_batch_params *params);
 int32_t rdrt_batch_get_stats(rdrt_ctx *ctx, rdrt_batch_stats *out);
+/* Times every launch shape (3 block sizes x 2 LDS stack depths x 2 occupancy
+ * targets x with / without the cull pass x child-pair / child-quad layout)
+ * on whole persistent batched calls of these parameters, `reps` timed calls
+ * each, and keeps the fastest for the context; best->ms is the call time.
+ * The outputs of params are overwritten.  Synchronizes.  best may be NULL.
+ * (No counterpart in the reference.) */
+int32_t rdrt_batch_autotune(rdrt_ctx *ctx, const rdrt_batch_params *params, uint32_t reps, rdrt_tune *best);
 
 /* ---- device primitives of the path (parity helpers) ---- */
 /* Stable stream compaction (shader/parallel-compute/src/stream_compaction.rs:

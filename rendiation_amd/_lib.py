@@ -138,6 +138,8 @@ SIGNATURES = {
     "rdrt_pt_backward_geometry": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "rdrt_geometry_vertex_range": (i32, [C.c_void_p, u32, u32, C.POINTER(u32), C.POINTER(u32)]),
     "rdrt_trace_batched": (i32, [C.c_void_p, C.POINTER(RdrtBatchParams)]),
+    "rdrt_pt_autotune": (i32, [C.c_void_p, C.POINTER(RdrtPtParams), C.c_void_p, u32, C.POINTER(RdrtTune)]),
+    "rdrt_batch_autotune": (i32, [C.c_void_p, C.POINTER(RdrtBatchParams), u32, C.POINTER(RdrtTune)]),
     "rdrt_autotune": (i32, [C.c_void_p, C.POINTER(RdrtTraceArgs), u32, C.POINTER(RdrtTune)]),
     "rdrt_set_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),
     "rdrt_get_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),

@@ -323,6 +323,24 @@ class AccelerationStructureSystem:
         a.n, a.bounces, a.seed, a.mode = n, bounces, seed & 0xFFFFFFFF, mode
         L.check(self._L.rdrt_trace_batched(self._h, C.byref(a)))
 
+    def batch_autotune(self, rays, ray_meta, hits, *, bounces: int, seed: int = 0, spawned=None,
+                       reps: int = 2) -> dict:
+        """Time every launch shape on whole persistent batched calls of these
+        arguments (outputs overwritten) and keep the fastest; returns the
+        choice, its ``ms`` the call time."""
+        n = rays.numel() // 8
+        _check_dev(rays, n * 8, "rays")
+        _check_dev(ray_meta, n * 4, "ray_meta")
+        _check_dev(hits, (bounces + 1) * n * 4, "hits")
+        if spawned is not None:
+            _check_dev(spawned, bounces * n * 8, "spawned")
+        a = L.RdrtBatchParams()
+        a.rays, a.ray_meta, a.hits, a.spawned = _ptr(rays), _ptr(ray_meta), _ptr(hits), _ptr(spawned)
+        a.n, a.bounces, a.seed, a.mode = n, bounces, seed & 0xFFFFFFFF, BATCH_PERSISTENT
+        t = L.RdrtTune()
+        L.check(self._L.rdrt_batch_autotune(self._h, C.byref(a), reps, C.byref(t)))
+        return self.tune()
+
     def batch_stats(self) -> BatchStats:
         """Counts and timing of the last batched trace (synchronizes)."""
         o = L.RdrtBatchStats()

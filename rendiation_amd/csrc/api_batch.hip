@@ -50,8 +50,10 @@ Plan make_plan(const rdrt_batch_params *a) {
 }
 
 // Enqueue all rounds of scene range k (the whole batch in persistent mode).
+// prefilled: every later round already holds miss records (the cull pass
+// wrote them), so paths that end need not write theirs.
 hipError_t enqueue_scene(rdrt_ctx *c, const rdrt_batch_params *a, const Plan &pl, uint32_t k, bool persist,
-                         uint32_t &launches) {
+                         bool prefilled, uint32_t &launches) {
   BatchState &s = c->batch;
   const uint32_t first = pl.off[k], nk = pl.off[k + 1] - first;
   if (nk == 0) return hipSuccess;
@@ -92,7 +94,7 @@ hipError_t enqueue_scene(rdrt_ctx *c, const rdrt_batch_params *a, const Plan &pl
     sa.record = a->spawned ? a->spawned + (size_t)r * a->n : nullptr;
     sa.hits = tp.hits;
     sa.hits_later = tp.hits + a->n;
-    sa.later_rounds = R - 1 - r;
+    sa.later_rounds = prefilled ? 0u : R - 1 - r;
     sa.hitx = base.hitx;
     sa.n = a->n;
     sa.round = r + 1;
@@ -197,7 +199,7 @@ int32_t rdrt_trace_batched(rdrt_ctx *c, const rdrt_batch_params *a) {
         e = enqueue_batch_init((uint32_t *)s.qa.ptr, 0u, a->n, (uint32_t *)s.counts.ptr, (uint32_t *)s.counts.ptr + 1,
                                pl.rounds, st);
       }
-      if (e == hipSuccess) e = enqueue_scene(c, a, pl, 0, true, launches);
+      if (e == hipSuccess) e = enqueue_scene(c, a, pl, 0, true, c->tune.cull != 0, launches);
       hipGraph_t g = nullptr;
       hipError_t e2 = hipStreamEndCapture(st, &g);
       if (e != hipSuccess || e2 != hipSuccess) {
@@ -215,10 +217,68 @@ int32_t rdrt_trace_batched(rdrt_ctx *c, const rdrt_batch_params *a) {
     // identity queue over all rays (scene k owns [off[k], off[k+1])); the count
     // slot written is the unused work word
     RDRT_HIP(enqueue_batch_init((uint32_t *)s.qa.ptr, 0u, a->n, (uint32_t *)s.work.ptr, nullptr, 0u, st));
-    for (uint32_t k = 0; k < pl.nscenes; k++) RDRT_HIP(enqueue_scene(c, a, pl, k, false, launches));
+    for (uint32_t k = 0; k < pl.nscenes; k++) RDRT_HIP(enqueue_scene(c, a, pl, k, false, false, launches));
     s.launches = launches;
   }
   RDRT_HIP(hipEventRecord(c->ev1, st));
+  return RDRT_OK;
+}
+
+int32_t rdrt_batch_autotune(rdrt_ctx *c, const rdrt_batch_params *a, uint32_t reps, rdrt_tune *best) {
+  if (!c || !a) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  if (a->mode != RDRT_BATCH_PERSISTENT) return fail(RDRT_ERR_INVALID_ARG, "the tuner times persistent batched calls");
+  if (a->n == 0) return fail(RDRT_ERR_INVALID_ARG, "the sample needs rays");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  if (reps == 0) reps = 2;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  RDRT_HIP(hipEventCreate(&e0));
+  hipError_t he = hipEventCreate(&e1);
+  if (he != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(RDRT_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(he));
+  }
+  const TraceTune before = c->tune;
+  TraceTune chosen = before;
+  float chosen_ms = -1.0f;
+  for (uint32_t block : {64u, 128u, 256u})
+    for (uint32_t lds : {8u, 16u})
+      for (uint32_t waves : {0u, 8u})
+        for (uint32_t shape = 0; shape < 4 && rc == RDRT_OK; shape++) {
+          TraceTune t;
+          t.block = block;
+          t.lds = lds;
+          t.waves = waves;
+          t.persist = before.persist;  // batched rounds always run persistent
+          t.cull = shape & 1u;
+          t.wide = shape >> 1;
+          if (stack_entries_for(push_depth(c, t)) == 0) continue;
+          c->tune = t;
+          rc = rdrt_trace_batched(c, a);  // capture and first run
+          if (rc == RDRT_OK && hipEventRecord(e0, c->work_stream) != hipSuccess) rc = fail(RDRT_ERR_HIP, "event");
+          for (uint32_t r = 0; r < reps && rc == RDRT_OK; r++) rc = rdrt_trace_batched(c, a);
+          if (rc != RDRT_OK) break;
+          float ms = 0.0f;
+          if (hipEventRecord(e1, c->work_stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(RDRT_ERR_HIP, "batch timing failed");
+            break;
+          }
+          ms /= (float)reps;
+          if (chosen_ms < 0.0f || ms < chosen_ms) {
+            chosen_ms = ms;
+            chosen = t;
+          }
+        }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != RDRT_OK) {
+    c->tune = before;
+    return rc;
+  }
+  c->tune = chosen;
+  c->tune_ms = chosen_ms;
+  if (best) return rdrt_get_tune(c, best);
   return RDRT_OK;
 }
 

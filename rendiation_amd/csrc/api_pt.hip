@@ -299,6 +299,67 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
   return RDRT_OK;
 }
 
+int32_t rdrt_pt_autotune(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance, uint32_t reps, rdrt_tune *best) {
+  if (!c || !prm || !radiance) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  if (prm->record || prm->diagnostics) return fail(RDRT_ERR_INVALID_ARG, "tune on plain frames (no record, no diagnostics)");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  if (reps == 0) reps = 3;
+  rdrt_pt_params q = *prm;
+  q.use_graph = 0;  // eager frames: no capture per shape, the same launches
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  RDRT_HIP(hipEventCreate(&e0));
+  hipError_t he = hipEventCreate(&e1);
+  if (he != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(RDRT_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(he));
+  }
+  const TraceTune before = c->tune;
+  TraceTune chosen = before;
+  float chosen_ms = -1.0f;
+  rc = RDRT_OK;
+  for (uint32_t block : {64u, 128u, 256u})
+    for (uint32_t lds : {8u, 16u})
+      for (uint32_t waves : {0u, 8u})
+        for (uint32_t shape = 0; shape < 4 && rc == RDRT_OK; shape++) {
+          TraceTune t;
+          t.block = block;
+          t.lds = lds;
+          t.waves = waves;
+          t.persist = shape & 1u;
+          t.wide = shape >> 1;
+          if (stack_entries_for(push_depth(c, t)) == 0) continue;
+          c->tune = t;
+          rc = rdrt_pt_render(c, &q, radiance);  // warm-up
+          if (rc == RDRT_OK && hipEventRecord(e0, c->work_stream) != hipSuccess) rc = fail(RDRT_ERR_HIP, "event");
+          for (uint32_t r = 0; r < reps && rc == RDRT_OK; r++) rc = rdrt_pt_render(c, &q, radiance);
+          if (rc != RDRT_OK) break;
+          float ms = 0.0f;
+          if (hipEventRecord(e1, c->work_stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(RDRT_ERR_HIP, "frame timing failed");
+            break;
+          }
+          ms /= (float)reps;
+          if (chosen_ms < 0.0f || ms < chosen_ms) {
+            chosen_ms = ms;
+            chosen = t;
+          }
+        }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != RDRT_OK) {
+    c->tune = before;
+    return rc;
+  }
+  // the cull pass is not part of a frame; keep the context's choice for rdrt_trace
+  chosen.cull = before.cull;
+  c->tune = chosen;
+  c->tune_ms = chosen_ms;
+  if (best) return rdrt_get_tune(c, best);
+  return RDRT_OK;
+}
+
 int32_t rdrt_pt_set_material_params(rdrt_ctx *c, const float *albedo, const float *emission, uint32_t n) {
   if (!c || !albedo || !emission) return fail(RDRT_ERR_INVALID_ARG, "null argument");
   if (n == 0 || n >= kVertSkyUp) return fail(RDRT_ERR_INVALID_ARG, "material count out of range");

@@ -60,62 +60,90 @@ __device__ __forceinline__ void block_compact2(bool ka, bool kb, uint32_t item, 
 }
 
 constexpr uint32_t kScanThreads = 1024;
+constexpr uint32_t kScanPer = 8;                         // counts per thread per tile
+constexpr uint32_t kScanTile = kScanThreads * kScanPer;  // counts per tile
 
 namespace {
 
 // Exclusive scan of the block counts of the first ceil(min(*count_in,
 // slots) / B) blocks (one workgroup of kScanThreads): offsets and totals.
-// cnt_b / off_b / total_b may be null.
+// cnt_b / off_b / total_b may be null.  The counts stream through LDS in
+// tiles of kScanTile with coalesced loads and stores; each thread scans
+// kScanPer consecutive counts of a tile, the workgroup scans the thread sums,
+// and a running carry links the tiles.
 __global__ __launch_bounds__(kScanThreads) void k_block_scan(const uint32_t *count_in, uint32_t slots, uint32_t B,
                                                              const uint32_t *cnt_a, const uint32_t *cnt_b,
                                                              uint32_t *off_a, uint32_t *off_b, uint32_t *total_a,
                                                              uint32_t *total_b) {
+  __shared__ uint32_t t_a[kScanTile], t_b[kScanTile];
   __shared__ uint32_t s_a[kScanThreads / 64], s_b[kScanThreads / 64];
   const uint32_t n = count_in ? min(*count_in, slots) : slots;
   const uint32_t nb = (n + B - 1) / B;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  const uint32_t chunk = (nb + kScanThreads - 1) / kScanThreads;
-  const uint32_t lo = min(tid * chunk, nb), hi = min(lo + chunk, nb);
-  uint32_t sa = 0, sb = 0;
-  for (uint32_t i = lo; i < hi; i++) {
-    sa += cnt_a[i];
-    if (cnt_b) sb += cnt_b[i];
-  }
-  // inclusive wave scans, then wave totals through LDS
-  uint32_t ia = sa, ib = sb;
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t xa = __shfl_up(ia, o, 64), xb = __shfl_up(ib, o, 64);
-    if (lane >= (uint32_t)o) {
-      ia += xa;
-      ib += xb;
+  const bool two = cnt_b != nullptr;
+  uint32_t carry_a = 0, carry_b = 0;
+  for (uint32_t base = 0; base < nb; base += kScanTile) {
+    for (uint32_t k = 0; k < kScanPer; k++) {
+      const uint32_t i = base + k * kScanThreads + tid;
+      t_a[k * kScanThreads + tid] = i < nb ? cnt_a[i] : 0u;
+      if (two) t_b[k * kScanThreads + tid] = i < nb ? cnt_b[i] : 0u;
     }
-  }
-  if (lane == 63) {
-    s_a[wave] = ia;
-    s_b[wave] = ib;
-  }
-  __syncthreads();
-  uint32_t wa = 0, wb = 0, ta = 0, tb = 0;
-  for (uint32_t w = 0; w < kScanThreads / 64; w++) {
-    if (w == wave) {
-      wa = ta;
-      wb = tb;
+    __syncthreads();
+    uint32_t sa = 0, sb = 0;
+    for (uint32_t k = 0; k < kScanPer; k++) {
+      sa += t_a[tid * kScanPer + k];
+      if (two) sb += t_b[tid * kScanPer + k];
     }
-    ta += s_a[w];
-    tb += s_b[w];
-  }
-  uint32_t ea = wa + ia - sa, eb = wb + ib - sb;  // exclusive prefix of this thread's chunk
-  for (uint32_t i = lo; i < hi; i++) {
-    off_a[i] = ea;
-    ea += cnt_a[i];
-    if (cnt_b) {
-      off_b[i] = eb;
-      eb += cnt_b[i];
+    // inclusive wave scans, then wave totals through LDS
+    uint32_t ia = sa, ib = sb;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t xa = __shfl_up(ia, o, 64), xb = __shfl_up(ib, o, 64);
+      if (lane >= (uint32_t)o) {
+        ia += xa;
+        ib += xb;
+      }
     }
+    if (lane == 63) {
+      s_a[wave] = ia;
+      s_b[wave] = ib;
+    }
+    __syncthreads();
+    uint32_t wa = 0, wb = 0, ta = 0, tb = 0;
+    for (uint32_t w = 0; w < kScanThreads / 64; w++) {
+      if (w == wave) {
+        wa = ta;
+        wb = tb;
+      }
+      ta += s_a[w];
+      tb += s_b[w];
+    }
+    // exclusive prefixes of this thread's counts, written back in place
+    uint32_t ea = carry_a + wa + ia - sa, eb = carry_b + wb + ib - sb;
+    for (uint32_t k = 0; k < kScanPer; k++) {
+      const uint32_t va = t_a[tid * kScanPer + k];
+      t_a[tid * kScanPer + k] = ea;
+      ea += va;
+      if (two) {
+        const uint32_t vb = t_b[tid * kScanPer + k];
+        t_b[tid * kScanPer + k] = eb;
+        eb += vb;
+      }
+    }
+    __syncthreads();
+    for (uint32_t k = 0; k < kScanPer; k++) {
+      const uint32_t i = base + k * kScanThreads + tid;
+      if (i < nb) {
+        off_a[i] = t_a[k * kScanThreads + tid];
+        if (two) off_b[i] = t_b[k * kScanThreads + tid];
+      }
+    }
+    carry_a += ta;
+    carry_b += tb;
+    __syncthreads();  // the tile's LDS is reused by the next tile
   }
   if (tid == 0) {
-    if (total_a) *total_a = ta;
-    if (total_b) *total_b = tb;
+    if (total_a) *total_a = carry_a;
+    if (total_b) *total_b = carry_b;
   }
 }
 

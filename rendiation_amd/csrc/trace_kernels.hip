@@ -126,11 +126,12 @@ __global__ __launch_bounds__(kCullBlock) void k_trace_cull(TraceParams p, CullSc
     float tn;
     keep = root && slab(r.o, recip(r.d), r.tmin, r.tmax, root->mn[0], root->mn[1], root->mn[2], root->mx[0],
                         root->mx[1], root->mx[2], tn);
-    if (!keep) {
-      write_result(p, p.tri_ids, i, Best{}, 0u, 0u, 0u);
-      for (uint32_t k = 0; k < cs.later_rounds; k++)
-        *reinterpret_cast<uint4 *>(cs.later + (size_t)(k + 1) * cs.stride + i) = make_uint4(RDRT_MISS, 0u, 0u, 0u);
-    }
+    // Every ray gets miss records, kept ones too (their traversal, later in
+    // stream order, overwrites them): whole lines are written instead of a
+    // scatter of partial ones.
+    write_result(p, p.tri_ids, i, Best{}, 0u, 0u, 0u);
+    for (uint32_t k = 0; k < cs.later_rounds; k++)
+      *reinterpret_cast<uint4 *>(cs.later + (size_t)(k + 1) * cs.stride + i) = make_uint4(RDRT_MISS, 0u, 0u, 0u);
   }
   compact::block_compact2<kCullBlock>(keep, false, i, bid, cs.tmp, nullptr, cs.blk, nullptr);
 }

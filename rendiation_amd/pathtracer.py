@@ -145,6 +145,23 @@ class PathTracer:
             self.recorded_frames += 1
         return out
 
+    def autotune(self, camera, cfg: PathTracingConfig, out=None, reps: int = 3) -> dict:
+        """Time every traversal launch shape on eager frames of this camera
+        and configuration and keep the fastest for the system (``out`` is
+        overwritten); returns the choice, its ``ms`` the frame time."""
+        import torch
+
+        if cfg.record or cfg.diagnostics:
+            raise ValueError("tune on plain frames (record and diagnostics off)")
+        if out is None:
+            out = torch.empty((cfg.height, cfg.width, 3), dtype=torch.float32, device=f"cuda:{self.system.device}")
+        if not out.is_contiguous() or out.numel() != cfg.width * cfg.height * 3:
+            raise ValueError("out must be a contiguous height*width*3 float32 tensor")
+        prm = cfg.to_c(camera)
+        t = L.RdrtTune()
+        L.check(self.system._L.rdrt_pt_autotune(self.system._h, C.byref(prm), int(out.data_ptr()), reps, C.byref(t)))
+        return self.system.tune()
+
     def set_material_params(self, albedo, emission) -> None:
         """Replace the materials with device tensors (n, 3) float32 each,
         copied on the system's stream (no host round trip)."""

@@ -199,3 +199,27 @@ def test_cull_pass_equals_plain_rounds(batch_case, device):
         assert np.array_equal(sp[r][alive], sc[r][alive]), r
     assert stc.rays[0] < stp.rays[0]  # round 0 traverses only the rays that enter a scene box
     assert stc.rays[1:] == stp.rays[1:]
+
+
+def test_batch_autotune(batch_case, device):
+    """The batched tuner keeps one shape; persistent and sequential calls
+    with it stay bit-identical, and equal the oracle except near-ties."""
+    import torch
+
+    sys, orc, _, rays, meta, offs = batch_case
+    _say("batched autotune")
+    n = rays.shape[0]
+    d_rays = torch.from_numpy(np.ascontiguousarray(rays)).to(device)
+    d_meta = torch.from_numpy(np.ascontiguousarray(meta).view(np.int32)).to(device)
+    hits = torch.zeros((BOUNCES + 1) * n * 4, dtype=torch.int32, device=device)
+    try:
+        choice = sys.batch_autotune(d_rays, d_meta, hits, bounces=BOUNCES, seed=SEED, reps=1)
+        print("batch autotune:", choice)
+        assert choice["block_size"] in (64, 128, 256) and choice["ms"] > 0
+        hp, _, _ = _run(sys, rays, meta, device, BATCH_PERSISTENT)
+        hs, _, _ = _run(sys, rays, meta, device, BATCH_SEQUENTIAL, offs=offs)
+    finally:
+        sys.set_tune()
+    assert np.array_equal(hp, hs)
+    oh, _ = orc.trace(rays, meta)[:2]
+    assert compare(hp[0], oh).ok()

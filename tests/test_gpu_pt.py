@@ -240,3 +240,26 @@ def test_shared_launches_equal_separate_launches(device):
     with pytest.raises(Exception):
         pt.render(cam, dataclasses.replace(cfg, diagnostics=_lib.RDRT_PT_DIAG_COUNT_FAST
                                                  | _lib.RDRT_PT_DIAG_COUNT_VISITS))
+
+
+def test_frame_autotune(device):
+    """The frame autotuner keeps one shape of the search space; frames
+    afterwards equal an eager frame rendered with that shape set by hand."""
+    import dataclasses
+
+    scene = synth.pt_scene(seed=3, n_meshes=16, tess=24, extent=5.0)
+    cam = Camera(position=(0.0, 0.0, 18.0))
+    _say("frame autotune")
+    s = _system(scene)
+    pt = PathTracer(s)
+    cfg = PathTracingConfig(width=96, height=64, spp=1, max_bounces=3, use_graph=True)
+    choice = pt.autotune(cam, cfg, reps=1)
+    print("pt autotune:", choice)
+    assert choice["block_size"] in (64, 128, 256) and choice["wide"] in (0, 1) and choice["ms"] > 0
+    img = pt.render(cam, cfg).cpu().numpy()
+    s.set_tune(choice["block_size"], choice["lds_entries"], choice["waves"], choice["persistent"], choice["cull"],
+               choice["wide"])
+    eager = pt.render(cam, dataclasses.replace(cfg, use_graph=False)).cpu().numpy()
+    assert np.array_equal(img, eager)
+    with pytest.raises(ValueError):
+        pt.autotune(cam, dataclasses.replace(cfg, record=True))
