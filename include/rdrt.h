@@ -159,6 +159,7 @@ typedef struct rdrt_scene_info {
   uint32_t quad_stack_entries;  /* the same for the child-quad layout */
   uint64_t device_bytes;
   double build_ms;
+  uint32_t tri_quads, tlas_quads;  /* child-quad nodes (and their quantized twins) */
 } rdrt_scene_info;
 
 /* ---- errors / version ---- */
@@ -193,7 +194,8 @@ int32_t rdrt_get_scene_info(const rdrt_ctx *ctx, rdrt_scene_info *out);
 /* Copies of the built reference-layout arrays (host memory, sized from
  * rdrt_scene_info), for inspection and parity checks.  which:
  * 0 = BLAS forest (48 B nodes), 1 = TLAS forest (48 B nodes),
- * 2 = triangle redirect (u32), 3 = instance source order (u32). */
+ * 2 = triangle redirect (u32), 3 = instance source order (u32),
+ * 4 = BLAS child quads (128 B nodes), 5 = their quantized twins (64 B). */
 int32_t rdrt_copy_built(const rdrt_ctx *ctx, uint32_t which, void *host_out, size_t bytes);
 
 /* ---- tracing ---- */
@@ -214,12 +216,13 @@ typedef struct rdrt_tune {
   uint32_t waves;       /* 8 = compile for 8 waves per SIMD (at most 64 VGPRs); 0 = the compiler's choice */
   uint32_t persistent;  /* 1 = resident-size grid whose waves pull 64-ray chunks from a counter */
   uint32_t cull;        /* 1 = a first pass finishes rays that miss their scene's root box and queues the rest */
-  uint32_t wide;        /* 1 = walk the child-quad layout (four child boxes per 128-byte node) */
+  uint32_t wide;        /* node layout: 0 = child pairs (64 B), 1 = child quads (four boxes, 128 B),
+                           2 = quantized child quads (four 8-bit boxes, 64 B) */
   float ms;             /* rdrt_autotune: per-launch time of the chosen shape on the sample */
 } rdrt_tune;
 /* Times every shape of the search space (3 block sizes x 2 LDS stack depths
  * x 2 occupancy targets x grid / persistent x with / without the cull pass
- * x child-pair / child-quad layout) on a sample launch (the args of
+ * x the three node layouts) on a sample launch (the args of
  * rdrt_trace; its hits are overwritten), `reps` timed launches each, and
  * keeps the fastest for every later traversal of this context (rdrt_trace,
  * the path tracer, batched tracing).  Synchronizes.  best may be NULL. */
@@ -309,8 +312,8 @@ int32_t rdrt_pt_render(rdrt_ctx *ctx, const rdrt_pt_params *params, float *radia
 /* Ray counts of the last frame (synchronizes). */
 int32_t rdrt_pt_get_stats(rdrt_ctx *ctx, rdrt_pt_stats *out);
 /* Times every launch shape of the fast traversal (3 block sizes x 2 LDS
- * stack depths x 2 occupancy targets x grid / persistent x child-pair /
- * child-quad layout) on eager frames of these parameters (record and
+ * stack depths x 2 occupancy targets x grid / persistent x the three node
+ * layouts) on eager frames of these parameters (record and
  * diagnostics off), `reps` timed frames each, and keeps the fastest for the
  * context, as rdrt_autotune does for single launches; best->ms is the frame
  * time.  radiance_dev is overwritten.  Synchronizes.  best may be NULL.
@@ -380,7 +383,7 @@ typedef struct rdrt_batch_stats {
 int32_t rdrt_trace_batched(rdrt_ctx *ctx, const rdrt_batch_params *params);
 int32_t rdrt_batch_get_stats(rdrt_ctx *ctx, rdrt_batch_stats *out);
 /* Times every launch shape (3 block sizes x 2 LDS stack depths x 2 occupancy
- * targets x with / without the cull pass x child-pair / child-quad layout)
+ * targets x with / without the cull pass x the three node layouts)
  * on whole persistent batched calls of these parameters, `reps` timed calls
  * each, and keeps the fastest for the context; best->ms is the call time.
  * The outputs of params are overwritten.  Synchronizes.  best may be NULL.

@@ -121,6 +121,8 @@ class SceneInfo:
     quad_stack_entries: int
     device_bytes: int
     build_ms: float
+    tri_quads: int = 0
+    tlas_quads: int = 0
 
 
 @dataclass
@@ -233,19 +235,22 @@ class AccelerationStructureSystem:
         L.check(self._L.rdrt_get_scene_info(self._h, C.byref(info)))
         return SceneInfo(info.tri_nodes, info.tlas_nodes, info.tris, info.verts, info.instances, info.geometries,
                          info.blas, info.tlas, info.max_blas_depth, info.max_tlas_depth, info.stack_entries,
-                         info.quad_stack_entries, info.device_bytes, info.build_ms)
+                         info.quad_stack_entries, info.device_bytes, info.build_ms, info.tri_quads, info.tlas_quads)
 
     def copy_built(self, which: str) -> np.ndarray:
-        """Built reference-layout arrays: 'blas_forest' / 'tlas_forest' (48 B
-        nodes as (n, 12) uint32), 'redirect', 'instance_order' (uint32)."""
+        """Built arrays: 'blas_forest' / 'tlas_forest' (48 B nodes as (n, 12)
+        uint32), 'redirect', 'instance_order' (uint32), 'blas_quads' (128 B
+        child-quad nodes as (n, 32) uint32), 'blas_qquads' (64 B quantized
+        twins as (n, 16) uint32)."""
         info = self.scene_info()
-        table = {"blas_forest": (0, info.tri_nodes * 12), "tlas_forest": (1, info.tlas_nodes * 12),
-                 "redirect": (2, info.tris), "instance_order": (3, info.instances)}
-        code, n = table[which]
+        table = {"blas_forest": (0, info.tri_nodes * 12, 12), "tlas_forest": (1, info.tlas_nodes * 12, 12),
+                 "redirect": (2, info.tris, 0), "instance_order": (3, info.instances, 0),
+                 "blas_quads": (4, info.tri_quads * 32, 32), "blas_qquads": (5, info.tri_quads * 16, 16)}
+        code, n, width = table[which]
         out = np.zeros(max(1, n), dtype=np.uint32)
         L.check(self._L.rdrt_copy_built(self._h, code, out.ctypes.data, out.nbytes))
         out = out[:n]
-        return out.reshape(-1, 12) if which.endswith("forest") else out
+        return out.reshape(-1, width) if width else out
 
     # ---- shading attributes ----
     def set_vertex_normals(self, blas: BlasHandle, geometry: int, normals: np.ndarray) -> None:

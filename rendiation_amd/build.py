@@ -24,8 +24,9 @@ LIB = PKG / "librdrt.so"
 ARCH = os.environ.get("RDRT_OFFLOAD_ARCH", "gfx950")
 HOST_SOURCES = ["bvh_build.cpp", "scene.cpp"]
 HIP_SOURCES = ["trace_fast_256.hip", "trace_fast_128.hip", "trace_fast_64.hip", "trace_quad_256.hip",
-               "trace_quad_128.hip", "trace_quad_64.hip", "trace_kernels.hip", "pt_kernels.hip", "batch_kernels.hip",
-               "api.hip", "api_pt.hip", "api_batch.hip"]
+               "trace_quad_128.hip", "trace_quad_64.hip", "trace_qquad_256.hip", "trace_qquad_128.hip",
+               "trace_qquad_64.hip", "trace_kernels.hip", "pt_kernels.hip", "batch_kernels.hip", "api.hip",
+               "api_pt.hip", "api_batch.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 HIP_FLAGS = [

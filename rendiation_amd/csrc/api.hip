@@ -42,6 +42,8 @@ void free_device_scene(DeviceScene &d) {
   release(d.tlas_pairs);
   release(d.tri_quads);
   release(d.tlas_quads);
+  release(d.tri_qquads);
+  release(d.tlas_qquads);
   release(d.tlas_quad_roots);
   release(d.tris);
   release(d.redirect);
@@ -77,6 +79,8 @@ TraceParams scene_trace_params(const rdrt_ctx *c) {
   p.tlas_pairs = d.tlas_pairs.ptr;
   p.tri_quads = d.tri_quads.ptr;
   p.tlas_quads = d.tlas_quads.ptr;
+  p.tri_qquads = d.tri_qquads.ptr;
+  p.tlas_qquads = d.tlas_qquads.ptr;
   p.tlas_quad_roots = d.tlas_quad_roots.ptr;
   p.insts = d.insts.ptr;
   p.tlas_roots = d.tlas_roots.ptr;
@@ -301,6 +305,8 @@ int32_t rdrt_commit(rdrt_ctx *c) {
   RDRT_HIP(upload(d.tlas_pairs, b.tlas_pairs, d.bytes));
   RDRT_HIP(upload(d.tri_quads, b.tri_quads, d.bytes));
   RDRT_HIP(upload(d.tlas_quads, b.tlas_quads, d.bytes));
+  RDRT_HIP(upload(d.tri_qquads, b.tri_qquads, d.bytes));
+  RDRT_HIP(upload(d.tlas_qquads, b.tlas_qquads, d.bytes));
   RDRT_HIP(upload(d.tlas_quad_roots, b.tlas_quad_roots, d.bytes));
   RDRT_HIP(upload(d.tris, b.tris, d.bytes));
   RDRT_HIP(upload(d.redirect, b.redirect, d.bytes));
@@ -338,6 +344,8 @@ int32_t rdrt_get_scene_info(const rdrt_ctx *c, rdrt_scene_info *o) {
   o->quad_stack_entries = stack_entries_for(b.blas_stack4 + b.tlas_stack4);
   o->device_bytes = c->dev.bytes;
   o->build_ms = b.build_ms;
+  o->tri_quads = (uint32_t)b.tri_quads.size();
+  o->tlas_quads = (uint32_t)b.tlas_quads.size();
   return RDRT_OK;
 }
 
@@ -365,6 +373,14 @@ int32_t rdrt_copy_built(const rdrt_ctx *c, uint32_t which, void *out, size_t byt
       for (const InstRec &r : b.insts) order.push_back(r.order);
       src = order.data();
       need = order.size() * 4;
+      break;
+    case 4:
+      src = b.tri_quads.data();
+      need = b.tri_quads.size() * sizeof(QuadNode);
+      break;
+    case 5:
+      src = b.tri_qquads.data();
+      need = b.tri_qquads.size() * sizeof(QuadNodeQ);
       break;
     default:
       return fail(RDRT_ERR_INVALID_ARG, "unknown array");
@@ -439,9 +455,9 @@ int32_t rdrt_set_tune(rdrt_ctx *c, const rdrt_tune *t) {
   if (!c || !t) return fail(RDRT_ERR_INVALID_ARG, "null argument");
   if ((t->block_size != 64 && t->block_size != 128 && t->block_size != 256) ||
       (t->lds_entries != 8 && t->lds_entries != 16) || (t->waves != 0 && t->waves != 8) || t->persistent > 1 ||
-      t->cull > 1 || t->wide > 1)
+      t->cull > 1 || t->wide > 2)
     return fail(RDRT_ERR_INVALID_ARG,
-                "block_size 64/128/256, lds_entries 8/16, waves 0/8, persistent 0/1, cull 0/1, wide 0/1");
+                "block_size 64/128/256, lds_entries 8/16, waves 0/8, persistent 0/1, cull 0/1, wide 0/1/2");
   TraceTune n;
   n.block = t->block_size;
   n.lds = t->lds_entries;
@@ -496,7 +512,7 @@ int32_t rdrt_autotune(rdrt_ctx *c, const rdrt_trace_args *a, uint32_t reps, rdrt
   for (uint32_t block : {64u, 128u, 256u})
     for (uint32_t lds : {8u, 16u})
       for (uint32_t waves : {0u, 8u})
-        for (uint32_t shape = 0; shape < 8; shape++) {
+        for (uint32_t shape = 0; shape < 12; shape++) {
           TraceTune t;
           t.block = block;
           t.lds = lds;

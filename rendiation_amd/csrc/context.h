@@ -69,6 +69,7 @@ struct Scratch {
 struct DeviceScene {
   DevArray<PairNode> tri_pairs, tlas_pairs;
   DevArray<QuadNode> tri_quads, tlas_quads;
+  DevArray<QuadNodeQ> tri_qquads, tlas_qquads;
   DevArray<TriPre> tris;
   DevArray<uint32_t> redirect, tri_ids, indices, binding, tri_material, tlas_quad_roots;
   DevArray<float> vertices, normals;

@@ -50,6 +50,22 @@ struct alignas(128) QuadNode {
 };
 static_assert(sizeof(QuadNode) == 128, "QuadNode must be 128 B");
 
+// Quantized child-quad node, 64 B: the node box's lower corner, a
+// power-of-two step per axis and every child box as 8-bit multiples of the
+// steps from that corner (byte c of each word = child c).  Lower bounds are
+// rounded down and upper bounds up, checked with the decoding arithmetic
+// (corner + q * step, one rounding), so a decoded box always contains the
+// child's exact box: the traversal may enter a few more nodes, never fewer,
+// and the triangle tests are unchanged.  Node i has the refs of QuadNode i.
+struct alignas(64) QuadNodeQ {
+  float ox, oy, oz, sx;  // lower corner, x step
+  uint32_t ref[4];
+  uint32_t lox, hix, loy, hiy;
+  uint32_t loz, hiz;
+  float sy, sz;
+};
+static_assert(sizeof(QuadNodeQ) == 64, "QuadNodeQ must be 64 B");
+
 // Child ref: bit 31 clear = index of an inner PairNode; bit 31 set = leaf
 // with `count` (bits 0..4, < 32) primitives from `start` (bits 5..30).
 constexpr uint32_t kLeafBit = 0x80000000u;

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -275,6 +276,82 @@ struct QuadConv {
   }
 };
 
+// Quantized twin of a child-quad node (see QuadNodeQ).  Per axis the step
+// is the smallest power of two for which every child's rounded bounds fit in
+// 8 bits; each bound is then moved outward until corner + q * step (FP32,
+// one rounding, as the kernels decode it) contains the exact bound.
+QuadNodeQ quantize_quad(const QuadNode &q) {
+  QuadNodeQ out;
+  std::memset(&out, 0, sizeof(out));
+  int n = 0;
+  while (n < 4 && q.ref[n] != kInvalid) n++;
+  const float *mn[3] = {q.mnx, q.mny, q.mnz}, *mx[3] = {q.mxx, q.mxy, q.mxz};
+  float corner[3], step[3];
+  uint32_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int a = 0; a < 3; a++) {
+    float l = mn[a][0], h = mx[a][0];
+    for (int k = 1; k < n; k++) {
+      l = std::min(l, mn[a][k]);
+      h = std::max(h, mx[a][k]);
+    }
+    corner[a] = l;
+    const double ext = (double)h - (double)l;
+    const double min_step = std::ldexp(1.0, -126), max_step = std::ldexp(1.0, 127);
+    double s = min_step;
+    if (ext > 0.0 && std::isfinite(ext)) s = std::max(min_step, std::ldexp(1.0, (int)std::ceil(std::log2(ext / 255.0))));
+    if (!std::isfinite(ext)) s = 2.0 * max_step;
+    for (;;) {
+      if (s > max_step) {
+        // no power-of-two step fits (non-finite bounds): the whole axis
+        // decodes to [corner, +inf), which contains every child
+        step[a] = FLT_MAX;
+        lo[a] = 0u;
+        hi[a] = 0u;
+        for (int k = 0; k < n; k++) hi[a] |= 255u << (8 * k);
+        break;
+      }
+      const float sf = (float)s;
+      bool ok = true;
+      uint32_t wl = 0, wh = 0;
+      for (int k = 0; k < n && ok; k++) {
+        double ql = std::floor(((double)mn[a][k] - (double)l) / s), qh = std::ceil(((double)mx[a][k] - (double)l) / s);
+        ql = std::min(std::max(ql, 0.0), 255.0);
+        qh = std::max(qh, 0.0);
+        uint32_t il = (uint32_t)ql, ih = (uint32_t)std::min(qh, 256.0);
+        while (il > 0 && l + (float)il * sf > mn[a][k]) il--;
+        while (ih <= 255 && l + (float)ih * sf < mx[a][k]) ih++;
+        if (ih > 255) {
+          ok = false;
+          break;
+        }
+        wl |= il << (8 * k);
+        wh |= ih << (8 * k);
+      }
+      if (ok) {
+        step[a] = sf;
+        lo[a] = wl;
+        hi[a] = wh;
+        break;
+      }
+      s *= 2.0;
+    }
+  }
+  out.ox = corner[0];
+  out.oy = corner[1];
+  out.oz = corner[2];
+  out.sx = step[0];
+  out.sy = step[1];
+  out.sz = step[2];
+  out.lox = lo[0];
+  out.hix = hi[0];
+  out.loy = lo[1];
+  out.hiy = hi[1];
+  out.loz = lo[2];
+  out.hiz = hi[2];
+  for (int k = 0; k < 4; k++) out.ref[k] = q.ref[k];
+  return out;
+}
+
 // Child-pair conversion of the fast kernel's TLAS tree, whose leaves hold
 // single instances addressed by their forest position: map[order index].  A
 // depth-capped leaf with several instances becomes a chain of pair nodes.
@@ -402,6 +479,17 @@ bool validate_refs(const BuiltScene &s, std::string &err) {
       err = "internal: TLAS child-quad ref out of range";
       return false;
     }
+  auto twins = [](const std::vector<QuadNode> &a, const std::vector<QuadNodeQ> &b) {
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); i++)
+      for (int k = 0; k < 4; k++)
+        if (a[i].ref[k] != b[i].ref[k]) return false;
+    return true;
+  };
+  if (!twins(s.tri_quads, s.tri_qquads) || !twins(s.tlas_quads, s.tlas_qquads)) {
+    err = "internal: quantized quads disagree with the quads";
+    return false;
+  }
   for (const GeoRec &g : s.geos)
     if (!ref_ok(g.quad_root, s.tri_quads.size(), s.tris.size())) {
       err = "internal: geometry quad root out of range";
@@ -655,6 +743,14 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
     out.tlas_quad_roots.push_back(qc.ref_of(root.root_ref, d));
     out.tlas_stack4 = std::max(out.tlas_stack4, d);
   }
+  // quantized twins of the child-quad nodes
+  out.tri_qquads.resize(out.tri_quads.size());
+  parallel_for((out.tri_quads.size() + 65535) / 65536, hw, [&](size_t c) {
+    const size_t e = std::min(out.tri_quads.size(), (c + 1) * 65536);
+    for (size_t k = c * 65536; k < e; k++) out.tri_qquads[k] = quantize_quad(out.tri_quads[k]);
+  });
+  out.tlas_qquads.resize(out.tlas_quads.size());
+  for (size_t k = 0; k < out.tlas_quads.size(); k++) out.tlas_qquads[k] = quantize_quad(out.tlas_quads[k]);
   out.binding = src.binding;
   if (!validate_refs(out, err)) return RDRT_ERR_STATE;
   out.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -60,6 +60,7 @@ struct BuiltScene {
   uint32_t blas_stack = 0, tlas_stack = 0;  // deepest push count of the child-pair trees
   // child-quad layout (collapsed from the child-pair trees; same leaves)
   std::vector<QuadNode> tri_quads, tlas_quads;
+  std::vector<QuadNodeQ> tri_qquads, tlas_qquads;  // quantized twins (same indices and refs)
   std::vector<uint32_t> tlas_quad_roots;      // per TLAS (kInvalid when deleted)
   uint32_t blas_stack4 = 0, tlas_stack4 = 0;  // deepest push count of the child-quad trees
   double build_ms = 0.0;

@@ -279,12 +279,52 @@ __device__ __forceinline__ bool step_quad(const QuadNode *nodes, uint32_t &ref, 
   return true;
 }
 
+// Byte c of a quantized node word, decoded: corner + q * step (q * step is
+// exact, the sum rounds once, as on the host that chose q).
+__device__ __forceinline__ float dequant(float corner, uint32_t word, int c, float step) {
+  return corner + (float)((word >> (8 * c)) & 0xffu) * step;
+}
+
+template <class Push>
+__device__ __forceinline__ bool step_qquad(const QuadNodeQ *nodes, uint32_t &ref, F3 o, F3 inv_d, float tmin,
+                                           float tmax, Push &&push) {
+  const float4 *q = reinterpret_cast<const float4 *>(nodes + ref);
+  const float4 a = q[0];                                  // corner, x step
+  const uint4 rf = *reinterpret_cast<const uint4 *>(q + 1);
+  const uint4 xy = *reinterpret_cast<const uint4 *>(q + 2);  // lox, hix, loy, hiy
+  const float4 zs = q[3];                                 // loz, hiz (bits), y step, z step
+  const uint32_t loz = __float_as_uint(zs.x), hiz = __float_as_uint(zs.y);
+  float t[4];
+  bool h[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+    h[c] = slab(o, inv_d, tmin, tmax, dequant(a.x, xy.x, c, a.w), dequant(a.y, xy.z, c, zs.z),
+                dequant(a.z, loz, c, zs.w), dequant(a.x, xy.y, c, a.w), dequant(a.y, xy.w, c, zs.z),
+                dequant(a.z, hiz, c, zs.w), t[c]);
+  h[2] = h[2] && rf.z != kInvalid;  // slots 0, 1 always used
+  h[3] = h[3] && rf.w != kInvalid;
+  const uint32_t n = (uint32_t)h[0] + (uint32_t)h[1] + (uint32_t)h[2] + (uint32_t)h[3];
+  if (n == 0) return false;
+  float k0 = quad_key(h[0], t[0]), k1 = quad_key(h[1], t[1]), k2 = quad_key(h[2], t[2]), k3 = quad_key(h[3], t[3]);
+  uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
+  order2(k0, r0, k1, r1);
+  order2(k2, r2, k3, r3);
+  order2(k0, r0, k2, r2);
+  order2(k1, r1, k3, r3);
+  order2(k1, r1, k2, r2);
+  if (n > 3) push(r3);
+  if (n > 2) push(r2);
+  if (n > 1) push(r1);
+  ref = r0;
+  return true;
+}
+
 // ------------------------------------------------------------------------
 // Fast kernel: front-to-back child-pair traversal.  Stack: the first K
 // entries of each lane live in LDS (column layout), deeper ones (rare) in
 // the lane's private memory; S bounds the total.
 // ------------------------------------------------------------------------
-template <int K, int S, int B, bool CNT, bool QUAD>
+template <int K, int S, int B, bool CNT, int LAYOUT>
 __device__ __forceinline__ void trace_fast_one(const TraceParams &p, uint32_t rid, uint32_t *stk) {
   uint32_t spill[S > K ? S - K : 1];
   auto push = [&](uint32_t &sp, uint32_t v) {
@@ -313,11 +353,13 @@ __device__ __forceinline__ void trace_fast_one(const TraceParams &p, uint32_t ri
   if (root && slab(r.o, winv, r.tmin, far_, root->mn[0], root->mn[1], root->mn[2], root->mx[0], root->mx[1],
                    root->mx[2], tn)) {
     uint32_t sp = 0;
-    uint32_t ref = QUAD ? p.tlas_quad_roots[root - p.tlas_roots] : root->root_ref;
+    uint32_t ref = LAYOUT ? p.tlas_quad_roots[root - p.tlas_roots] : root->root_ref;
     auto push_tlas = [&](uint32_t v) { push(sp, v); };
     for (;;) {
       if (!(ref & kLeafBit)) {
-        if constexpr (QUAD) {
+        if constexpr (LAYOUT == 2) {
+          if (step_qquad(p.tlas_qquads, ref, r.o, winv, r.tmin, far_, push_tlas)) continue;
+        } else if constexpr (LAYOUT == 1) {
           if (step_quad(p.tlas_quads, ref, r.o, winv, r.tmin, far_, push_tlas)) continue;
         } else {
           if (step_pair(p.tlas_pairs, ref, r.o, winv, r.tmin, far_, push_tlas)) continue;
@@ -349,13 +391,15 @@ __device__ __forceinline__ void trace_fast_one(const TraceParams &p, uint32_t ri
               continue;
             // BLAS traversal; its stack entries sit above the TLAS ones.
             uint32_t bsp = sp;
-            uint32_t bref = QUAD ? geo.quad_root : geo.root_ref;
+            uint32_t bref = LAYOUT ? geo.quad_root : geo.root_ref;
             auto push_blas = [&](uint32_t v) { push(bsp, v); };
             for (;;) {
               if (!(bref & kLeafBit)) {
                 if constexpr (CNT) cn++;
                 const float fs = far_ * s;
-                if constexpr (QUAD) {
+                if constexpr (LAYOUT == 2) {
+                  if (step_qquad(p.tri_qquads, bref, bo, binv, near_s, fs, push_blas)) continue;
+                } else if constexpr (LAYOUT == 1) {
                   if (step_quad(p.tri_quads, bref, bo, binv, near_s, fs, push_blas)) continue;
                 } else {
                   if (step_pair(p.tri_pairs, bref, bo, binv, near_s, fs, push_blas)) continue;
@@ -423,7 +467,7 @@ __device__ __forceinline__ void select_second(TraceParams &p) {
 // One ray per thread over a grid sized for both segments' slots; the live
 // counts may be smaller (device queue sizes).  The blocks of the second
 // segment follow those of the first, so a block's segment is uniform.
-template <int K, int S, int B, int W, bool CNT, bool QUAD>
+template <int K, int S, int B, int W, bool CNT, int LAYOUT>
 __global__ __launch_bounds__(B, W) void k_trace_fast(TraceParams p) {
   __shared__ uint32_t lds[K * B];
   const uint32_t n0 = live_first(p), n1 = live_second(p);
@@ -441,7 +485,7 @@ __global__ __launch_bounds__(B, W) void k_trace_fast(TraceParams p) {
   if (slot >= n) return;
   const uint32_t rid = p.queue ? p.queue[slot] : slot;
   if (rid >= p.n) return;  // a queue entry always names one of the n rays
-  trace_fast_one<K, S, B, CNT, QUAD>(p, rid, lds + threadIdx.x);
+  trace_fast_one<K, S, B, CNT, LAYOUT>(p, rid, lds + threadIdx.x);
 }
 
 // Persistent variant: a grid sized to the resident capacity; each wave
@@ -450,7 +494,7 @@ __global__ __launch_bounds__(B, W) void k_trace_fast(TraceParams p) {
 // across the chip.  Every wave exits once a fetch lands past the count.
 // The second segment's slots start at the first 64-aligned position after
 // the first segment's, so every chunk lies in one segment.
-template <int K, int S, int B, int W, bool CNT, bool QUAD>
+template <int K, int S, int B, int W, bool CNT, int LAYOUT>
 __global__ __launch_bounds__(B, W) void k_trace_persist(TraceParams p) {
   __shared__ uint32_t lds[K * B];
   const uint32_t n0 = live_first(p), n1 = live_second(p);
@@ -472,7 +516,7 @@ __global__ __launch_bounds__(B, W) void k_trace_persist(TraceParams p) {
     const uint32_t slot = base + lane;
     if (slot < n) {
       const uint32_t rid = q.queue ? q.queue[slot] : slot;
-      if (rid < q.n) trace_fast_one<K, S, B, CNT, QUAD>(q, rid, lds + threadIdx.x);
+      if (rid < q.n) trace_fast_one<K, S, B, CNT, LAYOUT>(q, rid, lds + threadIdx.x);
     }
   }
 }
@@ -481,51 +525,51 @@ __global__ void k_zero_work(uint32_t *w) { *w = 0u; }
 
 // One fast-traversal launch with block size B, total stack S and tuning t.
 // K = stack entries per lane in LDS, W = minimum waves per SIMD asked of the
-// register allocator (1 = its own choice), QUAD = child-quad layout.
-template <int K, int S, int B, int W, bool QUAD>
+// register allocator (1 = its own choice), LAYOUT = TraceTune::wide.
+template <int K, int S, int B, int W, int LAYOUT>
 hipError_t launch_fast_kw(const TraceParams &p, bool persist, hipStream_t st) {
   if (persist) {
     if (!p.work) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(1), 0, st, p.work);
     if (p.counters) {
-      auto k = k_trace_persist<K, S, B, W, true, QUAD>;
+      auto k = k_trace_persist<K, S, B, W, true, LAYOUT>;
       uint32_t g = std::min(resident_grid((const void *)k, B), grid_slots(p, B) / B);
       hipLaunchKernelGGL(k, dim3(g), dim3(B), 0, st, p);
     } else {
-      auto k = k_trace_persist<K, S, B, W, false, QUAD>;
+      auto k = k_trace_persist<K, S, B, W, false, LAYOUT>;
       uint32_t g = std::min(resident_grid((const void *)k, B), grid_slots(p, B) / B);
       hipLaunchKernelGGL(k, dim3(g), dim3(B), 0, st, p);
     }
     return hipGetLastError();
   }
   dim3 grid(grid_slots(p, B) / B);
-  if (p.counters) hipLaunchKernelGGL((k_trace_fast<K, S, B, W, true, QUAD>), grid, dim3(B), 0, st, p);
-  else hipLaunchKernelGGL((k_trace_fast<K, S, B, W, false, QUAD>), grid, dim3(B), 0, st, p);
+  if (p.counters) hipLaunchKernelGGL((k_trace_fast<K, S, B, W, true, LAYOUT>), grid, dim3(B), 0, st, p);
+  else hipLaunchKernelGGL((k_trace_fast<K, S, B, W, false, LAYOUT>), grid, dim3(B), 0, st, p);
   return hipGetLastError();
 }
 
-template <int S, int B, bool QUAD>
+template <int S, int B, int LAYOUT>
 hipError_t launch_fast_s(const TraceParams &p, const TraceTune &t, hipStream_t st) {
   constexpr int K8 = S < 8 ? S : 8;
   constexpr int K16 = S < 16 ? S : 16;
   const bool persist = t.persist != 0;
   if (t.waves >= 8) {
-    if (t.lds <= 8) return launch_fast_kw<K8, S, B, 8, QUAD>(p, persist, st);
-    return launch_fast_kw<K16, S, B, 8, QUAD>(p, persist, st);
+    if (t.lds <= 8) return launch_fast_kw<K8, S, B, 8, LAYOUT>(p, persist, st);
+    return launch_fast_kw<K16, S, B, 8, LAYOUT>(p, persist, st);
   }
-  if (t.lds <= 8) return launch_fast_kw<K8, S, B, 1, QUAD>(p, persist, st);
-  return launch_fast_kw<K16, S, B, 1, QUAD>(p, persist, st);
+  if (t.lds <= 8) return launch_fast_kw<K8, S, B, 1, LAYOUT>(p, persist, st);
+  return launch_fast_kw<K16, S, B, 1, LAYOUT>(p, persist, st);
 }
 
-template <int B, bool QUAD>
+template <int B, int LAYOUT>
 hipError_t launch_fast_block(const TraceParams &p, uint32_t s, const TraceTune &t, hipStream_t st) {
   switch (s) {
-    case 8: return launch_fast_s<8, B, QUAD>(p, t, st);
-    case 16: return launch_fast_s<16, B, QUAD>(p, t, st);
-    case 24: return launch_fast_s<24, B, QUAD>(p, t, st);
-    case 32: return launch_fast_s<32, B, QUAD>(p, t, st);
-    case 48: return launch_fast_s<48, B, QUAD>(p, t, st);
-    case 64: return launch_fast_s<64, B, QUAD>(p, t, st);
+    case 8: return launch_fast_s<8, B, LAYOUT>(p, t, st);
+    case 16: return launch_fast_s<16, B, LAYOUT>(p, t, st);
+    case 24: return launch_fast_s<24, B, LAYOUT>(p, t, st);
+    case 32: return launch_fast_s<32, B, LAYOUT>(p, t, st);
+    case 48: return launch_fast_s<48, B, LAYOUT>(p, t, st);
+    case 64: return launch_fast_s<64, B, LAYOUT>(p, t, st);
     default: return hipErrorInvalidValue;
   }
 }

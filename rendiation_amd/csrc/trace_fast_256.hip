@@ -5,7 +5,7 @@
 namespace rdrt {
 
 hipError_t launch_fast_256(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st) {
-  return launch_fast_block<256, false>(p, stack, t, st);
+  return launch_fast_block<256, 0>(p, stack, t, st);
 }
 
 }  // namespace rdrt

@@ -5,7 +5,7 @@
 namespace rdrt {
 
 hipError_t launch_fast_64(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st) {
-  return launch_fast_block<64, false>(p, stack, t, st);
+  return launch_fast_block<64, 0>(p, stack, t, st);
 }
 
 }  // namespace rdrt

@@ -49,6 +49,8 @@ struct TraceParams {
   const TlasRoot *tlas_roots;
   const QuadNode *tri_quads;         // child-quad layout of the same trees
   const QuadNode *tlas_quads;
+  const QuadNodeQ *tri_qquads;       // its quantized twin
+  const QuadNodeQ *tlas_qquads;
   const uint32_t *tlas_quad_roots;   // per TLAS
   const uint32_t *binding;
   uint32_t nbinding, ntlas;
@@ -78,7 +80,8 @@ struct TraceTune {
   uint32_t persist = 0;  // 1 = resident-size grid whose waves pull 64-ray chunks from a counter
   uint32_t cull = 0;     // 1 = a first pass finishes the rays that miss their scene's root box and
                          //     queues the others densely for the traversal
-  uint32_t wide = 0;     // 1 = walk the child-quad layout (four boxes per node) instead of child pairs
+  uint32_t wide = 0;     // node layout: 0 = child pairs, 1 = child quads (four boxes per node),
+                         //     2 = quantized child quads (64-byte nodes)
 };
 
 // Scratch of a culled launch: tmp (slots words), blk (2 * ceil(slots /
@@ -113,10 +116,14 @@ uint32_t resident_grid(const void *kernel, int block);
 hipError_t launch_fast_64(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
 hipError_t launch_fast_128(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
 hipError_t launch_fast_256(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
-// The same over the child-quad layout (trace_quad_<B>.hip).
+// The same over the child-quad layout (trace_quad_<B>.hip) and its
+// quantized twin (trace_qquad_<B>.hip).
 hipError_t launch_quad_64(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
 hipError_t launch_quad_128(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
 hipError_t launch_quad_256(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
+hipError_t launch_qquad_64(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
+hipError_t launch_qquad_128(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
+hipError_t launch_qquad_256(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st);
 
 }  // namespace rdrt
 

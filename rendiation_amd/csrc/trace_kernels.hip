@@ -205,7 +205,15 @@ hipError_t launch_trace(const TraceParams &p, uint32_t mode, uint32_t push_depth
   }
   const uint32_t s = stack_entries_for(push_depth);
   if (s == 0) return hipErrorInvalidValue;
-  if (t.wide) {
+  if (t.wide == 2) {
+    if (!p.tri_qquads || !p.tlas_qquads || !p.tlas_quad_roots) return hipErrorInvalidValue;
+    switch (t.block) {
+      case 64: return launch_qquad_64(p, s, t, stream);
+      case 128: return launch_qquad_128(p, s, t, stream);
+      default: return launch_qquad_256(p, s, t, stream);
+    }
+  }
+  if (t.wide == 1) {
     if (!p.tri_quads || !p.tlas_quads || !p.tlas_quad_roots) return hipErrorInvalidValue;
     switch (t.block) {
       case 64: return launch_quad_64(p, s, t, stream);

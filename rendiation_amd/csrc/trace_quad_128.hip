@@ -6,7 +6,7 @@
 namespace rdrt {
 
 hipError_t launch_quad_128(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st) {
-  return launch_fast_block<128, true>(p, stack, t, st);
+  return launch_fast_block<128, 1>(p, stack, t, st);
 }
 
 }  // namespace rdrt

@@ -6,7 +6,7 @@
 namespace rdrt {
 
 hipError_t launch_quad_256(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st) {
-  return launch_fast_block<256, true>(p, stack, t, st);
+  return launch_fast_block<256, 1>(p, stack, t, st);
 }
 
 }  // namespace rdrt

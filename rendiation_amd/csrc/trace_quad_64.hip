@@ -6,7 +6,7 @@
 namespace rdrt {
 
 hipError_t launch_quad_64(const TraceParams &p, uint32_t stack, const TraceTune &t, hipStream_t st) {
-  return launch_fast_block<64, true>(p, stack, t, st);
+  return launch_fast_block<64, 1>(p, stack, t, st);
 }
 
 }  // namespace rdrt

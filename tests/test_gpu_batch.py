@@ -63,7 +63,7 @@ def _dead_rays(rays, dead):
     return out
 
 
-@pytest.mark.parametrize("wide", [0, 1])
+@pytest.mark.parametrize("wide", [0, 1, 2])
 def test_rounds_match_oracle(batch_case, wide, device):
     sys, orc, _, rays, meta, _ = batch_case
     n = rays.shape[0]

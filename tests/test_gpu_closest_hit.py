@@ -4,7 +4,7 @@
   hit context and the per-ray node / triangle / instance visit counters.
 * RDRT_TRAVERSAL_FAST must equal it except on near-ties (|dt| <= 1e-5 t),
   with bit-identical t, u, v wherever the primitive ids agree, over both
-  node layouts (child pairs and child quads).
+  node layouts (child pairs, child quads, quantized child quads).
 """
 
 import numpy as np
@@ -72,7 +72,7 @@ def test_reference_order_kernel_is_bit_exact(closest_hit_scene, device):
     assert np.array_equal(gc, rc)
 
 
-@pytest.mark.parametrize("wide", [0, 1])
+@pytest.mark.parametrize("wide", [0, 1, 2])
 def test_fast_kernel_matches_except_near_ties(closest_hit_scene, wide, device):
     sys, rays, (rh, rm, _) = closest_hit_scene
     _say(f"closest-hit 100k tris x 1M rays, fast kernel, wide={wide}")
@@ -89,7 +89,8 @@ def test_fast_kernel_matches_except_near_ties(closest_hit_scene, wide, device):
     assert np.array_equal(gm[same], rm[same])
 
 
-@pytest.mark.parametrize("traversal,wide", [(TRAVERSAL_REFERENCE, 0), (TRAVERSAL_FAST, 0), (TRAVERSAL_FAST, 1)])
+@pytest.mark.parametrize("traversal,wide", [(TRAVERSAL_REFERENCE, 0), (TRAVERSAL_FAST, 0), (TRAVERSAL_FAST, 1),
+                                            (TRAVERSAL_FAST, 2)])
 def test_instanced_scene_with_flags(traversal, wide, device):
     blas_list, tlas_list, binding, rays, meta = instanced_scene(seed=3, nrays=60_000)
     opts = BuildOptions()
@@ -122,7 +123,7 @@ def test_uniform_ray_flags(flags, device):
     sys, orc = _build_pair([[(pos, None, 1)]], [[{"transform": np.eye(4), "blas": 0}]], [0], BuildOptions())
     _say(f"uniform flags {flags:#x}")
     rh, _, rc = orc.trace(rays, flags=flags, with_counters=True)
-    for trav, wide in ((TRAVERSAL_REFERENCE, 0), (TRAVERSAL_FAST, 0), (TRAVERSAL_FAST, 1)):
+    for trav, wide in ((TRAVERSAL_REFERENCE, 0), (TRAVERSAL_FAST, 0), (TRAVERSAL_FAST, 1), (TRAVERSAL_FAST, 2)):
         sys.set_tune(wide=wide)
         gh, _, gc = _gpu_trace(sys, rays, device, traversal=trav, flags=flags)
         if trav == TRAVERSAL_REFERENCE:
@@ -158,14 +159,14 @@ def test_launch_shapes_agree_and_autotune(closest_hit_scene, device):
     sys, rays_np, _ = closest_hit_scene
     _say("launch shapes + autotune, 100k tris x 1M rays")
     base = {}
-    for wide in (0, 1):  # each layout has its own visit order
+    for wide in (0, 1, 2):  # each layout has its own visit order
         sys.set_tune(wide=wide)
         base[wide], _, _ = _gpu_trace(sys, rays_np, device)
     rays = torch.from_numpy(np.ascontiguousarray(rays_np)).to(device)
     hits = torch.empty((rays_np.shape[0], 4), dtype=torch.int32, device=device)
     for shape in [(64, 8, 0, 0, 0, 0), (128, 16, 8, 1, 0, 0), (256, 8, 8, 0, 1, 0), (256, 16, 0, 1, 1, 0),
                   (64, 16, 8, 0, 1, 0), (64, 8, 0, 0, 0, 1), (128, 16, 8, 1, 0, 1), (256, 8, 8, 0, 1, 1),
-                  (256, 16, 0, 1, 1, 1)]:
+                  (256, 16, 0, 1, 1, 1), (64, 16, 0, 0, 1, 2), (128, 8, 8, 1, 0, 2), (256, 16, 0, 1, 1, 2)]:
         sys.set_tune(*shape)
         sys.trace(rays, hits)
         sys.sync()

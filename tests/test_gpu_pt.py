@@ -134,7 +134,7 @@ def test_floor_on_gpu(device):
     assert st["shadow_rays"][:1] == [32 * 32]
 
 
-@pytest.mark.parametrize("spp,wide", [(1, 0), (4, 0), (1, 1)])
+@pytest.mark.parametrize("spp,wide", [(1, 0), (4, 0), (1, 1), (1, 2)])
 def test_parametric_scene_against_fp64_oracle(spp, wide, device):
     scene = synth.pt_scene(seed=1, n_meshes=64, tess=48)
     cam = Camera()
