@@ -190,6 +190,18 @@ uint32_t rdrt_bind_tlas_max_len(const rdrt_ctx *ctx);
 int32_t rdrt_build(rdrt_ctx *ctx);
 /* Build if needed and upload to the device. */
 int32_t rdrt_commit(rdrt_ctx *ctx);
+/* Refit after a vertex update (no counterpart in the reference, which
+ * rebuilds): positions_dev holds every vertex of the committed scene in the
+ * scene vertex order (rdrt_geometry_vertex_range), 3 f32 each; normals_dev,
+ * if not NULL, the vertex normals likewise.  Every BLAS keeps its trees and
+ * gets its boxes and triangle records recomputed on the device (the
+ * reference forest, the child pairs, the child quads and their quantized
+ * twins); the TLAS layer (instances and TLAS trees) is rebuilt on the host
+ * from the new BLAS boxes.  Synchronizes.  Captured frames and batched calls
+ * are re-captured and a recorded frame can no longer be differentiated.  The
+ * host-side copies (rdrt_copy_built) keep the committed geometry, and a later
+ * rebuild starts from it. */
+int32_t rdrt_refit(rdrt_ctx *ctx, const float *positions_dev, const float *normals_dev);
 int32_t rdrt_get_scene_info(const rdrt_ctx *ctx, rdrt_scene_info *out);
 /* Copies of the built reference-layout arrays (host memory, sized from
  * rdrt_scene_info), for inspection and parity checks.  which:

@@ -352,6 +352,17 @@ class AccelerationStructureSystem:
         L.check(self._L.rdrt_batch_get_stats(self._h, C.byref(o)))
         return BatchStats(rounds=o.rounds, rays=list(o.rays)[:o.rounds], launches=o.launches, ms=float(o.ms))
 
+    def refit(self, positions, normals=None) -> None:
+        """Refit every BVH to new vertex positions (device tensor, every vertex
+        of the committed scene in the scene vertex order, 3 f32 each) and
+        optionally new vertex normals: same BLAS trees with recomputed boxes
+        and triangle records, TLAS rebuilt.  Synchronizes."""
+        info = self.scene_info()
+        _check_dev(positions, info.verts * 3, "positions")
+        if normals is not None:
+            _check_dev(normals, info.verts * 3, "normals")
+        L.check(self._L.rdrt_refit(self._h, _ptr(positions), _ptr(normals)))
+
     def geometry_vertex_range(self, blas: BlasHandle, geometry: int = 0):
         """(first, count) of a geometry's vertices in the scene vertex order."""
         first, count = L.u32(), L.u32()

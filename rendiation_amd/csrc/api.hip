@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "context.h"
+#include "refit.h"
 
 using namespace rdrt;
 
@@ -61,6 +62,14 @@ void free_device_scene(DeviceScene &d) {
   d.bytes = 0;
 }
 
+void release_refit(rdrt_ctx *c) {
+  release(c->refit_forest);
+  release(c->refit_pairs);
+  release(c->refit_quads);
+  c->refit_boxes.free();
+  c->refit_ready = false;
+}
+
 int32_t need_device(rdrt_ctx *c) {
   if (!c->has_device) return fail(RDRT_ERR_NO_DEVICE, "context was created without a device (device = -1)");
   RDRT_HIP(hipSetDevice(c->cfg.device));
@@ -97,6 +106,36 @@ TraceParams scene_trace_params(const rdrt_ctx *c) {
 uint32_t push_depth(const rdrt_ctx *c, const TraceTune &t) {
   return t.wide ? c->built.blas_stack4 + c->built.tlas_stack4 : c->built.blas_stack + c->built.tlas_stack;
 }
+
+namespace {
+
+// Replaces a device array with a new host array, keeping the byte count.
+template <class T>
+hipError_t reupload(DevArray<T> &a, const std::vector<T> &h, uint64_t &bytes) {
+  const uint64_t old = (uint64_t)(a.count ? a.count : 1) * sizeof(T);
+  if (a.ptr) bytes -= std::min(bytes, old);
+  release(a);
+  return upload(a, h, bytes);
+}
+
+// Captured frames and batched calls hold the scene's device pointers.
+void drop_captures(rdrt_ctx *c) {
+  PtState &s = c->pt;
+  if (s.exec) (void)hipGraphExecDestroy(s.exec);
+  if (s.graph) (void)hipGraphDestroy(s.graph);
+  s.exec = nullptr;
+  s.graph = nullptr;
+  s.have_cached = false;
+  s.last_recorded = false;
+  BatchState &bs = c->batch;
+  if (bs.exec) (void)hipGraphExecDestroy(bs.exec);
+  if (bs.graph) (void)hipGraphDestroy(bs.graph);
+  bs.exec = nullptr;
+  bs.graph = nullptr;
+  bs.cached.clear();
+}
+
+}  // namespace
 
 }  // namespace rdrt
 
@@ -168,6 +207,7 @@ int32_t rdrt_ctx_destroy(rdrt_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     if (c->work_stream) (void)hipStreamSynchronize(c->work_stream);
     free_device_scene(c->dev);
+    release_refit(c);
     c->pt.free_all();
     c->batch.free_all();
     c->compact_scratch.free();
@@ -299,6 +339,7 @@ int32_t rdrt_commit(rdrt_ctx *c) {
   if (c->uploaded) return RDRT_OK;
   RDRT_HIP(hipStreamSynchronize(c->stream));
   free_device_scene(c->dev);
+  release_refit(c);
   const BuiltScene &b = c->built;
   DeviceScene &d = c->dev;
   RDRT_HIP(upload(d.tri_pairs, b.tri_pairs, d.bytes));
@@ -322,6 +363,79 @@ int32_t rdrt_commit(rdrt_ctx *c) {
   RDRT_HIP(upload(d.normals, b.normals, d.bytes));
   RDRT_HIP(upload(d.tri_material, b.tri_material, d.bytes));
   c->uploaded = true;
+  return RDRT_OK;
+}
+
+int32_t rdrt_refit(rdrt_ctx *c, const float *positions, const float *normals) {
+  if (!c || !positions) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  if (!c->uploaded) return fail(RDRT_ERR_NOT_BUILT, "scene is not committed (call rdrt_commit)");
+  BuiltScene &b = c->built;
+  DeviceScene &d = c->dev;
+  if (normals && d.normals.count == 0) return fail(RDRT_ERR_STATE, "the scene has no vertex normals to replace");
+  hipStream_t st = c->stream;
+  if (!c->refit_ready) {
+    const RefitLevels L = refit_levels(b);
+    uint64_t scratch_bytes = 0;
+    RDRT_HIP(upload(c->refit_forest, L.forest, scratch_bytes));
+    RDRT_HIP(upload(c->refit_pairs, L.pairs, scratch_bytes));
+    RDRT_HIP(upload(c->refit_quads, L.quads, scratch_bytes));
+    c->refit_forest_off = L.forest_off;
+    c->refit_pairs_off = L.pairs_off;
+    c->refit_quads_off = L.quads_off;
+    c->refit_ready = true;
+  }
+  const uint32_t ntris = (uint32_t)d.tri_ids.count;
+  RDRT_HIP(c->refit_boxes.reserve((size_t)std::max<uint32_t>(ntris, 1u) * 6 * sizeof(float)));
+  if (d.vertices.count)
+    RDRT_HIP(hipMemcpyAsync(d.vertices.ptr, positions, d.vertices.count * sizeof(float), hipMemcpyDeviceToDevice, st));
+  if (normals)
+    RDRT_HIP(hipMemcpyAsync(d.normals.ptr, normals, d.normals.count * sizeof(float), hipMemcpyDeviceToDevice, st));
+  RefitArgs a{};
+  a.vertices = d.vertices.ptr;
+  a.indices = d.indices.ptr;
+  a.tri_ids = d.tri_ids.ptr;
+  a.redirect = d.redirect.ptr;
+  a.nslots = ntris;
+  a.tri_box = (float *)c->refit_boxes.ptr;
+  a.tris = d.tris.ptr;
+  a.forest = d.tri_forest.ptr;
+  a.pairs = d.tri_pairs.ptr;
+  a.quads = d.tri_quads.ptr;
+  a.qquads = d.tri_qquads.ptr;
+  a.geos = d.geos.ptr;
+  a.ngeos = (uint32_t)d.geos.count;
+  a.forest_nodes = c->refit_forest.ptr;
+  a.pair_nodes = c->refit_pairs.ptr;
+  a.quad_nodes = c->refit_quads.ptr;
+  auto levels = [](const std::vector<uint32_t> &off) { return off.empty() ? 0u : (uint32_t)off.size() - 1; };
+  RDRT_HIP(enqueue_refit(a, c->refit_forest_off.data(), levels(c->refit_forest_off), c->refit_pairs_off.data(),
+                         levels(c->refit_pairs_off), c->refit_quads_off.data(), levels(c->refit_quads_off), st));
+  // The TLAS layer is small: rebuilt on the host from the refitted BLAS boxes.
+  if (!b.geos.empty())
+    RDRT_HIP(hipMemcpyAsync(b.geos.data(), d.geos.ptr, b.geos.size() * sizeof(GeoRec), hipMemcpyDeviceToHost, st));
+  RDRT_HIP(hipStreamSynchronize(st));
+  for (size_t k = 0; k < b.blas_box.size(); k++) {
+    Box bx = box_empty();
+    for (uint32_t g = b.blas_geo_begin[k]; g < b.blas_geo_end[k]; g++)
+      for (int j = 0; j < 3; j++) {
+        bx.mn[j] = std::fmin(bx.mn[j], b.geos[g].mn[j]);
+        bx.mx[j] = std::fmax(bx.mx[j], b.geos[g].mx[j]);
+      }
+    b.blas_box[k] = bx;
+  }
+  std::string err;
+  rc = build_tlas_layer(c->src, b, err);
+  if (rc != RDRT_OK) return fail(rc, err);
+  drop_captures(c);
+  RDRT_HIP(reupload(d.insts, b.insts, d.bytes));
+  RDRT_HIP(reupload(d.tlas_forest, b.tlas_forest, d.bytes));
+  RDRT_HIP(reupload(d.tlas_pairs, b.tlas_pairs, d.bytes));
+  RDRT_HIP(reupload(d.tlas_quads, b.tlas_quads, d.bytes));
+  RDRT_HIP(reupload(d.tlas_qquads, b.tlas_qquads, d.bytes));
+  RDRT_HIP(reupload(d.tlas_roots, b.tlas_roots, d.bytes));
+  RDRT_HIP(reupload(d.tlas_quad_roots, b.tlas_quad_roots, d.bytes));
   return RDRT_OK;
 }
 

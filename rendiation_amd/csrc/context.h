@@ -145,6 +145,11 @@ struct rdrt_ctx {
   rdrt::Scratch trace_work;  // work counter of persistent rdrt_trace launches
   rdrt::Scratch trace_cull;  // cull pass scratch of rdrt_trace
   float tune_ms = 0.0f;      // per-launch time of the autotuned choice
+  // refit state (built on the first rdrt_refit after a commit)
+  bool refit_ready = false;
+  rdrt::DevArray<uint32_t> refit_forest, refit_pairs, refit_quads;
+  std::vector<uint32_t> refit_forest_off, refit_pairs_off, refit_quads_off;
+  rdrt::Scratch refit_boxes;
 };
 
 namespace rdrt {

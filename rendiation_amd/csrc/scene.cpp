@@ -545,6 +545,164 @@ bool validate_refs(const BuiltScene &s, std::string &err) {
 
 }  // namespace
 
+// The TLAS layer (naive/mod.rs:262-320, 435-454) from the BLAS boxes in
+// `out`: instance records, the reference TLAS forest, and the fast kernels'
+// single-instance TLAS in pair, quad and quantized-quad layouts.  Clears and
+// refills those arrays, so a refit of the BLAS boxes can rebuild it.
+int build_tlas_layer(const SceneSrc &src, BuiltScene &out, std::string &err) {
+  out.insts.clear();
+  out.tlas_forest.clear();
+  out.tlas_pairs.clear();
+  out.tlas_quads.clear();
+  out.tlas_qquads.clear();
+  out.tlas_roots.clear();
+  out.tlas_quad_roots.clear();
+  out.max_tlas_depth = 0;
+  out.tlas_stack = 0;
+  out.tlas_stack4 = 0;
+  for (uint32_t ti = 0; ti < src.tlas.size(); ti++) {
+    TlasRoot root;
+    std::memset(&root, 0, sizeof(root));
+    if (!src.tlas_live[ti]) {
+      root.ref_root = kInvalid;
+      root.root_ref = kInvalid;
+      out.tlas_roots.push_back(root);
+      out.tlas_quad_roots.push_back(kInvalid);
+      continue;
+    }
+    const auto &inst = src.tlas[ti];
+    std::vector<Box> boxes;
+    for (const rdrt_instance &in : inst) {
+      if (in.blas >= src.blas.size() || !out.blas_has_box[in.blas]) {
+        err = "instance references a deleted, empty or unknown BLAS";
+        return RDRT_ERR_INVALID_ARG;
+      }
+      boxes.push_back(transform_box(out.blas_box[in.blas], load_m4(in.transform)));
+    }
+    Tree tree = build_tree(boxes, src.tlas_opt, 1);
+    out.max_tlas_depth = std::max(out.max_tlas_depth, tree.depth);
+    uint32_t inst_base = (uint32_t)out.insts.size();
+    for (uint32_t k : tree.order) {
+      const rdrt_instance &in = inst[k];
+      M4 m = load_m4(in.transform);
+      uint32_t flags = in.flags;
+      if (det3(m) < 0.0f) flags ^= RDRT_INSTANCE_TRIANGLE_FLIP_FACING;
+      InstRec r;
+      std::memset(&r, 0, sizeof(r));
+      M4 inv = inverse_or_identity(m);
+      std::memcpy(r.inv, &inv, 64);
+      put_box(r.mn, r.mx, boxes[k]);
+      r.mask = in.mask;
+      r.flags = flags;
+      r.geo_begin = out.blas_geo_begin[in.blas];
+      r.geo_end = out.blas_geo_end[in.blas];
+      r.custom_index = in.instance_custom_index;
+      r.order = k;
+      r.sbt_offset = in.sbt_offset;
+      r.blas = in.blas;
+      out.insts.push_back(r);
+    }
+    root.ref_root = (uint32_t)out.tlas_forest.size();
+    append_forest(tree, inst_base, out.tlas_forest);
+    put_box(root.mn, root.mx, tree.nodes[0].box);
+    // The fast kernel's TLAS: the same binned SAH over the same boxes but with
+    // one instance per leaf, so a ray tests only instances whose boxes it
+    // enters (the reference's 10-instance leaves are linear scans).  Any tree
+    // gives the same closest hit; only its visit order differs.
+    if ((uint64_t)inst_base + inst.size() > (uint64_t)kLeafMaxStart + 1) {
+      err = "too many instances for the child-pair layout";
+      return RDRT_ERR_CAPACITY;
+    }
+    std::vector<Box> fboxes(inst.size());
+    std::vector<uint32_t> fmap(inst.size());
+    for (uint32_t j = 0; j < tree.order.size(); j++) {
+      fboxes[j] = boxes[tree.order[j]];
+      fmap[j] = inst_base + j;
+    }
+    TreeOptions fine = src.tlas_opt;
+    fine.bin_size = 1;
+    Tree ftree = build_tree(fboxes, fine, 1);
+    SinglePairConv pc{ftree, fmap, out.tlas_pairs};
+    uint32_t d = 0;
+    root.root_ref = pc.ref_of(0, d);
+    out.tlas_stack = std::max(out.tlas_stack, d);
+    out.tlas_roots.push_back(root);
+    QuadConv qc{out.tlas_pairs, out.tlas_quads};
+    out.tlas_quad_roots.push_back(qc.ref_of(root.root_ref, d));
+    out.tlas_stack4 = std::max(out.tlas_stack4, d);
+  }
+  out.tlas_qquads.resize(out.tlas_quads.size());
+  for (size_t k = 0; k < out.tlas_quads.size(); k++) out.tlas_qquads[k] = quantize_quad(out.tlas_quads[k]);
+  return RDRT_OK;
+}
+
+namespace {
+
+// Groups (node, depth) pairs by depth, deepest first.
+void by_depth(const std::vector<std::pair<uint32_t, uint32_t>> &nd, std::vector<uint32_t> &order,
+              std::vector<uint32_t> &off) {
+  uint32_t maxd = 0;
+  for (const auto &p : nd) maxd = std::max(maxd, p.second);
+  std::vector<uint32_t> count(nd.empty() ? 0 : maxd + 1, 0);
+  for (const auto &p : nd) count[p.second]++;
+  off.assign(count.size() + 1, 0);
+  for (size_t l = 0; l < count.size(); l++) off[l + 1] = off[l] + count[count.size() - 1 - l];
+  std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+  order.resize(nd.size());
+  for (const auto &p : nd) order[fill[count.size() - 1 - p.second]++] = p.first;
+}
+
+}  // namespace
+
+RefitLevels refit_levels(const BuiltScene &b) {
+  RefitLevels r;
+  std::vector<std::pair<uint32_t, uint32_t>> nd, stack;
+  // reference forest: every node (leaf <=> hit == miss; right child = miss of the left)
+  for (const GeoRec &g : b.geos) {
+    stack.assign(1, {g.ref_root, 0u});
+    while (!stack.empty()) {
+      auto [i, d] = stack.back();
+      stack.pop_back();
+      nd.push_back({i, d});
+      const RefNode &n = b.tri_forest[i];
+      if (n.hit_next != n.miss_next) {
+        stack.push_back({n.hit_next, d + 1});
+        stack.push_back({b.tri_forest[n.hit_next].miss_next, d + 1});
+      }
+    }
+  }
+  by_depth(nd, r.forest, r.forest_off);
+  nd.clear();
+  for (const GeoRec &g : b.geos) {
+    if (g.root_ref & kLeafBit) continue;
+    stack.assign(1, {g.root_ref, 0u});
+    while (!stack.empty()) {
+      auto [i, d] = stack.back();
+      stack.pop_back();
+      nd.push_back({i, d});
+      const PairNode &p = b.tri_pairs[i];
+      if (!(p.lref & kLeafBit)) stack.push_back({p.lref, d + 1});
+      if (!(p.rref & kLeafBit)) stack.push_back({p.rref, d + 1});
+    }
+  }
+  by_depth(nd, r.pairs, r.pairs_off);
+  nd.clear();
+  for (const GeoRec &g : b.geos) {
+    if (g.quad_root & kLeafBit) continue;
+    stack.assign(1, {g.quad_root, 0u});
+    while (!stack.empty()) {
+      auto [i, d] = stack.back();
+      stack.pop_back();
+      nd.push_back({i, d});
+      const QuadNode &q = b.tri_quads[i];
+      for (int k = 0; k < 4; k++)
+        if (q.ref[k] != kInvalid && !(q.ref[k] & kLeafBit)) stack.push_back({q.ref[k], d + 1});
+    }
+  }
+  by_depth(nd, r.quads, r.quads_off);
+  return r;
+}
+
 int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
   auto t0 = std::chrono::steady_clock::now();
   out = BuiltScene();
@@ -671,86 +829,20 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
     }
   });
 
-  // ---- TLAS (naive/mod.rs:262-320, 435-454) ----
-  for (uint32_t ti = 0; ti < src.tlas.size(); ti++) {
-    TlasRoot root;
-    std::memset(&root, 0, sizeof(root));
-    if (!src.tlas_live[ti]) {
-      root.ref_root = kInvalid;
-      root.root_ref = kInvalid;
-      out.tlas_roots.push_back(root);
-      out.tlas_quad_roots.push_back(kInvalid);
-      continue;
-    }
-    const auto &inst = src.tlas[ti];
-    std::vector<Box> boxes;
-    for (const rdrt_instance &in : inst) {
-      if (in.blas >= src.blas.size() || !blas_has_box[in.blas]) {
-        err = "instance references a deleted, empty or unknown BLAS";
-        return RDRT_ERR_INVALID_ARG;
-      }
-      boxes.push_back(transform_box(blas_box[in.blas], load_m4(in.transform)));
-    }
-    Tree tree = build_tree(boxes, src.tlas_opt, 1);
-    out.max_tlas_depth = std::max(out.max_tlas_depth, tree.depth);
-    uint32_t inst_base = (uint32_t)out.insts.size();
-    for (uint32_t k : tree.order) {
-      const rdrt_instance &in = inst[k];
-      M4 m = load_m4(in.transform);
-      uint32_t flags = in.flags;
-      if (det3(m) < 0.0f) flags ^= RDRT_INSTANCE_TRIANGLE_FLIP_FACING;
-      InstRec r;
-      std::memset(&r, 0, sizeof(r));
-      M4 inv = inverse_or_identity(m);
-      std::memcpy(r.inv, &inv, 64);
-      put_box(r.mn, r.mx, boxes[k]);
-      r.mask = in.mask;
-      r.flags = flags;
-      r.geo_begin = blas_geo_begin[in.blas];
-      r.geo_end = blas_geo_end[in.blas];
-      r.custom_index = in.instance_custom_index;
-      r.order = k;
-      r.sbt_offset = in.sbt_offset;
-      r.blas = in.blas;
-      out.insts.push_back(r);
-    }
-    root.ref_root = (uint32_t)out.tlas_forest.size();
-    append_forest(tree, inst_base, out.tlas_forest);
-    put_box(root.mn, root.mx, tree.nodes[0].box);
-    // The fast kernel's TLAS: the same binned SAH over the same boxes but with
-    // one instance per leaf, so a ray tests only instances whose boxes it
-    // enters (the reference's 10-instance leaves are linear scans).  Any tree
-    // gives the same closest hit; only its visit order differs.
-    if ((uint64_t)inst_base + inst.size() > (uint64_t)kLeafMaxStart + 1) {
-      err = "too many instances for the child-pair layout";
-      return RDRT_ERR_CAPACITY;
-    }
-    std::vector<Box> fboxes(inst.size());
-    std::vector<uint32_t> fmap(inst.size());
-    for (uint32_t j = 0; j < tree.order.size(); j++) {
-      fboxes[j] = boxes[tree.order[j]];
-      fmap[j] = inst_base + j;
-    }
-    TreeOptions fine = src.tlas_opt;
-    fine.bin_size = 1;
-    Tree ftree = build_tree(fboxes, fine, 1);
-    SinglePairConv pc{ftree, fmap, out.tlas_pairs};
-    uint32_t d = 0;
-    root.root_ref = pc.ref_of(0, d);
-    out.tlas_stack = std::max(out.tlas_stack, d);
-    out.tlas_roots.push_back(root);
-    QuadConv qc{out.tlas_pairs, out.tlas_quads};
-    out.tlas_quad_roots.push_back(qc.ref_of(root.root_ref, d));
-    out.tlas_stack4 = std::max(out.tlas_stack4, d);
-  }
   // quantized twins of the child-quad nodes
   out.tri_qquads.resize(out.tri_quads.size());
   parallel_for((out.tri_quads.size() + 65535) / 65536, hw, [&](size_t c) {
     const size_t e = std::min(out.tri_quads.size(), (c + 1) * 65536);
     for (size_t k = c * 65536; k < e; k++) out.tri_qquads[k] = quantize_quad(out.tri_quads[k]);
   });
-  out.tlas_qquads.resize(out.tlas_quads.size());
-  for (size_t k = 0; k < out.tlas_quads.size(); k++) out.tlas_qquads[k] = quantize_quad(out.tlas_quads[k]);
+  out.blas_box = blas_box;
+  out.blas_has_box = blas_has_box;
+  out.blas_geo_begin = blas_geo_begin;
+  out.blas_geo_end = blas_geo_end;
+  {
+    const int rc = build_tlas_layer(src, out, err);
+    if (rc != RDRT_OK) return rc;
+  }
   out.binding = src.binding;
   if (!validate_refs(out, err)) return RDRT_ERR_STATE;
   out.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -63,12 +63,29 @@ struct BuiltScene {
   std::vector<QuadNodeQ> tri_qquads, tlas_qquads;  // quantized twins (same indices and refs)
   std::vector<uint32_t> tlas_quad_roots;      // per TLAS (kInvalid when deleted)
   uint32_t blas_stack4 = 0, tlas_stack4 = 0;  // deepest push count of the child-quad trees
+  // per BLAS: object-space box (union of its geometries), whether it has
+  // one, and its geometry range (inputs of the TLAS layer)
+  std::vector<Box> blas_box;
+  std::vector<char> blas_has_box;
+  std::vector<uint32_t> blas_geo_begin, blas_geo_end;
   double build_ms = 0.0;
 };
 
 // Builds every BVH and both layouts.  Returns RDRT_OK or an RDRT_ERR_* code
 // with a message in `err`.
 int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err);
+// BLAS node indices of every layout grouped by depth, deepest level first:
+// the order in which a refit recomputes boxes bottom-up (a level only reads
+// the level below).  *_off[l]..*_off[l + 1] is level l of that order.
+struct RefitLevels {
+  std::vector<uint32_t> forest, pairs, quads;
+  std::vector<uint32_t> forest_off, pairs_off, quads_off;
+};
+RefitLevels refit_levels(const BuiltScene &b);
+
+// Rebuilds the TLAS layer of `out` (instances, TLAS trees in every layout)
+// from out.blas_box and the BLAS geometry ranges.
+int build_tlas_layer(const SceneSrc &src, BuiltScene &out, std::string &err);
 
 }  // namespace rdrt
 
