@@ -22,7 +22,8 @@ diff: differentiable path tracing -- 64 spheres/tori at 90x90 (1.04M
   the gradient error against the FP64 oracle beside it.
 diffgeo: gradient w.r.t. 1M vertex positions and normals of a 1000x1000
   height field, 16 views at 512x512 in one frame, 2 spp, 3 bounces, paths
-  recomputed by a replay of the frame; value = ms of forward + adjoint.
+  recomputed by a replay of the frame; value = ms of forward + adjoint.  Also
+  reports the BVH refit after a descent step on the positions (ms_refit).
 
 roofline: HBM-bound.  Algorithmic bytes per traced ray = 32 (ray) + 16 (hit)
 + 48 x node visits + 52 x triangle tests + 176 x instance tests, with visit
@@ -581,6 +582,22 @@ def differentiable_geometry(args):
     if not args.no_cpu_baseline:
         extra["cpu_baseline"], extra["gradient_error"] = cpu_baseline_diffgeo(scene, cams[:2], cfg, target, system,
                                                                              tracer)
+    # a descent step on the positions, then the BVH refit to them (device
+    # BLAS refit + host TLAS rebuild; synchronizes), alternating with the
+    # committed positions
+    with torch.cuda.stream(stream):
+        pos0 = torch.from_numpy(hp).to(dev).reshape(-1)
+        step = g_pos.reshape(-1) * (1e-3 / float(g_pos.abs().max().item() + 1e-30))
+        pos1 = (pos0 - step).contiguous()
+    system.sync()
+    refit_ms = []
+    for k in range(6):
+        w0 = time.perf_counter()
+        system.refit(pos1 if k % 2 == 0 else pos0)
+        refit_ms.append((time.perf_counter() - w0) * 1e3)
+    config["ms_refit"] = round(float(np.median(refit_ms[1:])), 3)
+    config["ms_refit_first"] = round(refit_ms[0], 3)
+    config["refit_speedup_vs_build"] = round(build_upload_ms / float(np.median(refit_ms[1:])), 1)
     system.close()
     rec = _record(ms_fb, "ms", args, ms_fb, config, None, extra)
     rec["higher_is_better"] = False
