@@ -186,8 +186,18 @@ int32_t rdrt_tlas_create(rdrt_ctx *ctx, const rdrt_instance *inst, uint32_t n, u
 int32_t rdrt_tlas_destroy(rdrt_ctx *ctx, uint32_t handle);
 int32_t rdrt_bind_tlas(rdrt_ctx *ctx, const uint32_t *handles, uint32_t n);
 uint32_t rdrt_bind_tlas_max_len(const rdrt_ctx *ctx);
-/* Host-side build of every BVH (no device work). */
+/* Host-side build of every BVH (no device work), unless device builds are
+ * on (rdrt_set_device_build). */
 int32_t rdrt_build(rdrt_ctx *ctx);
+/* enable = 1: the next rdrt_build / rdrt_commit builds every BLAS tree (the
+ * reference's 4-bin SAH tree and the fast kernels' tree) on the device,
+ * level by level, with the host builders' split decisions (same nodes; only
+ * the order inside leaves under a reference median split may differ), then
+ * the layouts on the host as before.  Not supported (RDRT_ERR_CAPACITY):
+ * more than 4 reference SAH bins, a reference median split of more than 4096
+ * primitives.  enable = 0 (default): host builds.  (No counterpart in the
+ * reference, whose build is single-threaded on the host.) */
+int32_t rdrt_set_device_build(rdrt_ctx *ctx, uint32_t enable);
 /* Build if needed and upload to the device. */
 int32_t rdrt_commit(rdrt_ctx *ctx);
 /* Refit after a vertex update (no counterpart in the reference, which

@@ -141,6 +141,7 @@ SIGNATURES = {
     "rdrt_pt_autotune": (i32, [C.c_void_p, C.POINTER(RdrtPtParams), C.c_void_p, u32, C.POINTER(RdrtTune)]),
     "rdrt_batch_autotune": (i32, [C.c_void_p, C.POINTER(RdrtBatchParams), u32, C.POINTER(RdrtTune)]),
     "rdrt_refit": (i32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "rdrt_set_device_build": (i32, [C.c_void_p, u32]),
     "rdrt_autotune": (i32, [C.c_void_p, C.POINTER(RdrtTraceArgs), u32, C.POINTER(RdrtTune)]),
     "rdrt_set_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),
     "rdrt_get_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),

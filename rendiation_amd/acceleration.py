@@ -352,6 +352,10 @@ class AccelerationStructureSystem:
         L.check(self._L.rdrt_batch_get_stats(self._h, C.byref(o)))
         return BatchStats(rounds=o.rounds, rays=list(o.rays)[:o.rounds], launches=o.launches, ms=float(o.ms))
 
+    def set_device_build(self, enable: bool = True) -> None:
+        """Build the BLAS trees on the device at the next build / commit."""
+        L.check(self._L.rdrt_set_device_build(self._h, 1 if enable else 0))
+
     def refit(self, positions, normals=None) -> None:
         """Refit every BVH to new vertex positions (device tensor, every vertex
         of the committed scene in the scene vertex order, 3 f32 each) and

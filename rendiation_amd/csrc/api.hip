@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "context.h"
+#include "device_build.h"
 #include "refit.h"
 
 using namespace rdrt;
@@ -323,10 +324,27 @@ int32_t rdrt_build(rdrt_ctx *c) {
   if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
   if (c->built_valid) return RDRT_OK;
   std::string err;
-  int32_t rc = build_scene(c->src, c->built, err);
+  int32_t rc;
+  if (c->device_build) {
+    rc = need_device(c);
+    if (rc) return rc;
+    std::unique_ptr<DeviceTreeBuilder> trees = make_device_tree_builder(c->stream);
+    rc = build_scene(c->src, c->built, err, trees.get());
+  } else {
+    rc = build_scene(c->src, c->built, err);
+  }
   if (rc) return fail(rc, err);
   c->built_valid = true;
   c->uploaded = false;
+  return RDRT_OK;
+}
+
+int32_t rdrt_set_device_build(rdrt_ctx *c, uint32_t enable) {
+  if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
+  if (enable > 1) return fail(RDRT_ERR_INVALID_ARG, "enable is 0 or 1");
+  if (enable && !c->has_device) return fail(RDRT_ERR_NO_DEVICE, "a device build needs a device context");
+  if (c->device_build != (enable != 0)) invalidate(c);
+  c->device_build = enable != 0;
   return RDRT_OK;
 }
 

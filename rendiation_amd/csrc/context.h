@@ -145,6 +145,7 @@ struct rdrt_ctx {
   rdrt::Scratch trace_work;  // work counter of persistent rdrt_trace launches
   rdrt::Scratch trace_cull;  // cull pass scratch of rdrt_trace
   float tune_ms = 0.0f;      // per-launch time of the autotuned choice
+  bool device_build = false;  // build the BLAS trees on the device (rdrt_set_device_build)
   // refit state (built on the first rdrt_refit after a commit)
   bool refit_ready = false;
   rdrt::DevArray<uint32_t> refit_forest, refit_pairs, refit_quads;

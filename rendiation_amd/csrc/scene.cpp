@@ -703,7 +703,7 @@ RefitLevels refit_levels(const BuiltScene &b) {
   return r;
 }
 
-int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
+int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTreeBuilder *trees) {
   auto t0 = std::chrono::steady_clock::now();
   out = BuiltScene();
   unsigned hw = std::max(1u, std::thread::hardware_concurrency());
@@ -714,11 +714,13 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
     std::vector<uint32_t> tri_idx;  // 3 per triangle, local vertex ids
     Tree tree;  // the reference's tree (forest, redirect, k_trace_ref)
     Tree fast;  // the fast traversal's tree (child pairs, TriPre order)
+    std::vector<Box> boxes;  // triangle boxes, kept for a device build
+    bool bad = false;        // an index beyond the geometry's vertices
   };
   std::vector<Job> jobs;
   for (uint32_t b = 0; b < src.blas.size(); b++) {
     if (!src.blas_live[b]) continue;
-    for (uint32_t g = 0; g < src.blas[b].size(); g++) jobs.push_back(Job{b, g, {}, {}, {}});
+    for (uint32_t g = 0; g < src.blas[b].size(); g++) jobs.push_back(Job{b, g, {}, {}, {}, {}, false});
   }
   unsigned per_job = jobs.size() >= hw ? 1u : std::max(1u, hw / (unsigned)std::max<size_t>(1, jobs.size()));
   parallel_for(jobs.size(), std::min<unsigned>(hw, (unsigned)jobs.size()), [&](size_t j) {
@@ -730,6 +732,11 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
       job.tri_idx.resize(nv - nv % 3);
       for (size_t v = 0; v < job.tri_idx.size(); v++) job.tri_idx[v] = (uint32_t)v;
     }
+    for (uint32_t v : job.tri_idx)
+      if (v >= nv) {
+        job.bad = true;  // reported below, before any box is read
+        return;
+      }
     size_t nt = job.tri_idx.size() / 3;
     std::vector<Box> boxes(nt);
     for (size_t t = 0; t < nt; t++) {
@@ -737,9 +744,24 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err) {
       for (int k = 0; k < 3; k++) grow_point(bx, &g.pos[3 * (size_t)job.tri_idx[3 * t + k]]);
       boxes[t] = bx;
     }
+    if (trees) {
+      job.boxes = std::move(boxes);  // built on the device below
+      return;
+    }
     job.tree = build_tree(boxes, src.blas_opt, per_job);
     job.fast = build_fast_tree(boxes, FastTreeOptions{}, per_job);
   });
+  for (const Job &job : jobs)
+    if (job.bad) {
+      err = "triangle index out of range of the geometry's vertices";
+      return RDRT_ERR_INVALID_ARG;
+    }
+  if (trees)
+    for (Job &job : jobs) {
+      const int rc = trees->build(job.boxes, src.blas_opt, FastTreeOptions{}, job.tree, job.fast, err);
+      if (rc != RDRT_OK) return rc;
+      std::vector<Box>().swap(job.boxes);
+    }
   for (const Job &job : jobs) {
     const GeometrySrc &g = src.blas[job.blas][job.geo];
     for (uint32_t v : job.tri_idx)

@@ -73,7 +73,18 @@ struct BuiltScene {
 
 // Builds every BVH and both layouts.  Returns RDRT_OK or an RDRT_ERR_* code
 // with a message in `err`.
-int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err);
+// Builds the two trees of one geometry (the reference's tree and the fast
+// kernels' tree) from its triangle boxes, with the host builders'
+// semantics.  Implemented on the device (device_build.hip).
+struct DeviceTreeBuilder {
+  virtual ~DeviceTreeBuilder() = default;
+  virtual int build(const std::vector<Box> &boxes, const TreeOptions &ref_opt, const FastTreeOptions &fast_opt,
+                    Tree &ref, Tree &fast, std::string &err) = 0;
+};
+
+// `trees`, when given, builds every BLAS tree (one geometry after another)
+// instead of the host builders.
+int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTreeBuilder *trees = nullptr);
 // BLAS node indices of every layout grouped by depth, deepest level first:
 // the order in which a refit recomputes boxes bottom-up (a level only reads
 // the level below).  *_off[l]..*_off[l + 1] is level l of that order.
