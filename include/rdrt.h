@@ -160,6 +160,8 @@ typedef struct rdrt_scene_info {
   uint64_t device_bytes;
   double build_ms;
   uint32_t tri_quads, tlas_quads;  /* child-quad nodes (and their quantized twins) */
+  double build_phase_ms[4];        /* cumulative at the end of: BLAS trees, forest + pairs + quads,
+                                      triangle records, quantized quads */
 } rdrt_scene_info;
 
 /* ---- errors / version ---- */

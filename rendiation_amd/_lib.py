@@ -59,7 +59,8 @@ class RdrtSceneInfo(C.Structure):
     _fields_ = [("tri_nodes", u32), ("tlas_nodes", u32), ("tris", u32), ("verts", u32),
                 ("instances", u32), ("geometries", u32), ("blas", u32), ("tlas", u32),
                 ("max_blas_depth", u32), ("max_tlas_depth", u32), ("stack_entries", u32), ("quad_stack_entries", u32),
-                ("device_bytes", C.c_uint64), ("build_ms", C.c_double), ("tri_quads", u32), ("tlas_quads", u32)]
+                ("device_bytes", C.c_uint64), ("build_ms", C.c_double), ("tri_quads", u32), ("tlas_quads", u32),
+                ("build_phase_ms", C.c_double * 4)]
 
 
 class RdrtMaterial(C.Structure):

@@ -123,6 +123,7 @@ class SceneInfo:
     build_ms: float
     tri_quads: int = 0
     tlas_quads: int = 0
+    build_phase_ms: tuple = ()
 
 
 @dataclass
@@ -235,7 +236,8 @@ class AccelerationStructureSystem:
         L.check(self._L.rdrt_get_scene_info(self._h, C.byref(info)))
         return SceneInfo(info.tri_nodes, info.tlas_nodes, info.tris, info.verts, info.instances, info.geometries,
                          info.blas, info.tlas, info.max_blas_depth, info.max_tlas_depth, info.stack_entries,
-                         info.quad_stack_entries, info.device_bytes, info.build_ms, info.tri_quads, info.tlas_quads)
+                         info.quad_stack_entries, info.device_bytes, info.build_ms, info.tri_quads, info.tlas_quads,
+                         tuple(round(float(x), 1) for x in info.build_phase_ms))
 
     def copy_built(self, which: str) -> np.ndarray:
         """Built arrays: 'blas_forest' / 'tlas_forest' (48 B nodes as (n, 12)

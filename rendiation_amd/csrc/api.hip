@@ -476,6 +476,7 @@ int32_t rdrt_get_scene_info(const rdrt_ctx *c, rdrt_scene_info *o) {
   o->quad_stack_entries = stack_entries_for(b.blas_stack4 + b.tlas_stack4);
   o->device_bytes = c->dev.bytes;
   o->build_ms = b.build_ms;
+  for (int k = 0; k < 4; k++) o->build_phase_ms[k] = b.phase_ms[k];
   o->tri_quads = (uint32_t)b.tri_quads.size();
   o->tlas_quads = (uint32_t)b.tlas_quads.size();
   return RDRT_OK;

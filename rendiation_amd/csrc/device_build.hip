@@ -121,13 +121,15 @@ __device__ u32 block_node(const Dev &d) {
   return s_nd;
 }
 
-__global__ __launch_bounds__(kB) void k_centres(Dev d) {
+// Centres, the identity order and every primitive's root (geometry g's
+// root is node g; geo_of holds it per primitive).
+__global__ __launch_bounds__(kB) void k_centres(Dev d, const u32 *geo_of) {
   const u32 i = blockIdx.x * kB + threadIdx.x;
   if (i >= d.n) return;
   const DBox b = prim_box(d, i);
   for (int k = 0; k < 3; k++) d.cen[3 * (size_t)i + k] = (b.mn[k] + b.mx[k]) * 0.5f;  // box3.rs:90-93
   d.order[i] = i;
-  d.node_of[i] = 0u;
+  d.node_of[i] = geo_of[i];
 }
 
 // Union of the primitive boxes of every active node flagged `want` (1 =
@@ -170,7 +172,9 @@ __global__ void k_decode_boxes(Dev d, const u32 *keys, u32 want) {
   if (j >= d.m) return;
   const u32 nd = d.lo + j;
   if (want == 2 && !(d.nflags[nd] & 4u)) return;  // only children of median splits
-  for (int k = 0; k < 6; k++) d.nbox[6 * (size_t)nd + k] = funkey(keys[6 * (size_t)j + k]);
+  const bool empty = d.nbeg[nd] == d.nend[nd];     // a geometry without triangles
+  for (int k = 0; k < 6; k++)
+    d.nbox[6 * (size_t)nd + k] = empty ? (k < 3 ? __builtin_inff() : -__builtin_inff()) : funkey(keys[6 * (size_t)j + k]);
 }
 
 // Centre bounds of every active node (fast tree).
@@ -600,15 +604,20 @@ __global__ void k_init_bins(u32 *bins, u32 m, u32 words, u32 nbins) {
   }
 }
 
-__global__ void k_root(Dev d) {
-  d.nbeg[0] = 0u;
-  d.nend[0] = d.n;
-  d.ndepth[0] = 0u;
-  d.nchild[0] = kInvalid;
-  d.nflags[0] = 0u;
-  d.count[0] = 1u;
-  d.count[1] = 0u;
-  d.count[2] = 0u;
+// One root per geometry over its primitive range [off[g], off[g + 1]).
+__global__ void k_roots(Dev d, const u32 *off, u32 ngeo) {
+  const u32 g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g == 0) {
+    d.count[0] = ngeo;
+    d.count[1] = 0u;
+    d.count[2] = 0u;
+  }
+  if (g >= ngeo) return;
+  d.nbeg[g] = off[g];
+  d.nend[g] = off[g + 1];
+  d.ndepth[g] = 0u;
+  d.nchild[g] = kInvalid;
+  d.nflags[g] = 0u;
 }
 
 __global__ void k_reset_counts(u32 *count, u32 total) {
@@ -644,8 +653,9 @@ class Builder final : public DeviceTreeBuilder {
  public:
   explicit Builder(hipStream_t st) : st_(st) {}
 
-  int build(const std::vector<Box> &boxes, const TreeOptions &ref_opt, const FastTreeOptions &fast_opt, Tree &ref,
-            Tree &fast, std::string &err) override {
+  int build(const std::vector<const std::vector<Box> *> &geos, const TreeOptions &ref_opt,
+            const FastTreeOptions &fast_opt, std::vector<Tree> &ref, std::vector<Tree> &fast,
+            std::string &err) override {
     const u32 rb = ref_opt.bins < 2 ? 4u : ref_opt.bins;
     if (rb > kMaxRefBins) {
       err = "device build supports at most 4 reference SAH bins";
@@ -655,21 +665,28 @@ class Builder final : public DeviceTreeBuilder {
       err = "device build supports 16 fast-tree bins";
       return RDRT_ERR_CAPACITY;
     }
-    if (boxes.size() >= (1u << 30)) {
-      err = "device build supports fewer than 2^30 primitives per geometry";
+    size_t total = 0;
+    for (const auto *g : geos) total += g->size();
+    if (total + geos.size() >= (1u << 30)) {
+      err = "device build supports fewer than 2^30 primitives per scene";
       return RDRT_ERR_CAPACITY;
     }
-    if (boxes.empty()) {  // a single empty leaf either way
-      ref = build_tree(boxes, ref_opt, 1);
-      fast = build_fast_tree(boxes, fast_opt, 1);
+    ref.assign(geos.size(), Tree());
+    fast.assign(geos.size(), Tree());
+    if (geos.empty()) return RDRT_OK;
+    if (total == 0) {  // only empty geometries: single empty leaves
+      for (size_t g = 0; g < geos.size(); g++) {
+        ref[g] = build_tree(*geos[g], ref_opt, 1);
+        fast[g] = build_fast_tree(*geos[g], fast_opt, 1);
+      }
       return RDRT_OK;
     }
     error_.clear();
-    hipError_t e = upload(boxes);
+    hipError_t e = upload(geos);
     if (e == hipSuccess) e = run(0, ref_opt, fast_opt, ref);
     if (e == hipSuccess) e = run(1, ref_opt, fast_opt, fast);
     if (!error_.empty()) {
-      err = "device build does not support this geometry: " + error_;
+      err = "device build does not support this scene: " + error_;
       return RDRT_ERR_CAPACITY;
     }
     if (e != hipSuccess) {
@@ -681,32 +698,46 @@ class Builder final : public DeviceTreeBuilder {
 
  private:
   hipStream_t st_;
-  Buf pbox_, cen_, order_, order2_, node_of_, node_of2_, keys_, scan_, bsum_, nodes_, bins_, cb_, count_;
+  Buf pbox_, cen_, order_, order2_, node_of_, node_of2_, keys_, scan_, bsum_, nodes_, bins_, cb_, count_, geo_of_,
+      off_;
   Dev d_{};
   size_t cap_nodes_ = 0;
+  std::vector<u32> off_h_;  // primitive offset of every geometry (+ total)
   std::string error_;
 
-  hipError_t upload(const std::vector<Box> &boxes) {
-    const size_t n = boxes.size();
+  hipError_t upload(const std::vector<const std::vector<Box> *> &geos) {
+    const u32 ng = (u32)geos.size();
+    off_h_.assign(ng + 1, 0u);
+    for (u32 g = 0; g < ng; g++) off_h_[g + 1] = off_h_[g] + (u32)geos[g]->size();
+    const size_t n = off_h_[ng];
     std::vector<float> h(6 * n);
-    for (size_t i = 0; i < n; i++) {
-      std::memcpy(&h[6 * i], boxes[i].mn, 12);
-      std::memcpy(&h[6 * i + 3], boxes[i].mx, 12);
-    }
+    std::vector<u32> gof(n);
+    for (u32 g = 0; g < ng; g++)
+      for (size_t i = 0; i < geos[g]->size(); i++) {
+        const Box &b = (*geos[g])[i];
+        const size_t k = off_h_[g] + i;
+        std::memcpy(&h[6 * k], b.mn, 12);
+        std::memcpy(&h[6 * k + 3], b.mx, 12);
+        gof[k] = g;
+      }
     DB_TRY(pbox_.reserve(6 * n * 4));
     DB_TRY(cen_.reserve(3 * n * 4));
     DB_TRY(order_.reserve(n * 4));
     DB_TRY(order2_.reserve(n * 4));
     DB_TRY(node_of_.reserve(n * 4));
     DB_TRY(node_of2_.reserve(n * 4));
+    DB_TRY(geo_of_.reserve(n * 4));
+    DB_TRY(off_.reserve((ng + 1) * 4));
     DB_TRY(keys_.reserve((n + 1) * 16));
     DB_TRY(scan_.reserve((n + 1) * 16));
     DB_TRY(bsum_.reserve((size_t)(grid_of(n + 1, kScanTile) + 1) * 16));
     DB_TRY(count_.reserve(16));
-    cap_nodes_ = 2 * n + 1;
+    cap_nodes_ = 2 * n + ng;  // every geometry a binary tree over its primitives
     // per node: box 6 f; begin, end, depth, child, flags, lc, axis; origin 3 f, step 3 f; best
     DB_TRY(nodes_.reserve(cap_nodes_ * 20 * 4));
     DB_TRY(hipMemcpyAsync(pbox_.p, h.data(), 6 * n * 4, hipMemcpyHostToDevice, st_));
+    DB_TRY(hipMemcpyAsync(geo_of_.p, gof.data(), n * 4, hipMemcpyHostToDevice, st_));
+    DB_TRY(hipMemcpyAsync(off_.p, off_h_.data(), (ng + 1) * 4, hipMemcpyHostToDevice, st_));
     Dev d{};
     d.n = (u32)n;
     d.pbox = (const float *)pbox_.p;
@@ -730,12 +761,13 @@ class Builder final : public DeviceTreeBuilder {
     d.ns = (float *)(w + 10 * cap_nodes_);
     d.nbest = w + 13 * cap_nodes_;
     d_ = d;
-    return hipStreamSynchronize(st_);  // before the host staging array goes
+    return hipStreamSynchronize(st_);  // before the host staging arrays go
   }
 
-  hipError_t run(u32 mode, const TreeOptions &ro, const FastTreeOptions &fo, Tree &out) {
+  hipError_t run(u32 mode, const TreeOptions &ro, const FastTreeOptions &fo, std::vector<Tree> &out) {
     Dev d = d_;
     const size_t n = d.n;
+    const u32 ng = (u32)off_h_.size() - 1;
     d.order = (u32 *)order_.p;
     d.order2 = (u32 *)order2_.p;
     d.node_of = (u32 *)node_of_.p;
@@ -748,17 +780,17 @@ class Builder final : public DeviceTreeBuilder {
     d.max_leaf = std::max(1u, std::min(fo.max_leaf, kLeafMaxCount));
     d.node_cost = fo.node_cost;
     d.prim_cost = fo.prim_cost;
-    // the root: every primitive, its box the union of them
-    hipLaunchKernelGGL(k_centres, dim3(grid_of(n)), dim3(kB), 0, st_, d);
-    hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, st_, d);
-    DB_TRY(cb_.reserve(6 * 4));
+    // the roots: one per geometry, boxes the unions of their primitives
+    hipLaunchKernelGGL(k_centres, dim3(grid_of(n)), dim3(kB), 0, st_, d, (const u32 *)geo_of_.p);
+    hipLaunchKernelGGL(k_roots, dim3(grid_of(ng, 64)), dim3(64), 0, st_, d, (const u32 *)off_.p, ng);
+    DB_TRY(cb_.reserve((size_t)ng * 6 * 4));
     u32 *keys = (u32 *)cb_.p;
     d.lo = 0;
-    d.m = 1;
-    hipLaunchKernelGGL(k_init_keys, dim3(1), dim3(64), 0, st_, keys, 1u);
+    d.m = ng;
+    hipLaunchKernelGGL(k_init_keys, dim3(std::min(grid_of((size_t)ng * 6), 4096u)), dim3(kB), 0, st_, keys, ng);
     hipLaunchKernelGGL(k_range_boxes, dim3(grid_of(n)), dim3(kB), 0, st_, d, keys);
-    hipLaunchKernelGGL(k_decode_boxes, dim3(1), dim3(64), 0, st_, d, (const u32 *)keys, 1u);
-    u32 lo = 0, m = 1, total = 1;
+    hipLaunchKernelGGL(k_decode_boxes, dim3(grid_of(ng, 64)), dim3(64), 0, st_, d, (const u32 *)keys, 1u);
+    u32 lo = 0, m = ng, total = ng;
     while (m > 0) {
       d.lo = lo;
       d.m = m;
@@ -790,7 +822,7 @@ class Builder final : public DeviceTreeBuilder {
       DB_TRY(hipStreamSynchronize(st_));
       if (counts[2]) {
         error_ = counts[2] & kErrMedian ? "a median split of more than 4096 primitives (reference tree)"
-                                        : "more nodes than 2n + 1 (bin_size 0)";
+                                        : "more nodes than 2n + 1 per geometry (bin_size 0)";
         return hipErrorNotSupported;
       }
       const u32 made = counts[0] - total;
@@ -817,44 +849,52 @@ class Builder final : public DeviceTreeBuilder {
   }
 
   // Level-order nodes (children after parents, a split's two children
-  // adjacent) to the host builders' preorder Tree.
-  hipError_t download(const Dev &d, u32 total, Tree &out) {
+  // adjacent; node g the root of geometry g) to one preorder Tree per
+  // geometry, with geometry-local primitive indices and ranges.
+  hipError_t download(const Dev &d, u32 total, std::vector<Tree> &out) {
     const size_t n = d.n;
+    const u32 ng = (u32)off_h_.size() - 1;
     std::vector<float> box(6 * (size_t)total);
-    std::vector<u32> beg(total), end(total), child(total), flags(total), depth(total);
-    out.order.resize(n);
+    std::vector<u32> beg(total), end(total), child(total), flags(total), depth(total), order(n);
     DB_TRY(hipMemcpyAsync(box.data(), d.nbox, box.size() * 4, hipMemcpyDeviceToHost, st_));
     DB_TRY(hipMemcpyAsync(beg.data(), d.nbeg, total * 4, hipMemcpyDeviceToHost, st_));
     DB_TRY(hipMemcpyAsync(end.data(), d.nend, total * 4, hipMemcpyDeviceToHost, st_));
     DB_TRY(hipMemcpyAsync(child.data(), d.nchild, total * 4, hipMemcpyDeviceToHost, st_));
     DB_TRY(hipMemcpyAsync(flags.data(), d.nflags, total * 4, hipMemcpyDeviceToHost, st_));
     DB_TRY(hipMemcpyAsync(depth.data(), d.ndepth, total * 4, hipMemcpyDeviceToHost, st_));
-    DB_TRY(hipMemcpyAsync(out.order.data(), d.order, n * 4, hipMemcpyDeviceToHost, st_));
+    DB_TRY(hipMemcpyAsync(order.data(), d.order, n * 4, hipMemcpyDeviceToHost, st_));
     DB_TRY(hipStreamSynchronize(st_));
     std::vector<u32> size(total, 1u);
     for (u32 i = total; i-- > 0;)
       if (flags[i] & 1u) size[i] = 1u + size[child[i]] + size[child[i] + 1];
-    out.nodes.clear();
-    out.nodes.reserve(total);
-    out.depth = 0;
-    std::vector<u32> stack{0u};
-    while (!stack.empty()) {
-      const u32 i = stack.back();
-      stack.pop_back();
-      TreeNode t;
-      for (int k = 0; k < 3; k++) {
-        t.box.mn[k] = box[6 * (size_t)i + k];
-        t.box.mx[k] = box[6 * (size_t)i + 3 + k];
-      }
-      t.begin = beg[i];
-      t.end = end[i];
-      t.size = size[i];
-      t.inner = (flags[i] & 1u) != 0;
-      out.nodes.push_back(t);
-      out.depth = std::max(out.depth, depth[i]);
-      if (t.inner) {
-        stack.push_back(child[i] + 1);
-        stack.push_back(child[i]);
+    std::vector<u32> stack;
+    for (u32 g = 0; g < ng; g++) {
+      Tree &t = out[g];
+      const u32 base = off_h_[g];
+      t.order.assign(order.begin() + base, order.begin() + off_h_[g + 1]);
+      for (u32 &p : t.order) p -= base;
+      t.nodes.clear();
+      t.nodes.reserve(size[g]);
+      t.depth = 0;
+      stack.assign(1, g);
+      while (!stack.empty()) {
+        const u32 i = stack.back();
+        stack.pop_back();
+        TreeNode tn;
+        for (int k = 0; k < 3; k++) {
+          tn.box.mn[k] = box[6 * (size_t)i + k];
+          tn.box.mx[k] = box[6 * (size_t)i + 3 + k];
+        }
+        tn.begin = beg[i] - base;
+        tn.end = end[i] - base;
+        tn.size = size[i];
+        tn.inner = (flags[i] & 1u) != 0;
+        t.nodes.push_back(tn);
+        t.depth = std::max(t.depth, depth[i]);
+        if (tn.inner) {
+          stack.push_back(child[i] + 1);
+          stack.push_back(child[i]);
+        }
       }
     }
     return hipSuccess;

@@ -756,12 +756,18 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
       err = "triangle index out of range of the geometry's vertices";
       return RDRT_ERR_INVALID_ARG;
     }
-  if (trees)
-    for (Job &job : jobs) {
-      const int rc = trees->build(job.boxes, src.blas_opt, FastTreeOptions{}, job.tree, job.fast, err);
-      if (rc != RDRT_OK) return rc;
-      std::vector<Box>().swap(job.boxes);
+  if (trees) {
+    std::vector<const std::vector<Box> *> geos;
+    for (const Job &job : jobs) geos.push_back(&job.boxes);
+    std::vector<Tree> ref, fast;
+    const int rc = trees->build(geos, src.blas_opt, FastTreeOptions{}, ref, fast, err);
+    if (rc != RDRT_OK) return rc;
+    for (size_t j = 0; j < jobs.size(); j++) {
+      jobs[j].tree = std::move(ref[j]);
+      jobs[j].fast = std::move(fast[j]);
+      std::vector<Box>().swap(jobs[j].boxes);
     }
+  }
   for (const Job &job : jobs) {
     const GeometrySrc &g = src.blas[job.blas][job.geo];
     for (uint32_t v : job.tri_idx)
@@ -771,6 +777,7 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
       }
   }
 
+  out.phase_ms[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();  // trees
   // ---- assemble BLAS arrays in reference order ----
   std::vector<Box> blas_box(src.blas.size(), box_empty());
   std::vector<char> blas_has_box(src.blas.size(), 0);
@@ -782,6 +789,57 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
         any_normals |= !g.normals.empty();
         any_material |= !g.tri_material.empty();
       }
+  // Per geometry, in parallel: forest, child pairs and child quads with
+  // local node indices (leaf refs already global), relocated when appended.
+  struct Local {
+    std::vector<RefNode> forest;
+    std::vector<PairNode> pairs;
+    std::vector<QuadNode> quads;
+    uint32_t root_ref = 0, quad_root = 0, stack = 0, stack4 = 0;
+    bool ok = true;
+    std::string err;
+  };
+  std::vector<Local> local(jobs.size());
+  {
+    std::vector<uint32_t> prim_start(jobs.size(), 0);
+    uint64_t acc = 0;
+    for (size_t j = 0; j < jobs.size(); j++) {
+      prim_start[j] = (uint32_t)acc;
+      acc += jobs[j].tri_idx.size() / 3;
+    }
+    parallel_for(jobs.size(), hw, [&](size_t j) {
+      Local &l = local[j];
+      append_forest(jobs[j].tree, prim_start[j], l.forest);
+      PairConv pc{jobs[j].fast, prim_start[j], l.pairs, l.err};
+      l.root_ref = pc.ref_of(0, l.stack);
+      l.ok = pc.ok;
+      QuadConv qc{l.pairs, l.quads};
+      l.quad_root = qc.ref_of(l.root_ref, l.stack4);
+    });
+  }
+  auto inner = [](uint32_t r) { return r != kInvalid && !(r & kLeafBit); };
+  {
+    size_t nf = 0, np = 0, nq = 0, nt = 0, nv = 0;
+    size_t ji = 0;
+    for (uint32_t b = 0; b < src.blas.size(); b++)
+      if (src.blas_live[b])
+        for (uint32_t gi = 0; gi < src.blas[b].size(); gi++, ji++) {
+          nf += local[ji].forest.size();
+          np += local[ji].pairs.size();
+          nq += local[ji].quads.size();
+          nt += jobs[ji].tri_idx.size() / 3;
+          nv += src.blas[b][gi].pos.size() / 3;
+        }
+    out.tri_forest.reserve(nf);
+    out.tri_pairs.reserve(np);
+    out.tri_quads.reserve(nq);
+    out.redirect.reserve(nt);
+    out.tri_ids.reserve(nt);
+    out.indices.reserve(3 * nt);
+    out.vertices.reserve(3 * nv);
+    if (any_normals) out.normals.reserve(3 * nv);
+    if (any_material) out.tri_material.reserve(nt);
+  }
   {
     size_t ji = 0;
     for (uint32_t b = 0; b < src.blas.size(); b++) {
@@ -817,19 +875,39 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
         GeoRec gr;
         std::memset(&gr, 0, sizeof(gr));
         put_box(gr.mn, gr.mx, job.tree.nodes[0].box);
-        gr.ref_root = (uint32_t)out.tri_forest.size();
         gr.primitive_start = primitive_start;
         gr.geometry_idx = gi;
         gr.flags = g.flags;
-        append_forest(job.tree, primitive_start, out.tri_forest);
-        PairConv pc{job.fast, primitive_start, out.tri_pairs, err};
-        uint32_t d = 0;
-        gr.root_ref = pc.ref_of(0, d);
-        if (!pc.ok) return RDRT_ERR_CAPACITY;
-        out.blas_stack = std::max(out.blas_stack, d);
-        QuadConv qc{out.tri_pairs, out.tri_quads};
-        gr.quad_root = qc.ref_of(gr.root_ref, d);
-        out.blas_stack4 = std::max(out.blas_stack4, d);
+        Local &l = local[ji];
+        if (!l.ok) {
+          err = l.err;
+          return RDRT_ERR_CAPACITY;
+        }
+        const uint32_t fb = (uint32_t)out.tri_forest.size(), pb = (uint32_t)out.tri_pairs.size(),
+                       qb = (uint32_t)out.tri_quads.size();
+        gr.ref_root = fb;
+        for (RefNode n : l.forest) {
+          if (n.hit_next != kInvalid) n.hit_next += fb;
+          if (n.miss_next != kInvalid) n.miss_next += fb;
+          out.tri_forest.push_back(n);
+        }
+        for (PairNode p : l.pairs) {
+          if (inner(p.lref)) p.lref += pb;
+          if (inner(p.rref)) p.rref += pb;
+          out.tri_pairs.push_back(p);
+        }
+        for (QuadNode q : l.quads) {
+          for (int k = 0; k < 4; k++)
+            if (inner(q.ref[k])) q.ref[k] += qb;
+          out.tri_quads.push_back(q);
+        }
+        gr.root_ref = inner(l.root_ref) ? l.root_ref + pb : l.root_ref;
+        gr.quad_root = inner(l.quad_root) ? l.quad_root + qb : l.quad_root;
+        out.blas_stack = std::max(out.blas_stack, l.stack);
+        out.blas_stack4 = std::max(out.blas_stack4, l.stack4);
+        std::vector<RefNode>().swap(l.forest);
+        std::vector<PairNode>().swap(l.pairs);
+        std::vector<QuadNode>().swap(l.quads);
         out.max_blas_depth = std::max(out.max_blas_depth, job.tree.depth);
         out.geos.push_back(gr);
         grow_box(blas_box[b], job.tree.nodes[0].box);
@@ -838,6 +916,7 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
       blas_geo_end[b] = (uint32_t)out.geos.size();
     }
   }
+  out.phase_ms[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();  // forest, pairs, quads
   // triangle pre-pass, in the fast tree's leaf order
   out.tris.resize(out.tri_ids.size());
   parallel_for((out.tris.size() + 65535) / 65536, hw, [&](size_t c) {
@@ -851,6 +930,7 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
     }
   });
 
+  out.phase_ms[2] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();  // triangle records
   // quantized twins of the child-quad nodes
   out.tri_qquads.resize(out.tri_quads.size());
   parallel_for((out.tri_quads.size() + 65535) / 65536, hw, [&](size_t c) {
@@ -862,6 +942,7 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
   out.blas_geo_begin = blas_geo_begin;
   out.blas_geo_end = blas_geo_end;
   {
+    out.phase_ms[3] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();  // quantized quads
     const int rc = build_tlas_layer(src, out, err);
     if (rc != RDRT_OK) return rc;
   }

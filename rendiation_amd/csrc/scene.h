@@ -69,17 +69,22 @@ struct BuiltScene {
   std::vector<char> blas_has_box;
   std::vector<uint32_t> blas_geo_begin, blas_geo_end;
   double build_ms = 0.0;
+  // cumulative ms at the end of: trees, forest + pairs + quads, triangle
+  // records, quantized quads (then the TLAS layer and checks)
+  double phase_ms[4] = {0.0, 0.0, 0.0, 0.0};
 };
 
 // Builds every BVH and both layouts.  Returns RDRT_OK or an RDRT_ERR_* code
 // with a message in `err`.
-// Builds the two trees of one geometry (the reference's tree and the fast
-// kernels' tree) from its triangle boxes, with the host builders'
-// semantics.  Implemented on the device (device_build.hip).
+// Builds the two trees of every geometry (the reference's tree and the
+// fast kernels' tree) from their triangle boxes, with the host builders'
+// semantics, all geometries together.  Implemented on the device
+// (device_build.hip).
 struct DeviceTreeBuilder {
   virtual ~DeviceTreeBuilder() = default;
-  virtual int build(const std::vector<Box> &boxes, const TreeOptions &ref_opt, const FastTreeOptions &fast_opt,
-                    Tree &ref, Tree &fast, std::string &err) = 0;
+  virtual int build(const std::vector<const std::vector<Box> *> &geos, const TreeOptions &ref_opt,
+                    const FastTreeOptions &fast_opt, std::vector<Tree> &ref, std::vector<Tree> &fast,
+                    std::string &err) = 0;
 };
 
 // `trees`, when given, builds every BLAS tree (one geometry after another)

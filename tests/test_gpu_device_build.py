@@ -3,8 +3,9 @@
 The device builds both trees of every geometry level by level with the host
 builders' split arithmetic, so the built layouts equal the host build's: the
 reference forest node for node (boxes, links, ranges) and the fast tree's
-child quads exactly; the triangle order inside a leaf may differ only under a
-reference median split (compared as sets).  Traversal after a device build
+child quads exactly; a reference median split may choose differently among
+triangles whose centres tie, so leaves are compared by their triangles'
+centres.  Traversal after a device build
 equals the oracle bit for bit in the reference order.
 """
 
@@ -57,15 +58,21 @@ def _system(pos, idx, device, opts):
     return s
 
 
-def _leaf_sets(forest, redirect, tris):
-    """{(begin, end): sorted triangle vertex tuples} of the forest's leaves
-    (by geometry, so identical triangles are interchangeable: a median split
-    among equal centres may pick either)."""
+def _centres(tris):
+    """Box centres of triangles (n, 3, 3), as the builders compute them."""
+    return (tris.min(axis=1) + tris.max(axis=1)) * np.float32(0.5)
+
+
+def _leaf_sets(forest, redirect, cen):
+    """{(begin, end): sorted box centres of its triangles} of the forest's
+    leaves: a median split chooses freely among triangles whose centres tie
+    (the two triangles of a height-field cell share one), so leaves are
+    compared by centres, not triangle ids."""
     out = {}
     for n in forest:
         if n[3] == n[7]:  # hit == miss: a leaf
             b, e = int(n[8]), int(n[9])
-            out[(b, e)] = tuple(sorted(tuple(tris[t].reshape(-1).tolist()) for t in redirect[b:e]))
+            out[(b, e)] = tuple(sorted(tuple(cen[t].tolist()) for t in redirect[b:e]))
     return out
 
 
@@ -82,7 +89,8 @@ def test_device_build_equals_host_build(name, pos, idx, device):
     assert np.array_equal(hf[:, [3, 7, 8, 9]], df[:, [3, 7, 8, 9]]), name
     hr, dr = host.copy_built("redirect"), dev.copy_built("redirect")
     tris = pos.reshape(-1, 3)[idx.reshape(-1)].reshape(-1, 3, 3) if idx is not None else pos.reshape(-1, 3, 3)
-    assert _leaf_sets(hf, hr, tris) == _leaf_sets(df, dr, tris), name
+    cen = _centres(tris)
+    assert _leaf_sets(hf, hr, cen) == _leaf_sets(df, dr, cen), name
     hq, dq = host.copy_built("blas_quads"), dev.copy_built("blas_quads")
     assert hq.shape == dq.shape, name
     assert np.array_equal(hq[:, 24:28], dq[:, 24:28]), name  # refs: same tree, same leaves
@@ -116,3 +124,44 @@ def test_device_built_scene_traces_like_the_oracle(device):
     assert np.array_equal(hm.cpu().numpy().view(np.uint32), rm)
     assert np.array_equal(cnt.cpu().numpy().view(np.uint32), rc)
     s.close()
+
+
+def test_device_build_of_many_geometries(device):
+    """Every geometry of a scene in one device build: several BLAS (one
+    with two geometries), each equal to its host build."""
+    opts = BuildOptions()
+    hp, hi, _ = synth.height_field(40, seed=6)
+    blas = [[(synth.random_triangles(7, 5000), None)], [(hp, hi), (synth.random_triangles(8, 1), None)],
+            [(synth.random_triangles(9, 300, spread=0.2), None)]]
+    _say("device build of several BLAS at once")
+
+    def make(device_build):
+        s = AccelerationStructureSystem(device=0 if device_build else None, options=opts)
+        if device_build:
+            s.set_device_build(True)
+        hs = [s.create_bottom_level_acceleration_structure([TriangleGeometry(p, i) for p, i in geos]) for geos in blas]
+        t = s.create_top_level_acceleration_structure(
+            [TopLevelAccelerationStructureSourceInstance(np.eye(4), h) for h in hs])
+        s.bind_tlas([t])
+        s.build()
+        return s
+
+    host, dev = make(False), make(True)
+    for what in ("blas_forest", "blas_quads"):
+        a, b = host.copy_built(what), dev.copy_built(what)
+        assert a.shape == b.shape, what
+        nf = 24 if what == "blas_quads" else None
+        if nf:
+            assert np.array_equal(a[:, :nf].view(np.float32), b[:, :nf].view(np.float32))
+            assert np.array_equal(a[:, nf:], b[:, nf:])
+        else:
+            cols = [0, 1, 2, 4, 5, 6]
+            assert np.array_equal(a[:, cols].view(np.float32), b[:, cols].view(np.float32))
+            assert np.array_equal(a[:, [3, 7, 8, 9]], b[:, [3, 7, 8, 9]])
+    forest, hr, dr = host.copy_built("blas_forest"), host.copy_built("redirect"), dev.copy_built("redirect")
+    tris = np.concatenate([p.reshape(-1, 3)[i.reshape(-1)].reshape(-1, 3, 3) if i is not None else p.reshape(-1, 3, 3)
+                           for geos in blas for p, i in geos])
+    cen = _centres(tris)
+    assert _leaf_sets(forest, hr, cen) == _leaf_sets(forest, dr, cen)
+    host.close()
+    dev.close()
