@@ -90,6 +90,7 @@ struct Dev {
   int *naxis;
   float *na, *ns;      // bin origin and step (reference) or centre min and scale (fast)
   u32 *nbest;          // fast: best bin
+  u32 *nright, *ngeo;  // right-child steps on the path from the root; geometry
   u32 *count;          // [0] node count, [1] median splits this level, [2] error bits
   u32 cap;             // node capacity
   // per active slot
@@ -106,7 +107,13 @@ __device__ __forceinline__ DBox prim_box(const Dev &d, u32 p) {
   const float *q = d.pbox + 6 * (size_t)p;
   return DBox{{q[0], q[1], q[2]}, {q[3], q[4], q[5]}};
 }
-__device__ __forceinline__ bool active(const Dev &d, u32 nd) { return nd != kInvalid && nd - d.lo < d.m; }
+constexpr u32 kDone = 8u;    // node flag: its subtree was built whole by one thread (k_small)
+constexpr u32 kSmall = 64u;  // nodes of at most this many primitives are built by one thread
+
+// A node of this level still built by the level kernels.
+__device__ __forceinline__ bool active(const Dev &d, u32 nd) {
+  return nd != kInvalid && nd - d.lo < d.m && !(d.nflags[nd] & kDone);
+}
 
 // One block's primitives all lie in one node when its first and last do
 // (node ranges are contiguous).  Returns that node or kInvalid.
@@ -211,6 +218,7 @@ __global__ void k_prep(Dev d) {
   const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= d.m) return;
   const u32 nd = d.lo + j;
+  if (d.nflags[nd] & kDone) return;
   if (d.mode == 0) {
     DBox b;
     for (int k = 0; k < 3; k++) {
@@ -329,6 +337,8 @@ __device__ void new_children(Dev &d, u32 nd, const DBox &lb, const DBox &rb, u32
     d.ndepth[c + s] = depth;
     d.nchild[c + s] = kInvalid;
     d.nflags[c + s] = median ? 4u : 0u;
+    d.nright[c + s] = d.nright[nd] + (u32)s;
+    d.ngeo[c + s] = d.ngeo[nd];
   }
   d.nchild[nd] = c;
   d.nflags[nd] |= 1u | (median ? 2u : 0u);
@@ -341,6 +351,7 @@ __global__ void k_split(Dev d) {
   const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= d.m) return;
   const u32 nd = d.lo + j;
+  if (d.nflags[nd] & kDone) return;
   const u32 b = d.nbeg[nd], e = d.nend[nd], n = e - b, depth = d.ndepth[nd];
   const u32 *q = d.bins + (size_t)kMaxSlotWords * j;
   DBox nb;
@@ -442,6 +453,7 @@ __global__ void k_median_sort(Dev d) {
   const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= d.m) return;
   const u32 nd = d.lo + j;
+  if (d.nflags[nd] & kDone) return;
   if (!(d.nflags[nd] & 2u) || d.mode != 0) return;
   const u32 b = d.nbeg[nd], e = d.nend[nd];
   const int axis = d.naxis[nd];
@@ -457,6 +469,212 @@ __global__ void k_median_sort(Dev d) {
     }
     for (u32 k = i; k > lo; k--) d.order[k] = d.order[k - 1];
     d.order[lo] = p;
+  }
+}
+
+// Whole subtrees of small nodes (at most kSmall primitives), one thread
+// each, with the host builders' recursion (bvh_build.cpp Builder::build /
+// FastBuilder::build): the same decisions as the level kernels without
+// their atomics.  order2 over the node's range is the scratch of the stable
+// reorders (the level's scatter rewrites it afterwards).
+__device__ DBox range_union(const Dev &d, u32 b, u32 e) {
+  DBox r = dempty();
+  for (u32 i = b; i < e; i++) dgrow(r, prim_box(d, d.order[i]));
+  return r;
+}
+
+__device__ bool small_children(Dev &d, u32 id, const DBox &lb, const DBox &rb, u32 mid, u32 *stack, int &sp) {
+  const u32 c = atomicAdd(&d.count[0], 2u);
+  if (c + 2u > d.cap || sp + 2 > 128) {
+    atomicOr(&d.count[2], kErrNodes);
+    return false;
+  }
+  const u32 b = d.nbeg[id], e = d.nend[id], depth = d.ndepth[id] + 1;
+  const DBox kids[2] = {lb, rb};
+  const u32 beg[2] = {b, mid}, end[2] = {mid, e};
+  for (int q = 0; q < 2; q++) {
+    for (int k = 0; k < 3; k++) {
+      d.nbox[6 * (size_t)(c + q) + k] = kids[q].mn[k];
+      d.nbox[6 * (size_t)(c + q) + 3 + k] = kids[q].mx[k];
+    }
+    d.nbeg[c + q] = beg[q];
+    d.nend[c + q] = end[q];
+    d.ndepth[c + q] = depth;
+    d.nchild[c + q] = kInvalid;
+    d.nflags[c + q] = kDone;
+    d.nright[c + q] = d.nright[id] + (u32)q;
+    d.ngeo[c + q] = d.ngeo[id];
+  }
+  d.nchild[id] = c;
+  d.nflags[id] |= 1u | kDone;
+  d.nlc[id] = mid - b;
+  stack[sp++] = c + 1;
+  stack[sp++] = c;
+  return true;
+}
+
+__global__ void k_small(Dev d) {
+  const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= d.m) return;
+  const u32 root = d.lo + j;
+  if ((d.nflags[root] & kDone) || d.nend[root] - d.nbeg[root] > kSmall) return;
+  d.nflags[root] |= kDone;
+  d.nchild[root] = kInvalid;
+  u32 stack[128];
+  int sp = 0;
+  stack[sp++] = root;
+  while (sp > 0) {
+    const u32 id = stack[--sp];
+    const u32 b = d.nbeg[id], e = d.nend[id], n = e - b, depth = d.ndepth[id];
+    DBox box;
+    for (int k = 0; k < 3; k++) {
+      box.mn[k] = d.nbox[6 * (size_t)id + k];
+      box.mx[k] = d.nbox[6 * (size_t)id + 3 + k];
+    }
+    if (d.mode == 0) {
+      if (!(depth < d.max_depth && n > d.bin_size)) continue;  // utils.rs:20-37
+      const u32 K = d.K;
+      const int axis = dlongest(box);
+      const float a0 = box.mn[axis], step = (box.mx[axis] - box.mn[axis]) / (float)K;
+      u32 cnt[kMaxRefBins] = {0u, 0u, 0u, 0u};
+      DBox bb[kMaxRefBins] = {dempty(), dempty(), dempty(), dempty()};
+      for (u32 i = b; i < e; i++) {
+        const u32 p = d.order[i];
+        const u32 w = ref_bin(d.cen[3 * (size_t)p + axis], a0, step, K);
+        cnt[w]++;
+        dgrow(bb[w], prim_box(d, p));
+      }
+      u32 empty = 0;
+      for (u32 k = 0; k < K; k++) empty += cnt[k] == 0 ? 1u : 0u;
+      if (empty == K - 1) {  // median on the longest axis: stable sort by centre, halves
+        for (u32 i = b + 1; i < e; i++) {
+          const u32 p = d.order[i];
+          const float c = d.cen[3 * (size_t)p + axis];
+          u32 k = i;
+          while (k > b && c < d.cen[3 * (size_t)d.order[k - 1] + axis]) {
+            d.order[k] = d.order[k - 1];
+            k--;
+          }
+          d.order[k] = p;
+        }
+        const u32 mid = b + n / 2;
+        if (!small_children(d, id, range_union(d, b, mid), range_union(d, mid, e), mid, stack, sp)) return;
+        continue;
+      }
+      float best_cost = __builtin_inff();
+      DBox best_l = dempty(), best_r = dempty();
+      u32 best_lc = 0;
+      for (u32 q = 0; q + 1 < K; q++) {
+        DBox l = dempty(), r = dempty();
+        u32 lc = 0, rc = 0;
+        for (u32 k = 0; k <= q; k++) {
+          lc += cnt[k];
+          dgrow(l, bb[k]);
+        }
+        for (u32 k = q + 1; k < K; k++) {
+          rc += cnt[k];
+          dgrow(r, bb[k]);
+        }
+        const float cost = darea(l) * (float)lc + darea(r) * (float)rc;
+        if (cost < best_cost) {
+          best_cost = cost;
+          best_l = l;
+          best_r = r;
+          best_lc = lc;
+        }
+      }
+      // stable reorder by bin
+      u32 off[kMaxRefBins];
+      off[0] = 0;
+      for (u32 k = 1; k < K; k++) off[k] = off[k - 1] + cnt[k - 1];
+      for (u32 i = b; i < e; i++) {
+        const u32 p = d.order[i];
+        d.order2[b + off[ref_bin(d.cen[3 * (size_t)p + axis], a0, step, K)]++] = p;
+      }
+      for (u32 i = b; i < e; i++) d.order[i] = d.order2[i];
+      if (!small_children(d, id, best_l, best_r, b + best_lc, stack, sp)) return;
+      continue;
+    }
+    // fast tree
+    if (n <= 1 || depth >= d.max_depth) continue;
+    const u32 K = d.K;
+    float cmn[3], scale[3];
+    bool live_any = false;
+    for (int k = 0; k < 3; k++) {
+      float lo = __builtin_inff(), hi = -__builtin_inff();
+      for (u32 i = b; i < e; i++) {
+        const float c = d.cen[3 * (size_t)d.order[i] + k];
+        lo = fminf(lo, c);
+        hi = fmaxf(hi, c);
+      }
+      const float ext = hi - lo;
+      cmn[k] = lo;
+      scale[k] = ext > 0.0f ? (float)K / ext : 0.0f;
+      live_any |= ext > 0.0f;
+    }
+    if (!live_any) {  // every centre coincides: object median
+      const u32 mid = b + n / 2;
+      if (!small_children(d, id, range_union(d, b, mid), range_union(d, mid, e), mid, stack, sp)) return;
+      continue;
+    }
+    u32 cnt[3 * kFastBins];
+    DBox bb[3 * kFastBins];
+    for (u32 k = 0; k < 3 * K; k++) {
+      cnt[k] = 0u;
+      bb[k] = dempty();
+    }
+    for (u32 i = b; i < e; i++) {
+      const u32 p = d.order[i];
+      const DBox pb = prim_box(d, p);
+      for (int a = 0; a < 3; a++) {
+        if (!(scale[a] > 0.0f)) continue;
+        const u32 w = fast_bin(d.cen[3 * (size_t)p + a], cmn[a], scale[a], K);
+        cnt[a * K + w]++;
+        dgrow(bb[a * K + w], pb);
+      }
+    }
+    const float parent_area = darea(box);
+    double best_cost = __builtin_inf();
+    int best_axis = -1;
+    u32 best_bin = 0;
+    double right_cost[kFastBins];
+    for (int axis = 0; axis < 3; axis++) {
+      if (!(scale[axis] > 0.0f)) continue;
+      DBox r = dempty();
+      u32 rc = 0;
+      for (u32 k = K - 1; k >= 1; k--) {
+        dgrow(r, bb[axis * K + k]);
+        rc += cnt[axis * K + k];
+        right_cost[k] = (double)darea(r) * rc;
+      }
+      DBox l = dempty();
+      u32 lc = 0;
+      for (u32 k = 0; k + 1 < K; k++) {
+        dgrow(l, bb[axis * K + k]);
+        lc += cnt[axis * K + k];
+        if (lc == 0 || lc == n) continue;
+        const double cost = (double)darea(l) * lc + right_cost[k + 1];
+        if (cost < best_cost) {
+          best_cost = cost;
+          best_axis = axis;
+          best_bin = k;
+        }
+      }
+    }
+    const double split_cost = d.node_cost + d.prim_cost * best_cost / fmax((double)parent_area, 1e-30);
+    if (n <= d.max_leaf && split_cost >= d.prim_cost * n) continue;  // leaf
+    if (best_axis < 0) continue;                                    // unreachable: a live axis splits
+    DBox lb = dempty(), rb = dempty();
+    for (u32 k = 0; k < K; k++) dgrow(k <= best_bin ? lb : rb, bb[best_axis * K + k]);
+    // stable partition by the chosen bin boundary
+    u32 lo = b, hi = 0;
+    for (u32 i = b; i < e; i++) {
+      const u32 p = d.order[i];
+      if (fast_bin(d.cen[3 * (size_t)p + best_axis], cmn[best_axis], scale[best_axis], K) <= best_bin) d.order[lo++] = p;
+      else d.order2[b + hi++] = p;
+    }
+    for (u32 i = 0; i < hi; i++) d.order[lo + i] = d.order2[b + i];
+    if (!small_children(d, id, lb, rb, lo, stack, sp)) return;
   }
 }
 
@@ -588,6 +806,45 @@ __global__ __launch_bounds__(kB) void k_scatter(Dev d) {
   d.node_of2[dest] = d.nchild[nd] + (dest < b + d.nlc[nd] ? 0u : 1u);
 }
 
+// Preorder from one scan.  Every inner node has two children and the
+// leaves of a geometry tile its primitive range, so with L(p) the number of
+// leaves starting before position p: a subtree holds 2 (L(end) - L(begin)) - 1
+// nodes, and a node's preorder index is its depth plus the nodes of every
+// subtree left of its path, 2 (L(begin) - L(geometry start)) - r, r being the
+// right-child steps on its path.
+__global__ void k_leaf_marks(Dev d, u32 total) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  if (!(d.nflags[i] & 1u) && d.nbeg[i] < d.nend[i]) d.keys[d.nbeg[i]] = make_uint4(1u, 0u, 0u, 0u);
+}
+
+__global__ void k_clear_keys(Dev d) {
+  const u32 i = blockIdx.x * kB + threadIdx.x;
+  if (i <= d.n) d.keys[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+__global__ void k_emit(Dev d, u32 total, const u32 *goff, const u32 *gbase, TreeNode *out) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const u32 g = d.ngeo[i], b = d.nbeg[i], e = d.nend[i];
+  const u32 lb = d.scan[b].x, le = d.scan[e].x, l0 = d.scan[goff[g]].x;
+  TreeNode t;
+  for (int k = 0; k < 3; k++) {
+    t.box.mn[k] = d.nbox[6 * (size_t)i + k];
+    t.box.mx[k] = d.nbox[6 * (size_t)i + 3 + k];
+  }
+  t.begin = b - goff[g];
+  t.end = e - goff[g];
+  t.inner = (d.nflags[i] & 1u) != 0;
+  t.size = b < e ? 2u * (le - lb) - 1u : 1u;
+  const u32 at = gbase[g] + d.ndepth[i] + 2u * (lb - l0) - d.nright[i];
+  if (at >= total) {  // not a full binary tree over tiling leaves: reported, nothing written
+    atomicOr(&d.count[2], kErrNodes);
+    return;
+  }
+  out[at] = t;
+}
+
 u32 grid_of(size_t n, u32 b = kB) { return (u32)std::max<size_t>(1, (n + b - 1) / b); }
 
 __global__ void k_init_keys(u32 *keys, u32 m) {
@@ -618,6 +875,8 @@ __global__ void k_roots(Dev d, const u32 *off, u32 ngeo) {
   d.ndepth[g] = 0u;
   d.nchild[g] = kInvalid;
   d.nflags[g] = 0u;
+  d.nright[g] = 0u;
+  d.ngeo[g] = g;
 }
 
 __global__ void k_reset_counts(u32 *count, u32 total) {
@@ -699,7 +958,7 @@ class Builder final : public DeviceTreeBuilder {
  private:
   hipStream_t st_;
   Buf pbox_, cen_, order_, order2_, node_of_, node_of2_, keys_, scan_, bsum_, nodes_, bins_, cb_, count_, geo_of_,
-      off_;
+      off_, gbase_, emit_;
   Dev d_{};
   size_t cap_nodes_ = 0;
   std::vector<u32> off_h_;  // primitive offset of every geometry (+ total)
@@ -734,7 +993,7 @@ class Builder final : public DeviceTreeBuilder {
     DB_TRY(count_.reserve(16));
     cap_nodes_ = 2 * n + ng;  // every geometry a binary tree over its primitives
     // per node: box 6 f; begin, end, depth, child, flags, lc, axis; origin 3 f, step 3 f; best
-    DB_TRY(nodes_.reserve(cap_nodes_ * 20 * 4));
+    DB_TRY(nodes_.reserve(cap_nodes_ * 22 * 4));
     DB_TRY(hipMemcpyAsync(pbox_.p, h.data(), 6 * n * 4, hipMemcpyHostToDevice, st_));
     DB_TRY(hipMemcpyAsync(geo_of_.p, gof.data(), n * 4, hipMemcpyHostToDevice, st_));
     DB_TRY(hipMemcpyAsync(off_.p, off_h_.data(), (ng + 1) * 4, hipMemcpyHostToDevice, st_));
@@ -760,6 +1019,8 @@ class Builder final : public DeviceTreeBuilder {
     d.na = (float *)(w + 7 * cap_nodes_);
     d.ns = (float *)(w + 10 * cap_nodes_);
     d.nbest = w + 13 * cap_nodes_;
+    d.nright = w + 14 * cap_nodes_;
+    d.ngeo = w + 15 * cap_nodes_;
     d_ = d;
     return hipStreamSynchronize(st_);  // before the host staging arrays go
   }
@@ -799,6 +1060,7 @@ class Builder final : public DeviceTreeBuilder {
       d.bins = (u32 *)bins_.p;
       d.cb = (u32 *)cb_.p;
       const u32 g = grid_of(n), gn = grid_of(m, 64);
+      hipLaunchKernelGGL(k_small, dim3(gn), dim3(64), 0, st_, d);
       if (mode == 1) {
         hipLaunchKernelGGL(k_init_keys, dim3(std::min(grid_of((size_t)m * 6), 4096u)), dim3(kB), 0, st_, d.cb, m);
         hipLaunchKernelGGL(k_centre_bounds, dim3(g), dim3(kB), 0, st_, d);
@@ -848,55 +1110,53 @@ class Builder final : public DeviceTreeBuilder {
     return download(d, total, out);
   }
 
-  // Level-order nodes (children after parents, a split's two children
-  // adjacent; node g the root of geometry g) to one preorder Tree per
-  // geometry, with geometry-local primitive indices and ranges.
-  hipError_t download(const Dev &d, u32 total, std::vector<Tree> &out) {
+  // Preorder nodes of every geometry emitted on the device (k_emit), then
+  // one copy of them and of the order to the host, split per geometry
+  // (geometry-local primitive indices and ranges).
+  hipError_t download(Dev &d, u32 total, std::vector<Tree> &out) {
     const size_t n = d.n;
     const u32 ng = (u32)off_h_.size() - 1;
-    std::vector<float> box(6 * (size_t)total);
-    std::vector<u32> beg(total), end(total), child(total), flags(total), depth(total), order(n);
-    DB_TRY(hipMemcpyAsync(box.data(), d.nbox, box.size() * 4, hipMemcpyDeviceToHost, st_));
-    DB_TRY(hipMemcpyAsync(beg.data(), d.nbeg, total * 4, hipMemcpyDeviceToHost, st_));
-    DB_TRY(hipMemcpyAsync(end.data(), d.nend, total * 4, hipMemcpyDeviceToHost, st_));
-    DB_TRY(hipMemcpyAsync(child.data(), d.nchild, total * 4, hipMemcpyDeviceToHost, st_));
-    DB_TRY(hipMemcpyAsync(flags.data(), d.nflags, total * 4, hipMemcpyDeviceToHost, st_));
-    DB_TRY(hipMemcpyAsync(depth.data(), d.ndepth, total * 4, hipMemcpyDeviceToHost, st_));
-    DB_TRY(hipMemcpyAsync(order.data(), d.order, n * 4, hipMemcpyDeviceToHost, st_));
+    hipLaunchKernelGGL(k_clear_keys, dim3(grid_of(n + 1)), dim3(kB), 0, st_, d);
+    hipLaunchKernelGGL(k_leaf_marks, dim3(grid_of(total)), dim3(kB), 0, st_, d, total);
+    const u32 sb = grid_of(n + 1, kScanTile);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(sb), dim3(kB), 0, st_, d);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st_, d, sb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(sb), dim3(kB), 0, st_, d);
+    // leaves per geometry -> node counts and bases
+    std::vector<uint4> at(ng + 1);
+    for (u32 g = 0; g <= ng; g++)
+      DB_TRY(hipMemcpyAsync(&at[g], d.scan + off_h_[g], sizeof(uint4), hipMemcpyDeviceToHost, st_));
     DB_TRY(hipStreamSynchronize(st_));
-    std::vector<u32> size(total, 1u);
-    for (u32 i = total; i-- > 0;)
-      if (flags[i] & 1u) size[i] = 1u + size[child[i]] + size[child[i] + 1];
-    std::vector<u32> stack;
+    std::vector<u32> gbase(ng + 1, 0u);
+    for (u32 g = 0; g < ng; g++) {
+      const u32 leaves = at[g + 1].x - at[g].x;
+      gbase[g + 1] = gbase[g] + (leaves ? 2u * leaves - 1u : 1u);
+    }
+    if (gbase[ng] != total) return hipErrorIllegalState;  // the node count disagrees with the leaves
+    DB_TRY(gbase_.reserve((ng + 1) * 4));
+    DB_TRY(emit_.reserve((size_t)total * sizeof(TreeNode)));
+    DB_TRY(hipMemcpyAsync(gbase_.p, gbase.data(), (ng + 1) * 4, hipMemcpyHostToDevice, st_));
+    hipLaunchKernelGGL(k_emit, dim3(grid_of(total)), dim3(kB), 0, st_, d, total, (const u32 *)off_.p,
+                       (const u32 *)gbase_.p, (TreeNode *)emit_.p);
+    std::vector<TreeNode> nodes(total);
+    std::vector<u32> order(n), depth(total);
+    DB_TRY(hipMemcpyAsync(nodes.data(), emit_.p, (size_t)total * sizeof(TreeNode), hipMemcpyDeviceToHost, st_));
+    DB_TRY(hipMemcpyAsync(order.data(), d.order, n * 4, hipMemcpyDeviceToHost, st_));
+    DB_TRY(hipMemcpyAsync(depth.data(), d.ndepth, (size_t)total * 4, hipMemcpyDeviceToHost, st_));
+    std::vector<u32> ngeo(total);
+    DB_TRY(hipMemcpyAsync(ngeo.data(), d.ngeo, (size_t)total * 4, hipMemcpyDeviceToHost, st_));
+    u32 errs = 0;
+    DB_TRY(hipMemcpyAsync(&errs, d.count + 2, 4, hipMemcpyDeviceToHost, st_));
+    DB_TRY(hipStreamSynchronize(st_));
+    if (errs) return hipErrorIllegalState;
     for (u32 g = 0; g < ng; g++) {
       Tree &t = out[g];
-      const u32 base = off_h_[g];
-      t.order.assign(order.begin() + base, order.begin() + off_h_[g + 1]);
-      for (u32 &p : t.order) p -= base;
-      t.nodes.clear();
-      t.nodes.reserve(size[g]);
+      t.nodes.assign(nodes.begin() + gbase[g], nodes.begin() + gbase[g + 1]);
+      t.order.assign(order.begin() + off_h_[g], order.begin() + off_h_[g + 1]);
+      for (u32 &p : t.order) p -= off_h_[g];
       t.depth = 0;
-      stack.assign(1, g);
-      while (!stack.empty()) {
-        const u32 i = stack.back();
-        stack.pop_back();
-        TreeNode tn;
-        for (int k = 0; k < 3; k++) {
-          tn.box.mn[k] = box[6 * (size_t)i + k];
-          tn.box.mx[k] = box[6 * (size_t)i + 3 + k];
-        }
-        tn.begin = beg[i] - base;
-        tn.end = end[i] - base;
-        tn.size = size[i];
-        tn.inner = (flags[i] & 1u) != 0;
-        t.nodes.push_back(tn);
-        t.depth = std::max(t.depth, depth[i]);
-        if (tn.inner) {
-          stack.push_back(child[i] + 1);
-          stack.push_back(child[i]);
-        }
-      }
     }
+    for (u32 i = 0; i < total; i++) out[ngeo[i]].depth = std::max(out[ngeo[i]].depth, depth[i]);
     return hipSuccess;
   }
 };
