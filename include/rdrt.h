@@ -179,6 +179,10 @@ int32_t rdrt_blas_begin(rdrt_ctx *ctx);
 /* One geometry of the pending BLAS: positions are xyz f32 triples (copied);
  * indices_or_null = NULL means a non-indexed triangle list.  A rejected
  * geometry aborts the pending BLAS (rdrt_blas_end then fails). */
+/* As rdrt_blas_add_triangles with 16-bit indices (the bindless mesh's u16
+ * index format, bindless_mesh_bridge.rs:11-41). */
+int32_t rdrt_blas_add_triangles_u16(rdrt_ctx *ctx, const float *pos_xyz, uint32_t nverts, const uint16_t *idx,
+                                    uint32_t ntris, uint32_t flags);
 int32_t rdrt_blas_add_triangles(rdrt_ctx *ctx, const float *pos_xyz, uint32_t nverts,
                                 const uint32_t *indices_or_null, uint32_t ntris,
                                 uint32_t geometry_flags);
@@ -222,8 +226,27 @@ int32_t rdrt_get_scene_info(const rdrt_ctx *ctx, rdrt_scene_info *out);
  * 4 = BLAS child quads (128 B nodes), 5 = their quantized twins (64 B). */
 int32_t rdrt_copy_built(const rdrt_ctx *ctx, uint32_t which, void *host_out, size_t bytes);
 
+/* Shading attributes of one closest hit (48 B): the bindless mesh fetch
+ * of the reference's closest-hit stage (bindless_mesh_bridge.rs:43-123) and
+ * its shader-binding-table hit group (shader/ray-tracing/src/api/ctx.rs:53-55). */
+typedef struct rdrt_hit_attributes {
+  float position[3];         /* world hit point: origin + t * direction */
+  float shading_normal[3];   /* barycentric vertex normal through the normal matrix, normalized, on the
+                                geometric normal's side (the geometric normal without vertex normals) */
+  float geometric_normal[3]; /* (a - b) x (a - c) through the normal matrix, normalized, facing the ray origin */
+  float uv[2];               /* barycentric texture coordinates (0, 0 without them) */
+  uint32_t sbt_record;       /* sbt_offset + sbt_stride * geometry + instance sbt offset; RDRT_MISS on a miss */
+} rdrt_hit_attributes;
+
 /* ---- tracing ---- */
 int32_t rdrt_trace(rdrt_ctx *ctx, const rdrt_trace_args *args);
+/* Attributes of n closest hits traced before (device arrays: the rays, their
+ * metadata or NULL for tlas_idx, the hits and hit metadata rdrt_trace
+ * wrote); misses give zeros and sbt_record RDRT_MISS.  Enqueued on the
+ * context stream. */
+int32_t rdrt_fetch_hit_attributes(rdrt_ctx *ctx, const rdrt_ray *rays, const rdrt_ray_meta *ray_meta,
+                                  const rdrt_hit *hits, const rdrt_hit_meta *hit_meta, uint32_t n, uint32_t tlas_idx,
+                                  uint32_t sbt_offset, uint32_t sbt_stride, rdrt_hit_attributes *out);
 /* Closest hit with flags 0, cull mask ~0, TLAS binding 0, the context's
  * default traversal. */
 int32_t rdrt_trace_closest(rdrt_ctx *ctx, const rdrt_ray *rays_dev, uint32_t n, rdrt_hit *hits_dev);
@@ -278,6 +301,10 @@ int32_t rdrt_blas_set_vertex_normals(rdrt_ctx *ctx, uint32_t blas, uint32_t geom
 /* Per-triangle material offsets of one geometry (copied).  A hit's material
  * is materials[instance_custom_index + offset], as the reference looks the
  * surface up by the instance's custom id (feature/path_tracing/ray_hit.rs:65-75). */
+/* Per-vertex texture coordinates of a geometry (uv pairs), the uv channel
+ * of the bindless mesh (scene/rendering/gpu-ray-tracing/src/
+ * bindless_mesh_bridge.rs:65-77). */
+int32_t rdrt_blas_set_vertex_uvs(rdrt_ctx *ctx, uint32_t blas, uint32_t geometry, const float *uvs, uint32_t nverts);
 int32_t rdrt_blas_set_triangle_materials(rdrt_ctx *ctx, uint32_t blas, uint32_t geometry,
                                          const uint32_t *offsets, uint32_t ntris);
 int32_t rdrt_set_materials(rdrt_ctx *ctx, const rdrt_material *materials, uint32_t n);

@@ -143,6 +143,10 @@ SIGNATURES = {
     "rdrt_batch_autotune": (i32, [C.c_void_p, C.POINTER(RdrtBatchParams), u32, C.POINTER(RdrtTune)]),
     "rdrt_refit": (i32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "rdrt_set_device_build": (i32, [C.c_void_p, u32]),
+    "rdrt_fetch_hit_attributes": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, u32, u32, u32, u32,
+                                       C.c_void_p]),
+    "rdrt_blas_set_vertex_uvs": (i32, [C.c_void_p, u32, u32, C.c_void_p, u32]),
+    "rdrt_blas_add_triangles_u16": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p, u32, u32]),
     "rdrt_autotune": (i32, [C.c_void_p, C.POINTER(RdrtTraceArgs), u32, C.POINTER(RdrtTune)]),
     "rdrt_set_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),
     "rdrt_get_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),

@@ -72,7 +72,7 @@ class TriangleGeometry:
     """``BottomLevelAccelerationStructureBuildBuffer::Triangles`` plus its flags."""
 
     positions: np.ndarray  # (n, 3) float32
-    indices: Optional[np.ndarray] = None  # (m, 3) or (3m,) uint32; None = non-indexed
+    indices: Optional[np.ndarray] = None  # (m, 3) or (3m,) uint32 or uint16; None = non-indexed
     flags: int = GEOMETRY_FLAG_OPAQUE
 
 
@@ -183,6 +183,11 @@ class AccelerationStructureSystem:
             if s.indices is None:
                 L.check(self._L.rdrt_blas_add_triangles(self._h, pos.ctypes.data, pos.shape[0], None,
                                                          pos.shape[0] // 3, s.flags))
+            elif np.asarray(s.indices).dtype == np.uint16:  # the 16-bit index format
+                idx = np.ascontiguousarray(s.indices, dtype=np.uint16).reshape(-1)
+                keep.append(idx)
+                L.check(self._L.rdrt_blas_add_triangles_u16(self._h, pos.ctypes.data, pos.shape[0], idx.ctypes.data,
+                                                             idx.size // 3, s.flags))
             else:
                 idx = np.ascontiguousarray(s.indices, dtype=np.uint32).reshape(-1)
                 keep.append(idx)
@@ -258,6 +263,26 @@ class AccelerationStructureSystem:
     def set_vertex_normals(self, blas: BlasHandle, geometry: int, normals: np.ndarray) -> None:
         n = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
         L.check(self._L.rdrt_blas_set_vertex_normals(self._h, blas.index, geometry, n.ctypes.data, n.shape[0]))
+
+    def set_vertex_uvs(self, blas: BlasHandle, geometry: int, uvs: np.ndarray) -> None:
+        """Per-vertex texture coordinates (nverts, 2) of one geometry."""
+        u = np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1)
+        L.check(self._L.rdrt_blas_set_vertex_uvs(self._h, blas.index, geometry, u.ctypes.data, u.size // 2))
+
+    def hit_attributes(self, rays, hits, hit_meta, out, *, ray_meta=None, tlas_idx: int = 0, sbt_offset: int = 0,
+                       sbt_stride: int = 1) -> None:
+        """Shading attributes of traced closest hits into ``out`` (device
+        tensor of n * 12 32-bit words: position, shading normal, geometric
+        normal, uv, hit-group record)."""
+        n = rays.numel() // 8
+        _check_dev(rays, n * 8, "rays")
+        _check_dev(hits, n * 4, "hits")
+        _check_dev(hit_meta, n * 4, "hit_meta")
+        _check_dev(out, n * 12, "out")
+        if ray_meta is not None:
+            _check_dev(ray_meta, n * 4, "ray_meta")
+        L.check(self._L.rdrt_fetch_hit_attributes(self._h, _ptr(rays), _ptr(ray_meta), _ptr(hits), _ptr(hit_meta), n,
+                                                  tlas_idx, sbt_offset, sbt_stride, _ptr(out)))
 
     def set_triangle_materials(self, blas: BlasHandle, geometry: int, offsets: np.ndarray) -> None:
         o = np.ascontiguousarray(offsets, dtype=np.uint32).reshape(-1)

@@ -26,7 +26,8 @@ HOST_SOURCES = ["bvh_build.cpp", "scene.cpp"]
 HIP_SOURCES = ["trace_fast_256.hip", "trace_fast_128.hip", "trace_fast_64.hip", "trace_quad_256.hip",
                "trace_quad_128.hip", "trace_quad_64.hip", "trace_qquad_256.hip", "trace_qquad_128.hip",
                "trace_qquad_64.hip", "trace_kernels.hip", "pt_kernels.hip", "batch_kernels.hip", "api.hip",
-               "api_pt.hip", "api_batch.hip", "refit_kernels.hip", "device_build.hip"]
+               "api_pt.hip", "api_batch.hip", "refit_kernels.hip", "device_build.hip",
+               "attrib_kernels.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 HIP_FLAGS = [

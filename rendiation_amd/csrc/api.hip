@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "context.h"
+#include "attrib.h"
 #include "device_build.h"
 #include "refit.h"
 
@@ -47,6 +48,9 @@ void free_device_scene(DeviceScene &d) {
   release(d.tri_qquads);
   release(d.tlas_qquads);
   release(d.tlas_quad_roots);
+  release(d.inst_lookup);
+  release(d.tlas_inst_first);
+  release(d.uvs);
   release(d.tris);
   release(d.redirect);
   release(d.tri_ids);
@@ -271,6 +275,14 @@ int32_t rdrt_blas_add_triangles(rdrt_ctx *c, const float *pos, uint32_t nverts, 
   return RDRT_OK;
 }
 
+int32_t rdrt_blas_add_triangles_u16(rdrt_ctx *c, const float *pos, uint32_t nverts, const uint16_t *idx,
+                                    uint32_t ntris, uint32_t flags) {
+  if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
+  if (!idx) return rdrt_blas_add_triangles(c, pos, nverts, nullptr, ntris, flags);
+  std::vector<uint32_t> wide(idx, idx + 3 * (size_t)ntris);
+  return rdrt_blas_add_triangles(c, pos, nverts, wide.data(), ntris, flags);
+}
+
 int32_t rdrt_blas_end(rdrt_ctx *c, uint32_t *out) {
   if (!c || !out) return fail(RDRT_ERR_INVALID_ARG, "null argument");
   if (!c->in_blas) return fail(RDRT_ERR_STATE, "rdrt_blas_end without rdrt_blas_begin");
@@ -379,6 +391,9 @@ int32_t rdrt_commit(rdrt_ctx *c) {
   RDRT_HIP(upload(d.tri_forest, b.tri_forest, d.bytes));
   RDRT_HIP(upload(d.tlas_forest, b.tlas_forest, d.bytes));
   RDRT_HIP(upload(d.normals, b.normals, d.bytes));
+  RDRT_HIP(upload(d.uvs, b.uvs, d.bytes));
+  RDRT_HIP(upload(d.inst_lookup, b.inst_lookup, d.bytes));
+  RDRT_HIP(upload(d.tlas_inst_first, b.tlas_inst_first, d.bytes));
   RDRT_HIP(upload(d.tri_material, b.tri_material, d.bytes));
   c->uploaded = true;
   return RDRT_OK;
@@ -454,6 +469,8 @@ int32_t rdrt_refit(rdrt_ctx *c, const float *positions, const float *normals) {
   RDRT_HIP(reupload(d.tlas_qquads, b.tlas_qquads, d.bytes));
   RDRT_HIP(reupload(d.tlas_roots, b.tlas_roots, d.bytes));
   RDRT_HIP(reupload(d.tlas_quad_roots, b.tlas_quad_roots, d.bytes));
+  RDRT_HIP(reupload(d.inst_lookup, b.inst_lookup, d.bytes));
+  RDRT_HIP(reupload(d.tlas_inst_first, b.tlas_inst_first, d.bytes));
   return RDRT_OK;
 }
 
@@ -581,6 +598,41 @@ int32_t rdrt_trace(rdrt_ctx *c, const rdrt_trace_args *a) {
   else RDRT_HIP(launch_trace(p, a->traversal, push, c->tune, c->stream));
   RDRT_HIP(hipEventRecord(c->ev1, c->stream));
   c->timed = true;
+  return RDRT_OK;
+}
+
+int32_t rdrt_fetch_hit_attributes(rdrt_ctx *c, const rdrt_ray *rays, const rdrt_ray_meta *ray_meta,
+                                  const rdrt_hit *hits, const rdrt_hit_meta *hit_meta, uint32_t n, uint32_t tlas_idx,
+                                  uint32_t sbt_offset, uint32_t sbt_stride, rdrt_hit_attributes *out) {
+  if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
+  if (n && (!rays || !hits || !hit_meta || !out)) return fail(RDRT_ERR_INVALID_ARG, "null rays, hits, metadata or output");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  if (!c->uploaded) return fail(RDRT_ERR_NOT_BUILT, "scene is not committed (call rdrt_commit)");
+  const DeviceScene &d = c->dev;
+  AttribArgs a{};
+  a.rays = rays;
+  a.meta = ray_meta;
+  a.hits = hits;
+  a.hit_meta = hit_meta;
+  a.n = n;
+  a.tlas_idx = tlas_idx;
+  a.sbt_offset = sbt_offset;
+  a.sbt_stride = sbt_stride;
+  a.out = out;
+  a.binding = d.binding.ptr;
+  a.nbinding = (uint32_t)d.binding.count;
+  a.ntlas = (uint32_t)d.tlas_inst_first.count;
+  a.ninst = (uint32_t)d.inst_lookup.count;
+  a.tlas_inst_first = d.tlas_inst_first.ptr;
+  a.inst_lookup = d.inst_lookup.ptr;
+  a.insts = d.insts.ptr;
+  a.geos = d.geos.ptr;
+  a.indices = d.indices.ptr;
+  a.vertices = d.vertices.ptr;
+  a.normals = d.normals.count ? d.normals.ptr : nullptr;
+  a.uvs = d.uvs.count ? d.uvs.ptr : nullptr;
+  RDRT_HIP(enqueue_hit_attributes(a, c->stream));
   return RDRT_OK;
 }
 

@@ -102,6 +102,17 @@ int32_t rdrt_blas_set_vertex_normals(rdrt_ctx *c, uint32_t blas, uint32_t geomet
   return RDRT_OK;
 }
 
+int32_t rdrt_blas_set_vertex_uvs(rdrt_ctx *c, uint32_t blas, uint32_t geometry, const float *uvs, uint32_t nverts) {
+  if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
+  GeometrySrc *g = geometry_of(c, blas, geometry);
+  if (!g) return fail(RDRT_ERR_INVALID_ARG, "unknown BLAS or geometry");
+  if (nverts != g->pos.size() / 3 || (nverts && !uvs))
+    return fail(RDRT_ERR_INVALID_ARG, "uvs must give one pair per vertex");
+  g->uvs.assign(uvs, uvs + 2 * (size_t)nverts);
+  invalidate(c);
+  return RDRT_OK;
+}
+
 int32_t rdrt_blas_set_triangle_materials(rdrt_ctx *c, uint32_t blas, uint32_t geometry, const uint32_t *offsets,
                                          uint32_t ntris) {
   if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");

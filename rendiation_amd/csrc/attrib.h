@@ -1,0 +1,33 @@
+// attrib.h -- launch interface of the hit-attribute kernel.
+#ifndef RDRT_ATTRIB_H
+#define RDRT_ATTRIB_H
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/rdrt.h"
+#include "rdrt_layout.h"
+
+namespace rdrt {
+
+struct AttribArgs {
+  const rdrt_ray *rays;
+  const rdrt_ray_meta *meta;  // or null: tlas_idx
+  const rdrt_hit *hits;
+  const rdrt_hit_meta *hit_meta;
+  uint32_t n, tlas_idx, sbt_offset, sbt_stride;
+  rdrt_hit_attributes *out;
+  // scene
+  const uint32_t *binding;
+  uint32_t nbinding, ntlas, ninst;
+  const uint32_t *tlas_inst_first, *inst_lookup;
+  const InstRec *insts;
+  const GeoRec *geos;
+  const uint32_t *indices;
+  const float *vertices, *normals, *uvs;  // normals / uvs null when the scene has none
+};
+
+hipError_t enqueue_hit_attributes(const AttribArgs &a, hipStream_t stream);
+
+}  // namespace rdrt
+
+#endif

@@ -71,8 +71,8 @@ struct DeviceScene {
   DevArray<QuadNode> tri_quads, tlas_quads;
   DevArray<QuadNodeQ> tri_qquads, tlas_qquads;
   DevArray<TriPre> tris;
-  DevArray<uint32_t> redirect, tri_ids, indices, binding, tri_material, tlas_quad_roots;
-  DevArray<float> vertices, normals;
+  DevArray<uint32_t> redirect, tri_ids, indices, binding, tri_material, tlas_quad_roots, inst_lookup, tlas_inst_first;
+  DevArray<float> vertices, normals, uvs;
   DevArray<GeoRec> geos;
   DevArray<InstRec> insts;
   DevArray<TlasRoot> tlas_roots;

@@ -551,6 +551,8 @@ bool validate_refs(const BuiltScene &s, std::string &err) {
 // refills those arrays, so a refit of the BLAS boxes can rebuild it.
 int build_tlas_layer(const SceneSrc &src, BuiltScene &out, std::string &err) {
   out.insts.clear();
+  out.inst_lookup.clear();
+  out.tlas_inst_first.clear();
   out.tlas_forest.clear();
   out.tlas_pairs.clear();
   out.tlas_quads.clear();
@@ -563,6 +565,7 @@ int build_tlas_layer(const SceneSrc &src, BuiltScene &out, std::string &err) {
   for (uint32_t ti = 0; ti < src.tlas.size(); ti++) {
     TlasRoot root;
     std::memset(&root, 0, sizeof(root));
+    out.tlas_inst_first.push_back((uint32_t)out.insts.size());
     if (!src.tlas_live[ti]) {
       root.ref_root = kInvalid;
       root.root_ref = kInvalid;
@@ -582,6 +585,8 @@ int build_tlas_layer(const SceneSrc &src, BuiltScene &out, std::string &err) {
     Tree tree = build_tree(boxes, src.tlas_opt, 1);
     out.max_tlas_depth = std::max(out.max_tlas_depth, tree.depth);
     uint32_t inst_base = (uint32_t)out.insts.size();
+    out.inst_lookup.resize(inst_base + inst.size());
+    for (uint32_t j = 0; j < tree.order.size(); j++) out.inst_lookup[inst_base + tree.order[j]] = inst_base + j;
     for (uint32_t k : tree.order) {
       const rdrt_instance &in = inst[k];
       M4 m = load_m4(in.transform);
@@ -782,11 +787,12 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
   std::vector<Box> blas_box(src.blas.size(), box_empty());
   std::vector<char> blas_has_box(src.blas.size(), 0);
   std::vector<uint32_t> blas_geo_begin(src.blas.size(), 0), blas_geo_end(src.blas.size(), 0);
-  bool any_normals = false, any_material = false;
+  bool any_normals = false, any_material = false, any_uvs = false;
   for (uint32_t b = 0; b < src.blas.size(); b++)
     if (src.blas_live[b])
       for (const GeometrySrc &g : src.blas[b]) {
         any_normals |= !g.normals.empty();
+        any_uvs |= !g.uvs.empty();
         any_material |= !g.tri_material.empty();
       }
   // Per geometry, in parallel: forest, child pairs and child quads with
@@ -838,6 +844,7 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
     out.indices.reserve(3 * nt);
     out.vertices.reserve(3 * nv);
     if (any_normals) out.normals.reserve(3 * nv);
+    if (any_uvs) out.uvs.reserve(2 * nv);
     if (any_material) out.tri_material.reserve(nt);
   }
   {
@@ -861,6 +868,11 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
           size_t nv = g.pos.size() / 3;
           if (g.normals.size() == 3 * nv) out.normals.insert(out.normals.end(), g.normals.begin(), g.normals.end());
           else out.normals.resize(out.normals.size() + 3 * nv, 0.0f);  // zero = use the geometric normal
+        }
+        if (any_uvs) {
+          size_t nv = g.pos.size() / 3;
+          if (g.uvs.size() == 2 * nv) out.uvs.insert(out.uvs.end(), g.uvs.begin(), g.uvs.end());
+          else out.uvs.resize(out.uvs.size() + 2 * nv, 0.0f);
         }
         uint32_t max_mat = 0;
         if (any_material) {

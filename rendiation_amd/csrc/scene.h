@@ -24,6 +24,7 @@ struct GeometrySrc {
   // and per-triangle material offsets (empty = 0)
   std::vector<float> normals;
   std::vector<uint32_t> tri_material;
+  std::vector<float> uvs;  // texture coordinates (uv per vertex, empty = none)
 };
 
 struct SceneSrc {
@@ -48,6 +49,10 @@ struct BuiltScene {
   std::vector<float> normals;
   std::vector<uint32_t> tri_material;
   std::vector<uint32_t> geo_max_material;
+  std::vector<float> uvs;  // 2 per vertex when any geometry has them (zeros elsewhere), else empty
+  // instance records by source position: inst_lookup[tlas_inst_first[t] + k]
+  // is the record of instance k of TLAS t
+  std::vector<uint32_t> inst_lookup, tlas_inst_first;
   // shared records
   std::vector<GeoRec> geos;
   std::vector<InstRec> insts;
