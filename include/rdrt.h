@@ -238,6 +238,26 @@ typedef struct rdrt_hit_attributes {
   uint32_t sbt_record;       /* sbt_offset + sbt_stride * geometry + instance sbt offset; RDRT_MISS on a miss */
 } rdrt_hit_attributes;
 
+/* ---- host picking (content/mesh/core/src/feature/bvh.rs:57-86) ---- */
+#define RDRT_FACE_FRONT 0u
+#define RDRT_FACE_BACK 1u
+#define RDRT_FACE_DOUBLE 2u
+typedef struct rdrt_pick_hit {
+  uint32_t primitive; /* triangle of the geometry, RDRT_MISS when nothing is hit */
+  float distance;     /* along the direction (normalize it for a metric distance) */
+  float position[3];  /* origin + direction * distance */
+  uint32_t pad;
+} rdrt_pick_hit;
+/* Nearest hits of n rays (host array: origin xyz, direction xyz per ray)
+ * against one geometry of a BLAS, on the host through the geometry's
+ * reference tree, with the reference's host tests: the GTE ray/triangle test
+ * with a face side (math/geometry/src/dimension3/intersection.rs:3-76), the
+ * ray/box test of the same file (:128-196), no range limit beyond t >= 0,
+ * the first of equally near hits kept.  Needs a built scene (rdrt_build; a
+ * host-only context is enough); uses the committed positions. */
+int32_t rdrt_pick(rdrt_ctx *ctx, uint32_t blas, uint32_t geometry, const float *rays, uint32_t n, uint32_t face_side,
+                  rdrt_pick_hit *out);
+
 /* ---- tracing ---- */
 int32_t rdrt_trace(rdrt_ctx *ctx, const rdrt_trace_args *args);
 /* Attributes of n closest hits traced before (device arrays: the rays, their

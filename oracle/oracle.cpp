@@ -1777,3 +1777,163 @@ int orc_pt_replay_geometry(const orc_scene *s, const orc_pt_params *P, const dou
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ host picking --
+// content/mesh/core/src/feature/bvh.rs:57-86 intersect_nearest_bvh over the
+// math/geometry host tests.
+namespace {
+
+// math/geometry/src/dimension3/intersection.rs:3-76 (GTE IntrRay3Triangle3);
+// the flipped triangle of triangle.rs:76-82 for FaceSide::Back.
+float pick_tri(V3 o, V3 d, V3 a, V3 b, V3 c, uint32_t side) {
+  const bool back = side == 1u, culling = side != 2u;
+  if (back) {
+    V3 t = a;
+    a = c;
+    c = t;
+  }
+  const V3 e1 = b - a;
+  const V3 e2 = c - a;
+  const V3 nrm = cross(e1, e2);
+  float DdN = dot(d, nrm);
+  float sign;
+  if (DdN > 0.0f) {
+    if (culling) return NAN;
+    sign = 1.0f;
+  } else if (DdN < 0.0f) {
+    sign = -1.0f;
+    DdN = -DdN;
+  } else {
+    return NAN;
+  }
+  const V3 q = o - a;
+  const float DdQxE2 = sign * dot(d, cross(q, e2));
+  if (DdQxE2 < 0.0f) return NAN;
+  const float DdE1xQ = sign * dot(d, cross(e1, q));
+  if (DdE1xQ < 0.0f) return NAN;
+  if (DdQxE2 + DdE1xQ > DdN) return NAN;
+  const float QdN = -sign * dot(q, nrm);
+  if (QdN < 0.0f) return NAN;
+  return QdN / DdN;
+}
+
+// intersection.rs:128-196 then is_some() (:203-208): one slab pair per axis
+// in x, y, z order with the NaN-aware narrowing.
+bool pick_slab(V3 o, V3 d, V3 mn, V3 mx) {
+  const float oo[3] = {o.x, o.y, o.z};
+  const float inv[3] = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+  const float lo[3] = {mn.x, mn.y, mn.z}, hi[3] = {mx.x, mx.y, mx.z};
+  float t_min = 0.0f, t_max = 0.0f;
+  for (int k = 0; k < 3; k++) {
+    float a, b;
+    if (inv[k] >= 0.0f) {
+      a = (lo[k] - oo[k]) * inv[k];
+      b = (hi[k] - oo[k]) * inv[k];
+    } else {
+      a = (hi[k] - oo[k]) * inv[k];
+      b = (lo[k] - oo[k]) * inv[k];
+    }
+    if (k == 0) {
+      t_min = a;
+      t_max = b;
+      continue;
+    }
+    if (t_min > b || a > t_max) return false;
+    if (a > t_min || std::isnan(t_min)) t_min = a;
+    if (b < t_max || std::isnan(t_max)) t_max = b;
+  }
+  return !(t_max < 0.0f);
+}
+
+struct PickState {
+  const orc_scene *s;
+  uint32_t prim_start;
+  V3 o, d;
+  uint32_t side;
+  bool found;
+  uint32_t prim;
+  float dist;
+};
+
+void pick_leaf_prim(PickState &st, uint32_t tri) {
+  const orc_scene *s = st.s;
+  const float t = pick_tri(st.o, st.d, s->vertices[s->indices[3 * (size_t)tri]],
+                           s->vertices[s->indices[3 * (size_t)tri + 1]], s->vertices[s->indices[3 * (size_t)tri + 2]],
+                           st.side);
+  if (std::isnan(t)) return;
+  if (st.found && !(t < st.dist)) return;  // is_near_than: distance < (intersect_util.rs:15-17)
+  st.found = true;
+  st.dist = t;
+  st.prim = tri - st.prim_start;
+}
+
+// Depth-first, right subtree before left: the order in which the reference's
+// stack (push left, push right, pop) visits the nodes.
+void pick_node(PickState &st, uint32_t i) {
+  const orc_node &nd = st.s->tri_forest[i];
+  const bool leaf = nd.hit_next == nd.miss_next;
+  if (leaf) {
+    for (uint32_t k = nd.content_range[0]; k < nd.content_range[1]; k++) pick_leaf_prim(st, st.s->redirect[k]);
+    return;
+  }
+  if (!pick_slab(st.o, st.d, v3(nd.aabb_min[0], nd.aabb_min[1], nd.aabb_min[2]),
+                 v3(nd.aabb_max[0], nd.aabb_max[1], nd.aabb_max[2])))
+    return;
+  const uint32_t left = nd.hit_next;
+  const uint32_t right = st.s->tri_forest[left].miss_next;
+  pick_node(st, right);
+  pick_node(st, left);
+}
+
+}  // namespace
+
+extern "C" {
+
+float orc_pick_triangle(const float o[3], const float d[3], const float a[3], const float b[3], const float c[3],
+                        uint32_t side) {
+  return pick_tri(v3(o[0], o[1], o[2]), v3(d[0], d[1], d[2]), v3(a[0], a[1], a[2]), v3(b[0], b[1], b[2]),
+                  v3(c[0], c[1], c[2]), side);
+}
+
+int orc_pick_box(const float o[3], const float d[3], const float bmin[3], const float bmax[3]) {
+  return pick_slab(v3(o[0], o[1], o[2]), v3(d[0], d[1], d[2]), v3(bmin[0], bmin[1], bmin[2]),
+                   v3(bmax[0], bmax[1], bmax[2]))
+             ? 1
+             : 0;
+}
+
+int orc_pick(const orc_scene *s, uint32_t blas, uint32_t geometry, const float *rays, uint32_t n, uint32_t side,
+             int bvh, orc_pick_hit *out) {
+  if (!s->built || blas >= s->blas_meta.size() || side > 2u) return -1;
+  const auto bm = s->blas_meta[blas];
+  if (geometry >= bm.second - bm.first) return -1;
+  const auto &geo = s->geo_meta[bm.first + geometry];
+  // the geometry's triangles in index order: its leaves cover one
+  // contiguous range of the redirect array starting at its root
+  const orc_node &root = s->tri_forest[geo.root];
+  std::vector<uint32_t> tris(s->redirect.begin() + root.content_range[0], s->redirect.begin() + root.content_range[1]);
+  std::sort(tris.begin(), tris.end());
+  for (uint32_t r = 0; r < n; r++) {
+    const float *q = rays + 6 * (size_t)r;
+    PickState st{s, geo.primitive_start, v3(q[0], q[1], q[2]), v3(q[3], q[4], q[5]), side, false, 0u, 0.0f};
+    if (bvh) {
+      pick_node(st, geo.root);
+    } else {
+      for (uint32_t t : tris) pick_leaf_prim(st, t);
+    }
+    orc_pick_hit h;
+    std::memset(&h, 0, sizeof(h));
+    h.prim = st.found ? st.prim : 0xffffffffu;
+    if (st.found) {
+      h.distance = st.dist;
+      const V3 p = st.o + st.d * st.dist;  // hyper_ray.rs:11-13
+      h.position[0] = p.x;
+      h.position[1] = p.y;
+      h.position[2] = p.z;
+    }
+    out[r] = h;
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -138,6 +138,28 @@ void orc_intersect_triangle(const float o[3], const float d[3], float near_,
 int orc_intersect_aabb(const float o[3], const float d[3], float near_,
                        float far_, const float bmin[3], const float bmax[3]);
 
+/* Host picking (content/mesh/core/src/feature/bvh.rs:57-86): the GTE
+ * ray/triangle test with a face side, 0 = Front, 1 = Back, 2 = Double
+ * (math/geometry/src/dimension3/intersection.rs:3-76); returns the distance,
+ * NaN for no hit. */
+float orc_pick_triangle(const float o[3], const float d[3], const float a[3], const float b[3], const float c[3],
+                        uint32_t side);
+/* The host ray/box test as a bool (intersection.rs:128-208). */
+int orc_pick_box(const float o[3], const float d[3], const float bmin[3], const float bmax[3]);
+typedef struct orc_pick_hit {
+  uint32_t prim; /* ~0u on miss */
+  float distance;
+  float position[3];
+  uint32_t pad;
+} orc_pick_hit; /* 24 B */
+/* Nearest hits of n rays (6 floats: origin, direction) against geometry
+ * `geometry` of BLAS `blas` of a built scene.  bvh = 1 walks the geometry's
+ * forest in the reference order (bvh/mod.rs:94-114: right child first, leaf
+ * without box test); bvh = 0 tests every triangle in index order.  The first
+ * of equally near hits is kept (intersect_util.rs:52-61). */
+int orc_pick(const orc_scene *s, uint32_t blas, uint32_t geometry, const float *rays, uint32_t n, uint32_t side,
+             int bvh, orc_pick_hit *out);
+
 /* Sobol dimension 0 / 1 (sobol.rs:40-68). */
 float orc_van_der_corput(uint32_t n, uint32_t scramble);
 float orc_sobol(uint32_t n, uint32_t scramble);

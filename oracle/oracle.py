@@ -67,6 +67,9 @@ _SIG = {
     "orc_trace_brute": (C.c_int, [P, P, u32, u32, u32, u32, P, C.c_int]),
     "orc_intersect_triangle": (None, [P, P, f32, f32, P, P, P, C.c_int, C.c_int, P]),
     "orc_intersect_aabb": (C.c_int, [P, P, f32, f32, P, P]),
+    "orc_pick_triangle": (f32, [P, P, P, P, P, u32]),
+    "orc_pick_box": (C.c_int, [P, P, P, P]),
+    "orc_pick": (C.c_int, [P, u32, u32, P, u32, u32, C.c_int, P]),
     "orc_van_der_corput": (f32, [u32, u32]),
     "orc_sobol": (f32, [u32, u32]),
     "orc_compact": (u32, [P, P, u32, P]),
@@ -324,6 +327,16 @@ class OracleScene:
             raise RuntimeError("oracle pt_replay_geometry failed")
         return float(out.value)
 
+    def pick(self, blas: int, geometry: int, rays: np.ndarray, side: int = 2, bvh: bool = True):
+        """Host picking: (prim uint32, distance f32, position (n, 3) f32)."""
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        out = np.zeros((max(1, r.shape[0]), 6), dtype=np.uint32)
+        rc = self.L.orc_pick(self.h, blas, geometry, r.ctypes.data, r.shape[0], side, int(bvh), out.ctypes.data)
+        if rc != 0:
+            raise RuntimeError("oracle pick failed")
+        out = out[:r.shape[0]]
+        return out[:, 0].copy(), out[:, 1].view(np.float32).copy(), out[:, 2:5].view(np.float32).copy()
+
     def trace_brute(self, rays: np.ndarray, flags=0, cull_mask=0xFFFFFFFF, tlas_idx=0, threads: int = 0):
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
         hits = np.zeros((rays.shape[0], 4), dtype=np.uint32)
@@ -348,6 +361,18 @@ def intersect_aabb(o, d, near, far, bmin, bmax) -> bool:
     a = [np.ascontiguousarray(x, dtype=np.float32) for x in (o, d, bmin, bmax)]
     return bool(L.orc_intersect_aabb(a[0].ctypes.data, a[1].ctypes.data, near, far, a[2].ctypes.data,
                                      a[3].ctypes.data))
+
+
+def pick_triangle(o, d, a, b, c, side=2) -> float:
+    """Host GTE ray/triangle test with a face side (0 Front, 1 Back, 2
+    Double); NaN when there is no hit."""
+    v = [np.ascontiguousarray(x, dtype=np.float32) for x in (o, d, a, b, c)]
+    return float(lib().orc_pick_triangle(*[x.ctypes.data for x in v], side))
+
+
+def pick_box(o, d, bmin, bmax) -> bool:
+    v = [np.ascontiguousarray(x, dtype=np.float32) for x in (o, d, bmin, bmax)]
+    return bool(lib().orc_pick_box(*[x.ctypes.data for x in v]))
 
 
 def van_der_corput(n: int, scramble: int) -> float:

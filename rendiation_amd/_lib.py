@@ -82,6 +82,7 @@ class RdrtPtStats(C.Structure):
 
 
 RDRT_PT_DIAG_TIME_TRACES = 0x1
+RDRT_FACE_FRONT, RDRT_FACE_BACK, RDRT_FACE_DOUBLE = 0, 1, 2
 RDRT_PT_DIAG_COUNT_FAST = 0x4
 RDRT_PT_DIAG_COUNT_VISITS = 0x2
 
@@ -147,6 +148,7 @@ SIGNATURES = {
                                        C.c_void_p]),
     "rdrt_blas_set_vertex_uvs": (i32, [C.c_void_p, u32, u32, C.c_void_p, u32]),
     "rdrt_blas_add_triangles_u16": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p, u32, u32]),
+    "rdrt_pick": (i32, [C.c_void_p, u32, u32, C.c_void_p, u32, u32, C.c_void_p]),
     "rdrt_autotune": (i32, [C.c_void_p, C.POINTER(RdrtTraceArgs), u32, C.POINTER(RdrtTune)]),
     "rdrt_set_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),
     "rdrt_get_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),

@@ -264,6 +264,17 @@ class AccelerationStructureSystem:
         n = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
         L.check(self._L.rdrt_blas_set_vertex_normals(self._h, blas.index, geometry, n.ctypes.data, n.shape[0]))
 
+    def pick(self, blas: BlasHandle, geometry: int, rays: np.ndarray, face_side: int = 2):
+        """Nearest hits of host rays (n, 6: origin, direction) against one
+        geometry on the host (the reference's picking tests).  Returns
+        (primitive uint32 (MISS when none), distance f32, position (n, 3))."""
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        out = np.zeros((max(1, r.shape[0]), 6), dtype=np.uint32)
+        L.check(self._L.rdrt_pick(self._h, blas.index, geometry, r.ctypes.data, r.shape[0], face_side,
+                                  out.ctypes.data))
+        out = out[:r.shape[0]]
+        return out[:, 0].copy(), out[:, 1].view(np.float32).copy(), out[:, 2:5].view(np.float32).copy()
+
     def set_vertex_uvs(self, blas: BlasHandle, geometry: int, uvs: np.ndarray) -> None:
         """Per-vertex texture coordinates (nverts, 2) of one geometry."""
         u = np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1)

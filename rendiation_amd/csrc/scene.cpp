@@ -965,3 +965,145 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
 }
 
 }  // namespace rdrt
+
+namespace rdrt {
+
+// ---- host picking: intersect_nearest_bvh (content/mesh/core/src/feature/
+// bvh.rs:57-86) with the host geometry tests of math/geometry/src/dimension3/
+// intersection.rs -- single-rounding FP32 in the reference operation order.
+namespace {
+
+struct P3 {
+  float x, y, z;
+};
+inline P3 psub(P3 a, P3 b) { return P3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline float pdot(P3 a, P3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline P3 pcross(P3 a, P3 b) { return P3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+// Ray3 x Box3 as a bool (intersection.rs:128-196 and its is_some() at :203-208).
+bool pick_box(P3 o, P3 d, const float *mn, const float *mx) {
+  float t_max, t_min, ty_min, ty_max, tz_min, tz_max;
+  const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
+  if (ix >= 0.0f) {
+    t_min = (mn[0] - o.x) * ix;
+    t_max = (mx[0] - o.x) * ix;
+  } else {
+    t_min = (mx[0] - o.x) * ix;
+    t_max = (mn[0] - o.x) * ix;
+  }
+  if (iy >= 0.0f) {
+    ty_min = (mn[1] - o.y) * iy;
+    ty_max = (mx[1] - o.y) * iy;
+  } else {
+    ty_min = (mx[1] - o.y) * iy;
+    ty_max = (mn[1] - o.y) * iy;
+  }
+  if ((t_min > ty_max) || (ty_min > t_max)) return false;
+  if (ty_min > t_min || std::isnan(t_min)) t_min = ty_min;
+  if (ty_max < t_max || std::isnan(t_max)) t_max = ty_max;
+  if (iz >= 0.0f) {
+    tz_min = (mn[2] - o.z) * iz;
+    tz_max = (mx[2] - o.z) * iz;
+  } else {
+    tz_min = (mx[2] - o.z) * iz;
+    tz_max = (mn[2] - o.z) * iz;
+  }
+  if ((t_min > tz_max) || (tz_min > t_max)) return false;
+  if (tz_min > t_min || std::isnan(t_min)) t_min = tz_min;
+  if (tz_max < t_max || std::isnan(t_max)) t_max = tz_max;
+  (void)t_min;
+  return !(t_max < 0.0f);
+}
+
+// Ray3 x Triangle with a face side (intersection.rs:3-76, from GTE's
+// IntrRay3Triangle3; Back takes the flipped triangle, triangle.rs:76-82).  Returns the distance, or NaN when there is no hit.
+float pick_triangle(P3 o, P3 d, P3 a, P3 b, P3 c, uint32_t side) {
+  if (side == RDRT_FACE_BACK) std::swap(a, c);  // triangle.rs:76-82 flip
+  const bool culling = side != RDRT_FACE_DOUBLE;
+  const P3 e1 = psub(b, a), e2 = psub(c, a), n = pcross(e1, e2);
+  float ddn = pdot(d, n), sign;
+  if (ddn > 0.0f) {
+    if (culling) return NAN;
+    sign = 1.0f;
+  } else if (ddn < 0.0f) {
+    sign = -1.0f;
+    ddn = -ddn;
+  } else {
+    return NAN;
+  }
+  const P3 diff = psub(o, a);
+  const float ddqxe2 = sign * pdot(d, pcross(diff, e2));
+  if (ddqxe2 < 0.0f) return NAN;
+  const float dde1xq = sign * pdot(d, pcross(e1, diff));
+  if (dde1xq < 0.0f) return NAN;
+  if (ddqxe2 + dde1xq > ddn) return NAN;
+  const float qdn = -sign * pdot(diff, n);
+  if (qdn < 0.0f) return NAN;
+  return qdn / ddn;
+}
+
+}  // namespace
+
+int pick_nearest(const BuiltScene &b, uint32_t geo, const float *rays, uint32_t n, uint32_t side, rdrt_pick_hit *out,
+                 std::string &err) {
+  if (geo >= b.geos.size()) {
+    err = "unknown geometry";
+    return RDRT_ERR_INVALID_ARG;
+  }
+  if (side > RDRT_FACE_DOUBLE) {
+    err = "face side is RDRT_FACE_FRONT, RDRT_FACE_BACK or RDRT_FACE_DOUBLE";
+    return RDRT_ERR_INVALID_ARG;
+  }
+  const GeoRec &g = b.geos[geo];
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  parallel_for((n + 255) / 256, hw, [&](size_t chunk) {
+    std::vector<uint32_t> stack;
+    const uint32_t r1 = std::min<uint32_t>(n, (uint32_t)(chunk + 1) * 256);
+    for (uint32_t r = (uint32_t)chunk * 256; r < r1; r++) {
+      const float *q = rays + 6 * (size_t)r;
+      const P3 o{q[0], q[1], q[2]}, d{q[3], q[4], q[5]};
+      rdrt_pick_hit h;
+      std::memset(&h, 0, sizeof(h));
+      h.primitive = RDRT_MISS;
+      bool found = false;
+      // traverse_branch_leaf_visitor (content/space/src/bvh/mod.rs:94-114 on
+      // utility/abstract-tree/src/lib.rs:33-51): a stack; a branch whose box
+      // the ray hits pushes its children left then right (mod.rs:38-45), so
+      // the right one is visited first; a leaf is visited without a box test
+      stack.assign(1, g.ref_root);
+      while (!stack.empty()) {
+        const uint32_t i = stack.back();
+        stack.pop_back();
+        const RefNode &nd = b.tri_forest[i];
+        if (nd.hit_next != nd.miss_next) {
+          if (pick_box(o, d, nd.mn, nd.mx)) {
+            stack.push_back(nd.hit_next);
+            stack.push_back(b.tri_forest[nd.hit_next].miss_next);
+          }
+          continue;
+        }
+        for (uint32_t s = nd.begin; s < nd.end; s++) {
+          const uint32_t tri = b.redirect[s];
+          const float *va = &b.vertices[3 * (size_t)b.indices[3 * (size_t)tri]];
+          const float *vb = &b.vertices[3 * (size_t)b.indices[3 * (size_t)tri + 1]];
+          const float *vc = &b.vertices[3 * (size_t)b.indices[3 * (size_t)tri + 2]];
+          const float t = pick_triangle(o, d, P3{va[0], va[1], va[2]}, P3{vb[0], vb[1], vb[2]},
+                                        P3{vc[0], vc[1], vc[2]}, side);
+          if (std::isnan(t)) continue;
+          if (!found || t < h.distance) {  // refresh: strictly nearer (intersect_util.rs:15-17,52-61)
+            found = true;
+            h.primitive = tri - g.primitive_start;
+            h.distance = t;
+            h.position[0] = o.x + d.x * t;  // hyper_ray.rs:11-13
+            h.position[1] = o.y + d.y * t;
+            h.position[2] = o.z + d.z * t;
+          }
+        }
+      }
+      out[r] = h;
+    }
+  });
+  return RDRT_OK;
+}
+
+}  // namespace rdrt

@@ -104,6 +104,12 @@ struct RefitLevels {
 };
 RefitLevels refit_levels(const BuiltScene &b);
 
+// Nearest hits of n host rays (origin, direction: 6 f32 each) against one
+// geometry (global index) through its reference tree, with the reference's
+// host picking semantics (rdrt_pick).
+int pick_nearest(const BuiltScene &b, uint32_t geo, const float *rays, uint32_t n, uint32_t side, rdrt_pick_hit *out,
+                 std::string &err);
+
 // Rebuilds the TLAS layer of `out` (instances, TLAS trees in every layout)
 // from out.blas_box and the BLAS geometry ranges.
 int build_tlas_layer(const SceneSrc &src, BuiltScene &out, std::string &err);
