@@ -257,6 +257,12 @@ typedef struct rdrt_pick_hit {
  * host-only context is enough); uses the committed positions. */
 int32_t rdrt_pick(rdrt_ctx *ctx, uint32_t blas, uint32_t geometry, const float *rays, uint32_t n, uint32_t face_side,
                   rdrt_pick_hit *out);
+/* Every hit of one ray (6 floats) against one geometry, in the order the
+ * reference's walk finds them (intersect_list_bvh,
+ * content/mesh/core/src/feature/bvh.rs:23-55).  *count receives the number
+ * of hits; the first min(count, capacity) are written to out. */
+int32_t rdrt_pick_all(rdrt_ctx *ctx, uint32_t blas, uint32_t geometry, const float *ray, uint32_t face_side,
+                      rdrt_pick_hit *out, uint32_t capacity, uint32_t *count);
 
 /* ---- tracing ---- */
 int32_t rdrt_trace(rdrt_ctx *ctx, const rdrt_trace_args *args);

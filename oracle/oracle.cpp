@@ -1853,6 +1853,7 @@ struct PickState {
   bool found;
   uint32_t prim;
   float dist;
+  std::vector<std::pair<uint32_t, float>> *all;  // every hit in visit order, or null
 };
 
 void pick_leaf_prim(PickState &st, uint32_t tri) {
@@ -1861,6 +1862,7 @@ void pick_leaf_prim(PickState &st, uint32_t tri) {
                            s->vertices[s->indices[3 * (size_t)tri + 1]], s->vertices[s->indices[3 * (size_t)tri + 2]],
                            st.side);
   if (std::isnan(t)) return;
+  if (st.all) st.all->push_back({tri - st.prim_start, t});
   if (st.found && !(t < st.dist)) return;  // is_near_than: distance < (intersect_util.rs:15-17)
   st.found = true;
   st.dist = t;
@@ -1915,7 +1917,7 @@ int orc_pick(const orc_scene *s, uint32_t blas, uint32_t geometry, const float *
   std::sort(tris.begin(), tris.end());
   for (uint32_t r = 0; r < n; r++) {
     const float *q = rays + 6 * (size_t)r;
-    PickState st{s, geo.primitive_start, v3(q[0], q[1], q[2]), v3(q[3], q[4], q[5]), side, false, 0u, 0.0f};
+    PickState st{s, geo.primitive_start, v3(q[0], q[1], q[2]), v3(q[3], q[4], q[5]), side, false, 0u, 0.0f, nullptr};
     if (bvh) {
       pick_node(st, geo.root);
     } else {
@@ -1932,6 +1934,31 @@ int orc_pick(const orc_scene *s, uint32_t blas, uint32_t geometry, const float *
       h.position[2] = p.z;
     }
     out[r] = h;
+  }
+  return 0;
+}
+
+int orc_pick_all(const orc_scene *s, uint32_t blas, uint32_t geometry, const float *ray, uint32_t side,
+                 orc_pick_hit *out, uint32_t capacity, uint32_t *count) {
+  if (!s->built || blas >= s->blas_meta.size() || side > 2u) return -1;
+  const auto bm = s->blas_meta[blas];
+  if (geometry >= bm.second - bm.first) return -1;
+  const auto &geo = s->geo_meta[bm.first + geometry];
+  std::vector<std::pair<uint32_t, float>> all;
+  PickState st{s, geo.primitive_start, v3(ray[0], ray[1], ray[2]), v3(ray[3], ray[4], ray[5]), side, false, 0u, 0.0f,
+               &all};
+  pick_node(st, geo.root);
+  *count = (uint32_t)all.size();
+  for (uint32_t k = 0; k < all.size() && k < capacity; k++) {
+    orc_pick_hit h;
+    std::memset(&h, 0, sizeof(h));
+    h.prim = all[k].first;
+    h.distance = all[k].second;
+    const V3 p = st.o + st.d * h.distance;
+    h.position[0] = p.x;
+    h.position[1] = p.y;
+    h.position[2] = p.z;
+    out[k] = h;
   }
   return 0;
 }

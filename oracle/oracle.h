@@ -159,6 +159,11 @@ typedef struct orc_pick_hit {
  * of equally near hits is kept (intersect_util.rs:52-61). */
 int orc_pick(const orc_scene *s, uint32_t blas, uint32_t geometry, const float *rays, uint32_t n, uint32_t side,
              int bvh, orc_pick_hit *out);
+/* Every hit of one ray in the walk's visit order (bvh.rs:23-55
+ * intersect_list_bvh); *count = number of hits, the first `capacity`
+ * written. */
+int orc_pick_all(const orc_scene *s, uint32_t blas, uint32_t geometry, const float *ray, uint32_t side,
+                 orc_pick_hit *out, uint32_t capacity, uint32_t *count);
 
 /* Sobol dimension 0 / 1 (sobol.rs:40-68). */
 float orc_van_der_corput(uint32_t n, uint32_t scramble);

@@ -70,6 +70,7 @@ _SIG = {
     "orc_pick_triangle": (f32, [P, P, P, P, P, u32]),
     "orc_pick_box": (C.c_int, [P, P, P, P]),
     "orc_pick": (C.c_int, [P, u32, u32, P, u32, u32, C.c_int, P]),
+    "orc_pick_all": (C.c_int, [P, u32, u32, P, u32, P, u32, P]),
     "orc_van_der_corput": (f32, [u32, u32]),
     "orc_sobol": (f32, [u32, u32]),
     "orc_compact": (u32, [P, P, u32, P]),
@@ -335,6 +336,17 @@ class OracleScene:
         if rc != 0:
             raise RuntimeError("oracle pick failed")
         out = out[:r.shape[0]]
+        return out[:, 0].copy(), out[:, 1].view(np.float32).copy(), out[:, 2:5].view(np.float32).copy()
+
+    def pick_all(self, blas: int, geometry: int, ray, side: int = 2):
+        r = np.ascontiguousarray(ray, dtype=np.float32).reshape(6)
+        cnt = C.c_uint32(0)
+        self.L.orc_pick_all(self.h, blas, geometry, r.ctypes.data, side, None, 0, C.byref(cnt))
+        out = np.zeros((max(1, cnt.value), 6), dtype=np.uint32)
+        rc = self.L.orc_pick_all(self.h, blas, geometry, r.ctypes.data, side, out.ctypes.data, cnt.value, C.byref(cnt))
+        if rc != 0:
+            raise RuntimeError("oracle pick failed")
+        out = out[:cnt.value]
         return out[:, 0].copy(), out[:, 1].view(np.float32).copy(), out[:, 2:5].view(np.float32).copy()
 
     def trace_brute(self, rays: np.ndarray, flags=0, cull_mask=0xFFFFFFFF, tlas_idx=0, threads: int = 0):

@@ -149,6 +149,7 @@ SIGNATURES = {
     "rdrt_blas_set_vertex_uvs": (i32, [C.c_void_p, u32, u32, C.c_void_p, u32]),
     "rdrt_blas_add_triangles_u16": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p, u32, u32]),
     "rdrt_pick": (i32, [C.c_void_p, u32, u32, C.c_void_p, u32, u32, C.c_void_p]),
+    "rdrt_pick_all": (i32, [C.c_void_p, u32, u32, C.c_void_p, u32, C.c_void_p, u32, C.c_void_p]),
     "rdrt_autotune": (i32, [C.c_void_p, C.POINTER(RdrtTraceArgs), u32, C.POINTER(RdrtTune)]),
     "rdrt_set_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),
     "rdrt_get_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),

@@ -275,6 +275,22 @@ class AccelerationStructureSystem:
         out = out[:r.shape[0]]
         return out[:, 0].copy(), out[:, 1].view(np.float32).copy(), out[:, 2:5].view(np.float32).copy()
 
+    def pick_all(self, blas: BlasHandle, geometry: int, ray, face_side: int = 2):
+        """Every hit of one host ray (6 floats) in the reference's visit
+        order: (primitive uint32, distance f32, position (k, 3))."""
+        r = np.ascontiguousarray(ray, dtype=np.float32).reshape(6)
+        cap = 64
+        while True:
+            out = np.zeros((cap, 6), dtype=np.uint32)
+            cnt = C.c_uint32(0)
+            L.check(self._L.rdrt_pick_all(self._h, blas.index, geometry, r.ctypes.data, face_side, out.ctypes.data,
+                                          cap, C.byref(cnt)))
+            if cnt.value <= cap:
+                break
+            cap = cnt.value
+        out = out[:cnt.value]
+        return out[:, 0].copy(), out[:, 1].view(np.float32).copy(), out[:, 2:5].view(np.float32).copy()
+
     def set_vertex_uvs(self, blas: BlasHandle, geometry: int, uvs: np.ndarray) -> None:
         """Per-vertex texture coordinates (nverts, 2) of one geometry."""
         u = np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1)

@@ -365,6 +365,20 @@ int32_t rdrt_pick(rdrt_ctx *c, uint32_t blas, uint32_t geometry, const float *ra
   return RDRT_OK;
 }
 
+int32_t rdrt_pick_all(rdrt_ctx *c, uint32_t blas, uint32_t geometry, const float *ray, uint32_t face_side,
+                      rdrt_pick_hit *out, uint32_t capacity, uint32_t *count) {
+  if (!c || !ray || !count || (capacity && !out)) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  int32_t rc = rdrt_build(c);
+  if (rc) return rc;
+  const BuiltScene &b = c->built;
+  if (blas >= b.blas_geo_begin.size() || geometry >= b.blas_geo_end[blas] - b.blas_geo_begin[blas])
+    return fail(RDRT_ERR_INVALID_ARG, "unknown BLAS or geometry");
+  std::string err;
+  rc = pick_all(b, b.blas_geo_begin[blas] + geometry, ray, face_side, out, capacity, count, err);
+  if (rc) return fail(rc, err);
+  return RDRT_OK;
+}
+
 int32_t rdrt_set_device_build(rdrt_ctx *c, uint32_t enable) {
   if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
   if (enable > 1) return fail(RDRT_ERR_INVALID_ARG, "enable is 0 or 1");

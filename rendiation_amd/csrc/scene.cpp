@@ -1042,10 +1042,50 @@ float pick_triangle(P3 o, P3 d, P3 a, P3 b, P3 c, uint32_t side) {
   return qdn / ddn;
 }
 
-}  // namespace
+// Every hit of one ray in the reference's visit order: visit(triangle, t).
+template <class V>
+void pick_walk(const BuiltScene &b, const GeoRec &g, P3 o, P3 d, uint32_t side, std::vector<uint32_t> &stack,
+               V visit) {
+  // traverse_branch_leaf_visitor (content/space/src/bvh/mod.rs:94-114 on
+  // utility/abstract-tree/src/lib.rs:33-51): a stack; a branch whose box
+  // the ray hits pushes its children left then right (mod.rs:38-45), so
+  // the right one is visited first; a leaf is visited without a box test
+  stack.assign(1, g.ref_root);
+  while (!stack.empty()) {
+    const uint32_t i = stack.back();
+    stack.pop_back();
+    const RefNode &nd = b.tri_forest[i];
+    if (nd.hit_next != nd.miss_next) {
+      if (pick_box(o, d, nd.mn, nd.mx)) {
+        stack.push_back(nd.hit_next);
+        stack.push_back(b.tri_forest[nd.hit_next].miss_next);
+      }
+      continue;
+    }
+    for (uint32_t s = nd.begin; s < nd.end; s++) {
+      const uint32_t tri = b.redirect[s];
+      const float *va = &b.vertices[3 * (size_t)b.indices[3 * (size_t)tri]];
+      const float *vb = &b.vertices[3 * (size_t)b.indices[3 * (size_t)tri + 1]];
+      const float *vc = &b.vertices[3 * (size_t)b.indices[3 * (size_t)tri + 2]];
+      const float t =
+          pick_triangle(o, d, P3{va[0], va[1], va[2]}, P3{vb[0], vb[1], vb[2]}, P3{vc[0], vc[1], vc[2]}, side);
+      if (!std::isnan(t)) visit(tri, t);
+    }
+  }
+}
 
-int pick_nearest(const BuiltScene &b, uint32_t geo, const float *rays, uint32_t n, uint32_t side, rdrt_pick_hit *out,
-                 std::string &err) {
+rdrt_pick_hit make_pick_hit(uint32_t prim, P3 o, P3 d, float t) {
+  rdrt_pick_hit h;
+  std::memset(&h, 0, sizeof(h));
+  h.primitive = prim;
+  h.distance = t;
+  h.position[0] = o.x + d.x * t;  // hyper_ray.rs:11-13
+  h.position[1] = o.y + d.y * t;
+  h.position[2] = o.z + d.z * t;
+  return h;
+}
+
+int pick_check(const BuiltScene &b, uint32_t geo, uint32_t side, std::string &err) {
   if (geo >= b.geos.size()) {
     err = "unknown geometry";
     return RDRT_ERR_INVALID_ARG;
@@ -1054,6 +1094,29 @@ int pick_nearest(const BuiltScene &b, uint32_t geo, const float *rays, uint32_t 
     err = "face side is RDRT_FACE_FRONT, RDRT_FACE_BACK or RDRT_FACE_DOUBLE";
     return RDRT_ERR_INVALID_ARG;
   }
+  return RDRT_OK;
+}
+
+}  // namespace
+
+int pick_all(const BuiltScene &b, uint32_t geo, const float *ray, uint32_t side, rdrt_pick_hit *out, uint32_t capacity,
+             uint32_t *count, std::string &err) {
+  if (int rc = pick_check(b, geo, side, err)) return rc;
+  const P3 o{ray[0], ray[1], ray[2]}, d{ray[3], ray[4], ray[5]};
+  std::vector<uint32_t> stack;
+  uint32_t k = 0;
+  const uint32_t prim0 = b.geos[geo].primitive_start;
+  pick_walk(b, b.geos[geo], o, d, side, stack, [&](uint32_t tri, float t) {
+    if (k < capacity) out[k] = make_pick_hit(tri - prim0, o, d, t);
+    k++;
+  });
+  *count = k;
+  return RDRT_OK;
+}
+
+int pick_nearest(const BuiltScene &b, uint32_t geo, const float *rays, uint32_t n, uint32_t side, rdrt_pick_hit *out,
+                 std::string &err) {
+  if (int rc = pick_check(b, geo, side, err)) return rc;
   const GeoRec &g = b.geos[geo];
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
   parallel_for((n + 255) / 256, hw, [&](size_t chunk) {
@@ -1062,45 +1125,24 @@ int pick_nearest(const BuiltScene &b, uint32_t geo, const float *rays, uint32_t 
     for (uint32_t r = (uint32_t)chunk * 256; r < r1; r++) {
       const float *q = rays + 6 * (size_t)r;
       const P3 o{q[0], q[1], q[2]}, d{q[3], q[4], q[5]};
-      rdrt_pick_hit h;
-      std::memset(&h, 0, sizeof(h));
-      h.primitive = RDRT_MISS;
       bool found = false;
-      // traverse_branch_leaf_visitor (content/space/src/bvh/mod.rs:94-114 on
-      // utility/abstract-tree/src/lib.rs:33-51): a stack; a branch whose box
-      // the ray hits pushes its children left then right (mod.rs:38-45), so
-      // the right one is visited first; a leaf is visited without a box test
-      stack.assign(1, g.ref_root);
-      while (!stack.empty()) {
-        const uint32_t i = stack.back();
-        stack.pop_back();
-        const RefNode &nd = b.tri_forest[i];
-        if (nd.hit_next != nd.miss_next) {
-          if (pick_box(o, d, nd.mn, nd.mx)) {
-            stack.push_back(nd.hit_next);
-            stack.push_back(b.tri_forest[nd.hit_next].miss_next);
-          }
-          continue;
+      uint32_t best = 0;
+      float best_t = 0.0f;
+      pick_walk(b, g, o, d, side, stack, [&](uint32_t tri, float t) {
+        if (!found || t < best_t) {  // refresh: strictly nearer (intersect_util.rs:15-17,52-61)
+          found = true;
+          best = tri;
+          best_t = t;
         }
-        for (uint32_t s = nd.begin; s < nd.end; s++) {
-          const uint32_t tri = b.redirect[s];
-          const float *va = &b.vertices[3 * (size_t)b.indices[3 * (size_t)tri]];
-          const float *vb = &b.vertices[3 * (size_t)b.indices[3 * (size_t)tri + 1]];
-          const float *vc = &b.vertices[3 * (size_t)b.indices[3 * (size_t)tri + 2]];
-          const float t = pick_triangle(o, d, P3{va[0], va[1], va[2]}, P3{vb[0], vb[1], vb[2]},
-                                        P3{vc[0], vc[1], vc[2]}, side);
-          if (std::isnan(t)) continue;
-          if (!found || t < h.distance) {  // refresh: strictly nearer (intersect_util.rs:15-17,52-61)
-            found = true;
-            h.primitive = tri - g.primitive_start;
-            h.distance = t;
-            h.position[0] = o.x + d.x * t;  // hyper_ray.rs:11-13
-            h.position[1] = o.y + d.y * t;
-            h.position[2] = o.z + d.z * t;
-          }
-        }
+      });
+      if (found) {
+        out[r] = make_pick_hit(best - g.primitive_start, o, d, best_t);
+      } else {
+        rdrt_pick_hit h;
+        std::memset(&h, 0, sizeof(h));
+        h.primitive = RDRT_MISS;
+        out[r] = h;
       }
-      out[r] = h;
     }
   });
   return RDRT_OK;

@@ -109,6 +109,11 @@ RefitLevels refit_levels(const BuiltScene &b);
 // host picking semantics (rdrt_pick).
 int pick_nearest(const BuiltScene &b, uint32_t geo, const float *rays, uint32_t n, uint32_t side, rdrt_pick_hit *out,
                  std::string &err);
+// Every hit of one ray against one geometry, in the reference's visit order
+// (rdrt_pick_all); *count is the number of hits, of which the first
+// `capacity` are written.
+int pick_all(const BuiltScene &b, uint32_t geo, const float *ray, uint32_t side, rdrt_pick_hit *out, uint32_t capacity,
+             uint32_t *count, std::string &err);
 
 // Rebuilds the TLAS layer of `out` (instances, TLAS trees in every layout)
 // from out.blas_box and the BLAS geometry ranges.

@@ -326,3 +326,31 @@ def test_pick_u16_indices_and_errors():
     with pytest.raises(RdrtError):
         sys.pick(b, 0, rays, face_side=3)
     sys.close()
+
+
+def test_pick_all_matches_oracle_order():
+    """intersect_list_bvh: every hit in the walk's visit order.  The oracle's
+    list is pinned by its multiset (the sorted distances equal the brute
+    force's hits) and by its minimum (the nearest pick)."""
+    pos = synth.random_triangles(61, 1500, spread=0.25)
+    sys, orc, h = _systems([[(pos, None, 1)]], 2)
+    rays = _aimed_rays(pos, 62, 150)
+    many = 0
+    for r in rays:
+        for side in (FRONT, DOUBLE):
+            got = sys.pick_all(h[0], 0, r, side)
+            want = orc.pick_all(0, 0, r, side)
+            _assert_same(got, want)
+            # the same hits as testing every triangle
+            tris = pos.reshape(-1, 3, 3)
+            brute = [(i, O.pick_triangle(r[:3], r[3:], *tris[i], side)) for i in range(len(tris))]
+            brute = sorted((t, i) for i, t in brute if not np.isnan(t))
+            assert sorted(zip(got[1].tolist(), got[0].tolist())) == [(np.float32(t), i) for t, i in brute]
+            near = sys.pick(h[0], 0, r[None], side)
+            if len(got[0]):
+                k = int(np.argmin(got[1]))  # first of the minima
+                assert near[0][0] == got[0][k] and near[1][0] == got[1][k]
+            else:
+                assert near[0][0] == MISS
+            many += len(got[0]) > 3
+    assert many > 20
