@@ -238,6 +238,72 @@ typedef struct rdrt_hit_attributes {
   uint32_t sbt_record;       /* sbt_offset + sbt_stride * geometry + instance sbt offset; RDRT_MISS on a miss */
 } rdrt_hit_attributes;
 
+/* ---- device task graph (shader/task-graph/src/runtime) ---- */
+/* Groups of tasks that spawn, sleep on, wake and finish one another, polled
+ * round by round on the context's stream.  A task is in one of the states of
+ * task_pool.rs:85-107 (free, awake, going to sleep, asleep, finished).  The
+ * generic graph offers two task kinds, enough for the reference's own task
+ * graph tests (shader/task-graph/src/test.rs): */
+#define RDRT_TASK_READY 0u      /* resolves on its first poll (BaseShaderFuture) */
+#define RDRT_TASK_SPAWN_THEN 1u /* first poll spawns a task of `child_group` with payload + 1 and itself as parent,
+                                   sleeps until it finishes, takes its payload and resolves */
+typedef struct rdrt_task_group_desc {
+  uint32_t kind;          /* RDRT_TASK_* */
+  uint32_t child_group;   /* RDRT_TASK_SPAWN_THEN: the group it spawns into (may be itself) */
+  uint32_t max_in_flight; /* slots = max_in_flight * capacity (mod.rs:121) */
+  uint32_t pad;
+} rdrt_task_group_desc;
+typedef struct rdrt_task_graph rdrt_task_graph;
+/* DeviceTaskGraphBuildSource::build (mod.rs:97-160); at most 8 groups. */
+int32_t rdrt_task_graph_create(rdrt_ctx *ctx, const rdrt_task_group_desc *groups, uint32_t ngroups, uint32_t capacity,
+                               rdrt_task_graph **out);
+int32_t rdrt_task_graph_destroy(rdrt_task_graph *g);
+/* dispatch_allocate_init_task (mod.rs:223-303): `count` root tasks of
+ * `group`, payload = spawn index, no parent. */
+int32_t rdrt_task_graph_spawn(rdrt_task_graph *g, uint32_t group, uint32_t count);
+/* execute (mod.rs:396-460): `rounds` rounds; each polls every group in
+ * order (prepare, poll the active tasks, compact the active list). */
+int32_t rdrt_task_graph_execute(rdrt_task_graph *g, uint32_t rounds);
+/* read_back_execution_states (mod.rs:305-370), per group: awake tasks,
+ * tasks asleep or finished (slots - empty - awake) and empty slots; errors
+ * (optional): bit 0 pool exhausted, bit 1 active list full, bit 2 free list
+ * full.  Waits for the stream. */
+int32_t rdrt_task_graph_states(rdrt_task_graph *g, uint32_t *wake, uint32_t *sleep_or_finished, uint32_t *empty,
+                               uint32_t *errors);
+
+/* ---- ambient occlusion on the task graph (scene/rendering/gpu-ray-tracing/src/feature/ao.rs:176-315) ---- */
+#define RDRT_AO_FORMAT_F32 0u   /* one float per pixel */
+#define RDRT_AO_FORMAT_RGBA8 1u /* RGBA8 unorm per pixel, as the reference's AO texture */
+typedef struct rdrt_ao_params {
+  uint32_t width, height;
+  uint32_t sample_index; /* samples accumulated so far (the reference's AO sample count) */
+  uint32_t format;       /* RDRT_AO_FORMAT_* */
+  float view_projection_inv[16];
+  float camera_position[3];
+  uint32_t rounds;      /* task-graph rounds per tile; 0 = 8 (ao.rs:300) */
+  uint32_t tile_pixels; /* pixels per task-graph execution, whole rows; 0 = 512 * 512 (wavefront_compute/mod.rs:134) */
+  uint32_t use_graph;   /* replay a captured graph of the call */
+  uint32_t pad[2];
+} rdrt_ao_params;
+/* One AO sample for every pixel of the TLAS bound at index 0: camera ray
+ * (primary hits cull back faces), then an occlusion ray from the hit,
+ * cosine-distributed around the shading normal by the Hammersley point of
+ * the sample index, first hit in [0.01, 100]; the pixel's running mean in
+ * `accum` (device memory) becomes (mean * s + v) / (s + 1) while s < 256
+ * (v = 1 when the occlusion ray or the camera ray misses, 0 otherwise).
+ * Runs as the reference's ray-tracing pipeline does: trace, ray-gen,
+ * closest-hit and miss task groups on the device task graph. */
+int32_t rdrt_ao_render(rdrt_ctx *ctx, const rdrt_ao_params *params, void *accum);
+/* States of the AO task graph after the last rdrt_ao_render, as
+ * rdrt_task_graph_states (5 groups: trace, ray gen, closest hit, occlusion
+ * closest hit, miss). */
+int32_t rdrt_ao_states(rdrt_ctx *ctx, uint32_t *wake, uint32_t *sleep_or_finished, uint32_t *empty, uint32_t *errors);
+/* The unit-hemisphere direction (tangent space, z = normal) of sample
+ * index s: hammersley_2d(s, 256) through sample_hemisphere_cos
+ * (shader/library/src/sampling.rs:33-68), sine and cosine rounded from
+ * double precision. */
+int32_t rdrt_ao_hemisphere_sample(uint32_t s, float out[3]);
+
 /* ---- host picking (content/mesh/core/src/feature/bvh.rs:57-86) ---- */
 #define RDRT_FACE_FRONT 0u
 #define RDRT_FACE_BACK 1u

@@ -1964,3 +1964,138 @@ int orc_pick_all(const orc_scene *s, uint32_t blas, uint32_t geometry, const flo
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------- ambient occlusion --
+// scene/rendering/gpu-ray-tracing/src/feature/ao.rs:192-304 per pixel, in
+// plain control flow (ray gen -> closest hit -> occlusion ray -> running
+// mean), FP32 with the reference operation order.
+namespace {
+
+// sampler.rs:11-16 pcg and :64-71 next()
+float ao_pcg_next(uint32_t &state) {
+  const uint32_t prev = state * 747796405u + 2891336453u;
+  const uint32_t word = ((prev >> ((prev >> 28u) + 4u)) ^ prev) * 277803737u;
+  state = prev;
+  const uint32_t r = (word >> 22u) ^ word;
+  const uint32_t bits = 0x3f800000u | (r >> 9u);
+  float f;
+  std::memcpy(&f, &bits, 4);
+  return f - 1.0f;
+}
+
+// One pixel's sample value: 1 when the camera ray or the occlusion ray
+// misses, 0 when both hit.
+float ao_value(const orc_scene *s, const orc_ao_params &P, uint32_t x, uint32_t y, const float local[3]) {
+  uint32_t state = xxhash32(x, y, P.sample_index);
+  const float r0 = ao_pcg_next(state);
+  const float r1 = ao_pcg_next(state);
+  const float u = (float)x / (float)P.width + r0 / (float)P.width;
+  const float v = (float)y / (float)P.height + r1 / (float)P.height;
+  const V3 cam = v3(P.camera_position[0], P.camera_position[1], P.camera_position[2]);
+  const V3 dir = normalize(f_unproject(P.view_projection_inv, u, v) - cam);
+  RayIn ray;
+  ray.o = cam;
+  ray.d = dir;
+  ray.tmin = FLT_EPSILON;  // ShaderRayRange::default (api/ty.rs:49-56)
+  ray.tmax = FLT_MAX;
+  ray.flags = F_CULL_BACK;  // ao.rs:209
+  ray.mask = 0xffffffffu;
+  ray.tlas_idx = 0;
+  orc_hit h;
+  orc_hit_meta hm;
+  Counters cnt;
+  uint32_t ext[2] = {0, 0};
+  if (!traverse_one(s, ray, h, hm, cnt, ext)) return 1.0f;  // miss shader (ao.rs:304-312)
+  // closest hit (ao.rs:252-291) with the bindless mesh normal
+  // (bindless_mesh_bridge.rs:86-123)
+  const auto &geo = s->geo_meta[ext[0]];
+  const uint32_t tri = geo.primitive_start + h.prim_id;
+  const uint32_t ia = s->indices[3 * (size_t)tri], ib = s->indices[3 * (size_t)tri + 1],
+                 ic = s->indices[3 * (size_t)tri + 2];
+  const M4 &inv = s->inst_transform_inv[ext[1]];
+  const V3 pos = ray.o + ray.d * h.t;  // api/ctx.rs:209-211
+  const V3 pa = s->vertices[ia], pb = s->vertices[ib], pc = s->vertices[ic];
+  V3 gn = normalize(f_normal_to_world(inv, cross(pa - pb, pa - pc)));
+  if (dot(ray.o - pos, gn) < 0.0f) gn = gn * -1.0f;
+  V3 sn = gn;
+  if (!s->normals.empty()) {
+    const float w = 1.0f - h.u - h.v;
+    const V3 n = s->normals[ia] * w + s->normals[ib] * h.u + s->normals[ic] * h.v;
+    if (dot(n, n) > 0.0f) {
+      sn = normalize(f_normal_to_world(inv, n));
+      if (dot(gn, sn) < 0.0f) sn = sn * -1.0f;
+    }
+  }
+  // tbn (sampling.rs:70-83) times the hemisphere sample
+  const float sign = sn.z < 0.0f ? -1.0f : 1.0f;
+  const float a = -1.0f / (sign + sn.z);
+  const float b = sn.x * sn.y * a;
+  const V3 t = normalize(v3(1.0f + sign * sn.x * sn.x * a, sign * b, -sign * sn.x));
+  const V3 bt = normalize(v3(b, sign + sn.y * sn.y * a, -sn.y));
+  const V3 aodir = t * local[0] + bt * local[1] + sn * local[2];
+  RayIn occ;
+  occ.o = pos;
+  occ.d = aodir;
+  occ.tmin = 0.01f;  // ao.rs:279-282
+  occ.tmax = 100.0f;
+  occ.flags = F_ACCEPT_FIRST_HIT_AND_END_SEARCH;  // ao.rs:276
+  occ.mask = 0xffffffffu;
+  occ.tlas_idx = 0;
+  orc_hit h2;
+  orc_hit_meta hm2;
+  Counters cnt2;
+  return traverse_one(s, occ, h2, hm2, cnt2) ? 0.0f : 1.0f;
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_ao_hemisphere_sample(uint32_t s, float out[3]) {
+  // hammersley_2d(s, 256) (sampling.rs:33-50): (s / 256, radical inverse)
+  const float ux = (float)s / 256.0f;
+  uint32_t r = 0;
+  for (int k = 0; k < 32; k++) r |= ((s >> k) & 1u) << (31 - k);  // bit reversal
+  const float uy = (float)r * 2.3283064e-10f;
+  // sample_hemisphere_cos (sampling.rs:61-68) with 2 pi as an f32
+  const float phi = (2.0f * 3.14159265358979323846f) * uy;
+  const float cos_theta = std::sqrt(1.0f - ux);
+  const float sin_theta = std::sqrt(1.0f - cos_theta * cos_theta);
+  out[0] = (float)std::cos((double)phi) * sin_theta;
+  out[1] = (float)std::sin((double)phi) * sin_theta;
+  out[2] = cos_theta;
+}
+
+int orc_ao_render(const orc_scene *s, const orc_ao_params *P, float *accum, uint8_t *accum_rgba8, float *values,
+                  int nthreads) {
+  if (!s->built || P->width == 0 || P->height == 0) return -1;
+  float local[3];
+  orc_ao_hemisphere_sample(P->sample_index, local);
+  const int64_t npix = (int64_t)P->width * P->height;
+  const float prev_count = (float)P->sample_index;
+  const float all = prev_count + 1.0f;
+  const bool write = P->sample_index < 256u;  // MAX_SAMPLE (ao.rs:5, 232-243)
+#ifdef _OPENMP
+  int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nt)
+#else
+  (void)nthreads;
+#endif
+  for (int64_t i = 0; i < npix; i++) {
+    const uint32_t x = (uint32_t)(i % P->width), y = (uint32_t)(i / P->width);
+    const float val = ao_value(s, *P, x, y, local);
+    if (values) values[i] = val;
+    if (accum && write) accum[i] = (accum[i] * prev_count + val) / all;
+    if (accum_rgba8) {
+      const float prev = (float)accum_rgba8[4 * i] / 255.0f;
+      float nv = write ? (prev * prev_count + val) / all : prev;
+      nv = std::fmin(std::fmax(nv, 0.0f), 1.0f);
+      const uint8_t q = (uint8_t)std::nearbyint(nv * 255.0f);
+      accum_rgba8[4 * i] = accum_rgba8[4 * i + 1] = accum_rgba8[4 * i + 2] = q;
+      accum_rgba8[4 * i + 3] = 255;
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -165,6 +165,22 @@ int orc_pick(const orc_scene *s, uint32_t blas, uint32_t geometry, const float *
 int orc_pick_all(const orc_scene *s, uint32_t blas, uint32_t geometry, const float *ray, uint32_t side,
                  orc_pick_hit *out, uint32_t capacity, uint32_t *count);
 
+/* Ambient occlusion (scene/rendering/gpu-ray-tracing/src/feature/
+ * ao.rs:192-304): one sample per pixel with plain control flow. */
+typedef struct orc_ao_params {
+  uint32_t width, height, sample_index, pad;
+  float view_projection_inv[16]; /* column-major */
+  float camera_position[3];
+  float pad2;
+} orc_ao_params;
+/* The tangent-space occlusion direction of sample index s. */
+void orc_ao_hemisphere_sample(uint32_t s, float out[3]);
+/* One AO sample: values (optional) gets each pixel's 0 / 1, accum (f32,
+ * optional) and accum_rgba8 (optional) the running means.  The scene's
+ * TLAS bound at index 0 is traced. */
+int orc_ao_render(const orc_scene *s, const orc_ao_params *P, float *accum, uint8_t *accum_rgba8, float *values,
+                  int nthreads);
+
 /* Sobol dimension 0 / 1 (sobol.rs:40-68). */
 float orc_van_der_corput(uint32_t n, uint32_t scramble);
 float orc_sobol(uint32_t n, uint32_t scramble);

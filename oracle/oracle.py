@@ -31,6 +31,11 @@ class OrcSizes(C.Structure):
                                    "tlas", "max_blas_depth", "max_tlas_depth")]
 
 
+class OrcAoParams(C.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("sample_index", u32), ("pad", u32),
+                ("view_projection_inv", f32 * 16), ("camera_position", f32 * 3), ("pad2", f32)]
+
+
 class OrcInstance(C.Structure):
     _fields_ = [("transform", f32 * 16), ("instance_custom_index", u32), ("mask", u32), ("sbt_offset", u32),
                 ("flags", u32), ("blas", u32), ("pad", u32 * 3)]
@@ -70,6 +75,8 @@ _SIG = {
     "orc_pick_triangle": (f32, [P, P, P, P, P, u32]),
     "orc_pick_box": (C.c_int, [P, P, P, P]),
     "orc_pick": (C.c_int, [P, u32, u32, P, u32, u32, C.c_int, P]),
+    "orc_ao_hemisphere_sample": (None, [u32, P]),
+    "orc_ao_render": (C.c_int, [P, P, P, P, P, C.c_int]),
     "orc_pick_all": (C.c_int, [P, u32, u32, P, u32, P, u32, P]),
     "orc_van_der_corput": (f32, [u32, u32]),
     "orc_sobol": (f32, [u32, u32]),
@@ -349,6 +356,20 @@ class OracleScene:
         out = out[:cnt.value]
         return out[:, 0].copy(), out[:, 1].view(np.float32).copy(), out[:, 2:5].view(np.float32).copy()
 
+    def ao_render(self, width: int, height: int, sample_index: int, vp_inv_colmajor, cam_pos, accum=None,
+                  accum_rgba8=None, threads: int = 0):
+        """One AO sample; returns the per-pixel values (0 / 1) and updates the
+        running means in place when given."""
+        p = OrcAoParams()
+        p.width, p.height, p.sample_index = width, height, sample_index
+        p.view_projection_inv[:] = [float(x) for x in vp_inv_colmajor]
+        p.camera_position[:] = [float(x) for x in cam_pos]
+        vals = np.zeros(width * height, dtype=np.float32)
+        rc = self.L.orc_ao_render(self.h, C.byref(p), _p(accum), _p(accum_rgba8), vals.ctypes.data, threads)
+        if rc != 0:
+            raise RuntimeError("oracle AO failed")
+        return vals.reshape(height, width)
+
     def trace_brute(self, rays: np.ndarray, flags=0, cull_mask=0xFFFFFFFF, tlas_idx=0, threads: int = 0):
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
         hits = np.zeros((rays.shape[0], 4), dtype=np.uint32)
@@ -385,6 +406,12 @@ def pick_triangle(o, d, a, b, c, side=2) -> float:
 def pick_box(o, d, bmin, bmax) -> bool:
     v = [np.ascontiguousarray(x, dtype=np.float32) for x in (o, d, bmin, bmax)]
     return bool(lib().orc_pick_box(*[x.ctypes.data for x in v]))
+
+
+def ao_hemisphere_sample(s: int) -> np.ndarray:
+    out = np.zeros(3, dtype=np.float32)
+    lib().orc_ao_hemisphere_sample(s, out.ctypes.data)
+    return out
 
 
 def van_der_corput(n: int, scramble: int) -> float:

@@ -22,6 +22,16 @@ class RdrtRay(C.Structure):
                 ("dx", f32), ("dy", f32), ("dz", f32), ("tmax", f32)]
 
 
+class RdrtTaskGroupDesc(C.Structure):
+    _fields_ = [("kind", u32), ("child_group", u32), ("max_in_flight", u32), ("pad", u32)]
+
+
+class RdrtAoParams(C.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("sample_index", u32), ("format", u32),
+                ("view_projection_inv", f32 * 16), ("camera_position", f32 * 3), ("rounds", u32),
+                ("tile_pixels", u32), ("use_graph", u32), ("pad", u32 * 2)]
+
+
 class RdrtRayMeta(C.Structure):
     _fields_ = [("flags", u32), ("cull_mask", u32), ("tlas_idx", u32), ("pad", u32)]
 
@@ -83,6 +93,8 @@ class RdrtPtStats(C.Structure):
 
 RDRT_PT_DIAG_TIME_TRACES = 0x1
 RDRT_FACE_FRONT, RDRT_FACE_BACK, RDRT_FACE_DOUBLE = 0, 1, 2
+RDRT_TASK_READY, RDRT_TASK_SPAWN_THEN = 0, 1
+RDRT_AO_FORMAT_F32, RDRT_AO_FORMAT_RGBA8 = 0, 1
 RDRT_PT_DIAG_COUNT_FAST = 0x4
 RDRT_PT_DIAG_COUNT_VISITS = 0x2
 
@@ -150,6 +162,14 @@ SIGNATURES = {
     "rdrt_blas_add_triangles_u16": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p, u32, u32]),
     "rdrt_pick": (i32, [C.c_void_p, u32, u32, C.c_void_p, u32, u32, C.c_void_p]),
     "rdrt_pick_all": (i32, [C.c_void_p, u32, u32, C.c_void_p, u32, C.c_void_p, u32, C.c_void_p]),
+    "rdrt_task_graph_create": (i32, [C.c_void_p, C.c_void_p, u32, u32, C.c_void_p]),
+    "rdrt_task_graph_destroy": (i32, [C.c_void_p]),
+    "rdrt_task_graph_spawn": (i32, [C.c_void_p, u32, u32]),
+    "rdrt_task_graph_execute": (i32, [C.c_void_p, u32]),
+    "rdrt_task_graph_states": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "rdrt_ao_render": (i32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "rdrt_ao_states": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "rdrt_ao_hemisphere_sample": (i32, [u32, C.c_void_p]),
     "rdrt_autotune": (i32, [C.c_void_p, C.POINTER(RdrtTraceArgs), u32, C.POINTER(RdrtTune)]),
     "rdrt_set_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),
     "rdrt_get_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),

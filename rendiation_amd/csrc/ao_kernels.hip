@@ -1,0 +1,330 @@
+// ao_kernels.hip -- the poll kernels of the ambient-occlusion pipeline on the
+// device task graph (scene/rendering/gpu-ray-tracing/src/feature/
+// ao.rs:192-304), and the trace task's poll split around the traversal
+// launch (shader/ray-tracing/src/backend/wavefront_compute/trace_task.rs).
+#include "ao.h"
+#include "pt_device.h"
+
+#pragma clang fp contract(off)
+
+namespace rdrt {
+namespace {
+
+using pt::V3;
+using pt::v3;
+using tg::Graph;
+using tg::Group;
+
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kMaxGrid = 1024;
+constexpr float kFltEps = 1.1920928955078125e-7f;  // ShaderRayRange::default (api/ty.rs:49-56)
+constexpr float kFltMax = 3.402823466e+38f;
+
+uint32_t grid_for(uint32_t n) {
+  const uint32_t g = (n + kBlock - 1) / kBlock;
+  return g < 1u ? 1u : (g > kMaxGrid ? kMaxGrid : g);
+}
+
+__device__ __forceinline__ void write_ray(rdrt_ray *r, V3 o, float tmin, V3 d, float tmax) {
+  float4 *q = reinterpret_cast<float4 *>(r);
+  q[0] = make_float4(o.x, o.y, o.z, tmin);
+  q[1] = make_float4(d.x, d.y, d.z, tmax);
+}
+__device__ __forceinline__ V3 load3(const float *p) { return v3(p[0], p[1], p[2]); }
+
+// Payload words.  Trace: [0] ray type (0 primary, 1 occlusion), [1] payload
+// value, [2] the group of the awaited closest-hit / miss task.  Ray gen:
+// [0] tile pixel.  Closest / miss: [0] payload value.
+__device__ __forceinline__ uint32_t *pw(const Group &G, uint32_t idx) { return G.payload + (size_t)idx * G.words; }
+
+// Starts a trace task (then_trace): the ray goes to the trace group's ray
+// array at the new task's slot.
+__device__ __forceinline__ uint32_t spawn_trace(const AoLaunch &L, uint32_t parent, uint32_t ptype, uint32_t ray_type,
+                                                float value, V3 o, float tmin, V3 d, float tmax) {
+  const Group &T = L.gr.g[kAoTrace];
+  const uint32_t c = tg::spawn(T, parent, ptype);
+  if (c == tg::kNone) return c;
+  uint32_t *w = pw(T, c);
+  w[0] = ray_type;
+  w[1] = __float_as_uint(value);
+  w[2] = tg::kNone;
+  write_ray(L.rays + c, o, tmin, d, tmax);
+  return c;
+}
+
+// Reads back a finished child's payload value and frees it; false while it
+// is still running (TaskFuture poll, task_group.rs:464-488).
+__device__ __forceinline__ bool take_child(const Graph &gr, uint32_t cgroup, uint32_t c, float &value) {
+  if (cgroup >= gr.n || c == tg::kNone) return false;
+  const Group &C = gr.g[cgroup];
+  if (C.state[c] != tg::kFinished) return false;
+  value = __uint_as_float(pw(C, c)[cgroup == kAoTrace ? 1 : 0]);
+  tg::cleanup(C, c);
+  return true;
+}
+
+// ---- group 0: trace ----
+// First half of the poll: tasks waiting on their closest-hit / miss child
+// take its payload and resolve; new tasks are queued for the traversal.
+__global__ void k_trace_pre(AoLaunch L) {
+  const Group &T = L.gr.g[kAoTrace];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += gridDim.x * blockDim.x) {
+    uint32_t idx;
+    if (!tg::poll_entry(T, i, idx)) break;
+    uint32_t *w = pw(T, idx);
+    if (T.stage[idx] == 0u) {
+      const uint32_t q = w[0] == 0u ? 0u : 1u;
+      const uint32_t pos = atomicAdd(L.tcount + q, 1u);
+      (q ? L.tq1 : L.tq0)[pos] = idx;
+      continue;
+    }
+    float v;
+    if (take_child(L.gr, w[2], T.child[idx], v)) {
+      w[1] = __float_as_uint(v);
+      tg::end_poll(L.gr, kAoTrace, idx, true);
+    } else {
+      tg::end_poll(L.gr, kAoTrace, idx, false);
+    }
+  }
+}
+
+// Second half, after the traversal: each traced task spawns the closest-hit
+// task of its hit group (ao.rs:20-47: every instance maps ray type 0 to the
+// occlusion closest hit, ray type 1 to the one that writes 0) or the miss
+// task, passes it the payload and sleeps.  The last block clears the queue
+// counts for the next round.
+__global__ void k_trace_post(AoLaunch L) {
+  const Group &T = L.gr.g[kAoTrace];
+  const uint32_t n0 = L.tcount[0], n1 = L.tcount[1];
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n0 + n1; j += gridDim.x * blockDim.x) {
+    const uint32_t idx = j < n0 ? L.tq0[j] : L.tq1[j - n0];
+    uint32_t *w = pw(T, idx);
+    const bool hit = L.hits[idx].prim_id != RDRT_MISS;
+    const uint32_t cg = hit ? (w[0] == 0u ? kAoClosest : kAoClosestTest) : kAoMiss;
+    const Group &C = L.gr.g[cg];
+    const uint32_t c = tg::spawn(C, idx, kAoTrace);
+    if (c != tg::kNone) pw(C, c)[0] = w[1];
+    w[2] = cg;
+    T.child[idx] = c;
+    T.stage[idx] = 1u;
+    tg::end_poll(L.gr, kAoTrace, idx, false);
+  }
+  __shared__ bool last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(L.tcount + 2, 1u) == gridDim.x - 1u;
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    L.tcount[0] = 0u;
+    L.tcount[1] = 0u;
+    L.tcount[2] = 0u;
+  }
+}
+
+// ---- group 1: ray generation (ao.rs:192-245) ----
+// PCGRandomSampler (scene/rendering/gpu-ray-tracing/src/sampler.rs:11-16,
+// 48-71): seeded by xxhash32(pixel, sample index); next() maps the top 23
+// bits to [0, 1).
+__device__ __forceinline__ float pcg_next(uint32_t &state) {
+  const uint32_t prev = state * 747796405u + 2891336453u;
+  const uint32_t word = ((prev >> ((prev >> 28u) + 4u)) ^ prev) * 277803737u;
+  state = prev;
+  const uint32_t r = (word >> 22u) ^ word;
+  return __uint_as_float(0x3f800000u | (r >> 9u)) - 1.0f;
+}
+
+__device__ __forceinline__ void accumulate(const AoLaunch &L, uint32_t x, uint32_t y, uint32_t s, float payload) {
+  const size_t px = (size_t)y * L.width + x;
+  const float prev_count = (float)s;
+  const float all = prev_count + 1.0f;
+  if (L.format == RDRT_AO_FORMAT_F32) {
+    float *a = static_cast<float *>(L.accum) + px;
+    const float prev = *a;
+    if (s < kAoMaxSample) *a = (prev * prev_count + payload) / all;
+    return;
+  }
+  // RGBA8 unorm like the reference's texture: the stored value is read back
+  // as k / 255 and written rounded to the nearest of those
+  uchar4 *a = static_cast<uchar4 *>(L.accum) + px;
+  const uchar4 old = *a;
+  const float prev = (float)old.x / 255.0f;
+  float v = s < kAoMaxSample ? (prev * prev_count + payload) / all : prev;
+  v = fminf(fmaxf(v, 0.0f), 1.0f);
+  const unsigned char q = (unsigned char)rintf(v * 255.0f);
+  *a = make_uchar4(q, q, q, 255);
+}
+
+__global__ void k_raygen(AoLaunch L) {
+  const Group &G = L.gr.g[kAoRayGen];
+  const uint32_t s = __float_as_uint(L.frame[23]);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += gridDim.x * blockDim.x) {
+    uint32_t idx;
+    if (!tg::poll_entry(G, i, idx)) break;
+    const uint32_t tp = pw(G, idx)[0];
+    const uint32_t x = L.tile_x + tp % L.tile_w, y = L.tile_y + tp / L.tile_w;
+    if (G.stage[idx] == 0u) {
+      uint32_t state = pt::xxhash32(x, y, s);
+      const float r0 = pcg_next(state);
+      const float r1 = pcg_next(state);
+      // camera.rs:66-99
+      const float u = (float)x / (float)L.width + r0 / (float)L.width;
+      const float v = (float)y / (float)L.height + r1 / (float)L.height;
+      const V3 target = pt::unproject(L.frame, u, v);
+      const V3 cam = load3(L.frame + 16);
+      const V3 d = pt::normalize3(target - cam);
+      // payload 0: occluded unless the miss shader writes 1 (ao.rs:216-219)
+      const uint32_t c = spawn_trace(L, idx, kAoRayGen, 0u, 0.0f, cam, kFltEps, d, kFltMax);
+      G.child[idx] = c;
+      G.stage[idx] = 1u;
+      tg::end_poll(L.gr, kAoRayGen, idx, false);
+      continue;
+    }
+    float payload;
+    if (take_child(L.gr, kAoTrace, G.child[idx], payload)) {
+      accumulate(L, x, y, s, payload);
+      tg::end_poll(L.gr, kAoRayGen, idx, true);
+    } else {
+      tg::end_poll(L.gr, kAoRayGen, idx, false);
+    }
+  }
+}
+
+// ---- group 2: closest hit of primary rays (ao.rs:252-291) ----
+// World position o + t d, shading normal from the bindless mesh
+// (bindless_mesh_bridge.rs:86-123), the occlusion direction tbn(n) * the
+// frame's cosine-weighted hemisphere sample, range [0.01, 100], first hit.
+__global__ void k_closest(AoLaunch L) {
+  const Group &G = L.gr.g[kAoClosest];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += gridDim.x * blockDim.x) {
+    uint32_t idx;
+    if (!tg::poll_entry(G, i, idx)) break;
+    if (G.stage[idx] == 0u) {
+      const uint32_t p = G.parent[idx];  // the trace task: its ray and hit
+      const rdrt_hit h = L.hits[p];
+      const uint2 hx = L.hitx[p];
+      const float4 *rq = reinterpret_cast<const float4 *>(L.rays + p);
+      const float4 q0 = rq[0], q1 = rq[1];
+      const V3 o = v3(q0.x, q0.y, q0.z), d = v3(q1.x, q1.y, q1.z);
+      const GeoRec &g = L.geos[hx.x];
+      const InstRec &in = L.insts[hx.y];
+      const size_t tri = (size_t)g.primitive_start + h.prim_id;
+      const uint32_t ia = L.indices[3 * tri], ib = L.indices[3 * tri + 1], ic = L.indices[3 * tri + 2];
+      const V3 pos = o + d * h.t;  // api/ctx.rs:209-211
+      const V3 pa = load3(L.vertices + 3 * (size_t)ia), pb = load3(L.vertices + 3 * (size_t)ib),
+               pc = load3(L.vertices + 3 * (size_t)ic);
+      const float *m = in.inv;  // normal matrix: transpose of the world-to-object 3x3
+      auto nmat = [m](V3 n) {
+        return v3(m[0] * n.x + m[1] * n.y + m[2] * n.z, m[4] * n.x + m[5] * n.y + m[6] * n.z,
+                  m[8] * n.x + m[9] * n.y + m[10] * n.z);
+      };
+      V3 gn = pt::normalize3(nmat(pt::cross3(pa - pb, pa - pc)));
+      if (pt::dot3(o - pos, gn) < 0.0f) gn = gn * -1.0f;
+      V3 sn = gn;
+      if (L.normals) {
+        const float w = 1.0f - h.u - h.v;
+        const V3 n = load3(L.normals + 3 * (size_t)ia) * w + load3(L.normals + 3 * (size_t)ib) * h.u +
+                     load3(L.normals + 3 * (size_t)ic) * h.v;
+        if (pt::dot3(n, n) > 0.0f) {
+          sn = pt::normalize3(nmat(n));
+          if (pt::dot3(gn, sn) < 0.0f) sn = sn * -1.0f;
+        }
+      }
+      const V3 dir = pt::tbn_apply(sn, load3(L.frame + 20));
+      // payload 1: unoccluded unless the occlusion closest hit writes 0
+      const uint32_t c = spawn_trace(L, idx, kAoClosest, 1u, 1.0f, pos, 0.01f, dir, 100.0f);
+      G.child[idx] = c;
+      G.stage[idx] = 1u;
+      tg::end_poll(L.gr, kAoClosest, idx, false);
+      continue;
+    }
+    float v;
+    if (take_child(L.gr, kAoTrace, G.child[idx], v)) {
+      pw(G, idx)[0] = __float_as_uint(v);
+      tg::end_poll(L.gr, kAoClosest, idx, true);
+    } else {
+      tg::end_poll(L.gr, kAoClosest, idx, false);
+    }
+  }
+}
+
+// ---- groups 3 and 4: store a constant and resolve (ao.rs:299-314) ----
+__global__ void k_store(AoLaunch L, uint32_t gid, float value) {
+  const Group &G = L.gr.g[gid];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += gridDim.x * blockDim.x) {
+    uint32_t idx;
+    if (!tg::poll_entry(G, i, idx)) break;
+    pw(G, idx)[0] = __float_as_uint(value);
+    tg::end_poll(L.gr, gid, idx, true);
+  }
+}
+
+__global__ void k_clear_counts(uint32_t *c) {
+  if (threadIdx.x < 3) c[threadIdx.x] = 0u;
+}
+
+}  // namespace
+
+hipError_t enqueue_ao_tile(const AoLaunch &L, uint32_t tile_h, uint32_t rounds, const TraceParams &scene,
+                           uint32_t push_depth, const TraceTune &tune, uint32_t *work, hipStream_t st) {
+  const Graph &gr = L.gr;
+  hipError_t e = tg::enqueue_reset(gr, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(64), 0, st, L.tcount);
+  e = tg::enqueue_spawn_roots(gr, kAoRayGen, L.tile_w * tile_h, st);
+  if (e != hipSuccess) return e;
+  const uint32_t ntrace = gr.g[kAoTrace].size;
+  TraceParams tp = scene;
+  tp.rays = L.rays;
+  tp.hits = L.hits;
+  tp.hitx = L.hitx;
+  tp.meta = nullptr;
+  tp.hit_meta = nullptr;
+  tp.counters = nullptr;
+  tp.n = ntrace + 1;
+  tp.queue = L.tq0;
+  tp.count = L.tcount;
+  tp.slots = ntrace;
+  tp.flags = RDRT_RAY_FLAG_CULL_BACK_FACING_TRIANGLES;  // ao.rs:209
+  tp.cull_mask = 0xffffffffu;
+  tp.tlas_idx = 0;
+  tp.work = work;
+  tp.rays2 = L.rays;
+  tp.hits2 = L.hits;
+  tp.hitx2 = L.hitx;
+  tp.queue2 = L.tq1;
+  tp.count2 = L.tcount + 1;
+  tp.slots2 = ntrace;
+  tp.flags2 = RDRT_RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH;  // ao.rs:276
+  for (uint32_t r = 0; r < rounds; r++) {
+    for (uint32_t gid = 0; gid < kAoGroups; gid++) {
+      if ((e = tg::enqueue_prepare(gr, gid, st)) != hipSuccess) return e;
+      const uint32_t grid = grid_for(gr.g[gid].size);
+      switch (gid) {
+        case kAoTrace:
+          hipLaunchKernelGGL(k_trace_pre, dim3(grid), dim3(kBlock), 0, st, L);
+          if ((e = launch_trace(tp, RDRT_TRAVERSAL_FAST, push_depth, tune, st)) != hipSuccess) return e;
+          hipLaunchKernelGGL(k_trace_post, dim3(grid), dim3(kBlock), 0, st, L);
+          break;
+        case kAoRayGen:
+          hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(kBlock), 0, st, L);
+          break;
+        case kAoClosest:
+          hipLaunchKernelGGL(k_closest, dim3(grid), dim3(kBlock), 0, st, L);
+          break;
+        case kAoClosestTest:
+          hipLaunchKernelGGL(k_store, dim3(grid), dim3(kBlock), 0, st, L, gid, 0.0f);
+          break;
+        default:
+          hipLaunchKernelGGL(k_store, dim3(grid), dim3(kBlock), 0, st, L, gid, 1.0f);
+          break;
+      }
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      if ((e = tg::enqueue_compact(gr, gid, st)) != hipSuccess) return e;
+    }
+  }
+  return hipSuccess;
+}
+
+}  // namespace rdrt

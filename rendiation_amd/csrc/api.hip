@@ -132,6 +132,7 @@ void drop_captures(rdrt_ctx *c) {
   s.graph = nullptr;
   s.have_cached = false;
   s.last_recorded = false;
+  drop_ao_capture(c->ao);
   BatchState &bs = c->batch;
   if (bs.exec) (void)hipGraphExecDestroy(bs.exec);
   if (bs.graph) (void)hipGraphDestroy(bs.graph);
@@ -215,6 +216,7 @@ int32_t rdrt_ctx_destroy(rdrt_ctx *c) {
     release_refit(c);
     c->pt.free_all();
     c->batch.free_all();
+    free_ao(c->ao);
     c->compact_scratch.free();
     c->trace_work.free();
     c->trace_cull.free();

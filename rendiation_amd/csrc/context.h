@@ -119,6 +119,11 @@ struct BatchState {
 
 void free_device_scene(DeviceScene &d);
 
+// Ambient-occlusion pipeline state of one context (api_ao.hip).
+struct AoState;
+void free_ao(AoState *&s);
+void drop_ao_capture(AoState *s);
+
 }  // namespace rdrt
 
 struct rdrt_ctx {
@@ -151,6 +156,7 @@ struct rdrt_ctx {
   rdrt::DevArray<uint32_t> refit_forest, refit_pairs, refit_quads;
   std::vector<uint32_t> refit_forest_off, refit_pairs_off, refit_quads_off;
   rdrt::Scratch refit_boxes;
+  rdrt::AoState *ao = nullptr;  // created by the first rdrt_ao_render
 };
 
 namespace rdrt {
