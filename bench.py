@@ -822,6 +822,105 @@ def cpu_baseline_batched(scenes, rays_np, meta_np, offs, bounces):
 PMC_STEPS = 3
 
 
+# ------------------------------------------------------- ambient occlusion
+def ambient_occlusion(args):
+    """One AO sample per step of the path-tracing scene at the reference's AO
+    size (the 1920x1080 frame clamped to 512 * 512 pixels, ao.rs:176,
+    lib.rs:44-53), run as the reference's task-graph pipeline."""
+    import numpy as np
+    import torch
+
+    from rendiation_amd import synth
+    from rendiation_amd.acceleration import AccelerationStructureSystem
+    from rendiation_amd.pathtracer import Camera, load_scene
+    from rendiation_amd.task_graph import AORenderer
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    scene = synth.pt_scene(seed=0, n_meshes=64, tess=256)
+    system = AccelerationStructureSystem(device=0, stream=stream.cuda_stream)
+    load_scene(system, scene)
+    info = system.scene_info()
+    ratio = np.sqrt(np.float32(512 * 512) / np.float32(1920 * 1080)).astype(np.float32)
+    W, H = int(np.floor(np.float32(1920) * ratio)), int(np.floor(np.float32(1080) * ratio))
+    cam = Camera(aspect=W / H)
+    r = AORenderer(system, W, H)
+    if args.tune:
+        system.set_tune(*[int(x) for x in args.tune.split(",")])
+        tune = system.tune()
+    elif not args.no_autotune:
+        tune = r.autotune(cam, reps=3)
+    else:
+        tune = system.tune()
+    if args.profile_only:
+        for _ in range(args.steps):
+            r.render(cam)
+        system.sync()
+        system.close()
+        return None
+    for _ in range(args.warmup):
+        r.render(cam)
+    system.sync()
+    r.reset_sample()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        r.render(cam)
+    e1.record(stream)
+    system.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms_ev = e0.elapsed_time(e1) / args.steps
+    ms_per_step = wall * 1e3 / args.steps
+    prim, occ = r.ray_counts()
+    value = (prim + occ) / (ms_per_step * 1e-3) / 1e6
+    st = r.states()
+    # parity on the first sample: a fresh buffer at sample 0 holds the values
+    r.reset_sample()
+    r.buffer.zero_()
+    torch.cuda.synchronize()
+    r.render(cam)
+    system.sync()
+    first = r.buffer.cpu().numpy()
+    rays0 = sum(r.ray_counts())
+    config = {
+        "workload": f"ao: 64 spheres/tori at 256x256 (8.39M tris), {W}x{H} (1920x1080 clamped to 512^2 pixels), "
+                    "1 sample per step, task-graph pipeline (trace, ray gen, 2 closest hit, miss), 8 rounds",
+        "tris": info.tris, "width": W, "height": H, "camera_rays": prim, "occlusion_rays": occ,
+        "ms_events": round(ms_ev, 4), "graph_replay": True, "launch_shape": tune,
+        "drained": st.is_empty() and not any(st.errors),
+    }
+    extra = {"ms_per_sample": round(ms_per_step, 4)}
+    if not args.no_cpu_baseline:
+        extra["cpu_baseline"], extra["parity"] = cpu_baseline_ao(scene, cam, W, H, first, rays0)
+    system.close()
+    return _record(value, "Mrays/s", args, ms_per_step, config, None, extra)
+
+
+def cpu_baseline_ao(scene, cam, W, H, first, rays):
+    """The oracle's AO (reference traversal, all host threads) on the same
+    frame at sample 0: its rays/s (counting the rays the GPU traced for that
+    sample), and the fraction of pixels where the GPU's first sample
+    differs."""
+    import numpy as np
+
+    sys.path.insert(0, str(Path(__file__).resolve().parent))
+    from tests.pt_util import oracle_scene_from
+
+    o = oracle_scene_from(scene)
+    vp = cam.view_projection_inv().T.reshape(-1)
+    cp = [float(np.float32(x)) for x in cam.position]
+    t0 = time.perf_counter()
+    vals = o.ao_render(W, H, 0, vp, cp)
+    dt = time.perf_counter() - t0
+    diff = float((first.reshape(H, W) != vals).mean())
+    return ({"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": _host_threads(), "kind": "port",
+             "sample": f"whole {W}x{H} frame, sample 0"}, {"differing_pixels": round(diff, 6)})
+
+
 def pmc_traffic(workload: str, kernels, extra=()):
     """Physical DRAM-side bytes of `kernel` per step: two rocprofv3 --pmc
     child runs (FETCH_SIZE, then WRITE_SIZE) of this benchmark in
@@ -879,7 +978,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt", choices=["pt", "closest_hit", "batched", "diff", "diffgeo"])
+    ap.add_argument("--workload", default="pt", choices=["pt", "closest_hit", "batched", "diff", "diffgeo", "ao"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 256 MiB device-to-device copy (a reference dispatch for HBM counters)")
@@ -903,7 +1002,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     rec = {"pt": path_tracing, "closest_hit": closest_hit, "batched": batched,
-           "diff": differentiable, "diffgeo": differentiable_geometry}[args.workload](args)
+           "diff": differentiable, "diffgeo": differentiable_geometry, "ao": ambient_occlusion}[args.workload](args)
     if rec is None:
         return
     traffic = None

@@ -298,6 +298,14 @@ int32_t rdrt_ao_render(rdrt_ctx *ctx, const rdrt_ao_params *params, void *accum)
  * rdrt_task_graph_states (5 groups: trace, ray gen, closest hit, occlusion
  * closest hit, miss). */
 int32_t rdrt_ao_states(rdrt_ctx *ctx, uint32_t *wake, uint32_t *sleep_or_finished, uint32_t *empty, uint32_t *errors);
+/* Times every traversal launch shape on eager AO calls with these
+ * parameters (`accum` is overwritten) and keeps the fastest for the
+ * context, as rdrt_pt_autotune does for path-tracing frames. */
+int32_t rdrt_ao_autotune(rdrt_ctx *ctx, const rdrt_ao_params *params, void *accum, uint32_t reps,
+                         struct rdrt_tune *best);
+/* Camera rays and occlusion rays traced by the last rdrt_ao_render (waits
+ * for the stream). */
+int32_t rdrt_ao_ray_counts(rdrt_ctx *ctx, uint32_t out[2]);
 /* The unit-hemisphere direction (tangent space, z = normal) of sample
  * index s: hammersley_2d(s, 256) through sample_hemisphere_cos
  * (shader/library/src/sampling.rs:33-68), sine and cosine rounded from

@@ -169,6 +169,8 @@ SIGNATURES = {
     "rdrt_task_graph_states": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "rdrt_ao_render": (i32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "rdrt_ao_states": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "rdrt_ao_ray_counts": (i32, [C.c_void_p, C.c_void_p]),
+    "rdrt_ao_autotune": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, u32, C.c_void_p]),
     "rdrt_ao_hemisphere_sample": (i32, [u32, C.c_void_p]),
     "rdrt_autotune": (i32, [C.c_void_p, C.POINTER(RdrtTraceArgs), u32, C.POINTER(RdrtTune)]),
     "rdrt_set_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),

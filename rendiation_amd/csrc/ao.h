@@ -38,7 +38,8 @@ struct AoLaunch {
   rdrt_hit *hits;
   uint2 *hitx;
   uint32_t *tq0, *tq1;  // this round's primary / occlusion rays to traverse
-  uint32_t *tcount;     // [0], [1]: their counts; [2]: blocks done
+  uint32_t *tcount;     // [0], [1]: their counts; [2]: blocks done; [3], [4]: primary /
+                        // occlusion rays traced by the call
   const float *frame;   // kAoFrameWords
   uint32_t width, height, tile_x, tile_y, tile_w;
   void *accum;  // f32 per pixel, or RGBA8 per pixel
@@ -51,9 +52,11 @@ struct AoLaunch {
 };
 
 // One tile: reset the graph, spawn tile_w * tile_h ray-gen tasks, run
-// `rounds` rounds (each group: prepare, poll, compact).
+// `rounds` rounds (each group: prepare, poll, compact).  `after` receives
+// the graph with the parities the enqueued launches leave.
 hipError_t enqueue_ao_tile(const AoLaunch &L, uint32_t tile_h, uint32_t rounds, const TraceParams &scene,
-                           uint32_t push_depth, const TraceTune &tune, uint32_t *work, hipStream_t st);
+                           uint32_t push_depth, const TraceTune &tune, uint32_t *work, bool first_tile,
+                           hipStream_t st, tg::Graph *after = nullptr);
 
 }  // namespace rdrt
 

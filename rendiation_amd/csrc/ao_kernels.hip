@@ -37,30 +37,30 @@ __device__ __forceinline__ V3 load3(const float *p) { return v3(p[0], p[1], p[2]
 // [0] tile pixel.  Closest / miss: [0] payload value.
 __device__ __forceinline__ uint32_t *pw(const Group &G, uint32_t idx) { return G.payload + (size_t)idx * G.words; }
 
-// Starts a trace task (then_trace): the ray goes to the trace group's ray
-// array at the new task's slot.
-__device__ __forceinline__ uint32_t spawn_trace(const AoLaunch &L, uint32_t parent, uint32_t ptype, uint32_t ray_type,
-                                                float value, V3 o, float tmin, V3 d, float tmax) {
+// Starts a trace task (then_trace) for the lanes where `pred` holds: the
+// ray goes to the trace group's ray array at the new task's slot.
+__device__ __forceinline__ uint32_t spawn_trace(const AoLaunch &L, bool pred, uint32_t parent, uint32_t ptype,
+                                                uint32_t ray_type, float value, V3 o, float tmin, V3 d, float tmax) {
   const Group &T = L.gr.g[kAoTrace];
-  const uint32_t c = tg::spawn(T, parent, ptype);
-  if (c == tg::kNone) return c;
-  uint32_t *w = pw(T, c);
-  w[0] = ray_type;
-  w[1] = __float_as_uint(value);
-  w[2] = tg::kNone;
-  write_ray(L.rays + c, o, tmin, d, tmax);
+  const uint32_t c = tg::spawn(T, pred, parent, ptype);
+  if (c != tg::kNone) {
+    uint32_t *w = pw(T, c);
+    w[0] = ray_type;
+    w[1] = __float_as_uint(value);
+    w[2] = tg::kNone;
+    write_ray(L.rays + c, o, tmin, d, tmax);
+  }
   return c;
 }
 
-// Reads back a finished child's payload value and frees it; false while it
-// is still running (TaskFuture poll, task_group.rs:464-488).
-__device__ __forceinline__ bool take_child(const Graph &gr, uint32_t cgroup, uint32_t c, float &value) {
-  if (cgroup >= gr.n || c == tg::kNone) return false;
-  const Group &C = gr.g[cgroup];
-  if (C.state[c] != tg::kFinished) return false;
-  value = __uint_as_float(pw(C, c)[cgroup == kAoTrace ? 1 : 0]);
-  tg::cleanup(C, c);
-  return true;
+// For the lanes where `pred` holds: if the awaited child c of group cgroup
+// has finished, read back its payload value and free it (TaskFuture poll,
+// task_group.rs:464-488).  Returns whether it had.
+__device__ __forceinline__ bool take_child(const Graph &gr, bool pred, uint32_t cgroup, uint32_t c, float &value) {
+  const bool done = pred && cgroup < gr.n && c != tg::kNone && gr.g[cgroup].state[c] == tg::kFinished;
+  if (done) value = __uint_as_float(pw(gr.g[cgroup], c)[cgroup == kAoTrace ? 1 : 0]);
+  tg::cleanup_in(gr, done, cgroup, c);
+  return done;
 }
 
 // ---- group 0: trace ----
@@ -68,24 +68,19 @@ __device__ __forceinline__ bool take_child(const Graph &gr, uint32_t cgroup, uin
 // take its payload and resolve; new tasks are queued for the traversal.
 __global__ void k_trace_pre(AoLaunch L) {
   const Group &T = L.gr.g[kAoTrace];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += gridDim.x * blockDim.x) {
-    uint32_t idx;
-    if (!tg::poll_entry(T, i, idx)) break;
+  tg::for_each_polled(T, [&](uint32_t, bool valid, uint32_t idx) {
     uint32_t *w = pw(T, idx);
-    if (T.stage[idx] == 0u) {
-      const uint32_t q = w[0] == 0u ? 0u : 1u;
-      const uint32_t pos = atomicAdd(L.tcount + q, 1u);
-      (q ? L.tq1 : L.tq0)[pos] = idx;
-      continue;
-    }
-    float v;
-    if (take_child(L.gr, w[2], T.child[idx], v)) {
-      w[1] = __float_as_uint(v);
-      tg::end_poll(L.gr, kAoTrace, idx, true);
-    } else {
-      tg::end_poll(L.gr, kAoTrace, idx, false);
-    }
-  }
+    const bool fresh = valid && T.stage[idx] == 0u;
+    const bool occ = fresh && w[0] != 0u;
+    const uint32_t p0 = tg::block_add(L.tcount + 0, fresh && !occ);
+    const uint32_t p1 = tg::block_add(L.tcount + 1, occ);
+    if (fresh) (occ ? L.tq1 : L.tq0)[occ ? p1 : p0] = idx;
+    const bool waiting = valid && !fresh;
+    float v = 0.0f;
+    const bool done = take_child(L.gr, waiting, waiting ? w[2] : tg::kNone, waiting ? T.child[idx] : tg::kNone, v);
+    if (done) w[1] = __float_as_uint(v);
+    tg::end_poll(L.gr, kAoTrace, waiting, idx, done);
+  });
 }
 
 // Second half, after the traversal: each traced task spawns the closest-hit
@@ -96,18 +91,27 @@ __global__ void k_trace_pre(AoLaunch L) {
 __global__ void k_trace_post(AoLaunch L) {
   const Group &T = L.gr.g[kAoTrace];
   const uint32_t n0 = L.tcount[0], n1 = L.tcount[1];
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n0 + n1; j += gridDim.x * blockDim.x) {
-    const uint32_t idx = j < n0 ? L.tq0[j] : L.tq1[j - n0];
+  for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < n0 + n1; j0 += gridDim.x * blockDim.x) {
+    const uint32_t j = j0 + threadIdx.x;
+    const bool valid = j < n0 + n1;
+    const uint32_t idx = !valid ? 0u : (j < n0 ? L.tq0[j] : L.tq1[j - n0]);
     uint32_t *w = pw(T, idx);
-    const bool hit = L.hits[idx].prim_id != RDRT_MISS;
+    const bool hit = valid && L.hits[idx].prim_id != RDRT_MISS;
     const uint32_t cg = hit ? (w[0] == 0u ? kAoClosest : kAoClosestTest) : kAoMiss;
-    const Group &C = L.gr.g[cg];
-    const uint32_t c = tg::spawn(C, idx, kAoTrace);
-    if (c != tg::kNone) pw(C, c)[0] = w[1];
-    w[2] = cg;
-    T.child[idx] = c;
-    T.stage[idx] = 1u;
-    tg::end_poll(L.gr, kAoTrace, idx, false);
+    uint32_t c = tg::kNone;
+    for (uint32_t g = kAoClosest; g <= kAoMiss; g++) {
+      const bool mine = valid && cg == g;
+      if (!__syncthreads_or(mine)) continue;
+      const uint32_t cc = tg::spawn(L.gr.g[g], mine, idx, kAoTrace);
+      if (mine) c = cc;
+    }
+    if (valid) {
+      if (c != tg::kNone) pw(L.gr.g[cg], c)[0] = w[1];
+      w[2] = cg;
+      T.child[idx] = c;
+      T.stage[idx] = 1u;
+    }
+    tg::end_poll(L.gr, kAoTrace, valid, idx, false);
   }
   __shared__ bool last;
   __syncthreads();
@@ -117,6 +121,8 @@ __global__ void k_trace_post(AoLaunch L) {
   }
   __syncthreads();
   if (last && threadIdx.x == 0) {
+    L.tcount[3] += n0;
+    L.tcount[4] += n1;
     L.tcount[0] = 0u;
     L.tcount[1] = 0u;
     L.tcount[2] = 0u;
@@ -159,12 +165,12 @@ __device__ __forceinline__ void accumulate(const AoLaunch &L, uint32_t x, uint32
 __global__ void k_raygen(AoLaunch L) {
   const Group &G = L.gr.g[kAoRayGen];
   const uint32_t s = __float_as_uint(L.frame[23]);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += gridDim.x * blockDim.x) {
-    uint32_t idx;
-    if (!tg::poll_entry(G, i, idx)) break;
-    const uint32_t tp = pw(G, idx)[0];
+  tg::for_each_polled(G, [&](uint32_t, bool valid, uint32_t idx) {
+    const uint32_t tp = valid ? pw(G, idx)[0] : 0u;
     const uint32_t x = L.tile_x + tp % L.tile_w, y = L.tile_y + tp / L.tile_w;
-    if (G.stage[idx] == 0u) {
+    const bool first = valid && G.stage[idx] == 0u;
+    V3 cam = v3(0.0f, 0.0f, 0.0f), d = v3(0.0f, 0.0f, 1.0f);
+    if (first) {
       uint32_t state = pt::xxhash32(x, y, s);
       const float r0 = pcg_next(state);
       const float r1 = pcg_next(state);
@@ -172,106 +178,103 @@ __global__ void k_raygen(AoLaunch L) {
       const float u = (float)x / (float)L.width + r0 / (float)L.width;
       const float v = (float)y / (float)L.height + r1 / (float)L.height;
       const V3 target = pt::unproject(L.frame, u, v);
-      const V3 cam = load3(L.frame + 16);
-      const V3 d = pt::normalize3(target - cam);
-      // payload 0: occluded unless the miss shader writes 1 (ao.rs:216-219)
-      const uint32_t c = spawn_trace(L, idx, kAoRayGen, 0u, 0.0f, cam, kFltEps, d, kFltMax);
+      cam = load3(L.frame + 16);
+      d = pt::normalize3(target - cam);
+    }
+    // payload 0: occluded unless the miss shader writes 1 (ao.rs:216-219)
+    const uint32_t c = spawn_trace(L, first, idx, kAoRayGen, 0u, 0.0f, cam, kFltEps, d, kFltMax);
+    if (first) {
       G.child[idx] = c;
       G.stage[idx] = 1u;
-      tg::end_poll(L.gr, kAoRayGen, idx, false);
-      continue;
     }
-    float payload;
-    if (take_child(L.gr, kAoTrace, G.child[idx], payload)) {
-      accumulate(L, x, y, s, payload);
-      tg::end_poll(L.gr, kAoRayGen, idx, true);
-    } else {
-      tg::end_poll(L.gr, kAoRayGen, idx, false);
-    }
-  }
+    const bool waiting = valid && !first;
+    float payload = 0.0f;
+    const bool done = take_child(L.gr, waiting, kAoTrace, waiting ? G.child[idx] : tg::kNone, payload);
+    if (done) accumulate(L, x, y, s, payload);
+    tg::end_poll(L.gr, kAoRayGen, valid, idx, done);
+  });
 }
 
 // ---- group 2: closest hit of primary rays (ao.rs:252-291) ----
 // World position o + t d, shading normal from the bindless mesh
 // (bindless_mesh_bridge.rs:86-123), the occlusion direction tbn(n) * the
 // frame's cosine-weighted hemisphere sample, range [0.01, 100], first hit.
-__global__ void k_closest(AoLaunch L) {
-  const Group &G = L.gr.g[kAoClosest];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += gridDim.x * blockDim.x) {
-    uint32_t idx;
-    if (!tg::poll_entry(G, i, idx)) break;
-    if (G.stage[idx] == 0u) {
-      const uint32_t p = G.parent[idx];  // the trace task: its ray and hit
-      const rdrt_hit h = L.hits[p];
-      const uint2 hx = L.hitx[p];
-      const float4 *rq = reinterpret_cast<const float4 *>(L.rays + p);
-      const float4 q0 = rq[0], q1 = rq[1];
-      const V3 o = v3(q0.x, q0.y, q0.z), d = v3(q1.x, q1.y, q1.z);
-      const GeoRec &g = L.geos[hx.x];
-      const InstRec &in = L.insts[hx.y];
-      const size_t tri = (size_t)g.primitive_start + h.prim_id;
-      const uint32_t ia = L.indices[3 * tri], ib = L.indices[3 * tri + 1], ic = L.indices[3 * tri + 2];
-      const V3 pos = o + d * h.t;  // api/ctx.rs:209-211
-      const V3 pa = load3(L.vertices + 3 * (size_t)ia), pb = load3(L.vertices + 3 * (size_t)ib),
-               pc = load3(L.vertices + 3 * (size_t)ic);
-      const float *m = in.inv;  // normal matrix: transpose of the world-to-object 3x3
-      auto nmat = [m](V3 n) {
-        return v3(m[0] * n.x + m[1] * n.y + m[2] * n.z, m[4] * n.x + m[5] * n.y + m[6] * n.z,
-                  m[8] * n.x + m[9] * n.y + m[10] * n.z);
-      };
-      V3 gn = pt::normalize3(nmat(pt::cross3(pa - pb, pa - pc)));
-      if (pt::dot3(o - pos, gn) < 0.0f) gn = gn * -1.0f;
-      V3 sn = gn;
-      if (L.normals) {
-        const float w = 1.0f - h.u - h.v;
-        const V3 n = load3(L.normals + 3 * (size_t)ia) * w + load3(L.normals + 3 * (size_t)ib) * h.u +
-                     load3(L.normals + 3 * (size_t)ic) * h.v;
-        if (pt::dot3(n, n) > 0.0f) {
-          sn = pt::normalize3(nmat(n));
-          if (pt::dot3(gn, sn) < 0.0f) sn = sn * -1.0f;
-        }
-      }
-      const V3 dir = pt::tbn_apply(sn, load3(L.frame + 20));
-      // payload 1: unoccluded unless the occlusion closest hit writes 0
-      const uint32_t c = spawn_trace(L, idx, kAoClosest, 1u, 1.0f, pos, 0.01f, dir, 100.0f);
-      G.child[idx] = c;
-      G.stage[idx] = 1u;
-      tg::end_poll(L.gr, kAoClosest, idx, false);
-      continue;
-    }
-    float v;
-    if (take_child(L.gr, kAoTrace, G.child[idx], v)) {
-      pw(G, idx)[0] = __float_as_uint(v);
-      tg::end_poll(L.gr, kAoClosest, idx, true);
-    } else {
-      tg::end_poll(L.gr, kAoClosest, idx, false);
+__device__ __forceinline__ void occlusion_ray(const AoLaunch &L, uint32_t p, V3 &pos, V3 &dir) {
+  const rdrt_hit h = L.hits[p];  // the parent trace task's ray and hit
+  const uint2 hx = L.hitx[p];
+  const float4 *rq = reinterpret_cast<const float4 *>(L.rays + p);
+  const float4 q0 = rq[0], q1 = rq[1];
+  const V3 o = v3(q0.x, q0.y, q0.z), d = v3(q1.x, q1.y, q1.z);
+  const GeoRec &g = L.geos[hx.x];
+  const InstRec &in = L.insts[hx.y];
+  const size_t tri = (size_t)g.primitive_start + h.prim_id;
+  const uint32_t ia = L.indices[3 * tri], ib = L.indices[3 * tri + 1], ic = L.indices[3 * tri + 2];
+  pos = o + d * h.t;  // api/ctx.rs:209-211
+  const V3 pa = load3(L.vertices + 3 * (size_t)ia), pb = load3(L.vertices + 3 * (size_t)ib),
+           pc = load3(L.vertices + 3 * (size_t)ic);
+  const float *m = in.inv;  // normal matrix: transpose of the world-to-object 3x3
+  auto nmat = [m](V3 n) {
+    return v3(m[0] * n.x + m[1] * n.y + m[2] * n.z, m[4] * n.x + m[5] * n.y + m[6] * n.z,
+              m[8] * n.x + m[9] * n.y + m[10] * n.z);
+  };
+  V3 gn = pt::normalize3(nmat(pt::cross3(pa - pb, pa - pc)));
+  if (pt::dot3(o - pos, gn) < 0.0f) gn = gn * -1.0f;
+  V3 sn = gn;
+  if (L.normals) {
+    const float w = 1.0f - h.u - h.v;
+    const V3 n = load3(L.normals + 3 * (size_t)ia) * w + load3(L.normals + 3 * (size_t)ib) * h.u +
+                 load3(L.normals + 3 * (size_t)ic) * h.v;
+    if (pt::dot3(n, n) > 0.0f) {
+      sn = pt::normalize3(nmat(n));
+      if (pt::dot3(gn, sn) < 0.0f) sn = sn * -1.0f;
     }
   }
+  dir = pt::tbn_apply(sn, load3(L.frame + 20));
+}
+
+__global__ void k_closest(AoLaunch L) {
+  const Group &G = L.gr.g[kAoClosest];
+  tg::for_each_polled(G, [&](uint32_t, bool valid, uint32_t idx) {
+    const bool first = valid && G.stage[idx] == 0u;
+    V3 pos = v3(0.0f, 0.0f, 0.0f), dir = v3(0.0f, 0.0f, 1.0f);
+    if (first) occlusion_ray(L, G.parent[idx], pos, dir);
+    // payload 1: unoccluded unless the occlusion closest hit writes 0
+    const uint32_t c = spawn_trace(L, first, idx, kAoClosest, 1u, 1.0f, pos, 0.01f, dir, 100.0f);
+    if (first) {
+      G.child[idx] = c;
+      G.stage[idx] = 1u;
+    }
+    const bool waiting = valid && !first;
+    float v = 0.0f;
+    const bool done = take_child(L.gr, waiting, kAoTrace, waiting ? G.child[idx] : tg::kNone, v);
+    if (done) pw(G, idx)[0] = __float_as_uint(v);
+    tg::end_poll(L.gr, kAoClosest, valid, idx, done);
+  });
 }
 
 // ---- groups 3 and 4: store a constant and resolve (ao.rs:299-314) ----
 __global__ void k_store(AoLaunch L, uint32_t gid, float value) {
   const Group &G = L.gr.g[gid];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += gridDim.x * blockDim.x) {
-    uint32_t idx;
-    if (!tg::poll_entry(G, i, idx)) break;
-    pw(G, idx)[0] = __float_as_uint(value);
-    tg::end_poll(L.gr, gid, idx, true);
-  }
+  tg::for_each_polled(G, [&](uint32_t, bool valid, uint32_t idx) {
+    if (valid) pw(G, idx)[0] = __float_as_uint(value);
+    tg::end_poll(L.gr, gid, valid, idx, true);
+  });
 }
 
-__global__ void k_clear_counts(uint32_t *c) {
-  if (threadIdx.x < 3) c[threadIdx.x] = 0u;
+__global__ void k_clear_counts(uint32_t *c, uint32_t n) {
+  if (threadIdx.x < n) c[threadIdx.x] = 0u;
 }
 
 }  // namespace
 
-hipError_t enqueue_ao_tile(const AoLaunch &L, uint32_t tile_h, uint32_t rounds, const TraceParams &scene,
-                           uint32_t push_depth, const TraceTune &tune, uint32_t *work, hipStream_t st) {
-  const Graph &gr = L.gr;
+hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds, const TraceParams &scene,
+                           uint32_t push_depth, const TraceTune &tune, uint32_t *work, bool first_tile,
+                           hipStream_t st, tg::Graph *after) {
+  AoLaunch L = L0;  // its graph's parities advance as launches are enqueued
+  Graph &gr = L.gr;
   hipError_t e = tg::enqueue_reset(gr, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(64), 0, st, L.tcount);
+  hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(64), 0, st, L.tcount, first_tile ? 5u : 3u);
   e = tg::enqueue_spawn_roots(gr, kAoRayGen, L.tile_w * tile_h, st);
   if (e != hipSuccess) return e;
   const uint32_t ntrace = gr.g[kAoTrace].size;
@@ -285,7 +288,9 @@ hipError_t enqueue_ao_tile(const AoLaunch &L, uint32_t tile_h, uint32_t rounds, 
   tp.n = ntrace + 1;
   tp.queue = L.tq0;
   tp.count = L.tcount;
-  tp.slots = ntrace;
+  // each queue holds at most one ray per pixel of the tile
+  const uint32_t per_queue = L.tile_w * tile_h;
+  tp.slots = per_queue;
   tp.flags = RDRT_RAY_FLAG_CULL_BACK_FACING_TRIANGLES;  // ao.rs:209
   tp.cull_mask = 0xffffffffu;
   tp.tlas_idx = 0;
@@ -295,35 +300,36 @@ hipError_t enqueue_ao_tile(const AoLaunch &L, uint32_t tile_h, uint32_t rounds, 
   tp.hitx2 = L.hitx;
   tp.queue2 = L.tq1;
   tp.count2 = L.tcount + 1;
-  tp.slots2 = ntrace;
+  tp.slots2 = per_queue;
   tp.flags2 = RDRT_RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH;  // ao.rs:276
-  for (uint32_t r = 0; r < rounds; r++) {
-    for (uint32_t gid = 0; gid < kAoGroups; gid++) {
-      if ((e = tg::enqueue_prepare(gr, gid, st)) != hipSuccess) return e;
-      const uint32_t grid = grid_for(gr.g[gid].size);
-      switch (gid) {
-        case kAoTrace:
-          hipLaunchKernelGGL(k_trace_pre, dim3(grid), dim3(kBlock), 0, st, L);
-          if ((e = launch_trace(tp, RDRT_TRAVERSAL_FAST, push_depth, tune, st)) != hipSuccess) return e;
-          hipLaunchKernelGGL(k_trace_post, dim3(grid), dim3(kBlock), 0, st, L);
-          break;
-        case kAoRayGen:
-          hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(kBlock), 0, st, L);
-          break;
-        case kAoClosest:
-          hipLaunchKernelGGL(k_closest, dim3(grid), dim3(kBlock), 0, st, L);
-          break;
-        case kAoClosestTest:
-          hipLaunchKernelGGL(k_store, dim3(grid), dim3(kBlock), 0, st, L, gid, 0.0f);
-          break;
-        default:
-          hipLaunchKernelGGL(k_store, dim3(grid), dim3(kBlock), 0, st, L, gid, 1.0f);
-          break;
+  auto poll = [&](uint32_t gid) -> hipError_t {
+    const uint32_t grid = grid_for(gr.g[gid].size);
+    switch (gid) {
+      case kAoTrace: {
+        hipLaunchKernelGGL(k_trace_pre, dim3(grid), dim3(kBlock), 0, st, L);
+        hipError_t e = launch_trace(tp, RDRT_TRAVERSAL_FAST, push_depth, tune, st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_trace_post, dim3(grid), dim3(kBlock), 0, st, L);
+        break;
       }
-      if ((e = hipGetLastError()) != hipSuccess) return e;
-      if ((e = tg::enqueue_compact(gr, gid, st)) != hipSuccess) return e;
+      case kAoRayGen:
+        hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(kBlock), 0, st, L);
+        break;
+      case kAoClosest:
+        hipLaunchKernelGGL(k_closest, dim3(grid), dim3(kBlock), 0, st, L);
+        break;
+      case kAoClosestTest:
+        hipLaunchKernelGGL(k_store, dim3(grid), dim3(kBlock), 0, st, L, gid, 0.0f);
+        break;
+      default:
+        hipLaunchKernelGGL(k_store, dim3(grid), dim3(kBlock), 0, st, L, gid, 1.0f);
+        break;
     }
-  }
+    return hipGetLastError();
+  };
+  e = tg::enqueue_rounds(gr, rounds, poll, st);
+  if (e != hipSuccess) return e;
+  if (after) *after = gr;
   return hipSuccess;
 }
 

@@ -16,6 +16,8 @@ namespace rdrt {
 
 struct AoState {
   tg::GraphMemory mem;
+  tg::Graph call_end{};   // the graph as a call's launches leave it
+  tg::Graph last{};       // call_end of the last call, advanced by rdrt_ao_states
   uint32_t capacity = 0;  // pixels per tile the buffers hold
   Scratch rays, hits, hitx, tq, counts, frame, work;
   float *host_frame = nullptr;  // pinned staging of the frame block
@@ -54,7 +56,7 @@ namespace {
 
 // Counters of every group after a prepare and a compaction of each
 // (read_back_execution_states, mod.rs:305-370).
-int32_t read_states(const tg::Graph &gr, hipStream_t st, uint32_t *wake, uint32_t *sof, uint32_t *empty,
+int32_t read_states(tg::Graph &gr, hipStream_t st, uint32_t *wake, uint32_t *sof, uint32_t *empty,
                     uint32_t *errors) {
   for (uint32_t k = 0; k < gr.n; k++) {
     RDRT_HIP(tg::enqueue_prepare(gr, k, st));
@@ -67,7 +69,7 @@ int32_t read_states(const tg::Graph &gr, hipStream_t st, uint32_t *wake, uint32_
   RDRT_HIP(hipStreamSynchronize(st));
   for (uint32_t k = 0; k < gr.n; k++) {
     const uint32_t *c = ctr.data() + (size_t)k * tg::kCtrWords;
-    const uint32_t w = c[tg::kActive], e = c[tg::kEmpty];
+    const uint32_t w = c[tg::kActive0 + gr.g[k].cur], e = c[tg::kEmpty0 + gr.g[k].pp];
     if (wake) wake[k] = w;
     if (empty) empty[k] = e;
     if (sof) sof[k] = gr.g[k].size - (e + w <= gr.g[k].size ? e + w : gr.g[k].size);
@@ -112,7 +114,7 @@ int32_t reserve_ao(rdrt_ctx *c, uint32_t cap) {
   RDRT_HIP(s.hits.reserve((size_t)(ntrace + 1) * sizeof(rdrt_hit)));
   RDRT_HIP(s.hitx.reserve((size_t)(ntrace + 1) * sizeof(uint2)));
   RDRT_HIP(s.tq.reserve((size_t)2 * ntrace * 4));
-  RDRT_HIP(s.counts.reserve(4 * 4));
+  RDRT_HIP(s.counts.reserve(8 * 4));
   RDRT_HIP(s.frame.reserve(kAoFrameWords * sizeof(float)));
   RDRT_HIP(s.work.reserve(4));
   s.capacity = cap;
@@ -188,12 +190,10 @@ int32_t rdrt_task_graph_execute(rdrt_task_graph *g, uint32_t rounds) {
   int32_t rc = need_device(g->ctx);
   if (rc) return rc;
   hipStream_t st = g->ctx->work_stream;
-  for (uint32_t r = 0; r < rounds; r++)
-    for (uint32_t k = 0; k < g->mem.dev.n; k++) {
-      RDRT_HIP(tg::enqueue_prepare(g->mem.dev, k, st));
-      RDRT_HIP(tg::enqueue_generic_poll(g->mem.dev, k, g->kinds.data(), st));
-      RDRT_HIP(tg::enqueue_compact(g->mem.dev, k, st));
-    }
+  tg::Graph &gr = g->mem.dev;
+  const tg::KindDesc *kinds = g->kinds.data();
+  RDRT_HIP(tg::enqueue_rounds(gr, rounds, [&](uint32_t gid) { return tg::enqueue_generic_poll(gr, gid, kinds, st); },
+                              st));
   return RDRT_OK;
 }
 
@@ -272,13 +272,14 @@ int32_t rdrt_ao_render(rdrt_ctx *c, const rdrt_ao_params *prm, void *accum) {
       AoLaunch T = L;
       T.tile_y = y0;
       const uint32_t h = prm->height - y0 < rows ? prm->height - y0 : rows;
-      hipError_t e = enqueue_ao_tile(T, h, rounds, tp, depth, c->tune, work, st);
+      hipError_t e = enqueue_ao_tile(T, h, rounds, tp, depth, c->tune, work, y0 == 0, st, &s.call_end);
       if (e != hipSuccess) return e;
     }
     return hipSuccess;
   };
   if (!prm->use_graph) {
     RDRT_HIP(enqueue_all());
+    s.last = s.call_end;
     return RDRT_OK;
   }
   std::vector<uint8_t> key(sizeof(L) + sizeof(tp) + sizeof(TraceTune) + 3 * 4);
@@ -303,6 +304,85 @@ int32_t rdrt_ao_render(rdrt_ctx *c, const rdrt_ao_params *prm, void *accum) {
     s.key = key;
   }
   RDRT_HIP(hipGraphLaunch(s.exec, st));
+  s.last = s.call_end;
+  return RDRT_OK;
+}
+
+int32_t rdrt_ao_autotune(rdrt_ctx *c, const rdrt_ao_params *prm, void *accum, uint32_t reps, struct rdrt_tune *best) {
+  if (!c || !prm || !accum) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  if (reps == 0) reps = 3;
+  rdrt_ao_params q = *prm;
+  q.use_graph = 0;  // eager calls: no capture per shape, the same launches
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  RDRT_HIP(hipEventCreate(&e0));
+  hipError_t he = hipEventCreate(&e1);
+  if (he != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(RDRT_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(he));
+  }
+  const TraceTune before = c->tune;
+  TraceTune chosen = before;
+  float chosen_ms = -1.0f;
+  for (uint32_t block : {64u, 128u, 256u})
+    for (uint32_t lds : {8u, 16u})
+      for (uint32_t waves : {0u, 8u})
+        for (uint32_t shape = 0; shape < 6 && rc == RDRT_OK; shape++) {
+          TraceTune t;
+          t.block = block;
+          t.lds = lds;
+          t.waves = waves;
+          t.persist = shape & 1u;
+          t.wide = shape >> 1;
+          if (stack_entries_for(push_depth(c, t)) == 0) continue;
+          c->tune = t;
+          rc = rdrt_ao_render(c, &q, accum);  // warm-up
+          if (rc == RDRT_OK && hipEventRecord(e0, c->work_stream) != hipSuccess) rc = fail(RDRT_ERR_HIP, "event");
+          for (uint32_t r = 0; r < reps && rc == RDRT_OK; r++) rc = rdrt_ao_render(c, &q, accum);
+          if (rc != RDRT_OK) break;
+          float ms = 0.0f;
+          if (hipEventRecord(e1, c->work_stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(RDRT_ERR_HIP, "AO timing failed");
+            break;
+          }
+          ms /= (float)reps;
+          if (chosen_ms < 0.0f || ms < chosen_ms) {
+            chosen_ms = ms;
+            chosen = t;
+          }
+        }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != RDRT_OK) {
+    c->tune = before;
+    return rc;
+  }
+  c->tune = chosen;
+  c->tune_ms = chosen_ms;
+  if (best) {
+    best->block_size = chosen.block;
+    best->lds_entries = chosen.lds;
+    best->waves = chosen.waves;
+    best->persistent = chosen.persist;
+    best->cull = chosen.cull;
+    best->wide = chosen.wide;
+    best->ms = chosen_ms;
+  }
+  return RDRT_OK;
+}
+
+int32_t rdrt_ao_ray_counts(rdrt_ctx *c, uint32_t out[2]) {
+  if (!c || !out) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  if (!c->ao || !c->ao->capacity) return fail(RDRT_ERR_STATE, "no AO frame rendered yet");
+  uint32_t h[5];
+  RDRT_HIP(hipMemcpyAsync(h, c->ao->counts.ptr, sizeof(h), hipMemcpyDeviceToHost, c->work_stream));
+  RDRT_HIP(hipStreamSynchronize(c->work_stream));
+  out[0] = h[3];
+  out[1] = h[4];
   return RDRT_OK;
 }
 
@@ -311,7 +391,7 @@ int32_t rdrt_ao_states(rdrt_ctx *c, uint32_t *wake, uint32_t *sof, uint32_t *emp
   int32_t rc = need_device(c);
   if (rc) return rc;
   if (!c->ao || !c->ao->capacity) return fail(RDRT_ERR_STATE, "no AO frame rendered yet");
-  return read_states(c->ao->mem.dev, c->work_stream, wake, sof, empty, errors);
+  return read_states(c->ao->last, c->work_stream, wake, sof, empty, errors);
 }
 
 }  // extern "C"

@@ -33,17 +33,18 @@ constexpr uint32_t kSleep = 3;      // pending, out of the active list
 constexpr uint32_t kFinished = 4;   // resolved, waiting for its parent to read it back
 constexpr uint32_t kNone = 0xffffffffu;  // no parent / no task
 
-// Counter words of a group.
+// Counter words of a group.  The active, empty and removed counts are
+// double-buffered: a prepare or a compaction reads the current word and
+// writes the other one, and the host flips the group's parities as it
+// enqueues, so no launch needs a cross-block handshake to commit a count.
 enum Ctr : uint32_t {
-  kActive = 0,    // entries in the current active list
-  kEmpty = 1,     // entries in the empty stack
-  kRemoved = 2,   // slots freed since the group last ran
-  kSnapshot = 3,  // active entries the current poll covers
-  kCur = 4,       // which active buffer is current (0 / 1)
-  kDone = 5,      // blocks finished (last-block commit of prepare / compact)
-  kNext = 6,      // entries written to the other active buffer by compaction
+  kActive0 = 0,   // entries in active buffer 0 / 1
+  kEmpty0 = 2,    // entries in the empty stack (parity pp)
+  kRemoved0 = 4,  // slots freed since the group last ran (parity pp)
+  kSnapshot = 6,  // active entries the current poll covers
   kError = 7,     // bit 0: empty stack exhausted; bit 1: active list full; bit 2: removed list full
-  kCtrWords = 8,
+  kDone = 8,      // blocks finished (last-block commit of the root spawn)
+  kCtrWords = 12,
 };
 
 constexpr uint32_t kMaxGroups = 8;
@@ -56,6 +57,7 @@ struct Group {
   uint32_t *empty, *removed;                         // size each
   uint32_t *ctr;
   uint32_t size, words;
+  uint32_t cur, pp;  // host-kept parities: current active buffer; empty / removed counts
 };
 
 struct Graph {
@@ -63,84 +65,206 @@ struct Graph {
   uint32_t n;
 };
 
+__device__ __forceinline__ uint32_t *active_count(const Group &G) { return G.ctr + kActive0 + G.cur; }
+__device__ __forceinline__ uint32_t *empty_count(const Group &G) { return G.ctr + kEmpty0 + G.pp; }
+__device__ __forceinline__ uint32_t *removed_count(const Group &G) { return G.ctr + kRemoved0 + G.pp; }
+
 __device__ __forceinline__ uint32_t *active_list(const Group &G, uint32_t cur) { return cur ? G.list1 : G.list0; }
 
-__device__ __forceinline__ void push_active(const Group &G, uint32_t idx) {
-  const uint32_t cur = __hip_atomic_load(G.ctr + kCur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const uint32_t pos = atomicAdd(G.ctr + kActive, 1u);
-  if (pos < G.size) active_list(G, cur)[pos] = idx;
-  else atomicOr(G.ctr + kError, 2u);
+// ---- block-level list operations ----
+// Every function below is called by all threads of a block together (in
+// block-uniform control flow), each thread with its own predicate; a list's
+// counter then takes one atomic per block instead of one per thread.  Blocks
+// have at most kMaxBlock threads.
+
+constexpr uint32_t kMaxBlock = 256;
+
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+__device__ __forceinline__ uint32_t rank_in(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// spawn_new_task (task_group.rs:428-446, task_pool.rs:143-172): take an
-// empty slot, make it a woken task with the given parent and zeroed poll
-// state, and append it to the active list.  Returns the slot, or kNone when
-// the pool is full (error bit 0).  The caller writes the payload.
-__device__ __forceinline__ uint32_t spawn(const Group &G, uint32_t parent, uint32_t ptype) {
-  const uint32_t old = atomicSub(G.ctr + kEmpty, 1u);
-  if (old == 0u || old > G.size) {
-    atomicAdd(G.ctr + kEmpty, 1u);
-    atomicOr(G.ctr + kError, 1u);
-    return kNone;
+// This thread's rank among the block's threads where `pred` holds, and
+// their number (`total`, the same in every thread).
+__device__ __forceinline__ uint32_t block_rank(bool pred, uint32_t &total) {
+  __shared__ uint32_t wave_off[kMaxBlock / 64 + 1];
+  const uint64_t m = __ballot(pred);
+  const uint32_t wid = threadIdx.x >> 6;
+  const uint32_t nw = (blockDim.x + 63) >> 6;
+  if (lane_id() == 0) wave_off[wid] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < nw; w++) {
+      const uint32_t c = wave_off[w];
+      wave_off[w] = t;
+      t += c;
+    }
+    wave_off[nw] = t;
   }
-  const uint32_t idx = G.empty[old - 1u];
-  G.state[idx] = kWaken;
-  G.parent[idx] = parent;
-  G.ptype[idx] = ptype;
-  G.stage[idx] = 0u;
-  G.child[idx] = kNone;
-  push_active(G, idx);
+  __syncthreads();
+  const uint32_t r = wave_off[wid] + rank_in(m);
+  total = wave_off[nw];
+  __syncthreads();  // the array is reused by the next call
+  return r;
+}
+
+// Old counter value + this thread's rank among the threads where `pred`
+// holds (meaningful for those threads only).
+__device__ __forceinline__ uint32_t block_add(uint32_t *ctr, bool pred) {
+  __shared__ uint32_t base;
+  uint32_t total;
+  const uint32_t r = block_rank(pred, total);
+  if (total == 0) return 0u;
+  if (threadIdx.x == 0) base = atomicAdd(ctr, total);
+  __syncthreads();
+  const uint32_t b = base;
+  __syncthreads();
+  return b + r;
+}
+
+__device__ __forceinline__ void push_active(const Group &G, bool pred, uint32_t idx) {
+  const uint32_t pos = block_add(active_count(G), pred);
+  if (pred) {
+    if (pos < G.size) active_list(G, G.cur)[pos] = idx;
+    else atomicOr(G.ctr + kError, 2u);
+  }
+}
+
+// Pops one empty slot per thread where `pred` holds; kNone where the stack
+// ran out (error bit 0).  Pops never race with pushes: freed slots return to
+// the stack only in the group's prepare.
+__device__ __forceinline__ uint32_t pop_empty(const Group &G, bool pred) {
+  __shared__ int old_sh;
+  uint32_t total;
+  const int r = (int)block_rank(pred, total);
+  if (total == 0) return kNone;
+  const int k = (int)total;
+  if (threadIdx.x == 0) {
+    const int old = (int)atomicSub(empty_count(G), (uint32_t)k);
+    const int got = old < 0 ? 0 : (old < k ? old : k);
+    if (got < k) {
+      atomicAdd(empty_count(G), (uint32_t)(k - got));
+      atomicOr(G.ctr + kError, 1u);
+    }
+    old_sh = old;
+  }
+  __syncthreads();
+  const int old = old_sh;
+  __syncthreads();
+  if (!pred || r >= old) return kNone;
+  return G.empty[old - 1 - r];
+}
+
+// spawn_new_task (task_group.rs:428-446, task_pool.rs:143-172): an empty
+// slot becomes a woken task with the given parent and zeroed poll state, at
+// the end of the active list.  Returns the slot or kNone; the caller writes
+// the payload.
+__device__ __forceinline__ uint32_t spawn(const Group &G, bool pred, uint32_t parent, uint32_t ptype) {
+  const uint32_t idx = pop_empty(G, pred);
+  const bool ok = idx != kNone;
+  if (ok) {
+    G.state[idx] = kWaken;
+    G.parent[idx] = parent;
+    G.ptype[idx] = ptype;
+    G.stage[idx] = 0u;
+    G.child[idx] = kNone;
+  }
+  push_active(G, ok, idx);
   return idx;
 }
 
-// wake_task_dyn (task_group.rs:448-462): a task still in the active list
-// (going to sleep in this very round) is only marked awake.
-__device__ __forceinline__ void wake(const Group &G, uint32_t idx) {
-  const uint32_t was = atomicExch(G.state + idx, kWaken);
-  if (was != kGoToSleep) push_active(G, idx);
+// wake_task_dyn (task_group.rs:448-462) of task pidx of group pg (per lane):
+// a task still in its active list (going to sleep in this very round) is
+// only marked awake.
+__device__ __forceinline__ void wake(const Graph &gr, bool pred, uint32_t pg, uint32_t pidx) {
+  bool push = false;
+  if (pred && pg < gr.n) push = atomicExch(gr.g[pg].state + pidx, kWaken) != kGoToSleep;
+  for (uint32_t g = 0; g < gr.n; g++) {
+    const bool mine = push && pg == g;
+    if (__syncthreads_or(mine)) push_active(gr.g[g], mine, pidx);
+  }
 }
 
 // cleanup_finished_task_state_and_payload (task_group.rs:490-498).
-__device__ __forceinline__ void cleanup(const Group &G, uint32_t idx) {
-  const uint32_t pos = atomicAdd(G.ctr + kRemoved, 1u);
-  if (pos < G.size) G.removed[pos] = idx;
-  else atomicOr(G.ctr + kError, 4u);
-  G.state[idx] = kNotExist;
+__device__ __forceinline__ void cleanup(const Group &G, bool pred, uint32_t idx) {
+  const uint32_t pos = block_add(removed_count(G), pred);
+  if (pred) {
+    if (pos < G.size) G.removed[pos] = idx;
+    else atomicOr(G.ctr + kError, 4u);
+    G.state[idx] = kNotExist;
+  }
+}
+// The same for a per-lane group.
+__device__ __forceinline__ void cleanup_in(const Graph &gr, bool pred, uint32_t g_of_lane, uint32_t idx) {
+  for (uint32_t g = 0; g < gr.n; g++) {
+    const bool mine = pred && g_of_lane == g;
+    if (__syncthreads_or(mine)) cleanup(gr.g[g], mine, idx);
+  }
 }
 
 // The end of a poll (task_group.rs:118-150): a resolved task finishes and
 // wakes its parent (or frees itself when it has none); a pending one goes
-// to sleep.
-__device__ __forceinline__ void end_poll(const Graph &gr, uint32_t gid, uint32_t idx, bool resolved) {
+// to sleep.  `valid` is false for lanes past the end of the work.
+__device__ __forceinline__ void end_poll(const Graph &gr, uint32_t gid, bool valid, uint32_t idx, bool resolved) {
   const Group &G = gr.g[gid];
-  if (!resolved) {
-    G.state[idx] = kGoToSleep;
-    return;
+  uint32_t p = kNone, pt = kNone;
+  if (valid) {
+    if (resolved) {
+      G.state[idx] = kFinished;
+      p = G.parent[idx];
+      pt = G.ptype[idx];
+    } else {
+      G.state[idx] = kGoToSleep;
+    }
   }
-  G.state[idx] = kFinished;
-  const uint32_t p = G.parent[idx], pt = G.ptype[idx];
-  if (p == kNone) cleanup(G, idx);
-  else if (pt < gr.n) wake(gr.g[pt], p);
+  const bool fin = valid && resolved;
+  cleanup(G, fin && p == kNone, idx);
+  wake(gr, fin && p != kNone, pt, p);
 }
 
-// The poll of one active entry: thread i < snapshot polls list[i].
-__device__ __forceinline__ bool poll_entry(const Group &G, uint32_t i, uint32_t &idx) {
+// Block-uniform loop over the entries a poll covers: body(i, valid, idx).
+template <class F>
+__device__ __forceinline__ void for_each_polled(const Group &G, F body) {
   const uint32_t snap = G.ctr[kSnapshot];
-  if (i >= snap) return false;
-  idx = active_list(G, G.ctr[kCur])[i];
-  return true;
+  const uint32_t *list = active_list(G, G.cur);
+  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < snap; i0 += gridDim.x * blockDim.x) {
+    const uint32_t i = i0 + threadIdx.x;
+    const bool valid = i < snap;
+    body(i, valid, valid ? list[i] : 0u);
+  }
 }
 
 // ---- host side ----
 
-// Enqueue helpers (task_graph.hip).  `grid_cap` bounds the blocks of the
-// list kernels.
-hipError_t enqueue_reset(const Graph &gr, hipStream_t st);
-hipError_t enqueue_prepare(const Graph &gr, uint32_t gid, hipStream_t st);
-hipError_t enqueue_compact(const Graph &gr, uint32_t gid, hipStream_t st);
+// Enqueue helpers (task_graph.hip).  Those that take a mutable Graph flip
+// the parities of the groups they prepare or compact; later launches must
+// be given the updated Graph.
+hipError_t enqueue_reset(Graph &gr, hipStream_t st);
+hipError_t enqueue_prepare(Graph &gr, uint32_t gid, hipStream_t st);
+hipError_t enqueue_compact(Graph &gr, uint32_t gid, hipStream_t st);
+// The compaction of group gc followed by the prepare of group gp (either
+// kNone), in one launch when they are different groups.
+hipError_t enqueue_list(Graph &gr, uint32_t gc, uint32_t gp, hipStream_t st);
+
+// `rounds` rounds of execute (mod.rs:396-460): per group, prepare, then
+// poll(gid) (which enqueues the group's poll launches), then compact; each
+// compaction shares its launch with the next group's prepare.
+template <class PollFn>
+hipError_t enqueue_rounds(Graph &gr, uint32_t rounds, PollFn poll, hipStream_t st) {
+  uint32_t pending = kNone;  // the group whose compaction is still due
+  for (uint32_t r = 0; r < rounds; r++)
+    for (uint32_t gid = 0; gid < gr.n; gid++) {
+      hipError_t e = enqueue_list(gr, pending, gid, st);
+      if (e == hipSuccess) e = poll(gid);
+      if (e != hipSuccess) return e;
+      pending = gid;
+    }
+  return enqueue_list(gr, pending, kNone, st);
+}
 // dispatch_allocate_init_task (mod.rs:223-303) for the generic task kinds:
 // `count` root tasks whose payload word 0 is their spawn index.
-hipError_t enqueue_spawn_roots(const Graph &gr, uint32_t gid, uint32_t count, hipStream_t st);
+hipError_t enqueue_spawn_roots(Graph &gr, uint32_t gid, uint32_t count, hipStream_t st);
 
 // Generic task kinds (the C-ABI task graph): kind 0 resolves on its first
 // poll; kind 1 spawns one task of group `child_group` with payload

@@ -9,10 +9,11 @@ namespace {
 
 constexpr uint32_t kBlock = 256;
 constexpr uint32_t kMaxGrid = 1024;
+constexpr uint32_t kListGrid = 256;  // list kernels: one block per CU at most
 
-uint32_t grid_for(uint32_t n) {
+uint32_t grid_for(uint32_t n, uint32_t cap = kMaxGrid) {
   const uint32_t g = (n + kBlock - 1) / kBlock;
-  return g < 1u ? 1u : (g > kMaxGrid ? kMaxGrid : g);
+  return g < 1u ? 1u : (g > cap ? cap : g);
 }
 
 // true in thread 0 of the block that finished last (after a fence), so it
@@ -38,52 +39,78 @@ __global__ void k_reset(Graph gr) {
       G.state[i] = kNotExist;
       if (i < G.size) G.empty[i] = G.size - i;
     }
-    if (blockIdx.x == 0 && threadIdx.x < kCtrWords) G.ctr[threadIdx.x] = threadIdx.x == kEmpty ? G.size : 0u;
+    if (blockIdx.x == 0 && threadIdx.x < kCtrWords) G.ctr[threadIdx.x] = threadIdx.x == kEmpty0 ? G.size : 0u;
   }
 }
 
 // prepare_execution (task_group.rs:213-233): slots freed since the group
-// last ran return to the empty stack; the entries to poll are fixed.
-__global__ void k_prepare(Graph gr, uint32_t gid) {
-  const Group &G = gr.g[gid];
-  const uint32_t ne = G.ctr[kEmpty], nr = G.ctr[kRemoved];
+// last ran return to the empty stack; the entries to poll are fixed.  Reads
+// the counts of parity pp, writes those of parity 1 - pp (one writer).
+__device__ __forceinline__ void prepare_part(const Group &G) {
+  const uint32_t ne = *empty_count(G), nr = *removed_count(G);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nr; i += gridDim.x * blockDim.x)
     if (ne + i < G.size) G.empty[ne + i] = G.removed[i];
-  if (last_block(G.ctr + kDone)) {
-    G.ctr[kEmpty] = ne + nr < G.size ? ne + nr : G.size;
-    G.ctr[kRemoved] = 0u;
-    G.ctr[kSnapshot] = G.ctr[kActive] < G.size ? G.ctr[kActive] : G.size;
-    G.ctr[kDone] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const uint32_t na = *active_count(G);
+    G.ctr[kEmpty0 + (G.pp ^ 1u)] = ne + nr < G.size ? ne + nr : G.size;
+    G.ctr[kRemoved0 + (G.pp ^ 1u)] = 0u;
+    G.ctr[kSnapshot] = na < G.size ? na : G.size;
+    G.ctr[kActive0 + (G.cur ^ 1u)] = 0u;  // the compaction's target
   }
 }
 
 // use_compact_alive_tasks (task_group.rs:168-210, dispatch_compact.rs:60-98):
 // awake tasks move to the other buffer, tasks going to sleep leave it.
-__global__ void k_compact(Graph gr, uint32_t gid) {
-  const Group &G = gr.g[gid];
-  const uint32_t cur = G.ctr[kCur];
-  const uint32_t n = G.ctr[kActive] < G.size ? G.ctr[kActive] : G.size;
-  const uint32_t *src = active_list(G, cur);
-  uint32_t *dst = active_list(G, cur ^ 1u);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t idx = src[i];
-    const uint32_t s = G.state[idx];
+__device__ __forceinline__ void compact_part(const Group &G) {
+  const uint32_t n = *active_count(G) < G.size ? *active_count(G) : G.size;
+  const uint32_t *src = active_list(G, G.cur);
+  uint32_t *dst = active_list(G, G.cur ^ 1u);
+  uint32_t *dst_count = G.ctr + kActive0 + (G.cur ^ 1u);
+  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+    const uint32_t i = i0 + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t idx = valid ? src[i] : 0u;
+    const uint32_t s = valid ? G.state[idx] : kNotExist;
     if (s == kGoToSleep) G.state[idx] = kSleep;
-    if (s == kWaken) dst[atomicAdd(G.ctr + kNext, 1u)] = idx;
-  }
-  if (last_block(G.ctr + kDone)) {
-    G.ctr[kActive] = G.ctr[kNext];
-    G.ctr[kNext] = 0u;
-    G.ctr[kCur] = cur ^ 1u;
-    G.ctr[kDone] = 0u;
+    const bool keep = s == kWaken;
+    const uint32_t pos = block_add(dst_count, keep);
+    if (keep) dst[pos] = idx;
   }
 }
 
+// The compaction of group gc and the prepare of another group gp in one
+// launch (either may be kNone): nothing of one touches the other group.
+__global__ void k_list(Graph gr, uint32_t gc, uint32_t gp) {
+  if (gc != kNone) compact_part(gr.g[gc]);
+  if (gp != kNone) prepare_part(gr.g[gp]);
+}
+
+// dispatch_allocate_init_task (mod.rs:223-303) right after a prepare:
+// roots take the top `count` empty slots in order and the next active
+// entries, without atomics.
 __global__ void k_spawn_roots(Graph gr, uint32_t gid, uint32_t count) {
   const Group &G = gr.g[gid];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-    const uint32_t idx = spawn(G, kNone, kNone);
-    if (idx != kNone && G.words) G.payload[(size_t)idx * G.words] = i;
+  const uint32_t ne = *empty_count(G), na = *active_count(G) < G.size ? *active_count(G) : G.size;
+  const uint32_t room = G.size - na;
+  uint32_t k = count < ne ? count : ne;
+  k = k < room ? k : room;
+  uint32_t *list = active_list(G, G.cur);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x) {
+    const uint32_t idx = G.empty[ne - 1 - i];
+    G.state[idx] = kWaken;
+    G.parent[idx] = kNone;
+    G.ptype[idx] = kNone;
+    G.stage[idx] = 0u;
+    G.child[idx] = kNone;
+    if (G.words) G.payload[(size_t)idx * G.words] = i;
+    list[na + i] = idx;
+  }
+  if (last_block(G.ctr + kDone)) {
+    *empty_count(G) = ne - k;
+    *active_count(G) = na + k;
+    if (count > ne) atomicOr(G.ctr + kError, 1u);
+    else if (count > room) atomicOr(G.ctr + kError, 2u);
+    G.ctr[kDone] = 0u;
   }
 }
 
@@ -91,32 +118,27 @@ __global__ void k_spawn_roots(Graph gr, uint32_t gid, uint32_t count) {
 // child of `child_group`, await it) (future/then.rs semantics).
 __global__ void k_generic_poll(Graph gr, uint32_t gid, uint32_t kind, uint32_t child_group) {
   const Group &G = gr.g[gid];
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += stride) {
-    uint32_t idx;
-    if (!poll_entry(G, i, idx)) break;
+  const Group &C = gr.g[child_group];
+  for_each_polled(G, [&](uint32_t, bool valid, uint32_t idx) {
     if (kind == 0u) {
-      end_poll(gr, gid, idx, true);
-      continue;
+      end_poll(gr, gid, valid, idx, true);
+      return;
     }
-    const Group &C = gr.g[child_group];
-    if (G.stage[idx] == 0u) {
-      const uint32_t c = spawn(C, idx, gid);
+    const bool first = valid && G.stage[idx] == 0u;
+    // first poll: spawn the child with payload + 1; it has not finished yet
+    const uint32_t c = spawn(C, first, idx, gid);
+    if (first) {
       if (c != kNone && C.words) C.payload[(size_t)c * C.words] = G.payload[(size_t)idx * G.words] + 1u;
       G.child[idx] = c;
       G.stage[idx] = 1u;
-      end_poll(gr, gid, idx, false);  // the child was just spawned: not finished
-      continue;
     }
-    const uint32_t c = G.child[idx];
-    if (c != kNone && C.state[c] == kFinished) {
-      if (C.words) G.payload[(size_t)idx * G.words] = C.payload[(size_t)c * C.words];
-      cleanup(C, c);
-      end_poll(gr, gid, idx, true);
-    } else {
-      end_poll(gr, gid, idx, false);
-    }
-  }
+    // later polls: a finished child is read back and freed
+    const uint32_t w = (valid && !first) ? G.child[idx] : kNone;
+    const bool done = w != kNone && C.state[w] == kFinished;
+    if (done && C.words) G.payload[(size_t)idx * G.words] = C.payload[(size_t)w * C.words];
+    cleanup(C, done, w);
+    end_poll(gr, gid, valid, idx, done);
+  });
 }
 
 uint32_t max_size(const Graph &gr) {
@@ -127,25 +149,35 @@ uint32_t max_size(const Graph &gr) {
 
 }  // namespace
 
-hipError_t enqueue_reset(const Graph &gr, hipStream_t st) {
+hipError_t enqueue_reset(Graph &gr, hipStream_t st) {
+  for (uint32_t k = 0; k < gr.n; k++) gr.g[k].cur = gr.g[k].pp = 0u;
   hipLaunchKernelGGL(k_reset, dim3(grid_for(max_size(gr))), dim3(kBlock), 0, st, gr);
   return hipGetLastError();
 }
 
-hipError_t enqueue_prepare(const Graph &gr, uint32_t gid, hipStream_t st) {
-  hipLaunchKernelGGL(k_prepare, dim3(grid_for(gr.g[gid].size)), dim3(kBlock), 0, st, gr, gid);
+hipError_t enqueue_list(Graph &gr, uint32_t gc, uint32_t gp, hipStream_t st) {
+  if (gc == kNone && gp == kNone) return hipSuccess;
+  if (gc != kNone && gc == gp) {  // one group: compact, then prepare
+    hipError_t e = enqueue_list(gr, gc, kNone, st);
+    return e != hipSuccess ? e : enqueue_list(gr, kNone, gp, st);
+  }
+  uint32_t n = 0;
+  if (gc != kNone) n = gr.g[gc].size;
+  if (gp != kNone && gr.g[gp].size > n) n = gr.g[gp].size;
+  hipLaunchKernelGGL(k_list, dim3(grid_for(n, kListGrid)), dim3(kBlock), 0, st, gr, gc, gp);
+  if (gc != kNone) gr.g[gc].cur ^= 1u;
+  if (gp != kNone) gr.g[gp].pp ^= 1u;
   return hipGetLastError();
 }
 
-hipError_t enqueue_compact(const Graph &gr, uint32_t gid, hipStream_t st) {
-  hipLaunchKernelGGL(k_compact, dim3(grid_for(gr.g[gid].size)), dim3(kBlock), 0, st, gr, gid);
-  return hipGetLastError();
-}
+hipError_t enqueue_prepare(Graph &gr, uint32_t gid, hipStream_t st) { return enqueue_list(gr, kNone, gid, st); }
 
-hipError_t enqueue_spawn_roots(const Graph &gr, uint32_t gid, uint32_t count, hipStream_t st) {
+hipError_t enqueue_compact(Graph &gr, uint32_t gid, hipStream_t st) { return enqueue_list(gr, gid, kNone, st); }
+
+hipError_t enqueue_spawn_roots(Graph &gr, uint32_t gid, uint32_t count, hipStream_t st) {
   hipError_t e = enqueue_prepare(gr, gid, st);
   if (e != hipSuccess || count == 0) return e;
-  hipLaunchKernelGGL(k_spawn_roots, dim3(grid_for(count)), dim3(kBlock), 0, st, gr, gid, count);
+  hipLaunchKernelGGL(k_spawn_roots, dim3(grid_for(count, kListGrid)), dim3(kBlock), 0, st, gr, gid, count);
   return hipGetLastError();
 }
 

@@ -137,6 +137,23 @@ class AORenderer:
         self.sample_count += 1
         return self.buffer
 
+    def autotune(self, camera, reps: int = 3) -> dict:
+        """Time every traversal launch shape on AO calls of this camera and
+        keep the fastest for the system (the buffer is overwritten and the
+        sample count restarts); returns the choice."""
+        p = self.params(camera)
+        t = L.RdrtTune()
+        L.check(self.system._L.rdrt_ao_autotune(self.system._h, C.byref(p), int(self.buffer.data_ptr()), reps,
+                                                C.byref(t)))
+        self.sample_count = 0
+        return self.system.tune()
+
+    def ray_counts(self):
+        """(camera rays, occlusion rays) traced by the last render."""
+        out = np.zeros(2, dtype=np.uint32)
+        L.check(self.system._L.rdrt_ao_ray_counts(self.system._h, out.ctypes.data))
+        return int(out[0]), int(out[1])
+
     def states(self) -> TaskGraphExecutionStates:
         arrs = [np.zeros(5, dtype=np.uint32) for _ in range(4)]
         L.check(self.system._L.rdrt_ao_states(self.system._h, *[a.ctypes.data for a in arrs]))
