@@ -89,13 +89,15 @@ def _kernel_order(n_rays: int, visits, ms: float) -> dict:
 
 def _call_kernels(tune) -> str:
     k = "k_trace_persist" if tune and tune.get("persistent") else "k_trace_fast"
+    if tune and tune.get("coop"):
+        k = f"{k} / k_trace_coop (launches of at most {tune['coop']} rays)"
     if tune and tune.get("cull"):
         return f"k_trace_cull + k_block_scan + k_scatter2 + {k} (one rdrt_trace call)"
     return k
 
 
 def _tune_arg(tune) -> str:
-    return ",".join(str(tune[k]) for k in ("block_size", "lds_entries", "waves", "persistent", "cull", "wide"))
+    return ",".join(str(tune[k]) for k in ("block_size", "lds_entries", "waves", "persistent", "cull", "wide", "coop"))
 
 
 def _record(value, unit, args, ms_per_step, config, roofline, extra=None):
@@ -985,7 +987,7 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic passes")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--tune", default="",
-                    help="launch shape block,lds,waves,persistent[,cull[,wide]] (no autotune)")
+                    help="launch shape block,lds,waves,persistent[,cull[,wide[,coop]]] (no autotune)")
     ap.add_argument("--profile-only", action="store_true", help="run the timed launches only (counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -1009,7 +1011,7 @@ def main():
     if not args.no_traffic and args.workload in ("pt", "closest_hit", "batched"):
         # counter passes in child processes, with this run's launch shape
         shape = rec["config"].get("launch_shape")
-        extra, kernels = (), ("k_trace_fast", "k_trace_persist")
+        extra, kernels = (), ("k_trace_fast", "k_trace_persist", "k_trace_coop")
         if shape:
             extra = ("--tune", _tune_arg(shape))
             if shape.get("cull"):

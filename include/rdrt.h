@@ -366,6 +366,9 @@ typedef struct rdrt_tune {
   uint32_t wide;        /* node layout: 0 = child pairs (64 B), 1 = child quads (four boxes, 128 B),
                            2 = quantized child quads (four 8-bit boxes, 64 B) */
   float ms;             /* rdrt_autotune: per-launch time of the chosen shape on the sample */
+  uint32_t coop;        /* launches with at most this many live rays (device-side count) are traced by the
+                           cooperative kernel, 8 lanes per ray: the TLAS walk in lockstep, the instances it
+                           accepts dealt to the lanes; 0 = never.  Fast layouts only, not with visit counting */
 } rdrt_tune;
 /* Times every shape of the search space (3 block sizes x 2 LDS stack depths
  * x 2 occupancy targets x grid / persistent x with / without the cull pass

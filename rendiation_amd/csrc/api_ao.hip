@@ -328,13 +328,14 @@ int32_t rdrt_ao_autotune(rdrt_ctx *c, const rdrt_ao_params *prm, void *accum, ui
   for (uint32_t block : {64u, 128u, 256u})
     for (uint32_t lds : {8u, 16u})
       for (uint32_t waves : {0u, 8u})
-        for (uint32_t shape = 0; shape < 6 && rc == RDRT_OK; shape++) {
+        for (uint32_t shape = 0; shape < 18 && rc == RDRT_OK; shape++) {
           TraceTune t;
           t.block = block;
           t.lds = lds;
           t.waves = waves;
           t.persist = shape & 1u;
-          t.wide = shape >> 1;
+          t.wide = (shape >> 1) % 3u;
+          t.coop = kCoopChoices[shape / 6u];
           if (stack_entries_for(push_depth(c, t)) == 0) continue;
           c->tune = t;
           rc = rdrt_ao_render(c, &q, accum);  // warm-up
@@ -369,6 +370,7 @@ int32_t rdrt_ao_autotune(rdrt_ctx *c, const rdrt_ao_params *prm, void *accum, ui
     best->cull = chosen.cull;
     best->wide = chosen.wide;
     best->ms = chosen_ms;
+    best->coop = chosen.coop;
   }
   return RDRT_OK;
 }

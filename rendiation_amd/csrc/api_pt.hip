@@ -332,13 +332,14 @@ int32_t rdrt_pt_autotune(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance
   for (uint32_t block : {64u, 128u, 256u})
     for (uint32_t lds : {8u, 16u})
       for (uint32_t waves : {0u, 8u})
-        for (uint32_t shape = 0; shape < 6 && rc == RDRT_OK; shape++) {
+        for (uint32_t shape = 0; shape < 18 && rc == RDRT_OK; shape++) {
           TraceTune t;
           t.block = block;
           t.lds = lds;
           t.waves = waves;
           t.persist = shape & 1u;
-          t.wide = shape >> 1;
+          t.wide = (shape >> 1) % 3u;
+          t.coop = kCoopChoices[shape / 6u];
           if (stack_entries_for(push_depth(c, t)) == 0) continue;
           c->tune = t;
           rc = rdrt_pt_render(c, &q, radiance);  // warm-up

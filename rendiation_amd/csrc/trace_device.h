@@ -464,6 +464,194 @@ __device__ __forceinline__ void select_second(TraceParams &p) {
   p.counters = nullptr;
 }
 
+// ------------------------------------------------------------------------
+// Cooperative kernel: kCoopLanes consecutive lanes trace one ray.  Every lane
+// of the group walks the TLAS in lockstep (same nodes, same far bound: the
+// group's nearest hit so far); the instances it accepts are dealt to the
+// lanes in batches of kCoopLanes, each lane walks its instance's BLAS alone,
+// and after each batch the group takes the nearest far bound.  A ray's chain
+// of dependent node loads becomes the TLAS walk plus the longest BLAS walk of
+// each batch, which is what bounds small launches (the tail bounces of a
+// path tracer): they last as long as their slowest ray.  The result is the
+// nearest accepted hit, as in the one-ray-per-lane kernel; among hits with
+// equal t the lowest lane of the last batch wins (a visit-order tie, as
+// between the layouts).
+// ------------------------------------------------------------------------
+constexpr uint32_t kCoopLanes = 8;
+
+__device__ __forceinline__ float group_min(float v) {
+  for (int m = (int)kCoopLanes / 2; m > 0; m >>= 1) v = fminf(v, __shfl_xor(v, m, (int)kCoopLanes));
+  return v;
+}
+__device__ __forceinline__ bool group_any(bool b) {
+  int v = b ? 1 : 0;
+  for (int m = (int)kCoopLanes / 2; m > 0; m >>= 1) v |= __shfl_xor(v, m, (int)kCoopLanes);
+  return v != 0;
+}
+
+template <int K, int S, int B, int LAYOUT>
+__device__ __forceinline__ void trace_coop_one(const TraceParams &p, uint32_t rid, uint32_t *stk, uint32_t glane) {
+  uint32_t spill[S > K ? S - K : 1];
+  auto push = [&](uint32_t &sp, uint32_t v) {
+    if (sp < (uint32_t)K) stk[sp * B] = v;
+    else if (sp < (uint32_t)S) spill[sp - K] = v;
+    else {
+      atomicOr(p.status, 1u);  // unreachable when S covers the built depth
+      return;
+    }
+    sp++;
+  };
+  auto pop = [&](uint32_t &sp) -> uint32_t {
+    sp--;
+    return sp < (uint32_t)K ? stk[sp * B] : spill[sp - K];
+  };
+
+  RayIn r = load_ray(p, rid);
+  float far_ = r.tmax;  // group-uniform outside a batch's BLAS walks
+  Best best;
+  float best_t = 0.0f;  // this lane's own nearest accepted hit
+  bool done = false;
+  uint32_t cand = kInvalid, cnt = 0, sp = 0;
+
+  // this lane's instance of the batch (already accepted by the TLAS walk)
+  auto run_instance = [&](uint32_t i) {
+    const InstRec &in = p.insts[i];
+    const uint32_t fl = merge_flags(r.flags, in.flags);
+    F3 bo, bd;
+    float s;
+    object_ray(in.inv, r.o, r.d, bo, bd, s);
+    const F3 binv = recip(bd);
+    const float near_s = r.tmin * s;
+    bool ce, cb;
+    cull_triangle(fl, ce, cb);
+    for (uint32_t g = in.geo_begin; g < in.geo_end && !done; g++) {
+      const GeoRec &geo = p.geos[g];
+      if (!geometry_passes(fl, geo.flags)) continue;
+      float tg;
+      if (!slab(bo, binv, near_s, far_ * s, geo.mn[0], geo.mn[1], geo.mn[2], geo.mx[0], geo.mx[1], geo.mx[2], tg))
+        continue;
+      uint32_t bsp = sp;  // above the TLAS entries
+      uint32_t bref = LAYOUT ? geo.quad_root : geo.root_ref;
+      auto push_blas = [&](uint32_t v) { push(bsp, v); };
+      for (;;) {
+        if (!(bref & kLeafBit)) {
+          const float fs = far_ * s;
+          if constexpr (LAYOUT == 2) {
+            if (step_qquad(p.tri_qquads, bref, bo, binv, near_s, fs, push_blas)) continue;
+          } else if constexpr (LAYOUT == 1) {
+            if (step_quad(p.tri_quads, bref, bo, binv, near_s, fs, push_blas)) continue;
+          } else {
+            if (step_pair(p.tri_pairs, bref, bo, binv, near_s, fs, push_blas)) continue;
+          }
+        } else {
+          const uint32_t k0 = (bref & ~kLeafBit) >> kLeafCountBits, k1 = k0 + (bref & kLeafMaxCount);
+          for (uint32_t k = k0; k < k1; k++) {
+            const float4 *q = reinterpret_cast<const float4 *>(p.tris + k);
+            float4 a = q[0], b = q[1], c = q[2], d = q[3];
+            float t, u, v;
+            bool back;
+            if (tri_core(bo, bd, near_s, far_ * s, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(c.x, c.y, c.z),
+                         f3(d.x, d.y, d.z), a.w, b.w, c.w, d.w, ce, cb, t, u, v, back)) {
+              far_ = t / s;
+              best_t = far_;
+              best.k = k;
+              best.geo = g;
+              best.inst = i;
+              best.u = u;
+              best.v = v;
+              best.back = back;
+              if (fl & F_END_SEARCH) {
+                done = true;
+                break;
+              }
+            }
+          }
+          if (done) break;
+        }
+        if (bsp == sp) break;
+        bref = pop(bsp);
+      }
+    }
+  };
+  // every lane of the group: walk the batch, then share the far bound
+  auto flush = [&]() {
+    if (cand != kInvalid) run_instance(cand);
+    cand = kInvalid;
+    cnt = 0;
+    far_ = group_min(far_);
+    done = group_any(done);
+    r = load_ray(p, rid);
+  };
+
+  const TlasRoot *root = bound_root(p, r.tlas_idx);
+  F3 winv = recip(r.d);
+  float tn;
+  if (root && slab(r.o, winv, r.tmin, far_, root->mn[0], root->mn[1], root->mn[2], root->mx[0], root->mx[1],
+                   root->mx[2], tn)) {
+    uint32_t ref = LAYOUT ? p.tlas_quad_roots[root - p.tlas_roots] : root->root_ref;
+    auto push_tlas = [&](uint32_t v) { push(sp, v); };
+    for (;;) {
+      if (!(ref & kLeafBit)) {
+        if constexpr (LAYOUT == 2) {
+          if (step_qquad(p.tlas_qquads, ref, r.o, winv, r.tmin, far_, push_tlas)) continue;
+        } else if constexpr (LAYOUT == 1) {
+          if (step_quad(p.tlas_quads, ref, r.o, winv, r.tmin, far_, push_tlas)) continue;
+        } else {
+          if (step_pair(p.tlas_pairs, ref, r.o, winv, r.tmin, far_, push_tlas)) continue;
+        }
+      } else {
+        const uint32_t i0 = (ref & ~kLeafBit) >> kLeafCountBits, i1 = i0 + (ref & kLeafMaxCount);
+        for (uint32_t i = i0; i < i1 && !done; i++) {
+          const InstRec &in = p.insts[i];
+          float ti;
+          if (!slab(r.o, winv, r.tmin, far_, in.mn[0], in.mn[1], in.mn[2], in.mx[0], in.mx[1], in.mx[2], ti))
+            continue;
+          if ((r.mask & in.mask) == 0) continue;
+          if (merge_flags(r.flags, in.flags) & F_SKIP_TRIANGLES) continue;
+          if (glane == cnt) cand = i;
+          if (++cnt == kCoopLanes) {
+            flush();
+            winv = recip(r.d);
+          }
+        }
+        if (done) break;
+      }
+      if (sp == 0) break;
+      ref = pop(sp);
+    }
+    if (cnt) flush();
+  }
+  // the lowest lane whose own hit is the group's nearest reports it
+  const bool mine = best.k != kInvalid && best_t == far_;
+  const uint64_t m = __ballot(mine);
+  const uint32_t lane = __lane_id();
+  const uint32_t gm = (uint32_t)(m >> (lane & ~(kCoopLanes - 1u))) & ((1u << kCoopLanes) - 1u);
+  const uint32_t win = gm ? (uint32_t)__ffs(gm) - 1u : 0u;
+  if (glane == win) {
+    if (!gm) best.k = kInvalid;
+    best.t = far_;
+    write_result(p, p.tri_ids, rid, best, 0u, 0u, 0u);
+  }
+}
+
+template <int K, int S, int B, int LAYOUT>
+__global__ __launch_bounds__(B) void k_trace_coop(TraceParams p) {
+  __shared__ uint32_t lds[K * B];
+  const uint32_t n0 = live_first(p), n1 = live_second(p);
+  if (n0 + n1 > p.coop_max) return;  // the one-ray-per-lane kernel traces launches this large
+  const uint32_t gi = (blockIdx.x * B + threadIdx.x) / kCoopLanes;
+  uint32_t slot = gi, n = n0;
+  if (gi >= n0) {
+    slot = gi - n0;
+    n = n1;
+    select_second(p);
+  }
+  if (slot >= n) return;  // whole groups: the lanes of a group share gi
+  const uint32_t rid = p.queue ? p.queue[slot] : slot;
+  if (rid >= p.n) return;
+  trace_coop_one<K, S, B, LAYOUT>(p, rid, lds + threadIdx.x, threadIdx.x & (kCoopLanes - 1u));
+}
+
 // One ray per thread over a grid sized for both segments' slots; the live
 // counts may be smaller (device queue sizes).  The blocks of the second
 // segment follow those of the first, so a block's segment is uniform.
@@ -471,6 +659,7 @@ template <int K, int S, int B, int W, bool CNT, int LAYOUT>
 __global__ __launch_bounds__(B, W) void k_trace_fast(TraceParams p) {
   __shared__ uint32_t lds[K * B];
   const uint32_t n0 = live_first(p), n1 = live_second(p);
+  if (p.coop_max && n0 + n1 <= p.coop_max) return;  // the cooperative kernel traces it
   const uint32_t b0 = (n0 + B - 1) / B;
   const uint32_t used = b0 + (n1 + B - 1) / B;
   if (blockIdx.x >= used) return;
@@ -498,6 +687,7 @@ template <int K, int S, int B, int W, bool CNT, int LAYOUT>
 __global__ __launch_bounds__(B, W) void k_trace_persist(TraceParams p) {
   __shared__ uint32_t lds[K * B];
   const uint32_t n0 = live_first(p), n1 = live_second(p);
+  if (p.coop_max && n0 + n1 <= p.coop_max) return;  // the cooperative kernel traces it
   const uint32_t start1 = (n0 + 63u) & ~63u;
   const uint32_t total = n1 ? start1 + n1 : n0;
   const uint32_t lane = threadIdx.x & 63u;
@@ -527,7 +717,16 @@ __global__ void k_zero_work(uint32_t *w) { *w = 0u; }
 // K = stack entries per lane in LDS, W = minimum waves per SIMD asked of the
 // register allocator (1 = its own choice), LAYOUT = TraceTune::wide.
 template <int K, int S, int B, int W, int LAYOUT>
-hipError_t launch_fast_kw(const TraceParams &p, bool persist, hipStream_t st) {
+hipError_t launch_fast_kw(const TraceParams &p0, bool persist, uint32_t coop, hipStream_t st) {
+  TraceParams p = p0;
+  p.coop_max = p.counters ? 0u : coop;  // visit counting stays one ray per lane
+  if (p.coop_max) {
+    // enough groups for every live ray when the launch qualifies
+    const uint32_t slots = slots_of(p) + (p.count2 ? p.slots2 : 0u);
+    const uint32_t groups = std::min(p.coop_max, slots);
+    const uint32_t grid = (groups * kCoopLanes + B - 1) / B;
+    if (grid) hipLaunchKernelGGL((k_trace_coop<K, S, B, LAYOUT>), dim3(grid), dim3(B), 0, st, p);
+  }
   if (persist) {
     if (!p.work) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(1), 0, st, p.work);
@@ -554,11 +753,11 @@ hipError_t launch_fast_s(const TraceParams &p, const TraceTune &t, hipStream_t s
   constexpr int K16 = S < 16 ? S : 16;
   const bool persist = t.persist != 0;
   if (t.waves >= 8) {
-    if (t.lds <= 8) return launch_fast_kw<K8, S, B, 8, LAYOUT>(p, persist, st);
-    return launch_fast_kw<K16, S, B, 8, LAYOUT>(p, persist, st);
+    if (t.lds <= 8) return launch_fast_kw<K8, S, B, 8, LAYOUT>(p, persist, t.coop, st);
+    return launch_fast_kw<K16, S, B, 8, LAYOUT>(p, persist, t.coop, st);
   }
-  if (t.lds <= 8) return launch_fast_kw<K8, S, B, 1, LAYOUT>(p, persist, st);
-  return launch_fast_kw<K16, S, B, 1, LAYOUT>(p, persist, st);
+  if (t.lds <= 8) return launch_fast_kw<K8, S, B, 1, LAYOUT>(p, persist, t.coop, st);
+  return launch_fast_kw<K16, S, B, 1, LAYOUT>(p, persist, t.coop, st);
 }
 
 template <int B, int LAYOUT>

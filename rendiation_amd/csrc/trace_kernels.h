@@ -38,6 +38,10 @@ struct TraceParams {
   const uint32_t *queue2;
   const uint32_t *count2;  // null = no second segment
   uint32_t slots2, flags2;
+  // Cooperative traversal (fast layouts): when both segments together hold
+  // at most coop_max live rays, the cooperative kernel traces the launch and
+  // the one-ray-per-lane kernel returns at once; 0 = never.
+  uint32_t coop_max;
 
   const PairNode *tri_pairs;
   const TriPre *tris;
@@ -82,7 +86,12 @@ struct TraceTune {
                          //     queues the others densely for the traversal
   uint32_t wide = 0;     // node layout: 0 = child pairs, 1 = child quads (four boxes per node),
                          //     2 = quantized child quads (64-byte nodes)
+  uint32_t coop = 0;     // launches with at most this many live rays use the cooperative kernel
+                         //     (8 lanes per ray); 0 = never
 };
+
+// Cooperative-kernel thresholds the frame and AO autotuners try.
+constexpr uint32_t kCoopChoices[3] = {0u, 1u << 15, 1u << 18};
 
 // Scratch of a culled launch: tmp (slots words), blk (2 * ceil(slots /
 // kCullBlock) words), queue (slots words), count (1 word).

@@ -89,16 +89,55 @@ def test_fast_kernel_matches_except_near_ties(closest_hit_scene, wide, device):
     assert np.array_equal(gm[same], rm[same])
 
 
-@pytest.mark.parametrize("traversal,wide", [(TRAVERSAL_REFERENCE, 0), (TRAVERSAL_FAST, 0), (TRAVERSAL_FAST, 1),
-                                            (TRAVERSAL_FAST, 2)])
-def test_instanced_scene_with_flags(traversal, wide, device):
+@pytest.mark.parametrize("wide", [0, 1, 2])
+def test_cooperative_kernel_matches_except_near_ties(closest_hit_scene, wide, device):
+    """Every launch through the cooperative kernel (8 lanes per ray): the
+    same hits as the oracle except near ties, as the one-ray-per-lane
+    kernel."""
+    sys, rays, (rh, rm, _) = closest_hit_scene
+    _say(f"closest-hit 100k tris x 1M rays, cooperative kernel, wide={wide}")
+    sys.set_tune(wide=wide, coop=1 << 22)
+    try:
+        gh, gm = _gpu_trace_nc(sys, rays, device)
+    finally:
+        sys.set_tune()
+    d = compare(gh, rh)
+    print(d)
+    assert d.ok(), d
+    assert d.value_mismatch_bits == 0, d
+    same = gh[:, 0] == rh[:, 0]
+    assert np.array_equal(gm[same], rm[same])
+
+
+def _gpu_trace_nc(sys, rays_np, device, meta_np=None, **kw):
+    """A fast launch without visit counters (counting keeps one ray per lane)."""
+    import torch
+
+    rays = torch.from_numpy(np.ascontiguousarray(rays_np)).to(device)
+    n = rays_np.shape[0]
+    hits = torch.empty((n, 4), dtype=torch.int32, device=device)
+    hm = torch.empty((n, 4), dtype=torch.int32, device=device)
+    meta = None if meta_np is None else torch.from_numpy(np.ascontiguousarray(meta_np).view(np.int32)).to(device)
+    sys.trace(rays, hits, ray_meta=meta, hit_meta=hm, traversal=TRAVERSAL_FAST, **kw)
+    sys.sync()
+    return hits.cpu().numpy().view(np.uint32), hm.cpu().numpy().view(np.uint32)
+
+
+@pytest.mark.parametrize("traversal,wide,coop", [(TRAVERSAL_REFERENCE, 0, 0), (TRAVERSAL_FAST, 0, 0),
+                                                 (TRAVERSAL_FAST, 1, 0), (TRAVERSAL_FAST, 2, 0),
+                                                 (TRAVERSAL_FAST, 1, 1 << 22), (TRAVERSAL_FAST, 2, 1 << 22)])
+def test_instanced_scene_with_flags(traversal, wide, coop, device):
     blas_list, tlas_list, binding, rays, meta = instanced_scene(seed=3, nrays=60_000)
     opts = BuildOptions()
     sys, orc = _build_pair(blas_list, tlas_list, binding, opts)
-    _say(f"instanced scene, traversal={traversal} wide={wide}")
+    _say(f"instanced scene, traversal={traversal} wide={wide} coop={coop}")
     rh, rm, rc = orc.trace(rays, meta=meta, with_meta=True, with_counters=True)
-    sys.set_tune(wide=wide)
-    gh, gm, gc = _gpu_trace(sys, rays, device, meta_np=meta, traversal=traversal)
+    sys.set_tune(wide=wide, coop=coop)
+    if coop:
+        gh, gm = _gpu_trace_nc(sys, rays, device, meta_np=meta)
+        gc = None
+    else:
+        gh, gm, gc = _gpu_trace(sys, rays, device, meta_np=meta, traversal=traversal)
     end_search = (meta[:, 0] & RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH) != 0
     if traversal == TRAVERSAL_REFERENCE:
         assert np.array_equal(gh, rh), compare(gh, rh)
@@ -131,6 +170,10 @@ def test_uniform_ray_flags(flags, device):
             assert np.array_equal(gc, rc)
         else:
             assert compare(gh, rh).ok()
+            sys.set_tune(wide=wide, coop=1 << 22)  # the cooperative kernel
+            gh, _ = _gpu_trace_nc(sys, rays, device, flags=flags)
+            assert compare(gh, rh).ok()
+    sys.set_tune()
 
 
 def test_edge_cases(device):

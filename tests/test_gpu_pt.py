@@ -134,14 +134,17 @@ def test_floor_on_gpu(device):
     assert st["shadow_rays"][:1] == [32 * 32]
 
 
-@pytest.mark.parametrize("spp,wide", [(1, 0), (4, 0), (1, 1), (1, 2)])
-def test_parametric_scene_against_fp64_oracle(spp, wide, device):
+@pytest.mark.parametrize("spp,wide,coop", [(1, 0, 0), (4, 0, 0), (1, 1, 0), (1, 2, 0), (1, 1, 1 << 22),
+                                           (1, 1, 2000)])
+def test_parametric_scene_against_fp64_oracle(spp, wide, coop, device):
+    """coop = 1 << 22: every traversal launch through the cooperative kernel;
+    2000: only the small tail launches."""
     scene = synth.pt_scene(seed=1, n_meshes=64, tess=48)
     cam = Camera()
     cfg = PathTracingConfig(width=192, height=108, spp=spp, max_bounces=4, rr_start=2, seed=5)
-    _say(f"parametric scene 64 meshes tess 48, 192x108 spp {spp}, wide={wide}")
+    _say(f"parametric scene 64 meshes tess 48, 192x108 spp {spp}, wide={wide}, coop={coop}")
     s = _system(scene)
-    s.set_tune(wide=wide)
+    s.set_tune(wide=wide, coop=coop)
     img, st = _render(s, cam, cfg)
     ref, ext, sh = oracle_scene_from(scene).pt_render(oracle_params(cam, cfg))
     rmse = float(np.sqrt(np.mean((img - ref) ** 2)))
@@ -255,10 +258,10 @@ def test_frame_autotune(device):
     cfg = PathTracingConfig(width=96, height=64, spp=1, max_bounces=3, use_graph=True)
     choice = pt.autotune(cam, cfg, reps=1)
     print("pt autotune:", choice)
-    assert choice["block_size"] in (64, 128, 256) and choice["wide"] in (0, 1) and choice["ms"] > 0
+    assert choice["block_size"] in (64, 128, 256) and choice["wide"] in (0, 1, 2) and choice["ms"] > 0
     img = pt.render(cam, cfg).cpu().numpy()
     s.set_tune(choice["block_size"], choice["lds_entries"], choice["waves"], choice["persistent"], choice["cull"],
-               choice["wide"])
+               choice["wide"], choice["coop"])
     eager = pt.render(cam, dataclasses.replace(cfg, use_graph=False)).cpu().numpy()
     assert np.array_equal(img, eager)
     with pytest.raises(ValueError):
