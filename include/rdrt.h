@@ -176,6 +176,13 @@ int32_t rdrt_sync(rdrt_ctx *ctx);
 
 /* ---- acceleration structures (backend.rs:120-151) ---- */
 int32_t rdrt_blas_begin(rdrt_ctx *ctx);
+/* A one-geometry BLAS in one call: rdrt_blas_begin, rdrt_blas_add_triangles,
+ * rdrt_blas_end.  opts_or_null, if given, must equal the context's BLAS
+ * build options (zero fields = the reference defaults); the trees of one
+ * context share them (RDRT_ERR_UNSUPPORTED otherwise). */
+int32_t rdrt_blas_create(rdrt_ctx *ctx, const float *pos_xyz, uint32_t nverts, const uint32_t *indices_or_null,
+                         uint32_t ntris, uint32_t geometry_flags, const rdrt_build_opts *opts_or_null,
+                         uint32_t *out_handle);
 /* One geometry of the pending BLAS: positions are xyz f32 triples (copied);
  * indices_or_null = NULL means a non-indexed triangle list.  A rejected
  * geometry aborts the pending BLAS (rdrt_blas_end then fails). */
@@ -354,6 +361,16 @@ int32_t rdrt_trace_closest(rdrt_ctx *ctx, const rdrt_ray *rays_dev, uint32_t n, 
  * rdrt_trace call, measured with HIP events on the context stream; requires
  * rdrt_sync first. */
 int32_t rdrt_last_trace_ms(rdrt_ctx *ctx, float *ms);
+/* Context metrics in one record. */
+typedef struct rdrt_metrics {
+  float last_call_ms;    /* device time of the last trace, frame or batched call (waits for it; -1 if none) */
+  float tune_ms;         /* the chosen shape's time in the last autotune (0 if none) */
+  uint64_t device_bytes; /* device memory of the committed scene */
+  double build_ms;       /* host time of the last build */
+  uint32_t status;       /* device status word: bit 0 = a traversal stack would have overflowed */
+  uint32_t pad;
+} rdrt_metrics;
+int32_t rdrt_get_metrics(rdrt_ctx *ctx, rdrt_metrics *out);
 
 /* ---- launch tuning of the fast traversal (no counterpart in the reference:
  *      its workgroup size is fixed at 128, traverse_gpu.rs) ---- */
@@ -463,6 +480,9 @@ typedef struct rdrt_pt_stats {
 /* Renders one frame: radiance_dev receives views*width*height RGB f32 (the
  * mean of spp paths per pixel).  Scene committed and materials set first. */
 int32_t rdrt_pt_render(rdrt_ctx *ctx, const rdrt_pt_params *params, float *radiance_dev);
+/* The same call under the name of the forward half of the differentiable
+ * frame (rdrt_pt_backward is the other). */
+int32_t rdrt_pt_forward(rdrt_ctx *ctx, const rdrt_pt_params *params, float *radiance_dev);
 /* Ray counts of the last frame (synchronizes). */
 int32_t rdrt_pt_get_stats(rdrt_ctx *ctx, rdrt_pt_stats *out);
 /* Times every launch shape of the fast traversal (3 block sizes x 2 LDS
@@ -550,6 +570,10 @@ int32_t rdrt_batch_autotune(rdrt_ctx *ctx, const rdrt_batch_params *params, uint
  * increasing order, count_dev (device u32) their number. */
 int32_t rdrt_compact(rdrt_ctx *ctx, const uint32_t *mask_dev, uint32_t n, uint32_t *out_idx_dev,
                      uint32_t *count_dev);
+/* One scrambled 2-D Sobol point on the host, bit-identical to the device
+ * samplers: out2[0] = van_der_corput(n, scramble0), out2[1] = sobol(n,
+ * scramble1) (math/statistics/src/sampling/sobol.rs:40-68). */
+int32_t rdrt_sobol(uint32_t n, uint32_t scramble0, uint32_t scramble1, float out2[2]);
 /* Scrambled Sobol samples (math/statistics/src/sampling/sobol.rs:40-68):
  * out_dev[2i] = van_der_corput(index[i], scramble[2i]),
  * out_dev[2i+1] = sobol(index[i], scramble[2i+1]).  Device pointers. */

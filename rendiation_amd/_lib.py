@@ -32,6 +32,11 @@ class RdrtAoParams(C.Structure):
                 ("tile_pixels", u32), ("use_graph", u32), ("pad", u32 * 2)]
 
 
+class RdrtMetrics(C.Structure):
+    _fields_ = [("last_call_ms", f32), ("tune_ms", f32), ("device_bytes", C.c_uint64), ("build_ms", C.c_double),
+                ("status", u32), ("pad", u32)]
+
+
 class RdrtRayMeta(C.Structure):
     _fields_ = [("flags", u32), ("cull_mask", u32), ("tlas_idx", u32), ("pad", u32)]
 
@@ -172,6 +177,10 @@ SIGNATURES = {
     "rdrt_ao_ray_counts": (i32, [C.c_void_p, C.c_void_p]),
     "rdrt_ao_autotune": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, u32, C.c_void_p]),
     "rdrt_ao_hemisphere_sample": (i32, [u32, C.c_void_p]),
+    "rdrt_blas_create": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p, u32, u32, C.c_void_p, C.c_void_p]),
+    "rdrt_pt_forward": (i32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "rdrt_sobol": (i32, [u32, u32, u32, C.c_void_p]),
+    "rdrt_get_metrics": (i32, [C.c_void_p, C.c_void_p]),
     "rdrt_autotune": (i32, [C.c_void_p, C.POINTER(RdrtTraceArgs), u32, C.POINTER(RdrtTune)]),
     "rdrt_set_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),
     "rdrt_get_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),

@@ -775,6 +775,62 @@ int32_t rdrt_trace_closest(rdrt_ctx *c, const rdrt_ray *rays, uint32_t n, rdrt_h
   return rdrt_trace(c, &a);
 }
 
+int32_t rdrt_blas_create(rdrt_ctx *c, const float *pos, uint32_t nverts, const uint32_t *idx, uint32_t ntris,
+                         uint32_t geometry_flags, const rdrt_build_opts *opts, uint32_t *out) {
+  if (!c || !out) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  if (opts) {
+    const TreeOptions o = tree_opt(opts->blas_bin_size, opts->blas_max_depth, opts->sah_bins, 2);
+    const TreeOptions &h = c->src.blas_opt;
+    if (o.bin_size != h.bin_size || o.max_depth != h.max_depth || o.bins != h.bins)
+      return fail(RDRT_ERR_UNSUPPORTED, "per-BLAS build options must equal the context's");
+  }
+  int32_t rc = rdrt_blas_begin(c);
+  if (rc) return rc;
+  rc = rdrt_blas_add_triangles(c, pos, nverts, idx, ntris, geometry_flags);
+  if (rc) {
+    c->pending.clear();
+    c->in_blas = false;
+    return rc;
+  }
+  return rdrt_blas_end(c, out);
+}
+
+int32_t rdrt_sobol(uint32_t n, uint32_t s0, uint32_t s1, float out2[2]) {
+  if (!out2) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  // 24-bit fixed point to [0, 1 - 2^-23] (sobol.rs:47-50,64-67)
+  auto to_float = [](uint32_t x) {
+    const float f = (float)((x >> 8) & 0xffffffu) / 16777216.0f;
+    return f < 1.0f - 1.1920928955078125e-7f ? f : 1.0f - 1.1920928955078125e-7f;
+  };
+  uint32_t r = 0;  // radical inverse: the bits of n reversed
+  for (int k = 0; k < 32; k++) r |= ((n >> k) & 1u) << (31 - k);
+  out2[0] = to_float(r ^ s0);
+  uint32_t v = 1u << 31, x = s1;  // dimension 1: v_{k+1} = v_k ^ (v_k >> 1)
+  for (uint32_t m = n; m; m >>= 1, v ^= v >> 1)
+    if (m & 1u) x ^= v;
+  out2[1] = to_float(x);
+  return RDRT_OK;
+}
+
+int32_t rdrt_get_metrics(rdrt_ctx *c, rdrt_metrics *o) {
+  if (!c || !o) return fail(RDRT_ERR_INVALID_ARG, "null argument");
+  std::memset(o, 0, sizeof(*o));
+  o->last_call_ms = -1.0f;
+  o->tune_ms = c->tune_ms;
+  o->device_bytes = c->dev.bytes;
+  o->build_ms = c->built_valid ? c->built.build_ms : 0.0;
+  if (!c->has_device) return RDRT_OK;
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  if (c->timed) {
+    RDRT_HIP(hipEventSynchronize(c->ev1));
+    RDRT_HIP(hipEventElapsedTime(&o->last_call_ms, c->ev0, c->ev1));
+  }
+  RDRT_HIP(hipStreamSynchronize(c->work_stream));
+  RDRT_HIP(hipMemcpy(&o->status, c->status, 4, hipMemcpyDeviceToHost));
+  return RDRT_OK;
+}
+
 int32_t rdrt_last_trace_ms(rdrt_ctx *c, float *ms) {
   if (!c || !ms) return fail(RDRT_ERR_INVALID_ARG, "null argument");
   if (!c->timed) return fail(RDRT_ERR_STATE, "no trace recorded");

@@ -310,6 +310,10 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
   return RDRT_OK;
 }
 
+int32_t rdrt_pt_forward(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) {
+  return rdrt_pt_render(c, prm, radiance);
+}
+
 int32_t rdrt_pt_autotune(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance, uint32_t reps, rdrt_tune *best) {
   if (!c || !prm || !radiance) return fail(RDRT_ERR_INVALID_ARG, "null argument");
   if (prm->record || prm->diagnostics) return fail(RDRT_ERR_INVALID_ARG, "tune on plain frames (no record, no diagnostics)");
