@@ -628,6 +628,7 @@ int32_t rdrt_trace(rdrt_ctx *c, const rdrt_trace_args *a) {
   else RDRT_HIP(launch_trace(p, a->traversal, push, c->tune, c->stream));
   RDRT_HIP(hipEventRecord(c->ev1, c->stream));
   c->timed = true;
+  c->call_timed = true;
   return RDRT_OK;
 }
 
@@ -759,6 +760,7 @@ int32_t rdrt_autotune(rdrt_ctx *c, const rdrt_trace_args *a, uint32_t reps, rdrt
   c->tune = chosen;
   c->tune_ms = chosen_ms;
   c->timed = false;
+  c->call_timed = false;
   if (best) return rdrt_get_tune(c, best);
   return RDRT_OK;
 }
@@ -822,7 +824,7 @@ int32_t rdrt_get_metrics(rdrt_ctx *c, rdrt_metrics *o) {
   if (!c->has_device) return RDRT_OK;
   int32_t rc = need_device(c);
   if (rc) return rc;
-  if (c->timed) {
+  if (c->call_timed) {
     RDRT_HIP(hipEventSynchronize(c->ev1));
     RDRT_HIP(hipEventElapsedTime(&o->last_call_ms, c->ev0, c->ev1));
   }

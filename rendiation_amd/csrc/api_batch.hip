@@ -221,6 +221,7 @@ int32_t rdrt_trace_batched(rdrt_ctx *c, const rdrt_batch_params *a) {
     s.launches = launches;
   }
   RDRT_HIP(hipEventRecord(c->ev1, st));
+  c->call_timed = true;
   return RDRT_OK;
 }
 

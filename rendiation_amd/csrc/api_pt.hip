@@ -304,6 +304,7 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
     RDRT_HIP(enqueue_pt_frame(L, tp, push_depth(c, c->tune), c->tune, st));
   }
   RDRT_HIP(hipEventRecord(c->ev1, st));
+  c->call_timed = true;
   s.last_bounces = prm->max_bounces;
   s.last_frame = L;
   s.last_recorded = prm->record != 0;

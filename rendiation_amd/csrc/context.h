@@ -141,7 +141,8 @@ struct rdrt_ctx {
   rdrt::DeviceScene dev;
   uint32_t *status = nullptr;  // device status word of the kernels
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  bool timed = false;
+  bool timed = false;       // ev0 / ev1 bracket the last rdrt_trace call
+  bool call_timed = false;  // ev0 / ev1 bracket the last trace, frame or batched call
   std::vector<rdrt::Material> materials;
   rdrt::PtState pt;
   rdrt::BatchState batch;
