@@ -377,7 +377,8 @@ int32_t rdrt_get_metrics(rdrt_ctx *ctx, rdrt_metrics *out);
 typedef struct rdrt_tune {
   uint32_t block_size;  /* threads per block: 64, 128, 256 */
   uint32_t lds_entries; /* traversal stack entries per lane in LDS (8, 16); deeper ones in private memory */
-  uint32_t waves;       /* 8 = compile for 8 waves per SIMD (at most 64 VGPRs); 0 = the compiler's choice */
+  uint32_t waves;       /* 8 / 6 = compiled for 8 / 6 waves per SIMD (at most 64 / 80 VGPRs); 0 = the
+                           compiler's choice */
   uint32_t persistent;  /* 1 = resident-size grid whose waves pull 64-ray chunks from a counter */
   uint32_t cull;        /* 1 = a first pass finishes rays that miss their scene's root box and queues the rest */
   uint32_t wide;        /* node layout: 0 = child pairs (64 B), 1 = child quads (four boxes, 128 B),

@@ -670,10 +670,11 @@ int32_t rdrt_fetch_hit_attributes(rdrt_ctx *c, const rdrt_ray *rays, const rdrt_
 int32_t rdrt_set_tune(rdrt_ctx *c, const rdrt_tune *t) {
   if (!c || !t) return fail(RDRT_ERR_INVALID_ARG, "null argument");
   if ((t->block_size != 64 && t->block_size != 128 && t->block_size != 256) ||
-      (t->lds_entries != 8 && t->lds_entries != 16) || (t->waves != 0 && t->waves != 8) || t->persistent > 1 ||
+      (t->lds_entries != 8 && t->lds_entries != 16) || (t->waves != 0 && t->waves != 6 && t->waves != 8) ||
+      t->persistent > 1 ||
       t->cull > 1 || t->wide > 2 || t->coop > (1u << 22))
     return fail(RDRT_ERR_INVALID_ARG,
-                "block_size 64/128/256, lds_entries 8/16, waves 0/8, persistent 0/1, cull 0/1, wide 0/1/2, "
+                "block_size 64/128/256, lds_entries 8/16, waves 0/6/8, persistent 0/1, cull 0/1, wide 0/1/2, "
                 "coop <= 2^22");
   TraceTune n;
   n.block = t->block_size;
@@ -730,7 +731,7 @@ int32_t rdrt_autotune(rdrt_ctx *c, const rdrt_trace_args *a, uint32_t reps, rdrt
   float chosen_ms = -1.0f;
   for (uint32_t block : {64u, 128u, 256u})
     for (uint32_t lds : {8u, 16u})
-      for (uint32_t waves : {0u, 8u})
+      for (uint32_t waves : {0u, 6u, 8u})
         for (uint32_t shape = 0; shape < 12; shape++) {
           TraceTune t;
           t.block = block;

@@ -327,7 +327,7 @@ int32_t rdrt_ao_autotune(rdrt_ctx *c, const rdrt_ao_params *prm, void *accum, ui
   float chosen_ms = -1.0f;
   for (uint32_t block : {64u, 128u, 256u})
     for (uint32_t lds : {8u, 16u})
-      for (uint32_t waves : {0u, 8u})
+      for (uint32_t waves : {0u, 6u, 8u})
         for (uint32_t shape = 0; shape < 18 && rc == RDRT_OK; shape++) {
           TraceTune t;
           t.block = block;

@@ -634,8 +634,8 @@ __device__ __forceinline__ void trace_coop_one(const TraceParams &p, uint32_t ri
   }
 }
 
-template <int K, int S, int B, int LAYOUT>
-__global__ __launch_bounds__(B) void k_trace_coop(TraceParams p) {
+template <int K, int S, int B, int W, int LAYOUT>
+__global__ __launch_bounds__(B, W) void k_trace_coop(TraceParams p) {
   __shared__ uint32_t lds[K * B];
   const uint32_t n0 = live_first(p), n1 = live_second(p);
   if (n0 + n1 > p.coop_max) return;  // the one-ray-per-lane kernel traces launches this large
@@ -725,7 +725,7 @@ hipError_t launch_fast_kw(const TraceParams &p0, bool persist, uint32_t coop, hi
     const uint32_t slots = slots_of(p) + (p.count2 ? p.slots2 : 0u);
     const uint32_t groups = std::min(p.coop_max, slots);
     const uint32_t grid = (groups * kCoopLanes + B - 1) / B;
-    if (grid) hipLaunchKernelGGL((k_trace_coop<K, S, B, LAYOUT>), dim3(grid), dim3(B), 0, st, p);
+    if (grid) hipLaunchKernelGGL((k_trace_coop<K, S, B, W, LAYOUT>), dim3(grid), dim3(B), 0, st, p);
   }
   if (persist) {
     if (!p.work) return hipErrorInvalidValue;
@@ -755,6 +755,10 @@ hipError_t launch_fast_s(const TraceParams &p, const TraceTune &t, hipStream_t s
   if (t.waves >= 8) {
     if (t.lds <= 8) return launch_fast_kw<K8, S, B, 8, LAYOUT>(p, persist, t.coop, st);
     return launch_fast_kw<K16, S, B, 8, LAYOUT>(p, persist, t.coop, st);
+  }
+  if (t.waves >= 6) {
+    if (t.lds <= 8) return launch_fast_kw<K8, S, B, 6, LAYOUT>(p, persist, t.coop, st);
+    return launch_fast_kw<K16, S, B, 6, LAYOUT>(p, persist, t.coop, st);
   }
   if (t.lds <= 8) return launch_fast_kw<K8, S, B, 1, LAYOUT>(p, persist, t.coop, st);
   return launch_fast_kw<K16, S, B, 1, LAYOUT>(p, persist, t.coop, st);

@@ -80,7 +80,7 @@ uint32_t stack_entries_for(uint32_t push_depth);
 struct TraceTune {
   uint32_t block = 256;  // threads per block: 64, 128 or 256
   uint32_t lds = 16;     // stack entries per lane kept in LDS: 8 or 16 (deeper ones in private memory)
-  uint32_t waves = 0;    // 8 = ask the register allocator for 8 waves per SIMD; 0 = its own choice
+  uint32_t waves = 0;    // 8 / 6 = ask the register allocator for 8 / 6 waves per SIMD; 0 = its own choice
   uint32_t persist = 0;  // 1 = resident-size grid whose waves pull 64-ray chunks from a counter
   uint32_t cull = 0;     // 1 = a first pass finishes the rays that miss their scene's root box and
                          //     queues the others densely for the traversal

@@ -209,7 +209,8 @@ def test_launch_shapes_agree_and_autotune(closest_hit_scene, device):
     hits = torch.empty((rays_np.shape[0], 4), dtype=torch.int32, device=device)
     for shape in [(64, 8, 0, 0, 0, 0), (128, 16, 8, 1, 0, 0), (256, 8, 8, 0, 1, 0), (256, 16, 0, 1, 1, 0),
                   (64, 16, 8, 0, 1, 0), (64, 8, 0, 0, 0, 1), (128, 16, 8, 1, 0, 1), (256, 8, 8, 0, 1, 1),
-                  (256, 16, 0, 1, 1, 1), (64, 16, 0, 0, 1, 2), (128, 8, 8, 1, 0, 2), (256, 16, 0, 1, 1, 2)]:
+                  (256, 16, 0, 1, 1, 1), (64, 16, 0, 0, 1, 2), (128, 8, 8, 1, 0, 2), (256, 16, 0, 1, 1, 2),
+                  (64, 16, 6, 1, 0, 1), (256, 8, 6, 0, 1, 2), (128, 16, 6, 0, 0, 0)]:
         sys.set_tune(*shape)
         sys.trace(rays, hits)
         sys.sync()
