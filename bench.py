@@ -400,7 +400,16 @@ def differentiable(args):
         ga = torch.empty_like(albedo)
         ge = torch.empty_like(emission)
         tracer.set_material_params(torch.from_numpy(a_t).to(dev), torch.from_numpy(e_t).to(dev))
-        tracer.render(cam, cfg, out=target)
+    system.sync()
+    # launch shape: autotuned on forward frames of this workload
+    if args.tune:
+        system.set_tune(*[int(x) for x in args.tune.split(",")])
+        tune = system.tune()
+    elif not args.no_autotune:
+        tune = tracer.autotune(cam, cfg, out=target, reps=3)
+    else:
+        tune = system.tune()
+    tracer.render(cam, cfg, out=target)
     system.sync()
     fwd_cfg = cfg
     rec_cfg = dataclasses.replace(cfg, record=True)
@@ -456,7 +465,7 @@ def differentiable(args):
         "ms_forward_events": round(ev_f, 4), "ms_forward_backward": round(ms_fb, 4),
         "ms_forward_backward_events": round(ev_fb, 4), "fwd_bwd_over_fwd": round(ms_fb / ms_f, 4),
         "ms_backward_kernel": round(bwd_ms, 4), "loss": float(loss.item()),
-        "extension_rays": st["extension_rays"], "build_upload_ms": round(build_upload_ms, 1),
+        "extension_rays": st["extension_rays"], "build_upload_ms": round(build_upload_ms, 1), "launch_shape": tune,
     }
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
@@ -542,6 +551,14 @@ def differentiable_geometry(args):
     tsys.close()
     import dataclasses
 
+    # launch shape: autotuned on forward frames of all 16 views
+    if args.tune:
+        system.set_tune(*[int(x) for x in args.tune.split(",")])
+        tune = system.tune()
+    elif not args.no_autotune:
+        tune = tracer.autotune(cams, cfg, out=img, reps=1)
+    else:
+        tune = system.tune()
     rec_cfg = dataclasses.replace(cfg, record=True)
 
     def forward():
@@ -578,7 +595,7 @@ def differentiable_geometry(args):
         "ms_forward_backward": round(ms_fb, 4), "ms_forward_backward_events": round(ev_fb, 4),
         "fwd_bwd_over_fwd": round(ms_fb / ms_f, 4), "loss": float(loss.item()),
         "extension_rays": st["extension_rays"], "scene_gen_ms": round(gen_ms, 1),
-        "build_upload_ms": round(build_upload_ms, 1),
+        "build_upload_ms": round(build_upload_ms, 1), "launch_shape": tune,
     }
     extra = {}
     if not args.no_cpu_baseline:

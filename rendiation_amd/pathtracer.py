@@ -147,17 +147,28 @@ class PathTracer:
 
     def autotune(self, camera, cfg: PathTracingConfig, out=None, reps: int = 3) -> dict:
         """Time every traversal launch shape on eager frames of this camera
-        and configuration and keep the fastest for the system (``out`` is
-        overwritten); returns the choice, its ``ms`` the frame time."""
+        (or list of cameras, all views in each frame) and configuration and
+        keep the fastest for the system (``out`` is overwritten); returns the
+        choice, its ``ms`` the frame time."""
         import torch
 
         if cfg.record or cfg.diagnostics:
             raise ValueError("tune on plain frames (record and diagnostics off)")
+        cams = list(camera) if isinstance(camera, (list, tuple)) else [camera]
+        nv = len(cams)
         if out is None:
-            out = torch.empty((cfg.height, cfg.width, 3), dtype=torch.float32, device=f"cuda:{self.system.device}")
-        if not out.is_contiguous() or out.numel() != cfg.width * cfg.height * 3:
-            raise ValueError("out must be a contiguous height*width*3 float32 tensor")
-        prm = cfg.to_c(camera)
+            out = torch.empty((nv, cfg.height, cfg.width, 3), dtype=torch.float32, device=f"cuda:{self.system.device}")
+        if not out.is_contiguous() or out.numel() != nv * cfg.width * cfg.height * 3:
+            raise ValueError("out must be a contiguous views*height*width*3 float32 tensor")
+        prm = cfg.to_c(cams[0])
+        views = None
+        if nv > 1:  # every view in each timed frame, as render does
+            views = (L.f32 * (20 * nv))()
+            for v, cam in enumerate(cams):
+                views[20 * v:20 * v + 16] = [float(x) for x in cam.view_projection_inv().T.reshape(-1)]
+                views[20 * v + 16:20 * v + 19] = [float(np.float32(x)) for x in cam.position]
+            prm.views = nv
+            prm.view_params = C.cast(views, C.c_void_p)
         t = L.RdrtTune()
         L.check(self.system._L.rdrt_pt_autotune(self.system._h, C.byref(prm), int(out.data_ptr()), reps, C.byref(t)))
         return self.system.tune()
