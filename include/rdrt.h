@@ -264,7 +264,7 @@ typedef struct rdrt_task_graph rdrt_task_graph;
 /* DeviceTaskGraphBuildSource::build (mod.rs:97-160); at most 8 groups. */
 int32_t rdrt_task_graph_create(rdrt_ctx *ctx, const rdrt_task_group_desc *groups, uint32_t ngroups, uint32_t capacity,
                                rdrt_task_graph **out);
-int32_t rdrt_task_graph_destroy(rdrt_task_graph *g);
+int32_t rdrt_task_graph_destroy(rdrt_task_graph *g); /* before its context is destroyed */
 /* dispatch_allocate_init_task (mod.rs:223-303): `count` root tasks of
  * `group`, payload = spawn index, no parent. */
 int32_t rdrt_task_graph_spawn(rdrt_task_graph *g, uint32_t group, uint32_t count);
