@@ -68,12 +68,16 @@ __device__ __forceinline__ bool take_child(const Graph &gr, bool pred, uint32_t 
 // take its payload and resolve; new tasks are queued for the traversal.
 __global__ void k_trace_pre(AoLaunch L) {
   const Group &T = L.gr.g[kAoTrace];
+  uint32_t *cnt = L.tcount + 2 * L.qpar;
+  // the other parity's counts (last used by the previous round, whose
+  // launches have all finished) start the next round at zero
+  if (blockIdx.x == 0 && threadIdx.x < 2) L.tcount[2 * (L.qpar ^ 1u) + threadIdx.x] = 0u;
   tg::for_each_polled(T, [&](uint32_t, bool valid, uint32_t idx) {
     uint32_t *w = pw(T, idx);
     const bool fresh = valid && T.stage[idx] == 0u;
     const bool occ = fresh && w[0] != 0u;
-    const uint32_t p0 = tg::block_add(L.tcount + 0, fresh && !occ);
-    const uint32_t p1 = tg::block_add(L.tcount + 1, occ);
+    const uint32_t p0 = tg::block_add(cnt + 0, fresh && !occ);
+    const uint32_t p1 = tg::block_add(cnt + 1, occ);
     if (fresh) (occ ? L.tq1 : L.tq0)[occ ? p1 : p0] = idx;
     const bool waiting = valid && !fresh;
     float v = 0.0f;
@@ -86,11 +90,14 @@ __global__ void k_trace_pre(AoLaunch L) {
 // Second half, after the traversal: each traced task spawns the closest-hit
 // task of its hit group (ao.rs:20-47: every instance maps ray type 0 to the
 // occlusion closest hit, ray type 1 to the one that writes 0) or the miss
-// task, passes it the payload and sleeps.  The last block clears the queue
-// counts for the next round.
+// task, passes it the payload and sleeps.
 __global__ void k_trace_post(AoLaunch L) {
   const Group &T = L.gr.g[kAoTrace];
-  const uint32_t n0 = L.tcount[0], n1 = L.tcount[1];
+  const uint32_t n0 = L.tcount[2 * L.qpar], n1 = L.tcount[2 * L.qpar + 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {  // nothing else writes these during the launch
+    L.tcount[4] += n0;
+    L.tcount[5] += n1;
+  }
   for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < n0 + n1; j0 += gridDim.x * blockDim.x) {
     const uint32_t j = j0 + threadIdx.x;
     const bool valid = j < n0 + n1;
@@ -112,20 +119,6 @@ __global__ void k_trace_post(AoLaunch L) {
       T.stage[idx] = 1u;
     }
     tg::end_poll(L.gr, kAoTrace, valid, idx, false);
-  }
-  __shared__ bool last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    last = atomicAdd(L.tcount + 2, 1u) == gridDim.x - 1u;
-  }
-  __syncthreads();
-  if (last && threadIdx.x == 0) {
-    L.tcount[3] += n0;
-    L.tcount[4] += n1;
-    L.tcount[0] = 0u;
-    L.tcount[1] = 0u;
-    L.tcount[2] = 0u;
   }
 }
 
@@ -274,7 +267,8 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
   Graph &gr = L.gr;
   hipError_t e = tg::enqueue_reset(gr, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(64), 0, st, L.tcount, first_tile ? 5u : 3u);
+  hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(64), 0, st, L.tcount, first_tile ? 6u : 4u);
+  L.qpar = 0u;
   e = tg::enqueue_spawn_roots(gr, kAoRayGen, L.tile_w * tile_h, st);
   if (e != hipSuccess) return e;
   const uint32_t ntrace = gr.g[kAoTrace].size;
@@ -287,7 +281,7 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
   tp.counters = nullptr;
   tp.n = ntrace + 1;
   tp.queue = L.tq0;
-  tp.count = L.tcount;
+  tp.count = L.tcount;  // per round below
   // each queue holds at most one ray per pixel of the tile
   const uint32_t per_queue = L.tile_w * tile_h;
   tp.slots = per_queue;
@@ -299,7 +293,7 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
   tp.hits2 = L.hits;
   tp.hitx2 = L.hitx;
   tp.queue2 = L.tq1;
-  tp.count2 = L.tcount + 1;
+  tp.count2 = L.tcount + 1;  // per round below
   tp.slots2 = per_queue;
   tp.flags2 = RDRT_RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH;  // ao.rs:276
   auto poll = [&](uint32_t gid) -> hipError_t {
@@ -307,9 +301,13 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
     switch (gid) {
       case kAoTrace: {
         hipLaunchKernelGGL(k_trace_pre, dim3(grid), dim3(kBlock), 0, st, L);
-        hipError_t e = launch_trace(tp, RDRT_TRAVERSAL_FAST, push_depth, tune, st);
+        TraceParams q = tp;
+        q.count = L.tcount + 2 * L.qpar;
+        q.count2 = L.tcount + 2 * L.qpar + 1;
+        hipError_t e = launch_trace(q, RDRT_TRAVERSAL_FAST, push_depth, tune, st);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_trace_post, dim3(grid), dim3(kBlock), 0, st, L);
+        hipLaunchKernelGGL(k_trace_post, dim3(grid_for(per_queue * 2)), dim3(kBlock), 0, st, L);
+        L.qpar ^= 1u;  // the next round's trace poll uses the other counts
         break;
       }
       case kAoRayGen:

@@ -380,11 +380,11 @@ int32_t rdrt_ao_ray_counts(rdrt_ctx *c, uint32_t out[2]) {
   int32_t rc = need_device(c);
   if (rc) return rc;
   if (!c->ao || !c->ao->capacity) return fail(RDRT_ERR_STATE, "no AO frame rendered yet");
-  uint32_t h[5];
+  uint32_t h[6];
   RDRT_HIP(hipMemcpyAsync(h, c->ao->counts.ptr, sizeof(h), hipMemcpyDeviceToHost, c->work_stream));
   RDRT_HIP(hipStreamSynchronize(c->work_stream));
-  out[0] = h[3];
-  out[1] = h[4];
+  out[0] = h[4];
+  out[1] = h[5];
   return RDRT_OK;
 }
 
