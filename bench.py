@@ -907,7 +907,7 @@ def ambient_occlusion(args):
     rays0 = sum(r.ray_counts())
     config = {
         "workload": f"ao: 64 spheres/tori at 256x256 (8.39M tris), {W}x{H} (1920x1080 clamped to 512^2 pixels), "
-                    "1 sample per step, task-graph pipeline (trace, ray gen, 2 closest hit, miss), 8 rounds",
+                    "1 sample per step, task-graph pipeline (trace, ray gen, 2 closest hit, miss), round hint 8 (5 run)",
         "tris": info.tris, "width": W, "height": H, "camera_rays": prim, "occlusion_rays": occ,
         "ms_events": round(ms_ev, 4), "graph_replay": True, "launch_shape": tune,
         "drained": st.is_empty() and not any(st.errors),

@@ -287,7 +287,8 @@ typedef struct rdrt_ao_params {
   uint32_t format;       /* RDRT_AO_FORMAT_* */
   float view_projection_inv[16];
   float camera_position[3];
-  uint32_t rounds;      /* task-graph rounds per tile; 0 = 8 (ao.rs:300) */
+  uint32_t rounds;      /* task-graph rounds per tile; 0 = 8 (ao.rs:300).  Rounds past the fifth are
+                           skipped: every AO task has finished by then, whatever the scene */
   uint32_t tile_pixels; /* pixels per task-graph execution, whole rows; 0 = 512 * 512 (wavefront_compute/mod.rs:134) */
   uint32_t use_graph;   /* replay a captured graph of the call */
   uint32_t pad[2];

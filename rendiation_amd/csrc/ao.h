@@ -27,6 +27,16 @@ namespace rdrt {
 
 constexpr uint32_t kAoTrace = 0, kAoRayGen = 1, kAoClosest = 2, kAoClosestTest = 3, kAoMiss = 4, kAoGroups = 5;
 constexpr uint32_t kAoMaxSample = 256;  // ao.rs:5
+// Rounds after which every AO task has finished, whatever the scene.  With
+// the groups polled in the order trace, ray gen, closest, closest (occlusion),
+// miss: round 1 ray gen spawns the camera trace; round 2 traces it and its
+// closest-hit or miss child runs (a miss resolves and wakes the trace);
+// round 3 resolves a missed camera trace and its ray gen, and traces the
+// occlusion ray whose child resolves; round 4 resolves the occlusion trace
+// and the closest hit; round 5 resolves the camera trace and the ray gen.
+// Later rounds find every group empty, so running min(rounds, 5) gives the
+// same buffer and the same drained graph.
+constexpr uint32_t kAoChainRounds = 5;
 // Words of the per-call frame block (device): view_projection_inv[16],
 // camera position[3], pad, local occlusion direction[3], sample index.
 constexpr uint32_t kAoFrameWords = 24;

@@ -225,7 +225,8 @@ int32_t rdrt_ao_render(rdrt_ctx *c, const rdrt_ao_params *prm, void *accum) {
   uint32_t rows = tile_pixels / prm->width;
   if (rows == 0) rows = 1;
   if (rows > prm->height) rows = prm->height;
-  const uint32_t rounds = prm->rounds ? prm->rounds : 8u;
+  const uint32_t rounds_hint = prm->rounds ? prm->rounds : 8u;  // ao.rs:300
+  const uint32_t rounds = rounds_hint < kAoChainRounds ? rounds_hint : kAoChainRounds;
   rc = reserve_ao(c, rows * prm->width);
   if (rc) return rc;
   AoState &s = *c->ao;
