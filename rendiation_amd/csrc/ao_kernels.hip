@@ -269,7 +269,7 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(64), 0, st, L.tcount, first_tile ? 6u : 4u);
   L.qpar = 0u;
-  e = tg::enqueue_spawn_roots(gr, kAoRayGen, L.tile_w * tile_h, st);
+  e = tg::enqueue_spawn_fresh(gr, kAoRayGen, L.tile_w * tile_h, st);
   if (e != hipSuccess) return e;
   const uint32_t ntrace = gr.g[kAoTrace].size;
   TraceParams tp = scene;

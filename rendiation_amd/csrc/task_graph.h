@@ -265,6 +265,9 @@ hipError_t enqueue_rounds(Graph &gr, uint32_t rounds, PollFn poll, hipStream_t s
 // dispatch_allocate_init_task (mod.rs:223-303) for the generic task kinds:
 // `count` root tasks whose payload word 0 is their spawn index.
 hipError_t enqueue_spawn_roots(Graph &gr, uint32_t gid, uint32_t count, hipStream_t st);
+// The same right after enqueue_reset (every slot empty): no prepare, no
+// atomics, counts written by one thread.  count <= the group's slots.
+hipError_t enqueue_spawn_fresh(Graph &gr, uint32_t gid, uint32_t count, hipStream_t st);
 
 // Generic task kinds (the C-ABI task graph): kind 0 resolves on its first
 // poll; kind 1 spawns one task of group `child_group` with payload

@@ -114,6 +114,28 @@ __global__ void k_spawn_roots(Graph gr, uint32_t gid, uint32_t count) {
   }
 }
 
+// Root spawns right after a reset (every slot empty, nothing active): root
+// i takes slot i + 1 and active entry i, and the counts are known on the
+// host, so one thread writes them.
+__global__ void k_spawn_fresh(Graph gr, uint32_t gid, uint32_t k) {
+  const Group &G = gr.g[gid];
+  uint32_t *list = active_list(G, G.cur);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x) {
+    const uint32_t idx = i + 1u;  // empty[size - 1 - i] after a reset
+    G.state[idx] = kWaken;
+    G.parent[idx] = kNone;
+    G.ptype[idx] = kNone;
+    G.stage[idx] = 0u;
+    G.child[idx] = kNone;
+    if (G.words) G.payload[(size_t)idx * G.words] = i;
+    list[i] = idx;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *empty_count(G) = G.size - k;
+    *active_count(G) = k;
+  }
+}
+
 // The generic kinds: 0 = BaseShaderFuture (ready at once); 1 = then(spawn a
 // child of `child_group`, await it) (future/then.rs semantics).
 __global__ void k_generic_poll(Graph gr, uint32_t gid, uint32_t kind, uint32_t child_group) {
@@ -178,6 +200,13 @@ hipError_t enqueue_spawn_roots(Graph &gr, uint32_t gid, uint32_t count, hipStrea
   hipError_t e = enqueue_prepare(gr, gid, st);
   if (e != hipSuccess || count == 0) return e;
   hipLaunchKernelGGL(k_spawn_roots, dim3(grid_for(count, kListGrid)), dim3(kBlock), 0, st, gr, gid, count);
+  return hipGetLastError();
+}
+
+hipError_t enqueue_spawn_fresh(Graph &gr, uint32_t gid, uint32_t count, hipStream_t st) {
+  if (count > gr.g[gid].size) return hipErrorInvalidValue;
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_spawn_fresh, dim3(grid_for(count, kListGrid)), dim3(kBlock), 0, st, gr, gid, count);
   return hipGetLastError();
 }
 
