@@ -57,14 +57,35 @@ __device__ __forceinline__ V3 normal_to_world(const float *inv, V3 n) {
 
 // path -> view, pixel (px, py) within the view, sample index n, and the
 // view's sample seed (seed + view)
+// Image pixel (views one after the other, each row-major) of pixel slot
+// gpix = path / spp.  When both sides are multiples of 8, consecutive slots
+// walk 8 x 8 tiles in row-major tile order, so the paths of one wave start
+// in a square of the image instead of a 64-pixel strip of one row (more
+// coherent rays); otherwise slots are row-major.
+__device__ __forceinline__ uint32_t image_pixel(const PtLaunch &P, uint32_t gpix, uint32_t &px, uint32_t &py,
+                                                uint32_t &view) {
+  const uint32_t npix = P.width * P.height;
+  view = gpix / npix;
+  const uint32_t s = gpix - view * npix;
+  if ((P.width & 7u) == 0u && (P.height & 7u) == 0u) {
+    const uint32_t tile = s >> 6, w = s & 63u, ntx = P.width >> 3;
+    px = (tile % ntx) * 8u + (w & 7u);
+    py = (tile / ntx) * 8u + (w >> 3);
+  } else {
+    px = s % P.width;
+    py = s / P.width;
+  }
+  return view * npix + py * P.width + px;
+}
+__device__ __forceinline__ uint32_t image_pixel(const PtLaunch &P, uint32_t gpix) {
+  uint32_t px, py, view;
+  return image_pixel(P, gpix, px, py, view);
+}
+
 __device__ __forceinline__ void pixel_of(const PtLaunch &P, uint32_t path, uint32_t &px, uint32_t &py, uint32_t &n,
                                          uint32_t &view, uint32_t &seed) {
-  const uint32_t gpix = path / P.spp, npix = P.width * P.height;
-  view = gpix / npix;
-  const uint32_t pixel = gpix - view * npix;
+  image_pixel(P, path / P.spp, px, py, view);
   n = P.sample_base + path % P.spp;
-  px = pixel % P.width;
-  py = pixel / P.width;
   seed = P.seed + view;
 }
 
@@ -224,7 +245,7 @@ __device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &ke
     }
     if (REPLAY) {
       const float w = __uint_as_float(P.record[(size_t)b * P.npaths + path].y);
-      const V3 a_pix = load3(P.adj + 3 * (size_t)(path / P.spp)) * (1.0f / (float)P.spp);
+      const V3 a_pix = load3(P.adj + 3 * (size_t)image_pixel(P, path / P.spp)) * (1.0f / (float)P.spp);
       V3 g_ns = v3(0.0f, 0.0f, 0.0f);
       if (w != 0.0f) g_ns = wi * (pt::dot3(a_pix, beta * albedo * load3(P.sun_radiance)) * kInvPi);
       const float cos_wo = pt::dot3(ns, wo);
@@ -309,12 +330,12 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_nee(PtLaunch P, uint32_t b) {
 }
 
 __global__ __launch_bounds__(kPtBlock) void k_pt_finalize(PtLaunch P) {
-  const uint32_t pixel = blockIdx.x * kPtBlock + threadIdx.x;
-  if (pixel >= P.nviews * P.width * P.height) return;
+  const uint32_t slot = blockIdx.x * kPtBlock + threadIdx.x;
+  if (slot >= P.nviews * P.width * P.height) return;
   V3 s = v3(0.0f, 0.0f, 0.0f);
-  for (uint32_t k = 0; k < P.spp; k++) s = s + load3(P.L + 3 * ((size_t)pixel * P.spp + k));
+  for (uint32_t k = 0; k < P.spp; k++) s = s + load3(P.L + 3 * ((size_t)slot * P.spp + k));
   const float inv = 1.0f / (float)P.spp;
-  store3(P.out + 3 * (size_t)pixel, s * inv);
+  store3(P.out + 3 * (size_t)image_pixel(P, slot), s * inv);
 }
 
 // standalone stable compaction (same machinery as the shade kernel)
@@ -405,7 +426,7 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_backward(PtLaunch P, const floa
       }
     }
     if (m == 0) continue;
-    const uint32_t pixel = path / P.spp;
+    const uint32_t pixel = image_pixel(P, path / P.spp);
     const V3 a_pix = load3(adj + 3 * (size_t)pixel) * inv_spp;
     V3 beta[S];
     beta[0] = v3(1.0f, 1.0f, 1.0f);

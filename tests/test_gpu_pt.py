@@ -134,15 +134,17 @@ def test_floor_on_gpu(device):
     assert st["shadow_rays"][:1] == [32 * 32]
 
 
-@pytest.mark.parametrize("spp,wide,coop", [(1, 0, 0), (4, 0, 0), (1, 1, 0), (1, 2, 0), (1, 1, 1 << 22),
-                                           (1, 1, 2000)])
-def test_parametric_scene_against_fp64_oracle(spp, wide, coop, device):
+@pytest.mark.parametrize("spp,wide,coop,height", [(1, 0, 0, 108), (4, 0, 0, 108), (1, 1, 0, 108), (1, 2, 0, 108),
+                                                  (1, 1, 1 << 22, 108), (1, 1, 2000, 108), (1, 1, 0, 112),
+                                                  (2, 1, 262144, 112)])
+def test_parametric_scene_against_fp64_oracle(spp, wide, coop, height, device):
     """coop = 1 << 22: every traversal launch through the cooperative kernel;
-    2000: only the small tail launches."""
+    2000: only the small tail launches.  Height 112 (both sides multiples of
+    8): paths walk the image in 8 x 8 tiles; 108: row by row."""
     scene = synth.pt_scene(seed=1, n_meshes=64, tess=48)
     cam = Camera()
-    cfg = PathTracingConfig(width=192, height=108, spp=spp, max_bounces=4, rr_start=2, seed=5)
-    _say(f"parametric scene 64 meshes tess 48, 192x108 spp {spp}, wide={wide}, coop={coop}")
+    cfg = PathTracingConfig(width=192, height=height, spp=spp, max_bounces=4, rr_start=2, seed=5)
+    _say(f"parametric scene 64 meshes tess 48, 192x{height} spp {spp}, wide={wide}, coop={coop}")
     s = _system(scene)
     s.set_tune(wide=wide, coop=coop)
     img, st = _render(s, cam, cfg)
