@@ -1971,12 +1971,17 @@ int orc_pick_all(const orc_scene *s, uint32_t blas, uint32_t geometry, const flo
 // mean), FP32 with the reference operation order.
 namespace {
 
-// sampler.rs:11-16 pcg and :64-71 next()
-float ao_pcg_next(uint32_t &state) {
+// sampler.rs:11-16 pcg: one LCG step, then the RXS-M-XS hash of the new state
+uint32_t ao_pcg_bits(uint32_t &state) {
   const uint32_t prev = state * 747796405u + 2891336453u;
   const uint32_t word = ((prev >> ((prev >> 28u) + 4u)) ^ prev) * 277803737u;
   state = prev;
-  const uint32_t r = (word >> 22u) ^ word;
+  return (word >> 22u) ^ word;
+}
+
+// sampler.rs:64-71 next(): the top 23 hash bits as a mantissa in [1, 2), less 1
+float ao_pcg_next(uint32_t &state) {
+  const uint32_t r = ao_pcg_bits(state);
   const uint32_t bits = 0x3f800000u | (r >> 9u);
   float f;
   std::memcpy(&f, &bits, 4);
@@ -2050,6 +2055,23 @@ float ao_value(const orc_scene *s, const orc_ao_params &P, uint32_t x, uint32_t 
 }  // namespace
 
 extern "C" {
+
+// The per-pixel sampler of the AO ray generation: seed = xxhash32(x, y,
+// sample_index) (sampler.rs:47-55), then n calls of next(); states[k],
+// bits[k] and values[k] are the state after call k, its 32-bit hash and the
+// float it returns.  Returns the seed.
+uint32_t orc_ao_sampler(uint32_t x, uint32_t y, uint32_t sample_index, uint32_t n, uint32_t *states, uint32_t *bits,
+                        float *values) {
+  const uint32_t seed = xxhash32(x, y, sample_index);
+  uint32_t st = seed;
+  for (uint32_t k = 0; k < n; ++k) {
+    uint32_t probe = st;
+    bits[k] = ao_pcg_bits(probe);
+    values[k] = ao_pcg_next(st);
+    states[k] = st;
+  }
+  return seed;
+}
 
 void orc_ao_hemisphere_sample(uint32_t s, float out[3]) {
   // hammersley_2d(s, 256) (sampling.rs:33-50): (s / 256, radical inverse)

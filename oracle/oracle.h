@@ -173,6 +173,10 @@ typedef struct orc_ao_params {
   float camera_position[3];
   float pad2;
 } orc_ao_params;
+/* The AO pixel sampler: seed from (x, y, sample_index), then n steps; per
+ * step the state after it, its 32-bit hash and the float in [0, 1). */
+uint32_t orc_ao_sampler(uint32_t x, uint32_t y, uint32_t sample_index, uint32_t n, uint32_t *states, uint32_t *bits,
+                        float *values);
 /* The tangent-space occlusion direction of sample index s. */
 void orc_ao_hemisphere_sample(uint32_t s, float out[3]);
 /* One AO sample: values (optional) gets each pixel's 0 / 1, accum (f32,

@@ -76,6 +76,7 @@ _SIG = {
     "orc_pick_box": (C.c_int, [P, P, P, P]),
     "orc_pick": (C.c_int, [P, u32, u32, P, u32, u32, C.c_int, P]),
     "orc_ao_hemisphere_sample": (None, [u32, P]),
+    "orc_ao_sampler": (u32, [u32, u32, u32, u32, P, P, P]),
     "orc_ao_render": (C.c_int, [P, P, P, P, P, C.c_int]),
     "orc_pick_all": (C.c_int, [P, u32, u32, P, u32, P, u32, P]),
     "orc_van_der_corput": (f32, [u32, u32]),
@@ -406,6 +407,15 @@ def pick_triangle(o, d, a, b, c, side=2) -> float:
 def pick_box(o, d, bmin, bmax) -> bool:
     v = [np.ascontiguousarray(x, dtype=np.float32) for x in (o, d, bmin, bmax)]
     return bool(lib().orc_pick_box(*[x.ctypes.data for x in v]))
+
+
+def ao_sampler(x: int, y: int, sample_index: int, n: int):
+    """(seed, states, hashes, floats) of n steps of the AO pixel sampler."""
+    states = np.zeros(n, dtype=np.uint32)
+    bits = np.zeros(n, dtype=np.uint32)
+    vals = np.zeros(n, dtype=np.float32)
+    seed = lib().orc_ao_sampler(x, y, sample_index, n, states.ctypes.data, bits.ctypes.data, vals.ctypes.data)
+    return int(seed), states, bits, vals
 
 
 def ao_hemisphere_sample(s: int) -> np.ndarray:
