@@ -14,7 +14,8 @@ namespace rdrt {
 
 void PtState::free_all() {
   for (Scratch *s : {&rays, &shadow_rays, &hits, &shadow_hits, &hitx, &beta, &L, &nee, &qa, &qb, &qs, &counts,
-                     &tmp_a, &tmp_b, &blk, &materials, &diag_counters, &diag_totals, &record, &views, &work})
+                     &tmp_a, &tmp_b, &blk, &materials, &diag_counters, &diag_totals, &record, &views, &work, &det_acc,
+                     &det_flags, &det_hdr, &grec, &loss_part})
     s->free();
   host_views.clear();
   materials_dirty = true;
@@ -34,6 +35,17 @@ void PtState::free_all() {
 }  // namespace rdrt
 
 namespace {
+
+// Device scratch for n reproducible sums.
+int32_t reserve_det(PtState &s, size_t n, DetBufs &db) {
+  RDRT_HIP(s.det_acc.reserve(std::max<size_t>(n, 1) * sizeof(det::u64)));
+  RDRT_HIP(s.det_flags.reserve(std::max<size_t>(n, 1) * sizeof(uint32_t)));
+  RDRT_HIP(s.det_hdr.reserve(sizeof(det::Header)));
+  db.acc = (det::u64 *)s.det_acc.ptr;
+  db.flags = (uint32_t *)s.det_flags.ptr;
+  db.h = (det::Header *)s.det_hdr.ptr;
+  return RDRT_OK;
+}
 
 int32_t reserve_pt(rdrt_ctx *c, uint32_t npaths) {
   PtState &s = c->pt;
@@ -406,7 +418,10 @@ int32_t rdrt_pt_backward(rdrt_ctx *c, const float *adj, float *grad_albedo, floa
   L.materials = (const Material *)s.materials.ptr;
   if (s.nmaterials_uploaded != L.nmaterials)
     return fail(RDRT_ERR_STATE, "the material count changed since the recorded frame");
-  RDRT_HIP(enqueue_pt_backward(L, adj, grad_albedo, grad_emission, c->work_stream));
+  DetBufs db;
+  rc = reserve_det(s, 6 * (size_t)L.nmaterials, db);
+  if (rc) return rc;
+  RDRT_HIP(enqueue_pt_backward(L, adj, grad_albedo, grad_emission, db, c->work_stream));
   return RDRT_OK;
 }
 
@@ -428,8 +443,14 @@ int32_t rdrt_pt_backward_geometry(rdrt_ctx *c, const float *adj, float *grad_pos
   L.g_pos = grad_pos;
   L.g_nrm = grad_nrm;  // zeroed by the replay; stays zero without vertex normals
   L.nvertices = (uint32_t)(c->dev.vertices.count / 3);
+  DetBufs db;
+  rc = reserve_det(s, 6 * (size_t)L.nvertices, db);
+  if (rc) return rc;
+  RDRT_HIP(s.grec.reserve(std::max<size_t>((size_t)L.max_bounces * L.npaths, 1) * 3 * sizeof(uint4)));
+  L.grec = (uint4 *)s.grec.ptr;
+  L.det_h = db.h;
   const TraceParams tp = scene_trace_params(c);
-  RDRT_HIP(enqueue_pt_replay(L, tp, push_depth(c, c->tune), c->tune, c->work_stream));
+  RDRT_HIP(enqueue_pt_replay(L, tp, push_depth(c, c->tune), c->tune, db, c->work_stream));
   return RDRT_OK;
 }
 
@@ -437,7 +458,8 @@ int32_t rdrt_l2_adjoint(rdrt_ctx *c, const float *a, const float *b, uint32_t n,
   if (!c || !loss || (n && (!a || !b || !adj))) return fail(RDRT_ERR_INVALID_ARG, "null argument");
   int32_t rc = need_device(c);
   if (rc) return rc;
-  RDRT_HIP(enqueue_l2_adjoint(a, b, n, adj, loss, c->work_stream));
+  RDRT_HIP(c->pt.loss_part.reserve(l2_adjoint_parts(n) * sizeof(double)));
+  RDRT_HIP(enqueue_l2_adjoint(a, b, n, adj, loss, (double *)c->pt.loss_part.ptr, c->work_stream));
   return RDRT_OK;
 }
 

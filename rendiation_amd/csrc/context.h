@@ -86,6 +86,9 @@ struct PtState {
   uint32_t pixels = 0;
   Scratch rays, shadow_rays, hits, shadow_hits, hitx, beta, L, nee, qa, qb, qs, counts, tmp_a, tmp_b, blk;
   Scratch materials, record, views, work;
+  // adjoint scratch: reproducible sums (int64 sums, flags, header), the
+  // replay's per-bounce gradient terms, the loss partials
+  Scratch det_acc, det_flags, det_hdr, grec, loss_part;
   std::vector<float> host_views;  // contents of `views`
   uint32_t nmaterials_uploaded = 0;
   bool materials_dirty = true;

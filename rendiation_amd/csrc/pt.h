@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/rdrt.h"
+#include "det_sum.h"
 #include "rdrt_layout.h"
 #include "trace_kernels.h"
 
@@ -54,10 +55,22 @@ struct PtLaunch {
   uint2 *record;
 
   // geometry adjoint (replay frames): pixel adjoint (views * width * height
-  // * 3), gradients w.r.t. vertex positions and normals (nvertices * 3)
+  // * 3), gradients w.r.t. vertex positions and normals (nvertices * 3),
+  // per-(bounce, queue slot) gradient terms (max_bounces * npaths * 3
+  // uint4) and the header of their reproducible sum
   const float *adj;
   float *g_pos, *g_nrm;
   uint32_t nvertices;
+  uint4 *grec;
+  det::Header *det_h;
+};
+
+// Device scratch of one reproducible accumulation (det_sum.h): n sums and
+// flags and the header.
+struct DetBufs {
+  det::u64 *acc;
+  uint32_t *flags;
+  det::Header *h;
 };
 
 constexpr uint32_t kVertSkyUp = 0xFFFFFFF0u, kVertSkyDown = 0xFFFFFFF1u, kVertEnd = 0xFFFFFFF2u;
@@ -77,19 +90,24 @@ hipError_t enqueue_pt_frame(const PtLaunch &L, const TraceParams &scene_tp, uint
                             const TraceTune &tune, hipStream_t stream, const PtDiag *diag = nullptr);
 
 // Gradient of sum(adj * frame) w.r.t. material albedo and emission from the
-// recorded vertices of frame L (grads zeroed here, then accumulated).
+// recorded vertices of frame L, written to grad_albedo / grad_emission (3 *
+// nmaterials each).  Reproducible: db holds 6 * nmaterials sums.
 hipError_t enqueue_pt_backward(const PtLaunch &L, const float *adj, float *grad_albedo, float *grad_emission,
-                               hipStream_t stream);
-// Replays the recorded frame L (same kernels, no shadow rays) and
-// accumulates the gradient w.r.t. vertex positions and normals into
-// L.g_pos / L.g_nrm (zeroed here).
+                               const DetBufs &db, hipStream_t stream);
+// Replays the recorded frame L (same kernels, no shadow rays), records the
+// gradient terms of every bounce in L.grec and writes the gradient w.r.t.
+// vertex positions and normals to L.g_pos / L.g_nrm (g_nrm may be null).
+// Reproducible: db holds 6 * nvertices sums.
 hipError_t enqueue_pt_replay(const PtLaunch &L, const TraceParams &scene_tp, uint32_t push_depth,
-                             const TraceTune &tune, hipStream_t stream);
+                             const TraceTune &tune, const DetBufs &db, hipStream_t stream);
 // Packs device albedo / emission arrays (n * 3 each) into Material records.
 hipError_t enqueue_pack_materials(const float *albedo, const float *emission, uint32_t n, Material *out,
                                   hipStream_t stream);
-// adj[i] = 2 (a[i] - b[i]); *loss = sum (a[i] - b[i])^2.
-hipError_t enqueue_l2_adjoint(const float *a, const float *b, uint32_t n, float *adj, double *loss,
+// adj[i] = 2 (a[i] - b[i]); *loss = sum (a[i] - b[i])^2 in FP64, summed in
+// a fixed order (per-block partials in `part`, l2_adjoint_parts(n) doubles,
+// then added in block order).
+uint32_t l2_adjoint_parts(uint32_t n);
+hipError_t enqueue_l2_adjoint(const float *a, const float *b, uint32_t n, float *adj, double *loss, double *part,
                               hipStream_t stream);
 
 // Stable stream compaction of indices whose mask is non-zero (block-local

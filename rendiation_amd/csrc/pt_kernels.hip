@@ -21,6 +21,7 @@
 #include <algorithm>
 
 #include "compact_device.h"
+#include "det_sum.h"
 #include "pt.h"
 #include "pt_device.h"
 
@@ -141,10 +142,37 @@ __device__ V3 recorded_suffix(const PtLaunch &P, uint32_t path, uint32_t first) 
   return T;
 }
 
-__device__ __forceinline__ void atomic_add3(float *g, uint32_t i, V3 v) {
-  unsafeAtomicAdd(g + 3 * (size_t)i, v.x);
-  unsafeAtomicAdd(g + 3 * (size_t)i + 1, v.y);
-  unsafeAtomicAdd(g + 3 * (size_t)i + 2, v.z);
+// Gradient terms of one replayed (path, bounce) w.r.t. the three vertices
+// of the hit triangle.  kind 1 (interpolated normals): normal terms A * w_k
+// and position terms B * -w_k for barycentric weights w = (1 - u - v, u,
+// v); kind 2 (geometric normal n = e1 x e2): position terms -(A + B), A, B;
+// kind 0: none.
+struct GeoTerm {
+  uint32_t i0, i1, i2, kind;
+  V3 A, B;
+  float bu, bv;
+};
+
+// The per-corner vectors of a term: pos[0..2] (and nrm[0..2] for kind 1).
+__device__ __forceinline__ void geo_corners(const GeoTerm &t, V3 pos[3], V3 nrm[3]) {
+  if (t.kind == 1u) {
+    const float bw = 1.0f - t.bu - t.bv;
+    nrm[0] = t.A * bw;
+    nrm[1] = t.A * t.bu;
+    nrm[2] = t.A * t.bv;
+    pos[0] = t.B * -bw;
+    pos[1] = t.B * -t.bu;
+    pos[2] = t.B * -t.bv;
+  } else {
+    pos[0] = (t.A + t.B) * -1.0f;
+    pos[1] = t.A;
+    pos[2] = t.B;
+    nrm[0] = nrm[1] = nrm[2] = v3(0.0f, 0.0f, 0.0f);
+  }
+}
+
+__device__ __forceinline__ uint32_t max_bits3(V3 v) {
+  return max(det::abs_bits(v.x), max(det::abs_bits(v.y), det::abs_bits(v.z)));
 }
 
 // closest hit + miss for one path (ray_hit.rs:6-104, ray_miss.rs:136-158).
@@ -157,7 +185,7 @@ __device__ __forceinline__ void atomic_add3(float *g, uint32_t i, V3 v) {
 // a (ns . wo) / cos_wo; (u, v) are where the fixed object ray meets the
 // triangle.
 template <bool REPLAY>
-__device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &keep, bool &shadow) {
+__device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &keep, bool &shadow, GeoTerm &gt) {
   const uint4 h = *reinterpret_cast<const uint4 *>(P.hits + path);
   const float4 *rq = reinterpret_cast<const float4 *>(P.rays + path);
   const float4 r0 = rq[0], r1 = rq[1];
@@ -260,11 +288,15 @@ __device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &ke
         const float *R = in.inv;
         const V3 g_n = v3(R[0] * g_m.x + R[4] * g_m.y + R[8] * g_m.z, R[1] * g_m.x + R[5] * g_m.y + R[9] * g_m.z,
                           R[2] * g_m.x + R[6] * g_m.y + R[10] * g_m.z);
+        gt.i0 = i0;
+        gt.i1 = i1;
+        gt.i2 = i2;
         if (interp) {
-          const float bw = 1.0f - bu - bv;
-          atomic_add3(P.g_nrm, i0, g_n * bw);
-          atomic_add3(P.g_nrm, i1, g_n * bu);
-          atomic_add3(P.g_nrm, i2, g_n * bv);
+          gt.kind = 1u;
+          gt.A = g_n;
+          gt.bu = bu;
+          gt.bv = bv;
+          gt.B = v3(0.0f, 0.0f, 0.0f);
           const V3 n0 = load3(P.normals + 3 * (size_t)i0);
           const float g_u = pt::dot3(g_n, load3(P.normals + 3 * (size_t)i1) - n0);
           const float g_v = pt::dot3(g_n, load3(P.normals + 3 * (size_t)i2) - n0);
@@ -272,18 +304,14 @@ __device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &ke
           const V3 c0 = v3(-(R[0] * d.x + R[4] * d.y + R[8] * d.z), -(R[1] * d.x + R[5] * d.y + R[9] * d.z),
                            -(R[2] * d.x + R[6] * d.y + R[10] * d.z));
           const float det = pt::dot3(c0, pt::cross3(e1, e2));
-          if (det != 0.0f) {
-            const V3 G = (pt::cross3(e2, c0) * g_u + pt::cross3(c0, e1) * g_v) * (1.0f / det);
-            atomic_add3(P.g_pos, i0, G * -bw);
-            atomic_add3(P.g_pos, i1, G * -bu);
-            atomic_add3(P.g_pos, i2, G * -bv);
-          }
+          if (det != 0.0f) gt.B = (pt::cross3(e2, c0) * g_u + pt::cross3(c0, e1) * g_v) * (1.0f / det);
         } else {
           // n = e1 x e2
-          const V3 g_e1 = pt::cross3(e2, g_n), g_e2 = pt::cross3(g_n, e1);
-          atomic_add3(P.g_pos, i1, g_e1);
-          atomic_add3(P.g_pos, i2, g_e2);
-          atomic_add3(P.g_pos, i0, (g_e1 + g_e2) * -1.0f);
+          gt.kind = 2u;
+          gt.A = pt::cross3(e2, g_n);
+          gt.B = pt::cross3(g_n, e1);
+          gt.bu = 0.0f;
+          gt.bv = 0.0f;
         }
       }
     }
@@ -299,6 +327,9 @@ __device__ void shade_one(const PtLaunch &P, uint32_t b, uint32_t path, bool &ke
 
 // Shades queue slots [bid * B, bid * B + B) and compacts the survivors and
 // the shadow rays block-locally (the scan and scatter launches follow).
+// REPLAY: every shaded slot i < count of bounce b < max_bounces writes its
+// gradient term to grec[b * npaths + i] (3 x 16 bytes) and notes the
+// largest |corner value| for the reproducible sum that follows.
 template <int B, bool REPLAY>
 __global__ __launch_bounds__(B) void k_pt_shade(PtLaunch P, uint32_t b) {
   const uint32_t bid = blockIdx.x;
@@ -309,11 +340,67 @@ __global__ __launch_bounds__(B) void k_pt_shade(PtLaunch P, uint32_t b) {
   const uint32_t i = bid * B + threadIdx.x;
   bool keep = false, shadow = false;
   uint32_t path = 0;
+  GeoTerm gt;
+  gt.kind = 0u;
   if (i < count_in) {
     path = qin[i];
-    if (path < P.npaths) shade_one<REPLAY>(P, b, path, keep, shadow);
+    if (path < P.npaths) shade_one<REPLAY>(P, b, path, keep, shadow, gt);
+  }
+  if (REPLAY && b < P.max_bounces) {
+    uint32_t bits = 0u;
+    if (i < count_in) {
+      uint4 *r = P.grec + 3 * ((size_t)b * P.npaths + i);
+      r[0] = make_uint4(gt.i0, gt.i1, gt.i2, gt.kind);
+      r[1] = make_uint4(__float_as_uint(gt.A.x), __float_as_uint(gt.A.y), __float_as_uint(gt.A.z),
+                        __float_as_uint(gt.bu));
+      r[2] = make_uint4(__float_as_uint(gt.B.x), __float_as_uint(gt.B.y), __float_as_uint(gt.B.z),
+                        __float_as_uint(gt.bv));
+      if (gt.kind) {
+        V3 pc[3], nc[3];
+        geo_corners(gt, pc, nc);
+        for (int k = 0; k < 3; k++) bits = max(bits, max(max_bits3(pc[k]), max_bits3(nc[k])));
+      }
+    }
+    det::det_note_max(P.det_h, bits);
   }
   block_compact2<B>(keep, shadow, path, bid, P.tmp_a, P.tmp_b, P.blk, P.blk + P.blk_stride);
+}
+
+// Adds the recorded gradient terms of every replayed bounce into the
+// position sums g_acc[0, 3 nv) and the normal sums g_acc[3 nv, 6 nv).
+__global__ __launch_bounds__(kPtBlock) void k_geo_scatter(PtLaunch P, det::u64 *acc, uint32_t *flags) {
+  const size_t j = (size_t)blockIdx.x * kPtBlock + threadIdx.x;
+  const uint32_t b = (uint32_t)(j / P.npaths), i = (uint32_t)(j - (size_t)b * P.npaths);
+  if (b >= P.max_bounces || i >= min(P.counts[b], P.npaths)) return;
+  const uint4 *r = P.grec + 3 * j;
+  const uint4 r0 = r[0];
+  if (r0.w == 0u) return;
+  const uint4 r1 = r[1], r2 = r[2];
+  GeoTerm gt;
+  gt.i0 = r0.x;
+  gt.i1 = r0.y;
+  gt.i2 = r0.z;
+  gt.kind = r0.w;
+  gt.A = v3(__uint_as_float(r1.x), __uint_as_float(r1.y), __uint_as_float(r1.z));
+  gt.bu = __uint_as_float(r1.w);
+  gt.B = v3(__uint_as_float(r2.x), __uint_as_float(r2.y), __uint_as_float(r2.z));
+  gt.bv = __uint_as_float(r2.w);
+  V3 pc[3], nc[3];
+  geo_corners(gt, pc, nc);
+  const int32_t e = P.det_h->e;
+  const size_t nv3 = 3 * (size_t)P.nvertices;
+  const uint32_t vi[3] = {gt.i0, gt.i1, gt.i2};
+  for (int k = 0; k < 3; k++) {
+    const size_t o = 3 * (size_t)vi[k];
+    det::det_add(acc, flags, o, pc[k].x, e);
+    det::det_add(acc, flags, o + 1, pc[k].y, e);
+    det::det_add(acc, flags, o + 2, pc[k].z, e);
+    if (gt.kind == 1u) {
+      det::det_add(acc, flags, nv3 + o, nc[k].x, e);
+      det::det_add(acc, flags, nv3 + o + 1, nc[k].y, e);
+      det::det_add(acc, flags, nv3 + o + 2, nc[k].z, e);
+    }
+  }
 }
 
 // add the light of unoccluded shadow rays
@@ -389,21 +476,34 @@ __global__ __launch_bounds__(kPtBlock) void k_compact(const uint32_t *mask, uint
 // beta_0 = 1, beta_{k+1} = beta_k a_k and T_k = e_k + a_k (w_k sun + T_{k+1}):
 //   dC/de_k = beta_k,   dC/da_k = beta_k (w_k sun + T_{k+1}),
 // scaled by the pixel's adjoint / spp.  Fully unrolled over S slots so the
-// per-vertex arrays stay in registers.  Gradients gather in LDS (when
-// `lds`) and are flushed with one global atomic add per touched entry.
-template <int S>
-__global__ __launch_bounds__(kPtBlock) void k_pt_backward(PtLaunch P, const float *adj, float *ga, float *ge,
-                                                          bool lds) {
-  extern __shared__ float g_lds[];
+// per-vertex arrays stay in registers.  The terms go into the reproducible
+// sums of det_sum.h, entry 3 mi + c of acc for albedo and 3 nm + 3 mi + c
+// for emission: MAXPASS only notes the largest |term|; the second pass adds
+// the quantized terms, block-locally in LDS (when `lds`; 64-bit integer
+// adds, so order does not matter) and then once per touched entry in
+// global memory.
+template <int S, bool MAXPASS>
+__global__ __launch_bounds__(kPtBlock) void k_pt_backward(PtLaunch P, const float *adj, det::u64 *acc,
+                                                          uint32_t *flags, det::Header *h, bool lds) {
+  extern __shared__ det::u64 g_lds[];
   const uint32_t nm = P.nmaterials, slots = P.max_bounces + 2;
-  if (lds) {
-    for (uint32_t i = threadIdx.x; i < 6 * nm; i += kPtBlock) g_lds[i] = 0.0f;
+  if (!MAXPASS && lds) {
+    for (uint32_t i = threadIdx.x; i < 6 * nm; i += kPtBlock) g_lds[i] = 0ull;
     __syncthreads();
   }
-  float *g_a = lds ? g_lds : ga;
-  float *g_e = lds ? g_lds + 3 * nm : ge;
+  const int32_t e = MAXPASS ? 0 : h->e;
+  uint32_t bits = 0u;
   const V3 sun = load3(P.sun_radiance);
   const float inv_spp = 1.0f / (float)P.spp;
+  auto add = [&](uint32_t entry, float x) {
+    if (MAXPASS) {
+      bits = max(bits, det::abs_bits(x));
+    } else if (x != 0.0f) {
+      if (!det::det_finite(x)) atomicOr(flags + entry, det::special_of(x));
+      else if (lds) atomicAdd(g_lds + entry, (det::u64)det::det_quant(x, e));
+      else atomicAdd(acc + entry, (det::u64)det::det_quant(x, e));
+    }
+  };
   for (uint32_t path = blockIdx.x * kPtBlock + threadIdx.x; path < P.npaths; path += gridDim.x * kPtBlock) {
     uint32_t code[S];
     float w[S];
@@ -443,33 +543,26 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_backward(PtLaunch P, const floa
         const uint32_t mi = code[k];
         const Material &mt = P.materials[mi];
         const V3 a = v3(mt.albedo[0], mt.albedo[1], mt.albedo[2]);
-        const V3 e = v3(mt.emission[0], mt.emission[1], mt.emission[2]);
+        const V3 em = v3(mt.emission[0], mt.emission[1], mt.emission[2]);
         const V3 lit = sun * w[k];
         const V3 de = a_pix * beta[k];
         const V3 da = de * (lit + T);
-        if (lds) {
-          atomicAdd(g_a + 3 * mi, da.x);
-          atomicAdd(g_a + 3 * mi + 1, da.y);
-          atomicAdd(g_a + 3 * mi + 2, da.z);
-          atomicAdd(g_e + 3 * mi, de.x);
-          atomicAdd(g_e + 3 * mi + 1, de.y);
-          atomicAdd(g_e + 3 * mi + 2, de.z);
-        } else {
-          unsafeAtomicAdd(g_a + 3 * mi, da.x);
-          unsafeAtomicAdd(g_a + 3 * mi + 1, da.y);
-          unsafeAtomicAdd(g_a + 3 * mi + 2, da.z);
-          unsafeAtomicAdd(g_e + 3 * mi, de.x);
-          unsafeAtomicAdd(g_e + 3 * mi + 1, de.y);
-          unsafeAtomicAdd(g_e + 3 * mi + 2, de.z);
-        }
-        T = e + a * (lit + T);
+        add(3 * mi, da.x);
+        add(3 * mi + 1, da.y);
+        add(3 * mi + 2, da.z);
+        add(3 * nm + 3 * mi, de.x);
+        add(3 * nm + 3 * mi + 1, de.y);
+        add(3 * nm + 3 * mi + 2, de.z);
+        T = em + a * (lit + T);
       }
   }
-  if (lds) {
+  if (MAXPASS) {
+    det::det_note_max(h, bits);
+  } else if (lds) {
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < 6 * nm; i += kPtBlock) {
-      const float v = g_lds[i];
-      if (v != 0.0f) unsafeAtomicAdd(i < 3 * nm ? ga + i : ge + (i - 3 * nm), v);
+      const det::u64 v = g_lds[i];
+      if (v) atomicAdd(acc + i, v);
     }
   }
 }
@@ -491,8 +584,11 @@ __global__ void k_pack_materials(const float *albedo, const float *emission, uin
   out[i] = m;
 }
 
+// Per-block partial sums of squares (fixed lane/wave order) into part[];
+// k_l2_sum adds the partials in block order.  Same grid for the same n, so
+// the loss is the same bits on every call.
 __global__ __launch_bounds__(kPtBlock) void k_l2_adjoint(const float *a, const float *b, uint32_t n, float *adj,
-                                                         double *loss) {
+                                                         double *part) {
   __shared__ double s[kPtBlock / 64];
   double acc = 0.0;
   for (uint32_t i = blockIdx.x * kPtBlock + threadIdx.x; i < n; i += gridDim.x * kPtBlock) {
@@ -506,11 +602,22 @@ __global__ __launch_bounds__(kPtBlock) void k_l2_adjoint(const float *a, const f
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int k = 0; k < kPtBlock / 64; k++) t += s[k];
-    unsafeAtomicAdd(loss, t);
+    part[blockIdx.x] = t;
   }
 }
 
-__global__ void k_zero_f64(double *p) { *p = 0.0; }
+__global__ __launch_bounds__(kPtBlock) void k_l2_sum(const double *part, uint32_t nparts, double *loss) {
+  __shared__ double s[kPtBlock];
+  double t = 0.0;
+  for (uint32_t i = threadIdx.x; i < nparts; i += kPtBlock) t += part[i];
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (uint32_t w = kPtBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *loss = s[0];
+}
 
 __global__ void k_sobol(const uint32_t *index, const uint32_t *scr, uint32_t n, float *out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -618,11 +725,11 @@ hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_
 }
 
 hipError_t enqueue_pt_replay(const PtLaunch &P, const TraceParams &scene, uint32_t push_depth,
-                             const TraceTune &tune, hipStream_t st) {
+                             const TraceTune &tune, const DetBufs &db, hipStream_t st) {
   const uint32_t grid = (P.npaths + kPtBlock - 1) / kPtBlock;
-  const uint32_t nv = P.nvertices;
-  hipLaunchKernelGGL(k_fill_f32, dim3(256), dim3(256), 0, st, P.g_pos, 3 * nv);
-  if (P.g_nrm) hipLaunchKernelGGL(k_fill_f32, dim3(256), dim3(256), 0, st, P.g_nrm, 3 * nv);
+  const size_t nv3 = 3 * (size_t)P.nvertices;
+  hipError_t e = det::enqueue_clear(db.acc, db.flags, 2 * nv3, db.h, st);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_pt_begin, dim3(256), dim3(256), 0, st, P);
   hipLaunchKernelGGL(k_pt_camera, dim3(grid), dim3(kPtBlock), 0, st, P);
   for (uint32_t b = 0; b <= P.max_bounces; b++) {
@@ -640,30 +747,49 @@ hipError_t enqueue_pt_replay(const PtLaunch &P, const TraceParams &scene, uint32
     tp.cull_mask = 0xffffffffu;
     tp.tlas_idx = 0;
     tp.work = P.work;
-    hipError_t e = launch_trace(tp, RDRT_TRAVERSAL_FAST, push_depth, tune, st);
+    e = launch_trace(tp, RDRT_TRAVERSAL_FAST, push_depth, tune, st);
     if (e != hipSuccess) return e;
+    if (b == P.max_bounces) break;  // the last vertex adds no geometry term
     hipLaunchKernelGGL((k_pt_shade<kPtBlock, true>), dim3(grid), dim3(kPtBlock), 0, st, P, b);
     enqueue_queue_scatter(P, b, grid, st);
-    if (b == P.max_bounces) break;
   }
+  // each record adds to at most 3 entries of one vertex component
+  e = det::enqueue_scale(db.h, 3ull * P.max_bounces * P.npaths, st);
+  if (e != hipSuccess) return e;
+  const size_t nrec = (size_t)P.max_bounces * P.npaths;
+  if (nrec) hipLaunchKernelGGL(k_geo_scatter, dim3((uint32_t)((nrec + kPtBlock - 1) / kPtBlock)), dim3(kPtBlock), 0,
+                               st, P, db.acc, db.flags);
+  e = det::enqueue_finish(db.acc, db.flags, nv3, db.h, P.g_pos, st);
+  if (e == hipSuccess && P.g_nrm) e = det::enqueue_finish(db.acc + nv3, db.flags + nv3, nv3, db.h, P.g_nrm, st);
+  if (e != hipSuccess) return e;
   return hipGetLastError();
 }
 
-hipError_t enqueue_pt_backward(const PtLaunch &P, const float *adj, float *ga, float *ge, hipStream_t st) {
+hipError_t enqueue_pt_backward(const PtLaunch &P, const float *adj, float *ga, float *ge, const DetBufs &db,
+                               hipStream_t st) {
   const uint32_t nm = P.nmaterials;
-  hipLaunchKernelGGL(k_fill_f32, dim3(64), dim3(256), 0, st, ga, 3 * nm);
-  hipLaunchKernelGGL(k_fill_f32, dim3(64), dim3(256), 0, st, ge, 3 * nm);
-  const size_t lds_bytes = (size_t)6 * nm * sizeof(float);
+  hipError_t e = det::enqueue_clear(db.acc, db.flags, 6 * (size_t)nm, db.h, st);
+  if (e != hipSuccess) return e;
+  const size_t lds_bytes = (size_t)6 * nm * sizeof(det::u64);
   const bool lds = lds_bytes <= 48 * 1024;
   const uint32_t grid = std::min<uint32_t>((P.npaths + kPtBlock - 1) / kPtBlock, 2048u);
   const uint32_t slots = P.max_bounces + 2;
   const size_t sh = lds ? lds_bytes : 0;
+  auto pass = [&](auto kmax, auto kacc) {
+    hipLaunchKernelGGL(kmax, dim3(grid), dim3(kPtBlock), 0, st, P, adj, db.acc, db.flags, db.h, false);
+    // a path adds at most `slots` terms to one entry
+    (void)det::enqueue_scale(db.h, (uint64_t)P.npaths * slots, st);
+    hipLaunchKernelGGL(kacc, dim3(grid), dim3(kPtBlock), sh, st, P, adj, db.acc, db.flags, db.h, lds);
+  };
   if (slots <= 4)
-    hipLaunchKernelGGL(k_pt_backward<4>, dim3(grid), dim3(kPtBlock), sh, st, P, adj, ga, ge, lds);
+    pass(k_pt_backward<4, true>, k_pt_backward<4, false>);
   else if (slots <= 8)
-    hipLaunchKernelGGL(k_pt_backward<8>, dim3(grid), dim3(kPtBlock), sh, st, P, adj, ga, ge, lds);
+    pass(k_pt_backward<8, true>, k_pt_backward<8, false>);
   else
-    hipLaunchKernelGGL(k_pt_backward<kPtMaxBounces + 2>, dim3(grid), dim3(kPtBlock), sh, st, P, adj, ga, ge, lds);
+    pass(k_pt_backward<kPtMaxBounces + 2, true>, k_pt_backward<kPtMaxBounces + 2, false>);
+  e = det::enqueue_finish(db.acc, db.flags, 3 * (size_t)nm, db.h, ga, st);
+  if (e == hipSuccess) e = det::enqueue_finish(db.acc + 3 * nm, db.flags + 3 * nm, 3 * (size_t)nm, db.h, ge, st);
+  if (e != hipSuccess) return e;
   return hipGetLastError();
 }
 
@@ -674,12 +800,13 @@ hipError_t enqueue_pack_materials(const float *albedo, const float *emission, ui
   return hipGetLastError();
 }
 
-hipError_t enqueue_l2_adjoint(const float *a, const float *b, uint32_t n, float *adj, double *loss,
+uint32_t l2_adjoint_parts(uint32_t n) { return std::max<uint32_t>(1u, std::min<uint32_t>((n + kPtBlock - 1) / kPtBlock, 1024u)); }
+
+hipError_t enqueue_l2_adjoint(const float *a, const float *b, uint32_t n, float *adj, double *loss, double *part,
                               hipStream_t st) {
-  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, st, loss);
-  if (n == 0) return hipGetLastError();
-  const uint32_t grid = std::min<uint32_t>((n + kPtBlock - 1) / kPtBlock, 1024u);
-  hipLaunchKernelGGL(k_l2_adjoint, dim3(grid), dim3(kPtBlock), 0, st, a, b, n, adj, loss);
+  const uint32_t grid = l2_adjoint_parts(n);
+  hipLaunchKernelGGL(k_l2_adjoint, dim3(grid), dim3(kPtBlock), 0, st, a, b, n, adj, part);
+  hipLaunchKernelGGL(k_l2_sum, dim3(1), dim3(kPtBlock), 0, st, (const double *)part, grid, loss);
   return hipGetLastError();
 }
 

@@ -219,4 +219,62 @@ def test_geometry_replay_leaves_material_gradient_intact(geo_case):
     ga1, ge1 = torch.empty((2, 3), device="cuda"), torch.empty((2, 3), device="cuda")
     tracer.backward(a_t, ga1, ge1)
     torch.cuda.synchronize()
-    assert torch.allclose(ga0, ga1, rtol=1e-5) and torch.allclose(ge0, ge1, rtol=1e-5)
+    # reproducible sums: the same bits, not merely close
+    assert torch.equal(ga0, ga1) and torch.equal(ge0, ge1)
+
+
+def test_adjoints_are_bit_reproducible(geo_case):
+    """Two backward calls on one recorded frame give identical bits for the
+    material and the geometry gradients, and two L2 calls the same loss."""
+    import torch
+
+    _, system, tracer, cams, cfg, adj = geo_case
+    _say("bit-reproducible material / geometry gradients and L2 loss")
+    nv = system.scene_info().verts
+    img = tracer.render(cams, cfg).clone()
+    a_t = torch.from_numpy(adj.astype(np.float32)).cuda()
+    outs = []
+    for _ in range(2):
+        ga, ge = torch.empty((2, 3), device="cuda"), torch.empty((2, 3), device="cuda")
+        g_pos = torch.full((nv, 3), 7.0, dtype=torch.float32, device="cuda")
+        g_nrm = torch.full((nv, 3), 7.0, dtype=torch.float32, device="cuda")
+        tracer.backward_geometry(a_t, g_pos, g_nrm)
+        tracer.backward(a_t, ga, ge)
+        loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+        adj2 = torch.empty_like(img)
+        tracer.l2_adjoint(img, img * 0.5 + 0.01, adj2, loss)
+        torch.cuda.synchronize()
+        outs.append((ga, ge, g_pos, g_nrm, loss))
+    for x, y in zip(outs[0], outs[1]):
+        assert torch.equal(x, y)
+    assert float(outs[0][2].abs().sum()) > 0 and float(outs[0][3].abs().sum()) > 0
+    assert float(outs[0][4].item()) > 0
+    ref = float(((img.double() - (img * 0.5 + 0.01).double()) ** 2).sum())
+    assert abs(outs[0][4].item() - ref) <= 1e-9 * ref
+
+
+def test_non_finite_adjoint_marks_touched_entries(geo_case):
+    """A NaN adjoint makes exactly the entries that any path term touches
+    NaN (as float addition would); entries no term touches stay 0."""
+    import torch
+
+    _, system, tracer, cams, cfg, adj = geo_case
+    _say("NaN adjoint: touched gradient entries NaN, others 0")
+    nv = system.scene_info().verts
+    tracer.render(cams, cfg)
+    ones = torch.ones((2, cfg.height, cfg.width, 3), dtype=torch.float32, device="cuda")
+    ga, ge = torch.empty((2, 3), device="cuda"), torch.empty((2, 3), device="cuda")
+    tracer.backward(ones, ga, ge)
+    gp = torch.empty((nv, 3), dtype=torch.float32, device="cuda")
+    gn = torch.empty((nv, 3), dtype=torch.float32, device="cuda")
+    tracer.backward_geometry(ones, gp, gn)
+    nan = torch.full_like(ones, float("nan"))
+    ga2, ge2 = torch.empty_like(ga), torch.empty_like(ge)
+    tracer.backward(nan, ga2, ge2)
+    gp2, gn2 = torch.empty_like(gp), torch.empty_like(gn)
+    tracer.backward_geometry(nan, gp2, gn2)
+    torch.cuda.synchronize()
+    for clean, bad in ((ga, ga2), (ge, ge2), (gp, gp2), (gn, gn2)):
+        touched = clean != 0
+        assert bool(torch.isnan(bad[touched]).all())
+        assert bool((bad[~touched & ~torch.isnan(bad)] == 0).all())
