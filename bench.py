@@ -1,8 +1,16 @@
 """Benchmark of the ray-query hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pt|closest_hit|batched|diff|diffgeo]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|pt|closest_hit|batched|diff|diffgeo|ao]
 
-pt (default): wavefront path tracer on the parametric scene -- 64 UV spheres
+all (default): one JSON line for the whole metric -- the path-traced frame
+  (pt) is the headline (value, roofline, cpu_baseline), and closest_hit,
+  batched, diff, diffgeo and ao follow under "sub", each with its own
+  timing, roofline, CPU baseline and error figures.  Every workload that
+  traces carries "fractions": the HBM fraction of its traversal over the
+  reference tree's visits, over the kernel's own visits and over the
+  physical DRAM bytes (rocprofv3 FETCH_SIZE + WRITE_SIZE), and the L2 (TCC)
+  hit rate of the same dispatches.
+pt: wavefront path tracer on the parametric scene -- 64 UV spheres
   and tori tessellated at 256x256 (8,388,608 triangles, one BLAS each, 64
   random affine instances in [-10, 10]^3), pinhole camera at 1920x1080, 1 spp,
   Lambertian materials (albedo U[0.1, 0.9], 4 emissive meshes), up to 4
@@ -71,20 +79,49 @@ def _bytes(n_rays: int, visits) -> int:
     return 48 * n_rays + 48 * int(visits[0]) + 52 * int(visits[1]) + 176 * int(visits[2])
 
 
-def _kernel_bytes(n_rays: int, visits) -> int:
-    """Bytes the fast kernel touches: ray and hit, 64-byte child-pair nodes,
-    64-byte triangle records, 176-byte instance records."""
-    return 48 * n_rays + 64 * int(visits[0]) + 64 * int(visits[1]) + 176 * int(visits[2])
+NODE_BYTES = {0: 64, 1: 128, 2: 64}  # child-pair, child-quad, quantized child-quad node by the tune's `wide`
 
 
-def _kernel_order(n_rays: int, visits, ms: float) -> dict:
+def _kernel_bytes(n_rays: int, visits, wide: int = 0) -> int:
+    """Bytes the fast kernel touches: ray and hit, one node of its layout per
+    node visit (64 B pair, 128 B quad, 64 B quantized quad), a 64-byte
+    triangle record per test, a 128-byte instance record per instance test."""
+    return 48 * n_rays + NODE_BYTES.get(int(wide), 64) * int(visits[0]) + 64 * int(visits[1]) + 128 * int(visits[2])
+
+
+def _kernel_order(n_rays: int, visits, ms: float, wide: int = 0) -> dict:
     """The same rate over the visits of the fast kernel's own tree and order
-    (node-pair visits, triangle tests, instance tests): the bytes the kernel
+    (node visits, triangle tests, instance tests): the bytes the kernel
     really asks of the memory system per launch."""
-    b = _kernel_bytes(n_rays, visits)
+    b = _kernel_bytes(n_rays, visits, wide)
     gbps = b / (ms * 1e-3) / 1e9
     return {"visits_per_ray": [round(float(x) / n_rays, 3) for x in visits], "bytes": b,
             "achieved": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+
+
+def _pt_traversal(pt, cams, cfg, wide: int) -> dict:
+    """Traversal rates of one eager frame of this configuration: the time of
+    its traversal launches (HIP events on the frame's stream), the rays they
+    traced, and the rate over the reference tree's visits (reference order)
+    and over the fast kernel's own visits (kernel order)."""
+    import dataclasses
+
+    from rendiation_amd import _lib
+
+    pt.render(cams, dataclasses.replace(cfg, record=False, diagnostics=_lib.RDRT_PT_DIAG_TIME_TRACES))
+    d_time = pt.stats()
+    pt.render(cams, dataclasses.replace(cfg, record=False, diagnostics=_lib.RDRT_PT_DIAG_COUNT_VISITS))
+    d_ref = pt.stats()
+    pt.render(cams, dataclasses.replace(cfg, record=False, diagnostics=_lib.RDRT_PT_DIAG_COUNT_FAST))
+    d_fast = pt.stats()
+    rays = int(sum(d_time["extension_rays"]) + sum(d_time["shadow_rays"]))
+    ms = d_time["trace_ms"]
+    b_ref = _bytes(rays, d_ref["visits"])
+    ref = {"bytes": b_ref, "achieved": round(b_ref / (ms * 1e-3) / 1e9, 1),
+           "frac": round(b_ref / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+           "visits_per_ray": [round(v / rays, 3) for v in d_ref["visits"]]}
+    return {"trace_ms": round(ms, 4), "trace_launches": d_time["trace_launches"], "rays": rays,
+            "reference_order": ref, "kernel_order": _kernel_order(rays, d_fast["visits"], ms, wide)}
 
 
 def _call_kernels(tune) -> str:
@@ -182,16 +219,10 @@ def path_tracing(args):
     value = rays / (ms_per_step * 1e-3) / 1e6
     image = out.cpu().numpy().astype(np.float64)
 
-    # diagnostics frames: traversal launch times, then reference-order visits
-    import dataclasses
-
-    pt.render(cam, dataclasses.replace(cfg, diagnostics=_lib.RDRT_PT_DIAG_TIME_TRACES))
-    d_time = pt.stats()
-    pt.render(cam, dataclasses.replace(cfg, diagnostics=_lib.RDRT_PT_DIAG_COUNT_VISITS))
-    d_vis = pt.stats()
-    system.sync()
-    bytes_frame = _bytes(rays, d_vis["visits"])
-    achieved = bytes_frame / (d_time["trace_ms"] * 1e-3) / 1e9
+    # diagnostics frames: traversal launch times, reference-order and kernel-order visits
+    trav = _pt_traversal(pt, cam, cfg, tune.get("wide", 0))
+    bytes_frame = trav["reference_order"]["bytes"]
+    achieved = bytes_frame / (trav["trace_ms"] * 1e-3) / 1e9
 
     config = {
         "workload": "pt: 64 spheres/tori at 256x256 (8.39M tris), 1920x1080, 1 spp, <=4 bounces, RR from 2, seed 0",
@@ -205,9 +236,9 @@ def path_tracing(args):
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
         "kernel": f"{_call_kernels(dict(tune, cull=0))} (all traversal launches of one frame)",
-        "trace_ms_per_frame": round(d_time["trace_ms"], 4), "trace_launches": d_time["trace_launches"],
-        "bytes_per_frame": bytes_frame,
-        "visits_per_ray": [round(v / rays, 3) for v in d_vis["visits"]],
+        "trace_ms_per_frame": trav["trace_ms"], "trace_launches": trav["trace_launches"],
+        "bytes_per_frame": bytes_frame, "visits_per_ray": trav["reference_order"]["visits_per_ray"],
+        "kernel_order": trav["kernel_order"],
     }
     extra = {"ms_per_frame": round(ms_per_step, 4)}
     if not args.no_cpu_baseline:
@@ -232,11 +263,19 @@ def cpu_baseline_pt(scene, cam, cfg, image):
     ref, ext, sh = o.pt_render_rows(prm, r0, r1, threads=threads)
     dt = time.perf_counter() - t0
     rays = int(ext.sum() + sh.sum())
+    # one thread (the reference loop is sequential) on the middle 4 rows
+    s0 = cfg.height // 2 - 2
+    t1 = time.perf_counter()
+    _, ext1, sh1 = o.pt_render_rows(prm, s0, s0 + 4, threads=1)
+    dt1 = time.perf_counter() - t1
+    rays1 = int(ext1.sum() + sh1.sum())
     gpu = image[r0:r1]
     rmse = float(np.sqrt(np.mean((gpu - ref) ** 2)))
     pix = np.abs(gpu - ref).max(axis=2) / np.maximum(ref.max(axis=2), 1e-6)
     base = {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"rows {r0}-{r1 - 1} of the frame ({(r1 - r0) * cfg.width} paths, {rays} rays), FP64 shading",
+            "single_thread_mrays": round(rays1 / dt1 / 1e6, 4),
+            "single_thread_sample": f"rows {s0}-{s0 + 3} ({4 * cfg.width} paths, {rays1} rays), 1 thread",
             "cpu_model": _cpu_model()}
     err = {"vs": "FP64 CPU oracle on the same rows", "rmse": rmse, "rel_rmse": rmse / max(float(ref.mean()), 1e-12),
            "pixels_within_1e-3": float((pix <= 1e-3).mean()), "bound_rel_rmse": 1e-2}
@@ -326,7 +365,7 @@ def closest_hit(args):
         "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
         "kernel": _call_kernels(tune), "kernel_ms": round(kernel_ms, 4), "bytes_per_launch": bytes_per_launch,
         "visits_per_ray": [round(float(x) / n, 3) for x in c],
-        "kernel_order": _kernel_order(n, c_fast, kernel_ms),
+        "kernel_order": _kernel_order(n, c_fast, kernel_ms, tune.get("wide", 0)),
     }
     extra = {}
     if not args.no_cpu_baseline:
@@ -409,6 +448,12 @@ def differentiable(args):
         tune = tracer.autotune(cam, cfg, out=target, reps=3)
     else:
         tune = system.tune()
+    if args.profile_only:  # forward frames alone, for hardware counters
+        for _ in range(args.steps):
+            tracer.render(cam, cfg, out=target)
+        system.sync()
+        system.close()
+        return None
     tracer.render(cam, cfg, out=target)
     system.sync()
     fwd_cfg = cfg
@@ -452,6 +497,7 @@ def differentiable(args):
     system.sync()
     bwd_ms = b0.elapsed_time(b1) / args.steps
     st = tracer.stats()
+    trav = _pt_traversal(tracer, cam, cfg, tune.get("wide", 0))
     npaths = cfg.width * cfg.height * cfg.spp
     # backward bytes: vertex records (8 B x slots), pixel adjoint (12 B / path), gradient flush (24 B / material / block)
     slots = cfg.max_bounces + 2
@@ -472,7 +518,7 @@ def differentiable(args):
         "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": "k_pt_backward",
         "bytes_per_launch": bwd_bytes,
     }
-    extra = {}
+    extra = {"forward_traversal": trav}
     if not args.no_cpu_baseline:
         extra["cpu_baseline"], extra["gradient_error"] = cpu_baseline_diff(scene, cam, cfg, adj, ga, ge)
     system.close()
@@ -559,6 +605,12 @@ def differentiable_geometry(args):
         tune = tracer.autotune(cams, cfg, out=img, reps=1)
     else:
         tune = system.tune()
+    if args.profile_only:  # forward frames alone, for hardware counters
+        for _ in range(args.steps):
+            tracer.render(cams, cfg, out=img)
+        system.sync()
+        system.close()
+        return None
     rec_cfg = dataclasses.replace(cfg, record=True)
 
     def forward():
@@ -586,6 +638,7 @@ def differentiable_geometry(args):
     ms_f, ev_f = timed(forward)
     ms_fb, ev_fb = timed(forward_backward)
     st = tracer.stats()
+    trav = _pt_traversal(tracer, cams, cfg, tune.get("wide", 0))
     npaths = len(cams) * cfg.width * cfg.height * cfg.spp
     config = {
         "workload": "diffgeo: 1000x1000-vertex height field (8 sinusoids, amplitude 0.05), 16 views at 512x512 in "
@@ -597,7 +650,7 @@ def differentiable_geometry(args):
         "extension_rays": st["extension_rays"], "scene_gen_ms": round(gen_ms, 1),
         "build_upload_ms": round(build_upload_ms, 1), "launch_shape": tune,
     }
-    extra = {}
+    extra = {"forward_traversal": trav}
     if not args.no_cpu_baseline:
         extra["cpu_baseline"], extra["gradient_error"] = cpu_baseline_diffgeo(scene, cams[:2], cfg, target, system,
                                                                              tracer)
@@ -789,7 +842,7 @@ def batched(args):
         "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
         "kernel": "k_trace_persist + k_spawn (whole batched call, graph replay)",
         "bytes_per_call": bytes_call, "visits_per_ray": [round(float(v) / total, 3) for v in visits],
-        "kernel_order": _kernel_order(total, visits_fast, ev_p),
+        "kernel_order": _kernel_order(total, visits_fast, ev_p, tune.get("wide", 0)),
         "note": "achieved counts the reference tree's visits (4-bin longest-axis SAH) on every round's rays; the "
                 "kernel walks its own full-SAH tree, which visits far fewer nodes on these large scenes, and "
                 "finishes the rays that miss their scene box in the cull pass -- so the reference-order rate can "
@@ -940,13 +993,11 @@ def cpu_baseline_ao(scene, cam, W, H, first, rays):
              "sample": f"whole {W}x{H} frame, sample 0"}, {"differing_pixels": round(diff, 6)})
 
 
-def pmc_traffic(workload: str, kernels, extra=()):
-    """Physical DRAM-side bytes of `kernel` per step: two rocprofv3 --pmc
-    child runs (FETCH_SIZE, then WRITE_SIZE) of this benchmark in
-    --profile-only mode, summed over the dispatches whose names contain one
-    of `kernels`, each counter scaled by what it reads for a 256 MiB
-    device copy in the same run (--copy-calibration).  Returns a dict, or
-    None when rocprofv3 is unavailable or a pass fails."""
+def _pmc_pass(workload: str, counters, extra=()):
+    """One rocprofv3 --pmc child run of this benchmark in --profile-only mode
+    (after a 256 MiB device copy, --copy-calibration); returns the rows of
+    its counter CSV, or None when rocprofv3 is unavailable or the pass
+    fails or runs past its time limit."""
     import csv
     import shutil
     import signal
@@ -955,29 +1006,41 @@ def pmc_traffic(workload: str, kernels, extra=()):
     exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
     if not os.path.exists(exe):
         return None
+    d = tempfile.mkdtemp(prefix="rdrt_pmc_", dir="/tmp")
+    cmd = [exe, "--pmc", *counters, "-f", "csv", "-d", d, "-o", "p", "--", sys.executable, str(ROOT / "bench.py"),
+           "--workload", workload, "--steps", str(PMC_STEPS), "--warmup", "0", "--no-cpu-baseline", "--no-traffic",
+           "--copy-calibration", "--profile-only", *extra]
+    env = dict(os.environ, TMPDIR="/tmp")
+    try:
+        proc = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                                start_new_session=True)
+        try:
+            proc.wait(timeout=150)
+        except subprocess.TimeoutExpired:
+            os.killpg(proc.pid, signal.SIGKILL)
+            proc.wait()
+            return None
+        rows = []
+        for root, _, files in os.walk(d):
+            for f in files:
+                if f.endswith("counter_collection.csv"):
+                    rows += list(csv.DictReader(open(os.path.join(root, f))))
+        return rows
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def pmc_counters(workload: str, kernels, extra=()):
+    """Physical DRAM-side bytes and the L2 (TCC) hit rate of the dispatches
+    whose names contain one of `kernels`, per step: three child passes
+    (FETCH_SIZE; WRITE_SIZE; TCC_HIT_sum + TCC_MISS_sum).  FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) are each scaled by what the counter reads
+    for the 256 MiB copy of the same run.  Returns a dict, or None."""
     out = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-        d = tempfile.mkdtemp(prefix="rdrt_pmc_", dir="/tmp")
-        cmd = [exe, "--pmc", counter, "-f", "csv", "-d", d, "-o", "p", "--", sys.executable, str(ROOT / "bench.py"),
-               "--workload", workload, "--steps", str(PMC_STEPS), "--warmup", "0", "--no-cpu-baseline", "--no-traffic",
-               "--copy-calibration", "--profile-only", *extra]
-        env = dict(os.environ, TMPDIR="/tmp")
-        try:
-            proc = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
-                                    start_new_session=True)
-            try:
-                proc.wait(timeout=150)
-            except subprocess.TimeoutExpired:
-                os.killpg(proc.pid, signal.SIGKILL)
-                proc.wait()
-                return None
-            rows = []
-            for root, _, files in os.walk(d):
-                for f in files:
-                    if f.endswith("counter_collection.csv"):
-                        rows += list(csv.DictReader(open(os.path.join(root, f))))
-        finally:
-            shutil.rmtree(d, ignore_errors=True)
+        rows = _pmc_pass(workload, [counter], extra)
+        if not rows:
+            return None
         copy = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]
                 and float(r["Counter_Value"]) > 2 ** 16]
         vals = [float(r["Counter_Value"]) for r in rows if any(k in r["Kernel_Name"] for k in kernels)]
@@ -986,10 +1049,86 @@ def pmc_traffic(workload: str, kernels, extra=()):
         factor = 262144.0 / copy[0]  # KiB of the 256 MiB copy / what the counter read
         out[counter] = {"kib_per_step": sum(vals) / PMC_STEPS, "copy_factor": factor, "dispatches": len(vals)}
     total = sum(v["kib_per_step"] * v["copy_factor"] * 1024.0 for v in out.values())
-    return {"bytes_per_step": total, "fetch_kib": out["FETCH_SIZE"]["kib_per_step"],
-            "fetch_copy_factor": out["FETCH_SIZE"]["copy_factor"], "write_kib": out["WRITE_SIZE"]["kib_per_step"],
-            "write_copy_factor": out["WRITE_SIZE"]["copy_factor"], "dispatches_per_step":
-                out["FETCH_SIZE"]["dispatches"] / PMC_STEPS}
+    res = {"bytes_per_step": total, "fetch_kib": out["FETCH_SIZE"]["kib_per_step"],
+           "fetch_copy_factor": out["FETCH_SIZE"]["copy_factor"], "write_kib": out["WRITE_SIZE"]["kib_per_step"],
+           "write_copy_factor": out["WRITE_SIZE"]["copy_factor"],
+           "dispatches_per_step": out["FETCH_SIZE"]["dispatches"] / PMC_STEPS}
+    rows = _pmc_pass(workload, ["TCC_HIT_sum", "TCC_MISS_sum"], extra)
+    if rows:
+        sel = [r for r in rows if any(k in r["Kernel_Name"] for k in kernels)]
+        hit = sum(float(r["Counter_Value"]) for r in sel if r.get("Counter_Name", "").startswith("TCC_HIT"))
+        miss = sum(float(r["Counter_Value"]) for r in sel if r.get("Counter_Name", "").startswith("TCC_MISS"))
+        if hit + miss > 0:
+            res["l2_hit_rate"] = round(hit / (hit + miss), 4)
+            res["l2_requests_per_step"] = (hit + miss) / PMC_STEPS
+    return res
+
+
+TRACE_KERNELS = ("k_trace_fast", "k_trace_persist", "k_trace_coop")
+CULL_KERNELS = ("k_trace_cull", "k_block_scan", "k_scatter2")
+
+
+def _counter_kernels(workload: str, shape) -> tuple:
+    if workload == "batched":  # the whole call
+        return TRACE_KERNELS + CULL_KERNELS + ("k_spawn", "k_batch_init", "k_zero_work")
+    if workload == "closest_hit" and shape and shape.get("cull"):
+        return TRACE_KERNELS + CULL_KERNELS
+    return TRACE_KERNELS
+
+
+def _fractions(workload: str, rec: dict) -> dict:
+    """HBM fractions of the traversal of one step: over the reference tree's
+    visits, over the kernel's own visits, and over the physical DRAM bytes
+    (rocprof), with the L2 hit rate of the same dispatches."""
+    roof, cfg = rec.get("roofline") or {}, rec.get("config", {})
+    if workload in ("diff", "diffgeo"):
+        t = rec.get("forward_traversal", {})
+        ms, ref, ko = t.get("trace_ms"), t.get("reference_order", {}).get("frac"), t.get("kernel_order", {})
+    elif workload == "pt":
+        ms, ref, ko = roof.get("trace_ms_per_frame"), roof.get("frac"), roof.get("kernel_order", {})
+    elif workload == "closest_hit":
+        ms, ref, ko = roof.get("kernel_ms"), roof.get("frac"), roof.get("kernel_order", {})
+    elif workload == "batched":
+        ms, ref, ko = cfg.get("ms_persistent_events"), roof.get("frac"), roof.get("kernel_order", {})
+    else:
+        return {}
+    tr = rec.get("traffic_detail")
+    phys = None
+    if tr and ms:
+        phys = round(tr["bytes_per_step"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
+    return {"time_ms": ms, "reference_order": ref, "kernel_order": ko.get("frac"), "physical": phys,
+            "l2_hit_rate": tr.get("l2_hit_rate") if tr else None}
+
+
+WORKLOADS = ("pt", "closest_hit", "batched", "diff", "diffgeo", "ao")
+COUNTED = ("pt", "closest_hit", "batched", "diff", "diffgeo")
+TRAFFIC_BUDGET_S = 420.0  # no further counter passes past this point of the run
+
+
+def run_workload(name: str, args, t_start: float):
+    rec = {"pt": path_tracing, "closest_hit": closest_hit, "batched": batched, "diff": differentiable,
+           "diffgeo": differentiable_geometry, "ao": ambient_occlusion}[name](args)
+    if rec is None:
+        return None
+    traffic = None
+    if not args.no_traffic and name in COUNTED:
+        if time.perf_counter() - t_start < TRAFFIC_BUDGET_S:
+            # counter passes in child processes, with this run's launch shape
+            shape = rec["config"].get("launch_shape")
+            extra = ("--tune", _tune_arg(shape)) if shape else ()
+            traffic = pmc_counters(name, _counter_kernels(name, shape), extra)
+        else:
+            rec["traffic_skipped"] = "time budget of the run"
+    if traffic is not None:
+        rec["traffic_detail"] = traffic
+        if rec.get("roofline") and name != "diff":
+            rec["roofline"]["traffic"] = round(traffic["bytes_per_step"])
+    if name in COUNTED:
+        rec["fractions"] = _fractions(name, rec)
+    return rec
+
+
+_SUB_DROP = ("metric", "n_gpus", "steps", "warmup", "scaling", "vs_baseline", "data", "dtype")
 
 
 def main():
@@ -997,7 +1136,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt", choices=["pt", "closest_hit", "batched", "diff", "diffgeo", "ao"])
+    ap.add_argument("--workload", default="all", choices=["all", *WORKLOADS])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 256 MiB device-to-device copy (a reference dispatch for HBM counters)")
@@ -1020,26 +1159,22 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    rec = {"pt": path_tracing, "closest_hit": closest_hit, "batched": batched,
-           "diff": differentiable, "diffgeo": differentiable_geometry, "ao": ambient_occlusion}[args.workload](args)
-    if rec is None:
+    t_start = time.perf_counter()
+    if args.workload != "all":
+        rec = run_workload(args.workload, args, t_start)
+        if rec is not None:
+            print(json.dumps(rec))
         return
-    traffic = None
-    if not args.no_traffic and args.workload in ("pt", "closest_hit", "batched"):
-        # counter passes in child processes, with this run's launch shape
-        shape = rec["config"].get("launch_shape")
-        extra, kernels = (), ("k_trace_fast", "k_trace_persist", "k_trace_coop")
-        if shape:
-            extra = ("--tune", _tune_arg(shape))
-            if shape.get("cull"):
-                kernels = kernels + ("k_trace_cull", "k_block_scan", "k_scatter2")
-        if args.workload == "batched":  # the whole call
-            kernels = kernels + ("k_trace_cull", "k_block_scan", "k_scatter2", "k_spawn", "k_batch_init",
-                                 "k_zero_work")
-        traffic = pmc_traffic(args.workload, kernels, extra)
-    if traffic is not None and rec.get("roofline"):
-        rec["roofline"]["traffic"] = round(traffic["bytes_per_step"])
-        rec["roofline"]["traffic_detail"] = traffic
+    # the whole metric: the path-traced frame is the headline; the closest-hit,
+    # batched, differentiable and AO workloads ride along as sub-records
+    rec = run_workload("pt", args, t_start)
+    subs = {}
+    for name in WORKLOADS[1:]:
+        print(f"[bench] {name} ({time.perf_counter() - t_start:.0f} s into the run)", file=sys.stderr, flush=True)
+        r = run_workload(name, args, t_start)
+        subs[name] = {k: v for k, v in r.items() if k not in _SUB_DROP}
+    rec["sub"] = subs
+    rec["run_s"] = round(time.perf_counter() - t_start, 1)
     print(json.dumps(rec))
 
 

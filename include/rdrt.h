@@ -438,7 +438,7 @@ typedef struct rdrt_pt_params {
   float camera_position[3];
   uint32_t width, height, spp;
   uint32_t max_bounces; /* scattering events per path (<= 16) */
-  uint32_t rr_start;    /* first bounce with Russian roulette */
+  uint32_t rr_start;    /* first bounce with Russian roulette (a frame with rr_start < max_bounces is not differentiable) */
   uint32_t seed;        /* per-pixel sample scrambles: xxhash32(px, py, seed * 2^16 + dimension pair) */
   uint32_t sample_index_base;
   float sky_up[3], sky_down[3];            /* miss radiance for direction.y > 0 / <= 0 */
@@ -506,9 +506,20 @@ int32_t rdrt_pt_set_material_params(rdrt_ctx *ctx, const float *albedo_dev, cons
 /* Gradient of sum(adj * frame) over the last frame rendered with record = 1,
  * w.r.t. the material albedo and emission (n * 3 f32 each, overwritten),
  * with visibility and sampled directions held fixed.  adj_dev holds
- * width * height RGB f32 (dLoss/dframe).  The frame must not use Russian
- * roulette (rr_start >= max_bounces): its termination probability depends on
- * the albedo. */
+ * views * height * width RGB f32 (dLoss/dframe) in the frame's image order:
+ * view after view, each row-major, pixel (x, y) of view k at
+ * ((k * height + y) * width + x) * 3 -- not the order paths or tiles are
+ * traced in.  Frames with Russian roulette (rr_start < max_bounces, such as
+ * the default rr_start = 2 of the 4-bounce benchmark frame) are not
+ * differentiable and return RDRT_ERR_UNSUPPORTED: the termination
+ * probability depends on the albedo.
+ * The result is bit-reproducible: every term is rounded once to a 64-bit
+ * fixed-point integer with a per-call scale (from the largest |term| and the
+ * term count) and the integers are summed, so the order the device adds
+ * them in does not change a bit; each term is exact to about 1e-10 of the
+ * largest term.  A NaN term (or both infinities) makes its entry NaN, one
+ * infinity that infinity.  The gradient arrays are written with plain
+ * stores (no atomics on caller memory). */
 int32_t rdrt_pt_backward(rdrt_ctx *ctx, const float *adj_dev, float *grad_albedo_dev, float *grad_emission_dev);
 /* Gradient of sum(adj * frame) over the last frame rendered with record = 1
  * w.r.t. vertex positions and normals (scene vertex order, see
@@ -520,11 +531,15 @@ int32_t rdrt_pt_backward(rdrt_ctx *ctx, const float *adj_dev, float *grad_albedo
  * positions act through the barycentrics of the fixed ray's hit (or the face
  * normal).  The paths are recomputed by replaying the frame, not stored.
  * grad_nrm_dev may be NULL when no geometry has vertex normals.  Same frame
- * conditions as rdrt_pt_backward. */
+ * conditions, adjoint layout and bit-reproducible sums as rdrt_pt_backward
+ * (the replay keeps 48 bytes of gradient terms per path and bounce, and 12
+ * bytes of sums per vertex component, in context scratch). */
 int32_t rdrt_pt_backward_geometry(rdrt_ctx *ctx, const float *adj_dev, float *grad_positions_dev,
                                   float *grad_normals_dev);
 /* L2 loss helper: adj_dev[i] = 2 (a[i] - b[i]); *loss_dev (one f64) =
- * sum (a[i] - b[i])^2.  Device pointers; enqueued on the context's stream. */
+ * sum (a[i] - b[i])^2, added in a fixed order (per-block partials, then the
+ * partials in block order): the same n and inputs give the same bits.
+ * Device pointers; enqueued on the context's stream. */
 int32_t rdrt_l2_adjoint(rdrt_ctx *ctx, const float *a_dev, const float *b_dev, uint32_t n, float *adj_dev,
                         double *loss_dev);
 

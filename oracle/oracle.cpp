@@ -1232,6 +1232,22 @@ struct PtVert {
   double cos_wo = 0.0;
 };
 
+// The camera ray of pixel (px, py), sample n, of view `view` (ray_gen.rs:
+// 89-107, camera.rs:66-99), FP32: jitter = (vdc, sobol) of the pixel's
+// scrambles 0 and 1, uv = pixel / size + jitter / size, direction toward the
+// unprojected uv at NDC depth 1.
+void camera_ray(const orc_pt_params &P, uint32_t px, uint32_t py, uint32_t n, uint32_t view, V3 &o, V3 &dir) {
+  const uint32_t seed = P.seed + view;
+  const float *vp = P.view_params ? P.view_params + 20 * (size_t)view : P.view_projection_inv;
+  const float *campos = P.view_params ? P.view_params + 20 * (size_t)view + 16 : P.camera_position;
+  const float jx = orc_van_der_corput(n, scramble_of(px, py, seed, 0));
+  const float jy = orc_sobol(n, scramble_of(px, py, seed, 1));
+  const float u = (float)px / (float)P.width + jx / (float)P.width;
+  const float v = (float)py / (float)P.height + jy / (float)P.height;
+  o = v3(campos[0], campos[1], campos[2]);
+  dir = normalize(f_unproject(vp, u, v) - o);
+}
+
 // One path; returns its radiance.  `rec`, if given, receives its vertices.
 D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *ext_cnt, uint64_t *sh_cnt,
            std::vector<PtVert> *rec = nullptr) {
@@ -1241,15 +1257,8 @@ D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *
   const uint32_t n = P.sample_index_base + path % P.spp;
   const uint32_t px = pixel % P.width, py = pixel / P.width;
   const uint32_t seed = P.seed + view;
-  const float *vp = P.view_params ? P.view_params + 20 * (size_t)view : P.view_projection_inv;
-  const float *campos = P.view_params ? P.view_params + 20 * (size_t)view + 16 : P.camera_position;
-  // ray generation (ray_gen.rs:89-107, camera.rs:66-99), FP32
-  const float jx = orc_van_der_corput(n, scramble_of(px, py, seed, 0));
-  const float jy = orc_sobol(n, scramble_of(px, py, seed, 1));
-  const float u = (float)px / (float)P.width + jx / (float)P.width;
-  const float v = (float)py / (float)P.height + jy / (float)P.height;
-  V3 o32 = v3(campos[0], campos[1], campos[2]);
-  V3 dir = normalize(f_unproject(vp, u, v) - o32);
+  V3 o32, dir;
+  camera_ray(P, px, py, n, view, o32, dir);
   D3 beta = d3(1, 1, 1), L = d3(0, 0, 0);
   V3 beta_f = v3(1.0f, 1.0f, 1.0f);  // the FP32 throughput the roulette decides on
   const D3 sky_up = d3(P.sky_up[0], P.sky_up[1], P.sky_up[2]);
@@ -1365,6 +1374,20 @@ D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *
 }  // namespace
 
 extern "C" {
+
+uint32_t orc_xxhash32(uint32_t x, uint32_t y, uint32_t z) { return xxhash32(x, y, z); }
+
+void orc_camera_ray(const orc_pt_params *P, uint32_t px, uint32_t py, uint32_t sample, uint32_t view, float *o3,
+                    float *d3) {
+  V3 o, d;
+  camera_ray(*P, px, py, sample, view, o, d);
+  o3[0] = o.x;
+  o3[1] = o.y;
+  o3[2] = o.z;
+  d3[0] = d.x;
+  d3[1] = d.y;
+  d3[2] = d.z;
+}
 
 int orc_set_vertex_normals(orc_scene *s, uint32_t blas, uint32_t geo, const float *nrm, uint32_t nverts) {
   if (blas >= s->blas_data.size() || geo >= s->blas_data[blas].size()) return -1;

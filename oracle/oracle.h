@@ -213,6 +213,13 @@ typedef struct orc_pt_params {
   const float *view_params;
 } orc_pt_params;
 
+/* xxhash32 of the reference's samplers (sampler.rs:24-41) */
+uint32_t orc_xxhash32(uint32_t x, uint32_t y, uint32_t z);
+/* The path tracer's camera ray of pixel (px, py), sample `sample` (the
+ * sample index itself, not an offset), view `view`: origin and unit
+ * direction, FP32. */
+void orc_camera_ray(const orc_pt_params *params, uint32_t px, uint32_t py, uint32_t sample, uint32_t view,
+                    float *origin3, float *dir3);
 int orc_set_vertex_normals(orc_scene *s, uint32_t blas, uint32_t geometry, const float *normals_xyz,
                            uint32_t nverts);
 int orc_set_triangle_materials(orc_scene *s, uint32_t blas, uint32_t geometry, const uint32_t *offsets,

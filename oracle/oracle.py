@@ -79,6 +79,8 @@ _SIG = {
     "orc_ao_sampler": (u32, [u32, u32, u32, u32, P, P, P]),
     "orc_ao_render": (C.c_int, [P, P, P, P, P, C.c_int]),
     "orc_pick_all": (C.c_int, [P, u32, u32, P, u32, P, u32, P]),
+    "orc_xxhash32": (u32, [u32, u32, u32]),
+    "orc_camera_ray": (None, [P, u32, u32, u32, u32, P, P]),
     "orc_van_der_corput": (f32, [u32, u32]),
     "orc_sobol": (f32, [u32, u32]),
     "orc_compact": (u32, [P, P, u32, P]),
@@ -422,6 +424,18 @@ def ao_hemisphere_sample(s: int) -> np.ndarray:
     out = np.zeros(3, dtype=np.float32)
     lib().orc_ao_hemisphere_sample(s, out.ctypes.data)
     return out
+
+
+def xxhash32(x: int, y: int, z: int) -> int:
+    return int(lib().orc_xxhash32(x, y, z))
+
+
+def camera_ray(params: "OrcPtParams", px: int, py: int, sample: int, view: int = 0):
+    """(origin, unit direction) float32 of the path tracer's camera ray."""
+    o = np.zeros(3, dtype=np.float32)
+    d = np.zeros(3, dtype=np.float32)
+    lib().orc_camera_ray(C.byref(params), px, py, sample, view, o.ctypes.data, d.ctypes.data)
+    return o, d
 
 
 def van_der_corput(n: int, scramble: int) -> float:

@@ -163,6 +163,42 @@ def test_parametric_scene_against_fp64_oracle(spp, wide, coop, height, device):
         assert abs(int(g) - int(r)) <= 0.005 * max(int(r), 1) + 2
 
 
+def test_full_path_tracing_frame_against_fp64_oracle(device):
+    """The benchmark frame at full size: 64 UV spheres / tori tessellated at
+    256 x 256 (8,388,608 triangles, 64 BLAS, random affine instances in
+    [-10, 10]^3), 1920 x 1080 at 1 spp, up to 4 bounces, Russian roulette
+    from bounce 2, seed 0, through the graph-replayed frame with the default
+    launch shape and the autotuned one.  Per-bounce extension and shadow ray
+    counts equal the FP64 oracle's exactly (every path decision is FP32 on
+    both sides); the image within the stated error."""
+    scene = synth.pt_scene(seed=0, n_meshes=64, tess=256)
+    cam = Camera()
+    cfg = PathTracingConfig(width=1920, height=1080, spp=1, max_bounces=4, rr_start=2, seed=0, use_graph=True)
+    _say("full frame: 8.39M triangles, 1920x1080, 1 spp, 4 bounces, RR from 2")
+    s = _system(scene)
+    assert s.scene_info().tris == 64 * 2 * 256 * 256
+    pt = PathTracer(s)
+    imgs = [pt.render(cam, cfg).cpu().numpy().astype(np.float64)]
+    stats = [pt.stats()]
+    pt.autotune(cam, cfg, reps=1)
+    imgs.append(pt.render(cam, cfg).cpu().numpy().astype(np.float64))
+    stats.append(pt.stats())
+    s.sync()
+    ref, ext, sh = oracle_scene_from(scene).pt_render(oracle_params(cam, cfg))
+    for img, st in zip(imgs, stats):
+        rmse = float(np.sqrt(np.mean((img - ref) ** 2)))
+        rel = rmse / float(ref.mean())
+        pix_err = np.abs(img - ref).max(axis=2) / np.maximum(ref.max(axis=2), 1e-6)
+        frac = float((pix_err <= PIXEL_REL).mean())
+        print(f"rel_rmse={rel:.3e} pixels_within={frac:.6f} ext={st['extension_rays']} shadow={st['shadow_rays']}")
+        assert rel <= IMAGE_RMSE_REL
+        assert frac >= PIXEL_FRACTION
+        assert [int(x) for x in st["extension_rays"]] == [int(x) for x in ext]
+        assert [int(x) for x in st["shadow_rays"][:-1]] == [int(x) for x in sh[:-1]]
+    assert np.array_equal(imgs[0], imgs[1])  # launch shapes change the schedule, not the result
+    s.close()
+
+
 def test_graph_replay_equals_eager(device):
     scene = synth.pt_scene(seed=3, n_meshes=16, tess=32, extent=5.0)
     cam = Camera(position=(0.0, 0.0, 18.0))
