@@ -587,6 +587,18 @@ int32_t rdrt_batch_autotune(rdrt_ctx *ctx, const rdrt_batch_params *params, uint
  * increasing order, count_dev (device u32) their number. */
 int32_t rdrt_compact(rdrt_ctx *ctx, const uint32_t *mask_dev, uint32_t n, uint32_t *out_idx_dev,
                      uint32_t *count_dev);
+/* rdrt_compact with no device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops).  Same outputs as rdrt_compact, for n up to
+ * RDRT_COMPACT_STATIC_MAX (larger n returns RDRT_ERR_CAPACITY).  Its scratch
+ * is a fixed device array of the library (about 8.5 MB, held by each device
+ * the library runs on until the process ends); every call on one device
+ * shares it, so calls that overlap on one device (other streams, other
+ * contexts, other threads) race and must be ordered by the caller.  Enqueued
+ * on the context's stream. */
+#define RDRT_COMPACT_STATIC_MAX 2097152u
+int32_t rdrt_compact_static(rdrt_ctx *ctx, const uint32_t *mask_dev, uint32_t n, uint32_t *out_idx_dev,
+                            uint32_t *count_dev);
 /* One scrambled 2-D Sobol point on the host, bit-identical to the device
  * samplers: out2[0] = van_der_corput(n, scramble0), out2[1] = sobol(n,
  * scramble1) (math/statistics/src/sampling/sobol.rs:40-68). */

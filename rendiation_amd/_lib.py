@@ -186,6 +186,7 @@ SIGNATURES = {
     "rdrt_get_tune": (i32, [C.c_void_p, C.POINTER(RdrtTune)]),
     "rdrt_batch_get_stats": (i32, [C.c_void_p, C.POINTER(RdrtBatchStats)]),
     "rdrt_compact": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p]),
+    "rdrt_compact_static": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p]),
     "rdrt_sample_sobol": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, u32, C.c_void_p]),
 }
 

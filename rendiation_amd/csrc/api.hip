@@ -191,6 +191,7 @@ int32_t rdrt_ctx_create(const rdrt_config *cfg, rdrt_ctx **out) {
       delete c;
       return fail(RDRT_ERR_HIP, "device set-up failed");
     }
+    static_scratch_init(cfg->device);
     // Graph capture needs a real stream: the user's, or a blocking stream of
     // our own (ordered with the null stream) at the default priority.
     c->work_stream = c->stream;

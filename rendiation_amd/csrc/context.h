@@ -172,6 +172,10 @@ TraceParams scene_trace_params(const rdrt_ctx *c);
 // Deepest push count of the traversal layout that t selects.
 uint32_t push_depth(const rdrt_ctx *c, const TraceTune &t);
 int32_t cull_scratch(rdrt_ctx *c, uint32_t n, CullScratch &cs);
+// Looks up the device address of the fixed scratch of rdrt_compact_static
+// for `device` (current) once per process; a failure is reported by that
+// entry point.
+void static_scratch_init(int device);
 
 }  // namespace rdrt
 
