@@ -245,6 +245,18 @@ typedef struct rdrt_hit_attributes {
   uint32_t sbt_record;       /* sbt_offset + sbt_stride * geometry + instance sbt offset; RDRT_MISS on a miss */
 } rdrt_hit_attributes;
 
+/* The rest of the reference's closest-hit context (HitCtxInfo,
+ * shader/ray-tracing/src/api/ctx.rs:24-48; HitCtxStorage,
+ * wavefront_compute/ctx.rs:63-74), 160 B. */
+typedef struct rdrt_hit_transform {
+  float object_to_world[16];     /* gl_ObjectToWorldEXT: the instance transform as given (column-major) */
+  float world_to_object[16];     /* gl_WorldToObjectEXT: its inverse as the traversal computed it */
+  float object_ray_origin[3];    /* gl_ObjectRayOriginEXT: world_to_object * (origin, 1), w-divided */
+  float object_ray_direction[3]; /* gl_ObjectRayDirectionEXT: 3x3 of world_to_object * direction, normalized */
+  uint32_t instance_sbt_offset;  /* the instance's hit-group offset */
+  uint32_t pad;
+} rdrt_hit_transform;
+
 /* ---- device task graph (shader/task-graph/src/runtime) ---- */
 /* Groups of tasks that spawn, sleep on, wake and finish one another, polled
  * round by round on the context's stream.  A task is in one of the states of
@@ -355,6 +367,13 @@ int32_t rdrt_trace(rdrt_ctx *ctx, const rdrt_trace_args *args);
 int32_t rdrt_fetch_hit_attributes(rdrt_ctx *ctx, const rdrt_ray *rays, const rdrt_ray_meta *ray_meta,
                                   const rdrt_hit *hits, const rdrt_hit_meta *hit_meta, uint32_t n, uint32_t tlas_idx,
                                   uint32_t sbt_offset, uint32_t sbt_stride, rdrt_hit_attributes *out);
+/* The hit context of closest hits (transforms and object-space ray, as
+ * the reference's closest-hit stage sees them; traverse_gpu.rs:575-582,
+ * 661-683), for the same inputs as rdrt_fetch_hit_attributes; misses give
+ * zeros.  Enqueued on the context stream. */
+int32_t rdrt_fetch_hit_transforms(rdrt_ctx *ctx, const rdrt_ray *rays, const rdrt_ray_meta *ray_meta,
+                                  const rdrt_hit *hits, const rdrt_hit_meta *hit_meta, uint32_t n, uint32_t tlas_idx,
+                                  rdrt_hit_transform *out);
 /* Closest hit with flags 0, cull mask ~0, TLAS binding 0, the context's
  * default traversal. */
 int32_t rdrt_trace_closest(rdrt_ctx *ctx, const rdrt_ray *rays_dev, uint32_t n, rdrt_hit *hits_dev);
@@ -476,7 +495,7 @@ typedef struct rdrt_pt_stats {
   uint32_t trace_launches;
   uint64_t visits[3];           /* node visits, triangle tests, instance tests (RDRT_PT_DIAG_COUNT_*) */
   uint32_t max_ray_visits[33];  /* per launch in frame order: the largest visit sum of one ray (RDRT_PT_DIAG_COUNT_*) */
-  uint32_t pad;
+  float launch_ms[33];          /* per launch in frame order: its HIP-event time (RDRT_PT_DIAG_TIME_TRACES) */
 } rdrt_pt_stats;
 
 /* Renders one frame: radiance_dev receives views*width*height RGB f32 (the

@@ -498,6 +498,7 @@ struct Geometry {
   uint32_t ntris, flags;
   std::vector<float> normals;         // per vertex (path tracer), optional
   std::vector<uint32_t> tri_material;  // per triangle (path tracer), optional
+  std::vector<float> uvs;             // per vertex (hit attributes), optional
 };
 
 struct orc_scene {
@@ -528,7 +529,11 @@ struct orc_scene {
   bool built = false;
   // path-tracer shading data
   std::vector<V3> normals;            // parallel to vertices; zero = none
+  std::vector<float> uvs;             // 2 per vertex, parallel to vertices (zeros = none)
+  bool has_uvs = false;
   std::vector<uint32_t> tri_material;  // per global triangle
+  std::vector<M4> inst_transform;      // object-to-world of each instance slot (as given)
+  std::vector<uint32_t> tlas_inst_first;  // first instance slot of each TLAS
   std::vector<float> materials;        // 8 floats each: albedo xyz, pad, emission xyz, pad
 };
 
@@ -588,6 +593,10 @@ int orc_build(orc_scene *s) {
   s->indices.clear();
   s->vertices.clear();
   s->normals.clear();
+  s->uvs.clear();
+  s->has_uvs = false;
+  s->inst_transform.clear();
+  s->tlas_inst_first.clear();
   s->tri_material.clear();
   s->blas_meta.clear();
   s->geo_meta.clear();
@@ -678,6 +687,12 @@ int orc_build(orc_scene *s) {
       for (size_t v = 0; v < nv; v++)
         s->normals.push_back(g.normals.size() == 3 * nv ? v3(g.normals[3 * v], g.normals[3 * v + 1], g.normals[3 * v + 2])
                                                         : v3(0.0f, 0.0f, 0.0f));
+      for (size_t v = 0; v < nv; v++) {
+        const bool has = g.uvs.size() == 2 * nv;
+        s->uvs.push_back(has ? g.uvs[2 * v] : 0.0f);
+        s->uvs.push_back(has ? g.uvs[2 * v + 1] : 0.0f);
+        s->has_uvs = s->has_uvs || has;
+      }
       for (size_t t = 0; t < ntri; t++) s->tri_material.push_back(t < g.tri_material.size() ? g.tri_material[t] : 0u);
       TriBvh tb;
       tb.primitive_start = primitive_start;
@@ -713,6 +728,8 @@ int orc_build(orc_scene *s) {
     const auto &src = s->tlas_data_src[ti];
     uint32_t bvh_start = (uint32_t)s->tlas_forest.size();
     uint32_t primitive_start = (uint32_t)s->inst_blas.size();
+    while (s->tlas_inst_first.size() < ti) s->tlas_inst_first.push_back(primitive_start);
+    s->tlas_inst_first.push_back(primitive_start);
     std::vector<Box3> aabbs;
     for (const orc_instance &in : src) {
       if (in.blas >= blas_box.size() || !blas_has_box[in.blas]) return -1;  // reference: unwrap() panics
@@ -727,6 +744,7 @@ int orc_build(orc_scene *s) {
       uint32_t flags = in.flags;
       if (m3_det(m) < 0.0f) flags ^= GI_FLIP_FACING;
       s->inst_transform_inv.push_back(m4_inverse_or_identity(m));
+      s->inst_transform.push_back(m);
       s->inst_custom.push_back(in.instance_custom_index);
       s->inst_sbt.push_back(in.sbt_offset);
       s->inst_flags.push_back(flags);
@@ -740,6 +758,8 @@ int orc_build(orc_scene *s) {
       s->tlas_forest.push_back(to_gpu_node(bvh.nodes[k], links[k].first, links[k].second, bvh_start, primitive_start));
     s->tlas_bvh_root.push_back(bvh_start);
   }
+  while (s->tlas_inst_first.size() < s->tlas_data_src.size()) s->tlas_inst_first.push_back((uint32_t)s->inst_blas.size());
+  s->tlas_inst_first.push_back((uint32_t)s->inst_blas.size());  // end of the last TLAS
   s->built = true;
   return 0;
 }
@@ -1376,6 +1396,110 @@ D3 pt_path(const orc_scene *s, const orc_pt_params &P, uint32_t path, uint64_t *
 extern "C" {
 
 uint32_t orc_xxhash32(uint32_t x, uint32_t y, uint32_t z) { return xxhash32(x, y, z); }
+
+int orc_set_vertex_uvs(orc_scene *s, uint32_t blas, uint32_t geo, const float *uv, uint32_t nverts) {
+  if (blas >= s->blas_data.size() || geo >= s->blas_data[blas].size()) return -1;
+  Geometry &g = s->blas_data[blas][geo];
+  if ((size_t)nverts * 3 != g.pos.size()) return -1;
+  g.uvs.assign(uv, uv + 2 * (size_t)nverts);
+  s->built = false;
+  return 0;
+}
+
+// Shading attributes of closest hits, as the reference's closest-hit stage
+// fetches them through the bindless mesh bridge (scene/rendering/gpu-ray-
+// tracing/src/bindless_mesh_bridge.rs:43-123), in FP32 with its operation
+// order: world position o + t d (api/ctx.rs:209-211); geometric normal
+// normalize(N (a - b) x (a - c)) turned to face the ray origin, N the
+// transpose of the instance's world-to-object 3x3 (:95,104-114); shading
+// normal normalize(N (w na + u nb + v nc)), w = 1 - u - v, on the geometric
+// normal's side (:86-123; the geometric normal when the vertex normals are
+// zero); uv interpolated the same way; hit group offset + stride * geometry
+// + instance offset (shader/ray-tracing/src/api/ctx.rs:53-55).  And the hit
+// context of the reference's closest-hit stage (HitCtxInfo, api/ctx.rs:24-
+// 48, filled at traverse_gpu.rs:575-582,661-683): object-to-world (the
+// instance transform), world-to-object (its inverse), the object-space ray
+// (world-to-object origin with the w divide, normalized 3x3 direction;
+// traverse_gpu.rs:427-431).  attr: 12 words per hit (position 3, shading
+// normal 3, geometric normal 3, uv 2, hit group as u32; zeros and ~0 on a
+// miss); xf (may be NULL): 40 words per hit (object-to-world 16,
+// world-to-object 16, object origin 3, object direction 3, instance sbt
+// offset as u32, pad; zeros on a miss).
+int orc_hit_attributes(const orc_scene *s, const orc_ray *rays, const orc_ray_meta *meta, const orc_hit *hits,
+                       const orc_hit_meta *hmeta, uint32_t n, uint32_t tlas_idx, uint32_t sbt_offset,
+                       uint32_t sbt_stride, float *attr, float *xf) {
+  if (!s->built) return -1;
+  for (uint32_t i = 0; i < n; i++) {
+    float *o = attr + 12 * (size_t)i;
+    float *x = xf ? xf + 40 * (size_t)i : nullptr;
+    for (int k = 0; k < 12; k++) o[k] = 0.0f;
+    const uint32_t miss = 0xFFFFFFFFu;
+    std::memcpy(o + 11, &miss, 4);
+    if (x)
+      for (int k = 0; k < 40; k++) x[k] = 0.0f;
+    const orc_hit &h = hits[i];
+    const orc_hit_meta &hm = hmeta[i];
+    const uint32_t ti_bind = meta ? meta[i].tlas_idx : tlas_idx;
+    if (h.prim_id == miss || ti_bind >= s->binding.size()) continue;
+    const uint32_t th = s->binding[ti_bind];
+    if (th + 1 >= s->tlas_inst_first.size() || th >= s->tlas_live.size() || !s->tlas_live[th]) continue;
+    // the instance slot holding source instance hm.instance_id of this TLAS
+    uint32_t slot = miss;
+    for (uint32_t k = s->tlas_inst_first[th]; k < s->tlas_inst_first[th + 1]; k++)
+      if (s->inst_order[k] == hm.instance_id) slot = k;
+    if (slot == miss) continue;
+    const auto &bm = s->blas_meta[s->inst_blas[slot]];
+    if (hm.geometry_id >= bm.second) continue;
+    const auto &geo = s->geo_meta[bm.first + hm.geometry_id];
+    const uint32_t tri = geo.primitive_start + h.prim_id;
+    const uint32_t ia = s->indices[3 * (size_t)tri], ib = s->indices[3 * (size_t)tri + 1],
+                   ic = s->indices[3 * (size_t)tri + 2];
+    const M4 &inv = s->inst_transform_inv[slot];
+    const orc_ray &r = rays[i];
+    const V3 org = v3(r.ox, r.oy, r.oz), dir = v3(r.dx, r.dy, r.dz);
+    const V3 pos = org + dir * h.t;
+    const V3 pa = s->vertices[ia], pb = s->vertices[ib], pc = s->vertices[ic];
+    V3 gn = normalize(f_normal_to_world(inv, cross(pa - pb, pa - pc)));
+    if (dot(org - pos, gn) < 0.0f) gn = gn * -1.0f;
+    V3 sn = gn;
+    const float w = 1.0f - h.u - h.v;
+    if (!s->normals.empty()) {
+      const V3 nn = s->normals[ia] * w + s->normals[ib] * h.u + s->normals[ic] * h.v;
+      if (dot(nn, nn) > 0.0f) {
+        sn = normalize(f_normal_to_world(inv, nn));
+        if (dot(gn, sn) < 0.0f) sn = sn * -1.0f;
+      }
+    }
+    if (s->has_uvs)
+      for (int k = 0; k < 2; k++)
+        o[9 + k] = s->uvs[2 * (size_t)ia + k] * w + s->uvs[2 * (size_t)ib + k] * h.u + s->uvs[2 * (size_t)ic + k] * h.v;
+    o[0] = pos.x;
+    o[1] = pos.y;
+    o[2] = pos.z;
+    o[3] = sn.x;
+    o[4] = sn.y;
+    o[5] = sn.z;
+    o[6] = gn.x;
+    o[7] = gn.y;
+    o[8] = gn.z;
+    const uint32_t group = sbt_offset + sbt_stride * hm.geometry_id + s->inst_sbt[slot];
+    std::memcpy(o + 11, &group, 4);
+    if (x) {
+      std::memcpy(x, &s->inst_transform[slot], 64);
+      std::memcpy(x + 16, &inv, 64);
+      const V3 bo = m4_mul_point(inv, org);
+      const V3 bd = normalize(m3_mul(inv, dir));
+      x[32] = bo.x;
+      x[33] = bo.y;
+      x[34] = bo.z;
+      x[35] = bd.x;
+      x[36] = bd.y;
+      x[37] = bd.z;
+      std::memcpy(x + 38, &s->inst_sbt[slot], 4);
+    }
+  }
+  return 0;
+}
 
 void orc_camera_ray(const orc_pt_params *P, uint32_t px, uint32_t py, uint32_t sample, uint32_t view, float *o3,
                     float *d3) {

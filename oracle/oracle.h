@@ -213,6 +213,13 @@ typedef struct orc_pt_params {
   const float *view_params;
 } orc_pt_params;
 
+/* Per-vertex texture coordinates (2 f32 per vertex) of one geometry. */
+int orc_set_vertex_uvs(orc_scene *s, uint32_t blas, uint32_t geometry, const float *uv, uint32_t nverts);
+/* Shading attributes and hit context of closest hits (see oracle.cpp):
+ * attr 12 words per hit, xf (or NULL) 40 words per hit. */
+int orc_hit_attributes(const orc_scene *s, const orc_ray *rays, const orc_ray_meta *meta, const orc_hit *hits,
+                       const orc_hit_meta *hit_meta, uint32_t n, uint32_t tlas_idx, uint32_t sbt_offset,
+                       uint32_t sbt_stride, float *attr, float *xf);
 /* xxhash32 of the reference's samplers (sampler.rs:24-41) */
 uint32_t orc_xxhash32(uint32_t x, uint32_t y, uint32_t z);
 /* The path tracer's camera ray of pixel (px, py), sample `sample` (the

@@ -80,6 +80,8 @@ _SIG = {
     "orc_ao_render": (C.c_int, [P, P, P, P, P, C.c_int]),
     "orc_pick_all": (C.c_int, [P, u32, u32, P, u32, P, u32, P]),
     "orc_xxhash32": (u32, [u32, u32, u32]),
+    "orc_set_vertex_uvs": (C.c_int, [P, u32, u32, P, u32]),
+    "orc_hit_attributes": (C.c_int, [P, P, P, P, P, u32, u32, u32, u32, P, P]),
     "orc_camera_ray": (None, [P, u32, u32, u32, u32, P, P]),
     "orc_van_der_corput": (f32, [u32, u32]),
     "orc_sobol": (f32, [u32, u32]),
@@ -246,6 +248,30 @@ class OracleScene:
         if rc != 0:
             raise RuntimeError("oracle batch round failed (scene not built)")
         return hits, spawned, alive
+
+    def set_vertex_uvs(self, blas: int, geometry: int, uvs) -> None:
+        u = np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1, 2)
+        if self.L.orc_set_vertex_uvs(self.h, blas, geometry, u.ctypes.data, u.shape[0]) != 0:
+            raise ValueError("bad uvs")
+
+    def hit_attributes(self, rays: np.ndarray, hits: np.ndarray, hit_meta: np.ndarray, meta=None, tlas_idx=0,
+                       sbt_offset=0, sbt_stride=0):
+        """(attributes (n, 12) u32 words, hit context (n, 40) u32 words) of
+        closest hits, the bindless-mesh fetch and the hit context of the
+        reference's closest-hit stage (float words viewed as u32)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        hits = np.ascontiguousarray(hits).view(np.uint32).reshape(-1, 4)
+        hm = np.ascontiguousarray(hit_meta).view(np.uint32).reshape(-1, 4)
+        if meta is not None:
+            meta = np.ascontiguousarray(meta).view(np.uint32).reshape(-1, 4)
+        attr = np.zeros((n, 12), dtype=np.uint32)
+        xf = np.zeros((n, 40), dtype=np.uint32)
+        rc = self.L.orc_hit_attributes(self.h, rays.ctypes.data, _p(meta), hits.ctypes.data, hm.ctypes.data, n,
+                                       tlas_idx, sbt_offset, sbt_stride, attr.ctypes.data, xf.ctypes.data)
+        if rc != 0:
+            raise RuntimeError("oracle hit attributes failed (scene not built)")
+        return attr, xf
 
     # ---- path tracer ----
     def set_vertex_normals(self, blas: int, geometry: int, normals) -> None:

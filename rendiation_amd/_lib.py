@@ -93,7 +93,7 @@ class RdrtPtParams(C.Structure):
 class RdrtPtStats(C.Structure):
     _fields_ = [("bounces", u32), ("extension_rays", u32 * 17), ("shadow_rays", u32 * 17), ("frame_ms", f32),
                 ("trace_ms", f32), ("trace_launches", u32), ("visits", C.c_uint64 * 3),
-                ("max_ray_visits", u32 * 33), ("pad", u32)]
+                ("max_ray_visits", u32 * 33), ("launch_ms", f32 * 33)]
 
 
 RDRT_PT_DIAG_TIME_TRACES = 0x1
@@ -162,6 +162,8 @@ SIGNATURES = {
     "rdrt_refit": (i32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "rdrt_set_device_build": (i32, [C.c_void_p, u32]),
     "rdrt_fetch_hit_attributes": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, u32, u32, u32, u32,
+                                       C.c_void_p]),
+    "rdrt_fetch_hit_transforms": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, u32, u32,
                                        C.c_void_p]),
     "rdrt_blas_set_vertex_uvs": (i32, [C.c_void_p, u32, u32, C.c_void_p, u32]),
     "rdrt_blas_add_triangles_u16": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p, u32, u32]),

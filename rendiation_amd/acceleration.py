@@ -311,6 +311,21 @@ class AccelerationStructureSystem:
         L.check(self._L.rdrt_fetch_hit_attributes(self._h, _ptr(rays), _ptr(ray_meta), _ptr(hits), _ptr(hit_meta), n,
                                                   tlas_idx, sbt_offset, sbt_stride, _ptr(out)))
 
+    def hit_transforms(self, rays, hits, hit_meta, out, *, ray_meta=None, tlas_idx: int = 0) -> None:
+        """Hit context of traced closest hits into ``out`` (device tensor of
+        n * 40 32-bit words: object-to-world 16, world-to-object 16,
+        object-space ray origin 3 and direction 3, instance hit-group offset,
+        pad)."""
+        n = rays.numel() // 8
+        _check_dev(rays, n * 8, "rays")
+        _check_dev(hits, n * 4, "hits")
+        _check_dev(hit_meta, n * 4, "hit_meta")
+        _check_dev(out, n * 40, "out")
+        if ray_meta is not None:
+            _check_dev(ray_meta, n * 4, "ray_meta")
+        L.check(self._L.rdrt_fetch_hit_transforms(self._h, _ptr(rays), _ptr(ray_meta), _ptr(hits), _ptr(hit_meta), n,
+                                                  tlas_idx, _ptr(out)))
+
     def set_triangle_materials(self, blas: BlasHandle, geometry: int, offsets: np.ndarray) -> None:
         o = np.ascontiguousarray(offsets, dtype=np.uint32).reshape(-1)
         L.check(self._L.rdrt_blas_set_triangle_materials(self._h, blas.index, geometry, o.ctypes.data, o.size))

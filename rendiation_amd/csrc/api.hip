@@ -61,6 +61,7 @@ void free_device_scene(DeviceScene &d) {
   release(d.normals);
   release(d.geos);
   release(d.insts);
+  release(d.inst_fwd);
   release(d.tlas_roots);
   release(d.tri_forest);
   release(d.tlas_forest);
@@ -73,6 +74,42 @@ void release_refit(rdrt_ctx *c) {
   release(c->refit_quads);
   c->refit_boxes.free();
   c->refit_ready = false;
+}
+
+// Shading attributes and / or hit context of closest hits (one kernel).
+int32_t fetch_hit_data(rdrt_ctx *c, const rdrt_ray *rays, const rdrt_ray_meta *ray_meta, const rdrt_hit *hits,
+                       const rdrt_hit_meta *hit_meta, uint32_t n, uint32_t tlas_idx, uint32_t sbt_offset,
+                       uint32_t sbt_stride, rdrt_hit_attributes *out, rdrt_hit_transform *xf) {
+  int32_t rc = need_device(c);
+  if (rc) return rc;
+  if (!c->uploaded) return fail(RDRT_ERR_NOT_BUILT, "scene is not committed (call rdrt_commit)");
+  const DeviceScene &d = c->dev;
+  AttribArgs a{};
+  a.rays = rays;
+  a.meta = ray_meta;
+  a.hits = hits;
+  a.hit_meta = hit_meta;
+  a.n = n;
+  a.tlas_idx = tlas_idx;
+  a.sbt_offset = sbt_offset;
+  a.sbt_stride = sbt_stride;
+  a.out = out;
+  a.xf_out = xf;
+  a.binding = d.binding.ptr;
+  a.nbinding = (uint32_t)d.binding.count;
+  a.ntlas = (uint32_t)d.tlas_inst_first.count;
+  a.ninst = (uint32_t)d.inst_lookup.count;
+  a.tlas_inst_first = d.tlas_inst_first.ptr;
+  a.inst_lookup = d.inst_lookup.ptr;
+  a.insts = d.insts.ptr;
+  a.inst_fwd = d.inst_fwd.ptr;
+  a.geos = d.geos.ptr;
+  a.indices = d.indices.ptr;
+  a.vertices = d.vertices.ptr;
+  a.normals = d.normals.count ? d.normals.ptr : nullptr;
+  a.uvs = d.uvs.count ? d.uvs.ptr : nullptr;
+  RDRT_HIP(enqueue_hit_attributes(a, c->stream));
+  return RDRT_OK;
 }
 
 int32_t need_device(rdrt_ctx *c) {
@@ -418,6 +455,7 @@ int32_t rdrt_commit(rdrt_ctx *c) {
   RDRT_HIP(upload(d.vertices, b.vertices, d.bytes));
   RDRT_HIP(upload(d.geos, b.geos, d.bytes));
   RDRT_HIP(upload(d.insts, b.insts, d.bytes));
+  RDRT_HIP(upload(d.inst_fwd, b.inst_fwd, d.bytes));
   RDRT_HIP(upload(d.tlas_roots, b.tlas_roots, d.bytes));
   RDRT_HIP(upload(d.tri_forest, b.tri_forest, d.bytes));
   RDRT_HIP(upload(d.tlas_forest, b.tlas_forest, d.bytes));
@@ -494,6 +532,7 @@ int32_t rdrt_refit(rdrt_ctx *c, const float *positions, const float *normals) {
   if (rc != RDRT_OK) return fail(rc, err);
   drop_captures(c);
   RDRT_HIP(reupload(d.insts, b.insts, d.bytes));
+  RDRT_HIP(reupload(d.inst_fwd, b.inst_fwd, d.bytes));
   RDRT_HIP(reupload(d.tlas_forest, b.tlas_forest, d.bytes));
   RDRT_HIP(reupload(d.tlas_pairs, b.tlas_pairs, d.bytes));
   RDRT_HIP(reupload(d.tlas_quads, b.tlas_quads, d.bytes));
@@ -638,34 +677,15 @@ int32_t rdrt_fetch_hit_attributes(rdrt_ctx *c, const rdrt_ray *rays, const rdrt_
                                   uint32_t sbt_offset, uint32_t sbt_stride, rdrt_hit_attributes *out) {
   if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
   if (n && (!rays || !hits || !hit_meta || !out)) return fail(RDRT_ERR_INVALID_ARG, "null rays, hits, metadata or output");
-  int32_t rc = need_device(c);
-  if (rc) return rc;
-  if (!c->uploaded) return fail(RDRT_ERR_NOT_BUILT, "scene is not committed (call rdrt_commit)");
-  const DeviceScene &d = c->dev;
-  AttribArgs a{};
-  a.rays = rays;
-  a.meta = ray_meta;
-  a.hits = hits;
-  a.hit_meta = hit_meta;
-  a.n = n;
-  a.tlas_idx = tlas_idx;
-  a.sbt_offset = sbt_offset;
-  a.sbt_stride = sbt_stride;
-  a.out = out;
-  a.binding = d.binding.ptr;
-  a.nbinding = (uint32_t)d.binding.count;
-  a.ntlas = (uint32_t)d.tlas_inst_first.count;
-  a.ninst = (uint32_t)d.inst_lookup.count;
-  a.tlas_inst_first = d.tlas_inst_first.ptr;
-  a.inst_lookup = d.inst_lookup.ptr;
-  a.insts = d.insts.ptr;
-  a.geos = d.geos.ptr;
-  a.indices = d.indices.ptr;
-  a.vertices = d.vertices.ptr;
-  a.normals = d.normals.count ? d.normals.ptr : nullptr;
-  a.uvs = d.uvs.count ? d.uvs.ptr : nullptr;
-  RDRT_HIP(enqueue_hit_attributes(a, c->stream));
-  return RDRT_OK;
+  return fetch_hit_data(c, rays, ray_meta, hits, hit_meta, n, tlas_idx, sbt_offset, sbt_stride, out, nullptr);
+}
+
+int32_t rdrt_fetch_hit_transforms(rdrt_ctx *c, const rdrt_ray *rays, const rdrt_ray_meta *ray_meta,
+                                  const rdrt_hit *hits, const rdrt_hit_meta *hit_meta, uint32_t n, uint32_t tlas_idx,
+                                  rdrt_hit_transform *out) {
+  if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
+  if (n && (!rays || !hits || !hit_meta || !out)) return fail(RDRT_ERR_INVALID_ARG, "null rays, hits, metadata or output");
+  return fetch_hit_data(c, rays, ray_meta, hits, hit_meta, n, tlas_idx, 0, 0, nullptr, out);
 }
 
 int32_t rdrt_set_tune(rdrt_ctx *c, const rdrt_tune *t) {

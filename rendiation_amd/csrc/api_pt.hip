@@ -485,6 +485,7 @@ int32_t rdrt_pt_get_stats(rdrt_ctx *c, rdrt_pt_stats *o) {
       float ms = 0.0f;
       RDRT_HIP(hipEventElapsedTime(&ms, c->pt.ev_begin[i], c->pt.ev_end[i]));
       o->trace_ms += ms;
+      if (i < 33) o->launch_ms[i] = ms;
     }
   }
   if (c->pt.last_diag & (RDRT_PT_DIAG_COUNT_VISITS | RDRT_PT_DIAG_COUNT_FAST)) {

@@ -15,12 +15,14 @@ struct AttribArgs {
   const rdrt_hit *hits;
   const rdrt_hit_meta *hit_meta;
   uint32_t n, tlas_idx, sbt_offset, sbt_stride;
-  rdrt_hit_attributes *out;
+  rdrt_hit_attributes *out;  // or null
+  rdrt_hit_transform *xf_out;  // or null: hit context (transforms, object-space ray)
   // scene
   const uint32_t *binding;
   uint32_t nbinding, ntlas, ninst;
   const uint32_t *tlas_inst_first, *inst_lookup;
   const InstRec *insts;
+  const float *inst_fwd;  // 16 per instance record: object-to-world
   const GeoRec *geos;
   const uint32_t *indices;
   const float *vertices, *normals, *uvs;  // normals / uvs null when the scene has none

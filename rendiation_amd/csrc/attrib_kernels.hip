@@ -1,7 +1,10 @@
 // attrib_kernels.hip -- shading attributes of closest hits: the reference's
 // bindless mesh fetch (scene/rendering/gpu-ray-tracing/src/
 // bindless_mesh_bridge.rs:43-123) and its shader-binding-table hit group
-// (shader/ray-tracing/src/api/ctx.rs:53-55), one thread per hit.
+// (shader/ray-tracing/src/api/ctx.rs:53-55), and the rest of the closest-hit
+// context (HitCtxInfo, api/ctx.rs:24-48, as traverse_gpu.rs:575-582,661-683
+// fills it): object-to-world and world-to-object transforms and the
+// object-space ray; one thread per hit.
 #include "attrib.h"
 
 #pragma clang fp contract(off)
@@ -58,7 +61,13 @@ __global__ void k_hit_attributes(AttribArgs a) {
     ok = geo < a.insts[inst].geo_end;
   }
   if (!ok) {
-    a.out[i] = o;
+    if (a.out) a.out[i] = o;
+    if (a.xf_out) {
+      rdrt_hit_transform z;
+      float *zw = reinterpret_cast<float *>(&z);
+      for (int k = 0; k < 40; k++) zw[k] = 0.0f;
+      a.xf_out[i] = z;
+    }
     return;
   }
   const InstRec &in = a.insts[inst];
@@ -100,7 +109,33 @@ __global__ void k_hit_attributes(AttribArgs a) {
   o.geometric_normal[1] = gn.y;
   o.geometric_normal[2] = gn.z;
   o.sbt_record = a.sbt_offset + a.sbt_stride * hm.geometry_id + in.sbt_offset;  // ctx.rs:53-55
-  a.out[i] = o;
+  if (a.out) a.out[i] = o;
+  if (a.xf_out) {
+    // the object-space ray the traversal used for this instance
+    // (traverse_gpu.rs:427-431): inverse * (o, 1) with the w divide, and the
+    // inverse's 3x3 times d, normalized
+    rdrt_hit_transform x;
+    const float *m = in.inv, *f = a.inst_fwd + 16 * (size_t)inst;
+    for (int k = 0; k < 16; k++) {
+      x.object_to_world[k] = f[k];
+      x.world_to_object[k] = m[k];
+    }
+    const float ox = org.x * m[0] + org.y * m[4] + org.z * m[8] + 1.0f * m[12];
+    const float oy = org.x * m[1] + org.y * m[5] + org.z * m[9] + 1.0f * m[13];
+    const float oz = org.x * m[2] + org.y * m[6] + org.z * m[10] + 1.0f * m[14];
+    const float ow = org.x * m[3] + org.y * m[7] + org.z * m[11] + 1.0f * m[15];
+    const A3 bd = normalize3(A3{dir.x * m[0] + dir.y * m[4] + dir.z * m[8], dir.x * m[1] + dir.y * m[5] + dir.z * m[9],
+                                dir.x * m[2] + dir.y * m[6] + dir.z * m[10]});
+    x.object_ray_origin[0] = ox / ow;
+    x.object_ray_origin[1] = oy / ow;
+    x.object_ray_origin[2] = oz / ow;
+    x.object_ray_direction[0] = bd.x;
+    x.object_ray_direction[1] = bd.y;
+    x.object_ray_direction[2] = bd.z;
+    x.instance_sbt_offset = in.sbt_offset;
+    x.pad = 0u;
+    a.xf_out[i] = x;
+  }
 }
 
 }  // namespace

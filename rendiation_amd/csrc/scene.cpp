@@ -551,6 +551,7 @@ bool validate_refs(const BuiltScene &s, std::string &err) {
 // refills those arrays, so a refit of the BLAS boxes can rebuild it.
 int build_tlas_layer(const SceneSrc &src, BuiltScene &out, std::string &err) {
   out.insts.clear();
+  out.inst_fwd.clear();
   out.inst_lookup.clear();
   out.tlas_inst_first.clear();
   out.tlas_forest.clear();
@@ -606,6 +607,7 @@ int build_tlas_layer(const SceneSrc &src, BuiltScene &out, std::string &err) {
       r.sbt_offset = in.sbt_offset;
       r.blas = in.blas;
       out.insts.push_back(r);
+      out.inst_fwd.insert(out.inst_fwd.end(), in.transform, in.transform + 16);
     }
     root.ref_root = (uint32_t)out.tlas_forest.size();
     append_forest(tree, inst_base, out.tlas_forest);

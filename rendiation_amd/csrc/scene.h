@@ -56,6 +56,7 @@ struct BuiltScene {
   // shared records
   std::vector<GeoRec> geos;
   std::vector<InstRec> insts;
+  std::vector<float> inst_fwd;  // 16 per instance record: its object-to-world transform as given
   std::vector<TlasRoot> tlas_roots;
   std::vector<uint32_t> binding;
   // child-pair layout

@@ -221,7 +221,8 @@ class PathTracer:
         return {"extension_rays": list(s.extension_rays[:nb]), "shadow_rays": list(s.shadow_rays[:nb]),
                 "frame_ms": float(s.frame_ms), "trace_ms": float(s.trace_ms), "trace_launches": int(s.trace_launches),
                 "visits": [int(x) for x in s.visits],
-                "max_ray_visits": [int(x) for x in s.max_ray_visits[:s.trace_launches]]}
+                "max_ray_visits": [int(x) for x in s.max_ray_visits[:s.trace_launches]],
+                "launch_ms": [float(x) for x in s.launch_ms[:s.trace_launches]]}
 
 
 def _f32():

@@ -156,3 +156,34 @@ def test_16_bit_indices(device):
     h32, m32, o32 = _trace(s32, rays, device)
     h16, m16, o16 = _trace(s16, rays, device)
     assert np.array_equal(h32, h16) and np.array_equal(m32, m16) and np.array_equal(o32, o16)
+
+
+def test_attributes_and_hit_context_match_the_oracle(device):
+    """The device fetch against the oracle's twin on the same (GPU) hits:
+    same FP32 operations in the same order, so the same bits -- position,
+    normals, uv, hit group, both transforms and the object-space ray."""
+    import torch
+
+    from tests.test_hit_attributes_oracle import oracle_scene
+
+    _say("hit attributes and hit context vs the oracle twin")
+    s, ts, _, _ = _scene(device)
+    rays_np = _rays(50_000, seed=9)
+    n = rays_np.shape[0]
+    rays = torch.from_numpy(rays_np).to(device)
+    hits = torch.empty((n, 4), dtype=torch.int32, device=device)
+    hm = torch.empty((n, 4), dtype=torch.int32, device=device)
+    s.trace(rays, hits, hit_meta=hm)
+    attr = torch.empty((n, 12), dtype=torch.int32, device=device)
+    xf = torch.empty((n, 40), dtype=torch.int32, device=device)
+    s.hit_attributes(rays, hits, hm, attr, sbt_offset=3, sbt_stride=2)
+    s.hit_transforms(rays, hits, hm, xf)
+    s.sync()
+    h, m = hits.cpu().numpy().view(np.uint32), hm.cpu().numpy().view(np.uint32)
+    o, _ = oracle_scene()
+    want_a, want_x = o.hit_attributes(rays_np, h, m, sbt_offset=3, sbt_stride=2)
+    got_a, got_x = attr.cpu().numpy().view(np.uint32), xf.cpu().numpy().view(np.uint32)
+    assert (h[:, 0] != 0xFFFFFFFF).mean() > 0.2
+    assert np.array_equal(got_a, want_a), int(np.sum(np.any(got_a != want_a, axis=1)))
+    assert np.array_equal(got_x, want_x), int(np.sum(np.any(got_x != want_x, axis=1)))
+    s.close()
