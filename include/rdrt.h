@@ -332,6 +332,85 @@ int32_t rdrt_ao_ray_counts(rdrt_ctx *ctx, uint32_t out[2]);
  * double precision. */
 int32_t rdrt_ao_hemisphere_sample(uint32_t s, float out[3]);
 
+/* ---- programmable ray-tracing pipelines on the device task graph ----
+ * The reference's stage pipelines (shader/ray-tracing/src/api/pipeline.rs:
+ * 75-130: a ray-gen stage, closest-hit stages per hit group, miss stages;
+ * operator/mod.rs:34-69: then_trace chains) run by its wavefront executor
+ * (wavefront_compute/{mod.rs:113-190, pipeline.rs:95-207, trace_task.rs}).
+ * A stage is a HIP kernel of the caller, `__global__ void f(rdrt_stage_args
+ * a)` (compiled for gfx950 into any code object of the process; pass its
+ * host-side handle, `(const void *)f`).  Each round the library launches it
+ * over the stage's polled tasks -- slots[i] for i < *count, a grid covering
+ * `capacity` threads -- and every polled task either resolves (its payload
+ * goes back to the task that traced it) or traces one more ray
+ * (then_trace): it writes the trace call and the payload the traced ray's
+ * stages receive, and is polled again once that ray's closest-hit or miss
+ * stage has resolved, with polls[slot] one higher and the returned payload
+ * in trace_result.  The trace task between them is the library's: it
+ * traverses the ray, runs the closest-hit stage of the hit group
+ * sbt_offset + sbt_stride * geometry + instance sbt offset (ctx.rs:53-55)
+ * or the miss stage miss_index, and hands the payload back. */
+#define RDRT_STAGE_RESOLVE 0u
+#define RDRT_STAGE_TRACE 1u
+#define RDRT_PIPELINE_MAX_PAYLOAD 16u
+#define RDRT_PIPELINE_HIT_ATTRIBUTES 1u /* fill rdrt_stage_args.attributes for every traced ray */
+#define RDRT_PIPELINE_HIT_TRANSFORMS 2u /* fill rdrt_stage_args.transforms likewise */
+typedef struct rdrt_trace_call { /* ShaderRayTraceCall (api/ty.rs:35-99) */
+  float origin[3], tmin;
+  float direction[3], tmax;
+  uint32_t flags, cull_mask, tlas_idx, miss_index;
+  uint32_t sbt_offset, sbt_stride, pad[2];
+} rdrt_trace_call; /* 64 B */
+typedef struct rdrt_stage_args {
+  const uint32_t *count;         /* device word: tasks polled this round */
+  const uint32_t *slots;         /* their slots: slots[i] for i < *count */
+  uint32_t *payload;             /* payload_words per slot: ray gen: the task's own words (zero at launch);
+                                    closest hit / miss: in = the payload of the traced ray, out = returned */
+  const uint32_t *trace_result;  /* payload_words per slot: the payload the task's last trace returned */
+  const uint32_t *polls;         /* per slot: traces of the task that have returned (0 at its first poll) */
+  const uint32_t *launch;        /* per slot: launch index x + y * launch_size[0] of the chain's ray gen */
+  const uint32_t *trace_of;      /* per slot (closest hit / miss): trace slot of the ray and hit below */
+  const rdrt_ray *rays;          /* per trace slot: the traced ray */
+  const rdrt_ray_meta *ray_meta; /* per trace slot: its flags, cull mask and TLAS index */
+  const rdrt_hit *hits;          /* per trace slot: the closest hit (prim_id ~0 on a miss) */
+  const rdrt_hit_meta *hit_meta; /* per trace slot */
+  const rdrt_hit_attributes *attributes; /* per trace slot with RDRT_PIPELINE_HIT_ATTRIBUTES, else NULL */
+  const rdrt_hit_transform *transforms;  /* per trace slot with RDRT_PIPELINE_HIT_TRANSFORMS, else NULL */
+  uint32_t *action;              /* out per slot: RDRT_STAGE_RESOLVE (preset) or RDRT_STAGE_TRACE */
+  rdrt_trace_call *call;         /* out per slot, with RDRT_STAGE_TRACE */
+  uint32_t *call_payload;        /* out per slot, with RDRT_STAGE_TRACE: payload_words for the ray's stages */
+  void *user;                    /* the pointer given to rdrt_pipeline_trace_rays */
+  uint32_t launch_size[2];
+  uint32_t payload_words;
+  uint32_t capacity;             /* slots of the stage (the grid covers this many threads) */
+} rdrt_stage_args;
+typedef struct rdrt_pipeline_desc {
+  const void *ray_gen;        /* stage kernel handles */
+  const void *closest_hit[4]; /* per hit group; NULL: a hit resolves the trace with the payload as it was */
+  const void *miss[4];        /* per miss index; NULL: likewise */
+  uint32_t nclosest, nmiss;   /* nclosest + nmiss <= 6 */
+  uint32_t payload_words;     /* 1 .. RDRT_PIPELINE_MAX_PAYLOAD */
+  uint32_t max_in_flight;     /* trace tasks per launch entry in flight (pipeline.rs:211-257); 0 = 2 */
+  uint32_t round_hint;        /* task-graph rounds per tile (execution_round_hint); 0 = 8 */
+  uint32_t tile_pixels;       /* launch entries per tile, whole rows (wavefront_compute/mod.rs:134); 0 = 512*512 */
+  uint32_t hit_context;       /* RDRT_PIPELINE_HIT_*: per-hit data the closest-hit stages read (ctx.rs:24-48) */
+} rdrt_pipeline_desc;
+typedef struct rdrt_pipeline rdrt_pipeline;
+/* GPURaytracingPipelineAndBindingSource + executor: groups trace, ray gen,
+ * the closest-hit stages, the miss stages. */
+int32_t rdrt_pipeline_create(rdrt_ctx *ctx, const rdrt_pipeline_desc *desc, rdrt_pipeline **out);
+int32_t rdrt_pipeline_destroy(rdrt_pipeline *p); /* before its context is destroyed */
+/* RayTracingEncoderProvider::trace_ray (backend.rs:48-56): one ray-gen task
+ * per launch entry (width x height, tiles of whole rows), round_hint rounds
+ * per tile, on the context's stream and its TLAS bindings. */
+int32_t rdrt_pipeline_trace_rays(rdrt_pipeline *p, uint32_t width, uint32_t height, void *user);
+/* Task states per group after the last call, as rdrt_task_graph_states
+ * (groups: trace, ray gen, closest hit..., miss...). */
+int32_t rdrt_pipeline_states(rdrt_pipeline *p, uint32_t *wake, uint32_t *sleep_or_finished, uint32_t *empty,
+                             uint32_t *errors);
+/* Rays the trace group traversed in the last call (waits for the stream). */
+int32_t rdrt_pipeline_ray_count(rdrt_pipeline *p, uint64_t *out);
+
 /* ---- host picking (content/mesh/core/src/feature/bvh.rs:57-86) ---- */
 #define RDRT_FACE_FRONT 0u
 #define RDRT_FACE_BACK 1u

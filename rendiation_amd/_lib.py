@@ -32,6 +32,18 @@ class RdrtAoParams(C.Structure):
                 ("tile_pixels", u32), ("use_graph", u32), ("pad", u32 * 2)]
 
 
+class RdrtPipelineDesc(C.Structure):
+    _fields_ = [("ray_gen", C.c_void_p), ("closest_hit", C.c_void_p * 4), ("miss", C.c_void_p * 4),
+                ("nclosest", u32), ("nmiss", u32), ("payload_words", u32), ("max_in_flight", u32),
+                ("round_hint", u32), ("tile_pixels", u32), ("hit_context", u32)]
+
+
+class RdrtTraceCall(C.Structure):
+    _fields_ = [("origin", f32 * 3), ("tmin", f32), ("direction", f32 * 3), ("tmax", f32),
+                ("flags", u32), ("cull_mask", u32), ("tlas_idx", u32), ("miss_index", u32),
+                ("sbt_offset", u32), ("sbt_stride", u32), ("pad", u32 * 2)]
+
+
 class RdrtMetrics(C.Structure):
     _fields_ = [("last_call_ms", f32), ("tune_ms", f32), ("device_bytes", C.c_uint64), ("build_ms", C.c_double),
                 ("status", u32), ("pad", u32)]
@@ -189,6 +201,11 @@ SIGNATURES = {
     "rdrt_batch_get_stats": (i32, [C.c_void_p, C.POINTER(RdrtBatchStats)]),
     "rdrt_compact": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p]),
     "rdrt_compact_static": (i32, [C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p]),
+    "rdrt_pipeline_create": (i32, [C.c_void_p, C.POINTER(RdrtPipelineDesc), C.c_void_p]),
+    "rdrt_pipeline_destroy": (i32, [C.c_void_p]),
+    "rdrt_pipeline_trace_rays": (i32, [C.c_void_p, u32, u32, C.c_void_p]),
+    "rdrt_pipeline_states": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "rdrt_pipeline_ray_count": (i32, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "rdrt_sample_sobol": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, u32, C.c_void_p]),
 }
 

@@ -83,8 +83,7 @@ int32_t fetch_hit_data(rdrt_ctx *c, const rdrt_ray *rays, const rdrt_ray_meta *r
   int32_t rc = need_device(c);
   if (rc) return rc;
   if (!c->uploaded) return fail(RDRT_ERR_NOT_BUILT, "scene is not committed (call rdrt_commit)");
-  const DeviceScene &d = c->dev;
-  AttribArgs a{};
+  AttribArgs a = scene_attrib_args(c);
   a.rays = rays;
   a.meta = ray_meta;
   a.hits = hits;
@@ -95,6 +94,13 @@ int32_t fetch_hit_data(rdrt_ctx *c, const rdrt_ray *rays, const rdrt_ray_meta *r
   a.sbt_stride = sbt_stride;
   a.out = out;
   a.xf_out = xf;
+  RDRT_HIP(enqueue_hit_attributes(a, c->stream));
+  return RDRT_OK;
+}
+
+AttribArgs scene_attrib_args(const rdrt_ctx *c) {
+  const DeviceScene &d = c->dev;
+  AttribArgs a{};
   a.binding = d.binding.ptr;
   a.nbinding = (uint32_t)d.binding.count;
   a.ntlas = (uint32_t)d.tlas_inst_first.count;
@@ -108,8 +114,7 @@ int32_t fetch_hit_data(rdrt_ctx *c, const rdrt_ray *rays, const rdrt_ray_meta *r
   a.vertices = d.vertices.ptr;
   a.normals = d.normals.count ? d.normals.ptr : nullptr;
   a.uvs = d.uvs.count ? d.uvs.ptr : nullptr;
-  RDRT_HIP(enqueue_hit_attributes(a, c->stream));
-  return RDRT_OK;
+  return a;
 }
 
 int32_t need_device(rdrt_ctx *c) {

@@ -50,14 +50,10 @@ void free_ao(AoState *&s) {
   s = nullptr;
 }
 
-}  // namespace rdrt
-
-namespace {
-
 // Counters of every group after a prepare and a compaction of each
 // (read_back_execution_states, mod.rs:305-370).
-int32_t read_states(tg::Graph &gr, hipStream_t st, uint32_t *wake, uint32_t *sof, uint32_t *empty,
-                    uint32_t *errors) {
+int32_t read_task_states(tg::Graph &gr, hipStream_t st, uint32_t *wake, uint32_t *sof, uint32_t *empty,
+                         uint32_t *errors) {
   for (uint32_t k = 0; k < gr.n; k++) {
     RDRT_HIP(tg::enqueue_prepare(gr, k, st));
     RDRT_HIP(tg::enqueue_compact(gr, k, st));
@@ -77,6 +73,10 @@ int32_t read_states(tg::Graph &gr, hipStream_t st, uint32_t *wake, uint32_t *sof
   }
   return RDRT_OK;
 }
+
+}  // namespace rdrt
+
+namespace {
 
 void hemisphere_sample(uint32_t s, float out[3]) {
   // hammersley_2d(s, 256) (sampling.rs:33-50)
@@ -201,7 +201,7 @@ int32_t rdrt_task_graph_states(rdrt_task_graph *g, uint32_t *wake, uint32_t *sof
   if (!g) return fail(RDRT_ERR_INVALID_ARG, "null graph");
   int32_t rc = need_device(g->ctx);
   if (rc) return rc;
-  return read_states(g->mem.dev, g->ctx->work_stream, wake, sof, empty, errors);
+  return read_task_states(g->mem.dev, g->ctx->work_stream, wake, sof, empty, errors);
 }
 
 int32_t rdrt_ao_hemisphere_sample(uint32_t s, float out[3]) {
@@ -394,7 +394,7 @@ int32_t rdrt_ao_states(rdrt_ctx *c, uint32_t *wake, uint32_t *sof, uint32_t *emp
   int32_t rc = need_device(c);
   if (rc) return rc;
   if (!c->ao || !c->ao->capacity) return fail(RDRT_ERR_STATE, "no AO frame rendered yet");
-  return read_states(c->ao->last, c->work_stream, wake, sof, empty, errors);
+  return read_task_states(c->ao->last, c->work_stream, wake, sof, empty, errors);
 }
 
 }  // extern "C"

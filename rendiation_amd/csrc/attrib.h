@@ -15,6 +15,11 @@ struct AttribArgs {
   const rdrt_hit *hits;
   const rdrt_hit_meta *hit_meta;
   uint32_t n, tlas_idx, sbt_offset, sbt_stride;
+  // optional: per-ray (sbt offset, sbt stride, -, -) in place of the two above
+  const uint4 *sbt;
+  // optional work queue: thread i fetches hit queue[i] for i < *count (outputs
+  // indexed by the queue entry; n = the grid's threads)
+  const uint32_t *queue, *count;
   rdrt_hit_attributes *out;  // or null
   rdrt_hit_transform *xf_out;  // or null: hit context (transforms, object-space ray)
   // scene
@@ -29,6 +34,9 @@ struct AttribArgs {
 };
 
 hipError_t enqueue_hit_attributes(const AttribArgs &a, hipStream_t stream);
+
+// The committed scene's part of AttribArgs (api.hip).
+AttribArgs scene_attrib_args(const rdrt_ctx *c);
 
 }  // namespace rdrt
 

@@ -36,8 +36,9 @@ __device__ __forceinline__ A3 normal_mat(const float *inv, A3 n) {
 }
 
 __global__ void k_hit_attributes(AttribArgs a) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n || (a.queue && t >= *a.count)) return;
+  const uint32_t i = a.queue ? a.queue[t] : t;
   rdrt_hit_attributes o;
   for (int k = 0; k < 3; k++) o.position[k] = o.shading_normal[k] = o.geometric_normal[k] = 0.0f;
   o.uv[0] = o.uv[1] = 0.0f;
@@ -108,7 +109,8 @@ __global__ void k_hit_attributes(AttribArgs a) {
   o.geometric_normal[0] = gn.x;
   o.geometric_normal[1] = gn.y;
   o.geometric_normal[2] = gn.z;
-  o.sbt_record = a.sbt_offset + a.sbt_stride * hm.geometry_id + in.sbt_offset;  // ctx.rs:53-55
+  const uint32_t sbt_offset = a.sbt ? a.sbt[i].x : a.sbt_offset, sbt_stride = a.sbt ? a.sbt[i].y : a.sbt_stride;
+  o.sbt_record = sbt_offset + sbt_stride * hm.geometry_id + in.sbt_offset;  // ctx.rs:53-55
   if (a.out) a.out[i] = o;
   if (a.xf_out) {
     // the object-space ray the traversal used for this instance

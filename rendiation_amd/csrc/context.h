@@ -10,6 +10,7 @@
 #include "../../include/rdrt.h"
 #include "pt.h"
 #include "scene.h"
+#include "task_graph.h"
 #include "trace_kernels.h"
 
 namespace rdrt {
@@ -176,6 +177,11 @@ int32_t cull_scratch(rdrt_ctx *c, uint32_t n, CullScratch &cs);
 // for `device` (current) once per process; a failure is reported by that
 // entry point.
 void static_scratch_init(int device);
+// Counters of every task group after a prepare and a compaction of each
+// (read_back_execution_states, task-graph/src/runtime/mod.rs:305-370);
+// flips the graph's parities.  Waits for the stream.
+int32_t read_task_states(tg::Graph &gr, hipStream_t st, uint32_t *wake, uint32_t *sof, uint32_t *empty,
+                         uint32_t *errors);
 
 }  // namespace rdrt
 

@@ -113,3 +113,82 @@ def batched_pair(scenes, opts):
     blas_list = [[(v, None, 1)] for v in scenes]
     tlas_list = [[{"transform": eye, "blas": k}] for k in range(len(scenes))]
     return build_pair(blas_list, tlas_list, list(range(len(scenes))), opts)
+
+
+def _mat_translate(x, y, z):
+    m = np.eye(4)
+    m[:3, 3] = (x, y, z)
+    return m
+
+
+def _mat_scale(x, y, z):
+    return np.diag([x, y, z, 1.0])
+
+
+def _mat_rotate(axis: int, a: float):
+    """Right-handed rotation about x (0), y (1) or z (2): Mat4::rotate_*
+    (math/algebra/src/mat/mat4.rs:301-335, column-major storage)."""
+    c, s = np.cos(a), np.sin(a)
+    m = np.eye(4)
+    i, j = [(1, 2), (2, 0), (0, 1)][axis]
+    m[i, i], m[i, j], m[j, i], m[j, j] = c, -s, s, c
+    return m
+
+
+CUBE_POSITIONS = np.array([
+    0.5, 0.5, 0.5, -0.5, 0.5, 0.5, -0.5, -0.5, 0.5, 0.5, -0.5, 0.5,
+    0.5, 0.5, 0.5, 0.5, -0.5, 0.5, 0.5, -0.5, -0.5, 0.5, 0.5, -0.5,
+    0.5, 0.5, 0.5, 0.5, 0.5, -0.5, -0.5, 0.5, -0.5, -0.5, 0.5, 0.5,
+    -0.5, 0.5, 0.5, -0.5, 0.5, -0.5, -0.5, -0.5, -0.5, -0.5, -0.5, 0.5,
+    -0.5, -0.5, -0.5, 0.5, -0.5, -0.5, 0.5, -0.5, 0.5, -0.5, -0.5, 0.5,
+    0.5, -0.5, -0.5, -0.5, -0.5, -0.5, -0.5, 0.5, -0.5, 0.5, 0.5, -0.5], dtype=np.float32).reshape(24, 3)
+CUBE_INDICES = np.array([0, 1, 2, 2, 3, 0, 4, 5, 6, 6, 7, 4, 8, 9, 10, 10, 11, 8, 12, 13, 14, 14, 15, 12,
+                         16, 17, 18, 18, 19, 16, 20, 21, 22, 22, 23, 20], dtype=np.uint32).reshape(12, 3)
+
+
+def reference_default_scene(opts, bind: int = 4):
+    """reference_default_lists(bind) in the product system and the oracle."""
+    return build_pair(*reference_default_lists(bind), opts)
+
+
+def oracle_only(blas_list, tlas_list, binding, opts) -> OracleScene:
+    """The oracle half of build_pair (no device needed)."""
+    orc = OracleScene(opts.blas_bin_size, opts.blas_max_depth, opts.tlas_bin_size, opts.tlas_max_depth, opts.sah_bins)
+    for geos in blas_list:
+        orc.add_blas(geos)
+    for insts in tlas_list:
+        orc.add_tlas(insts)
+    orc.bind(binding)
+    orc.build()
+    return orc
+
+
+def reference_default_lists(bind: int = 4):
+    """The scene of the reference's ray-tracing tests
+    (shader/ray-tracing/src/backend/wavefront_compute/geometry/naive/test.rs:
+    9-225): BLAS sphere (32 x 16 UV grid), torus (radius 1, tube 0.2, 32 x
+    8) and the 24-vertex cube, all opaque; five TLAS (a 5 x 5 wall of cubes
+    at z = -10 with four bars, a ring of six bars, a tilted torus, a 5 x 5 x
+    5 block of cubes, a sphere of radius 5 at z = -10); TLAS ``bind`` (4 in
+    the reference) bound at index 0.  Vertices are generated in FP64 and
+    rounded to FP32 (the reference evaluates them in FP32)."""
+    sph_p, sph_i = synth.uv_sphere(32, 16)
+    tor_p, tor_i = synth.torus(32, 8, 1.0, 0.2)
+    blas_list = [[(sph_p, sph_i, 1)], [(tor_p, tor_i, 1)], [(CUBE_POSITIONS, CUBE_INDICES, 1)]]
+    SPHERE, TORUS, CUBE = 0, 1, 2
+    T, S, R = _mat_translate, _mat_scale, _mat_rotate
+    pi = float(np.float32(np.pi))
+    t0 = [{"transform": T(i * 1.5, j * 1.5, -10.0), "blas": CUBE} for i in range(-2, 3) for j in range(-2, 3)]
+    t0 += [{"transform": T(0.0, 4.5, -10.0) @ S(5, 1, 1), "blas": CUBE},
+           {"transform": T(0.0, -4.5, -10.0) @ R(1, pi) @ S(5, 1, 1), "blas": CUBE},
+           {"transform": T(4.5, -4.5, -10.0) @ R(1, pi * 0.5) @ S(5, 1, 1), "blas": CUBE},
+           {"transform": T(-4.5, -4.5, -10.0) @ R(1, -pi * 0.5) @ S(5, 1, 1), "blas": CUBE}]
+    t1 = []
+    for i in range(6):
+        a = i / 6.0 * pi * 2.0
+        t1.append({"transform": T(np.sin(a) * 4.0, np.cos(a) * 4.0, -5.0) @ R(2, -a) @ S(3, 0.5, 0.5), "blas": CUBE})
+    t2 = [{"transform": T(0.0, 0.0, -10.0) @ S(5, 5, 5) @ R(0, -0.5), "blas": TORUS}]
+    t3 = [{"transform": T(i * 1.5, j * 1.5, -8.0 + k * 1.5), "blas": CUBE}
+          for i in range(-2, 3) for j in range(-2, 3) for k in range(-2, 3)]
+    t4 = [{"transform": T(0.0, 0.0, -10.0) @ S(5, 5, 5), "blas": SPHERE}]
+    return blas_list, [t0, t1, t2, t3, t4], [bind]

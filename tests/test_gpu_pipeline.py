@@ -1,0 +1,182 @@
+"""Programmable pipelines through the C-ABI (rdrt_pipeline_*), with the
+caller's stage kernels of tests/stages/pipeline_stages.hip.
+
+* The pipeline of shader/ray-tracing/src/test.rs (ray gen -> trace ->
+  closest hit / miss -> ray-gen continuation writing an RGBA8 image) on the
+  scene of the reference's tests (naive/test.rs:9-225, TLAS 4 bound): the
+  closest hits match the oracle's under the closest-hit contract, the image
+  is the stages' function of those hits, and it matches the committed
+  oracle fixture except on near-tie pixels.
+* Nested traces (closest hit -> trace -> closest hit, three deep, payloads
+  handed down and back up): per depth, the rays match the FP32 restatement
+  of the stages (from the oracle's hit attributes of the device hits), the
+  hits match the oracle's, and the payloads returned to the ray gen are the
+  function of the hits the stages saw.
+* Tiling, max_in_flight and the round hint do not change the result; every
+  task finishes; the ray count is what the stages traced.
+"""
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from rendiation_amd.acceleration import BuildOptions
+from rendiation_amd.rt_pipeline import (
+    HIT_ATTRIBUTES,
+    GPURaytracingPipelineAndBindingSource,
+    RayTracingPipeline,
+    ShaderBindingTable,
+)
+from tests import pipeline_expect as PE
+from tests.parity import compare
+from tests.scenes import reference_default_scene
+
+pytestmark = pytest.mark.gpu
+
+STAGES = Path(__file__).resolve().parent / "stages" / "libpipeline_stages.so"
+GOLDEN = Path(__file__).resolve().parent / "golden" / "pipeline_debug_256.npz"
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def stages(device):
+    if not STAGES.exists():
+        raise FileNotFoundError(f"{STAGES} is missing: run tests/stages/build_stages.py")
+    lib = C.CDLL(str(STAGES))
+    lib.test_stage.restype = C.c_void_p
+    lib.test_stage.argtypes = [C.c_uint32]
+    return [lib.test_stage(k) for k in range(6)]
+
+
+def _pipeline(system, stages, base, payload_words=1, hit_context=0, **kw):
+    src = GPURaytracingPipelineAndBindingSource(payload_words=payload_words, hit_context=hit_context, **kw)
+    rg = src.register_ray_gen(stages[base])
+    ch = src.register_ray_closest_hit(stages[base + 1])
+    ms = src.register_ray_miss(stages[base + 2])
+    sbt = ShaderBindingTable(ray_type_count=1)
+    sbt.config_ray_generation(rg)
+    sbt.config_hit_group(0, 0, 0, ch)
+    sbt.config_missing(0, ms)
+    return RayTracingPipeline(system, src, sbt)
+
+
+def _run_debug(system, stages, w, h, **kw):
+    import torch
+
+    n = w * h
+    buf = torch.full((n * 16 + n * 4,), 0xAB, dtype=torch.uint8, device="cuda:0")
+    torch.cuda.synchronize()
+    p = _pipeline(system, stages, 0, **kw)
+    p.trace_ray(w, h, buf.data_ptr())
+    system.sync()
+    out = buf.cpu().numpy()
+    states = p.read_back_execution_states()
+    rays = p.ray_count()
+    p.close()
+    hits = out[: n * 16].view(np.uint32).reshape(n, 4)
+    img = out[n * 16:].reshape(n, 4)
+    return hits, img, states, rays
+
+
+@pytest.fixture(scope="module")
+def ref_scene(device):
+    return reference_default_scene(BuildOptions())
+
+
+def test_debug_pipeline_reference_scene(stages, ref_scene):
+    system, orc = ref_scene
+    w = h = 256
+    _say(f"debug pipeline {w}x{h}")
+    hits, img, states, nrays = _run_debug(system, stages, w, h)
+    assert states.is_empty() and not any(states.errors), states
+    assert nrays == w * h
+    rays = PE.camera_rays(w, h, 0.1, 100.0)
+    ref, _, _ = orc.trace(rays, flags=PE.CULL_BACK)
+    d = compare(hits, ref)
+    assert d.ok() and d.value_mismatch_bits == 0, d
+    assert (hits[:, 0] != PE.MISS).sum() > 0.2 * w * h  # the sphere covers much of the view
+    # the image is the stages' function of the hits they saw
+    assert np.array_equal(img, PE.debug_image(hits))
+    # and the oracle fixture's wherever the hit is the oracle's
+    g = np.load(GOLDEN)
+    same = hits[:, 0] == g["prim"]
+    assert same.sum() >= w * h - d.near_ties
+    assert np.array_equal(img[same], g["rgba"].reshape(-1, 4)[same])
+
+
+@pytest.mark.parametrize("tile_pixels,max_in_flight,round_hint", [(4096, 1, 3), (256 * 7 + 5, 3, 12), (1 << 20, 2, 0)])
+def test_debug_pipeline_tiling_invariant(stages, ref_scene, tile_pixels, max_in_flight, round_hint):
+    system, _ = ref_scene
+    w, h = 256, 200
+    _say(f"debug pipeline tiles {tile_pixels} in-flight {max_in_flight} rounds {round_hint}")
+    base_hits, base_img, _, _ = _run_debug(system, stages, w, h)
+    hits, img, states, nrays = _run_debug(system, stages, w, h, tile_pixels=tile_pixels, max_in_flight=max_in_flight,
+                                          execution_round_hint=round_hint)
+    assert states.is_empty() and not any(states.errors), states
+    assert nrays == w * h
+    assert np.array_equal(hits, base_hits)
+    assert np.array_equal(img, base_img)
+
+
+def test_debug_pipeline_too_few_rounds_leaves_tasks(stages, ref_scene):
+    # a trace needs three rounds (ray gen, trace + closest hit, trace
+    # resolves + ray gen): with two, the ray-gen tasks are still asleep
+    system, _ = ref_scene
+    _say("debug pipeline, 2 rounds")
+    hits, img, states, nrays = _run_debug(system, stages, 64, 64, execution_round_hint=2)
+    assert not states.is_empty()
+    assert nrays == 64 * 64
+    assert not any(states.errors)
+
+
+@pytest.mark.parametrize("bind", [3, 0])
+def test_chain_pipeline(stages, device, bind):
+    import torch
+
+    system, orc = reference_default_scene(BuildOptions(), bind=bind)
+    w, h, D = 192, 160, PE.DEPTH
+    n = w * h
+    _say(f"chain pipeline tlas {bind} {w}x{h}")
+    nbytes = n * D * (32 + 16 + 16) + n * 3 * 4
+    buf = torch.full((nbytes,), 0xFF, dtype=torch.uint8, device="cuda:0")
+    torch.cuda.synchronize()
+    p = _pipeline(system, stages, 3, payload_words=3, hit_context=HIT_ATTRIBUTES)
+    p.trace_ray(w, h, buf.data_ptr())
+    system.sync()
+    out = buf.cpu().numpy()
+    states = p.read_back_execution_states()
+    nrays = p.ray_count()
+    p.close()
+    off = 0
+    rays = out[off: off + n * D * 32].view(np.float32).reshape(n, D, 8)
+    off += n * D * 32
+    hits = out[off: off + n * D * 16].view(np.uint32).reshape(n, D, 4)
+    off += n * D * 16
+    meta = out[off: off + n * D * 16].view(np.uint32).reshape(n, D, 4)
+    off += n * D * 16
+    pay = out[off:].view(np.uint32).reshape(n, 3)
+
+    assert states.is_empty() and not any(states.errors), states
+    depth = PE.chain_depths(hits)
+    assert nrays == int(depth.sum())
+    assert (depth >= 2).sum() > 0.1 * n and (depth == 1).sum() > 0.05 * n  # some chains go deep, some miss at once
+    if bind == 3:
+        assert (depth == 3).sum() > 0.05 * n
+    assert np.array_equal(pay, PE.chain_payload(hits))
+    # depth 0: the camera rays
+    assert np.array_equal(rays[:, 0].view(np.uint32), PE.camera_rays(w, h, 0.0, 100.0, PE.CHAIN_ORIGIN).view(np.uint32))
+    for d in range(D):
+        live = depth > d
+        ref, _, _ = orc.trace(rays[live, d])
+        diff = compare(hits[live, d], ref)
+        assert diff.ok() and diff.value_mismatch_bits == 0, (d, diff)
+        if d + 1 < D:
+            nxt = live & (hits[:, d, 0] != PE.MISS)
+            attr, _ = orc.hit_attributes(rays[nxt, d], hits[nxt, d], meta[nxt, d])
+            expect = PE.mirror_rays(rays[nxt, d], attr)
+            assert np.array_equal(rays[nxt, d + 1].view(np.uint32), expect.view(np.uint32)), d
