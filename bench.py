@@ -134,7 +134,8 @@ def _call_kernels(tune) -> str:
 
 
 def _tune_arg(tune) -> str:
-    return ",".join(str(tune[k]) for k in ("block_size", "lds_entries", "waves", "persistent", "cull", "wide", "coop"))
+    return ",".join(str(tune.get(k, 0)) for k in ("block_size", "lds_entries", "waves", "persistent", "cull", "wide",
+                                                   "coop", "tail"))
 
 
 def _record(value, unit, args, ms_per_step, config, roofline, extra=None):
@@ -1143,7 +1144,7 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic passes")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--tune", default="",
-                    help="launch shape block,lds,waves,persistent[,cull[,wide[,coop]]] (no autotune)")
+                    help="launch shape block,lds,waves,persistent[,cull[,wide[,coop[,tail]]]] (no autotune)")
     ap.add_argument("--profile-only", action="store_true", help="run the timed launches only (counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:

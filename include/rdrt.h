@@ -486,6 +486,10 @@ typedef struct rdrt_tune {
   uint32_t coop;        /* launches with at most this many live rays (device-side count) are traced by the
                            cooperative kernel, 8 lanes per ray: the TLAS walk in lockstep, the instances it
                            accepts dealt to the lanes; 0 = never.  Fast layouts only, not with visit counting */
+  uint32_t tail;        /* path-tracer frames: from bounce `tail` (1 .. max_bounces - 1) on, one launch finishes
+                           every path, 8 lanes per path running its remaining bounces back to back (the frame
+                           waits for its slowest path instead of the slowest ray of every bounce); 0 = one set
+                           of launches per bounce.  Not with visit counting */
 } rdrt_tune;
 /* Times every shape of the search space (3 block sizes x 2 LDS stack depths
  * x 2 occupancy targets x grid / persistent x with / without the cull pass
@@ -586,12 +590,13 @@ int32_t rdrt_pt_forward(rdrt_ctx *ctx, const rdrt_pt_params *params, float *radi
 /* Ray counts of the last frame (synchronizes). */
 int32_t rdrt_pt_get_stats(rdrt_ctx *ctx, rdrt_pt_stats *out);
 /* Times every launch shape of the fast traversal (3 block sizes x 2 LDS
- * stack depths x 2 occupancy targets x grid / persistent x the three node
- * layouts) on eager frames of these parameters (record and
- * diagnostics off), `reps` timed frames each, and keeps the fastest for the
- * context, as rdrt_autotune does for single launches; best->ms is the frame
- * time.  radiance_dev is overwritten.  Synchronizes.  best may be NULL.
- * (No counterpart in the reference.) */
+ * stack depths x 3 occupancy targets x grid / persistent x the three node
+ * layouts x 3 cooperative thresholds) on eager frames of these parameters
+ * (record and diagnostics off), `reps` timed frames each, then the tail
+ * kernel from bounce 1, 2 or 3 with the fastest shape, and keeps the
+ * fastest for the context, as rdrt_autotune does for single launches;
+ * best->ms is the frame time.  radiance_dev is overwritten.  Synchronizes.
+ * best may be NULL.  (No counterpart in the reference.) */
 int32_t rdrt_pt_autotune(rdrt_ctx *ctx, const rdrt_pt_params *params, float *radiance_dev, uint32_t reps,
                          rdrt_tune *best);
 

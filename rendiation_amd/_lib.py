@@ -128,7 +128,7 @@ class RdrtBatchStats(C.Structure):
 
 class RdrtTune(C.Structure):
     _fields_ = [("block_size", u32), ("lds_entries", u32), ("waves", u32), ("persistent", u32), ("cull", u32),
-                ("wide", u32), ("ms", f32), ("coop", u32)]
+                ("wide", u32), ("ms", f32), ("coop", u32), ("tail", u32)]
 
 
 RDRT_BATCH_PERSISTENT = 0

@@ -460,11 +460,12 @@ class AccelerationStructureSystem:
         t = L.RdrtTune()
         L.check(self._L.rdrt_get_tune(self._h, C.byref(t)))
         return {"block_size": t.block_size, "lds_entries": t.lds_entries, "waves": t.waves,
-                "persistent": t.persistent, "cull": t.cull, "wide": t.wide, "ms": float(t.ms), "coop": t.coop}
+                "persistent": t.persistent, "cull": t.cull, "wide": t.wide, "ms": float(t.ms), "coop": t.coop,
+                "tail": t.tail}
 
     def set_tune(self, block_size: int = 256, lds_entries: int = 16, waves: int = 0, persistent: int = 0,
-                 cull: int = 0, wide: int = 0, coop: int = 0) -> None:
-        t = L.RdrtTune(block_size, lds_entries, waves, persistent, cull, wide, 0.0, coop)
+                 cull: int = 0, wide: int = 0, coop: int = 0, tail: int = 0) -> None:
+        t = L.RdrtTune(block_size, lds_entries, waves, persistent, cull, wide, 0.0, coop, tail)
         L.check(self._L.rdrt_set_tune(self._h, C.byref(t)))
 
     def sync(self) -> None:

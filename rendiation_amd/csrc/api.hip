@@ -698,10 +698,10 @@ int32_t rdrt_set_tune(rdrt_ctx *c, const rdrt_tune *t) {
   if ((t->block_size != 64 && t->block_size != 128 && t->block_size != 256) ||
       (t->lds_entries != 8 && t->lds_entries != 16) || (t->waves != 0 && t->waves != 6 && t->waves != 8) ||
       t->persistent > 1 ||
-      t->cull > 1 || t->wide > 2 || t->coop > (1u << 22))
+      t->cull > 1 || t->wide > 2 || t->coop > (1u << 22) || t->tail > kPtMaxBounces)
     return fail(RDRT_ERR_INVALID_ARG,
                 "block_size 64/128/256, lds_entries 8/16, waves 0/6/8, persistent 0/1, cull 0/1, wide 0/1/2, "
-                "coop <= 2^22");
+                "coop <= 2^22, tail <= 16");
   TraceTune n;
   n.block = t->block_size;
   n.lds = t->lds_entries;
@@ -710,6 +710,7 @@ int32_t rdrt_set_tune(rdrt_ctx *c, const rdrt_tune *t) {
   n.cull = t->cull;
   n.wide = t->wide;
   n.coop = t->coop;
+  n.tail = t->tail;
   if (c->built_valid && stack_entries_for(push_depth(c, n)) == 0)
     return fail(RDRT_ERR_CAPACITY, "BVH too deep for the fast traversal stack of this layout");
   c->tune = n;
@@ -727,6 +728,7 @@ int32_t rdrt_get_tune(rdrt_ctx *c, rdrt_tune *t) {
   t->wide = c->tune.wide;
   t->ms = c->tune_ms;
   t->coop = c->tune.coop;
+  t->tail = c->tune.tail;
   return RDRT_OK;
 }
 

@@ -14,7 +14,7 @@ namespace rdrt {
 
 void PtState::free_all() {
   for (Scratch *s : {&rays, &shadow_rays, &hits, &shadow_hits, &hitx, &beta, &L, &nee, &qa, &qb, &qs, &counts,
-                     &tmp_a, &tmp_b, &blk, &materials, &diag_counters, &diag_totals, &record, &views, &work, &det_acc,
+                     &tmp_a, &tmp_b, &blk, &materials, &diag_counters, &diag_totals, &record, &views, &work, &tflags, &det_acc,
                      &det_flags, &det_hdr, &grec, &loss_part})
     s->free();
   host_views.clear();
@@ -245,7 +245,7 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
   L.blk = (uint32_t *)s.blk.ptr;
   L.blk_stride = (npaths + kPtBlock - 1) / kPtBlock;
   L.status = c->status;
-  RDRT_HIP(s.work.reserve(4));
+  RDRT_HIP(s.work.reserve(8));
   L.work = (uint32_t *)s.work.ptr;
   L.out = radiance;
   if (prm->record) {
@@ -284,9 +284,15 @@ int32_t rdrt_pt_render(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance) 
     }
   }
   s.last_diag = prm->diagnostics;
+  // the tail kernel (not with visit counting: one ray per lane there)
+  if (c->tune.tail && c->tune.tail < prm->max_bounces && !counting) {
+    RDRT_HIP(s.tflags.reserve((size_t)npaths * 4));
+    L.tail_b0 = c->tune.tail;
+    L.tflags = (uint32_t *)s.tflags.ptr;
+  }
   // shadow rays of bounce b share a launch with extension rays of b + 1,
-  // except in visit-counting frames
-  s.last_launches = counting ? 2 * prm->max_bounces + 1 : prm->max_bounces + 1;
+  // except in visit-counting frames; the tail kernel is one launch
+  s.last_launches = counting ? 2 * prm->max_bounces + 1 : L.tail_b0 ? L.tail_b0 + 2 : prm->max_bounces + 1;
   RDRT_HIP(hipEventRecord(c->ev0, st));
   if (prm->diagnostics) {
     RDRT_HIP(enqueue_pt_frame(L, tp, push_depth(c, c->tune), c->tune, st, &diag));
@@ -346,6 +352,24 @@ int32_t rdrt_pt_autotune(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance
   TraceTune chosen = before;
   float chosen_ms = -1.0f;
   rc = RDRT_OK;
+  // average frame time of shape t, kept when it is the fastest so far
+  auto time_shape = [&](const TraceTune &t) -> int32_t {
+    c->tune = t;
+    int32_t r2 = rdrt_pt_render(c, &q, radiance);  // warm-up
+    if (r2 == RDRT_OK && hipEventRecord(e0, c->work_stream) != hipSuccess) r2 = fail(RDRT_ERR_HIP, "event");
+    for (uint32_t r = 0; r < reps && r2 == RDRT_OK; r++) r2 = rdrt_pt_render(c, &q, radiance);
+    if (r2 != RDRT_OK) return r2;
+    float ms = 0.0f;
+    if (hipEventRecord(e1, c->work_stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      return fail(RDRT_ERR_HIP, "frame timing failed");
+    ms /= (float)reps;
+    if (chosen_ms < 0.0f || ms < chosen_ms) {
+      chosen_ms = ms;
+      chosen = t;
+    }
+    return RDRT_OK;
+  };
   for (uint32_t block : {64u, 128u, 256u})
     for (uint32_t lds : {8u, 16u})
       for (uint32_t waves : {0u, 6u, 8u})
@@ -358,23 +382,15 @@ int32_t rdrt_pt_autotune(rdrt_ctx *c, const rdrt_pt_params *prm, float *radiance
           t.wide = (shape >> 1) % 3u;
           t.coop = kCoopChoices[shape / 6u];
           if (stack_entries_for(push_depth(c, t)) == 0) continue;
-          c->tune = t;
-          rc = rdrt_pt_render(c, &q, radiance);  // warm-up
-          if (rc == RDRT_OK && hipEventRecord(e0, c->work_stream) != hipSuccess) rc = fail(RDRT_ERR_HIP, "event");
-          for (uint32_t r = 0; r < reps && rc == RDRT_OK; r++) rc = rdrt_pt_render(c, &q, radiance);
-          if (rc != RDRT_OK) break;
-          float ms = 0.0f;
-          if (hipEventRecord(e1, c->work_stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-              hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-            rc = fail(RDRT_ERR_HIP, "frame timing failed");
-            break;
-          }
-          ms /= (float)reps;
-          if (chosen_ms < 0.0f || ms < chosen_ms) {
-            chosen_ms = ms;
-            chosen = t;
-          }
+          rc = time_shape(t);
         }
+  // then where the tail kernel takes over, with the fastest shape
+  const TraceTune base = chosen;
+  for (uint32_t tail = 1; tail < prm->max_bounces && tail <= 3 && rc == RDRT_OK; tail++) {
+    TraceTune t = base;
+    t.tail = tail;
+    rc = time_shape(t);
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != RDRT_OK) {

@@ -86,7 +86,7 @@ struct PtState {
   uint32_t capacity = 0;  // paths
   uint32_t pixels = 0;
   Scratch rays, shadow_rays, hits, shadow_hits, hitx, beta, L, nee, qa, qb, qs, counts, tmp_a, tmp_b, blk;
-  Scratch materials, record, views, work;
+  Scratch materials, record, views, work, tflags;
   // adjoint scratch: reproducible sums (int64 sums, flags, header), the
   // replay's per-bounce gradient terms, the loss partials
   Scratch det_acc, det_flags, det_hdr, grec, loss_part;

@@ -46,7 +46,12 @@ struct PtLaunch {
   uint32_t *tmp_a, *tmp_b, *blk;
   uint32_t blk_stride;
   uint32_t *status;
-  uint32_t *work;  // work counter of persistent traversal launches
+  uint32_t *work;  // [0]: work counter of persistent traversal launches; [1]: of the tail kernel
+  // tail: from bounce tail_b0 (>= 1) on, the tail kernel finishes every path
+  // (0 = per-bounce launches throughout); tflags[path]: what shade(tail_b0)
+  // left the path (bit 0: a shadow ray, bit 1: an extension ray)
+  uint32_t tail_b0;
+  uint32_t *tflags;
   float *out;  // width * height * 3
 
   // path vertices for the adjoint, or null: slot k of path p at
@@ -83,6 +88,18 @@ struct PtDiag {
   uint32_t *max_visits = nullptr;                     // per launch: largest visit sum of one ray
   bool fast_order = false;                            // count the fast traversal's visits
 };
+
+// The tail kernel of a frame (pt_tail_<layout>.hip): every path of the
+// queue shade(L.tail_b0) read, from its shadow ray of that bounce to its
+// end.  pe / ps: the extension / shadow traversal parameters (no queues).
+hipError_t launch_pt_tail(const PtLaunch &L, const TraceParams &pe, const TraceParams &ps, uint32_t push_depth,
+                          const TraceTune &tune, hipStream_t stream);
+hipError_t launch_pt_tail_0(const PtLaunch &L, const TraceParams &pe, const TraceParams &ps, uint32_t stack,
+                            const TraceTune &tune, hipStream_t stream);
+hipError_t launch_pt_tail_1(const PtLaunch &L, const TraceParams &pe, const TraceParams &ps, uint32_t stack,
+                            const TraceTune &tune, hipStream_t stream);
+hipError_t launch_pt_tail_2(const PtLaunch &L, const TraceParams &pe, const TraceParams &ps, uint32_t stack,
+                            const TraceTune &tune, hipStream_t stream);
 
 // Enqueues one frame on `stream` (capturable when diag is null: no memset
 // nodes, no host round trips).

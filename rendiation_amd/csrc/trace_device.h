@@ -489,8 +489,13 @@ __device__ __forceinline__ bool group_any(bool b) {
   return v != 0;
 }
 
+// Returns (group-uniform) the lane of the group that wrote the record, plus
+// kCoopHit when the ray hit.
+constexpr uint32_t kCoopHit = 0x100u;
+
 template <int K, int S, int B, int LAYOUT>
-__device__ __forceinline__ void trace_coop_one(const TraceParams &p, uint32_t rid, uint32_t *stk, uint32_t glane) {
+__device__ __forceinline__ uint32_t trace_coop_one(const TraceParams &p, uint32_t rid, uint32_t *stk,
+                                                   uint32_t glane) {
   uint32_t spill[S > K ? S - K : 1];
   auto push = [&](uint32_t &sp, uint32_t v) {
     if (sp < (uint32_t)K) stk[sp * B] = v;
@@ -632,6 +637,7 @@ __device__ __forceinline__ void trace_coop_one(const TraceParams &p, uint32_t ri
     best.t = far_;
     write_result(p, p.tri_ids, rid, best, 0u, 0u, 0u);
   }
+  return win | (gm ? kCoopHit : 0u);
 }
 
 template <int K, int S, int B, int W, int LAYOUT>
@@ -711,7 +717,7 @@ __global__ __launch_bounds__(B, W) void k_trace_persist(TraceParams p) {
   }
 }
 
-__global__ void k_zero_work(uint32_t *w) { *w = 0u; }
+[[maybe_unused]] __global__ void k_zero_work(uint32_t *w) { *w = 0u; }
 
 // One fast-traversal launch with block size B, total stack S and tuning t.
 // K = stack entries per lane in LDS, W = minimum waves per SIMD asked of the

@@ -88,6 +88,8 @@ struct TraceTune {
                          //     2 = quantized child quads (64-byte nodes)
   uint32_t coop = 0;     // launches with at most this many live rays use the cooperative kernel
                          //     (8 lanes per ray); 0 = never
+  uint32_t tail = 0;     // path-tracer frames: from this bounce on one tail launch finishes every path
+                         //     (8 lanes per path); 0 = per-bounce launches throughout
 };
 
 // Cooperative-kernel thresholds the frame and AO autotuners try.

@@ -283,6 +283,58 @@ def test_shared_launches_equal_separate_launches(device):
                                                  | _lib.RDRT_PT_DIAG_COUNT_VISITS))
 
 
+@pytest.mark.parametrize("tail,wide,block", [(1, 1, 256), (2, 0, 64), (3, 2, 128), (1, 2, 64)])
+def test_tail_kernel_equals_per_bounce_launches(tail, wide, block, device):
+    """The tail kernel (every path finished by one launch from bounce
+    `tail` on, 8 lanes per path) renders the frame of the per-bounce
+    launches: the same image and the same recorded path vertices bit for
+    bit, the same extension and shadow ray counts per bounce, and
+    tail + 2 traversal launches."""
+    import dataclasses
+
+    from rendiation_amd import _lib
+
+    scene = synth.pt_scene(seed=7, n_meshes=32, tess=40, extent=6.0)
+    cam = Camera(position=(0.0, 0.0, 20.0))
+    _say(f"tail kernel from bounce {tail}, layout {wide}, block {block}")
+    s = _system(scene)
+    pt = PathTracer(s)
+    cfg = PathTracingConfig(width=160, height=96, spp=2, max_bounces=4, rr_start=2, seed=3, use_graph=False,
+                            diagnostics=_lib.RDRT_PT_DIAG_TIME_TRACES)
+    # recorded frames (the adjoint's input) have no Russian roulette
+    rec_cfg = dataclasses.replace(cfg, rr_start=99, record=True)
+    import torch
+
+    nm = len(scene["albedo"])
+    adj = torch.from_numpy(synth.uniform_f32(11, 1, 96 * 160 * 3) * np.float32(2.0) - np.float32(1.0)).cuda()
+
+    def frames():
+        img = pt.render(cam, cfg).cpu().numpy()
+        st = pt.stats()
+        rec_img = pt.render(cam, rec_cfg).cpu().numpy()
+        ga = torch.zeros((nm, 3), dtype=torch.float32, device="cuda:0")
+        ge = torch.zeros_like(ga)
+        pt.backward(adj, ga, ge)  # reads the recorded path vertices
+        s.sync()
+        return img, st, rec_img, ga.cpu().numpy(), ge.cpu().numpy()
+
+    s.set_tune(block_size=block, wide=wide, persistent=1, coop=4096)
+    base, st_base, rec0, ga0, ge0 = frames()
+    s.set_tune(block_size=block, wide=wide, persistent=1, coop=4096, tail=tail)
+    img, st, rec, ga, ge = frames()
+    assert st["trace_launches"] == tail + 2
+    assert st["extension_rays"] == st_base["extension_rays"]
+    assert st["shadow_rays"] == st_base["shadow_rays"]
+    assert np.array_equal(img, base)
+    assert np.array_equal(rec, rec0)
+    assert np.array_equal(ga, ga0) and np.array_equal(ge, ge0)
+    # graph-replayed frames with the tail equal the eager one
+    g = dataclasses.replace(cfg, use_graph=True, diagnostics=0)
+    for _ in range(2):
+        assert np.array_equal(pt.render(cam, g).cpu().numpy(), base)
+    s.set_tune()
+
+
 def test_frame_autotune(device):
     """The frame autotuner keeps one shape of the search space; frames
     afterwards equal an eager frame rendered with that shape set by hand."""
@@ -299,7 +351,7 @@ def test_frame_autotune(device):
     assert choice["block_size"] in (64, 128, 256) and choice["wide"] in (0, 1, 2) and choice["ms"] > 0
     img = pt.render(cam, cfg).cpu().numpy()
     s.set_tune(choice["block_size"], choice["lds_entries"], choice["waves"], choice["persistent"], choice["cull"],
-               choice["wide"], choice["coop"])
+               choice["wide"], choice["coop"], choice["tail"])
     eager = pt.render(cam, dataclasses.replace(cfg, use_graph=False)).cpu().numpy()
     assert np.array_equal(img, eager)
     with pytest.raises(ValueError):
