@@ -97,7 +97,8 @@ def build_pair(blas_list, tlas_list, binding, opts):
         th.append(sys.create_top_level_acceleration_structure([
             TopLevelAccelerationStructureSourceInstance(
                 d["transform"], bh[d["blas"]], instance_custom_index=d.get("custom", 0), mask=d.get("mask", 0xFFFFFFFF),
-                flags=d.get("flags", 0)) for d in insts]))
+                instance_shader_binding_table_record_offset=d.get("sbt", 0), flags=d.get("flags", 0))
+            for d in insts]))
         orc.add_tlas(insts)
     sys.bind_tlas([th[i] for i in binding])
     orc.bind(binding)

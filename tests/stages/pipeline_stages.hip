@@ -198,12 +198,79 @@ __global__ void chain_miss(rdrt_stage_args a) {
   p[0] += 0x10000u;
 }
 
+// ---- sbt ----
+// Hit-group and miss dispatch.  The ray gen traces the chain's camera ray
+// with sbt offset id % 2, stride (id / 2) % 2 and miss index (id / 4) % 3,
+// payload 0xFFFF, and stores the returned payload; closest-hit stage K sets
+// the payload to K, miss stage K to 100 + K, and both record the ray, hit,
+// hit meta and (closest hit) hit context of their pixel.  User buffer: n
+// payload words, n rays, n hits, n hit metas, n hit contexts.
+
+struct SbtOut {
+  uint32_t *pay;
+  rdrt_ray *rays;
+  rdrt_hit *hits;
+  rdrt_hit_meta *meta;
+  rdrt_hit_transform *xf;
+};
+
+__device__ __forceinline__ SbtOut sbt_out(const rdrt_stage_args &a) {
+  const size_t n = (size_t)a.launch_size[0] * a.launch_size[1];
+  SbtOut o;
+  o.pay = reinterpret_cast<uint32_t *>(a.user);
+  o.rays = reinterpret_cast<rdrt_ray *>(o.pay + n);
+  o.hits = reinterpret_cast<rdrt_hit *>(o.rays + n);
+  o.meta = reinterpret_cast<rdrt_hit_meta *>(o.hits + n);
+  o.xf = reinterpret_cast<rdrt_hit_transform *>(o.meta + n);
+  return o;
+}
+
+__global__ void sbt_ray_gen(rdrt_stage_args a) {
+  uint32_t s;
+  if (!polled(a, s)) return;
+  const uint32_t id = a.launch[s];
+  if (a.polls[s] == 0) {
+    const float o[3] = {0.0137f, -0.0219f, 0.0f};
+    float d[3];
+    camera_dir(id, a, d);
+    set_call(a.call[s], o, d, 0.0f, 100.0f, 0u);
+    a.call[s].sbt_offset = id % 2u;
+    a.call[s].sbt_stride = (id / 2u) % 2u;
+    a.call[s].miss_index = (id / 4u) % 3u;
+    a.call_payload[(size_t)s * a.payload_words] = 0xFFFFu;
+    a.action[s] = RDRT_STAGE_TRACE;
+  } else {
+    sbt_out(a).pay[id] = a.trace_result[(size_t)s * a.payload_words];
+  }
+}
+
+template <uint32_t K, bool HIT>
+__global__ void sbt_stage(rdrt_stage_args a) {
+  uint32_t s;
+  if (!polled(a, s)) return;
+  const uint32_t id = a.launch[s], tr = a.trace_of[s];
+  const SbtOut o = sbt_out(a);
+  o.rays[id] = a.rays[tr];
+  o.hits[id] = a.hits[tr];
+  o.meta[id] = a.hit_meta[tr];
+  if (HIT) o.xf[id] = a.transforms[tr];
+  a.payload[(size_t)s * a.payload_words] = HIT ? K : 100u + K;
+}
+
 }  // namespace
 
 // Host-side handle of stage k: 0-2 debug ray gen / closest hit / miss,
-// 3-5 chain ray gen / closest hit / miss; null past the end.
+// 3-5 chain ray gen / closest hit / miss, 6 sbt ray gen, 7-10 sbt closest
+// hit 0-3, 11-12 sbt miss 0-1; null past the end.
 extern "C" const void *test_stage(uint32_t k) {
   switch (k) {
+    case 6: return reinterpret_cast<const void *>(&sbt_ray_gen);
+    case 7: return reinterpret_cast<const void *>(&sbt_stage<0, true>);
+    case 8: return reinterpret_cast<const void *>(&sbt_stage<1, true>);
+    case 9: return reinterpret_cast<const void *>(&sbt_stage<2, true>);
+    case 10: return reinterpret_cast<const void *>(&sbt_stage<3, true>);
+    case 11: return reinterpret_cast<const void *>(&sbt_stage<0, false>);
+    case 12: return reinterpret_cast<const void *>(&sbt_stage<1, false>);
     case 0: return reinterpret_cast<const void *>(&debug_ray_gen);
     case 1: return reinterpret_cast<const void *>(&debug_closest_hit);
     case 2: return reinterpret_cast<const void *>(&debug_miss);

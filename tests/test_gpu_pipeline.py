@@ -25,6 +25,7 @@ import pytest
 from rendiation_amd.acceleration import BuildOptions
 from rendiation_amd.rt_pipeline import (
     HIT_ATTRIBUTES,
+    HIT_TRANSFORMS,
     GPURaytracingPipelineAndBindingSource,
     RayTracingPipeline,
     ShaderBindingTable,
@@ -50,7 +51,7 @@ def stages(device):
     lib = C.CDLL(str(STAGES))
     lib.test_stage.restype = C.c_void_p
     lib.test_stage.argtypes = [C.c_uint32]
-    return [lib.test_stage(k) for k in range(6)]
+    return [lib.test_stage(k) for k in range(13)]
 
 
 def _pipeline(system, stages, base, payload_words=1, hit_context=0, **kw):
@@ -180,3 +181,90 @@ def test_chain_pipeline(stages, device, bind):
             attr, _ = orc.hit_attributes(rays[nxt, d], hits[nxt, d], meta[nxt, d])
             expect = PE.mirror_rays(rays[nxt, d], attr)
             assert np.array_equal(rays[nxt, d + 1].view(np.uint32), expect.view(np.uint32)), d
+
+
+def test_hit_group_and_miss_dispatch(stages, device):
+    """Rays with per-pixel sbt offset / stride / miss index on a scene with a
+    two-geometry BLAS and instance hit-group offsets: each ray runs the
+    closest-hit stage of hit group offset + stride * geometry + instance
+    offset (ctx.rs:53-55; sbt.rs:18-39) or the miss stage of its miss index,
+    and a group or index without a stage returns the payload as traced.  The
+    hit context handed to the closest-hit stages equals the device fetch of
+    rdrt_fetch_hit_transforms for the same hits."""
+    import torch
+
+    from tests.scenes import affine, build_pair
+    from rendiation_amd import synth
+
+    sph_p, sph_i = synth.uv_sphere(24, 12)
+    tor_p, tor_i = synth.torus(32, 8)
+    blas_list = [[(sph_p, sph_i, 1), (tor_p * np.float32(1.5), tor_i, 1)], [(sph_p * np.float32(0.7), sph_i, 1)]]
+    insts = []
+    for k in range(9):
+        x, y = (k % 3 - 1) * 2.5, (k // 3 - 1) * 2.5
+        insts.append({"transform": affine(None, (1.0, 1.0, 1.0), (x, y, -8.0 - 0.3 * k)), "blas": k % 2,
+                      "sbt": [0, 1, 2][k % 3]})
+    system, orc = build_pair(blas_list, [insts], [0], BuildOptions())
+    w, h = 160, 120
+    n = w * h
+    _say(f"hit-group dispatch {w}x{h}")
+    buf = torch.zeros((n * (4 + 32 + 16 + 16 + 160),), dtype=torch.uint8, device="cuda:0")
+    torch.cuda.synchronize()
+    src = GPURaytracingPipelineAndBindingSource(payload_words=1, hit_context=HIT_TRANSFORMS)
+    rg = src.register_ray_gen(stages[6])
+    chs = [src.register_ray_closest_hit(stages[7 + k]) for k in range(4)]
+    mss = [src.register_ray_miss(stages[11 + k]) for k in range(2)]
+    sbt = ShaderBindingTable(ray_type_count=1)
+    sbt.config_ray_generation(rg)
+    for k in range(4):
+        sbt.config_hit_group(k, 0, 0, chs[k])  # hit group index k (stride 1 per geometry)
+    for k in range(2):
+        sbt.config_missing(k, mss[k])
+    p = RayTracingPipeline(system, src, sbt)
+    p.trace_ray(w, h, buf.data_ptr())
+    system.sync()
+    out = buf.cpu().numpy()
+    states = p.read_back_execution_states()
+    p.close()
+    assert states.is_empty() and not any(states.errors), states
+    o = 0
+    pay = out[o:o + 4 * n].view(np.uint32); o += 4 * n
+    rays = out[o:o + 32 * n].view(np.float32).reshape(n, 8); o += 32 * n
+    hits = out[o:o + 16 * n].view(np.uint32).reshape(n, 4); o += 16 * n
+    meta = out[o:o + 16 * n].view(np.uint32).reshape(n, 4); o += 16 * n
+    xf = out[o:o + 160 * n].view(np.uint32).reshape(n, 40)
+    cam = PE.camera_rays(w, h, 0.0, 100.0, PE.CHAIN_ORIGIN)
+    ids = np.arange(n)
+    inst_sbt = np.array([d["sbt"] for d in insts], dtype=np.int64)
+
+    def group_of(m):  # hit group of a hit (geometry, instance) for pixel ids
+        return (ids % 2) + ((ids // 2) % 2) * m[:, 0].astype(np.int64) + inst_sbt[m[:, 1] % len(insts)]
+
+    miss_index = (ids // 4) % 3
+    ran = pay != 0xFFFF
+    hit = hits[:, 0] != PE.MISS
+    # a stage ran: it recorded exactly the camera ray, and it was the stage of its hit group / miss index
+    assert np.array_equal(rays[ran].view(np.uint32), cam[ran].view(np.uint32))
+    g = group_of(meta)
+    assert np.array_equal(pay[ran & hit], g[ran & hit])
+    assert np.array_equal(pay[ran & ~hit], 100 + miss_index[ran & ~hit])
+    # the hits the stages saw are the oracle's (closest-hit contract)
+    ref, ref_meta, _ = orc.trace(cam, with_meta=True)
+    d = compare(hits[ran], ref[ran])
+    assert d.ok() and d.value_mismatch_bits == 0, d
+    # no stage ran exactly where the oracle's hit has no stage (near ties aside)
+    ref_hit = ref[:, 0] != PE.MISS
+    ref_has_stage = np.where(ref_hit, group_of(ref_meta) < 4, miss_index < 2)
+    assert (ran == ref_has_stage).mean() >= 1.0 - 1e-3
+    for k in list(range(4)) + [100, 101]:
+        assert (pay == k).sum() > 50, k  # every stage ran
+    assert (~ran & ref_hit).sum() > 50 and (~ran & ~ref_hit).sum() > 50  # and both no-stage cases occur
+    # the hit context the closest-hit stages saw is the device fetch's
+    hk = ran & hit
+    rd = torch.from_numpy(np.ascontiguousarray(cam[hk])).cuda()
+    hd = torch.from_numpy(np.ascontiguousarray(hits[hk]).view(np.int32)).cuda()
+    md = torch.from_numpy(np.ascontiguousarray(meta[hk]).view(np.int32)).cuda()
+    fx = torch.zeros((int(hk.sum()), 40), dtype=torch.int32, device="cuda:0")
+    system.hit_transforms(rd, hd, md, fx)
+    system.sync()
+    assert np.array_equal(xf[hk], fx.cpu().numpy().view(np.uint32))
