@@ -826,29 +826,98 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
     });
   }
   auto inner = [](uint32_t r) { return r != kInvalid && !(r & kLeafBit); };
+  // Every geometry's slice of the output arrays, in reference order (one
+  // serial pass over the geometries), then the copies and relocations of all
+  // geometries in parallel.
+  struct Slice {
+    uint32_t blas, geo;
+    size_t f, p, q, t, v;  // first forest node, pair, quad, triangle, vertex
+  };
+  std::vector<Slice> slices(jobs.size());
   {
     size_t nf = 0, np = 0, nq = 0, nt = 0, nv = 0;
     size_t ji = 0;
     for (uint32_t b = 0; b < src.blas.size(); b++)
       if (src.blas_live[b])
         for (uint32_t gi = 0; gi < src.blas[b].size(); gi++, ji++) {
+          if (!local[ji].ok) {
+            err = local[ji].err;
+            return RDRT_ERR_CAPACITY;
+          }
+          slices[ji] = Slice{b, gi, nf, np, nq, nt, nv};
           nf += local[ji].forest.size();
           np += local[ji].pairs.size();
           nq += local[ji].quads.size();
           nt += jobs[ji].tri_idx.size() / 3;
           nv += src.blas[b][gi].pos.size() / 3;
         }
-    out.tri_forest.reserve(nf);
-    out.tri_pairs.reserve(np);
-    out.tri_quads.reserve(nq);
-    out.redirect.reserve(nt);
-    out.tri_ids.reserve(nt);
-    out.indices.reserve(3 * nt);
-    out.vertices.reserve(3 * nv);
-    if (any_normals) out.normals.reserve(3 * nv);
-    if (any_uvs) out.uvs.reserve(2 * nv);
-    if (any_material) out.tri_material.reserve(nt);
+    if (nt > 0xFFFFFFFFull / 3 || nv > 0xFFFFFFFFull || nf >= kInvalid || np >= kLeafBit || nq >= kLeafBit) {
+      err = "scene too large for 32-bit node, triangle or vertex indices";
+      return RDRT_ERR_CAPACITY;
+    }
+    out.tri_forest.resize(nf);
+    out.tri_pairs.resize(np);
+    out.tri_quads.resize(nq);
+    out.redirect.resize(nt);
+    out.tri_ids.resize(nt);
+    out.indices.resize(3 * nt);
+    out.vertices.resize(3 * nv);
+    if (any_normals) out.normals.resize(3 * nv);
+    if (any_uvs) out.uvs.resize(2 * nv);
+    if (any_material) out.tri_material.resize(nt);
   }
+  std::vector<uint32_t> geo_max_mat(jobs.size(), 0);
+  parallel_for(jobs.size(), hw, [&](size_t ji) {
+    const Job &job = jobs[ji];
+    const Slice &sl = slices[ji];
+    const GeometrySrc &g = src.blas[sl.blas][sl.geo];
+    const size_t nv = g.pos.size() / 3, nt = job.tri_idx.size() / 3;
+    const uint32_t primitive_start = (uint32_t)sl.t, vertex_start = (uint32_t)sl.v;
+    std::copy(g.pos.begin(), g.pos.begin() + 3 * nv, out.vertices.begin() + 3 * sl.v);
+    for (size_t k = 0; k < job.tree.order.size(); k++) out.redirect[sl.t + k] = primitive_start + job.tree.order[k];
+    for (size_t k = 0; k < job.fast.order.size(); k++) out.tri_ids[sl.t + k] = primitive_start + job.fast.order[k];
+    for (size_t k = 0; k < job.tri_idx.size(); k++) out.indices[3 * sl.t + k] = vertex_start + job.tri_idx[k];
+    if (any_normals) {
+      if (g.normals.size() == 3 * nv) std::copy(g.normals.begin(), g.normals.end(), out.normals.begin() + 3 * sl.v);
+      else std::fill(out.normals.begin() + 3 * sl.v, out.normals.begin() + 3 * (sl.v + nv), 0.0f);  // geometric normal
+    }
+    if (any_uvs) {
+      if (g.uvs.size() == 2 * nv) std::copy(g.uvs.begin(), g.uvs.end(), out.uvs.begin() + 2 * sl.v);
+      else std::fill(out.uvs.begin() + 2 * sl.v, out.uvs.begin() + 2 * (sl.v + nv), 0.0f);
+    }
+    if (any_material) {
+      uint32_t mx = 0;
+      for (size_t t = 0; t < nt; t++) {
+        const uint32_t m = t < g.tri_material.size() ? g.tri_material[t] : 0u;
+        out.tri_material[sl.t + t] = m;
+        mx = std::max(mx, m);
+      }
+      geo_max_mat[ji] = mx;
+    }
+    Local &l = local[ji];
+    const uint32_t fb = (uint32_t)sl.f, pb = (uint32_t)sl.p, qb = (uint32_t)sl.q;
+    for (size_t k = 0; k < l.forest.size(); k++) {
+      RefNode n = l.forest[k];
+      if (n.hit_next != kInvalid) n.hit_next += fb;
+      if (n.miss_next != kInvalid) n.miss_next += fb;
+      out.tri_forest[sl.f + k] = n;
+    }
+    for (size_t k = 0; k < l.pairs.size(); k++) {
+      PairNode pn = l.pairs[k];
+      if (inner(pn.lref)) pn.lref += pb;
+      if (inner(pn.rref)) pn.rref += pb;
+      out.tri_pairs[sl.p + k] = pn;
+    }
+    for (size_t k = 0; k < l.quads.size(); k++) {
+      QuadNode qn = l.quads[k];
+      for (int c = 0; c < 4; c++)
+        if (inner(qn.ref[c])) qn.ref[c] += qb;
+      out.tri_quads[sl.q + k] = qn;
+    }
+    std::vector<RefNode>().swap(l.forest);
+    std::vector<PairNode>().swap(l.pairs);
+    std::vector<QuadNode>().swap(l.quads);
+  });
   {
     size_t ji = 0;
     for (uint32_t b = 0; b < src.blas.size(); b++) {
@@ -860,68 +929,20 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
       for (uint32_t gi = 0; gi < src.blas[b].size(); gi++, ji++) {
         const Job &job = jobs[ji];
         const GeometrySrc &g = src.blas[b][gi];
-        uint32_t primitive_start = (uint32_t)(out.indices.size() / 3);
-        uint32_t vertex_start = (uint32_t)(out.vertices.size() / 3);
-        out.vertices.insert(out.vertices.end(), g.pos.begin(), g.pos.end() - (g.pos.size() % 3));
-        for (uint32_t k : job.tree.order) out.redirect.push_back(primitive_start + k);
-        for (uint32_t k : job.fast.order) out.tri_ids.push_back(primitive_start + k);
-        for (uint32_t v : job.tri_idx) out.indices.push_back(vertex_start + v);
-        if (any_normals) {
-          size_t nv = g.pos.size() / 3;
-          if (g.normals.size() == 3 * nv) out.normals.insert(out.normals.end(), g.normals.begin(), g.normals.end());
-          else out.normals.resize(out.normals.size() + 3 * nv, 0.0f);  // zero = use the geometric normal
-        }
-        if (any_uvs) {
-          size_t nv = g.pos.size() / 3;
-          if (g.uvs.size() == 2 * nv) out.uvs.insert(out.uvs.end(), g.uvs.begin(), g.uvs.end());
-          else out.uvs.resize(out.uvs.size() + 2 * nv, 0.0f);
-        }
-        uint32_t max_mat = 0;
-        if (any_material) {
-          size_t nt = job.tri_idx.size() / 3;
-          for (size_t t = 0; t < nt; t++) {
-            uint32_t m = t < g.tri_material.size() ? g.tri_material[t] : 0u;
-            out.tri_material.push_back(m);
-            max_mat = std::max(max_mat, m);
-          }
-        }
-        out.geo_max_material.push_back(max_mat);
+        const Slice &sl = slices[ji];
+        const Local &l = local[ji];
+        out.geo_max_material.push_back(geo_max_mat[ji]);
         GeoRec gr;
         std::memset(&gr, 0, sizeof(gr));
         put_box(gr.mn, gr.mx, job.tree.nodes[0].box);
-        gr.primitive_start = primitive_start;
+        gr.primitive_start = (uint32_t)sl.t;
         gr.geometry_idx = gi;
         gr.flags = g.flags;
-        Local &l = local[ji];
-        if (!l.ok) {
-          err = l.err;
-          return RDRT_ERR_CAPACITY;
-        }
-        const uint32_t fb = (uint32_t)out.tri_forest.size(), pb = (uint32_t)out.tri_pairs.size(),
-                       qb = (uint32_t)out.tri_quads.size();
-        gr.ref_root = fb;
-        for (RefNode n : l.forest) {
-          if (n.hit_next != kInvalid) n.hit_next += fb;
-          if (n.miss_next != kInvalid) n.miss_next += fb;
-          out.tri_forest.push_back(n);
-        }
-        for (PairNode p : l.pairs) {
-          if (inner(p.lref)) p.lref += pb;
-          if (inner(p.rref)) p.rref += pb;
-          out.tri_pairs.push_back(p);
-        }
-        for (QuadNode q : l.quads) {
-          for (int k = 0; k < 4; k++)
-            if (inner(q.ref[k])) q.ref[k] += qb;
-          out.tri_quads.push_back(q);
-        }
-        gr.root_ref = inner(l.root_ref) ? l.root_ref + pb : l.root_ref;
-        gr.quad_root = inner(l.quad_root) ? l.quad_root + qb : l.quad_root;
+        gr.ref_root = (uint32_t)sl.f;
+        gr.root_ref = inner(l.root_ref) ? l.root_ref + (uint32_t)sl.p : l.root_ref;
+        gr.quad_root = inner(l.quad_root) ? l.quad_root + (uint32_t)sl.q : l.quad_root;
         out.blas_stack = std::max(out.blas_stack, l.stack);
         out.blas_stack4 = std::max(out.blas_stack4, l.stack4);
-        std::vector<RefNode>().swap(l.forest);
-        std::vector<PairNode>().swap(l.pairs);
-        std::vector<QuadNode>().swap(l.quads);
         out.max_blas_depth = std::max(out.max_blas_depth, job.tree.depth);
         out.geos.push_back(gr);
         grow_box(blas_box[b], job.tree.nodes[0].box);
