@@ -268,3 +268,35 @@ def test_hit_group_and_miss_dispatch(stages, device):
     system.hit_transforms(rd, hd, md, fx)
     system.sync()
     assert np.array_equal(xf[hk], fx.cpu().numpy().view(np.uint32))
+
+
+def test_trace_pool_exhaustion_reports_and_drains(stages, device):
+    """Nested traces need two trace slots per launch entry at once; with
+    max_in_flight = 1 the trace group runs out of slots.  The spawns that
+    find no slot set the group's error bit and their tasks resolve without
+    tracing, so the call still drains (every task finished) and traces fewer
+    rays than the full chain would."""
+    import torch
+
+    system, _ = reference_default_scene(BuildOptions(), bind=3)
+    w, h, D = 96, 80, PE.DEPTH
+    n = w * h
+    _say("chain pipeline, trace pool of one slot per entry")
+    nbytes = n * D * (32 + 16 + 16) + n * 3 * 4
+    counts = []
+    errors = []
+    for mif in (1, 4):
+        buf = torch.full((nbytes,), 0xFF, dtype=torch.uint8, device="cuda:0")
+        torch.cuda.synchronize()
+        p = _pipeline(system, stages, 3, payload_words=3, hit_context=HIT_ATTRIBUTES, max_in_flight=mif,
+                      execution_round_hint=12)
+        p.trace_ray(w, h, buf.data_ptr())
+        system.sync()
+        st = p.read_back_execution_states()
+        counts.append(p.ray_count())
+        errors.append(st.errors[0])
+        p.close()
+        assert st.is_empty(), (mif, st)
+        assert not any(st.errors[1:]), (mif, st)
+    assert errors[0] & 1 and errors[1] == 0, errors
+    assert counts[0] < counts[1]
