@@ -392,7 +392,8 @@ typedef struct rdrt_pipeline_desc {
   uint32_t payload_words;     /* 1 .. RDRT_PIPELINE_MAX_PAYLOAD */
   uint32_t max_in_flight;     /* trace tasks per launch entry in flight (pipeline.rs:211-257); 0 = 2 */
   uint32_t round_hint;        /* task-graph rounds per tile (execution_round_hint); 0 = 8 */
-  uint32_t tile_pixels;       /* launch entries per tile, whole rows (wavefront_compute/mod.rs:134); 0 = 512*512 */
+  uint32_t tile_pixels;       /* launch entries per tile, whole rows (the reference: 512 * 512,
+                                 wavefront_compute/mod.rs:134); 0 = 2^22 */
   uint32_t hit_context;       /* RDRT_PIPELINE_HIT_*: per-hit data the closest-hit stages read (ctx.rs:24-48) */
 } rdrt_pipeline_desc;
 typedef struct rdrt_pipeline rdrt_pipeline;

@@ -36,6 +36,8 @@ struct rdrt_pipeline {
 
 namespace {
 
+constexpr uint32_t kDefaultTile = 1u << 22;
+
 template <class T>
 T *carve(rdrt_pipeline *p, size_t count, hipError_t &e) {
   p->bufs.emplace_back();
@@ -152,7 +154,11 @@ int32_t rdrt_pipeline_trace_rays(rdrt_pipeline *p, uint32_t width, uint32_t heig
   if ((uint64_t)width * height > (1ull << 28)) return fail(RDRT_ERR_CAPACITY, "more than 2^28 launch entries");
   if (stack_entries_for(push_depth(c, c->tune)) == 0)
     return fail(RDRT_ERR_CAPACITY, "BVH too deep for the fast traversal");
-  const uint32_t tile = p->desc.tile_pixels ? p->desc.tile_pixels : 512u * 512u;
+  // the reference executes 512 x 512 entries at a time (wavefront_compute/
+  // mod.rs:134) to bound its buffers; here the default is 2^22 entries (a
+  // 1080p launch in one tile): a tile's launches are latency-bound, so fewer,
+  // larger tiles run the same work in fewer rounds of launches
+  const uint32_t tile = p->desc.tile_pixels ? p->desc.tile_pixels : kDefaultTile;
   uint32_t rows = std::max<uint32_t>(1u, tile / width);
   rows = std::min(rows, height);
   rc = reserve_pipeline(p, rows * width);

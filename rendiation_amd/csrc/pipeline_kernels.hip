@@ -14,7 +14,7 @@ using tg::Graph;
 using tg::Group;
 
 constexpr uint32_t kBlock = 256;
-constexpr uint32_t kMaxGrid = 1024;
+constexpr uint32_t kMaxGrid = 8192;
 
 uint32_t grid_for(uint32_t n) {
   const uint32_t g = (n + kBlock - 1) / kBlock;

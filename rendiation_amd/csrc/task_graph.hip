@@ -9,7 +9,7 @@ namespace {
 
 constexpr uint32_t kBlock = 256;
 constexpr uint32_t kMaxGrid = 1024;
-constexpr uint32_t kListGrid = 256;  // list kernels: one block per CU at most
+constexpr uint32_t kListGrid = 2048;  // list kernels: a few iterations per block on the largest lists
 
 uint32_t grid_for(uint32_t n, uint32_t cap = kMaxGrid) {
   const uint32_t g = (n + kBlock - 1) / kBlock;

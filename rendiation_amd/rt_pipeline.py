@@ -38,8 +38,8 @@ class GPURaytracingPipelineAndBindingSource:
     every payload (1..16); ``max_in_flight``: trace tasks in flight per
     launch entry; ``execution_round_hint``: task-graph rounds per tile (a
     chain of k traces needs 2k + 1); ``tile_pixels``: launch entries per
-    tile; ``hit_context``: HIT_ATTRIBUTES / HIT_TRANSFORMS per-hit data the
-    stages read."""
+    tile (0: 2^22; the reference uses 512 * 512); ``hit_context``:
+    HIT_ATTRIBUTES / HIT_TRANSFORMS per-hit data the stages read."""
 
     payload_words: int = 1
     max_in_flight: int = 2
