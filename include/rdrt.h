@@ -395,6 +395,8 @@ typedef struct rdrt_pipeline_desc {
   uint32_t tile_pixels;       /* launch entries per tile, whole rows (the reference: 512 * 512,
                                  wavefront_compute/mod.rs:134); 0 = 2^22 */
   uint32_t hit_context;       /* RDRT_PIPELINE_HIT_*: per-hit data the closest-hit stages read (ctx.rs:24-48) */
+  uint32_t use_graph;         /* 1: capture a call's launches into a hipGraph and replay it while the launch
+                                 size, user pointer, scene and launch shape stay the same */
 } rdrt_pipeline_desc;
 typedef struct rdrt_pipeline rdrt_pipeline;
 /* GPURaytracingPipelineAndBindingSource + executor: groups trace, ray gen,

@@ -35,7 +35,7 @@ class RdrtAoParams(C.Structure):
 class RdrtPipelineDesc(C.Structure):
     _fields_ = [("ray_gen", C.c_void_p), ("closest_hit", C.c_void_p * 4), ("miss", C.c_void_p * 4),
                 ("nclosest", u32), ("nmiss", u32), ("payload_words", u32), ("max_in_flight", u32),
-                ("round_hint", u32), ("tile_pixels", u32), ("hit_context", u32)]
+                ("round_hint", u32), ("tile_pixels", u32), ("hit_context", u32), ("use_graph", u32)]
 
 
 class RdrtTraceCall(C.Structure):

@@ -27,7 +27,21 @@ struct rdrt_pipeline {
   PipeLaunch base{};
   std::vector<Scratch> bufs;
   Scratch work;
+  // captured call (use_graph): the launches of one (launch size, user
+  // pointer, scene, buffers, launch shape), replayed while they stay the same
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint8_t> key;
+  tg::Graph graph_last{};
+  void drop() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    key.clear();
+  }
   ~rdrt_pipeline() {
+    drop();
     mem.release();
     for (Scratch &s : bufs) s.free();
     work.free();
@@ -50,6 +64,7 @@ T *carve(rdrt_pipeline *p, size_t count, hipError_t &e) {
 int32_t reserve_pipeline(rdrt_pipeline *p, uint32_t cap) {
   if (cap <= p->capacity) return RDRT_OK;
   RDRT_HIP(hipStreamSynchronize(p->ctx->work_stream));
+  p->drop();
   p->mem.release();
   for (Scratch &s : p->bufs) s.free();
   p->bufs.clear();
@@ -123,6 +138,7 @@ int32_t rdrt_pipeline_create(rdrt_ctx *c, const rdrt_pipeline_desc *d, rdrt_pipe
   if (d->tile_pixels > (1u << 24)) return fail(RDRT_ERR_INVALID_ARG, "tile_pixels above 2^24");
   if (d->hit_context & ~(RDRT_PIPELINE_HIT_ATTRIBUTES | RDRT_PIPELINE_HIT_TRANSFORMS))
     return fail(RDRT_ERR_INVALID_ARG, "unknown hit_context bits");
+  if (d->use_graph > 1) return fail(RDRT_ERR_INVALID_ARG, "use_graph is 0 or 1");
   int32_t rc = need_device(c);
   if (rc) return rc;
   rdrt_pipeline *p = new rdrt_pipeline();
@@ -176,16 +192,56 @@ int32_t rdrt_pipeline_trace_rays(rdrt_pipeline *p, uint32_t width, uint32_t heig
   TraceTune tune = c->tune;
   tune.cull = 0;  // the queue is the trace group's own
   const uint32_t push = push_depth(c, tune);
-  bool first = true;
-  for (uint32_t y = 0; y < height; y += rows) {
-    const uint32_t h = std::min(rows, height - y);
-    L.tile_y = y;
-    tg::Graph after{};
-    RDRT_HIP(enqueue_pipeline_tile(L, p->stages, h, rounds, scene, attrib, push, tune, (uint32_t *)p->work.ptr, user, first,
-                                   c->work_stream, &after));
-    p->last = after;
-    first = false;
+  hipStream_t st = c->work_stream;
+  tg::Graph last{};
+  auto enqueue_all = [&]() -> hipError_t {
+    bool first = true;
+    for (uint32_t y = 0; y < height; y += rows) {
+      const uint32_t h = std::min(rows, height - y);
+      L.tile_y = y;
+      hipError_t e = enqueue_pipeline_tile(L, p->stages, h, rounds, scene, attrib, push, tune,
+                                           (uint32_t *)p->work.ptr, user, first, st, &last);
+      if (e != hipSuccess) return e;
+      first = false;
+    }
+    return hipSuccess;
+  };
+  if (!p->desc.use_graph) {
+    RDRT_HIP(enqueue_all());
+    p->last = last;
+    return RDRT_OK;
   }
+  std::vector<uint8_t> key(sizeof(L) + sizeof(scene) + sizeof(attrib) + sizeof(TraceTune) + 4 * 4 + sizeof(void *));
+  uint8_t *k = key.data();
+  std::memcpy(k, &L, sizeof(L));
+  k += sizeof(L);
+  std::memcpy(k, &scene, sizeof(scene));
+  k += sizeof(scene);
+  std::memcpy(k, &attrib, sizeof(attrib));
+  k += sizeof(attrib);
+  std::memcpy(k, &tune, sizeof(TraceTune));
+  k += sizeof(TraceTune);
+  const uint32_t extra[4] = {height, rows, rounds, push};
+  std::memcpy(k, extra, sizeof(extra));
+  k += sizeof(extra);
+  std::memcpy(k, &user, sizeof(void *));
+  if (!p->exec || key != p->key) {
+    p->drop();
+    RDRT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue_all();
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(st, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(RDRT_ERR_HIP, std::string("pipeline capture failed: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    p->graph = g;
+    RDRT_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    p->key = key;
+    p->graph_last = last;
+  }
+  RDRT_HIP(hipGraphLaunch(p->exec, st));
+  p->last = p->graph_last;
   return RDRT_OK;
 }
 

@@ -39,13 +39,15 @@ class GPURaytracingPipelineAndBindingSource:
     launch entry; ``execution_round_hint``: task-graph rounds per tile (a
     chain of k traces needs 2k + 1); ``tile_pixels``: launch entries per
     tile (0: 2^22; the reference uses 512 * 512); ``hit_context``:
-    HIT_ATTRIBUTES / HIT_TRANSFORMS per-hit data the stages read."""
+    HIT_ATTRIBUTES / HIT_TRANSFORMS per-hit data the stages read;
+    ``use_graph``: capture a call into a hipGraph and replay it."""
 
     payload_words: int = 1
     max_in_flight: int = 2
     execution_round_hint: int = 8
     tile_pixels: int = 0
     hit_context: int = 0
+    use_graph: bool = True
     ray_gen: List[int] = field(default_factory=list)
     closest_hit: List[int] = field(default_factory=list)
     miss: List[int] = field(default_factory=list)
@@ -122,6 +124,7 @@ class RayTracingPipeline:
         d.round_hint = source.execution_round_hint
         d.tile_pixels = source.tile_pixels
         d.hit_context = source.hit_context
+        d.use_graph = 1 if source.use_graph else 0
         self.system = system
         self.ngroups = 2 + nclosest + nmiss
         h = C.c_void_p()

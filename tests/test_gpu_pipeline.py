@@ -124,6 +124,38 @@ def test_debug_pipeline_tiling_invariant(stages, ref_scene, tile_pixels, max_in_
     assert np.array_equal(img, base_img)
 
 
+def test_graph_replay_equals_eager_calls(stages, ref_scene):
+    """A captured call replayed three times (the target cleared between
+    calls) gives the eager call's hits and image; a new launch size
+    recaptures."""
+    system, _ = ref_scene
+    _say("pipeline graph replay vs eager")
+    eager = _run_debug(system, stages, 200, 150, use_graph=False)
+    import torch
+
+    n = 200 * 150
+    buf = torch.full((n * 20,), 0xAB, dtype=torch.uint8, device="cuda:0")
+    torch.cuda.synchronize()
+    p = _pipeline(system, stages, 0, use_graph=True)
+    for _ in range(3):
+        buf.fill_(0xAB)
+        torch.cuda.synchronize()
+        p.trace_ray(200, 150, buf.data_ptr())
+        system.sync()
+        out = buf.cpu().numpy()
+        assert np.array_equal(out[: n * 16].view(np.uint32).reshape(n, 4), eager[0])
+        assert np.array_equal(out[n * 16:].reshape(n, 4), eager[1])
+        st = p.read_back_execution_states()
+        assert st.is_empty() and not any(st.errors)
+    small = _run_debug(system, stages, 64, 48, use_graph=False)
+    m = 64 * 48
+    p.trace_ray(64, 48, buf.data_ptr())
+    system.sync()
+    out = buf.cpu().numpy()
+    assert np.array_equal(out[: m * 16].view(np.uint32).reshape(m, 4), small[0])
+    p.close()
+
+
 def test_debug_pipeline_too_few_rounds_leaves_tasks(stages, ref_scene):
     # a trace needs three rounds (ray gen, trace + closest hit, trace
     # resolves + ray gen): with two, the ray-gen tasks are still asleep
