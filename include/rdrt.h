@@ -340,9 +340,10 @@ int32_t rdrt_ao_hemisphere_sample(uint32_t s, float out[3]);
  * A stage is a HIP kernel of the caller, `__global__ void f(rdrt_stage_args
  * a)` (compiled for gfx950 into any code object of the process; pass its
  * host-side handle, `(const void *)f`).  Each round the library launches it
- * over the stage's polled tasks -- slots[i] for i < *count, a grid covering
- * `capacity` threads -- and every polled task either resolves (its payload
- * goes back to the task that traced it) or traces one more ray
+ * over the stage's polled tasks -- slots[i] for i < *count, a grid of
+ * 256-thread blocks covering `capacity` threads -- and every polled task
+ * either resolves (its payload goes back to the task that traced it) or
+ * traces one more ray
  * (then_trace): it writes the trace call and the payload the traced ray's
  * stages receive, and is polled again once that ray's closest-hit or miss
  * stage has resolved, with polls[slot] one higher and the returned payload

@@ -15,7 +15,9 @@ using pt::v3;
 using tg::Graph;
 using tg::Group;
 
-constexpr uint32_t kBlock = 256;
+// 1024-thread blocks: a pass takes one counter atomic per block and list,
+// and same-address atomics serialize
+constexpr uint32_t kBlock = 1024;
 constexpr uint32_t kMaxGrid = 1024;
 constexpr float kFltEps = 1.1920928955078125e-7f;  // ShaderRayRange::default (api/ty.rs:49-56)
 constexpr float kFltMax = 3.402823466e+38f;

@@ -13,8 +13,11 @@ namespace {
 using tg::Graph;
 using tg::Group;
 
-constexpr uint32_t kBlock = 256;
-constexpr uint32_t kMaxGrid = 8192;
+// 1024-thread blocks: each pass takes one counter atomic per block and list,
+// and same-address atomics serialize, so fewer, larger blocks finish sooner
+constexpr uint32_t kBlock = 1024;
+constexpr uint32_t kMaxGrid = 4096;
+constexpr uint32_t kStageBlock = 256;  // the caller's stage kernels
 
 uint32_t grid_for(uint32_t n) {
   const uint32_t g = (n + kBlock - 1) / kBlock;
@@ -230,7 +233,8 @@ hipError_t enqueue_pipeline_tile(const PipeLaunch &L0, const void *const *stages
     a.payload_words = L.words;
     a.capacity = G.size;
     void *args[] = {&a};
-    hipError_t e2 = hipLaunchKernel(stages[gid], dim3((G.size + kBlock - 1) / kBlock), dim3(kBlock), args, 0, st);
+    hipError_t e2 =
+        hipLaunchKernel(stages[gid], dim3((G.size + kStageBlock - 1) / kStageBlock), dim3(kStageBlock), args, 0, st);
     if (e2 != hipSuccess) return e2;
     hipLaunchKernelGGL(k_stage_post, dim3(grid), dim3(kBlock), 0, st, L, gid);
     return hipGetLastError();

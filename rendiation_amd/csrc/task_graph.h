@@ -77,7 +77,7 @@ __device__ __forceinline__ uint32_t *active_list(const Group &G, uint32_t cur) {
 // counter then takes one atomic per block instead of one per thread.  Blocks
 // have at most kMaxBlock threads.
 
-constexpr uint32_t kMaxBlock = 256;
+constexpr uint32_t kMaxBlock = 1024;
 
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 __device__ __forceinline__ uint32_t rank_in(uint64_t m) {

@@ -7,9 +7,11 @@ namespace rdrt {
 namespace tg {
 namespace {
 
-constexpr uint32_t kBlock = 256;
+// 1024-thread blocks: a pass takes one counter atomic per block and list,
+// and same-address atomics serialize
+constexpr uint32_t kBlock = 1024;
 constexpr uint32_t kMaxGrid = 1024;
-constexpr uint32_t kListGrid = 2048;  // list kernels: a few iterations per block on the largest lists
+constexpr uint32_t kListGrid = 1024;
 
 uint32_t grid_for(uint32_t n, uint32_t cap = kMaxGrid) {
   const uint32_t g = (n + kBlock - 1) / kBlock;
