@@ -415,6 +415,24 @@ int32_t rdrt_pipeline_states(rdrt_pipeline *p, uint32_t *wake, uint32_t *sleep_o
 /* Rays the trace group traversed in the last call (waits for the stream). */
 int32_t rdrt_pipeline_ray_count(rdrt_pipeline *p, uint64_t *out);
 
+/* ---- buffers shared with another graphics API ----
+ * The reference's rays, hits and render targets live in wgpu buffers
+ * (shader/ray-tracing/src/api/backend.rs:48-56).  A Vulkan device exports
+ * a buffer's memory as an opaque POSIX file descriptor
+ * (VK_EXTERNAL_MEMORY_HANDLE_TYPE_OPAQUE_FD_BIT, vkGetMemoryFdKHR); this
+ * maps bytes [offset, offset + length) of that allocation (`size` bytes in
+ * all) into the context's device and returns their device address, which
+ * every rdrt_* call that takes device memory accepts.  On success the
+ * library owns `fd` (closed by the release); on failure the caller still
+ * does.  Work on the buffer is ordered by the caller (the context's stream
+ * against the other API's queue, e.g. with an exported semaphore). */
+typedef struct rdrt_external_buffer rdrt_external_buffer;
+int32_t rdrt_external_buffer_import(rdrt_ctx *ctx, int fd, uint64_t size, uint64_t offset, uint64_t length,
+                                    rdrt_external_buffer **out, void **dev_ptr);
+/* Waits for the context's stream, unmaps the buffer and releases the
+ * imported memory.  NULL is a no-op. */
+int32_t rdrt_external_buffer_release(rdrt_external_buffer *buffer);
+
 /* ---- host picking (content/mesh/core/src/feature/bvh.rs:57-86) ---- */
 #define RDRT_FACE_FRONT 0u
 #define RDRT_FACE_BACK 1u

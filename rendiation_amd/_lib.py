@@ -206,6 +206,9 @@ SIGNATURES = {
     "rdrt_pipeline_trace_rays": (i32, [C.c_void_p, u32, u32, C.c_void_p]),
     "rdrt_pipeline_states": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "rdrt_pipeline_ray_count": (i32, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "rdrt_external_buffer_import": (i32, [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p,
+                                          C.c_void_p]),
+    "rdrt_external_buffer_release": (i32, [C.c_void_p]),
     "rdrt_sample_sobol": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, u32, C.c_void_p]),
 }
 

@@ -28,7 +28,8 @@ HIP_SOURCES = ["trace_fast_256.hip", "trace_fast_128.hip", "trace_fast_64.hip", 
                "trace_qquad_64.hip", "trace_kernels.hip", "pt_kernels.hip", "batch_kernels.hip", "api.hip",
                "api_pt.hip", "api_batch.hip", "refit_kernels.hip", "device_build.hip",
                "attrib_kernels.hip", "task_graph.hip", "ao_kernels.hip", "api_ao.hip", "det_sum.hip", "api_static.hip",
-               "pipeline_kernels.hip", "api_pipeline.hip", "pt_tail_0.hip", "pt_tail_1.hip", "pt_tail_2.hip"]
+               "pipeline_kernels.hip", "api_pipeline.hip", "pt_tail_0.hip", "pt_tail_1.hip", "pt_tail_2.hip",
+               "api_external.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 HIP_FLAGS = [

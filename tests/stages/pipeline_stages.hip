@@ -280,3 +280,67 @@ extern "C" const void *test_stage(uint32_t k) {
     default: return nullptr;
   }
 }
+
+// ---- an exportable allocation for the external-buffer test ----
+// Device memory created with HIP's virtual memory API, exportable as a
+// POSIX file descriptor (what a Vulkan device hands out for an exported
+// buffer), mapped at *ptr for the test to fill and read.  Returns 0, 1 when
+// the device has no virtual memory management, or -1 on an error.
+extern "C" int test_exportable_create(size_t bytes, int *fd, void **ptr, size_t *mapped, unsigned long long *handle) {
+  int vmm = 0;
+  if (hipDeviceGetAttribute(&vmm, hipDeviceAttributeVirtualMemoryManagementSupported, 0) != hipSuccess || !vmm)
+    return 1;
+  hipMemAllocationProp prop = {};
+  prop.type = hipMemAllocationTypePinned;
+  prop.requestedHandleType = hipMemHandleTypePosixFileDescriptor;
+  prop.location.type = hipMemLocationTypeDevice;
+  prop.location.id = 0;
+  size_t gran = 0;
+  if (hipMemGetAllocationGranularity(&gran, &prop, hipMemAllocationGranularityMinimum) != hipSuccess || !gran)
+    return -1;
+  const size_t size = (bytes + gran - 1) / gran * gran;
+  hipMemGenericAllocationHandle_t h;
+  if (hipMemCreate(&h, size, &prop, 0) != hipSuccess) return -1;
+  int f = -1;
+  if (hipMemExportToShareableHandle(&f, h, hipMemHandleTypePosixFileDescriptor, 0) != hipSuccess) {
+    (void)hipMemRelease(h);
+    return -1;
+  }
+  void *p = nullptr;
+  if (hipMemAddressReserve(&p, size, 0, nullptr, 0) != hipSuccess) {
+    (void)hipMemRelease(h);
+    return -1;
+  }
+  if (hipMemMap(p, size, 0, h, 0) != hipSuccess) {
+    (void)hipMemAddressFree(p, size);
+    (void)hipMemRelease(h);
+    return -1;
+  }
+  hipMemAccessDesc acc = {};
+  acc.location.type = hipMemLocationTypeDevice;
+  acc.location.id = 0;
+  acc.flags = hipMemAccessFlagsProtReadWrite;
+  if (hipMemSetAccess(p, size, &acc, 1) != hipSuccess) {
+    (void)hipMemUnmap(p, size);
+    (void)hipMemAddressFree(p, size);
+    (void)hipMemRelease(h);
+    return -1;
+  }
+  *fd = f;
+  *ptr = p;
+  *mapped = size;
+  *handle = reinterpret_cast<unsigned long long>(h);
+  return 0;
+}
+
+extern "C" int test_exportable_destroy(void *ptr, size_t size, unsigned long long handle) {
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemUnmap(ptr, size);
+  if (e == hipSuccess) e = hipMemAddressFree(ptr, size);
+  if (e == hipSuccess) e = hipMemRelease(reinterpret_cast<hipMemGenericAllocationHandle_t>(handle));
+  return e == hipSuccess ? 0 : -1;
+}
+
+extern "C" int test_copy(void *dst, const void *src, size_t bytes, int kind) {
+  return hipMemcpy(dst, src, bytes, kind ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+}

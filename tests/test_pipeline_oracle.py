@@ -149,3 +149,15 @@ int main(void) {
     assert got[3] == L.RdrtPipelineDesc.hit_context.offset
     # rdrt_stage_args: 17 pointers, then launch_size[2], payload_words, capacity
     assert got[4:] == [17 * 8 + 16, 16 * 8, 17 * 8 + 12]
+
+
+def test_external_buffer_argument_checks():
+    system = AccelerationStructureSystem(device=None)
+    lib, ctx = system._L, system._h
+    out, ptr = C.c_void_p(), C.c_void_p()
+    assert lib.rdrt_external_buffer_import(ctx, -1, 4096, 0, 4096, C.byref(out), C.byref(ptr)) == -1
+    assert lib.rdrt_external_buffer_import(ctx, 3, 0, 0, 4096, C.byref(out), C.byref(ptr)) == -1
+    assert lib.rdrt_external_buffer_import(ctx, 3, 4096, 4000, 200, C.byref(out), C.byref(ptr)) == -1
+    assert lib.rdrt_external_buffer_import(ctx, 3, 4096, 0, 4096, C.byref(out), C.byref(ptr)) == -6  # no device
+    assert out.value is None and ptr.value is None
+    assert lib.rdrt_external_buffer_release(None) == 0
