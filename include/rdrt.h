@@ -547,6 +547,19 @@ int32_t rdrt_blas_set_vertex_normals(rdrt_ctx *ctx, uint32_t blas, uint32_t geom
 /* Per-triangle material offsets of one geometry (copied).  A hit's material
  * is materials[instance_custom_index + offset], as the reference looks the
  * surface up by the instance's custom id (feature/path_tracing/ray_hit.rs:65-75). */
+/* Any-hit behaviour of each triangle of a geometry, for hits while the
+ * geometry is not opaque for the ray (no GEOMETRY_FLAG_OPAQUE, or forced
+ * non-opaque by the ray or instance): RDRT_ANYHIT_ACCEPT commits the hit,
+ * RDRT_ANYHIT_END_SEARCH ends the traversal (with or without it), 0 ignores
+ * it -- the result of the reference's any-hit stage (ANYHIT_BEHAVIOR_*,
+ * traverse_cpu.rs:158-185) expressed per primitive, as an alpha-tested
+ * cutout needs.  Without a table every such hit is accepted.  Which hit an
+ * end-search behaviour stops at depends on the visit order (as in the
+ * reference); accept / ignore do not.  Takes effect at the next commit. */
+#define RDRT_ANYHIT_ACCEPT 1u
+#define RDRT_ANYHIT_END_SEARCH 2u
+int32_t rdrt_blas_set_anyhit(rdrt_ctx *ctx, uint32_t blas, uint32_t geometry, const uint8_t *behavior,
+                             uint32_t ntris);
 /* Per-vertex texture coordinates of a geometry (uv pairs), the uv channel
  * of the bindless mesh (scene/rendering/gpu-ray-tracing/src/
  * bindless_mesh_bridge.rs:65-77). */

@@ -490,6 +490,10 @@ void intersect_triangle(V3 o, V3 d, float rnear, float rfar, V3 v0, V3 v1, V3 v2
 
 }  // namespace
 
+// any-hit behaviour bits (naive/mod.rs ANYHIT_BEHAVIOR_*): accept the hit,
+// end the search
+constexpr uint32_t kAnyHitAccept = 1u, kAnyHitEndSearch = 2u;
+
 // ------------------------------------------------------------- scene -------
 struct Geometry {
   std::vector<float> pos;
@@ -499,6 +503,7 @@ struct Geometry {
   std::vector<float> normals;         // per vertex (path tracer), optional
   std::vector<uint32_t> tri_material;  // per triangle (path tracer), optional
   std::vector<float> uvs;             // per vertex (hit attributes), optional
+  std::vector<uint8_t> anyhit;        // per triangle: any-hit behaviour of non-opaque hits, optional
 };
 
 struct orc_scene {
@@ -532,10 +537,18 @@ struct orc_scene {
   std::vector<float> uvs;             // 2 per vertex, parallel to vertices (zeros = none)
   bool has_uvs = false;
   std::vector<uint32_t> tri_material;  // per global triangle
+  std::vector<uint8_t> anyhit;         // per global triangle (empty: every non-opaque hit is accepted)
   std::vector<M4> inst_transform;      // object-to-world of each instance slot (as given)
   std::vector<uint32_t> tlas_inst_first;  // first instance slot of each TLAS
   std::vector<float> materials;        // 8 floats each: albedo xyz, pad, emission xyz, pad
 };
+
+// The any-hit behaviour of a non-opaque hit on global triangle `tri`: the
+// caller's table, or accept (the reference tests' TEST_ANYHIT_BEHAVIOR).
+inline uint32_t anyhit_of(const orc_scene *s, uint32_t tri) {
+  return tri < s->anyhit.size() ? (uint32_t)s->anyhit[tri] : kAnyHitAccept;
+}
+
 
 extern "C" {
 
@@ -598,6 +611,7 @@ int orc_build(orc_scene *s) {
   s->inst_transform.clear();
   s->tlas_inst_first.clear();
   s->tri_material.clear();
+  s->anyhit.clear();
   s->blas_meta.clear();
   s->geo_meta.clear();
   s->tlas_bvh_root.clear();
@@ -694,6 +708,7 @@ int orc_build(orc_scene *s) {
         s->has_uvs = s->has_uvs || has;
       }
       for (size_t t = 0; t < ntri; t++) s->tri_material.push_back(t < g.tri_material.size() ? g.tri_material[t] : 0u);
+      for (size_t t = 0; t < ntri; t++) s->anyhit.push_back(t < g.anyhit.size() ? g.anyhit[t] : kAnyHitAccept);
       TriBvh tb;
       tb.primitive_start = primitive_start;
       tb.geometry_idx = (uint32_t)gi;
@@ -895,7 +910,6 @@ bool traverse_one(const orc_scene *s, const RayIn &ray, orc_hit &hit, orc_hit_me
         const auto &geo = s->geo_meta[gr];
         bool pass, is_opaque;
         cull_geometry(flags, geo.flags, pass, is_opaque);
-        (void)is_opaque;  // non-opaque geometry: the default any-hit accepts
         if (!pass) continue;
         bool cull_enable, cull_back;
         cull_triangle(flags, cull_enable, cull_back);
@@ -913,21 +927,27 @@ bool traverse_one(const orc_scene *s, const RayIn &ray, orc_hit &hit, orc_hit_me
             intersect_triangle(bo, bd, ray.tmin * scaling, far_ * scaling, v0, v1, v2, cull_enable, cull_back, r);
             if (r[0] != 0.0f) {
               float distance = r[1] / scaling;
-              far_ = distance;  // update_far
-              found = true;
-              hit.prim_id = tri - geo.primitive_start;
-              hit.t = distance;
-              hit.u = r[2];
-              hit.v = r[3];
-              hm.geometry_id = geo.geometry_idx;
-              hm.instance_id = s->inst_order[ti];
-              hm.instance_custom_index = s->inst_custom[ti];
-              hm.hit_kind = r[0] < 0.0f ? 0xFFu : 0xFEu;  // ty.rs:139-140
-              if (ext) {
-                ext[0] = gr;  // global geometry index
-                ext[1] = ti;  // instance index in forest order
+              // opaque hits commit; a non-opaque one takes its any-hit
+              // behaviour (traverse_cpu.rs:164-185)
+              uint32_t behavior = is_opaque ? kAnyHitAccept : anyhit_of(s, tri);
+              if (behavior & kAnyHitAccept) {
+                far_ = distance;  // update_far
+                found = true;
+                hit.prim_id = tri - geo.primitive_start;
+                hit.t = distance;
+                hit.u = r[2];
+                hit.v = r[3];
+                hm.geometry_id = geo.geometry_idx;
+                hm.instance_id = s->inst_order[ti];
+                hm.instance_custom_index = s->inst_custom[ti];
+                hm.hit_kind = r[0] < 0.0f ? 0xFFu : 0xFEu;  // ty.rs:139-140
+                if (ext) {
+                  ext[0] = gr;  // global geometry index
+                  ext[1] = ti;  // instance index in forest order
+                }
+                if (flags & F_ACCEPT_FIRST_HIT_AND_END_SEARCH) behavior |= kAnyHitEndSearch;
               }
-              if (flags & F_ACCEPT_FIRST_HIT_AND_END_SEARCH) return true;
+              if (behavior & kAnyHitEndSearch) return found;
             }
           }
         }
@@ -1019,7 +1039,8 @@ int orc_trace_brute(const orc_scene *s, const orc_ray *rays, uint32_t n, uint32_
             V3 v2 = s->vertices[s->indices[3 * (size_t)tri + 2]];
             float r[4];
             intersect_triangle(bo, bd, ray.tmin * scaling, far_ * scaling, v0, v1, v2, ce, cb, r);
-            if (r[0] != 0.0f) {
+            // ignored non-opaque hits are skipped (end-search behaviours depend on visit order: not here)
+            if (r[0] != 0.0f && (opq || (anyhit_of(s, tri) & kAnyHitAccept))) {
               far_ = r[1] / scaling;
               h.prim_id = tri - geo.primitive_start;
               h.t = far_;
@@ -1518,6 +1539,14 @@ int orc_set_vertex_normals(orc_scene *s, uint32_t blas, uint32_t geo, const floa
   Geometry &g = s->blas_data[blas][geo];
   if (nverts != g.pos.size() / 3) return -1;
   g.normals.assign(nrm, nrm + 3 * (size_t)nverts);
+  s->built = false;
+  return 0;
+}
+
+int orc_set_anyhit(orc_scene *s, uint32_t blas, uint32_t geo, const uint8_t *behavior, uint32_t ntris) {
+  if (blas >= s->blas_data.size() || geo >= s->blas_data[blas].size()) return -1;
+  Geometry &g = s->blas_data[blas][geo];
+  g.anyhit.assign(behavior, behavior + ntris);
   s->built = false;
   return 0;
 }

@@ -229,6 +229,11 @@ void orc_camera_ray(const orc_pt_params *params, uint32_t px, uint32_t py, uint3
                     float *origin3, float *dir3);
 int orc_set_vertex_normals(orc_scene *s, uint32_t blas, uint32_t geometry, const float *normals_xyz,
                            uint32_t nverts);
+// Any-hit behaviour of each triangle of a geometry (bit 0 accept the hit,
+// bit 1 end the search; naive/mod.rs ANYHIT_BEHAVIOR_*), used for hits on
+// non-opaque geometry (traverse_cpu.rs:164-185).  Without a table every
+// such hit is accepted.
+int orc_set_anyhit(orc_scene *s, uint32_t blas, uint32_t geometry, const uint8_t *behavior, uint32_t ntris);
 int orc_set_triangle_materials(orc_scene *s, uint32_t blas, uint32_t geometry, const uint32_t *offsets,
                                uint32_t ntris);
 /* 8 floats per material: albedo xyz, pad, emission xyz, pad */

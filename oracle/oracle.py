@@ -88,6 +88,7 @@ _SIG = {
     "orc_compact": (u32, [P, P, u32, P]),
     "orc_set_vertex_normals": (C.c_int, [P, u32, u32, P, u32]),
     "orc_set_triangle_materials": (C.c_int, [P, u32, u32, P, u32]),
+    "orc_set_anyhit": (C.c_int, [P, u32, u32, P, u32]),
     "orc_set_materials": (None, [P, P, u32]),
     "orc_pt_render": (C.c_int, [P, C.POINTER(OrcPtParams), P, P, C.c_int]),
     "orc_pt_render_rows": (C.c_int, [P, C.POINTER(OrcPtParams), u32, u32, P, P, C.c_int]),
@@ -278,6 +279,13 @@ class OracleScene:
         n = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
         if self.L.orc_set_vertex_normals(self.h, blas, geometry, n.ctypes.data, n.shape[0]) != 0:
             raise ValueError("bad normals")
+
+    def set_anyhit(self, blas: int, geometry: int, behavior) -> None:
+        """Per-triangle any-hit behaviour (bit 0 accept, bit 1 end search) of
+        hits on the geometry when it is not opaque for the ray."""
+        b = np.ascontiguousarray(behavior, dtype=np.uint8).reshape(-1)
+        if self.L.orc_set_anyhit(self.h, blas, geometry, b.ctypes.data, b.size) != 0:
+            raise ValueError("bad any-hit table")
 
     def set_triangle_materials(self, blas: int, geometry: int, offsets) -> None:
         o = np.ascontiguousarray(offsets, dtype=np.uint32).reshape(-1)

@@ -209,6 +209,7 @@ SIGNATURES = {
     "rdrt_external_buffer_import": (i32, [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p,
                                           C.c_void_p]),
     "rdrt_external_buffer_release": (i32, [C.c_void_p]),
+    "rdrt_blas_set_anyhit": (i32, [C.c_void_p, u32, u32, C.c_void_p, u32]),
     "rdrt_sample_sobol": (i32, [C.c_void_p, C.c_void_p, C.c_void_p, u32, C.c_void_p]),
 }
 

@@ -46,6 +46,8 @@ GEOMETRY_INSTANCE_FORCE_OPAQUE = 0x4
 GEOMETRY_INSTANCE_FORCE_NO_OPAQUE = 0x8
 GEOMETRY_FLAG_OPAQUE = 0x1
 GEOMETRY_FLAG_NO_DUPLICATE_ANYHIT_INVOCATION = 0x2
+ANYHIT_ACCEPT = 0x1  # any-hit behaviours (rdrt_blas_set_anyhit)
+ANYHIT_END_SEARCH = 0x2
 
 TRAVERSAL_FAST = 0
 TRAVERSAL_REFERENCE = 1
@@ -325,6 +327,13 @@ class AccelerationStructureSystem:
             _check_dev(ray_meta, n * 4, "ray_meta")
         L.check(self._L.rdrt_fetch_hit_transforms(self._h, _ptr(rays), _ptr(ray_meta), _ptr(hits), _ptr(hit_meta), n,
                                                   tlas_idx, _ptr(out)))
+
+    def set_anyhit(self, blas: BlasHandle, geometry: int, behavior: np.ndarray) -> None:
+        """Per-triangle any-hit behaviour (ANYHIT_ACCEPT | ANYHIT_END_SEARCH,
+        0 = ignore) of hits on the geometry while it is not opaque for the
+        ray; takes effect at the next create_comp_instance."""
+        b = np.ascontiguousarray(behavior, dtype=np.uint8).reshape(-1)
+        L.check(self._L.rdrt_blas_set_anyhit(self._h, blas.index, geometry, b.ctypes.data, b.size))
 
     def set_triangle_materials(self, blas: BlasHandle, geometry: int, offsets: np.ndarray) -> None:
         o = np.ascontiguousarray(offsets, dtype=np.uint32).reshape(-1)

@@ -57,6 +57,7 @@ void free_device_scene(DeviceScene &d) {
   release(d.indices);
   release(d.binding);
   release(d.tri_material);
+  release(d.anyhit);
   release(d.vertices);
   release(d.normals);
   release(d.geos);
@@ -147,6 +148,7 @@ TraceParams scene_trace_params(const rdrt_ctx *c) {
   p.tlas_forest = d.tlas_forest.ptr;
   p.indices = d.indices.ptr;
   p.vertices = d.vertices.ptr;
+  p.anyhit = d.anyhit.count ? d.anyhit.ptr : nullptr;
   return p;
 }
 
@@ -469,6 +471,7 @@ int32_t rdrt_commit(rdrt_ctx *c) {
   RDRT_HIP(upload(d.inst_lookup, b.inst_lookup, d.bytes));
   RDRT_HIP(upload(d.tlas_inst_first, b.tlas_inst_first, d.bytes));
   RDRT_HIP(upload(d.tri_material, b.tri_material, d.bytes));
+  RDRT_HIP(upload(d.anyhit, b.anyhit, d.bytes));
   c->uploaded = true;
   return RDRT_OK;
 }

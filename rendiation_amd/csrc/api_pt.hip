@@ -137,6 +137,20 @@ int32_t rdrt_blas_set_triangle_materials(rdrt_ctx *c, uint32_t blas, uint32_t ge
   return RDRT_OK;
 }
 
+int32_t rdrt_blas_set_anyhit(rdrt_ctx *c, uint32_t blas, uint32_t geometry, const uint8_t *behavior, uint32_t ntris) {
+  if (!c) return fail(RDRT_ERR_INVALID_ARG, "null context");
+  GeometrySrc *g = geometry_of(c, blas, geometry);
+  if (!g) return fail(RDRT_ERR_INVALID_ARG, "unknown BLAS or geometry");
+  uint32_t nt = g->indexed ? (uint32_t)(g->idx.size() / 3) : (uint32_t)(g->pos.size() / 9);
+  if (ntris != nt || (ntris && !behavior)) return fail(RDRT_ERR_INVALID_ARG, "one any-hit behaviour per triangle");
+  for (uint32_t t = 0; t < ntris; t++)
+    if (behavior[t] & ~(RDRT_ANYHIT_ACCEPT | RDRT_ANYHIT_END_SEARCH))
+      return fail(RDRT_ERR_INVALID_ARG, "unknown any-hit behaviour bits");
+  g->anyhit.assign(behavior, behavior + ntris);
+  invalidate(c);
+  return RDRT_OK;
+}
+
 int32_t rdrt_set_materials(rdrt_ctx *c, const rdrt_material *m, uint32_t n) {
   if (!c || (n && !m)) return fail(RDRT_ERR_INVALID_ARG, "null argument");
   c->materials.resize(n);

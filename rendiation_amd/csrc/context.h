@@ -73,6 +73,7 @@ struct DeviceScene {
   DevArray<QuadNodeQ> tri_qquads, tlas_qquads;
   DevArray<TriPre> tris;
   DevArray<uint32_t> redirect, tri_ids, indices, binding, tri_material, tlas_quad_roots, inst_lookup, tlas_inst_first;
+  DevArray<uint8_t> anyhit;
   DevArray<float> vertices, normals, uvs, inst_fwd;
   DevArray<GeoRec> geos;
   DevArray<InstRec> insts;

@@ -789,13 +789,14 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
   std::vector<Box> blas_box(src.blas.size(), box_empty());
   std::vector<char> blas_has_box(src.blas.size(), 0);
   std::vector<uint32_t> blas_geo_begin(src.blas.size(), 0), blas_geo_end(src.blas.size(), 0);
-  bool any_normals = false, any_material = false, any_uvs = false;
+  bool any_normals = false, any_material = false, any_uvs = false, any_anyhit = false;
   for (uint32_t b = 0; b < src.blas.size(); b++)
     if (src.blas_live[b])
       for (const GeometrySrc &g : src.blas[b]) {
         any_normals |= !g.normals.empty();
         any_uvs |= !g.uvs.empty();
         any_material |= !g.tri_material.empty();
+        any_anyhit |= !g.anyhit.empty();
       }
   // Per geometry, in parallel: forest, child pairs and child quads with
   // local node indices (leaf refs already global), relocated when appended.
@@ -865,6 +866,7 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
     if (any_normals) out.normals.resize(3 * nv);
     if (any_uvs) out.uvs.resize(2 * nv);
     if (any_material) out.tri_material.resize(nt);
+    if (any_anyhit) out.anyhit.resize(nt);
   }
   std::vector<uint32_t> geo_max_mat(jobs.size(), 0);
   parallel_for(jobs.size(), hw, [&](size_t ji) {
@@ -894,6 +896,8 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
       }
       geo_max_mat[ji] = mx;
     }
+    if (any_anyhit)
+      for (size_t t = 0; t < nt; t++) out.anyhit[sl.t + t] = t < g.anyhit.size() ? g.anyhit[t] : (uint8_t)RDRT_ANYHIT_ACCEPT;
     Local &l = local[ji];
     const uint32_t fb = (uint32_t)sl.f, pb = (uint32_t)sl.p, qb = (uint32_t)sl.q;
     for (size_t k = 0; k < l.forest.size(); k++) {

@@ -25,6 +25,9 @@ struct GeometrySrc {
   std::vector<float> normals;
   std::vector<uint32_t> tri_material;
   std::vector<float> uvs;  // texture coordinates (uv per vertex, empty = none)
+  // any-hit behaviour per triangle (RDRT_ANYHIT_*) of hits on this geometry
+  // when it is not opaque for the ray (empty = accept every hit)
+  std::vector<uint8_t> anyhit;
 };
 
 struct SceneSrc {
@@ -49,6 +52,7 @@ struct BuiltScene {
   std::vector<float> normals;
   std::vector<uint32_t> tri_material;
   std::vector<uint32_t> geo_max_material;
+  std::vector<uint8_t> anyhit;  // any-hit behaviour per triangle (empty when no geometry has a table)
   std::vector<float> uvs;  // 2 per vertex when any geometry has them (zeros elsewhere), else empty
   // instance records by source position: inst_lookup[tlas_inst_first[t] + k]
   // is the record of instance k of TLAS t

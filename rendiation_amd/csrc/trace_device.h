@@ -106,9 +106,19 @@ __device__ __forceinline__ uint32_t merge_flags(uint32_t f, uint32_t gi) {
   if (gi & GI_FORCE_NO_OPAQUE) f |= F_FORCE_NON_OPAQUE;
   return f;
 }
-// flag.rs:78-91 (pass only: non-opaque geometry takes the default any-hit, accept)
+// flag.rs:78-91.  Opaque for this ray: the geometry's flag, forced either way by the merged
+// ray / instance flags (flag.rs cull_geometry).
+__device__ __forceinline__ bool geometry_opaque(uint32_t f, uint32_t gflags) {
+  return ((gflags & RDRT_GEOMETRY_FLAG_OPAQUE) || (f & F_FORCE_OPAQUE)) && !(f & F_FORCE_NON_OPAQUE);
+}
+// Any-hit behaviour of a hit on global triangle `tri` (traverse_cpu.rs:
+// 164-185): an opaque hit commits; a non-opaque one takes the caller's
+// per-triangle behaviour, or is accepted without a table.
+__device__ __forceinline__ uint32_t anyhit_behavior(const TraceParams &p, bool opaque, uint32_t tri) {
+  return (opaque || !p.anyhit) ? RDRT_ANYHIT_ACCEPT : (uint32_t)p.anyhit[tri];
+}
 __device__ __forceinline__ bool geometry_passes(uint32_t f, uint32_t gflags) {
-  bool opaque = ((gflags & RDRT_GEOMETRY_FLAG_OPAQUE) || (f & F_FORCE_OPAQUE)) && !(f & F_FORCE_NON_OPAQUE);
+  bool opaque = geometry_opaque(f, gflags);
   return (opaque && !(f & F_CULL_OPAQUE)) || (!opaque && !(f & F_CULL_NON_OPAQUE));
 }
 // flag.rs:94-105
@@ -385,6 +395,7 @@ __device__ __forceinline__ void trace_fast_one(const TraceParams &p, uint32_t ri
           for (uint32_t g = in.geo_begin; g < in.geo_end && !done; g++) {
             const GeoRec &geo = p.geos[g];
             if (!geometry_passes(fl, geo.flags)) continue;
+            const bool opq = geometry_opaque(fl, geo.flags);
             float tg;
             if (!slab(bo, binv, near_s, far_ * s, geo.mn[0], geo.mn[1], geo.mn[2], geo.mx[0], geo.mx[1],
                       geo.mx[2], tg))
@@ -414,14 +425,18 @@ __device__ __forceinline__ void trace_fast_one(const TraceParams &p, uint32_t ri
                   bool back;
                   if (tri_core(bo, bd, near_s, far_ * s, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(c.x, c.y, c.z),
                                f3(d.x, d.y, d.z), a.w, b.w, c.w, d.w, ce, cb, t, u, v, back)) {
-                    far_ = t / s;
-                    best.k = k;
-                    best.geo = g;
-                    best.inst = i;
-                    best.u = u;
-                    best.v = v;
-                    best.back = back;
-                    if (fl & F_END_SEARCH) {
+                    uint32_t beh = opq ? RDRT_ANYHIT_ACCEPT : anyhit_behavior(p, false, p.tri_ids[k]);
+                    if (beh & RDRT_ANYHIT_ACCEPT) {
+                      far_ = t / s;
+                      best.k = k;
+                      best.geo = g;
+                      best.inst = i;
+                      best.u = u;
+                      best.v = v;
+                      best.back = back;
+                      if (fl & F_END_SEARCH) beh |= RDRT_ANYHIT_END_SEARCH;
+                    }
+                    if (beh & RDRT_ANYHIT_END_SEARCH) {
                       done = true;
                       break;
                     }
@@ -532,6 +547,7 @@ __device__ __forceinline__ uint32_t trace_coop_one(const TraceParams &p, uint32_
     for (uint32_t g = in.geo_begin; g < in.geo_end && !done; g++) {
       const GeoRec &geo = p.geos[g];
       if (!geometry_passes(fl, geo.flags)) continue;
+      const bool opq = geometry_opaque(fl, geo.flags);
       float tg;
       if (!slab(bo, binv, near_s, far_ * s, geo.mn[0], geo.mn[1], geo.mn[2], geo.mx[0], geo.mx[1], geo.mx[2], tg))
         continue;
@@ -557,15 +573,19 @@ __device__ __forceinline__ uint32_t trace_coop_one(const TraceParams &p, uint32_
             bool back;
             if (tri_core(bo, bd, near_s, far_ * s, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(c.x, c.y, c.z),
                          f3(d.x, d.y, d.z), a.w, b.w, c.w, d.w, ce, cb, t, u, v, back)) {
-              far_ = t / s;
-              best_t = far_;
-              best.k = k;
-              best.geo = g;
-              best.inst = i;
-              best.u = u;
-              best.v = v;
-              best.back = back;
-              if (fl & F_END_SEARCH) {
+              uint32_t beh = opq ? RDRT_ANYHIT_ACCEPT : anyhit_behavior(p, false, p.tri_ids[k]);
+              if (beh & RDRT_ANYHIT_ACCEPT) {
+                far_ = t / s;
+                best_t = far_;
+                best.k = k;
+                best.geo = g;
+                best.inst = i;
+                best.u = u;
+                best.v = v;
+                best.back = back;
+                if (fl & F_END_SEARCH) beh |= RDRT_ANYHIT_END_SEARCH;
+              }
+              if (beh & RDRT_ANYHIT_END_SEARCH) {
                 done = true;
                 break;
               }

@@ -63,6 +63,9 @@ struct TraceParams {
   const RefNode *tlas_forest;
   const uint32_t *indices;
   const float *vertices;
+  // any-hit behaviour (RDRT_ANYHIT_*) per global triangle of hits on
+  // geometry that is not opaque for the ray, or null (accept every hit)
+  const uint8_t *anyhit;
 };
 
 __host__ __device__ inline uint32_t slots_of(const TraceParams &p) { return p.slots ? p.slots : p.n; }

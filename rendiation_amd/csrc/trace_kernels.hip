@@ -65,6 +65,7 @@ __global__ __launch_bounds__(B) void k_trace_ref(TraceParams p) {
       for (uint32_t g = in.geo_begin; g < in.geo_end && !done; g++) {
         const GeoRec &geo = p.geos[g];
         if (!geometry_passes(fl, geo.flags)) continue;
+        const bool opq = geometry_opaque(fl, geo.flags);
         uint32_t bc = geo.ref_root;
         while (bc != kInvalid && !done) {
           cn++;
@@ -89,15 +90,19 @@ __global__ __launch_bounds__(B) void k_trace_ref(TraceParams p) {
             bool back;
             if (tri_core(bo, bd, r.tmin * s, far_ * s, v0, e1, e2, nn, uu, uv, vv, inverse_d, ce, cb, t, u, v,
                          back)) {
-              far_ = t / s;
-              best.k = k;
-              best.geo = g;
-              best.inst = i;
-              best.t = far_;
-              best.u = u;
-              best.v = v;
-              best.back = back;
-              if (fl & F_END_SEARCH) {
+              uint32_t beh = anyhit_behavior(p, opq, tri);
+              if (beh & RDRT_ANYHIT_ACCEPT) {
+                far_ = t / s;
+                best.k = k;
+                best.geo = g;
+                best.inst = i;
+                best.t = far_;
+                best.u = u;
+                best.v = v;
+                best.back = back;
+                if (fl & F_END_SEARCH) beh |= RDRT_ANYHIT_END_SEARCH;
+              }
+              if (beh & RDRT_ANYHIT_END_SEARCH) {
                 done = true;
                 break;
               }

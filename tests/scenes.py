@@ -81,10 +81,11 @@ def instanced_scene(seed: int = 3, nrays: int = 50_000):
     return blas_list, tlas_list, binding, rays, meta
 
 
-def build_pair(blas_list, tlas_list, binding, opts):
+def build_pair(blas_list, tlas_list, binding, opts, anyhit=None):
     """Same scene in the product system and in the oracle.
     blas_list: list of list of (pos, idx, flags); tlas_list: list of list of
-    instance dicts (transform, blas, custom, mask, flags)."""
+    instance dicts (transform, blas, custom, mask, flags); anyhit: optional
+    {(blas, geometry): per-triangle any-hit behaviours}."""
     sys = AccelerationStructureSystem(device=0, options=opts)
     orc = OracleScene(opts.blas_bin_size, opts.blas_max_depth, opts.tlas_bin_size, opts.tlas_max_depth, opts.sah_bins)
     bh = []
@@ -100,6 +101,9 @@ def build_pair(blas_list, tlas_list, binding, opts):
                 instance_shader_binding_table_record_offset=d.get("sbt", 0), flags=d.get("flags", 0))
             for d in insts]))
         orc.add_tlas(insts)
+    for (b, g), table in (anyhit or {}).items():
+        sys.set_anyhit(bh[b], g, table)
+        orc.set_anyhit(b, g, table)
     sys.bind_tlas([th[i] for i in binding])
     orc.bind(binding)
     sys.create_comp_instance()
