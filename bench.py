@@ -567,6 +567,7 @@ def differentiable_geometry(args):
 
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)
+    free0 = torch.cuda.mem_get_info(dev)[0]  # device memory free before anything of this workload
     n = 1000
     t0 = time.perf_counter()
     hp, hi, hn = synth.height_field(n, seed=0)
@@ -638,6 +639,9 @@ def differentiable_geometry(args):
 
     ms_f, ev_f = timed(forward)
     ms_fb, ev_fb = timed(forward_backward)
+    # every buffer of the workload is allocated by now (the library's are
+    # grow-only, torch's cached): the device memory it holds at its peak
+    peak_mb = (free0 - torch.cuda.mem_get_info(dev)[0]) / 2 ** 20
     st = tracer.stats()
     trav = _pt_traversal(tracer, cams, cfg, tune.get("wide", 0))
     npaths = len(cams) * cfg.width * cfg.height * cfg.spp
@@ -650,6 +654,8 @@ def differentiable_geometry(args):
         "fwd_bwd_over_fwd": round(ms_fb / ms_f, 4), "loss": float(loss.item()),
         "extension_rays": st["extension_rays"], "scene_gen_ms": round(gen_ms, 1),
         "build_upload_ms": round(build_upload_ms, 1), "launch_shape": tune,
+        "peak_device_mb": round(peak_mb, 1), "device_scene_mb": round(info.device_bytes / 2 ** 20, 1),
+        "path_vertices": "recomputed per bounce by a replay of the recorded frame (8 B per path and bounce kept)",
     }
     extra = {"forward_traversal": trav}
     if not args.no_cpu_baseline:
