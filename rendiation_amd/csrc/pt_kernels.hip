@@ -311,10 +311,6 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_backward(PtLaunch P, const floa
   }
 }
 
-__global__ void k_fill_f32(float *p, uint32_t n) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.0f;
-}
-
 __global__ void k_pack_materials(const float *albedo, const float *emission, uint32_t n, Material *out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
