@@ -64,6 +64,14 @@ __device__ __forceinline__ bool take_child(const Graph &gr, bool pred, uint32_t 
   tg::cleanup_in(gr, done, cgroup, c);
   return done;
 }
+// The same for a child of the trace group (every lane).
+__device__ __forceinline__ bool take_trace(const Graph &gr, bool pred, uint32_t c, float &value) {
+  const Group &T = gr.g[kAoTrace];
+  const bool done = pred && c != tg::kNone && T.state[c] == tg::kFinished;
+  if (done) value = __uint_as_float(pw(T, c)[1]);
+  tg::cleanup(T, done, c);
+  return done;
+}
 
 // ---- group 0: trace ----
 // First half of the poll: tasks waiting on their closest-hit / miss child
@@ -120,7 +128,7 @@ __global__ void k_trace_post(AoLaunch L) {
       T.child[idx] = c;
       T.stage[idx] = 1u;
     }
-    tg::end_poll(L.gr, kAoTrace, valid, idx, false);
+    tg::end_poll_in(L.gr, kAoTrace, valid, idx, false, tg::kNone);  // pending: nothing wakes
   }
 }
 
@@ -157,7 +165,7 @@ __device__ __forceinline__ void accumulate(const AoLaunch &L, uint32_t x, uint32
   *a = make_uchar4(q, q, q, 255);
 }
 
-__global__ void k_raygen(AoLaunch L) {
+__device__ __forceinline__ void poll_raygen(const AoLaunch &L) {
   const Group &G = L.gr.g[kAoRayGen];
   const uint32_t s = __float_as_uint(L.frame[23]);
   tg::for_each_polled(G, [&](uint32_t, bool valid, uint32_t idx) {
@@ -184,9 +192,9 @@ __global__ void k_raygen(AoLaunch L) {
     }
     const bool waiting = valid && !first;
     float payload = 0.0f;
-    const bool done = take_child(L.gr, waiting, kAoTrace, waiting ? G.child[idx] : tg::kNone, payload);
+    const bool done = take_trace(L.gr, waiting, waiting ? G.child[idx] : tg::kNone, payload);
     if (done) accumulate(L, x, y, s, payload);
-    tg::end_poll(L.gr, kAoRayGen, valid, idx, done);
+    tg::end_poll_in(L.gr, kAoRayGen, valid, idx, done, tg::kNone);  // roots
   });
 }
 
@@ -227,7 +235,7 @@ __device__ __forceinline__ void occlusion_ray(const AoLaunch &L, uint32_t p, V3 
   dir = pt::tbn_apply(sn, load3(L.frame + 20));
 }
 
-__global__ void k_closest(AoLaunch L) {
+__device__ __forceinline__ void poll_closest(const AoLaunch &L) {
   const Group &G = L.gr.g[kAoClosest];
   tg::for_each_polled(G, [&](uint32_t, bool valid, uint32_t idx) {
     const bool first = valid && G.stage[idx] == 0u;
@@ -241,23 +249,35 @@ __global__ void k_closest(AoLaunch L) {
     }
     const bool waiting = valid && !first;
     float v = 0.0f;
-    const bool done = take_child(L.gr, waiting, kAoTrace, waiting ? G.child[idx] : tg::kNone, v);
+    const bool done = take_trace(L.gr, waiting, waiting ? G.child[idx] : tg::kNone, v);
     if (done) pw(G, idx)[0] = __float_as_uint(v);
-    tg::end_poll(L.gr, kAoClosest, valid, idx, done);
+    tg::end_poll_in(L.gr, kAoClosest, valid, idx, done, kAoTrace);
   });
 }
 
 // ---- groups 3 and 4: store a constant and resolve (ao.rs:299-314) ----
-__global__ void k_store(AoLaunch L, uint32_t gid, float value) {
+__device__ __forceinline__ void poll_store(const AoLaunch &L, uint32_t gid, float value) {
   const Group &G = L.gr.g[gid];
   tg::for_each_polled(G, [&](uint32_t, bool valid, uint32_t idx) {
     if (valid) pw(G, idx)[0] = __float_as_uint(value);
-    tg::end_poll(L.gr, gid, valid, idx, true);
+    tg::end_poll_in(L.gr, gid, valid, idx, true, kAoTrace);
   });
 }
 
-__global__ void k_clear_counts(uint32_t *c, uint32_t n) {
-  if (threadIdx.x < n) c[threadIdx.x] = 0u;
+// Groups 1-4 of one round in one launch.  Polled in order they would run
+// one launch each; here every block polls its share of each group in turn.
+// The result is the same: these groups spawn only trace tasks (group 0),
+// wake only trace tasks, and read back only finished trace tasks, which no
+// poll of groups 1-4 finishes; none of them spawns, wakes or frees a task of
+// another of them, so no poll sees what another one does in this round.
+// `clear`: the first `clear` trace-count words start at zero (first round
+// of a tile, before any trace poll reads them).
+__global__ void k_poll_stages(AoLaunch L, uint32_t clear) {
+  if (blockIdx.x == 0 && threadIdx.x < clear) L.tcount[threadIdx.x] = 0u;
+  poll_raygen(L);
+  poll_closest(L);
+  poll_store(L, kAoClosestTest, 0.0f);
+  poll_store(L, kAoMiss, 1.0f);
 }
 
 }  // namespace
@@ -269,7 +289,6 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
   Graph &gr = L.gr;
   hipError_t e = tg::enqueue_reset(gr, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(64), 0, st, L.tcount, first_tile ? 6u : 4u);
   L.qpar = 0u;
   e = tg::enqueue_spawn_fresh(gr, kAoRayGen, L.tile_w * tile_h, st);
   if (e != hipSuccess) return e;
@@ -284,9 +303,11 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
   tp.n = ntrace + 1;
   tp.queue = L.tq0;
   tp.count = L.tcount;  // per round below
-  // each queue holds at most one ray per pixel of the tile
+  // each queue holds at most one ray per pixel of the tile, and so do both
+  // together: a pixel's chain has one trace task at a time in traversal
   const uint32_t per_queue = L.tile_w * tile_h;
   tp.slots = per_queue;
+  tp.live_max = per_queue;
   tp.flags = RDRT_RAY_FLAG_CULL_BACK_FACING_TRIANGLES;  // ao.rs:209
   tp.cull_mask = 0xffffffffu;
   tp.tlas_idx = 0;
@@ -298,36 +319,42 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
   tp.count2 = L.tcount + 1;  // per round below
   tp.slots2 = per_queue;
   tp.flags2 = RDRT_RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH;  // ao.rs:276
-  auto poll = [&](uint32_t gid) -> hipError_t {
-    const uint32_t grid = grid_for(gr.g[gid].size);
-    switch (gid) {
-      case kAoTrace: {
-        hipLaunchKernelGGL(k_trace_pre, dim3(grid), dim3(kBlock), 0, st, L);
+  // Each round polls the trace group, then groups 1-4 (one launch, see
+  // k_poll_stages), as the executor's round does (mod.rs:396-460); each
+  // list launch compacts the groups just polled and prepares the next ones.
+  // The first round of a tile skips the trace group: right after the reset
+  // and the root spawns it holds no task, so its poll would do nothing.
+  // Only rounds 2 and 3 trace: the camera rays that the ray gen spawns in
+  // round 1, the occlusion rays that the closest hit spawns in round 2; no
+  // later poll spawns a trace, so later trace polls only resolve tasks and
+  // run without the traversal and post launches.
+  constexpr uint32_t kStages = (1u << kAoRayGen) | (1u << kAoClosest) | (1u << kAoClosestTest) | (1u << kAoMiss);
+  constexpr uint32_t kTrace = 1u << kAoTrace;
+  uint32_t stage_grid = 1;
+  for (uint32_t g = kAoRayGen; g <= kAoMiss; g++) stage_grid = std::max(stage_grid, grid_for(gr.g[g].size));
+  for (uint32_t r = 0; r < rounds; r++) {
+    if (r > 0) {
+      e = tg::enqueue_lists(gr, kStages, kTrace, st);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(k_trace_pre, dim3(grid_for(ntrace)), dim3(kBlock), 0, st, L);
+      if (r <= 2) {
         TraceParams q = tp;
         q.count = L.tcount + 2 * L.qpar;
         q.count2 = L.tcount + 2 * L.qpar + 1;
-        hipError_t e = launch_trace(q, RDRT_TRAVERSAL_FAST, push_depth, tune, st);
+        e = launch_trace(q, RDRT_TRAVERSAL_FAST, push_depth, tune, st);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_trace_post, dim3(grid_for(per_queue * 2)), dim3(kBlock), 0, st, L);
-        L.qpar ^= 1u;  // the next round's trace poll uses the other counts
-        break;
       }
-      case kAoRayGen:
-        hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(kBlock), 0, st, L);
-        break;
-      case kAoClosest:
-        hipLaunchKernelGGL(k_closest, dim3(grid), dim3(kBlock), 0, st, L);
-        break;
-      case kAoClosestTest:
-        hipLaunchKernelGGL(k_store, dim3(grid), dim3(kBlock), 0, st, L, gid, 0.0f);
-        break;
-      default:
-        hipLaunchKernelGGL(k_store, dim3(grid), dim3(kBlock), 0, st, L, gid, 1.0f);
-        break;
+      L.qpar ^= 1u;  // the next round's trace poll uses the other counts
     }
-    return hipGetLastError();
-  };
-  e = tg::enqueue_rounds(gr, rounds, poll, st);
+    e = tg::enqueue_lists(gr, r > 0 ? kTrace : 0u, kStages, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_poll_stages, dim3(stage_grid), dim3(kBlock), 0, st, L,
+                       r == 0 ? (first_tile ? 6u : 4u) : 0u);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  e = tg::enqueue_lists(gr, kStages, 0u, st);
   if (e != hipSuccess) return e;
   if (after) *after = gr;
   return hipSuccess;

@@ -5,11 +5,12 @@
 // with the caller's HIP kernels as the ray-gen, closest-hit and miss stages.
 //
 // Groups: 0 = trace (the library's), 1 = ray gen, then the closest-hit
-// stages, then the miss stages.  Each round polls them in that order; a
-// stage poll is three launches -- pre (take returned traces), the caller's
-// stage kernel, post (spawn the traces it asked for, resolve the rest) --
-// and a trace poll is pre (queue new rays, hand back finished payloads), the
+// stages, then the miss stages.  Each round polls them in that order.  A
+// trace poll is pre (queue new rays, hand back finished payloads), the
 // traversal, post (spawn the hit group's closest-hit task or the miss task).
+// The stage groups of a round take one pre launch for all of them (take
+// returned traces), the caller's stage kernels in group order, and one post
+// launch (spawn the traces they asked for, resolve the rest).
 #ifndef RDRT_PIPELINE_H
 #define RDRT_PIPELINE_H
 

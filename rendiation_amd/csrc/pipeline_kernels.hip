@@ -101,7 +101,7 @@ __global__ void k_pipe_trace_post(PipeLaunch L) {
 // spawn index into the launch index and zeroes its payload; a task whose
 // trace has finished takes the trace's payload into trace_result, frees it
 // and counts the poll; every polled task's action is preset to resolve.
-__global__ void k_stage_pre(PipeLaunch L, uint32_t gid) {
+__device__ __forceinline__ void stage_pre(const PipeLaunch &L, uint32_t gid) {
   const Group &G = L.gr.g[gid];
   const Group &T = L.gr.g[kPipeTrace];
   const StageArrays &S = L.st[gid];
@@ -126,7 +126,7 @@ __global__ void k_stage_pre(PipeLaunch L, uint32_t gid) {
 // After the caller's kernel: a task that asked for a trace spawns the trace
 // task (its ray, flags, hit-group config and payload) and sleeps on it;
 // every other polled task resolves (its parent trace task wakes).
-__global__ void k_stage_post(PipeLaunch L, uint32_t gid) {
+__device__ __forceinline__ void stage_post(const PipeLaunch &L, uint32_t gid) {
   const Group &G = L.gr.g[gid];
   const Group &T = L.gr.g[kPipeTrace];
   const StageArrays &S = L.st[gid];
@@ -145,13 +145,27 @@ __global__ void k_stage_post(PipeLaunch L, uint32_t gid) {
       L.tawait[c] = tg::kNone;
     }
     if (tr) G.child[idx] = c;
-    // a trace that found no free slot (error bit set) resolves instead of sleeping for ever
-    tg::end_poll(L.gr, gid, valid, idx, valid && (!tr || c == tg::kNone));
+    // a trace that found no free slot (error bit set) resolves instead of
+    // sleeping for ever; a stage task's parent is a trace task or none
+    tg::end_poll_in(L.gr, gid, valid, idx, valid && (!tr || c == tg::kNone), kPipeTrace);
   });
 }
 
-__global__ void k_clear_pipe_counts(uint32_t *c, uint32_t n) {
-  if (threadIdx.x < n) c[threadIdx.x] = 0u;
+// The pre passes of every stage group in one launch, and their post passes
+// in another, with the caller's stage kernels between them in group order.
+// Polled one group at a time, pre(g), stage(g), post(g) would run before
+// pre(g + 1); the result is the same: stage tasks spawn, wake and read back
+// only trace tasks, and no stage pass finishes a trace task or returns a
+// freed trace slot to the empty stack, so what one stage group's passes do
+// is invisible to another's in the same round.  The caller's kernels still
+// run in group order.  `clear`: the first `clear` trace-count words start
+// at zero (first round of a tile, before any trace poll reads them).
+__global__ void k_stages_pre(PipeLaunch L, uint32_t clear) {
+  if (blockIdx.x == 0 && threadIdx.x < clear) L.tcount[threadIdx.x] = 0u;
+  for (uint32_t g = kPipeRayGen; g < L.gr.n; g++) stage_pre(L, g);
+}
+__global__ void k_stages_post(PipeLaunch L) {
+  for (uint32_t g = kPipeRayGen; g < L.gr.n; g++) stage_post(L, g);
 }
 
 }  // namespace
@@ -164,7 +178,6 @@ hipError_t enqueue_pipeline_tile(const PipeLaunch &L0, const void *const *stages
   Graph &gr = L.gr;
   hipError_t e = tg::enqueue_reset(gr, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_clear_pipe_counts, dim3(1), dim3(64), 0, st, L.tcount, first_tile ? 3u : 2u);
   L.qpar = 0u;
   e = tg::enqueue_spawn_fresh(gr, kPipeRayGen, L.tile_w * tile_h, st);
   if (e != hipSuccess) return e;
@@ -180,66 +193,86 @@ hipError_t enqueue_pipeline_tile(const PipeLaunch &L0, const void *const *stages
   tp.queue = L.tq;
   tp.slots = ntrace;
   tp.work = work;
-  auto poll = [&](uint32_t gid) -> hipError_t {
-    const Group &G = gr.g[gid];
-    const uint32_t grid = grid_for(G.size);
-    if (gid == kPipeTrace) {
-      hipLaunchKernelGGL(k_pipe_trace_pre, dim3(grid), dim3(kBlock), 0, st, L);
-      TraceParams q = tp;
-      q.count = L.tcount + L.qpar;
-      hipError_t e2 = launch_trace(q, RDRT_TRAVERSAL_FAST, push_depth, tune, st);
-      if (e2 != hipSuccess) return e2;
-      if (L.attrs || L.xforms) {
-        AttribArgs a = attrib;
-        a.rays = L.rays;
-        a.meta = L.meta;
-        a.hits = L.hits;
-        a.hit_meta = L.hit_meta;
-        a.sbt = L.sbt;
-        a.queue = L.tq;
-        a.count = q.count;
-        a.n = ntrace;
-        a.out = L.attrs;
-        a.xf_out = L.xforms;
-        e2 = enqueue_hit_attributes(a, st);
-        if (e2 != hipSuccess) return e2;
-      }
-      hipLaunchKernelGGL(k_pipe_trace_post, dim3(grid), dim3(kBlock), 0, st, L);
-      L.qpar ^= 1u;  // the next round's trace poll uses the other count
-      return hipGetLastError();
-    }
-    hipLaunchKernelGGL(k_stage_pre, dim3(grid), dim3(kBlock), 0, st, L, gid);
-    const StageArrays &S = L.st[gid];
-    rdrt_stage_args a;
-    a.count = G.ctr + tg::kSnapshot;
-    a.slots = G.cur ? G.list1 : G.list0;
-    a.payload = S.payload;
-    a.trace_result = S.result;
-    a.polls = G.stage;
-    a.launch = S.launch;
-    a.trace_of = S.trace_of;
-    a.rays = L.rays;
-    a.ray_meta = L.meta;
-    a.hits = L.hits;
-    a.hit_meta = L.hit_meta;
-    a.attributes = L.attrs;
-    a.transforms = L.xforms;
-    a.action = S.action;
-    a.call = S.call;
-    a.call_payload = S.call_payload;
-    a.user = user;
-    a.launch_size[0] = L.width;
-    a.launch_size[1] = L.height;
-    a.payload_words = L.words;
-    a.capacity = G.size;
-    void *args[] = {&a};
-    hipError_t e2 =
-        hipLaunchKernel(stages[gid], dim3((G.size + kStageBlock - 1) / kStageBlock), dim3(kStageBlock), args, 0, st);
+  auto poll_trace = [&]() -> hipError_t {
+    hipLaunchKernelGGL(k_pipe_trace_pre, dim3(grid_for(ntrace)), dim3(kBlock), 0, st, L);
+    TraceParams q = tp;
+    q.count = L.tcount + L.qpar;
+    hipError_t e2 = launch_trace(q, RDRT_TRAVERSAL_FAST, push_depth, tune, st);
     if (e2 != hipSuccess) return e2;
-    hipLaunchKernelGGL(k_stage_post, dim3(grid), dim3(kBlock), 0, st, L, gid);
+    if (L.attrs || L.xforms) {
+      AttribArgs a = attrib;
+      a.rays = L.rays;
+      a.meta = L.meta;
+      a.hits = L.hits;
+      a.hit_meta = L.hit_meta;
+      a.sbt = L.sbt;
+      a.queue = L.tq;
+      a.count = q.count;
+      a.n = ntrace;
+      a.out = L.attrs;
+      a.xf_out = L.xforms;
+      e2 = enqueue_hit_attributes(a, st);
+      if (e2 != hipSuccess) return e2;
+    }
+    hipLaunchKernelGGL(k_pipe_trace_post, dim3(grid_for(ntrace)), dim3(kBlock), 0, st, L);
+    L.qpar ^= 1u;  // the next round's trace poll uses the other count
     return hipGetLastError();
   };
-  e = tg::enqueue_rounds(gr, rounds, poll, st);
+  auto poll_stages = [&](uint32_t clear) -> hipError_t {
+    uint32_t grid = 1;
+    for (uint32_t g = kPipeRayGen; g < gr.n; g++) grid = std::max(grid, grid_for(gr.g[g].size));
+    hipLaunchKernelGGL(k_stages_pre, dim3(grid), dim3(kBlock), 0, st, L, clear);
+    for (uint32_t g = kPipeRayGen; g < gr.n; g++) {
+      const Group &G = gr.g[g];
+      const StageArrays &S = L.st[g];
+      rdrt_stage_args a;
+      a.count = G.ctr + tg::kSnapshot;
+      a.slots = G.cur ? G.list1 : G.list0;
+      a.payload = S.payload;
+      a.trace_result = S.result;
+      a.polls = G.stage;
+      a.launch = S.launch;
+      a.trace_of = S.trace_of;
+      a.rays = L.rays;
+      a.ray_meta = L.meta;
+      a.hits = L.hits;
+      a.hit_meta = L.hit_meta;
+      a.attributes = L.attrs;
+      a.transforms = L.xforms;
+      a.action = S.action;
+      a.call = S.call;
+      a.call_payload = S.call_payload;
+      a.user = user;
+      a.launch_size[0] = L.width;
+      a.launch_size[1] = L.height;
+      a.payload_words = L.words;
+      a.capacity = G.size;
+      void *args[] = {&a};
+      hipError_t e2 =
+          hipLaunchKernel(stages[g], dim3((G.size + kStageBlock - 1) / kStageBlock), dim3(kStageBlock), args, 0, st);
+      if (e2 != hipSuccess) return e2;
+    }
+    hipLaunchKernelGGL(k_stages_post, dim3(grid), dim3(kBlock), 0, st, L);
+    return hipGetLastError();
+  };
+  // Each round polls the trace group, then the stage groups (see
+  // k_stages_pre), as the executor's round does (mod.rs:396-460); each list
+  // launch compacts the groups just polled and prepares the next ones.  The
+  // first round of a tile skips the trace group: right after the reset and
+  // the root spawns it holds no task, so its poll would do nothing.
+  const uint32_t trace_mask = 1u << kPipeTrace;
+  const uint32_t stage_mask = ((gr.n >= 32 ? 0u : 1u << gr.n) - 1u) & ~trace_mask;
+  for (uint32_t r = 0; r < rounds; r++) {
+    if (r > 0) {
+      e = tg::enqueue_lists(gr, stage_mask, trace_mask, st);
+      if (e == hipSuccess) e = poll_trace();
+      if (e != hipSuccess) return e;
+    }
+    e = tg::enqueue_lists(gr, r > 0 ? trace_mask : 0u, stage_mask, st);
+    if (e == hipSuccess) e = poll_stages(r == 0 ? (first_tile ? 3u : 2u) : 0u);
+    if (e != hipSuccess) return e;
+  }
+  e = tg::enqueue_lists(gr, stage_mask, 0u, st);
   if (e != hipSuccess) return e;
   if (after) *after = gr;
   return hipSuccess;

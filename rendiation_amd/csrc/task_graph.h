@@ -223,6 +223,28 @@ __device__ __forceinline__ void end_poll(const Graph &gr, uint32_t gid, bool val
   wake(gr, fin && p != kNone, pt, p);
 }
 
+// end_poll for a group whose tasks' parents all belong to group pg (kNone:
+// tasks without parents), which saves the per-group passes of wake.
+__device__ __forceinline__ void end_poll_in(const Graph &gr, uint32_t gid, bool valid, uint32_t idx, bool resolved,
+                                            uint32_t pg) {
+  const Group &G = gr.g[gid];
+  uint32_t p = kNone;
+  if (valid) {
+    if (resolved) {
+      G.state[idx] = kFinished;
+      p = G.parent[idx];
+    } else {
+      G.state[idx] = kGoToSleep;
+    }
+  }
+  const bool fin = valid && resolved;
+  cleanup(G, fin && p == kNone, idx);
+  if (pg < gr.n) {
+    const bool push = fin && p != kNone && atomicExch(gr.g[pg].state + p, kWaken) != kGoToSleep;
+    push_active(gr.g[pg], push, p);
+  }
+}
+
 // Block-uniform loop over the entries a poll covers: body(i, valid, idx).
 template <class F>
 __device__ __forceinline__ void for_each_polled(const Group &G, F body) {
@@ -246,6 +268,9 @@ hipError_t enqueue_compact(Graph &gr, uint32_t gid, hipStream_t st);
 // The compaction of group gc followed by the prepare of group gp (either
 // kNone), in one launch when they are different groups.
 hipError_t enqueue_list(Graph &gr, uint32_t gc, uint32_t gp, hipStream_t st);
+// The compactions of the groups whose bits are set in cmask, then the
+// prepares of those in pmask; one launch when no group is in both.
+hipError_t enqueue_lists(Graph &gr, uint32_t cmask, uint32_t pmask, hipStream_t st);
 
 // `rounds` rounds of execute (mod.rs:396-460): per group, prepare, then
 // poll(gid) (which enqueues the group's poll launches), then compact; each

@@ -80,11 +80,14 @@ __device__ __forceinline__ void compact_part(const Group &G) {
   }
 }
 
-// The compaction of group gc and the prepare of another group gp in one
-// launch (either may be kNone): nothing of one touches the other group.
-__global__ void k_list(Graph gr, uint32_t gc, uint32_t gp) {
-  if (gc != kNone) compact_part(gr.g[gc]);
-  if (gp != kNone) prepare_part(gr.g[gp]);
+// The compactions of the groups in cmask and the prepares of the groups in
+// pmask in one launch (disjoint masks): nothing of one group's pass touches
+// another group.
+__global__ void k_list(Graph gr, uint32_t cmask, uint32_t pmask) {
+  for (uint32_t k = 0; k < gr.n; k++)
+    if (cmask >> k & 1u) compact_part(gr.g[k]);
+  for (uint32_t k = 0; k < gr.n; k++)
+    if (pmask >> k & 1u) prepare_part(gr.g[k]);
 }
 
 // dispatch_allocate_init_task (mod.rs:223-303) right after a prepare:
@@ -179,19 +182,28 @@ hipError_t enqueue_reset(Graph &gr, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t enqueue_list(Graph &gr, uint32_t gc, uint32_t gp, hipStream_t st) {
-  if (gc == kNone && gp == kNone) return hipSuccess;
-  if (gc != kNone && gc == gp) {  // one group: compact, then prepare
-    hipError_t e = enqueue_list(gr, gc, kNone, st);
-    return e != hipSuccess ? e : enqueue_list(gr, kNone, gp, st);
+hipError_t enqueue_lists(Graph &gr, uint32_t cmask, uint32_t pmask, hipStream_t st) {
+  const uint32_t all = gr.n >= 32 ? ~0u : (1u << gr.n) - 1u;
+  cmask &= all;
+  pmask &= all;
+  if (!cmask && !pmask) return hipSuccess;
+  if (cmask & pmask) {  // a group in both: its compaction first, then its prepare
+    hipError_t e = enqueue_lists(gr, cmask, 0u, st);
+    return e != hipSuccess ? e : enqueue_lists(gr, 0u, pmask, st);
   }
   uint32_t n = 0;
-  if (gc != kNone) n = gr.g[gc].size;
-  if (gp != kNone && gr.g[gp].size > n) n = gr.g[gp].size;
-  hipLaunchKernelGGL(k_list, dim3(grid_for(n, kListGrid)), dim3(kBlock), 0, st, gr, gc, gp);
-  if (gc != kNone) gr.g[gc].cur ^= 1u;
-  if (gp != kNone) gr.g[gp].pp ^= 1u;
+  for (uint32_t k = 0; k < gr.n; k++)
+    if (((cmask | pmask) >> k & 1u) && gr.g[k].size > n) n = gr.g[k].size;
+  hipLaunchKernelGGL(k_list, dim3(grid_for(n, kListGrid)), dim3(kBlock), 0, st, gr, cmask, pmask);
+  for (uint32_t k = 0; k < gr.n; k++) {
+    if (cmask >> k & 1u) gr.g[k].cur ^= 1u;
+    if (pmask >> k & 1u) gr.g[k].pp ^= 1u;
+  }
   return hipGetLastError();
+}
+
+hipError_t enqueue_list(Graph &gr, uint32_t gc, uint32_t gp, hipStream_t st) {
+  return enqueue_lists(gr, gc < gr.n ? 1u << gc : 0u, gp < gr.n ? 1u << gp : 0u, st);
 }
 
 hipError_t enqueue_prepare(Graph &gr, uint32_t gid, hipStream_t st) { return enqueue_list(gr, kNone, gid, st); }

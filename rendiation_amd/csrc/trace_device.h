@@ -748,10 +748,12 @@ hipError_t launch_fast_kw(const TraceParams &p0, bool persist, uint32_t coop, hi
   p.coop_max = p.counters ? 0u : coop;  // visit counting stays one ray per lane
   if (p.coop_max) {
     // enough groups for every live ray when the launch qualifies
-    const uint32_t slots = slots_of(p) + (p.count2 ? p.slots2 : 0u);
-    const uint32_t groups = std::min(p.coop_max, slots);
+    uint32_t live = slots_of(p) + (p.count2 ? p.slots2 : 0u);
+    if (p.live_max && p.live_max < live) live = p.live_max;
+    const uint32_t groups = std::min(p.coop_max, live);
     const uint32_t grid = (groups * kCoopLanes + B - 1) / B;
     if (grid) hipLaunchKernelGGL((k_trace_coop<K, S, B, W, LAYOUT>), dim3(grid), dim3(B), 0, st, p);
+    if (live <= p.coop_max) return hipGetLastError();  // the cooperative kernel always qualifies
   }
   if (persist) {
     if (!p.work) return hipErrorInvalidValue;

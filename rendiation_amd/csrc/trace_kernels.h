@@ -42,6 +42,10 @@ struct TraceParams {
   // at most coop_max live rays, the cooperative kernel traces the launch and
   // the one-ray-per-lane kernel returns at once; 0 = never.
   uint32_t coop_max;
+  // A bound the caller knows on the live rays of both segments together
+  // (0 = their slots): when coop_max reaches it, only the cooperative kernel
+  // is launched.
+  uint32_t live_max;
 
   const PairNode *tri_pairs;
   const TriPre *tris;
