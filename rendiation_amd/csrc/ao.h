@@ -47,10 +47,11 @@ struct AoLaunch {
   rdrt_ray *rays;
   rdrt_hit *hits;
   uint2 *hitx;
-  uint32_t *tq0, *tq1;  // this round's primary / occlusion rays to traverse
+  uint32_t *tq0, *tq1;  // the next traversal's primary / occlusion rays (queued at spawn), qcap each
   uint32_t *tcount;     // [2q], [2q + 1]: their counts in rounds of parity q; [4], [5]: primary /
                         // occlusion rays traced by the call
-  uint32_t qpar;        // parity of the current round (host-kept)
+  uint32_t qpar;        // parity of the queues being filled / traced (host-kept)
+  uint32_t qcap;
   const float *frame;   // kAoFrameWords
   uint32_t width, height, tile_x, tile_y, tile_w;
   void *accum;  // f32 per pixel, or RGBA8 per pixel

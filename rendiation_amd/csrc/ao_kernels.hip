@@ -1,7 +1,7 @@
 // ao_kernels.hip -- the poll kernels of the ambient-occlusion pipeline on the
 // device task graph (scene/rendering/gpu-ray-tracing/src/feature/
-// ao.rs:192-304), and the trace task's poll split around the traversal
-// launch (shader/ray-tracing/src/backend/wavefront_compute/trace_task.rs).
+// ao.rs:192-304), and the trace task's poll around the traversal launch
+// (shader/ray-tracing/src/backend/wavefront_compute/trace_task.rs).
 #include "ao.h"
 #include "pt_device.h"
 
@@ -40,17 +40,25 @@ __device__ __forceinline__ V3 load3(const float *p) { return v3(p[0], p[1], p[2]
 __device__ __forceinline__ uint32_t *pw(const Group &G, uint32_t idx) { return G.payload + (size_t)idx * G.words; }
 
 // Starts a trace task (then_trace) for the lanes where `pred` holds: the
-// ray goes to the trace group's ray array at the new task's slot.
+// ray goes to the trace group's ray array at the new task's slot, and the
+// slot to the traversal queue of its ray type for the next trace poll (a
+// new trace task's first poll is its traversal; every task spawned now is in
+// that poll, so queueing it here is what that poll would do).  ray_type is
+// the same in every lane.
 __device__ __forceinline__ uint32_t spawn_trace(const AoLaunch &L, bool pred, uint32_t parent, uint32_t ptype,
                                                 uint32_t ray_type, float value, V3 o, float tmin, V3 d, float tmax) {
   const Group &T = L.gr.g[kAoTrace];
   const uint32_t c = tg::spawn(T, pred, parent, ptype);
-  if (c != tg::kNone) {
+  const bool ok = c != tg::kNone;
+  const uint32_t pos = tg::block_add(L.tcount + 2 * L.qpar + ray_type, ok);
+  if (ok) {
     uint32_t *w = pw(T, c);
     w[0] = ray_type;
     w[1] = __float_as_uint(value);
     w[2] = tg::kNone;
     write_ray(L.rays + c, o, tmin, d, tmax);
+    if (pos < L.qcap) (ray_type ? L.tq1 : L.tq0)[pos] = c;
+    else atomicOr(T.ctr + tg::kError, 2u);  // more rays than the queue holds (not reached: one per pixel)
   }
   return c;
 }
@@ -74,62 +82,45 @@ __device__ __forceinline__ bool take_trace(const Graph &gr, bool pred, uint32_t 
 }
 
 // ---- group 0: trace ----
-// First half of the poll: tasks waiting on their closest-hit / miss child
-// take its payload and resolve; new tasks are queued for the traversal.
-__global__ void k_trace_pre(AoLaunch L) {
+// The poll after the traversal of the rays queued at spawn: each task just
+// traced spawns the closest-hit task of its hit group (ao.rs:20-47: every
+// instance maps ray type 0 to the occlusion closest hit, ray type 1 to the
+// one that writes 0) or the miss task, passes it the payload and sleeps; a
+// task woken by its finished closest-hit / miss task takes that payload,
+// frees it and resolves.
+__global__ void k_trace_poll(AoLaunch L) {
   const Group &T = L.gr.g[kAoTrace];
-  uint32_t *cnt = L.tcount + 2 * L.qpar;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {  // nothing else writes these during the launch
+    L.tcount[4] += L.tcount[2 * L.qpar];
+    L.tcount[5] += L.tcount[2 * L.qpar + 1];
+  }
   // the other parity's counts (last used by the previous round, whose
-  // launches have all finished) start the next round at zero
+  // launches have all finished) start the queues of the next round at zero
   if (blockIdx.x == 0 && threadIdx.x < 2) L.tcount[2 * (L.qpar ^ 1u) + threadIdx.x] = 0u;
   tg::for_each_polled(T, [&](uint32_t, bool valid, uint32_t idx) {
     uint32_t *w = pw(T, idx);
     const bool fresh = valid && T.stage[idx] == 0u;
-    const bool occ = fresh && w[0] != 0u;
-    const uint32_t p0 = tg::block_add(cnt + 0, fresh && !occ);
-    const uint32_t p1 = tg::block_add(cnt + 1, occ);
-    if (fresh) (occ ? L.tq1 : L.tq0)[occ ? p1 : p0] = idx;
-    const bool waiting = valid && !fresh;
-    float v = 0.0f;
-    const bool done = take_child(L.gr, waiting, waiting ? w[2] : tg::kNone, waiting ? T.child[idx] : tg::kNone, v);
-    if (done) w[1] = __float_as_uint(v);
-    tg::end_poll(L.gr, kAoTrace, waiting, idx, done);
-  });
-}
-
-// Second half, after the traversal: each traced task spawns the closest-hit
-// task of its hit group (ao.rs:20-47: every instance maps ray type 0 to the
-// occlusion closest hit, ray type 1 to the one that writes 0) or the miss
-// task, passes it the payload and sleeps.
-__global__ void k_trace_post(AoLaunch L) {
-  const Group &T = L.gr.g[kAoTrace];
-  const uint32_t n0 = L.tcount[2 * L.qpar], n1 = L.tcount[2 * L.qpar + 1];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {  // nothing else writes these during the launch
-    L.tcount[4] += n0;
-    L.tcount[5] += n1;
-  }
-  for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < n0 + n1; j0 += gridDim.x * blockDim.x) {
-    const uint32_t j = j0 + threadIdx.x;
-    const bool valid = j < n0 + n1;
-    const uint32_t idx = !valid ? 0u : (j < n0 ? L.tq0[j] : L.tq1[j - n0]);
-    uint32_t *w = pw(T, idx);
-    const bool hit = valid && L.hits[idx].prim_id != RDRT_MISS;
-    const uint32_t cg = hit ? (w[0] == 0u ? kAoClosest : kAoClosestTest) : kAoMiss;
+    const bool hit = fresh && L.hits[idx].prim_id != RDRT_MISS;
+    const uint32_t cg = !fresh ? tg::kNone : hit ? (w[0] == 0u ? kAoClosest : kAoClosestTest) : kAoMiss;
     uint32_t c = tg::kNone;
     for (uint32_t g = kAoClosest; g <= kAoMiss; g++) {
-      const bool mine = valid && cg == g;
+      const bool mine = cg == g;
       if (!__syncthreads_or(mine)) continue;
       const uint32_t cc = tg::spawn(L.gr.g[g], mine, idx, kAoTrace);
       if (mine) c = cc;
     }
-    if (valid) {
+    if (fresh) {
       if (c != tg::kNone) pw(L.gr.g[cg], c)[0] = w[1];
       w[2] = cg;
       T.child[idx] = c;
       T.stage[idx] = 1u;
     }
-    tg::end_poll_in(L.gr, kAoTrace, valid, idx, false, tg::kNone);  // pending: nothing wakes
-  }
+    const bool waiting = valid && !fresh;
+    float v = 0.0f;
+    const bool done = take_child(L.gr, waiting, waiting ? w[2] : tg::kNone, waiting ? T.child[idx] : tg::kNone, v);
+    if (done) w[1] = __float_as_uint(v);
+    tg::end_poll(L.gr, kAoTrace, valid, idx, done);  // a fresh task sleeps on its child
+  });
 }
 
 // ---- group 1: ray generation (ao.rs:192-245) ----
@@ -270,10 +261,7 @@ __device__ __forceinline__ void poll_store(const AoLaunch &L, uint32_t gid, floa
 // wake only trace tasks, and read back only finished trace tasks, which no
 // poll of groups 1-4 finishes; none of them spawns, wakes or frees a task of
 // another of them, so no poll sees what another one does in this round.
-// `clear`: the first `clear` trace-count words start at zero (first round
-// of a tile, before any trace poll reads them).
-__global__ void k_poll_stages(AoLaunch L, uint32_t clear) {
-  if (blockIdx.x == 0 && threadIdx.x < clear) L.tcount[threadIdx.x] = 0u;
+__global__ void k_poll_stages(AoLaunch L) {
   poll_raygen(L);
   poll_closest(L);
   poll_store(L, kAoClosestTest, 0.0f);
@@ -287,11 +275,11 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
                            hipStream_t st, tg::Graph *after) {
   AoLaunch L = L0;  // its graph's parities advance as launches are enqueued
   Graph &gr = L.gr;
-  hipError_t e = tg::enqueue_reset(gr, st);
+  // empty graph, the tile's ray-gen roots, the queue counts at zero (and the
+  // call's ray totals on its first tile)
+  hipError_t e = tg::enqueue_reset_spawn(gr, kAoRayGen, L.tile_w * tile_h, L.tcount, first_tile ? 6u : 4u, st);
   if (e != hipSuccess) return e;
   L.qpar = 0u;
-  e = tg::enqueue_spawn_fresh(gr, kAoRayGen, L.tile_w * tile_h, st);
-  if (e != hipSuccess) return e;
   const uint32_t ntrace = gr.g[kAoTrace].size;
   TraceParams tp = scene;
   tp.rays = L.rays;
@@ -306,6 +294,7 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
   // each queue holds at most one ray per pixel of the tile, and so do both
   // together: a pixel's chain has one trace task at a time in traversal
   const uint32_t per_queue = L.tile_w * tile_h;
+  L.qcap = per_queue;
   tp.slots = per_queue;
   tp.live_max = per_queue;
   tp.flags = RDRT_RAY_FLAG_CULL_BACK_FACING_TRIANGLES;  // ao.rs:209
@@ -319,15 +308,15 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
   tp.count2 = L.tcount + 1;  // per round below
   tp.slots2 = per_queue;
   tp.flags2 = RDRT_RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH;  // ao.rs:276
-  // Each round polls the trace group, then groups 1-4 (one launch, see
-  // k_poll_stages), as the executor's round does (mod.rs:396-460); each
-  // list launch compacts the groups just polled and prepares the next ones.
-  // The first round of a tile skips the trace group: right after the reset
-  // and the root spawns it holds no task, so its poll would do nothing.
-  // Only rounds 2 and 3 trace: the camera rays that the ray gen spawns in
-  // round 1, the occlusion rays that the closest hit spawns in round 2; no
-  // later poll spawns a trace, so later trace polls only resolve tasks and
-  // run without the traversal and post launches.
+  // Each round polls the trace group (the traversal of the queued rays,
+  // then k_trace_poll), then groups 1-4 (one launch, see k_poll_stages), as
+  // the executor's round does (mod.rs:396-460); each list launch compacts
+  // the groups just polled and prepares the next ones.  The first round of
+  // a tile skips the trace group: right after the reset and the root spawns
+  // it holds no task, so its poll would do nothing.  Only rounds 2 and 3
+  // trace: the camera rays that the ray gen spawns in round 1, the
+  // occlusion rays that the closest hit spawns in round 2; no later poll
+  // spawns a trace, so later trace polls run without the traversal.
   constexpr uint32_t kStages = (1u << kAoRayGen) | (1u << kAoClosest) | (1u << kAoClosestTest) | (1u << kAoMiss);
   constexpr uint32_t kTrace = 1u << kAoTrace;
   uint32_t stage_grid = 1;
@@ -336,26 +325,25 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
     if (r > 0) {
       e = tg::enqueue_lists(gr, kStages, kTrace, st);
       if (e != hipSuccess) return e;
-      hipLaunchKernelGGL(k_trace_pre, dim3(grid_for(ntrace)), dim3(kBlock), 0, st, L);
       if (r <= 2) {
         TraceParams q = tp;
         q.count = L.tcount + 2 * L.qpar;
         q.count2 = L.tcount + 2 * L.qpar + 1;
         e = launch_trace(q, RDRT_TRAVERSAL_FAST, push_depth, tune, st);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_trace_post, dim3(grid_for(per_queue * 2)), dim3(kBlock), 0, st, L);
       }
-      L.qpar ^= 1u;  // the next round's trace poll uses the other counts
+      hipLaunchKernelGGL(k_trace_poll, dim3(grid_for(ntrace)), dim3(kBlock), 0, st, L);
+      L.qpar ^= 1u;  // spawns from here on queue for the next trace poll
     }
     e = tg::enqueue_lists(gr, r > 0 ? kTrace : 0u, kStages, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_poll_stages, dim3(stage_grid), dim3(kBlock), 0, st, L,
-                       r == 0 ? (first_tile ? 6u : 4u) : 0u);
+    hipLaunchKernelGGL(k_poll_stages, dim3(stage_grid), dim3(kBlock), 0, st, L);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  e = tg::enqueue_lists(gr, kStages, 0u, st);
-  if (e != hipSuccess) return e;
+  // The last stage poll's compaction is left out: the next tile starts
+  // from a reset, and reading the states back prepares and compacts every
+  // group first, which keeps the same tasks whether or not it ran.
   if (after) *after = gr;
   return hipSuccess;
 }

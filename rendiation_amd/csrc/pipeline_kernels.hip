@@ -158,10 +158,8 @@ __device__ __forceinline__ void stage_post(const PipeLaunch &L, uint32_t gid) {
 // only trace tasks, and no stage pass finishes a trace task or returns a
 // freed trace slot to the empty stack, so what one stage group's passes do
 // is invisible to another's in the same round.  The caller's kernels still
-// run in group order.  `clear`: the first `clear` trace-count words start
-// at zero (first round of a tile, before any trace poll reads them).
-__global__ void k_stages_pre(PipeLaunch L, uint32_t clear) {
-  if (blockIdx.x == 0 && threadIdx.x < clear) L.tcount[threadIdx.x] = 0u;
+// run in group order.
+__global__ void k_stages_pre(PipeLaunch L) {
   for (uint32_t g = kPipeRayGen; g < L.gr.n; g++) stage_pre(L, g);
 }
 __global__ void k_stages_post(PipeLaunch L) {
@@ -176,11 +174,11 @@ hipError_t enqueue_pipeline_tile(const PipeLaunch &L0, const void *const *stages
                                  tg::Graph *after) {
   PipeLaunch L = L0;  // its graph's parities advance as launches are enqueued
   Graph &gr = L.gr;
-  hipError_t e = tg::enqueue_reset(gr, st);
+  // empty graph, the tile's ray-gen roots, the trace counts at zero (and
+  // the call's traced-ray total on its first tile)
+  hipError_t e = tg::enqueue_reset_spawn(gr, kPipeRayGen, L.tile_w * tile_h, L.tcount, first_tile ? 3u : 2u, st);
   if (e != hipSuccess) return e;
   L.qpar = 0u;
-  e = tg::enqueue_spawn_fresh(gr, kPipeRayGen, L.tile_w * tile_h, st);
-  if (e != hipSuccess) return e;
   const uint32_t ntrace = gr.g[kPipeTrace].size;
   TraceParams tp = scene;
   tp.rays = L.rays;
@@ -218,10 +216,10 @@ hipError_t enqueue_pipeline_tile(const PipeLaunch &L0, const void *const *stages
     L.qpar ^= 1u;  // the next round's trace poll uses the other count
     return hipGetLastError();
   };
-  auto poll_stages = [&](uint32_t clear) -> hipError_t {
+  auto poll_stages = [&]() -> hipError_t {
     uint32_t grid = 1;
     for (uint32_t g = kPipeRayGen; g < gr.n; g++) grid = std::max(grid, grid_for(gr.g[g].size));
-    hipLaunchKernelGGL(k_stages_pre, dim3(grid), dim3(kBlock), 0, st, L, clear);
+    hipLaunchKernelGGL(k_stages_pre, dim3(grid), dim3(kBlock), 0, st, L);
     for (uint32_t g = kPipeRayGen; g < gr.n; g++) {
       const Group &G = gr.g[g];
       const StageArrays &S = L.st[g];
@@ -269,11 +267,12 @@ hipError_t enqueue_pipeline_tile(const PipeLaunch &L0, const void *const *stages
       if (e != hipSuccess) return e;
     }
     e = tg::enqueue_lists(gr, r > 0 ? trace_mask : 0u, stage_mask, st);
-    if (e == hipSuccess) e = poll_stages(r == 0 ? (first_tile ? 3u : 2u) : 0u);
+    if (e == hipSuccess) e = poll_stages();
     if (e != hipSuccess) return e;
   }
-  e = tg::enqueue_lists(gr, stage_mask, 0u, st);
-  if (e != hipSuccess) return e;
+  // The last stage poll's compaction is left out: the next tile starts
+  // from a reset, and reading the states back prepares and compacts every
+  // group first, which keeps the same tasks whether or not it ran.
   if (after) *after = gr;
   return hipSuccess;
 }

@@ -294,6 +294,11 @@ hipError_t enqueue_spawn_roots(Graph &gr, uint32_t gid, uint32_t count, hipStrea
 // atomics, counts written by one thread.  count <= the group's slots.
 hipError_t enqueue_spawn_fresh(Graph &gr, uint32_t gid, uint32_t count, hipStream_t st);
 
+// enqueue_reset then enqueue_spawn_fresh(gid, count) in one launch, which
+// also sets the caller's nzero words at `zero` (at most 1024) to zero.
+hipError_t enqueue_reset_spawn(Graph &gr, uint32_t gid, uint32_t count, uint32_t *zero, uint32_t nzero,
+                               hipStream_t st);
+
 // Generic task kinds (the C-ABI task graph): kind 0 resolves on its first
 // poll; kind 1 spawns one task of group `child_group` with payload
 // (own payload + 1) and itself as parent, sleeps, and once that child has

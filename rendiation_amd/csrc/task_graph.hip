@@ -141,6 +141,34 @@ __global__ void k_spawn_fresh(Graph gr, uint32_t gid, uint32_t k) {
   }
 }
 
+// k_reset followed by k_spawn_fresh(gid, k) in one pass (every write of
+// the two depends on its slot only), plus nzero caller words set to zero.
+__global__ void k_reset_spawn(Graph gr, uint32_t gid, uint32_t k, uint32_t *zero, uint32_t nzero) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nzero) zero[t] = 0u;
+  for (uint32_t g = 0; g < gr.n; g++) {
+    const Group &G = gr.g[g];
+    const uint32_t kk = g == gid ? k : 0u;
+    for (uint32_t i = t; i <= G.size; i += stride) {
+      if (i >= 1u && i <= kk) {  // root i - 1 takes slot i and active entry i - 1
+        G.state[i] = kWaken;
+        G.parent[i] = kNone;
+        G.ptype[i] = kNone;
+        G.stage[i] = 0u;
+        G.child[i] = kNone;
+        if (G.words) G.payload[(size_t)i * G.words] = i - 1u;
+        G.list0[i - 1u] = i;
+      } else {
+        G.state[i] = kNotExist;
+      }
+      if (i < G.size) G.empty[i] = G.size - i;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < kCtrWords)
+      G.ctr[threadIdx.x] = threadIdx.x == kEmpty0 ? G.size - kk : threadIdx.x == kActive0 ? kk : 0u;
+  }
+}
+
 // The generic kinds: 0 = BaseShaderFuture (ready at once); 1 = then(spawn a
 // child of `child_group`, await it) (future/then.rs semantics).
 __global__ void k_generic_poll(Graph gr, uint32_t gid, uint32_t kind, uint32_t child_group) {
@@ -221,6 +249,14 @@ hipError_t enqueue_spawn_fresh(Graph &gr, uint32_t gid, uint32_t count, hipStrea
   if (count > gr.g[gid].size) return hipErrorInvalidValue;
   if (count == 0) return hipSuccess;
   hipLaunchKernelGGL(k_spawn_fresh, dim3(grid_for(count, kListGrid)), dim3(kBlock), 0, st, gr, gid, count);
+  return hipGetLastError();
+}
+
+hipError_t enqueue_reset_spawn(Graph &gr, uint32_t gid, uint32_t count, uint32_t *zero, uint32_t nzero,
+                               hipStream_t st) {
+  if (gid >= gr.n || count > gr.g[gid].size || nzero > kBlock) return hipErrorInvalidValue;
+  for (uint32_t k = 0; k < gr.n; k++) gr.g[k].cur = gr.g[k].pp = 0u;
+  hipLaunchKernelGGL(k_reset_spawn, dim3(grid_for(max_size(gr))), dim3(kBlock), 0, st, gr, gid, count, zero, nzero);
   return hipGetLastError();
 }
 
