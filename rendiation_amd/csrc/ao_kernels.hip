@@ -97,7 +97,8 @@ __global__ void k_trace_poll(AoLaunch L) {
   // the other parity's counts (last used by the previous round, whose
   // launches have all finished) start the queues of the next round at zero
   if (blockIdx.x == 0 && threadIdx.x < 2) L.tcount[2 * (L.qpar ^ 1u) + threadIdx.x] = 0u;
-  tg::for_each_polled(T, [&](uint32_t, bool valid, uint32_t idx) {
+  tg::begin_phase(T);
+  tg::for_each_active(T, [&](uint32_t, bool valid, uint32_t idx) {
     uint32_t *w = pw(T, idx);
     const bool fresh = valid && T.stage[idx] == 0u;
     const bool hit = fresh && L.hits[idx].prim_id != RDRT_MISS;
@@ -119,7 +120,7 @@ __global__ void k_trace_poll(AoLaunch L) {
     float v = 0.0f;
     const bool done = take_child(L.gr, waiting, waiting ? w[2] : tg::kNone, waiting ? T.child[idx] : tg::kNone, v);
     if (done) w[1] = __float_as_uint(v);
-    tg::end_poll(L.gr, kAoTrace, valid, idx, done);  // a fresh task sleeps on its child
+    tg::end_poll(L.gr, kAoTrace, valid, idx, done, true);  // a fresh task sleeps on its child
   });
 }
 
@@ -159,7 +160,7 @@ __device__ __forceinline__ void accumulate(const AoLaunch &L, uint32_t x, uint32
 __device__ __forceinline__ void poll_raygen(const AoLaunch &L) {
   const Group &G = L.gr.g[kAoRayGen];
   const uint32_t s = __float_as_uint(L.frame[23]);
-  tg::for_each_polled(G, [&](uint32_t, bool valid, uint32_t idx) {
+  tg::for_each_active(G, [&](uint32_t, bool valid, uint32_t idx) {
     const uint32_t tp = valid ? pw(G, idx)[0] : 0u;
     const uint32_t x = L.tile_x + tp % L.tile_w, y = L.tile_y + tp / L.tile_w;
     const bool first = valid && G.stage[idx] == 0u;
@@ -185,7 +186,7 @@ __device__ __forceinline__ void poll_raygen(const AoLaunch &L) {
     float payload = 0.0f;
     const bool done = take_trace(L.gr, waiting, waiting ? G.child[idx] : tg::kNone, payload);
     if (done) accumulate(L, x, y, s, payload);
-    tg::end_poll_in(L.gr, kAoRayGen, valid, idx, done, tg::kNone);  // roots
+    tg::end_poll_in(L.gr, kAoRayGen, valid, idx, done, tg::kNone, true);  // roots
   });
 }
 
@@ -228,7 +229,7 @@ __device__ __forceinline__ void occlusion_ray(const AoLaunch &L, uint32_t p, V3 
 
 __device__ __forceinline__ void poll_closest(const AoLaunch &L) {
   const Group &G = L.gr.g[kAoClosest];
-  tg::for_each_polled(G, [&](uint32_t, bool valid, uint32_t idx) {
+  tg::for_each_active(G, [&](uint32_t, bool valid, uint32_t idx) {
     const bool first = valid && G.stage[idx] == 0u;
     V3 pos = v3(0.0f, 0.0f, 0.0f), dir = v3(0.0f, 0.0f, 1.0f);
     if (first) occlusion_ray(L, G.parent[idx], pos, dir);
@@ -242,16 +243,16 @@ __device__ __forceinline__ void poll_closest(const AoLaunch &L) {
     float v = 0.0f;
     const bool done = take_trace(L.gr, waiting, waiting ? G.child[idx] : tg::kNone, v);
     if (done) pw(G, idx)[0] = __float_as_uint(v);
-    tg::end_poll_in(L.gr, kAoClosest, valid, idx, done, kAoTrace);
+    tg::end_poll_in(L.gr, kAoClosest, valid, idx, done, kAoTrace, true);
   });
 }
 
 // ---- groups 3 and 4: store a constant and resolve (ao.rs:299-314) ----
 __device__ __forceinline__ void poll_store(const AoLaunch &L, uint32_t gid, float value) {
   const Group &G = L.gr.g[gid];
-  tg::for_each_polled(G, [&](uint32_t, bool valid, uint32_t idx) {
+  tg::for_each_active(G, [&](uint32_t, bool valid, uint32_t idx) {
     if (valid) pw(G, idx)[0] = __float_as_uint(value);
-    tg::end_poll_in(L.gr, gid, valid, idx, true, kAoTrace);
+    tg::end_poll_in(L.gr, gid, valid, idx, true, kAoTrace, true);
   });
 }
 
@@ -262,6 +263,7 @@ __device__ __forceinline__ void poll_store(const AoLaunch &L, uint32_t gid, floa
 // poll of groups 1-4 finishes; none of them spawns, wakes or frees a task of
 // another of them, so no poll sees what another one does in this round.
 __global__ void k_poll_stages(AoLaunch L) {
+  for (uint32_t g = kAoRayGen; g <= kAoMiss; g++) tg::begin_phase(L.gr.g[g]);
   poll_raygen(L);
   poll_closest(L);
   poll_store(L, kAoClosestTest, 0.0f);
@@ -310,21 +312,23 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
   tp.flags2 = RDRT_RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH;  // ao.rs:276
   // Each round polls the trace group (the traversal of the queued rays,
   // then k_trace_poll), then groups 1-4 (one launch, see k_poll_stages), as
-  // the executor's round does (mod.rs:396-460); each list launch compacts
-  // the groups just polled and prepares the next ones.  The first round of
-  // a tile skips the trace group: right after the reset and the root spawns
-  // it holds no task, so its poll would do nothing.  Only rounds 2 and 3
-  // trace: the camera rays that the ray gen spawns in round 1, the
-  // occlusion rays that the closest hit spawns in round 2; no later poll
-  // spawns a trace, so later trace polls run without the traversal.
+  // the executor's round does (mod.rs:396-460).  The two are the phases of
+  // task_graph.h, so no list launch runs between them: the trace poll
+  // spawns, wakes and frees only tasks of groups 1-4, and their polls only
+  // trace tasks; each pool holds every task of a tile (a pixel's chain has
+  // one task of each group but the trace group, which holds two per pixel),
+  // so no freed slot is taken again before the next tile's reset.  The
+  // first round of a tile skips the trace group: right after the reset and
+  // the root spawns it holds no task.  Only rounds 2 and 3 trace: the camera
+  // rays that the ray gen spawns in round 1, the occlusion rays that the
+  // closest hit spawns in round 2; no later poll spawns a trace, so later
+  // trace polls run without the traversal.
   constexpr uint32_t kStages = (1u << kAoRayGen) | (1u << kAoClosest) | (1u << kAoClosestTest) | (1u << kAoMiss);
   constexpr uint32_t kTrace = 1u << kAoTrace;
   uint32_t stage_grid = 1;
   for (uint32_t g = kAoRayGen; g <= kAoMiss; g++) stage_grid = std::max(stage_grid, grid_for(gr.g[g].size));
   for (uint32_t r = 0; r < rounds; r++) {
     if (r > 0) {
-      e = tg::enqueue_lists(gr, kStages, kTrace, st);
-      if (e != hipSuccess) return e;
       if (r <= 2) {
         TraceParams q = tp;
         q.count = L.tcount + 2 * L.qpar;
@@ -333,17 +337,14 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
         if (e != hipSuccess) return e;
       }
       hipLaunchKernelGGL(k_trace_poll, dim3(grid_for(ntrace)), dim3(kBlock), 0, st, L);
+      tg::end_phase(gr, kTrace);
       L.qpar ^= 1u;  // spawns from here on queue for the next trace poll
     }
-    e = tg::enqueue_lists(gr, r > 0 ? kTrace : 0u, kStages, st);
-    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_poll_stages, dim3(stage_grid), dim3(kBlock), 0, st, L);
+    tg::end_phase(gr, kStages);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  // The last stage poll's compaction is left out: the next tile starts
-  // from a reset, and reading the states back prepares and compacts every
-  // group first, which keeps the same tasks whether or not it ran.
   if (after) *after = gr;
   return hipSuccess;
 }

@@ -205,8 +205,10 @@ __device__ __forceinline__ void cleanup_in(const Graph &gr, bool pred, uint32_t 
 
 // The end of a poll (task_group.rs:118-150): a resolved task finishes and
 // wakes its parent (or frees itself when it has none); a pending one goes
-// to sleep.  `valid` is false for lanes past the end of the work.
-__device__ __forceinline__ void end_poll(const Graph &gr, uint32_t gid, bool valid, uint32_t idx, bool resolved) {
+// to sleep.  `valid` is false for lanes past the end of the work; `phased`
+// as for end_poll_in.
+__device__ __forceinline__ void end_poll(const Graph &gr, uint32_t gid, bool valid, uint32_t idx, bool resolved,
+                                         bool phased = false) {
   const Group &G = gr.g[gid];
   uint32_t p = kNone, pt = kNone;
   if (valid) {
@@ -215,7 +217,7 @@ __device__ __forceinline__ void end_poll(const Graph &gr, uint32_t gid, bool val
       p = G.parent[idx];
       pt = G.ptype[idx];
     } else {
-      G.state[idx] = kGoToSleep;
+      G.state[idx] = phased ? kSleep : kGoToSleep;
     }
   }
   const bool fin = valid && resolved;
@@ -225,8 +227,10 @@ __device__ __forceinline__ void end_poll(const Graph &gr, uint32_t gid, bool val
 
 // end_poll for a group whose tasks' parents all belong to group pg (kNone:
 // tasks without parents), which saves the per-group passes of wake.
+// `phased`: the poll runs in a phase (below), so a pending task sleeps at
+// once.
 __device__ __forceinline__ void end_poll_in(const Graph &gr, uint32_t gid, bool valid, uint32_t idx, bool resolved,
-                                            uint32_t pg) {
+                                            uint32_t pg, bool phased = false) {
   const Group &G = gr.g[gid];
   uint32_t p = kNone;
   if (valid) {
@@ -234,7 +238,7 @@ __device__ __forceinline__ void end_poll_in(const Graph &gr, uint32_t gid, bool 
       G.state[idx] = kFinished;
       p = G.parent[idx];
     } else {
-      G.state[idx] = kGoToSleep;
+      G.state[idx] = phased ? kSleep : kGoToSleep;
     }
   }
   const bool fin = valid && resolved;
@@ -255,6 +259,42 @@ __device__ __forceinline__ void for_each_polled(const Group &G, F body) {
     const bool valid = i < snap;
     body(i, valid, valid ? list[i] : 0u);
   }
+}
+
+// ---- phases: rounds without list launches ----
+// Some executions split their groups into phases such that, while a phase
+// polls its groups, none of their tasks is spawned or woken and none of
+// their empty stacks is popped (the trace group and the stage groups of a
+// ray-tracing pipeline whose stages spawn and wake only trace tasks, and
+// whose pools hold every task of a tile, so no freed slot is needed again
+// before the tile ends).  Then a phase needs no prepare and no compaction:
+// its active lists do not change while it runs, so every block reads the
+// entries to poll from the active count; no polled task is awake when the
+// phase ends, so a pending task sleeps at once (end_poll `phased`) and the
+// compacted list is empty: the phase zeroes the other buffer's count, and
+// the host flips the groups' buffers after the launch (end_phase).  Freed
+// slots collect on the removed lists until the next prepare.
+
+// Block-uniform loop over a phase's entries of group G: body(i, valid, idx).
+template <class F>
+__device__ __forceinline__ void for_each_active(const Group &G, F body) {
+  const uint32_t n = min(*active_count(G), G.size);
+  const uint32_t *list = active_list(G, G.cur);
+  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+    const uint32_t i = i0 + threadIdx.x;
+    const bool valid = i < n;
+    body(i, valid, valid ? list[i] : 0u);
+  }
+}
+// Called by every thread of a phase launch for each polled group: the
+// buffer the group's next pushes go to starts empty.
+__device__ __forceinline__ void begin_phase(const Group &G) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) G.ctr[kActive0 + (G.cur ^ 1u)] = 0u;
+}
+// Host: after a phase launch over the groups in mask.
+inline void end_phase(Graph &gr, uint32_t mask) {
+  for (uint32_t k = 0; k < gr.n; k++)
+    if (mask >> k & 1u) gr.g[k].cur ^= 1u;
 }
 
 // ---- host side ----
