@@ -37,7 +37,9 @@ __global__ void k_pipe_trace_pre(PipeLaunch L) {
   // the other parity's count (used by the previous round, all of whose
   // launches have finished) starts the next round at zero
   if (blockIdx.x == 0 && threadIdx.x == 0) L.tcount[L.qpar ^ 1u] = 0u;
-  tg::for_each_polled(T, [&](uint32_t, bool valid, uint32_t idx) {
+  tg::begin_phase(T);
+  tg::refill_empty(T);  // trace slots are taken and freed only by stage polls
+  tg::for_each_active(T, [&](uint32_t, bool valid, uint32_t idx) {
     const bool fresh = valid && T.stage[idx] == 0u;
     const uint32_t pos = tg::block_add(cnt, fresh);
     if (fresh) L.tq[pos] = idx;
@@ -47,7 +49,7 @@ __global__ void k_pipe_trace_pre(PipeLaunch L) {
     const bool done = waiting && cg < L.gr.n && c != tg::kNone && L.gr.g[cg].state[c] == tg::kFinished;
     if (done) copy_words(L.tpay + (size_t)idx * L.words, L.st[cg].payload + (size_t)c * L.words, L.words);
     tg::cleanup_in(L.gr, done, cg, c);
-    tg::end_poll(L.gr, kPipeTrace, waiting, idx, done);
+    tg::end_poll(L.gr, kPipeTrace, waiting, idx, done, true);
   });
 }
 
@@ -92,7 +94,7 @@ __global__ void k_pipe_trace_post(PipeLaunch L) {
       T.stage[idx] = 1u;
     }
     // no stage for the ray, or no slot left for its task (error bit set): resolve now
-    tg::end_poll(L.gr, kPipeTrace, valid, idx, valid && c == tg::kNone);
+    tg::end_poll(L.gr, kPipeTrace, valid, idx, valid && c == tg::kNone, true);
   }
 }
 
@@ -105,7 +107,7 @@ __device__ __forceinline__ void stage_pre(const PipeLaunch &L, uint32_t gid) {
   const Group &G = L.gr.g[gid];
   const Group &T = L.gr.g[kPipeTrace];
   const StageArrays &S = L.st[gid];
-  tg::for_each_polled(G, [&](uint32_t, bool valid, uint32_t idx) {
+  tg::for_each_active(G, [&](uint32_t, bool valid, uint32_t idx) {
     const uint32_t c = valid ? G.child[idx] : tg::kNone;
     if (gid == kPipeRayGen && valid && G.stage[idx] == 0u && c == tg::kNone) {
       const uint32_t k = G.payload[idx];  // spawn index within the tile
@@ -130,7 +132,7 @@ __device__ __forceinline__ void stage_post(const PipeLaunch &L, uint32_t gid) {
   const Group &G = L.gr.g[gid];
   const Group &T = L.gr.g[kPipeTrace];
   const StageArrays &S = L.st[gid];
-  tg::for_each_polled(G, [&](uint32_t, bool valid, uint32_t idx) {
+  tg::for_each_active(G, [&](uint32_t, bool valid, uint32_t idx) {
     const bool tr = valid && S.action[idx] == RDRT_STAGE_TRACE;
     const uint32_t c = tg::spawn(T, tr, idx, gid);
     if (c != tg::kNone) {
@@ -147,7 +149,7 @@ __device__ __forceinline__ void stage_post(const PipeLaunch &L, uint32_t gid) {
     if (tr) G.child[idx] = c;
     // a trace that found no free slot (error bit set) resolves instead of
     // sleeping for ever; a stage task's parent is a trace task or none
-    tg::end_poll_in(L.gr, gid, valid, idx, valid && (!tr || c == tg::kNone), kPipeTrace);
+    tg::end_poll_in(L.gr, gid, valid, idx, valid && (!tr || c == tg::kNone), kPipeTrace, true);
   });
 }
 
@@ -158,8 +160,17 @@ __device__ __forceinline__ void stage_post(const PipeLaunch &L, uint32_t gid) {
 // only trace tasks, and no stage pass finishes a trace task or returns a
 // freed trace slot to the empty stack, so what one stage group's passes do
 // is invisible to another's in the same round.  The caller's kernels still
-// run in group order.
+// run in group order.  The pre launch also fixes each group's snapshot
+// (the count the caller's kernel reads) and returns the freed slots of the
+// closest-hit and miss groups to their stacks: those slots are taken and
+// freed only by trace polls.  Ray-gen roots are spawned once per tile.
 __global__ void k_stages_pre(PipeLaunch L) {
+  for (uint32_t g = kPipeRayGen; g < L.gr.n; g++) {
+    const Group &G = L.gr.g[g];
+    tg::begin_phase(G);
+    if (g > kPipeRayGen) tg::refill_empty(G);
+    if (blockIdx.x == 0 && threadIdx.x == 0) G.ctr[tg::kSnapshot] = min(*tg::active_count(G), G.size);
+  }
   for (uint32_t g = kPipeRayGen; g < L.gr.n; g++) stage_pre(L, g);
 }
 __global__ void k_stages_post(PipeLaunch L) {
@@ -254,25 +265,27 @@ hipError_t enqueue_pipeline_tile(const PipeLaunch &L0, const void *const *stages
     return hipGetLastError();
   };
   // Each round polls the trace group, then the stage groups (see
-  // k_stages_pre), as the executor's round does (mod.rs:396-460); each list
-  // launch compacts the groups just polled and prepares the next ones.  The
-  // first round of a tile skips the trace group: right after the reset and
-  // the root spawns it holds no task, so its poll would do nothing.
+  // k_stages_pre), as the executor's round does (mod.rs:396-460).  The two
+  // are the phases of task_graph.h, so no list launch runs between them:
+  // the trace poll spawns, wakes and frees only stage tasks, and the stage
+  // polls only trace tasks (and ray-gen roots, freed once).  The first
+  // round of a tile skips the trace group: right after the reset and the
+  // root spawns it holds no task, so its poll would do nothing.
   const uint32_t trace_mask = 1u << kPipeTrace;
   const uint32_t stage_mask = ((gr.n >= 32 ? 0u : 1u << gr.n) - 1u) & ~trace_mask;
+  const uint32_t refill_mask = stage_mask & ~(1u << kPipeRayGen);
   for (uint32_t r = 0; r < rounds; r++) {
     if (r > 0) {
-      e = tg::enqueue_lists(gr, stage_mask, trace_mask, st);
-      if (e == hipSuccess) e = poll_trace();
+      e = poll_trace();
       if (e != hipSuccess) return e;
+      tg::end_phase(gr, trace_mask);
+      tg::flip_refilled(gr, trace_mask);
     }
-    e = tg::enqueue_lists(gr, r > 0 ? trace_mask : 0u, stage_mask, st);
-    if (e == hipSuccess) e = poll_stages();
+    e = poll_stages();
     if (e != hipSuccess) return e;
+    tg::end_phase(gr, stage_mask);
+    tg::flip_refilled(gr, refill_mask);
   }
-  // The last stage poll's compaction is left out: the next tile starts
-  // from a reset, and reading the states back prepares and compacts every
-  // group first, which keeps the same tasks whether or not it ran.
   if (after) *after = gr;
   return hipSuccess;
 }

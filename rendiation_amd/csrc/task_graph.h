@@ -296,6 +296,24 @@ inline void end_phase(Graph &gr, uint32_t mask) {
   for (uint32_t k = 0; k < gr.n; k++)
     if (mask >> k & 1u) gr.g[k].cur ^= 1u;
 }
+// Freed slots of G back on its empty stack during a launch in which no
+// slot of G is popped or freed (a phase of the other groups, when G's tasks
+// are spawned and freed only by those groups' polls): reads the counts of
+// parity pp, writes those of parity 1 - pp; the host flips pp after the
+// launch (flip_refilled).  Every thread of the launch calls it.
+__device__ __forceinline__ void refill_empty(const Group &G) {
+  const uint32_t ne = *empty_count(G), nr = *removed_count(G);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nr; i += gridDim.x * blockDim.x)
+    if (ne + i < G.size) G.empty[ne + i] = G.removed[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    G.ctr[kEmpty0 + (G.pp ^ 1u)] = ne + nr < G.size ? ne + nr : G.size;
+    G.ctr[kRemoved0 + (G.pp ^ 1u)] = 0u;
+  }
+}
+inline void flip_refilled(Graph &gr, uint32_t mask) {
+  for (uint32_t k = 0; k < gr.n; k++)
+    if (mask >> k & 1u) gr.g[k].pp ^= 1u;
+}
 
 // ---- host side ----
 
