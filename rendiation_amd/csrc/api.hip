@@ -388,9 +388,11 @@ int32_t rdrt_build(rdrt_ctx *c) {
     rc = need_device(c);
     if (rc) return rc;
     std::unique_ptr<DeviceTreeBuilder> trees = make_device_tree_builder(c->stream);
-    rc = build_scene(c->src, c->built, err, trees.get());
+    rc = build_scene(c->src, c->built, err, trees.get(), true);
   } else {
-    rc = build_scene(c->src, c->built, err);
+    // with a device, the commit computes the triangle records and quantized
+    // quads there instead of uploading host copies
+    rc = build_scene(c->src, c->built, err, nullptr, c->has_device);
   }
   if (rc) return fail(rc, err);
   c->built_valid = true;
@@ -451,10 +453,10 @@ int32_t rdrt_commit(rdrt_ctx *c) {
   RDRT_HIP(upload(d.tlas_pairs, b.tlas_pairs, d.bytes));
   RDRT_HIP(upload(d.tri_quads, b.tri_quads, d.bytes));
   RDRT_HIP(upload(d.tlas_quads, b.tlas_quads, d.bytes));
-  RDRT_HIP(upload(d.tri_qquads, b.tri_qquads, d.bytes));
+  if (!b.device_records) RDRT_HIP(upload(d.tri_qquads, b.tri_qquads, d.bytes));
   RDRT_HIP(upload(d.tlas_qquads, b.tlas_qquads, d.bytes));
   RDRT_HIP(upload(d.tlas_quad_roots, b.tlas_quad_roots, d.bytes));
-  RDRT_HIP(upload(d.tris, b.tris, d.bytes));
+  if (!b.device_records) RDRT_HIP(upload(d.tris, b.tris, d.bytes));
   RDRT_HIP(upload(d.redirect, b.redirect, d.bytes));
   RDRT_HIP(upload(d.tri_ids, b.tri_ids, d.bytes));
   RDRT_HIP(upload(d.indices, b.indices, d.bytes));
@@ -472,6 +474,20 @@ int32_t rdrt_commit(rdrt_ctx *c) {
   RDRT_HIP(upload(d.tlas_inst_first, b.tlas_inst_first, d.bytes));
   RDRT_HIP(upload(d.tri_material, b.tri_material, d.bytes));
   RDRT_HIP(upload(d.anyhit, b.anyhit, d.bytes));
+  if (b.device_records) {
+    // computed here from the arrays just uploaded (the same arithmetic as
+    // the host builder's), instead of on the host and copied
+    const size_t nslots = b.tri_ids.size(), nquads = b.tri_quads.size();
+    RDRT_HIP(hipMalloc(&d.tris.ptr, std::max<size_t>(nslots, 1) * sizeof(TriPre)));
+    d.tris.count = nslots;
+    d.bytes += std::max<size_t>(nslots, 1) * sizeof(TriPre);
+    RDRT_HIP(hipMalloc(&d.tri_qquads.ptr, std::max<size_t>(nquads, 1) * sizeof(QuadNodeQ)));
+    d.tri_qquads.count = nquads;
+    d.bytes += std::max<size_t>(nquads, 1) * sizeof(QuadNodeQ);
+    RDRT_HIP(enqueue_scene_records(d.vertices.ptr, d.indices.ptr, d.tri_ids.ptr, (uint32_t)nslots, d.tris.ptr,
+                                   d.tri_quads.ptr, (uint32_t)nquads, d.tri_qquads.ptr, c->stream));
+    RDRT_HIP(hipStreamSynchronize(c->stream));
+  }
   c->uploaded = true;
   return RDRT_OK;
 }
@@ -584,6 +600,7 @@ int32_t rdrt_copy_built(const rdrt_ctx *c, uint32_t which, void *out, size_t byt
   const void *src = nullptr;
   size_t need = 0;
   std::vector<uint32_t> order;
+  std::vector<QuadNodeQ> qq;
   switch (which) {
     case 0:
       src = b.tri_forest.data();
@@ -607,8 +624,15 @@ int32_t rdrt_copy_built(const rdrt_ctx *c, uint32_t which, void *out, size_t byt
       need = b.tri_quads.size() * sizeof(QuadNode);
       break;
     case 5:
-      src = b.tri_qquads.data();
-      need = b.tri_qquads.size() * sizeof(QuadNodeQ);
+      if (b.device_records) {  // the device holds them: the host twin of the same arithmetic
+        qq.resize(b.tri_quads.size());
+        for (size_t k = 0; k < qq.size(); k++) qq[k] = quantize_quad(b.tri_quads[k]);
+        src = qq.data();
+        need = qq.size() * sizeof(QuadNodeQ);
+      } else {
+        src = b.tri_qquads.data();
+        need = b.tri_qquads.size() * sizeof(QuadNodeQ);
+      }
       break;
     default:
       return fail(RDRT_ERR_INVALID_ARG, "unknown array");

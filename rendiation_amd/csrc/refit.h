@@ -35,6 +35,13 @@ hipError_t enqueue_refit(const RefitArgs &a, const uint32_t *forest_off, uint32_
                          const uint32_t *pair_off, uint32_t pair_levels, const uint32_t *quad_off,
                          uint32_t quad_levels, hipStream_t stream);
 
+// The BLAS triangle records (TriPre of triangle tri_ids[k] in slot k) and
+// the quantized twins of the child quads, computed on the device with the
+// host builder's arithmetic (a commit whose build left them to the device).
+hipError_t enqueue_scene_records(const float *vertices, const uint32_t *indices, const uint32_t *tri_ids,
+                                 uint32_t nslots, TriPre *tris, const QuadNode *quads, uint32_t nquads,
+                                 QuadNodeQ *qquads, hipStream_t stream);
+
 }  // namespace rdrt
 
 #endif

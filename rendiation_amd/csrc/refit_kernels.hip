@@ -242,7 +242,32 @@ __global__ void k_refit_geos(RefitArgs a) {
 
 uint32_t blocks(uint32_t n) { return (n + kRefitBlock - 1) / kRefitBlock; }
 
+// The records a commit computes on the device: the triangle record of every
+// fast slot, the quantized twin of every child quad.
+__global__ __launch_bounds__(kRefitBlock) void k_scene_records(const float *vertices, const uint32_t *indices,
+                                                               const uint32_t *tri_ids, uint32_t nslots, TriPre *tris,
+                                                               const QuadNode *quads, uint32_t nquads,
+                                                               QuadNodeQ *qquads) {
+  const uint32_t k = blockIdx.x * kRefitBlock + threadIdx.x;
+  if (k < nslots) {
+    const uint32_t t = tri_ids[k];
+    const float *v[3];
+    for (int j = 0; j < 3; j++) v[j] = vertices + 3 * (size_t)indices[3 * (size_t)t + j];
+    tris[k] = pre_triangle(v[0], v[1], v[2]);
+  }
+  if (k < nquads) qquads[k] = quantize(quads[k]);
+}
+
 }  // namespace
+
+hipError_t enqueue_scene_records(const float *vertices, const uint32_t *indices, const uint32_t *tri_ids,
+                                 uint32_t nslots, TriPre *tris, const QuadNode *quads, uint32_t nquads,
+                                 QuadNodeQ *qquads, hipStream_t st) {
+  const uint32_t n = nslots > nquads ? nslots : nquads;
+  if (n) hipLaunchKernelGGL(k_scene_records, dim3(blocks(n)), dim3(kRefitBlock), 0, st, vertices, indices, tri_ids,
+                            nslots, tris, quads, nquads, qquads);
+  return hipGetLastError();
+}
 
 hipError_t enqueue_refit(const RefitArgs &a, const uint32_t *forest_off, uint32_t forest_levels,
                          const uint32_t *pair_off, uint32_t pair_levels, const uint32_t *quad_off,

@@ -276,6 +276,8 @@ struct QuadConv {
   }
 };
 
+}  // namespace
+
 // Quantized twin of a child-quad node (see QuadNodeQ).  Per axis the step
 // is the smallest power of two for which every child's rounded bounds fit in
 // 8 bits; each bound is then moved outward until corner + q * step (FP32,
@@ -351,6 +353,8 @@ QuadNodeQ quantize_quad(const QuadNode &q) {
   for (int k = 0; k < 4; k++) out.ref[k] = q.ref[k];
   return out;
 }
+
+namespace {
 
 // Child-pair conversion of the fast kernel's TLAS tree, whose leaves hold
 // single instances addressed by their forest position: map[order index].  A
@@ -443,7 +447,7 @@ bool ref_ok(uint32_t ref, size_t npairs, size_t nprims) {
 
 bool validate_refs(const BuiltScene &s, std::string &err) {
   for (const PairNode &p : s.tri_pairs)
-    if (!ref_ok(p.lref, s.tri_pairs.size(), s.tris.size()) || !ref_ok(p.rref, s.tri_pairs.size(), s.tris.size())) {
+    if (!ref_ok(p.lref, s.tri_pairs.size(), s.tri_ids.size()) || !ref_ok(p.rref, s.tri_pairs.size(), s.tri_ids.size())) {
       err = "internal: BLAS child-pair ref out of range";
       return false;
     }
@@ -454,7 +458,7 @@ bool validate_refs(const BuiltScene &s, std::string &err) {
       return false;
     }
   for (const GeoRec &g : s.geos)
-    if (!ref_ok(g.root_ref, s.tri_pairs.size(), s.tris.size()) || g.ref_root >= s.tri_forest.size()) {
+    if (!ref_ok(g.root_ref, s.tri_pairs.size(), s.tri_ids.size()) || g.ref_root >= s.tri_forest.size()) {
       err = "internal: geometry root out of range";
       return false;
     }
@@ -470,7 +474,7 @@ bool validate_refs(const BuiltScene &s, std::string &err) {
     return q.ref[0] != kInvalid && q.ref[1] != kInvalid;
   };
   for (const QuadNode &q : s.tri_quads)
-    if (!quad_ok(q, s.tri_quads.size(), s.tris.size())) {
+    if (!quad_ok(q, s.tri_quads.size(), s.tri_ids.size())) {
       err = "internal: BLAS child-quad ref out of range";
       return false;
     }
@@ -486,12 +490,12 @@ bool validate_refs(const BuiltScene &s, std::string &err) {
         if (a[i].ref[k] != b[i].ref[k]) return false;
     return true;
   };
-  if (!twins(s.tri_quads, s.tri_qquads) || !twins(s.tlas_quads, s.tlas_qquads)) {
+  if ((!s.device_records && !twins(s.tri_quads, s.tri_qquads)) || !twins(s.tlas_quads, s.tlas_qquads)) {
     err = "internal: quantized quads disagree with the quads";
     return false;
   }
   for (const GeoRec &g : s.geos)
-    if (!ref_ok(g.quad_root, s.tri_quads.size(), s.tris.size())) {
+    if (!ref_ok(g.quad_root, s.tri_quads.size(), s.tri_ids.size())) {
       err = "internal: geometry quad root out of range";
       return false;
     }
@@ -521,7 +525,7 @@ bool validate_refs(const BuiltScene &s, std::string &err) {
     err = "internal: forest link or range out of range";
     return false;
   }
-  if (s.tri_ids.size() != s.redirect.size() || s.tris.size() != s.tri_ids.size()) {
+  if (s.tri_ids.size() != s.redirect.size() || (!s.device_records && s.tris.size() != s.tri_ids.size())) {
     err = "internal: triangle arrays disagree";
     return false;
   }
@@ -710,7 +714,8 @@ RefitLevels refit_levels(const BuiltScene &b) {
   return r;
 }
 
-int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTreeBuilder *trees) {
+int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTreeBuilder *trees,
+                bool device_records) {
   auto t0 = std::chrono::steady_clock::now();
   out = BuiltScene();
   unsigned hw = std::max(1u, std::thread::hardware_concurrency());
@@ -956,8 +961,10 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
     }
   }
   out.phase_ms[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();  // forest, pairs, quads
-  // triangle pre-pass, in the fast tree's leaf order
-  out.tris.resize(out.tri_ids.size());
+  // triangle pre-pass, in the fast tree's leaf order (here, or on the
+  // device at commit)
+  out.device_records = device_records;
+  if (!device_records) out.tris.resize(out.tri_ids.size());
   parallel_for((out.tris.size() + 65535) / 65536, hw, [&](size_t c) {
     size_t e = std::min(out.tris.size(), (c + 1) * 65536);
     for (size_t k = c * 65536; k < e; k++) {
@@ -970,10 +977,10 @@ int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTr
   });
 
   out.phase_ms[2] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();  // triangle records
-  // quantized twins of the child-quad nodes
-  out.tri_qquads.resize(out.tri_quads.size());
-  parallel_for((out.tri_quads.size() + 65535) / 65536, hw, [&](size_t c) {
-    const size_t e = std::min(out.tri_quads.size(), (c + 1) * 65536);
+  // quantized twins of the child-quad nodes (here, or on the device)
+  if (!device_records) out.tri_qquads.resize(out.tri_quads.size());
+  parallel_for((out.tri_qquads.size() + 65535) / 65536, hw, [&](size_t c) {
+    const size_t e = std::min(out.tri_qquads.size(), (c + 1) * 65536);
     for (size_t k = c * 65536; k < e; k++) out.tri_qquads[k] = quantize_quad(out.tri_quads[k]);
   });
   out.blas_box = blas_box;

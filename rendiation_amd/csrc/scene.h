@@ -82,7 +82,14 @@ struct BuiltScene {
   // cumulative ms at the end of: trees, forest + pairs + quads, triangle
   // records, quantized quads (then the TLAS layer and checks)
   double phase_ms[4] = {0.0, 0.0, 0.0, 0.0};
+  // true: `tris` and `tri_qquads` stay empty; the commit computes them on
+  // the device from the uploaded vertices, indices, tri_ids and tri_quads
+  bool device_records = false;
 };
+
+// Quantized twin of a child-quad node (host; the device twin is in
+// refit_kernels.hip).
+QuadNodeQ quantize_quad(const QuadNode &q);
 
 // Builds every BVH and both layouts.  Returns RDRT_OK or an RDRT_ERR_* code
 // with a message in `err`.
@@ -98,8 +105,10 @@ struct DeviceTreeBuilder {
 };
 
 // `trees`, when given, builds every BLAS tree (one geometry after another)
-// instead of the host builders.
-int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTreeBuilder *trees = nullptr);
+// instead of the host builders.  `device_records`: leave the BLAS triangle
+// records and quantized quads to the device (BuiltScene::device_records).
+int build_scene(const SceneSrc &src, BuiltScene &out, std::string &err, DeviceTreeBuilder *trees = nullptr,
+                bool device_records = false);
 // BLAS node indices of every layout grouped by depth, deepest level first:
 // the order in which a refit recomputes boxes bottom-up (a level only reads
 // the level below).  *_off[l]..*_off[l + 1] is level l of that order.
