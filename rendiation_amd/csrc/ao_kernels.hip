@@ -48,37 +48,29 @@ __device__ __forceinline__ uint32_t *pw(const Group &G, uint32_t idx) { return G
 __device__ __forceinline__ uint32_t spawn_trace(const AoLaunch &L, bool pred, uint32_t parent, uint32_t ptype,
                                                 uint32_t ray_type, float value, V3 o, float tmin, V3 d, float tmax) {
   const Group &T = L.gr.g[kAoTrace];
-  const uint32_t c = tg::spawn(T, pred, parent, ptype);
+  // tg::spawn, with its push to the active list and the push to the
+  // traversal queue issued together
+  const uint32_t c = tg::pop_empty(T, pred);
   const bool ok = c != tg::kNone;
-  const uint32_t pos = tg::block_add(L.tcount + 2 * L.qpar + ray_type, ok);
+  uint32_t pos[2];
+  tg::block_add_n<2>({tg::active_count(T), L.tcount + 2 * L.qpar + ray_type}, {ok, ok}, pos);
   if (ok) {
+    T.state[c] = tg::kWaken;
+    T.parent[c] = parent;
+    T.ptype[c] = ptype;
+    T.stage[c] = 0u;
+    T.child[c] = tg::kNone;
+    if (pos[0] < T.size) tg::active_list(T, T.cur)[pos[0]] = c;
+    else atomicOr(T.ctr + tg::kError, 2u);
     uint32_t *w = pw(T, c);
     w[0] = ray_type;
     w[1] = __float_as_uint(value);
     w[2] = tg::kNone;
     write_ray(L.rays + c, o, tmin, d, tmax);
-    if (pos < L.qcap) (ray_type ? L.tq1 : L.tq0)[pos] = c;
+    if (pos[1] < L.qcap) (ray_type ? L.tq1 : L.tq0)[pos[1]] = c;
     else atomicOr(T.ctr + tg::kError, 2u);  // more rays than the queue holds (not reached: one per pixel)
   }
   return c;
-}
-
-// For the lanes where `pred` holds: if the awaited child c of group cgroup
-// has finished, read back its payload value and free it (TaskFuture poll,
-// task_group.rs:464-488).  Returns whether it had.
-__device__ __forceinline__ bool take_child(const Graph &gr, bool pred, uint32_t cgroup, uint32_t c, float &value) {
-  const bool done = pred && cgroup < gr.n && c != tg::kNone && gr.g[cgroup].state[c] == tg::kFinished;
-  if (done) value = __uint_as_float(pw(gr.g[cgroup], c)[cgroup == kAoTrace ? 1 : 0]);
-  tg::cleanup_in(gr, done, cgroup, c);
-  return done;
-}
-// The same for a child of the trace group (every lane).
-__device__ __forceinline__ bool take_trace(const Graph &gr, bool pred, uint32_t c, float &value) {
-  const Group &T = gr.g[kAoTrace];
-  const bool done = pred && c != tg::kNone && T.state[c] == tg::kFinished;
-  if (done) value = __uint_as_float(pw(T, c)[1]);
-  tg::cleanup(T, done, c);
-  return done;
 }
 
 // ---- group 0: trace ----
@@ -100,9 +92,22 @@ __global__ void k_trace_poll(AoLaunch L) {
   tg::begin_phase(T);
   tg::for_each_active(T, [&](uint32_t, bool valid, uint32_t idx) {
     uint32_t *w = pw(T, idx);
-    const bool fresh = valid && T.stage[idx] == 0u;
-    const bool hit = fresh && L.hits[idx].prim_id != RDRT_MISS;
-    const uint32_t cg = !fresh ? tg::kNone : hit ? (w[0] == 0u ? kAoClosest : kAoClosestTest) : kAoMiss;
+    // every load that no block operation orders, issued up front: a poll is
+    // a chain of dependent round trips, and each barrier ends a batch
+    const uint32_t stage = valid ? T.stage[idx] : 1u;
+    const uint32_t w0 = valid ? w[0] : 0u, w1 = valid ? w[1] : 0u, w2 = valid ? w[2] : tg::kNone;
+    const uint32_t child = valid ? T.child[idx] : tg::kNone;
+    const uint32_t parent = valid ? T.parent[idx] : tg::kNone, ptype = valid ? T.ptype[idx] : tg::kNone;
+    const uint32_t prim = valid ? L.hits[idx].prim_id : RDRT_MISS;
+    const bool fresh = valid && stage == 0u;
+    const bool waiting = valid && !fresh;
+    // the awaited closest-hit / miss task (payload word 0 = its value)
+    const bool cvalid = waiting && w2 >= kAoClosest && w2 <= kAoMiss && child != tg::kNone;
+    const uint32_t cstate = cvalid ? L.gr.g[w2].state[child] : tg::kNotExist;
+    const uint32_t cval = cvalid ? pw(L.gr.g[w2], child)[0] : 0u;
+    // a task just traced: the closest-hit task of its hit group or the miss task
+    const bool hit = fresh && prim != RDRT_MISS;
+    const uint32_t cg = !fresh ? tg::kNone : hit ? (w0 == 0u ? kAoClosest : kAoClosestTest) : kAoMiss;
     uint32_t c = tg::kNone;
     for (uint32_t g = kAoClosest; g <= kAoMiss; g++) {
       const bool mine = cg == g;
@@ -111,16 +116,31 @@ __global__ void k_trace_poll(AoLaunch L) {
       if (mine) c = cc;
     }
     if (fresh) {
-      if (c != tg::kNone) pw(L.gr.g[cg], c)[0] = w[1];
+      if (c != tg::kNone) pw(L.gr.g[cg], c)[0] = w1;
       w[2] = cg;
       T.child[idx] = c;
       T.stage[idx] = 1u;
     }
-    const bool waiting = valid && !fresh;
-    float v = 0.0f;
-    const bool done = take_child(L.gr, waiting, waiting ? w[2] : tg::kNone, waiting ? T.child[idx] : tg::kNone, v);
-    if (done) w[1] = __float_as_uint(v);
-    tg::end_poll(L.gr, kAoTrace, valid, idx, done, true);  // a fresh task sleeps on its child
+    // a waiting task whose child finished: takes its value, frees it
+    // (TaskFuture poll, task_group.rs:464-488), resolves and wakes its
+    // parent (a ray-gen or closest-hit task, asleep in the other phase);
+    // the others sleep
+    const bool done = cvalid && cstate == tg::kFinished;
+    if (done) w[1] = cval;
+    if (valid) T.state[idx] = done ? tg::kFinished : tg::kSleep;
+    const Group &C2 = L.gr.g[kAoClosest], &C3 = L.gr.g[kAoClosestTest], &C4 = L.gr.g[kAoMiss];
+    const Group &P1 = L.gr.g[kAoRayGen], &P2 = L.gr.g[kAoClosest];
+    uint32_t pos[5];
+    tg::block_add_n<5>({tg::removed_count(C2), tg::removed_count(C3), tg::removed_count(C4), tg::active_count(P1),
+                        tg::active_count(P2)},
+                       {done && w2 == kAoClosest, done && w2 == kAoClosestTest, done && w2 == kAoMiss,
+                        done && ptype == kAoRayGen, done && ptype == kAoClosest},
+                       pos);
+    tg::free_at(C2, done && w2 == kAoClosest, child, pos[0]);
+    tg::free_at(C3, done && w2 == kAoClosestTest, child, pos[1]);
+    tg::free_at(C4, done && w2 == kAoMiss, child, pos[2]);
+    tg::wake_at(P1, done && ptype == kAoRayGen, parent, pos[3]);
+    tg::wake_at(P2, done && ptype == kAoClosest, parent, pos[4]);
   });
 }
 
@@ -160,10 +180,16 @@ __device__ __forceinline__ void accumulate(const AoLaunch &L, uint32_t x, uint32
 __device__ __forceinline__ void poll_raygen(const AoLaunch &L) {
   const Group &G = L.gr.g[kAoRayGen];
   const uint32_t s = __float_as_uint(L.frame[23]);
+  const Group &T = L.gr.g[kAoTrace];
   tg::for_each_active(G, [&](uint32_t, bool valid, uint32_t idx) {
     const uint32_t tp = valid ? pw(G, idx)[0] : 0u;
     const uint32_t x = L.tile_x + tp % L.tile_w, y = L.tile_y + tp / L.tile_w;
-    const bool first = valid && G.stage[idx] == 0u;
+    const uint32_t stage = valid ? G.stage[idx] : 1u, child = valid ? G.child[idx] : tg::kNone;
+    const bool first = valid && stage == 0u;
+    // the awaited trace (loaded before the block operations below)
+    const bool tvalid = valid && !first && child != tg::kNone;
+    const uint32_t tstate = tvalid ? T.state[child] : tg::kNotExist;
+    const uint32_t tval = tvalid ? pw(T, child)[1] : 0u;
     V3 cam = v3(0.0f, 0.0f, 0.0f), d = v3(0.0f, 0.0f, 1.0f);
     if (first) {
       uint32_t state = pt::xxhash32(x, y, s);
@@ -182,11 +208,15 @@ __device__ __forceinline__ void poll_raygen(const AoLaunch &L) {
       G.child[idx] = c;
       G.stage[idx] = 1u;
     }
-    const bool waiting = valid && !first;
-    float payload = 0.0f;
-    const bool done = take_trace(L.gr, waiting, waiting ? G.child[idx] : tg::kNone, payload);
-    if (done) accumulate(L, x, y, s, payload);
-    tg::end_poll_in(L.gr, kAoRayGen, valid, idx, done, tg::kNone, true);  // roots
+    // a finished trace: its value goes into the running mean, the trace and
+    // this root are freed (a root resolves without a parent)
+    const bool done = tvalid && tstate == tg::kFinished;
+    if (done) accumulate(L, x, y, s, __uint_as_float(tval));
+    if (valid && !done) G.state[idx] = tg::kSleep;
+    uint32_t pos[2];
+    tg::block_add_n<2>({tg::removed_count(T), tg::removed_count(G)}, {done, done}, pos);
+    tg::free_at(T, done, child, pos[0]);
+    tg::free_at(G, done, idx, pos[1]);
   });
 }
 
@@ -229,30 +259,49 @@ __device__ __forceinline__ void occlusion_ray(const AoLaunch &L, uint32_t p, V3 
 
 __device__ __forceinline__ void poll_closest(const AoLaunch &L) {
   const Group &G = L.gr.g[kAoClosest];
+  const Group &T = L.gr.g[kAoTrace];
   tg::for_each_active(G, [&](uint32_t, bool valid, uint32_t idx) {
-    const bool first = valid && G.stage[idx] == 0u;
+    const uint32_t stage = valid ? G.stage[idx] : 1u, child = valid ? G.child[idx] : tg::kNone;
+    const uint32_t parent = valid ? G.parent[idx] : tg::kNone;
+    const bool first = valid && stage == 0u;
+    const bool tvalid = valid && !first && child != tg::kNone;
+    const uint32_t tstate = tvalid ? T.state[child] : tg::kNotExist;
+    const uint32_t tval = tvalid ? pw(T, child)[1] : 0u;
     V3 pos = v3(0.0f, 0.0f, 0.0f), dir = v3(0.0f, 0.0f, 1.0f);
-    if (first) occlusion_ray(L, G.parent[idx], pos, dir);
+    if (first) occlusion_ray(L, parent, pos, dir);
     // payload 1: unoccluded unless the occlusion closest hit writes 0
     const uint32_t c = spawn_trace(L, first, idx, kAoClosest, 1u, 1.0f, pos, 0.01f, dir, 100.0f);
     if (first) {
       G.child[idx] = c;
       G.stage[idx] = 1u;
     }
-    const bool waiting = valid && !first;
-    float v = 0.0f;
-    const bool done = take_trace(L.gr, waiting, waiting ? G.child[idx] : tg::kNone, v);
-    if (done) pw(G, idx)[0] = __float_as_uint(v);
-    tg::end_poll_in(L.gr, kAoClosest, valid, idx, done, kAoTrace, true);
+    // a finished occlusion trace: its value becomes this task's payload,
+    // the trace is freed, this task resolves and wakes its camera trace
+    const bool done = tvalid && tstate == tg::kFinished;
+    if (done) pw(G, idx)[0] = tval;
+    if (valid) G.state[idx] = done ? tg::kFinished : tg::kSleep;
+    uint32_t p2[2];
+    tg::block_add_n<2>({tg::removed_count(T), tg::active_count(T)}, {done, done}, p2);
+    tg::free_at(T, done, child, p2[0]);
+    tg::wake_at(T, done, parent, p2[1]);
   });
 }
 
 // ---- groups 3 and 4: store a constant and resolve (ao.rs:299-314) ----
 __device__ __forceinline__ void poll_store(const AoLaunch &L, uint32_t gid, float value) {
   const Group &G = L.gr.g[gid];
+  const Group &T = L.gr.g[kAoTrace];
   tg::for_each_active(G, [&](uint32_t, bool valid, uint32_t idx) {
-    if (valid) pw(G, idx)[0] = __float_as_uint(value);
-    tg::end_poll_in(L.gr, gid, valid, idx, true, kAoTrace, true);
+    const uint32_t parent = valid ? G.parent[idx] : tg::kNone;
+    if (valid) {
+      pw(G, idx)[0] = __float_as_uint(value);
+      G.state[idx] = tg::kFinished;
+    }
+    // resolves and wakes its trace (asleep in the other phase)
+    const bool wakes = valid && parent != tg::kNone;  // always: store tasks are spawned by traces
+    uint32_t pos[1];
+    tg::block_add_n<1>({tg::active_count(T)}, {wakes}, pos);
+    tg::wake_at(T, wakes, parent, pos[0]);
   });
 }
 

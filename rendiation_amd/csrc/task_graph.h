@@ -109,6 +109,44 @@ __device__ __forceinline__ uint32_t block_rank(bool pred, uint32_t &total) {
   return r;
 }
 
+// N independent block_add calls in one pass: pos[k] = old value of ctr[k]
+// + this thread's rank among the threads where pred[k] holds.  The N
+// counter atomics are issued together (lane k of wave 0 takes counter k),
+// so the block waits for one round trip instead of N.  A null counter must
+// have pred false in every thread.
+template <int N>
+__device__ __forceinline__ void block_add_n(uint32_t *const (&ctr)[N], const bool (&pred)[N], uint32_t (&pos)[N]) {
+  __shared__ uint32_t off[N][kMaxBlock / 64 + 1];
+  __shared__ uint32_t base[N];
+  const uint32_t wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  uint64_t m[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) m[k] = __ballot(pred[k]);
+  if (lane_id() == 0) {
+#pragma unroll
+    for (int k = 0; k < N; k++) off[k][wid] = (uint32_t)__popcll(m[k]);
+  }
+  __syncthreads();
+  if (threadIdx.x < (uint32_t)N) {
+    const uint32_t k = threadIdx.x;
+    uint32_t *c = ctr[0];
+#pragma unroll
+    for (int j = 1; j < N; j++)
+      if (k == (uint32_t)j) c = ctr[j];
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < nw; w++) {
+      const uint32_t v = off[k][w];
+      off[k][w] = t;
+      t += v;
+    }
+    base[k] = t ? atomicAdd(c, t) : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; k++) pos[k] = base[k] + off[k][wid] + rank_in(m[k]);
+  __syncthreads();  // the arrays are reused by the next call
+}
+
 // Old counter value + this thread's rank among the threads where `pred`
 // holds (meaningful for those threads only).
 __device__ __forceinline__ uint32_t block_add(uint32_t *ctr, bool pred) {
@@ -291,6 +329,27 @@ __device__ __forceinline__ void for_each_active(const Group &G, F body) {
 __device__ __forceinline__ void begin_phase(const Group &G) {
   if (blockIdx.x == 0 && threadIdx.x == 0) G.ctr[kActive0 + (G.cur ^ 1u)] = 0u;
 }
+// Phased polls of executions in which every task awaits at most one child
+// at a time: the parent of a resolving task is asleep in a group of the
+// other phase and nothing else wakes it in this phase, so the wake is a
+// store and a push (no exchange to learn the previous state).  The caller
+// issues the push through block_add_n with its other counters and passes
+// the position here.
+__device__ __forceinline__ void wake_at(const Group &PG, bool pred, uint32_t pidx, uint32_t pos) {
+  if (!pred) return;
+  PG.state[pidx] = kWaken;
+  if (pos < PG.size) active_list(PG, PG.cur)[pos] = pidx;
+  else atomicOr(PG.ctr + kError, 2u);
+}
+// Frees slot idx of G (cleanup) at removed-list position pos from
+// block_add_n on removed_count(G).
+__device__ __forceinline__ void free_at(const Group &G, bool pred, uint32_t idx, uint32_t pos) {
+  if (!pred) return;
+  if (pos < G.size) G.removed[pos] = idx;
+  else atomicOr(G.ctr + kError, 4u);
+  G.state[idx] = kNotExist;
+}
+
 // Host: after a phase launch over the groups in mask.
 inline void end_phase(Graph &gr, uint32_t mask) {
   for (uint32_t k = 0; k < gr.n; k++)
