@@ -29,8 +29,8 @@ struct DevArray {
   size_t count = 0;
 };
 
-template <class T>
-hipError_t upload(DevArray<T> &a, const std::vector<T> &h, uint64_t &bytes) {
+template <class T, class A>
+hipError_t upload(DevArray<T> &a, const std::vector<T, A> &h, uint64_t &bytes) {
   a.count = h.size();
   size_t n = h.empty() ? 1 : h.size();  // keep a valid pointer for empty arrays
   hipError_t e = hipMalloc(&a.ptr, n * sizeof(T));

@@ -483,7 +483,7 @@ bool validate_refs(const BuiltScene &s, std::string &err) {
       err = "internal: TLAS child-quad ref out of range";
       return false;
     }
-  auto twins = [](const std::vector<QuadNode> &a, const std::vector<QuadNodeQ> &b) {
+  auto twins = [](const auto &a, const auto &b) {
     if (a.size() != b.size()) return false;
     for (size_t i = 0; i < a.size(); i++)
       for (int k = 0; k < 4; k++)
@@ -513,7 +513,7 @@ bool validate_refs(const BuiltScene &s, std::string &err) {
       err = "internal: instance geometry range out of range";
       return false;
     }
-  auto forest_ok = [](const std::vector<RefNode> &f, size_t nprims) {
+  auto forest_ok = [](const auto &f, size_t nprims) {
     for (const RefNode &n : f) {
       if (n.hit_next != kInvalid && n.hit_next >= f.size()) return false;
       if (n.miss_next != kInvalid && n.miss_next >= f.size()) return false;

@@ -3,7 +3,9 @@
 #define RDRT_SCENE_H
 
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/rdrt.h"
@@ -40,20 +42,46 @@ struct SceneSrc {
   TreeOptions tlas_opt{10, 50, 4};
 };
 
+// Allocator whose new elements are left uninitialized (resize does not zero
+// them).  The scene arrays below are sized once and then every element is
+// written by the parallel assembly; value-initializing them first was a
+// serial zero fill of ~1.3 GB (plus its page faults) for a scene of 8.4M
+// triangles, longer than the assembly itself.
+template <class T>
+struct UninitAlloc : std::allocator<T> {
+  using value_type = T;
+  template <class U>
+  struct rebind {
+    using other = UninitAlloc<U>;
+  };
+  UninitAlloc() = default;
+  template <class U>
+  UninitAlloc(const UninitAlloc<U> &) noexcept {}
+  template <class U>
+  void construct(U *) noexcept {}  // default-init: no zeroing
+  template <class U, class... Args>
+  void construct(U *p, Args &&...args) {
+    ::new ((void *)p) U(std::forward<Args>(args)...);
+  }
+};
+template <class T>
+using BigVec = std::vector<T, UninitAlloc<T>>;
+
 struct BuiltScene {
   // reference layout (naive/mod.rs:322-494)
-  std::vector<RefNode> tri_forest, tlas_forest;
-  std::vector<uint32_t> redirect, indices;
-  std::vector<uint32_t> tri_ids;  // fast-traversal triangle slot -> global triangle index
-  std::vector<float> vertices;  // xyz
+  BigVec<RefNode> tri_forest;
+  std::vector<RefNode> tlas_forest;
+  BigVec<uint32_t> redirect, indices;
+  BigVec<uint32_t> tri_ids;  // fast-traversal triangle slot -> global triangle index
+  BigVec<float> vertices;    // xyz
   // shading attributes: normals parallel to vertices (empty when no geometry
   // has any), material offset per triangle (empty when all are 0), largest
   // material offset per geometry
-  std::vector<float> normals;
-  std::vector<uint32_t> tri_material;
+  BigVec<float> normals;
+  BigVec<uint32_t> tri_material;
   std::vector<uint32_t> geo_max_material;
-  std::vector<uint8_t> anyhit;  // any-hit behaviour per triangle (empty when no geometry has a table)
-  std::vector<float> uvs;  // 2 per vertex when any geometry has them (zeros elsewhere), else empty
+  BigVec<uint8_t> anyhit;  // any-hit behaviour per triangle (empty when no geometry has a table)
+  BigVec<float> uvs;       // 2 per vertex when any geometry has them (zeros elsewhere), else empty
   // instance records by source position: inst_lookup[tlas_inst_first[t] + k]
   // is the record of instance k of TLAS t
   std::vector<uint32_t> inst_lookup, tlas_inst_first;
@@ -64,13 +92,16 @@ struct BuiltScene {
   std::vector<TlasRoot> tlas_roots;
   std::vector<uint32_t> binding;
   // child-pair layout
-  std::vector<PairNode> tri_pairs, tlas_pairs;
-  std::vector<TriPre> tris;
+  BigVec<PairNode> tri_pairs;
+  std::vector<PairNode> tlas_pairs;
+  BigVec<TriPre> tris;
   uint32_t max_blas_depth = 0, max_tlas_depth = 0;
   uint32_t blas_stack = 0, tlas_stack = 0;  // deepest push count of the child-pair trees
   // child-quad layout (collapsed from the child-pair trees; same leaves)
-  std::vector<QuadNode> tri_quads, tlas_quads;
-  std::vector<QuadNodeQ> tri_qquads, tlas_qquads;  // quantized twins (same indices and refs)
+  BigVec<QuadNode> tri_quads;
+  std::vector<QuadNode> tlas_quads;
+  BigVec<QuadNodeQ> tri_qquads;  // quantized twins (same indices and refs)
+  std::vector<QuadNodeQ> tlas_qquads;
   std::vector<uint32_t> tlas_quad_roots;      // per TLAS (kInvalid when deleted)
   uint32_t blas_stack4 = 0, tlas_stack4 = 0;  // deepest push count of the child-quad trees
   // per BLAS: object-space box (union of its geometries), whether it has
