@@ -20,8 +20,10 @@
 // for operation, so the decisions are the host's.  The host converts the
 // level-order nodes to the preorder Tree.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "device_build.h"
@@ -30,6 +32,24 @@
 
 namespace rdrt {
 namespace {
+
+// f(i) for i < n on the host's threads (each index once).
+template <class F>
+void host_for(size_t n, F f) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const unsigned threads = (unsigned)std::min<size_t>(hw, n);
+  if (threads <= 1) {
+    for (size_t i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::vector<std::thread> pool;
+  for (unsigned w = 0; w < threads; w++)
+    pool.emplace_back([&] {
+      for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+    });
+  for (auto &th : pool) th.join();
+}
 
 constexpr uint32_t kB = 256;          // threads per block
 constexpr uint32_t kFastBins = 16;    // FastTreeOptions::bins the device supports
@@ -969,16 +989,17 @@ class Builder final : public DeviceTreeBuilder {
     off_h_.assign(ng + 1, 0u);
     for (u32 g = 0; g < ng; g++) off_h_[g + 1] = off_h_[g] + (u32)geos[g]->size();
     const size_t n = off_h_[ng];
-    std::vector<float> h(6 * n);
-    std::vector<u32> gof(n);
-    for (u32 g = 0; g < ng; g++)
+    BigVec<float> h(6 * n);  // every element written below, by geometry in parallel
+    BigVec<u32> gof(n);
+    host_for(ng, [&](size_t g) {
       for (size_t i = 0; i < geos[g]->size(); i++) {
         const Box &b = (*geos[g])[i];
         const size_t k = off_h_[g] + i;
         std::memcpy(&h[6 * k], b.mn, 12);
         std::memcpy(&h[6 * k + 3], b.mx, 12);
-        gof[k] = g;
+        gof[k] = (u32)g;
       }
+    });
     DB_TRY(pbox_.reserve(6 * n * 4));
     DB_TRY(cen_.reserve(3 * n * 4));
     DB_TRY(order_.reserve(n * 4));
@@ -1138,24 +1159,25 @@ class Builder final : public DeviceTreeBuilder {
     DB_TRY(hipMemcpyAsync(gbase_.p, gbase.data(), (ng + 1) * 4, hipMemcpyHostToDevice, st_));
     hipLaunchKernelGGL(k_emit, dim3(grid_of(total)), dim3(kB), 0, st_, d, total, (const u32 *)off_.p,
                        (const u32 *)gbase_.p, (TreeNode *)emit_.p);
-    std::vector<TreeNode> nodes(total);
-    std::vector<u32> order(n), depth(total);
+    // host copies, filled by the copies below (no zero fill first)
+    BigVec<TreeNode> nodes(total);
+    BigVec<u32> order(n), depth(total);
     DB_TRY(hipMemcpyAsync(nodes.data(), emit_.p, (size_t)total * sizeof(TreeNode), hipMemcpyDeviceToHost, st_));
     DB_TRY(hipMemcpyAsync(order.data(), d.order, n * 4, hipMemcpyDeviceToHost, st_));
     DB_TRY(hipMemcpyAsync(depth.data(), d.ndepth, (size_t)total * 4, hipMemcpyDeviceToHost, st_));
-    std::vector<u32> ngeo(total);
+    BigVec<u32> ngeo(total);
     DB_TRY(hipMemcpyAsync(ngeo.data(), d.ngeo, (size_t)total * 4, hipMemcpyDeviceToHost, st_));
     u32 errs = 0;
     DB_TRY(hipMemcpyAsync(&errs, d.count + 2, 4, hipMemcpyDeviceToHost, st_));
     DB_TRY(hipStreamSynchronize(st_));
     if (errs) return hipErrorIllegalState;
-    for (u32 g = 0; g < ng; g++) {
+    host_for(ng, [&](size_t g) {
       Tree &t = out[g];
       t.nodes.assign(nodes.begin() + gbase[g], nodes.begin() + gbase[g + 1]);
       t.order.assign(order.begin() + off_h_[g], order.begin() + off_h_[g + 1]);
       for (u32 &p : t.order) p -= off_h_[g];
       t.depth = 0;
-    }
+    });
     for (u32 i = 0; i < total; i++) out[ngeo[i]].depth = std::max(out[ngeo[i]].depth, depth[i]);
     return hipSuccess;
   }
