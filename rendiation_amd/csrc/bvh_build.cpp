@@ -35,11 +35,11 @@ uint32_t Tree::hit(uint32_t i) const { return nodes[i].inner ? i + 1 : miss(i); 
 
 namespace {
 
-// Rust f32::min/max drop a NaN operand, like fminf/fmaxf.
+// Rust f32::min/max drop a NaN operand, like fminf/fmaxf (min_num, max_num).
 inline void grow(Box &b, const Box &o) {
   for (int k = 0; k < 3; k++) {
-    b.mn[k] = std::fmin(b.mn[k], o.mn[k]);
-    b.mx[k] = std::fmax(b.mx[k], o.mx[k]);
+    b.mn[k] = min_num(b.mn[k], o.mn[k]);
+    b.mx[k] = max_num(b.mx[k], o.mx[k]);
   }
 }
 
@@ -239,8 +239,8 @@ struct FastBuilder {
     for (uint32_t i = b; i < e; i++)
       for (int k = 0; k < 3; k++) {
         const float c = centre[k][order[i]];
-        cmn[k] = std::fmin(cmn[k], c);
-        cmx[k] = std::fmax(cmx[k], c);
+        cmn[k] = min_num(cmn[k], c);
+        cmx[k] = max_num(cmx[k], c);
       }
     float scale[3];
     bool live[3];

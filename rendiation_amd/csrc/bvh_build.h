@@ -20,6 +20,13 @@ struct Box {
 };
 Box box_empty();
 
+// fminf / fmaxf, inline: a NaN operand is dropped (both NaN give NaN) and of
+// two equal operands (+0 and -0) the second is returned, as glibc's are.
+// Those are out-of-line library calls, which made the box unions most of
+// the host builders' time; Rust's f32::min / max follow the same NaN rule.
+inline float min_num(float a, float b) { return (a < b || b != b) ? a : b; }
+inline float max_num(float a, float b) { return (a > b || b != b) ? a : b; }
+
 struct TreeOptions {
   uint32_t bin_size;   // leaf while count <= bin_size
   uint32_t max_depth;  // leaf at depth >= max_depth

@@ -98,14 +98,14 @@ void transform_point(const M4 &m, const float p[3], float out[3]) {
 
 inline void grow_point(Box &b, const float p[3]) {
   for (int k = 0; k < 3; k++) {
-    b.mn[k] = std::fmin(b.mn[k], p[k]);
-    b.mx[k] = std::fmax(b.mx[k], p[k]);
+    b.mn[k] = min_num(b.mn[k], p[k]);
+    b.mx[k] = max_num(b.mx[k], p[k]);
   }
 }
 inline void grow_box(Box &b, const Box &o) {
   for (int k = 0; k < 3; k++) {
-    b.mn[k] = std::fmin(b.mn[k], o.mn[k]);
-    b.mx[k] = std::fmax(b.mx[k], o.mx[k]);
+    b.mn[k] = min_num(b.mn[k], o.mn[k]);
+    b.mx[k] = max_num(b.mx[k], o.mx[k]);
   }
 }
 inline bool is_empty(const Box &b) { return b.mx[0] < b.mn[0] || b.mx[1] < b.mn[1] || b.mx[2] < b.mn[2]; }
