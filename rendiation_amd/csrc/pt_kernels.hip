@@ -44,6 +44,9 @@ __global__ void k_pt_begin(PtLaunch P) {
   const uint32_t nc = 2 * P.max_bounces + 3;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x)
     P.counts[i] = i == 0 ? P.npaths : 0u;
+  // the work counters of the persistent traversal and of the tail kernel
+  // start at zero (so their launches need no clearing launch)
+  if (blockIdx.x == 0 && threadIdx.x < 2) P.work[threadIdx.x] = 0u;
 }
 
 // ray generation (ray_gen.rs:89-107 + camera.rs:66-99)
@@ -74,6 +77,9 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_camera(PtLaunch P) {
 template <int B, bool REPLAY>
 __global__ __launch_bounds__(B) void k_pt_shade(PtLaunch P, uint32_t b) {
   const uint32_t bid = blockIdx.x;
+  // between the traversal of bounce b and the next one: the work counters of
+  // the persistent traversal and the tail kernel start that one at zero
+  if (bid == 0 && threadIdx.x < 2) P.work[threadIdx.x] = 0u;
   const uint32_t count_in = min(P.counts[b], P.npaths);
   const uint32_t nblocks = (count_in + B - 1) / B;
   if (bid >= nblocks) return;  // whole block
@@ -425,6 +431,9 @@ hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_
     tp.cull_mask = 0xffffffffu;
     tp.tlas_idx = 0;
     tp.work = P.work;
+    // k_pt_begin / k_pt_shade leave the work counter at zero before each
+    // traversal of an undiagnosed frame (visit counting traces twice in a row)
+    tp.work_zeroed = count ? 0u : 1u;
     return tp;
   };
   auto shadow = [&](uint32_t b) {
@@ -455,8 +464,7 @@ hipError_t enqueue_pt_frame(const PtLaunch &P, const TraceParams &scene, uint32_
       pe.queue = ps.queue = nullptr;
       pe.count = ps.count = nullptr;
       if (diag && diag->ev_begin) (void)hipEventRecord(diag->ev_begin[launch], st);
-      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, P.work + 1);
-      e = launch_pt_tail(P, pe, ps, push_depth, tune, st);
+      e = launch_pt_tail(P, pe, ps, push_depth, tune, st);  // work[1] zeroed by k_pt_shade(b)
       if (diag && diag->ev_end) (void)hipEventRecord(diag->ev_end[launch], st);
       launch++;
       if (e != hipSuccess) return e;

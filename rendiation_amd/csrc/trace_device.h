@@ -757,7 +757,7 @@ hipError_t launch_fast_kw(const TraceParams &p0, bool persist, uint32_t coop, hi
   }
   if (persist) {
     if (!p.work) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(1), 0, st, p.work);
+    if (!p.work_zeroed) hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(1), 0, st, p.work);
     if (p.counters) {
       auto k = k_trace_persist<K, S, B, W, true, LAYOUT>;
       uint32_t g = std::min(resident_grid((const void *)k, B), grid_slots(p, B) / B);

@@ -46,6 +46,9 @@ struct TraceParams {
   // (0 = their slots): when coop_max reaches it, only the cooperative kernel
   // is launched.
   uint32_t live_max;
+  // 1: the caller's previous kernel on the stream left *work at zero, so a
+  // persistent launch needs no clearing launch first
+  uint32_t work_zeroed;
 
   const PairNode *tri_pairs;
   const TriPre *tris;
