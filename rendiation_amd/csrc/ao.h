@@ -52,6 +52,7 @@ struct AoLaunch {
                         // occlusion rays traced by the call
   uint32_t qpar;        // parity of the queues being filled / traced (host-kept)
   uint32_t qcap;
+  uint32_t *work;       // the persistent traversal's work counter (zeroed by the stage poll)
   const float *frame;   // kAoFrameWords
   uint32_t width, height, tile_x, tile_y, tile_w;
   void *accum;  // f32 per pixel, or RGBA8 per pixel

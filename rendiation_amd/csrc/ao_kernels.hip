@@ -313,6 +313,7 @@ __device__ __forceinline__ void poll_store(const AoLaunch &L, uint32_t gid, floa
 // another of them, so no poll sees what another one does in this round.
 __global__ void k_poll_stages(AoLaunch L) {
   for (uint32_t g = kAoRayGen; g <= kAoMiss; g++) tg::begin_phase(L.gr.g[g]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *L.work = 0u;  // for the next round's traversal
   poll_raygen(L);
   poll_closest(L);
   poll_store(L, kAoClosestTest, 0.0f);
@@ -352,6 +353,8 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
   tp.cull_mask = 0xffffffffu;
   tp.tlas_idx = 0;
   tp.work = work;
+  tp.work_zeroed = 1u;  // by k_poll_stages, the launch before each traversal
+  L.work = work;
   tp.rays2 = L.rays;
   tp.hits2 = L.hits;
   tp.hitx2 = L.hitx;

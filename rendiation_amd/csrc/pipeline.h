@@ -51,6 +51,7 @@ struct PipeLaunch {
   uint32_t *tq;     // this round's rays to traverse
   uint32_t *tcount;  // [q]: their count in rounds of parity q; [2]: rays traversed by the call
   uint32_t qpar;
+  uint32_t *work;    // the persistent traversal's work counter (zeroed by the trace pre-pass)
   StageArrays st[tg::kMaxGroups];    // groups 1..n-1
   // launch geometry of the tile
   uint32_t width, height, tile_x, tile_y, tile_w;

@@ -35,8 +35,12 @@ __global__ void k_pipe_trace_pre(PipeLaunch L) {
   const Group &T = L.gr.g[kPipeTrace];
   uint32_t *cnt = L.tcount + L.qpar;
   // the other parity's count (used by the previous round, all of whose
-  // launches have finished) starts the next round at zero
-  if (blockIdx.x == 0 && threadIdx.x == 0) L.tcount[L.qpar ^ 1u] = 0u;
+  // launches have finished) starts the next round at zero, and so does the
+  // work counter of the traversal that follows
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    L.tcount[L.qpar ^ 1u] = 0u;
+    *L.work = 0u;
+  }
   tg::begin_phase(T);
   tg::refill_empty(T);  // trace slots are taken and freed only by stage polls
   tg::for_each_active(T, [&](uint32_t, bool valid, uint32_t idx) {
@@ -202,6 +206,8 @@ hipError_t enqueue_pipeline_tile(const PipeLaunch &L0, const void *const *stages
   tp.queue = L.tq;
   tp.slots = ntrace;
   tp.work = work;
+  tp.work_zeroed = 1u;  // by k_pipe_trace_pre, right before each traversal
+  L.work = work;
   auto poll_trace = [&]() -> hipError_t {
     hipLaunchKernelGGL(k_pipe_trace_pre, dim3(grid_for(ntrace)), dim3(kBlock), 0, st, L);
     TraceParams q = tp;
