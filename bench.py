@@ -1202,18 +1202,25 @@ def _fractions(workload: str, rec: dict) -> dict:
         ms, ref, ko = roof.get("kernel_ms"), roof.get("frac"), roof.get("kernel_order", {})
     elif workload == "batched":
         ms, ref, ko = cfg.get("ms_persistent_events"), roof.get("frac"), roof.get("kernel_order", {})
+    elif workload in ("ao", "pipeline"):
+        # no visit counts on these paths: the traversal's physical bytes over
+        # the whole sample / call (its traversal time is not split out)
+        ms, ref, ko = rec.get("ms_per_sample", rec.get("ms_per_call")), None, {}
     else:
         return {}
     tr = rec.get("traffic_detail")
     phys = None
     if tr and ms:
         phys = round(tr["bytes_per_step"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
-    return {"time_ms": ms, "reference_order": ref, "kernel_order": ko.get("frac"), "physical": phys,
-            "l2_hit_rate": tr.get("l2_hit_rate") if tr else None}
+    out = {"time_ms": ms, "reference_order": ref, "kernel_order": ko.get("frac"), "physical": phys,
+           "l2_hit_rate": tr.get("l2_hit_rate") if tr else None}
+    if workload in ("ao", "pipeline"):
+        out["time_basis"] = "whole sample" if workload == "ao" else "whole call"
+    return out
 
 
 WORKLOADS = ("pt", "closest_hit", "batched", "diff", "diffgeo", "ao", "pipeline")
-COUNTED = ("pt", "closest_hit", "batched", "diff", "diffgeo")
+COUNTED = ("pt", "closest_hit", "batched", "diff", "diffgeo", "ao", "pipeline")
 TRAFFIC_BUDGET_S = 420.0  # no further counter passes past this point of the run
 
 
