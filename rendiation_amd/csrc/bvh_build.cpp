@@ -275,29 +275,36 @@ struct FastBuilder {
     double best_cost = std::numeric_limits<double>::infinity();
     int best_axis = -1;
     uint32_t best_bin = 0;
+    // The sweep visits the occupied bins only.  An empty bin adds nothing to
+    // a side's box (min / max with an empty box are exact) or count, so the
+    // cost after it equals the cost after the last occupied bin before it,
+    // and the first strict minimum -- the split chosen -- is the same.
     double right_cost[kMaxFastBins];
+    uint32_t occ[kMaxFastBins];
     for (int axis = 0; axis < 3; axis++) {
       if (!live[axis]) continue;
       const uint32_t *c = &cnt[axis * K];
       const Box *bx = &bb[axis * K];
+      uint32_t no = 0;
+      for (uint32_t k = 0; k < K; k++)
+        if (c[k]) occ[no++] = k;
       Box r = box_empty();
       uint32_t rc = 0;
-      for (uint32_t k = K - 1; k >= 1; k--) {
-        grow(r, bx[k]);
-        rc += c[k];
-        right_cost[k] = (double)surface_area(r) * rc;
+      for (uint32_t j = no; j-- > 1;) {  // right side from occupied bin j on
+        grow(r, bx[occ[j]]);
+        rc += c[occ[j]];
+        right_cost[j] = (double)surface_area(r) * rc;
       }
       Box l = box_empty();
       uint32_t lc = 0;
-      for (uint32_t k = 0; k + 1 < K; k++) {
-        grow(l, bx[k]);
-        lc += c[k];
-        if (lc == 0 || lc == n) continue;
-        const double cost = (double)surface_area(l) * lc + right_cost[k + 1];
+      for (uint32_t j = 0; j + 1 < no; j++) {  // left side up to occupied bin j: 0 < lc < n
+        grow(l, bx[occ[j]]);
+        lc += c[occ[j]];
+        const double cost = (double)surface_area(l) * lc + right_cost[j + 1];
         if (cost < best_cost) {
           best_cost = cost;
           best_axis = axis;
-          best_bin = k;
+          best_bin = occ[j];
         }
       }
     }
