@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -326,6 +327,20 @@ int32_t rdrt_ao_autotune(rdrt_ctx *c, const rdrt_ao_params *prm, void *accum, ui
   const TraceTune before = c->tune;
   TraceTune chosen = before;
   float chosen_ms = -1.0f;
+  // average call time of shape t over n calls
+  auto call_ms = [&](const TraceTune &t, uint32_t n, float &ms) -> int32_t {
+    c->tune = t;
+    int32_t r2 = rdrt_ao_render(c, &q, accum);  // warm-up
+    if (r2 == RDRT_OK && hipEventRecord(e0, c->work_stream) != hipSuccess) r2 = fail(RDRT_ERR_HIP, "event");
+    for (uint32_t r = 0; r < n && r2 == RDRT_OK; r++) r2 = rdrt_ao_render(c, &q, accum);
+    if (r2 != RDRT_OK) return r2;
+    if (hipEventRecord(e1, c->work_stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      return fail(RDRT_ERR_HIP, "AO timing failed");
+    ms /= (float)n;
+    return RDRT_OK;
+  };
+  std::vector<std::pair<float, TraceTune>> timed;
   for (uint32_t block : {64u, 128u, 256u})
     for (uint32_t lds : {8u, 16u})
       for (uint32_t waves : {0u, 6u, 8u})
@@ -338,23 +353,35 @@ int32_t rdrt_ao_autotune(rdrt_ctx *c, const rdrt_ao_params *prm, void *accum, ui
           t.wide = (shape >> 1) % 3u;
           t.coop = kCoopChoices[shape / 6u];
           if (stack_entries_for(push_depth(c, t)) == 0) continue;
-          c->tune = t;
-          rc = rdrt_ao_render(c, &q, accum);  // warm-up
-          if (rc == RDRT_OK && hipEventRecord(e0, c->work_stream) != hipSuccess) rc = fail(RDRT_ERR_HIP, "event");
-          for (uint32_t r = 0; r < reps && rc == RDRT_OK; r++) rc = rdrt_ao_render(c, &q, accum);
-          if (rc != RDRT_OK) break;
           float ms = 0.0f;
-          if (hipEventRecord(e1, c->work_stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-              hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-            rc = fail(RDRT_ERR_HIP, "AO timing failed");
-            break;
-          }
-          ms /= (float)reps;
+          rc = call_ms(t, reps, ms);
+          if (rc != RDRT_OK) break;
+          timed.push_back({ms, t});
           if (chosen_ms < 0.0f || ms < chosen_ms) {
             chosen_ms = ms;
             chosen = t;
           }
         }
+  // the fastest few, timed again on 3 x reps calls, twice, in turn (shapes
+  // within a few percent of each other are otherwise told apart by noise)
+  if (rc == RDRT_OK && timed.size() > 1) {
+    std::sort(timed.begin(), timed.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+    const size_t nk = std::min<size_t>(4, timed.size());
+    std::vector<float> sum(nk, 0.0f);
+    for (uint32_t pass = 0; pass < 2 && rc == RDRT_OK; pass++)
+      for (size_t k = 0; k < nk && rc == RDRT_OK; k++) {
+        float ms = 0.0f;
+        rc = call_ms(timed[k].second, 3 * reps, ms);
+        sum[k] += ms;
+      }
+    if (rc == RDRT_OK) {
+      size_t bk = 0;
+      for (size_t k = 1; k < nk; k++)
+        if (sum[k] < sum[bk]) bk = k;
+      chosen = timed[bk].second;
+      chosen_ms = sum[bk] / 2.0f;
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != RDRT_OK) {
