@@ -316,7 +316,10 @@ typedef struct rdrt_ao_params {
 int32_t rdrt_ao_render(rdrt_ctx *ctx, const rdrt_ao_params *params, void *accum);
 /* States of the AO task graph after the last rdrt_ao_render, as
  * rdrt_task_graph_states (5 groups: trace, ray gen, closest hit, occlusion
- * closest hit, miss). */
+ * closest hit, miss); errors bit 3 of the trace group: a task open after
+ * round 3 that is not a pixel's occlusion trace awaiting only the
+ * resolution of its chain (rounds 4 and 5 run as one pass over those
+ * chains; not reached). */
 int32_t rdrt_ao_states(rdrt_ctx *ctx, uint32_t *wake, uint32_t *sleep_or_finished, uint32_t *empty, uint32_t *errors);
 /* Times every traversal launch shape on eager AO calls with these
  * parameters (`accum` is overwritten) and keeps the fastest for the

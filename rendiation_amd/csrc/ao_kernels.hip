@@ -320,6 +320,70 @@ __global__ void k_poll_stages(AoLaunch L) {
   poll_store(L, kAoMiss, 1.0f);
 }
 
+// Rounds 4 and 5 of a tile in one launch, for executions of five rounds.
+// When round 4 starts no task will be spawned any more (the ray gen spawns
+// its camera trace in round 1, the closest hit its occlusion trace in round
+// 2, both traced by round 3), and the only awake tasks are the occlusion
+// traces of the pixels whose camera ray hit, woken in round 3 by their
+// finished closest-hit-test / miss task.  Each such pixel's chain takes
+// exactly the four polls left, one a phase: the occlusion trace takes its
+// child's value and resolves (round 4, trace phase), the closest hit it
+// wakes takes the value and resolves (round 4, stage phase), the camera
+// trace it wakes does the same (round 5, trace phase), and the ray gen adds
+// the value to the running mean (round 5, stage phase).  A chain touches only
+// its own five tasks, so one thread runs the four polls of one chain in
+// turn, loading the whole chain up front, and frees its five tasks as those
+// polls would: pixel values and task states come out as after the two
+// rounds, and every active list ends empty.  A task that does not fit this
+// shape is not polled and sets error bit 3 of the trace group.
+__global__ void k_finish_chains(AoLaunch L) {
+  const Group &T = L.gr.g[kAoTrace], &G = L.gr.g[kAoRayGen], &C2 = L.gr.g[kAoClosest];
+  const Group &C3 = L.gr.g[kAoClosestTest], &C4 = L.gr.g[kAoMiss];
+  for (uint32_t g = kAoTrace; g <= kAoMiss; g++) tg::begin_phase(L.gr.g[g]);
+  const uint32_t s = __float_as_uint(L.frame[23]);
+  tg::for_each_active(T, [&](uint32_t, bool valid, uint32_t idx) {
+    // the occlusion trace: its finished child and its parent (a closest hit)
+    const uint32_t *w = pw(T, idx);
+    const uint32_t stage = valid ? T.stage[idx] : 0u, w0 = valid ? w[0] : 0u, w2 = valid ? w[2] : tg::kNone;
+    const uint32_t child = valid ? T.child[idx] : tg::kNone;
+    const uint32_t p = valid ? T.parent[idx] : tg::kNone, ptype = valid ? T.ptype[idx] : tg::kNone;
+    const Group &CG = w2 == kAoMiss ? C4 : C3;
+    bool ok = valid && stage == 1u && w0 == 1u && (w2 == kAoClosestTest || w2 == kAoMiss) && child >= 1u &&
+              child <= CG.size && ptype == kAoClosest && p >= 1u && p <= C2.size;
+    const uint32_t cstate = ok ? CG.state[child] : tg::kNotExist, cval = ok ? pw(CG, child)[0] : 0u;
+    const uint32_t pstate = ok ? C2.state[p] : tg::kNotExist, pstage = ok ? C2.stage[p] : 0u;
+    const uint32_t pchild = ok ? C2.child[p] : tg::kNone;
+    const uint32_t q = ok ? C2.parent[p] : tg::kNone, qtype = ok ? C2.ptype[p] : tg::kNone;
+    ok = ok && cstate == tg::kFinished && pstate == tg::kSleep && pstage == 1u && pchild == idx &&
+         qtype == kAoTrace && q >= 1u && q <= T.size;
+    // the camera trace, asleep on the closest hit, and its ray gen
+    const uint32_t *wq = pw(T, ok ? q : 0u);
+    const uint32_t qstate = ok ? T.state[q] : tg::kNotExist, qstage = ok ? T.stage[q] : 0u;
+    const uint32_t qchild = ok ? T.child[q] : tg::kNone, qw0 = ok ? wq[0] : 1u, qw2 = ok ? wq[2] : tg::kNone;
+    const uint32_t r = ok ? T.parent[q] : tg::kNone, rtype = ok ? T.ptype[q] : tg::kNone;
+    ok = ok && qstate == tg::kSleep && qstage == 1u && qchild == p && qw0 == 0u && qw2 == kAoClosest &&
+         rtype == kAoRayGen && r >= 1u && r <= G.size;
+    const uint32_t rstate = ok ? G.state[r] : tg::kNotExist, rstage = ok ? G.stage[r] : 0u;
+    const uint32_t rchild = ok ? G.child[r] : tg::kNone, rparent = ok ? G.parent[r] : 0u;
+    const uint32_t tp = ok ? pw(G, r)[0] : 0u;
+    ok = ok && rstate == tg::kSleep && rstage == 1u && rchild == q && rparent == tg::kNone;
+    if (valid && !ok) atomicOr(T.ctr + tg::kError, 8u);
+    // the occlusion value (0 from the closest hit test, 1 from the miss)
+    // passes up the chain into the running mean
+    if (ok) accumulate(L, L.tile_x + tp % L.tile_w, L.tile_y + tp / L.tile_w, s, __uint_as_float(cval));
+    uint32_t pos[6];
+    tg::block_add_n<6>({tg::removed_count(C3), tg::removed_count(C4), tg::removed_count(T), tg::removed_count(T),
+                        tg::removed_count(C2), tg::removed_count(G)},
+                       {ok && w2 == kAoClosestTest, ok && w2 == kAoMiss, ok, ok, ok, ok}, pos);
+    tg::free_at(C3, ok && w2 == kAoClosestTest, child, pos[0]);
+    tg::free_at(C4, ok && w2 == kAoMiss, child, pos[1]);
+    tg::free_at(T, ok, idx, pos[2]);
+    tg::free_at(T, ok, q, pos[3]);
+    tg::free_at(C2, ok, p, pos[4]);
+    tg::free_at(G, ok, r, pos[5]);
+  });
+}
+
 }  // namespace
 
 hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds, const TraceParams &scene,
@@ -380,6 +444,13 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
   uint32_t stage_grid = 1;
   for (uint32_t g = kAoRayGen; g <= kAoMiss; g++) stage_grid = std::max(stage_grid, grid_for(gr.g[g].size));
   for (uint32_t r = 0; r < rounds; r++) {
+    if (r == 3 && rounds == kAoChainRounds) {  // rounds 4 and 5 (k_finish_chains)
+      hipLaunchKernelGGL(k_finish_chains, dim3(grid_for(ntrace)), dim3(kBlock), 0, st, L);
+      tg::end_phase(gr, kTrace | kStages);
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      break;
+    }
     if (r > 0) {
       if (r <= 2) {
         TraceParams q = tp;

@@ -7,7 +7,9 @@ ao.rs:192-304) on two schedules of the task-graph model:
 * the schedule of the device AO path: the four stage groups polled together
   as one phase, the trace group as the other, no prepare or compaction
   between them (a pending task sleeps at once), no trace poll in the first
-  round, the trace queue filled at spawn.
+  round, the trace queue filled at spawn;
+* that schedule with rounds 4 and 5 run as one pass over the chains still
+  open (the device path's k_finish_chains).
 
 For random camera / occlusion hit patterns both must give every pixel the
 same occlusion value after the same number of rounds, and the same task
@@ -153,6 +155,28 @@ class AoChain:
             self.phase([TRACE])
         self.phase([RAYGEN, CLOSEST, CLOSEST_TEST, MISS])
 
+    def finish_chains(self):
+        """Rounds 4 and 5 in one pass (the device path's k_finish_chains):
+        every awake task is an occlusion trace whose closest-hit-test / miss
+        child has finished; its pixel's chain takes the four remaining polls
+        in turn (occlusion trace, closest hit, camera trace, ray gen), each
+        woken task polled at once by the same pass."""
+        traces = list(self.active[TRACE])
+        assert all(not self.active[g] for g in range(1, 5))
+        self.active[TRACE] = []
+        for idx in traces:
+            g, t = TRACE, idx
+            for expect in (TRACE, CLOSEST, TRACE, RAYGEN):
+                assert g == expect
+                self.poll_one(g, t, True)
+                if g == RAYGEN:
+                    assert self.state[g][t] == NOT_EXIST
+                    break
+                assert self.state[g][t] == FINISHED
+                g, t = self.parent[g][t]
+                assert self.active[g] == [t]  # the one task it woke
+                self.active[g] = []
+
     def states(self):
         out = []
         for g in range(5):
@@ -183,6 +207,23 @@ def test_phases_equal_executor_rounds(seed):
         c.round_phased(r == 0)
     assert c.value == [0 if (ch and oh) else 1 for ch, oh in zip(camera_hit, occ_hit)]
     assert all(w == 0 and s == 0 for w, s, _ in c.states())
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_fused_last_two_rounds_equal_executor_rounds(seed):
+    rng = random.Random(100 + seed)
+    n = rng.randrange(1, 300)
+    camera_hit = [rng.random() < 0.6 for _ in range(n)]
+    occ_hit = [rng.random() < 0.5 for _ in range(n)]
+    a, b = AoChain(camera_hit, occ_hit), AoChain(camera_hit, occ_hit)
+    for r in range(5):
+        a.round_sequential()
+    for r in range(3):
+        b.round_phased(r == 0)
+    b.finish_chains()
+    assert a.value == b.value
+    assert a.states() == b.states()
+    assert all(w == 0 and s == 0 for w, s, _ in b.states())
 
 
 def test_four_rounds_leave_hit_pixels_unfinished():
