@@ -14,9 +14,14 @@ using tg::Graph;
 using tg::Group;
 
 // 1024-thread blocks: each pass takes one counter atomic per block and list,
-// and same-address atomics serialize, so fewer, larger blocks finish sooner
+// and same-address atomics serialize, so fewer, larger blocks finish sooner.
+// At most 512 of them (about as many as the chip holds at once: the passes
+// loop over larger lists): a pass over an empty group, as in the rounds
+// after a pipeline has drained, then costs a few microseconds instead of
+// the dispatch of a grid sized for the whole pool (1.86 -> 1.59 ms for the
+// 1080p pipeline test, whose last five of eight rounds find no task)
 constexpr uint32_t kBlock = 1024;
-constexpr uint32_t kMaxGrid = 4096;
+constexpr uint32_t kMaxGrid = 512;
 constexpr uint32_t kStageBlock = 256;  // the caller's stage kernels
 
 uint32_t grid_for(uint32_t n) {
