@@ -490,17 +490,26 @@ __device__ __forceinline__ void select_second(TraceParams &p) {
 // path tracer): they last as long as their slowest ray.  The result is the
 // nearest accepted hit, as in the one-ray-per-lane kernel; among hits with
 // equal t the lowest lane of the last batch wins (a visit-order tie, as
-// between the layouts).
+// between the layouts).  Launches of at most kCoopWideMax rays give each ray
+// 16 lanes: their time is their slowest rays' chains, and a batch of 16
+// instances takes a ray over twice as many instances per BLAS round (the
+// last bounces of the 1080p path-tracing frame: 2.88 -> 2.83 ms with every
+// launch of at most 32k rays cooperative); larger launches keep 8, where
+// the extra lanes cost more occupancy than they save.
 // ------------------------------------------------------------------------
 constexpr uint32_t kCoopLanes = 8;
+constexpr uint32_t kCoopWideLanes = 16;
+constexpr uint32_t kCoopWideMax = 32768;
 
+template <int G>
 __device__ __forceinline__ float group_min(float v) {
-  for (int m = (int)kCoopLanes / 2; m > 0; m >>= 1) v = fminf(v, __shfl_xor(v, m, (int)kCoopLanes));
+  for (int m = G / 2; m > 0; m >>= 1) v = fminf(v, __shfl_xor(v, m, G));
   return v;
 }
+template <int G>
 __device__ __forceinline__ bool group_any(bool b) {
   int v = b ? 1 : 0;
-  for (int m = (int)kCoopLanes / 2; m > 0; m >>= 1) v |= __shfl_xor(v, m, (int)kCoopLanes);
+  for (int m = G / 2; m > 0; m >>= 1) v |= __shfl_xor(v, m, G);
   return v != 0;
 }
 
@@ -508,7 +517,7 @@ __device__ __forceinline__ bool group_any(bool b) {
 // kCoopHit when the ray hit.
 constexpr uint32_t kCoopHit = 0x100u;
 
-template <int K, int S, int B, int LAYOUT>
+template <int K, int S, int B, int LAYOUT, int G = (int)kCoopLanes>
 __device__ __forceinline__ uint32_t trace_coop_one(const TraceParams &p, uint32_t rid, uint32_t *stk,
                                                    uint32_t glane) {
   uint32_t spill[S > K ? S - K : 1];
@@ -603,8 +612,8 @@ __device__ __forceinline__ uint32_t trace_coop_one(const TraceParams &p, uint32_
     if (cand != kInvalid) run_instance(cand);
     cand = kInvalid;
     cnt = 0;
-    far_ = group_min(far_);
-    done = group_any(done);
+    far_ = group_min<G>(far_);
+    done = group_any<G>(done);
     r = load_ray(p, rid);
   };
 
@@ -634,7 +643,7 @@ __device__ __forceinline__ uint32_t trace_coop_one(const TraceParams &p, uint32_
           if ((r.mask & in.mask) == 0) continue;
           if (merge_flags(r.flags, in.flags) & F_SKIP_TRIANGLES) continue;
           if (glane == cnt) cand = i;
-          if (++cnt == kCoopLanes) {
+          if (++cnt == (uint32_t)G) {
             flush();
             winv = recip(r.d);
           }
@@ -650,7 +659,7 @@ __device__ __forceinline__ uint32_t trace_coop_one(const TraceParams &p, uint32_
   const bool mine = best.k != kInvalid && best_t == far_;
   const uint64_t m = __ballot(mine);
   const uint32_t lane = __lane_id();
-  const uint32_t gm = (uint32_t)(m >> (lane & ~(kCoopLanes - 1u))) & ((1u << kCoopLanes) - 1u);
+  const uint32_t gm = (uint32_t)(m >> (lane & ~((uint32_t)G - 1u))) & ((1u << G) - 1u);
   const uint32_t win = gm ? (uint32_t)__ffs(gm) - 1u : 0u;
   if (glane == win) {
     if (!gm) best.k = kInvalid;
@@ -660,12 +669,10 @@ __device__ __forceinline__ uint32_t trace_coop_one(const TraceParams &p, uint32_
   return win | (gm ? kCoopHit : 0u);
 }
 
-template <int K, int S, int B, int W, int LAYOUT>
-__global__ __launch_bounds__(B, W) void k_trace_coop(TraceParams p) {
-  __shared__ uint32_t lds[K * B];
-  const uint32_t n0 = live_first(p), n1 = live_second(p);
-  if (n0 + n1 > p.coop_max) return;  // the one-ray-per-lane kernel traces launches this large
-  const uint32_t gi = (blockIdx.x * B + threadIdx.x) / kCoopLanes;
+// G lanes per ray: group gi of the grid takes the gi-th live ray.
+template <int K, int S, int B, int LAYOUT, int G>
+__device__ __forceinline__ void trace_coop_group(TraceParams &p, uint32_t n0, uint32_t n1, uint32_t *lds) {
+  const uint32_t gi = (blockIdx.x * B + threadIdx.x) / (uint32_t)G;
   uint32_t slot = gi, n = n0;
   if (gi >= n0) {
     slot = gi - n0;
@@ -675,7 +682,17 @@ __global__ __launch_bounds__(B, W) void k_trace_coop(TraceParams p) {
   if (slot >= n) return;  // whole groups: the lanes of a group share gi
   const uint32_t rid = p.queue ? p.queue[slot] : slot;
   if (rid >= p.n) return;
-  trace_coop_one<K, S, B, LAYOUT>(p, rid, lds + threadIdx.x, threadIdx.x & (kCoopLanes - 1u));
+  trace_coop_one<K, S, B, LAYOUT, G>(p, rid, lds + threadIdx.x, threadIdx.x & ((uint32_t)G - 1u));
+}
+
+template <int K, int S, int B, int W, int LAYOUT>
+__global__ __launch_bounds__(B, W) void k_trace_coop(TraceParams p) {
+  __shared__ uint32_t lds[K * B];
+  const uint32_t n0 = live_first(p), n1 = live_second(p);
+  if (n0 + n1 > p.coop_max) return;  // the one-ray-per-lane kernel traces launches this large
+  // the group width is uniform over the launch (the live count is)
+  if (n0 + n1 <= kCoopWideMax) trace_coop_group<K, S, B, LAYOUT, (int)kCoopWideLanes>(p, n0, n1, lds);
+  else trace_coop_group<K, S, B, LAYOUT, (int)kCoopLanes>(p, n0, n1, lds);
 }
 
 // One ray per thread over a grid sized for both segments' slots; the live
@@ -751,7 +768,9 @@ hipError_t launch_fast_kw(const TraceParams &p0, bool persist, uint32_t coop, hi
     uint32_t live = slots_of(p) + (p.count2 ? p.slots2 : 0u);
     if (p.live_max && p.live_max < live) live = p.live_max;
     const uint32_t groups = std::min(p.coop_max, live);
-    const uint32_t grid = (groups * kCoopLanes + B - 1) / B;
+    // 8 lanes per ray, or 16 when at most kCoopWideMax rays are live
+    const uint32_t threads = std::max(groups * kCoopLanes, std::min(groups, kCoopWideMax) * kCoopWideLanes);
+    const uint32_t grid = (threads + B - 1) / B;
     if (grid) hipLaunchKernelGGL((k_trace_coop<K, S, B, W, LAYOUT>), dim3(grid), dim3(B), 0, st, p);
     if (live <= p.coop_max) return hipGetLastError();  // the cooperative kernel always qualifies
   }

@@ -89,13 +89,16 @@ def test_fast_kernel_matches_except_near_ties(closest_hit_scene, wide, device):
     assert np.array_equal(gm[same], rm[same])
 
 
-@pytest.mark.parametrize("wide", [0, 1, 2])
-def test_cooperative_kernel_matches_except_near_ties(closest_hit_scene, wide, device):
-    """Every launch through the cooperative kernel (8 lanes per ray): the
+@pytest.mark.parametrize("wide,nrays", [(0, None), (1, None), (2, None), (0, 30_000), (1, 30_000), (2, 30_000)])
+def test_cooperative_kernel_matches_except_near_ties(closest_hit_scene, wide, nrays, device):
+    """Every launch through the cooperative kernel (8 lanes per ray; 16 for
+    the launches of at most 32,768 rays, here the first 30,000 rays): the
     same hits as the oracle except near ties, as the one-ray-per-lane
     kernel."""
     sys, rays, (rh, rm, _) = closest_hit_scene
-    _say(f"closest-hit 100k tris x 1M rays, cooperative kernel, wide={wide}")
+    if nrays is not None:
+        rays, rh, rm = rays[:nrays], rh[:nrays], rm[:nrays]
+    _say(f"closest-hit 100k tris x {rays.shape[0]} rays, cooperative kernel, wide={wide}")
     sys.set_tune(wide=wide, coop=1 << 22)
     try:
         gh, gm = _gpu_trace_nc(sys, rays, device)
