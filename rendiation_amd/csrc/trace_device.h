@@ -520,6 +520,8 @@ constexpr uint32_t kCoopHit = 0x100u;
 template <int K, int S, int B, int LAYOUT, int G = (int)kCoopLanes>
 __device__ __forceinline__ uint32_t trace_coop_one(const TraceParams &p, uint32_t rid, uint32_t *stk,
                                                    uint32_t glane) {
+  // the group's hit mask below is a 32-bit (1 << G) - 1
+  static_assert(G == 8 || G == 16, "cooperative groups of 8 or 16 lanes");
   uint32_t spill[S > K ? S - K : 1];
   auto push = [&](uint32_t &sp, uint32_t v) {
     if (sp < (uint32_t)K) stk[sp * B] = v;
