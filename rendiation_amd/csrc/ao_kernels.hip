@@ -320,6 +320,67 @@ __global__ void k_poll_stages(AoLaunch L) {
   poll_store(L, kAoMiss, 1.0f);
 }
 
+// The reset and round 1 of a tile in one launch.  After the reset (every
+// pool empty; root k of the tile's n ray-gen roots in slot k + 1 and active
+// entry k, payload k) round 1 polls only those roots, each for the first
+// time: it spawns the camera trace, queues its ray and sleeps.  With the
+// trace pool empty its slots come off the empty stack in order, so root k's
+// trace takes slot k + 1, active entry k and queue entry k; this launch
+// writes that state directly (what tg::enqueue_reset_spawn and the first
+// k_poll_stages leave, in one pass).
+__global__ void k_ao_begin(AoLaunch L, uint32_t n, uint32_t nzero) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nzero) L.tcount[t] = t == 0 ? n : 0u;  // the camera rays queued for round 2
+  if (t == 0) *L.work = 0u;                        // for round 2's traversal
+  const uint32_t s = __float_as_uint(L.frame[23]);
+  for (uint32_t g = 0; g < L.gr.n; g++) {
+    const Group &G = L.gr.g[g];
+    const uint32_t kk = (g == kAoRayGen || g == kAoTrace) ? n : 0u;
+    for (uint32_t i = t; i <= G.size; i += stride) {
+      if (i >= 1u && i <= kk) {
+        const uint32_t tp = i - 1u;  // the root's tile pixel
+        if (g == kAoRayGen) {        // asleep on its camera trace
+          G.state[i] = tg::kSleep;
+          G.parent[i] = tg::kNone;
+          G.ptype[i] = tg::kNone;
+          G.stage[i] = 1u;
+          G.child[i] = i;
+          pw(G, i)[0] = tp;
+        } else {  // the camera trace, awake and queued (spawn_trace)
+          const uint32_t x = L.tile_x + tp % L.tile_w, y = L.tile_y + tp / L.tile_w;
+          uint32_t state = pt::xxhash32(x, y, s);
+          const float r0 = pcg_next(state);
+          const float r1 = pcg_next(state);
+          // camera.rs:66-99
+          const float u = (float)x / (float)L.width + r0 / (float)L.width;
+          const float v = (float)y / (float)L.height + r1 / (float)L.height;
+          const V3 target = pt::unproject(L.frame, u, v);
+          const V3 cam = load3(L.frame + 16);
+          const V3 d = pt::normalize3(target - cam);
+          G.state[i] = tg::kWaken;
+          G.parent[i] = i;
+          G.ptype[i] = kAoRayGen;
+          G.stage[i] = 0u;
+          G.child[i] = tg::kNone;
+          uint32_t *w = pw(G, i);
+          w[0] = 0u;  // primary ray; payload 0 unless the miss writes 1 (ao.rs:216-219)
+          w[1] = __float_as_uint(0.0f);
+          w[2] = tg::kNone;
+          write_ray(L.rays + i, cam, kFltEps, d, kFltMax);
+          L.tq0[tp] = i;
+        }
+        G.list0[tp] = i;
+      } else {
+        G.state[i] = tg::kNotExist;
+      }
+      if (i < G.size) G.empty[i] = G.size - i;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < tg::kCtrWords)
+      G.ctr[threadIdx.x] = threadIdx.x == tg::kEmpty0 ? G.size - kk : threadIdx.x == tg::kActive0 ? kk : 0u;
+  }
+}
+
 // Rounds 4 and 5 of a tile in one launch, for executions of five rounds.
 // When round 4 starts no task will be spawned any more (the ray gen spawns
 // its camera trace in round 1, the closest hit its occlusion trace in round
@@ -391,10 +452,8 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
                            hipStream_t st, tg::Graph *after) {
   AoLaunch L = L0;  // its graph's parities advance as launches are enqueued
   Graph &gr = L.gr;
-  // empty graph, the tile's ray-gen roots, the queue counts at zero (and the
-  // call's ray totals on its first tile)
-  hipError_t e = tg::enqueue_reset_spawn(gr, kAoRayGen, L.tile_w * tile_h, L.tcount, first_tile ? 6u : 4u, st);
-  if (e != hipSuccess) return e;
+  const uint32_t nroots = L.tile_w * tile_h;
+  hipError_t e = hipSuccess;
   L.qpar = 0u;
   const uint32_t ntrace = gr.g[kAoTrace].size;
   TraceParams tp = scene;
@@ -443,7 +502,23 @@ hipError_t enqueue_ao_tile(const AoLaunch &L0, uint32_t tile_h, uint32_t rounds,
   constexpr uint32_t kTrace = 1u << kAoTrace;
   uint32_t stage_grid = 1;
   for (uint32_t g = kAoRayGen; g <= kAoMiss; g++) stage_grid = std::max(stage_grid, grid_for(gr.g[g].size));
-  for (uint32_t r = 0; r < rounds; r++) {
+  uint32_t r0 = 0;
+  if (rounds >= 1) {
+    // empty graph, the tile's ray-gen roots, the queue counts at zero (and
+    // the call's ray totals on its first tile), and round 1 (k_ao_begin)
+    if (nroots > gr.g[kAoRayGen].size || nroots > gr.g[kAoTrace].size || nroots > L.qcap) return hipErrorInvalidValue;
+    for (uint32_t k = 0; k < gr.n; k++) gr.g[k].cur = gr.g[k].pp = 0u;
+    hipLaunchKernelGGL(k_ao_begin, dim3(grid_for(gr.g[kAoTrace].size + 1)), dim3(kBlock), 0, st, L, nroots,
+                       first_tile ? 6u : 4u);
+    tg::end_phase(gr, kStages);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    r0 = 1;
+  } else {
+    e = tg::enqueue_reset_spawn(gr, kAoRayGen, nroots, L.tcount, first_tile ? 6u : 4u, st);
+    if (e != hipSuccess) return e;
+  }
+  for (uint32_t r = r0; r < rounds; r++) {
     if (r == 3 && rounds == kAoChainRounds) {  // rounds 4 and 5 (k_finish_chains)
       hipLaunchKernelGGL(k_finish_chains, dim3(grid_for(ntrace)), dim3(kBlock), 0, st, L);
       tg::end_phase(gr, kTrace | kStages);
