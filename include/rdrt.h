@@ -509,8 +509,9 @@ typedef struct rdrt_tune {
                            2 = quantized child quads (four 8-bit boxes, 64 B) */
   float ms;             /* rdrt_autotune: per-launch time of the chosen shape on the sample */
   uint32_t coop;        /* launches with at most this many live rays (device-side count) are traced by the
-                           cooperative kernel, 8 lanes per ray: the TLAS walk in lockstep, the instances it
-                           accepts dealt to the lanes; 0 = never.  Fast layouts only, not with visit counting */
+                           cooperative kernel, 8 lanes per ray (16 for launches of at most 32,768 live rays):
+                           the TLAS walk in lockstep, the instances it accepts dealt to the lanes; 0 = never.
+                           Fast layouts only, not with visit counting */
   uint32_t tail;        /* path-tracer frames: from bounce `tail` (1 .. max_bounces - 1) on, one launch finishes
                            every path, 8 lanes per path running its remaining bounces back to back (the frame
                            waits for its slowest path instead of the slowest ray of every bounce); 0 = one set
